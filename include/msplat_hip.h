@@ -1,0 +1,206 @@
+/*
+ * msplat HIP backend — C ABI.
+ *
+ * The drop-in boundary for msplat's backend interface core/metal/bindings.h (v1.1.4):
+ * every entry point below names the bindings.h function it replaces. A C++ backend file
+ * implementing bindings.h over MTensor (core/hip/bindings_hip.cpp) or any FFI
+ * (ctypes, cgo, JNI) calls these with plain device pointers and sizes.
+ *
+ * Conventions
+ *  - Pointers documented "device" are HIP device allocations (msplat_hip_malloc or any
+ *    other hipMalloc'ed memory); "host" pointers are read during the call.
+ *  - Gaussian parameters use the reference layouts (model.cpp:52-64): means (N,3),
+ *    log-scales (N,3), quaternions (N,4) stored (w,x,y,z), features_dc (N,3),
+ *    features_rest (N, bases-1, 3), opacity logits (N) — all float32, row-major.
+ *  - Every call is enqueued on the context's stream and returns without waiting unless
+ *    it says "syncs". Outputs alias per-context caches where the reference's do
+ *    (bindings.h:93-96) and stay valid until the next call on the same context.
+ *  - Every function returns MSPLAT_OK (0) or an error code; msplat_hip_last_error()
+ *    gives the message. No C++ exception crosses this boundary.
+ */
+#ifndef MSPLAT_HIP_H
+#define MSPLAT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MSPLAT_OK = 0,
+    MSPLAT_ERR_ARG = 1,      /* invalid argument (std::invalid_argument in the reference) */
+    MSPLAT_ERR_HIP = 2,      /* a HIP runtime call failed */
+    MSPLAT_ERR_RANGE = 3,    /* a size limit was exceeded (e.g. > 2^31 intersections, cpu:546-547) */
+    MSPLAT_ERR_CAPACITY = 4  /* densify with 3N > capacity (cpu:1347) */
+};
+
+typedef struct msplat_hip_ctx msplat_hip_ctx;
+
+/* One view: the values Model::prepareCam computes (model.cpp:498-546). */
+typedef struct {
+    float viewmat[16]; /* world -> camera (OpenCV axes), row-major */
+    float projmat[16]; /* projection * viewmat, row-major */
+    float cam_pos[3];
+    float fx, fy, cx, cy;
+    uint32_t width, height;
+    int32_t tiles_x, tiles_y; /* ceil(W/16), ceil(H/16) */
+    float clip_thresh;        /* 0.01 in Model */
+    float glob_scale;         /* 1.0 in Model */
+    uint32_t degree;          /* SH degree the rest coefficients are stored for */
+    uint32_t degrees_to_use;  /* SH degree evaluated this step */
+} msplat_hip_camera;
+
+/* Device pointers to the six parameter groups. */
+typedef struct {
+    int32_t num_points;
+    float *means, *scales, *quats, *features_dc, *features_rest, *opacities;
+} msplat_hip_gaussians;
+
+/* The six Adam groups (means, scales, quats, dc, rest, opacity), as msplat_train_step
+ * takes them (bindings.h:100-118). params[g] must equal the matching gaussians pointer. */
+typedef struct {
+    float *exp_avg[6], *exp_avg_sq[6];
+    float step_size[6]; /* lr / (1 - beta1^t) */
+    float bc2_sqrt[6];  /* sqrt(1 - beta2^t) */
+    float beta1, beta2, eps;
+} msplat_hip_adam;
+
+/* Densification statistics, accumulated by the train step (cpu:1311-1324). */
+typedef struct {
+    float *vis_counts, *xys_grad_norm, *max_2d_size; /* device, (N) */
+} msplat_hip_grad_stats;
+
+/* ── context, memory, streams (bindings.h:7-29) ─────────────────────────────── */
+
+const char *msplat_hip_last_error(void);
+
+/* Creates a context on `device` using `stream` (a hipStream_t; NULL creates a private
+ * stream). Replaces the global Metal context and FusedTensorCache (msplat_metal.mm:336-585). */
+int msplat_hip_ctx_create(int device, void *stream, msplat_hip_ctx **out);
+/* cleanup_msplat_metal (bindings.h:8): releases every cached buffer. */
+int msplat_hip_ctx_destroy(msplat_hip_ctx *ctx);
+int msplat_hip_ctx_set_stream(msplat_hip_ctx *ctx, void *stream);
+void *msplat_hip_ctx_stream(msplat_hip_ctx *ctx);
+/* msplat_gpu_sync (bindings.h:26): waits for everything enqueued on the context. */
+int msplat_hip_sync(msplat_hip_ctx *ctx);
+/* msplat_commit (bindings.h:23): nothing to commit on HIP; kept for the interface. */
+int msplat_hip_commit(msplat_hip_ctx *ctx);
+
+/* gpu_empty / gpu_zeros (bindings.h:18-19): device allocation. */
+int msplat_hip_malloc(size_t bytes, int zero, void **out);
+int msplat_hip_free(void *ptr);
+/* Copies on the context's stream; kind 0 = host->device, 1 = device->host (syncs),
+ * 2 = device->device (msplat_copy_buffer, bindings.h:143). */
+int msplat_hip_memcpy(msplat_hip_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
+int msplat_hip_memset(msplat_hip_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* ── render and train (bindings.h:81-118) ───────────────────────────────────── */
+
+/* msplat_render (bindings.h:81-91): projection + SH, (tile|depth) sort, forward
+ * compositing. background: host float[3]. *out_img receives the context's cached
+ * (H, W, 3) image (device), valid until the next call. */
+int msplat_hip_render(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam,
+                      const float background[3], float **out_img);
+
+/* msplat_train_step (bindings.h:100-118) without the prior-guided extras: forward,
+ * L1 + SSIM loss (ssim_weight, loss_inv_n = 1/(3HW)), backward, Adam on all six groups
+ * (SH groups fused into the projection backward), densification statistics.
+ * gt: device (H, W, 3). radii_out (device, N int32) and loss_out (host; syncs) may be
+ * NULL. The loss is also kept on the device (msplat_hip_last_loss). */
+int msplat_hip_train_step(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam,
+                          const float background[3], const float *gt, float ssim_weight, float loss_inv_n,
+                          const msplat_hip_adam *adam, const msplat_hip_grad_stats *stats, float inv_max_dim,
+                          int32_t *radii_out, float *loss_out);
+
+/* The loss of the last train step (host; syncs). */
+int msplat_hip_last_loss(msplat_hip_ctx *ctx, float *loss);
+
+/* ── differentiable rendering (new: torch.autograd.Function backend) ─────────
+ * render_backward takes dL/d(image) (device (H, W, 3)) of the last msplat_hip_render
+ * on this context and writes the parameter gradients (device, reference layouts). The
+ * view direction of the SH colour is not differentiated (as in the reference,
+ * cpu:1111-1137). Any output pointer may be NULL. */
+typedef struct {
+    float *v_means, *v_scales, *v_quats, *v_features_dc, *v_features_rest, *v_opacities;
+} msplat_hip_grads;
+int msplat_hip_render_backward(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam,
+                               const float background[3], const float *v_img, const msplat_hip_grads *out);
+
+/* ── the per-frame cache (the reference's Frame / FusedTensorCache) ──────────
+ * Device pointers to the intermediate buffers of the last render/train call, for tests
+ * and tools. Per gaussian: xys (N,2) as float4 records (x, y, conic.x, conic.y) and
+ * (conic.z, sigmoid(opacity), r, g) plus blue (N), depths, radii, aabb (N,2), tile
+ * counts, prefix offsets. Per intersection (sorted): keys (tile<<32 | depth bits),
+ * gaussian ids. tile_start (T+1): tile t owns sorted intersections
+ * [tile_start[t], tile_start[t+1]). Per pixel: image, final_T, final_idx (index into
+ * the sorted intersections), dL/d(image). Per gaussian reduced gradients (N,2)/(N,3)/
+ * (N,3)/(N) are filled when MSPLAT_FRAME_KEEP_GRADS is set. */
+typedef struct {
+    int32_t num_points, num_tiles;
+    int64_t num_isects;
+    float *splat_a, *splat_b, *splat_blue, *depths, *aabb;
+    int32_t *radii;
+    uint32_t *tile_count, *offsets;
+    uint64_t *isect_keys;
+    int32_t *isect_gid;
+    int32_t *tile_start;
+    float *image, *final_T, *v_image;
+    int32_t *final_idx;
+    float *v_xy, *v_conic, *v_rgb, *v_opacity;
+} msplat_hip_frame;
+int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *out);
+enum { MSPLAT_FRAME_KEEP_GRADS = 1 };
+int msplat_hip_set_flags(msplat_hip_ctx *ctx, int flags);
+
+/* Stage-wise entry points over the frame cache (test hooks, like the reference's
+ * _core._gpu_* hooks, python/bindings.cpp:655-738). Each runs one stage of
+ * msplat_render / msplat_train_step on the context's stream. */
+int msplat_hip_stage_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam);
+/* Binning from the frame's projection outputs (which a caller may overwrite first). */
+int msplat_hip_stage_bin(msplat_hip_ctx *ctx, const msplat_hip_camera *cam);
+int msplat_hip_stage_raster_forward(msplat_hip_ctx *ctx, const msplat_hip_camera *cam, const float background[3]);
+/* L1 + SSIM loss of the frame image against gt; fills v_image, returns the loss (syncs). */
+int msplat_hip_stage_loss(msplat_hip_ctx *ctx, const msplat_hip_camera *cam, const float *gt, float ssim_weight,
+                          float loss_inv_n, float *loss_out);
+
+/* ── densification and maintenance (bindings.h:120-147) ─────────────────────── */
+
+/* msplat_densify (bindings.h:120-135): classify, split (2 children, PCG normals),
+ * duplicate, cull, stable in-place compaction of the 18 buffers (params, exp_avg,
+ * exp_avg_sq), each with `capacity` rows. Syncs (returns the new count in *new_count). */
+int msplat_hip_densify(msplat_hip_ctx *ctx, int32_t n, int32_t capacity, float grad_thresh, float size_thresh,
+                       float screen_thresh, int check_screen, float cull_alpha_thresh, float cull_scale_thresh,
+                       float cull_screen_size, int check_huge, const float *xys_grad_norm, const float *vis_counts,
+                       const float *max_2d_size, float half_max_dim, float *const bufs[18], int32_t fr_stride,
+                       uint32_t seed, int32_t *new_count);
+
+/* msplat_opacity_reset (bindings.h:137-140). */
+int msplat_hip_opacity_reset(msplat_hip_ctx *ctx, float *opacities, float *exp_avg, float *exp_avg_sq, int32_t n,
+                             float reset_logit);
+
+/* msplat_radix_sort (bindings.h:145-147): stable LSD sort of keys (u64, device) with
+ * int32 payload (device) by the low key_bits bits, in place. */
+int msplat_hip_radix_sort(msplat_hip_ctx *ctx, uint64_t *keys, int32_t *vals, uint32_t n, int key_bits);
+
+/* msplat_knn3_mean_dist (bindings.h:149-151): exact mean distance to the 3 nearest
+ * other points. points (n,3) and mean_dist (n) are device pointers. */
+int msplat_hip_knn3_mean_dist(msplat_hip_ctx *ctx, const float *points, uint32_t n, float *mean_dist);
+
+/* msplat_init_gaussians (bindings.h:153-156). */
+int msplat_hip_init_gaussians(msplat_hip_ctx *ctx, const float *means, const uint8_t *rgb, uint32_t n, uint32_t seed,
+                              float opacity_logit, float *scales, float *quats, float *features_dc, float *opacities);
+
+/* msplat_image_metrics (bindings.h:170-172): PSNR, SSIM (clamp-to-edge), L1. Syncs. */
+int msplat_hip_image_metrics(msplat_hip_ctx *ctx, const float *rendered, const float *gt, int h, int w,
+                             double out[3]);
+
+/* msplat_pack_rgba8 (bindings.h:174-175): out is a device (n,4) uint8 buffer. */
+int msplat_hip_pack_rgba8(msplat_hip_ctx *ctx, const float *img, uint32_t n, uint8_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,643 @@
+// C ABI of the HIP backend (include/msplat_hip.h): contexts, device memory, and the
+// render / train / backward pipelines over the gfx950 kernels.
+//
+// A context owns one stream and a grow-only workspace: per-gaussian projection records,
+// per-intersection keys/ids/gradient slots, per-tile ranges and per-pixel buffers. The
+// intersection count is read back once per view (after the per-block count scan) to
+// size the sort; everything else stays on the device.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "common.h"
+#include "kernels.h"
+
+using namespace msplat;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIPCHECK(expr)                                                                                           \
+    do {                                                                                                         \
+        hipError_t e_ = (expr);                                                                                  \
+        if (e_ != hipSuccess) return set_error(MSPLAT_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHECKED(expr)               \
+    do {                            \
+        int rc_ = (expr);           \
+        if (rc_ != MSPLAT_OK) return rc_; \
+    } while (0)
+
+struct Buf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t v) { *p = v; }
+
+}  // namespace
+
+struct msplat_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int flags = 0;
+    // per gaussian
+    Buf splat_a, splat_b, splat_blue, depths, aabb, rect, radii, count, offsets, block_sums;
+    Buf v_xy, v_conic, v_rgb, v_opac;
+    // per intersection
+    Buf keys0, keys1, vals0, vals1, slot_a, slot_b, slot_c, hist, digit_total;
+    // per tile / pixel
+    Buf tile_start, image, final_T, final_idx, v_image, pix_loss, row_loss;
+    Buf grad_scratch;  // stands in for NULL gradient outputs of render_backward
+    // device scalars: [0] u64 isect total, [8] u32 total, [16] double loss sum, [24] float loss
+    Buf scalars;
+    uint64_t *h_total = nullptr;  // pinned
+    // frame state
+    int n = 0, ntiles = 0;
+    uint32_t width = 0, height = 0;
+    int tiles_x = 0, tiles_y = 0;
+    int64_t M = 0;
+    uint64_t *keys_sorted = nullptr;
+    int32_t *gid_sorted = nullptr;
+    float bg[3] = {0, 0, 0};
+
+    int grow(Buf &b, size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        if (b.cap >= bytes) return MSPLAT_OK;
+        if (b.p) {
+            HIPCHECK(hipStreamSynchronize(stream));
+            HIPCHECK(hipFree(b.p));
+            b.p = nullptr;
+            b.cap = 0;
+        }
+        size_t cap = bytes + bytes / 4;  // headroom against regrowth
+        HIPCHECK(hipMalloc(&b.p, cap));
+        b.cap = cap;
+        return MSPLAT_OK;
+    }
+
+    void release(Buf &b) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+    }
+
+    ~msplat_hip_ctx() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        Buf *all[] = {&splat_a, &splat_b, &splat_blue, &depths, &aabb, &rect, &radii, &count, &offsets, &block_sums,
+                      &v_xy, &v_conic, &v_rgb, &v_opac, &keys0, &keys1, &vals0, &vals1, &slot_a, &slot_b, &slot_c,
+                      &hist, &digit_total, &tile_start, &image, &final_T, &final_idx, &v_image, &pix_loss,
+                      &row_loss, &scalars, &grad_scratch};
+        for (Buf *b : all) release(*b);
+        if (h_total) (void)hipHostFree(h_total);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+
+    uint64_t *d_total64() { return (uint64_t *)scalars.p; }
+    uint32_t *d_total32() { return (uint32_t *)((char *)scalars.p + 8); }
+    double *d_loss_sum() { return (double *)((char *)scalars.p + 16); }
+    float *d_loss() { return (float *)((char *)scalars.p + 24); }
+};
+
+namespace {
+
+int check_camera(const msplat_hip_camera *c) {
+    if (!c) return set_error(MSPLAT_ERR_ARG, "camera is NULL");
+    if (c->width == 0 || c->height == 0) return set_error(MSPLAT_ERR_ARG, "image size must be positive");
+    if (c->tiles_x != (int)((c->width + TILE - 1) / TILE) || c->tiles_y != (int)((c->height + TILE - 1) / TILE))
+        return set_error(MSPLAT_ERR_ARG, "tile bounds (%d, %d) do not match %ux%u", c->tiles_x, c->tiles_y, c->width,
+                         c->height);
+    if (c->tiles_x > 0xFFFF || c->tiles_y > 0xFFFF) return set_error(MSPLAT_ERR_ARG, "image too large");
+    if (c->degree > 4) return set_error(MSPLAT_ERR_ARG, "SH degree above 4");
+    if (c->degrees_to_use > c->degree)
+        return set_error(MSPLAT_ERR_ARG, "degrees_to_use (%u) exceeds the stored SH degree (%u)", c->degrees_to_use,
+                         c->degree);
+    return MSPLAT_OK;
+}
+
+int check_gaussians(const msplat_hip_gaussians *g) {
+    if (!g) return set_error(MSPLAT_ERR_ARG, "gaussians is NULL");
+    if (g->num_points < 0) return set_error(MSPLAT_ERR_ARG, "negative gaussian count");
+    if (g->num_points > 0 && (!g->means || !g->scales || !g->quats || !g->features_dc || !g->opacities))
+        return set_error(MSPLAT_ERR_ARG, "a gaussian parameter pointer is NULL");
+    return MSPLAT_OK;
+}
+
+int tile_bits(int ntiles) {
+    int b = 0;
+    while ((1 << b) < ntiles) b++;
+    return b;
+}
+
+// projection + count scan; returns with ctx->M read back
+int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam) {
+    const int n = g->num_points;
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    const int nblocks = (n + 255) / 256;
+    CHECKED(ctx->grow(ctx->splat_a, 16 * N));
+    CHECKED(ctx->grow(ctx->splat_b, 16 * N));
+    CHECKED(ctx->grow(ctx->splat_blue, 4 * N));
+    CHECKED(ctx->grow(ctx->depths, 4 * N));
+    CHECKED(ctx->grow(ctx->aabb, 8 * N));
+    CHECKED(ctx->grow(ctx->rect, 8 * N));
+    CHECKED(ctx->grow(ctx->radii, 4 * N));
+    CHECKED(ctx->grow(ctx->count, 4 * N));
+    CHECKED(ctx->grow(ctx->offsets, 4 * N));
+    CHECKED(ctx->grow(ctx->block_sums, 4 * (size_t)(nblocks + 1)));
+    CHECKED(ctx->grow(ctx->scalars, 64));
+    ctx->n = n;
+    ctx->width = cam->width;
+    ctx->height = cam->height;
+    ctx->tiles_x = cam->tiles_x;
+    ctx->tiles_y = cam->tiles_y;
+    ctx->ntiles = cam->tiles_x * cam->tiles_y;
+    ProjectArgs a;
+    a.n = n;
+    a.means = g->means;
+    a.scales = g->scales;
+    a.quats = g->quats;
+    a.dc = g->features_dc;
+    a.rest = g->features_rest;
+    a.opac = g->opacities;
+    a.cam = *cam;
+    a.splat_a = (float4 *)ctx->splat_a.p;
+    a.splat_b = (float4 *)ctx->splat_b.p;
+    a.splat_blue = (float *)ctx->splat_blue.p;
+    a.depths = (float *)ctx->depths.p;
+    a.aabb = (float2 *)ctx->aabb.p;
+    a.rect = (uint2 *)ctx->rect.p;
+    a.radii = (int32_t *)ctx->radii.p;
+    a.count = (uint32_t *)ctx->count.p;
+    a.block_sums = (uint32_t *)ctx->block_sums.p;
+    launch_project(a, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    return MSPLAT_OK;
+}
+
+// count scan, M read-back, key generation, sort, tile ranges
+int do_bin(msplat_hip_ctx *ctx) {
+    const int n = ctx->n;
+    const int nblocks = (n + 255) / 256;
+    if (!ctx->h_total) HIPCHECK(hipHostMalloc((void **)&ctx->h_total, 64, hipHostMallocDefault));
+    launch_scan_blocks((uint32_t *)ctx->block_sums.p, nblocks, ctx->d_total64(), ctx->d_total32(), ctx->stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(ctx->h_total, ctx->d_total64(), 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    const uint64_t M = *ctx->h_total;
+    if (M > 0x7FFFFFFFull)
+        return set_error(MSPLAT_ERR_RANGE, "more than 2^31 tile intersections (%llu); lower the resolution",
+                         (unsigned long long)M);
+    ctx->M = (int64_t)M;
+    const size_t Mc = M > 0 ? M : 1;
+    CHECKED(ctx->grow(ctx->keys0, 8 * Mc));
+    CHECKED(ctx->grow(ctx->keys1, 8 * Mc));
+    CHECKED(ctx->grow(ctx->vals0, 4 * Mc));
+    CHECKED(ctx->grow(ctx->vals1, 4 * Mc));
+    CHECKED(ctx->grow(ctx->hist, 4 * radix_hist_words((uint32_t)Mc)));
+    CHECKED(ctx->grow(ctx->digit_total, 4 * 256));
+    CHECKED(ctx->grow(ctx->tile_start, 4 * (size_t)(ctx->ntiles + 1)));
+    KeygenArgs k;
+    k.n = n;
+    k.tiles_x = ctx->tiles_x;
+    k.count = (const uint32_t *)ctx->count.p;
+    k.block_bases = (const uint32_t *)ctx->block_sums.p;
+    k.rect = (const uint2 *)ctx->rect.p;
+    k.depths = (const float *)ctx->depths.p;
+    k.offsets = (uint32_t *)ctx->offsets.p;
+    k.keys = (uint64_t *)ctx->keys0.p;
+    k.vals = (int32_t *)ctx->vals0.p;
+    launch_keygen(k, ctx->stream);
+    uint64_t *kin = (uint64_t *)ctx->keys0.p, *kout = (uint64_t *)ctx->keys1.p;
+    int32_t *vin = (int32_t *)ctx->vals0.p, *vout = (int32_t *)ctx->vals1.p;
+    const int bits = 32 + tile_bits(ctx->ntiles);
+    for (int shift = 0; shift < bits; shift += 8) {
+        launch_radix_pass(kin, vin, kout, vout, ctx->d_total32(), (uint32_t)M, shift, (uint32_t *)ctx->hist.p,
+                          (uint32_t *)ctx->digit_total.p, ctx->stream);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    ctx->keys_sorted = kin;
+    ctx->gid_sorted = vin;
+    launch_tile_ranges(kin, ctx->d_total32(), (uint32_t)M, ctx->ntiles, (int32_t *)ctx->tile_start.p, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    return MSPLAT_OK;
+}
+
+int ensure_pixels(msplat_hip_ctx *ctx) {
+    const size_t P = (size_t)ctx->width * ctx->height;
+    CHECKED(ctx->grow(ctx->image, 12 * P));
+    CHECKED(ctx->grow(ctx->final_T, 4 * P));
+    CHECKED(ctx->grow(ctx->final_idx, 4 * P));
+    CHECKED(ctx->grow(ctx->v_image, 12 * P));
+    CHECKED(ctx->grow(ctx->pix_loss, 4 * P));
+    CHECKED(ctx->grow(ctx->row_loss, 8 * (size_t)ctx->height));
+    return MSPLAT_OK;
+}
+
+RasterArgs raster_args(msplat_hip_ctx *ctx) {
+    RasterArgs r;
+    r.width = ctx->width;
+    r.height = ctx->height;
+    r.tiles_x = ctx->tiles_x;
+    r.tiles_y = ctx->tiles_y;
+    r.tile_start = (const int32_t *)ctx->tile_start.p;
+    r.gid = ctx->gid_sorted;
+    r.splat_a = (const float4 *)ctx->splat_a.p;
+    r.splat_b = (const float4 *)ctx->splat_b.p;
+    r.splat_blue = (const float *)ctx->splat_blue.p;
+    for (int i = 0; i < 3; i++) r.bg[i] = ctx->bg[i];
+    r.image = (float *)ctx->image.p;
+    r.final_T = (float *)ctx->final_T.p;
+    r.final_idx = (int32_t *)ctx->final_idx.p;
+    r.v_image = (const float *)ctx->v_image.p;
+    r.rect = (const uint2 *)ctx->rect.p;
+    r.offsets = (const uint32_t *)ctx->offsets.p;
+    r.slot_a = (float4 *)ctx->slot_a.p;
+    r.slot_b = (float4 *)ctx->slot_b.p;
+    r.slot_c = (float *)ctx->slot_c.p;
+    return r;
+}
+
+int do_raster_forward(msplat_hip_ctx *ctx, const float bg[3]) {
+    for (int i = 0; i < 3; i++) ctx->bg[i] = bg ? bg[i] : 0.f;
+    CHECKED(ensure_pixels(ctx));
+    launch_raster_forward(raster_args(ctx), ctx->stream);
+    HIPCHECK(hipGetLastError());
+    return MSPLAT_OK;
+}
+
+int do_loss(msplat_hip_ctx *ctx, const float *gt, float ssim_weight, float inv_n) {
+    LossArgs l;
+    l.width = ctx->width;
+    l.height = ctx->height;
+    l.img = (const float *)ctx->image.p;
+    l.gt = gt;
+    l.ssim_weight = ssim_weight;
+    l.inv_n = inv_n;
+    l.v_img = (float *)ctx->v_image.p;
+    l.pix_loss = (float *)ctx->pix_loss.p;
+    l.row_loss = (double *)ctx->row_loss.p;
+    l.loss_sum = ctx->d_loss_sum();
+    l.loss_mean = ctx->d_loss();
+    launch_loss(l, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    return MSPLAT_OK;
+}
+
+int do_raster_backward(msplat_hip_ctx *ctx) {
+    const size_t Mc = ctx->M > 0 ? (size_t)ctx->M : 1;
+    CHECKED(ctx->grow(ctx->slot_a, 16 * Mc));
+    CHECKED(ctx->grow(ctx->slot_b, 16 * Mc));
+    CHECKED(ctx->grow(ctx->slot_c, 4 * Mc));
+    launch_raster_backward(raster_args(ctx), ctx->stream);
+    HIPCHECK(hipGetLastError());
+    return MSPLAT_OK;
+}
+
+int backward_args(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam, BackwardArgs &b) {
+    memset(&b, 0, sizeof b);
+    b.n = g->num_points;
+    b.means = g->means;
+    b.means_rw = g->means;
+    b.scales = g->scales;
+    b.quats = g->quats;
+    b.dc = g->features_dc;
+    b.rest = g->features_rest;
+    b.opac = g->opacities;
+    b.cam = *cam;
+    b.radii = (const int32_t *)ctx->radii.p;
+    b.count = (const uint32_t *)ctx->count.p;
+    b.offsets = (const uint32_t *)ctx->offsets.p;
+    b.splat_a = (const float4 *)ctx->splat_a.p;
+    b.splat_b = (const float4 *)ctx->splat_b.p;
+    b.slot_a = (const float4 *)ctx->slot_a.p;
+    b.slot_b = (const float4 *)ctx->slot_b.p;
+    b.slot_c = (const float *)ctx->slot_c.p;
+    if (ctx->flags & MSPLAT_FRAME_KEEP_GRADS) {
+        const size_t N = (size_t)(b.n > 0 ? b.n : 1);
+        CHECKED(ctx->grow(ctx->v_xy, 8 * N));
+        CHECKED(ctx->grow(ctx->v_conic, 12 * N));
+        CHECKED(ctx->grow(ctx->v_rgb, 12 * N));
+        CHECKED(ctx->grow(ctx->v_opac, 4 * N));
+        b.v_xy = (float *)ctx->v_xy.p;
+        b.v_conic = (float *)ctx->v_conic.p;
+        b.v_rgb = (float *)ctx->v_rgb.p;
+        b.v_opac = (float *)ctx->v_opac.p;
+    }
+    return MSPLAT_OK;
+}
+
+}  // namespace
+
+// ═══════════════════════════════ public API ════════════════════════════════
+
+extern "C" {
+
+const char *msplat_hip_last_error(void) { return g_last_error.c_str(); }
+
+int msplat_hip_ctx_create(int device, void *stream, msplat_hip_ctx **out) {
+    if (!out) return set_error(MSPLAT_ERR_ARG, "out is NULL");
+    HIPCHECK(hipSetDevice(device));
+    auto *ctx = new msplat_hip_ctx();
+    ctx->device = device;
+    if (stream) {
+        ctx->stream = (hipStream_t)stream;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete ctx;
+            return set_error(MSPLAT_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+        }
+        ctx->own_stream = true;
+    }
+    *out = ctx;
+    return MSPLAT_OK;
+}
+
+int msplat_hip_ctx_destroy(msplat_hip_ctx *ctx) {
+    delete ctx;
+    return MSPLAT_OK;
+}
+
+int msplat_hip_ctx_set_stream(msplat_hip_ctx *ctx, void *stream) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    if (ctx->own_stream) HIPCHECK(hipStreamDestroy(ctx->stream));
+    ctx->own_stream = false;
+    ctx->stream = (hipStream_t)stream;
+    return MSPLAT_OK;
+}
+
+void *msplat_hip_ctx_stream(msplat_hip_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int msplat_hip_sync(msplat_hip_ctx *ctx) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    return MSPLAT_OK;
+}
+
+int msplat_hip_commit(msplat_hip_ctx *ctx) { return ctx ? MSPLAT_OK : set_error(MSPLAT_ERR_ARG, "ctx is NULL"); }
+
+int msplat_hip_malloc(size_t bytes, int zero, void **out) {
+    if (!out) return set_error(MSPLAT_ERR_ARG, "out is NULL");
+    HIPCHECK(hipMalloc(out, bytes ? bytes : 4));
+    if (zero) {
+        HIPCHECK(hipMemset(*out, 0, bytes ? bytes : 4));
+        HIPCHECK(hipDeviceSynchronize());
+    }
+    return MSPLAT_OK;
+}
+
+int msplat_hip_free(void *ptr) {
+    if (ptr) HIPCHECK(hipFree(ptr));
+    return MSPLAT_OK;
+}
+
+int msplat_hip_memcpy(msplat_hip_ctx *ctx, void *dst, const void *src, size_t bytes, int kind) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    if (bytes == 0) return MSPLAT_OK;
+    if (kind == 0) {
+        HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));  // the host buffer may be reused on return
+    } else if (kind == 1) {
+        HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    } else if (kind == 2) {
+        HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        return set_error(MSPLAT_ERR_ARG, "unknown copy kind %d", kind);
+    }
+    return MSPLAT_OK;
+}
+
+int msplat_hip_memset(msplat_hip_ctx *ctx, void *dst, int value, size_t bytes) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    if (bytes) HIPCHECK(hipMemsetAsync(dst, value, bytes, ctx->stream));
+    return MSPLAT_OK;
+}
+
+int msplat_hip_stage_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(check_gaussians(g));
+    CHECKED(check_camera(cam));
+    return do_project(ctx, g, cam);
+}
+
+int msplat_hip_stage_bin(msplat_hip_ctx *ctx, const msplat_hip_camera *cam) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(check_camera(cam));
+    if (cam->tiles_x != ctx->tiles_x || cam->tiles_y != ctx->tiles_y)
+        return set_error(MSPLAT_ERR_ARG, "camera differs from the projected frame");
+    // rectangles, counts and per-block sums are re-derived from the frame's projection
+    // outputs, so that a caller may overwrite them before binning
+    launch_rect_count(ctx->n, (const float4 *)ctx->splat_a.p, (const float2 *)ctx->aabb.p,
+                      (const int32_t *)ctx->radii.p, ctx->tiles_x, ctx->tiles_y, (uint2 *)ctx->rect.p,
+                      (uint32_t *)ctx->count.p, (uint32_t *)ctx->block_sums.p, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    return do_bin(ctx);
+}
+
+int msplat_hip_stage_raster_forward(msplat_hip_ctx *ctx, const msplat_hip_camera *cam, const float background[3]) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(check_camera(cam));
+    return do_raster_forward(ctx, background);
+}
+
+int msplat_hip_stage_loss(msplat_hip_ctx *ctx, const msplat_hip_camera *cam, const float *gt, float ssim_weight,
+                          float loss_inv_n, float *loss_out) {
+    if (!ctx || !gt) return set_error(MSPLAT_ERR_ARG, "ctx or gt is NULL");
+    CHECKED(check_camera(cam));
+    CHECKED(do_loss(ctx, gt, ssim_weight, loss_inv_n));
+    if (loss_out) {
+        HIPCHECK(hipMemcpyAsync(loss_out, ctx->d_loss(), 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
+    return MSPLAT_OK;
+}
+
+int msplat_hip_render(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam,
+                      const float background[3], float **out_img) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(check_gaussians(g));
+    CHECKED(check_camera(cam));
+    CHECKED(do_project(ctx, g, cam));
+    CHECKED(do_bin(ctx));
+    CHECKED(do_raster_forward(ctx, background));
+    if (out_img) *out_img = (float *)ctx->image.p;
+    return MSPLAT_OK;
+}
+
+int msplat_hip_train_step(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam,
+                          const float background[3], const float *gt, float ssim_weight, float loss_inv_n,
+                          const msplat_hip_adam *adam, const msplat_hip_grad_stats *stats, float inv_max_dim,
+                          int32_t *radii_out, float *loss_out) {
+    if (!ctx || !gt || !adam || !stats) return set_error(MSPLAT_ERR_ARG, "ctx, gt, adam or stats is NULL");
+    CHECKED(check_gaussians(g));
+    CHECKED(check_camera(cam));
+    for (int i = 0; i < 6; i++)
+        if (g->num_points > 0 && (!adam->exp_avg[i] || !adam->exp_avg_sq[i]))
+            return set_error(MSPLAT_ERR_ARG, "Adam moment %d is NULL", i);
+    if (g->num_points > 0 && cam->degree > 0 && !g->features_rest)
+        return set_error(MSPLAT_ERR_ARG, "features_rest is NULL");
+    CHECKED(do_project(ctx, g, cam));
+    CHECKED(do_bin(ctx));
+    CHECKED(do_raster_forward(ctx, background));
+    CHECKED(do_loss(ctx, gt, ssim_weight, loss_inv_n));
+    CHECKED(do_raster_backward(ctx));
+    BackwardArgs b;
+    CHECKED(backward_args(ctx, g, cam, b));
+    for (int i = 0; i < 6; i++) {
+        b.m1[i] = adam->exp_avg[i];
+        b.m2[i] = adam->exp_avg_sq[i];
+        b.step[i] = adam->step_size[i];
+        b.bc2[i] = adam->bc2_sqrt[i];
+    }
+    b.beta1 = adam->beta1;
+    b.beta2 = adam->beta2;
+    b.eps = adam->eps;
+    b.vis = stats->vis_counts;
+    b.gn = stats->xys_grad_norm;
+    b.ms = stats->max_2d_size;
+    b.inv_max_dim = inv_max_dim;
+    launch_backward(b, BWD_ADAM, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    if (radii_out && g->num_points > 0)
+        HIPCHECK(hipMemcpyAsync(radii_out, ctx->radii.p, 4 * (size_t)g->num_points, hipMemcpyDeviceToDevice,
+                                ctx->stream));
+    if (loss_out) {
+        HIPCHECK(hipMemcpyAsync(loss_out, ctx->d_loss(), 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
+    return MSPLAT_OK;
+}
+
+int msplat_hip_last_loss(msplat_hip_ctx *ctx, float *loss) {
+    if (!ctx || !loss) return set_error(MSPLAT_ERR_ARG, "ctx or loss is NULL");
+    if (!ctx->scalars.p) return set_error(MSPLAT_ERR_ARG, "no train step has run on this context");
+    HIPCHECK(hipMemcpyAsync(loss, ctx->d_loss(), 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    return MSPLAT_OK;
+}
+
+int msplat_hip_render_backward(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam,
+                               const float background[3], const float *v_img, const msplat_hip_grads *out) {
+    if (!ctx || !v_img || !out) return set_error(MSPLAT_ERR_ARG, "ctx, v_img or out is NULL");
+    CHECKED(check_gaussians(g));
+    CHECKED(check_camera(cam));
+    if (g->num_points != ctx->n || cam->width != ctx->width || cam->height != ctx->height)
+        return set_error(MSPLAT_ERR_ARG, "render_backward does not match the last render on this context");
+    (void)background;
+    const size_t P = (size_t)ctx->width * ctx->height;
+    HIPCHECK(hipMemcpyAsync(ctx->v_image.p, v_img, 12 * P, hipMemcpyDeviceToDevice, ctx->stream));
+    CHECKED(do_raster_backward(ctx));
+    BackwardArgs b;
+    CHECKED(backward_args(ctx, g, cam, b));
+    const size_t N = (size_t)(g->num_points > 0 ? g->num_points : 1);
+    const uint32_t nb = sh_bases(cam->degree);
+    // NULL outputs go to a scratch area
+    float *tmp = nullptr;
+    size_t need = N * (3 + 3 + 4 + 3 + (nb - 1) * 3 + 1) * 4;
+    if (!out->v_means || !out->v_scales || !out->v_quats || !out->v_features_dc || !out->v_features_rest ||
+        !out->v_opacities) {
+        CHECKED(ctx->grow(ctx->grad_scratch, need));
+        tmp = (float *)ctx->grad_scratch.p;
+    }
+    float *cur = tmp;
+    auto pick = [&](float *p, size_t count) {
+        if (p) return p;
+        float *r = cur;
+        cur += count;
+        return r;
+    };
+    b.g_means = pick(out->v_means, 3 * N);
+    b.g_scales = pick(out->v_scales, 3 * N);
+    b.g_quats = pick(out->v_quats, 4 * N);
+    b.g_dc = pick(out->v_features_dc, 3 * N);
+    b.g_rest = pick(out->v_features_rest, (nb - 1) * 3 * N);
+    b.g_opac = pick(out->v_opacities, N);
+    launch_backward(b, BWD_GRADS, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    return MSPLAT_OK;
+}
+
+int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *f) {
+    if (!ctx || !f) return set_error(MSPLAT_ERR_ARG, "ctx or out is NULL");
+    memset(f, 0, sizeof *f);
+    f->num_points = ctx->n;
+    f->num_tiles = ctx->ntiles;
+    f->num_isects = ctx->M;
+    f->splat_a = (float *)ctx->splat_a.p;
+    f->splat_b = (float *)ctx->splat_b.p;
+    f->splat_blue = (float *)ctx->splat_blue.p;
+    f->depths = (float *)ctx->depths.p;
+    f->aabb = (float *)ctx->aabb.p;
+    f->radii = (int32_t *)ctx->radii.p;
+    f->tile_count = (uint32_t *)ctx->count.p;
+    f->offsets = (uint32_t *)ctx->offsets.p;
+    f->isect_keys = ctx->keys_sorted;
+    f->isect_gid = ctx->gid_sorted;
+    f->tile_start = (int32_t *)ctx->tile_start.p;
+    f->image = (float *)ctx->image.p;
+    f->final_T = (float *)ctx->final_T.p;
+    f->final_idx = (int32_t *)ctx->final_idx.p;
+    f->v_image = (float *)ctx->v_image.p;
+    f->v_xy = (float *)ctx->v_xy.p;
+    f->v_conic = (float *)ctx->v_conic.p;
+    f->v_rgb = (float *)ctx->v_rgb.p;
+    f->v_opacity = (float *)ctx->v_opac.p;
+    return MSPLAT_OK;
+}
+
+int msplat_hip_set_flags(msplat_hip_ctx *ctx, int flags) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    ctx->flags = flags;
+    return MSPLAT_OK;
+}
+
+int msplat_hip_radix_sort(msplat_hip_ctx *ctx, uint64_t *keys, int32_t *vals, uint32_t n, int key_bits) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    if (key_bits < 0 || key_bits > 64) return set_error(MSPLAT_ERR_ARG, "key_bits must be in [0, 64]");
+    if (n == 0 || key_bits == 0) return MSPLAT_OK;
+    if (!keys || !vals) return set_error(MSPLAT_ERR_ARG, "keys or vals is NULL");
+    CHECKED(ctx->grow(ctx->keys1, 8 * (size_t)n));
+    CHECKED(ctx->grow(ctx->vals1, 4 * (size_t)n));
+    CHECKED(ctx->grow(ctx->hist, 4 * radix_hist_words(n)));
+    CHECKED(ctx->grow(ctx->digit_total, 4 * 256));
+    CHECKED(ctx->grow(ctx->scalars, 64));
+    uint32_t *dn = ctx->d_total32();
+    fill_u32_kernel<<<1, 1, 0, ctx->stream>>>(dn, n);
+    uint64_t *kin = keys, *kout = (uint64_t *)ctx->keys1.p;
+    int32_t *vin = vals, *vout = (int32_t *)ctx->vals1.p;
+    for (int shift = 0; shift < key_bits; shift += 8) {
+        launch_radix_pass(kin, vin, kout, vout, dn, n, shift, (uint32_t *)ctx->hist.p, (uint32_t *)ctx->digit_total.p,
+                          ctx->stream);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    if (kin != keys) {
+        HIPCHECK(hipMemcpyAsync(keys, kin, 8 * (size_t)n, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHECK(hipMemcpyAsync(vals, vin, 4 * (size_t)n, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    HIPCHECK(hipGetLastError());
+    // the frame's sorted intersections lived in keys1/vals1; they are no longer valid
+    ctx->keys_sorted = nullptr;
+    ctx->gid_sorted = nullptr;
+    return MSPLAT_OK;
+}
+
+}  // extern "C"

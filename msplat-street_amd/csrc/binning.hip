@@ -1,0 +1,304 @@
+// Projection + SH colour, (tile | depth) key generation, stable LSD radix sort and
+// per-tile range extraction.
+//
+// Layout in HBM (per gaussian, SoA records written once per view):
+//   splat_a = (x, y, conic.x, conic.y)   float4
+//   splat_b = (conic.z, sigmoid(opacity), r, g)   float4
+//   splat_blue = b, depths, aabb (float2), rect (x0 | x1 << 16, y0 | y1 << 16), count
+// Per intersection: 64-bit keys (tile << 32 | fp32 depth bits) with the gaussian id as
+// payload, emitted in ascending gaussian order so that a stable sort reproduces the
+// reference's per-tile (depth, id) order (core/cpu/msplat_cpu.cpp:505-537).
+#include "kernels.h"
+#include "proj_math.h"
+
+namespace msplat {
+
+// Block-wide exclusive scan of one u32 per thread (256 threads, 4 waves).
+__device__ inline uint32_t block_scan_u32(uint32_t v, uint32_t &total, uint32_t *lds4) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) lds4[w] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+    total = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (i < w) pre += lds4[i];
+        total += lds4[i];
+    }
+    __syncthreads();
+    return pre + x - v;
+}
+
+// ── projection + SH colour (project_and_sh, msplat_cpu.cpp:452-500) ────────────
+__global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
+    __shared__ uint32_t lds4[4];
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    uint32_t count = 0;
+    if (g < a.n) {
+        a.radii[g] = 0;
+        a.count[g] = 0;
+        const float p[3] = {a.means[3 * g], a.means[3 * g + 1], a.means[3 * g + 2]};
+        float pv[3];
+        cam_space(a.cam.viewmat, p, pv);
+        if (pv[2] > a.cam.clip_thresh) {
+            const float s[3] = {exp_cr(a.scales[3 * g]), exp_cr(a.scales[3 * g + 1]), exp_cr(a.scales[3 * g + 2])};
+            const float q[4] = {a.quats[4 * g], a.quats[4 * g + 1], a.quats[4 * g + 2], a.quats[4 * g + 3]};
+            float cov[6], c2[3], conic[3], radius;
+            covariance3d(s, a.cam.glob_scale, q, cov);
+            const float tfx = 0.5f * (float)a.cam.width / a.cam.fx, tfy = 0.5f * (float)a.cam.height / a.cam.fy;
+            ewa_cov2d(cov, a.cam.viewmat, a.cam.fx, a.cam.fy, tfx, tfy, pv, c2);
+            if (conic_radius(c2, conic, radius)) {
+                float px, py;
+                to_pixel(a.cam.projmat, p, a.cam.width, a.cam.height, a.cam.cx, a.cam.cy, px, py);
+                const float ax = ceilf(3.f * sqrtf(c2[0])), ay = ceilf(3.f * sqrtf(c2[2]));
+                int x0, y0, x1, y1;
+                tile_rect(px, py, ax, ay, a.cam.tiles_x, a.cam.tiles_y, x0, y0, x1, y1);
+                if ((x1 - x0) * (y1 - y0) > 0) {
+                    float d[3], col[3];
+                    view_direction(p, a.cam.cam_pos, d);
+                    const uint32_t nb = sh_bases(a.cam.degree);
+                    sh_color(a.cam.degrees_to_use, d, a.dc + 3 * (size_t)g, a.rest + (size_t)(nb - 1) * 3 * g, col);
+                    const float opac = 1.f / (1.f + exp_cr(-a.opac[g]));
+                    a.splat_a[g] = make_float4(px, py, conic[0], conic[1]);
+                    a.splat_b[g] = make_float4(conic[2], opac, col[0], col[1]);
+                    a.splat_blue[g] = col[2];
+                    a.depths[g] = pv[2];
+                    a.aabb[g] = make_float2(ax, ay);
+                    a.rect[g] = make_uint2((uint32_t)x0 | ((uint32_t)x1 << 16), (uint32_t)y0 | ((uint32_t)y1 << 16));
+                    a.radii[g] = (int)radius;
+                    count = (uint32_t)((x1 - x0) * (y1 - y0));
+                    a.count[g] = count;
+                }
+            }
+        }
+    }
+    uint32_t total;
+    block_scan_u32(count, total, lds4);
+    if (threadIdx.x == 0) a.block_sums[blockIdx.x] = total;
+}
+
+// Tile rectangles and counts re-derived from a frame's projection outputs (pixel
+// centre in splat_a, aabb, radii), for binning outputs a caller injected.
+__global__ __launch_bounds__(256) void rect_count_kernel(int n, const float4 *splat_a, const float2 *aabb,
+                                                        const int32_t *radii, int tx, int ty, uint2 *rect,
+                                                        uint32_t *count, uint32_t *block_sums) {
+    __shared__ uint32_t lds4[4];
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    uint32_t c = 0;
+    if (g < n) {
+        if (radii[g] > 0) {
+            const float4 A = splat_a[g];
+            const float2 e = aabb[g];
+            int x0, y0, x1, y1;
+            tile_rect(A.x, A.y, e.x, e.y, tx, ty, x0, y0, x1, y1);
+            if ((x1 - x0) * (y1 - y0) > 0) c = (uint32_t)((x1 - x0) * (y1 - y0));
+            rect[g] = make_uint2((uint32_t)x0 | ((uint32_t)x1 << 16), (uint32_t)y0 | ((uint32_t)y1 << 16));
+        }
+        count[g] = c;
+    }
+    uint32_t total;
+    block_scan_u32(c, total, lds4);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of the per-block intersection counts (one block, sequential chunks);
+// writes the block bases and the total (u64, so that > 2^31 is detected).
+__global__ __launch_bounds__(256) void scan_blocks_kernel(uint32_t *sums, int nblocks, uint64_t *total,
+                                                          uint32_t *total32) {
+    __shared__ uint32_t lds4[4];
+    __shared__ uint64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nblocks; base += 256) {
+        const int i = base + threadIdx.x;
+        uint32_t v = i < nblocks ? sums[i] : 0, t;
+        uint32_t ex = block_scan_u32(v, t, lds4);
+        uint64_t c = carry;
+        if (i < nblocks) sums[i] = (uint32_t)(c + ex);
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + t;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *total = carry;
+        *total32 = carry > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)carry;
+    }
+}
+
+// Key emission: gaussian g writes its tiles in row-major order at offsets[g].
+__global__ __launch_bounds__(256) void keygen_kernel(KeygenArgs a) {
+    __shared__ uint32_t lds4[4];
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t c = g < a.n ? a.count[g] : 0;
+    uint32_t total;
+    const uint32_t off = a.block_bases[blockIdx.x] + block_scan_u32(c, total, lds4);
+    if (g >= a.n) return;
+    a.offsets[g] = off;
+    if (c == 0) return;
+    const uint2 r = a.rect[g];
+    const int x0 = r.x & 0xFFFF, x1 = r.x >> 16, y0 = r.y & 0xFFFF, y1 = r.y >> 16;
+    const uint64_t dbits = (uint64_t)__float_as_uint(a.depths[g]);
+    uint32_t k = off;
+    for (int y = y0; y < y1; y++)
+        for (int x = x0; x < x1; x++, k++) {
+            a.keys[k] = ((uint64_t)(uint32_t)(y * a.tiles_x + x) << 32) | dbits;
+            a.vals[k] = g;
+        }
+}
+
+// ── stable LSD radix sort, 8-bit digits ──────────────────────────────────────
+// Each block owns RS_ITEMS consecutive elements: an upsweep histogram, a per-digit
+// scan over blocks, and a scatter that ranks elements stably within the block with
+// wave-level multi-split (ballots over the digit bits).
+
+__global__ __launch_bounds__(256) void rs_hist_kernel(const uint64_t *keys, const uint32_t *n_ptr, int shift,
+                                                     uint32_t *hist, int nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t n = *n_ptr;
+    const uint32_t start = blockIdx.x * RS_ITEMS;
+    for (int r = 0; r < RS_ITEMS / 256; r++) {
+        uint32_t e = start + r * 256 + threadIdx.x;
+        if (e < n) atomicAdd(&h[(uint32_t)(keys[e] >> shift) & 0xFF], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// one block per digit: exclusive scan of that digit's counts over blocks, digit total
+__global__ __launch_bounds__(256) void rs_scan_kernel(uint32_t *hist, int nblocks, uint32_t *digit_total) {
+    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t carry;
+    uint32_t *row = hist + (size_t)blockIdx.x * nblocks;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nblocks; base += 256) {
+        const int i = base + threadIdx.x;
+        uint32_t v = i < nblocks ? row[i] : 0, t;
+        uint32_t ex = block_scan_u32(v, t, lds4);
+        uint32_t c = carry;
+        if (i < nblocks) row[i] = c + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + t;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) digit_total[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(256) void rs_scatter_kernel(const uint64_t *kin, const int32_t *vin, uint64_t *kout,
+                                                        int32_t *vout, const uint32_t *n_ptr, int shift,
+                                                        const uint32_t *hist, int nblocks,
+                                                        const uint32_t *digit_total) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t lds4[4];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // global base of each digit for this block = exclusive digit prefix + block offset
+    {
+        uint32_t t;
+        uint32_t ex = block_scan_u32(digit_total[tid], t, lds4);
+        run[tid] = ex + hist[(size_t)tid * nblocks + blockIdx.x];
+    }
+    const uint32_t n = *n_ptr;
+    const uint32_t start = blockIdx.x * RS_ITEMS;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int r = 0; r < RS_ITEMS / 256; r++) {
+        const uint32_t e = start + r * 256 + tid;
+        const bool valid = e < n;
+        uint64_t key = 0;
+        int32_t val = 0;
+        if (valid) {
+            key = kin[e];
+            val = vin[e];
+        }
+        const uint32_t digit = valid ? (uint32_t)(key >> shift) & 0xFF : 0;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (digit >> b) & 1;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = __popcll(peers & lt);
+#pragma unroll
+        for (int i = 0; i < 4; i++) wcnt[i][tid] = 0;
+        __syncthreads();
+        if (valid && rank == 0) wcnt[w][digit] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {   // counts -> exclusive offsets over the four waves, per digit
+            uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
+            uint32_t base = run[tid];
+            wcnt[0][tid] = base;
+            wcnt[1][tid] = base + c0;
+            wcnt[2][tid] = base + c0 + c1;
+            wcnt[3][tid] = base + c0 + c1 + c2;
+            run[tid] = base + c0 + c1 + c2 + c3;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = wcnt[w][digit] + rank;
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+    }
+}
+
+// tile_start[t] = first sorted intersection with tile >= t, for t in [0, T]
+__global__ __launch_bounds__(256) void tile_ranges_kernel(const uint64_t *keys, const uint32_t *n_ptr, int ntiles,
+                                                         int32_t *tile_start) {
+    const uint32_t n = *n_ptr;
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k > n) return;
+    const int t = k < n ? (int)(keys[k] >> 32) : ntiles;
+    const int tp = k > 0 ? (int)(keys[k - 1] >> 32) : -1;
+    for (int u = tp + 1; u <= t; u++) tile_start[u] = (int32_t)k;
+}
+
+// ── host launchers ───────────────────────────────────────────────────────────
+
+void launch_project(const ProjectArgs &a, hipStream_t s) {
+    const int nb = (a.n + 255) / 256;
+    if (nb > 0) project_kernel<<<nb, 256, 0, s>>>(a);
+}
+
+void launch_rect_count(int n, const float4 *splat_a, const float2 *aabb, const int32_t *radii, int tx, int ty,
+                       uint2 *rect, uint32_t *count, uint32_t *block_sums, hipStream_t s) {
+    const int nb = (n + 255) / 256;
+    if (nb > 0) rect_count_kernel<<<nb, 256, 0, s>>>(n, splat_a, aabb, radii, tx, ty, rect, count, block_sums);
+}
+
+void launch_scan_blocks(uint32_t *sums, int nblocks, uint64_t *total, uint32_t *total32, hipStream_t s) {
+    scan_blocks_kernel<<<1, 256, 0, s>>>(sums, nblocks, total, total32);
+}
+
+size_t radix_hist_words(uint32_t n_max) { return (size_t)256 * ((n_max + RS_ITEMS - 1) / RS_ITEMS + 1); }
+
+void launch_keygen(const KeygenArgs &a, hipStream_t s) {
+    const int nb = (a.n + 255) / 256;
+    if (nb > 0) keygen_kernel<<<nb, 256, 0, s>>>(a);
+}
+
+void launch_radix_pass(const uint64_t *kin, const int32_t *vin, uint64_t *kout, int32_t *vout, const uint32_t *n_dev,
+                       uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s) {
+    const int nblocks = (int)((n_max + RS_ITEMS - 1) / RS_ITEMS);
+    if (nblocks == 0) return;
+    rs_hist_kernel<<<nblocks, 256, 0, s>>>(kin, n_dev, shift, hist, nblocks);
+    rs_scan_kernel<<<256, 256, 0, s>>>(hist, nblocks, digit_total);
+    rs_scatter_kernel<<<nblocks, 256, 0, s>>>(kin, vin, kout, vout, n_dev, shift, hist, nblocks, digit_total);
+}
+
+void launch_tile_ranges(const uint64_t *keys, const uint32_t *n_dev, uint32_t n_max, int ntiles, int32_t *tile_start,
+                        hipStream_t s) {
+    const uint32_t nb = (n_max + 1 + 255) / 256;
+    tile_ranges_kernel<<<nb, 256, 0, s>>>(keys, n_dev, ntiles, tile_start);
+}
+
+}  // namespace msplat

@@ -1,0 +1,80 @@
+// Shared device helpers for the gfx950 kernels: constants of the splatting math,
+// wave64 reductions over DPP, and the tile geometry.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace msplat {
+
+constexpr int TILE = 16;           // binning tile edge (pixels)
+constexpr int TILE_PIX = TILE * TILE;
+constexpr int WAVE = 64;
+
+// Real SH basis constants (values of the real spherical harmonics, as fp32)
+__device__ constexpr float SH_C0 = 0.28209479177387814f;
+__device__ constexpr float SH_C1 = 0.4886025119029199f;
+__device__ constexpr float SH_C2_0 = 1.0925484305920792f, SH_C2_1 = -1.0925484305920792f,
+                           SH_C2_2 = 0.31539156525252005f, SH_C2_3 = -1.0925484305920792f,
+                           SH_C2_4 = 0.5462742152960396f;
+__device__ constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f,
+                           SH_C3_2 = -0.4570457994644658f, SH_C3_3 = 0.3731763325901154f,
+                           SH_C3_4 = -0.4570457994644658f, SH_C3_5 = 1.445305721320277f,
+                           SH_C3_6 = -0.5900435899266435f;
+__device__ constexpr float SH_C4_0 = 2.5033429417967046f, SH_C4_1 = -1.7701307697799304f,
+                           SH_C4_2 = 0.9461746957575601f, SH_C4_3 = -0.6690465435572892f,
+                           SH_C4_4 = 0.10578554691520431f, SH_C4_5 = -0.6690465435572892f,
+                           SH_C4_6 = 0.47308734787878004f, SH_C4_7 = -1.7701307697799304f,
+                           SH_C4_8 = 0.6258357354491761f;
+
+// Compositing thresholds
+__device__ constexpr float SIGMA_MAX = 5.55f;
+__device__ constexpr float ALPHA_MAX = 0.999f;
+__device__ constexpr float ALPHA_MIN = 1.f / 255.f;
+__device__ constexpr float T_MIN = 1e-4f;
+
+__host__ __device__ inline uint32_t sh_bases(uint32_t degree) {
+    return degree == 0 ? 1 : degree == 1 ? 4 : degree == 2 ? 9 : degree == 3 ? 16 : 25;
+}
+
+// exp(x) rounded once from double: agrees with a correctly rounded expf except in
+// double-rounding corner cases, so thresholds computed from it (culling, keys) match
+// a host libm.
+__device__ inline float exp_cr(float x) { return (float)exp((double)x); }
+__device__ inline float log_cr(float x) { return (float)log((double)x); }
+__device__ inline float sin_cr(float x) { return (float)sin((double)x); }
+__device__ inline float cos_cr(float x) { return (float)cos((double)x); }
+
+// Tile rectangle [x0, x1) x [y0, y1) of a splat with pixel center (cx, cy) and
+// half-extents (rx, ry), clamped to the grid.
+__device__ inline void tile_rect(float cx, float cy, float rx, float ry, int tx, int ty, int &x0, int &y0, int &x1,
+                                 int &y1) {
+    float ccx = cx / (float)TILE, ccy = cy / (float)TILE, rrx = rx / (float)TILE, rry = ry / (float)TILE;
+    x0 = min(max(0, (int)(ccx - rrx)), tx);
+    x1 = min(max(0, (int)(ccx + rrx + 1.f)), tx);
+    y0 = min(max(0, (int)(ccy - rry)), ty);
+    y1 = min(max(0, (int)(ccy + rry + 1.f)), ty);
+}
+
+// ── wave64 sum over DPP ────────────────────────────────────────────────────
+// Lanes of a 16-lane row are combined by quad permutes and the row (half-)mirrors,
+// the four rows by row broadcasts; the total ends in lane 63 and is returned as a
+// wave-uniform value.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+
+__device__ inline float wave_sum(float v) {
+    v += dpp_f<0xb1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f<0x4e>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f<0x141>(v);  // row_half_mirror
+    v += dpp_f<0x140>(v);  // row_mirror
+    v += dpp_f<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+
+}  // namespace msplat

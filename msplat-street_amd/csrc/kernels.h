@@ -1,0 +1,102 @@
+// Kernel argument records and host launchers of the gfx950 kernels (internal to the
+// library; the public surface is include/msplat_hip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/msplat_hip.h"
+
+namespace msplat {
+
+constexpr int RS_ITEMS = 4096;  // radix-sort elements per block (256 threads x 16)
+
+struct ProjectArgs {
+    int n;
+    const float *means, *scales, *quats, *dc, *rest, *opac;
+    msplat_hip_camera cam;
+    float4 *splat_a, *splat_b;
+    float *splat_blue, *depths;
+    float2 *aabb;
+    uint2 *rect;
+    int32_t *radii;
+    uint32_t *count, *block_sums;
+};
+
+struct KeygenArgs {
+    int n, tiles_x;
+    const uint32_t *count, *block_bases;
+    const uint2 *rect;
+    const float *depths;
+    uint32_t *offsets;
+    uint64_t *keys;
+    int32_t *vals;
+};
+
+struct RasterArgs {
+    uint32_t width, height;
+    int tiles_x, tiles_y;
+    const int32_t *tile_start, *gid;
+    const float4 *splat_a, *splat_b;
+    const float *splat_blue;
+    float bg[3];
+    // forward outputs / backward inputs
+    float *image, *final_T;
+    int32_t *final_idx;
+    // backward
+    const float *v_image;
+    const uint2 *rect;
+    const uint32_t *offsets;
+    float4 *slot_a, *slot_b;
+    float *slot_c;
+};
+
+struct LossArgs {
+    uint32_t width, height;
+    const float *img, *gt;
+    float ssim_weight, inv_n;
+    float *v_img, *pix_loss;
+    double *row_loss, *loss_sum;
+    float *loss_mean;  // loss_sum / (W H), float
+};
+
+enum BackwardMode { BWD_ADAM = 0, BWD_GRADS = 1 };
+
+struct BackwardArgs {
+    int n;
+    const float *means;
+    float *scales, *quats, *dc, *rest, *opac, *means_rw;  // updated in place in Adam mode
+    msplat_hip_camera cam;
+    const int32_t *radii;
+    const uint32_t *count, *offsets;
+    const float4 *splat_a, *splat_b;
+    const float4 *slot_a, *slot_b;
+    const float *slot_c;
+    // Adam mode
+    float *m1[6], *m2[6];
+    float step[6], bc2[6];
+    float beta1, beta2, eps;
+    float *vis, *gn, *ms;
+    float inv_max_dim;
+    // gradient mode
+    float *g_means, *g_scales, *g_quats, *g_dc, *g_rest, *g_opac;
+    // optional per-gaussian reduced gradients (test hook)
+    float *v_xy, *v_conic, *v_rgb, *v_opac;
+};
+
+void launch_project(const ProjectArgs &a, hipStream_t s);
+void launch_rect_count(int n, const float4 *splat_a, const float2 *aabb, const int32_t *radii, int tx, int ty,
+                       uint2 *rect, uint32_t *count, uint32_t *block_sums, hipStream_t s);
+void launch_scan_blocks(uint32_t *sums, int nblocks, uint64_t *total, uint32_t *total32, hipStream_t s);
+void launch_keygen(const KeygenArgs &a, hipStream_t s);
+void launch_radix_pass(const uint64_t *kin, const int32_t *vin, uint64_t *kout, int32_t *vout, const uint32_t *n_dev,
+                       uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s);
+size_t radix_hist_words(uint32_t n_max);
+void launch_tile_ranges(const uint64_t *keys, const uint32_t *n_dev, uint32_t n_max, int ntiles, int32_t *tile_start,
+                        hipStream_t s);
+void launch_raster_forward(const RasterArgs &a, hipStream_t s);
+void launch_raster_backward(const RasterArgs &a, hipStream_t s);
+void launch_loss(const LossArgs &a, hipStream_t s);
+void launch_backward(const BackwardArgs &a, BackwardMode mode, hipStream_t s);
+
+}  // namespace msplat

@@ -1,0 +1,204 @@
+"""ctypes binding of the HIP backend's C ABI (include/msplat_hip.h).
+
+Loads msplat/libmsplat_hip.so built in-tree; there is no fallback: if the library or a
+GPU is missing, every call raises. DeviceArray wraps one device allocation with numpy
+upload/download.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmsplat_hip.so")
+
+MSPLAT_OK = 0
+FRAME_KEEP_GRADS = 1
+fp = C.POINTER(C.c_float)
+
+
+class MsplatError(RuntimeError):
+    pass
+
+
+class Camera(C.Structure):
+    _fields_ = [("viewmat", C.c_float * 16), ("projmat", C.c_float * 16), ("cam_pos", C.c_float * 3),
+                ("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("tiles_x", C.c_int32), ("tiles_y", C.c_int32),
+                ("clip_thresh", C.c_float), ("glob_scale", C.c_float), ("degree", C.c_uint32),
+                ("degrees_to_use", C.c_uint32)]
+
+
+class Gaussians(C.Structure):
+    _fields_ = [("num_points", C.c_int32), ("means", C.c_void_p), ("scales", C.c_void_p), ("quats", C.c_void_p),
+                ("features_dc", C.c_void_p), ("features_rest", C.c_void_p), ("opacities", C.c_void_p)]
+
+
+class Adam(C.Structure):
+    _fields_ = [("exp_avg", C.c_void_p * 6), ("exp_avg_sq", C.c_void_p * 6), ("step_size", C.c_float * 6),
+                ("bc2_sqrt", C.c_float * 6), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float)]
+
+
+class GradStats(C.Structure):
+    _fields_ = [("vis_counts", C.c_void_p), ("xys_grad_norm", C.c_void_p), ("max_2d_size", C.c_void_p)]
+
+
+class Grads(C.Structure):
+    _fields_ = [("v_means", C.c_void_p), ("v_scales", C.c_void_p), ("v_quats", C.c_void_p),
+                ("v_features_dc", C.c_void_p), ("v_features_rest", C.c_void_p), ("v_opacities", C.c_void_p)]
+
+
+class Frame(C.Structure):
+    _fields_ = [("num_points", C.c_int32), ("num_tiles", C.c_int32), ("num_isects", C.c_int64),
+                ("splat_a", C.c_void_p), ("splat_b", C.c_void_p), ("splat_blue", C.c_void_p),
+                ("depths", C.c_void_p), ("aabb", C.c_void_p), ("radii", C.c_void_p), ("tile_count", C.c_void_p),
+                ("offsets", C.c_void_p), ("isect_keys", C.c_void_p), ("isect_gid", C.c_void_p),
+                ("tile_start", C.c_void_p), ("image", C.c_void_p), ("final_T", C.c_void_p),
+                ("v_image", C.c_void_p), ("final_idx", C.c_void_p), ("v_xy", C.c_void_p),
+                ("v_conic", C.c_void_p), ("v_rgb", C.c_void_p), ("v_opacity", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MsplatError(f"{LIB_PATH} is not built: run python msplat-street_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, u32, f32 = C.c_void_p, C.c_int32, C.c_uint32, C.c_float
+        sig = {
+            "msplat_hip_last_error": ([], C.c_char_p),
+            "msplat_hip_ctx_create": ([C.c_int, vp, C.POINTER(vp)], C.c_int),
+            "msplat_hip_ctx_destroy": ([vp], C.c_int),
+            "msplat_hip_ctx_set_stream": ([vp, vp], C.c_int),
+            "msplat_hip_ctx_stream": ([vp], vp),
+            "msplat_hip_sync": ([vp], C.c_int),
+            "msplat_hip_commit": ([vp], C.c_int),
+            "msplat_hip_malloc": ([C.c_size_t, C.c_int, C.POINTER(vp)], C.c_int),
+            "msplat_hip_free": ([vp], C.c_int),
+            "msplat_hip_memcpy": ([vp, vp, vp, C.c_size_t, C.c_int], C.c_int),
+            "msplat_hip_memset": ([vp, vp, C.c_int, C.c_size_t], C.c_int),
+            "msplat_hip_render": ([vp, C.POINTER(Gaussians), C.POINTER(Camera), fp, C.POINTER(vp)], C.c_int),
+            "msplat_hip_train_step": ([vp, C.POINTER(Gaussians), C.POINTER(Camera), fp, vp, f32, f32,
+                                       C.POINTER(Adam), C.POINTER(GradStats), f32, vp, fp], C.c_int),
+            "msplat_hip_last_loss": ([vp, fp], C.c_int),
+            "msplat_hip_render_backward": ([vp, C.POINTER(Gaussians), C.POINTER(Camera), fp, vp,
+                                            C.POINTER(Grads)], C.c_int),
+            "msplat_hip_frame_view": ([vp, C.POINTER(Frame)], C.c_int),
+            "msplat_hip_set_flags": ([vp, C.c_int], C.c_int),
+            "msplat_hip_stage_project": ([vp, C.POINTER(Gaussians), C.POINTER(Camera)], C.c_int),
+            "msplat_hip_stage_bin": ([vp, C.POINTER(Camera)], C.c_int),
+            "msplat_hip_stage_raster_forward": ([vp, C.POINTER(Camera), fp], C.c_int),
+            "msplat_hip_stage_loss": ([vp, C.POINTER(Camera), vp, f32, f32, fp], C.c_int),
+            "msplat_hip_radix_sort": ([vp, vp, vp, u32, C.c_int], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes, f.restype = args, res
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != MSPLAT_OK:
+        raise MsplatError(f"msplat HIP error {rc}: {lib().msplat_hip_last_error().decode()}")
+
+
+class Context:
+    """A HIP context: one stream plus the per-frame cache."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        self.ptr = C.c_void_p()
+        check(lib().msplat_hip_ctx_create(device, C.c_void_p(stream) if stream else None, C.byref(self.ptr)))
+
+    def close(self):
+        if self.ptr:
+            lib().msplat_hip_ctx_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(lib().msplat_hip_sync(self.ptr))
+
+    def frame(self) -> Frame:
+        f = Frame()
+        check(lib().msplat_hip_frame_view(self.ptr, C.byref(f)))
+        return f
+
+    def set_flags(self, flags: int):
+        check(lib().msplat_hip_set_flags(self.ptr, flags))
+
+    def download(self, ptr, shape, dtype) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        if out.nbytes:
+            check(lib().msplat_hip_memcpy(self.ptr, out.ctypes.data, C.c_void_p(ptr), out.nbytes, 1))
+        return out
+
+    def upload(self, ptr, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes:
+            check(lib().msplat_hip_memcpy(self.ptr, C.c_void_p(ptr), arr.ctypes.data, arr.nbytes, 0))
+
+
+class DeviceArray:
+    """One device allocation holding a numpy-shaped array."""
+
+    def __init__(self, ctx: Context, shape, dtype=np.float32, zero=True):
+        self.ctx, self.shape, self.dtype = ctx, tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().msplat_hip_malloc(max(self.nbytes, 4), 1 if zero else 0, C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, ctx, arr, dtype=None):
+        arr = np.ascontiguousarray(arr, dtype=dtype or np.asarray(arr).dtype)
+        d = cls(ctx, arr.shape, arr.dtype, zero=False)
+        d.upload(arr)
+        return d
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype).reshape(self.shape)
+        self.ctx.upload(self.ptr, arr)
+
+    def numpy(self) -> np.ndarray:
+        return self.ctx.download(self.ptr, self.shape, self.dtype)
+
+    def free(self):
+        if self.ptr:
+            lib().msplat_hip_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def make_camera_struct(cam: dict, degree: int, glob_scale: float = 1.0, clip: float = 0.01) -> Camera:
+    c = Camera()
+    c.viewmat[:] = [float(v) for v in np.asarray(cam["viewmat"], np.float32).ravel()]
+    c.projmat[:] = [float(v) for v in np.asarray(cam["projmat"], np.float32).ravel()]
+    c.cam_pos[:] = [float(v) for v in np.asarray(cam["cam_pos"], np.float32).ravel()]
+    c.fx, c.fy, c.cx, c.cy = (float(cam[k]) for k in ("fx", "fy", "cx", "cy"))
+    c.width, c.height = int(cam["W"]), int(cam["H"])
+    c.tiles_x, c.tiles_y = int(cam["tiles_x"]), int(cam["tiles_y"])
+    c.clip_thresh, c.glob_scale = clip, glob_scale
+    c.degree, c.degrees_to_use = int(degree), int(cam["dtu"])
+    return c
+
+
+def floats(v) -> "C.Array":
+    a = np.ascontiguousarray(v, np.float32)
+    return (C.c_float * a.size)(*a.tolist())

@@ -1,0 +1,118 @@
+"""Device-resident Gaussian state and the per-view backend calls over the C ABI.
+
+DeviceGaussians keeps the six parameter groups, their Adam moments and the
+densification statistics in capacity-sized device buffers (the reference's *_buf
+tensors with views of the active count, model.cpp:81-156).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import Adam, Context, DeviceArray, GradStats, Gaussians, check, floats, lib, make_camera_struct
+
+KEYS = ("means", "scales", "quats", "dc", "rest", "opac")
+WIDTH = {"means": 3, "scales": 3, "quats": 4, "dc": 3, "opac": 1}
+BETA1, BETA2, EPS = 0.9, 0.999, 1e-8
+
+
+def sh_bases(degree: int) -> int:
+    return {0: 1, 1: 4, 2: 9, 3: 16}.get(degree, 25)
+
+
+class DeviceGaussians:
+    def __init__(self, ctx: Context, scene: dict, capacity: int | None = None):
+        self.ctx = ctx
+        self.degree = int(scene["degree"])
+        self.n = int(scene["means"].shape[0])
+        self.capacity = max(int(capacity or self.n), self.n, 1)
+        self.rest_bases = sh_bases(self.degree) - 1
+        self.widths = dict(WIDTH, rest=max(self.rest_bases, 1) * 3)
+        self.params, self.m1, self.m2 = {}, {}, {}
+        for k in KEYS:
+            w = self.widths[k]
+            host = np.zeros((self.capacity, w), np.float32)
+            host[:self.n] = np.asarray(scene[k], np.float32).reshape(self.n, -1)[:, :w] if self.n else host[:0]
+            self.params[k] = DeviceArray.from_numpy(ctx, host)
+            self.m1[k] = DeviceArray(ctx, (self.capacity, w))
+            self.m2[k] = DeviceArray(ctx, (self.capacity, w))
+        self.vis = DeviceArray(ctx, (self.capacity,))
+        self.gn = DeviceArray(ctx, (self.capacity,))
+        self.ms = DeviceArray(ctx, (self.capacity,))
+        self.radii = DeviceArray(ctx, (self.capacity,), np.int32)
+        self.step_count = 0
+
+    def struct(self) -> Gaussians:
+        g = Gaussians()
+        g.num_points = self.n
+        g.means, g.scales, g.quats = self.params["means"].ptr, self.params["scales"].ptr, self.params["quats"].ptr
+        g.features_dc, g.features_rest = self.params["dc"].ptr, self.params["rest"].ptr
+        g.opacities = self.params["opac"].ptr
+        return g
+
+    def adam(self, step_sizes, bc2s, beta1=BETA1, beta2=BETA2, eps=EPS) -> Adam:
+        a = Adam()
+        for i, k in enumerate(KEYS):
+            a.exp_avg[i] = self.m1[k].ptr
+            a.exp_avg_sq[i] = self.m2[k].ptr
+            a.step_size[i] = float(step_sizes[i])
+            a.bc2_sqrt[i] = float(bc2s[i])
+        a.beta1, a.beta2, a.eps = beta1, beta2, eps
+        return a
+
+    def stats(self) -> GradStats:
+        s = GradStats()
+        s.vis_counts, s.xys_grad_norm, s.max_2d_size = self.vis.ptr, self.gn.ptr, self.ms.ptr
+        return s
+
+    def reset_stats(self):
+        for d in (self.vis, self.gn, self.ms):
+            check(lib().msplat_hip_memset(self.ctx.ptr, C.c_void_p(d.ptr), 0, d.nbytes))
+
+    def to_numpy(self) -> dict:
+        out = {"degree": self.degree}
+        for k in KEYS:
+            p = self.params[k].numpy()[:self.n]
+            out[k] = p if k != "rest" else p.reshape(self.n, max(self.rest_bases, 1), 3)
+            out["m1_" + k] = self.m1[k].numpy()[:self.n]
+            out["m2_" + k] = self.m2[k].numpy()[:self.n]
+        for k in ("vis", "gn", "ms"):
+            out[k] = getattr(self, k).numpy()[:self.n]
+        return out
+
+
+def render(ctx: Context, dg: DeviceGaussians, cam: dict, bg) -> int:
+    """Enqueues a render; returns the device pointer of the (H, W, 3) image."""
+    g, c = dg.struct(), make_camera_struct(cam, dg.degree)
+    out = C.c_void_p()
+    check(lib().msplat_hip_render(ctx.ptr, C.byref(g), C.byref(c), floats(bg), C.byref(out)))
+    return out.value
+
+
+def render_numpy(ctx: Context, dg: DeviceGaussians, cam: dict, bg) -> np.ndarray:
+    p = render(ctx, dg, cam, bg)
+    return ctx.download(p, (cam["H"], cam["W"], 3), np.float32)
+
+
+def adam_coeffs(lrs, step_count, beta1=BETA1, beta2=BETA2):
+    """(step sizes lr/bc1, bc2_sqrt) for Adam step t (model.cpp:638-651), float32."""
+    bc1 = np.float32(1) - np.float32(beta1) ** np.float32(step_count)
+    bc2 = np.float32(1) - np.float32(beta2) ** np.float32(step_count)
+    steps = (np.asarray(lrs, np.float32) / bc1).astype(np.float32)
+    return steps, np.full(6, np.sqrt(bc2), np.float32)
+
+
+def train_step(ctx: Context, dg: DeviceGaussians, cam: dict, bg, gt: DeviceArray, step_sizes, bc2s,
+               ssim_weight=0.2, read_loss=True) -> float | None:
+    g, c = dg.struct(), make_camera_struct(cam, dg.degree)
+    a, s = dg.adam(step_sizes, bc2s), dg.stats()
+    H, W = cam["H"], cam["W"]
+    inv_n = np.float32(1.0) / np.float32(H * W * 3)
+    inv_max = np.float32(1.0) / np.float32(max(H, W))
+    loss = C.c_float(0.0)
+    check(lib().msplat_hip_train_step(ctx.ptr, C.byref(g), C.byref(c), floats(bg), C.c_void_p(gt.ptr), ssim_weight,
+                                      float(inv_n), C.byref(a), C.byref(s), float(inv_max),
+                                      C.c_void_p(dg.radii.ptr), C.byref(loss) if read_loss else None))
+    return loss.value if read_loss else None

@@ -1,0 +1,359 @@
+"""ctypes wrappers over the oracle (liboracle.so) and the compiled reference
+(_ref/libmsplat_ref*.so). TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check
+and bench.py's cpu_baseline leg, never by the product.
+
+The oracle restates core/cpu/msplat_cpu.cpp (see msplat_oracle.cpp for the per-function
+citations); `Reference` calls the reference's own CPU backend through ref_driver.cpp.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+vp = C.c_void_p
+
+
+def _load(name):
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (make -C oracle)")
+    return C.CDLL(path)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _ptrs(arrays):
+    return (C.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+
+
+def sh_bases(degree: int) -> int:
+    return {0: 1, 1: 4, 2: 9, 3: 16}.get(degree, 25)
+
+
+class Oracle:
+    """The CPU restatement. Methods take/return numpy arrays in the reference layouts."""
+
+    def __init__(self):
+        L = self.lib = _load("liboracle.so")
+        for s, fp in (("f32", f32p), ("f64", f64p)):
+            T = C.c_float if s == "f32" else C.c_double
+            f = getattr(L, f"orc_{s}_project")
+            f.argtypes = [C.c_int, fp, fp, T, fp, fp, fp, T, T, T, T, C.c_uint32, C.c_uint32, C.c_int, C.c_int, T,
+                          C.c_uint32, C.c_uint32, fp, fp, fp, fp, fp, fp, fp, fp, i32p]
+            f = getattr(L, f"orc_{s}_count_isects")
+            f.argtypes = [C.c_int, fp, fp, i32p, C.c_int, C.c_int, vp]
+            f.restype = C.c_int64
+            f = getattr(L, f"orc_{s}_bin_sort")
+            f.argtypes = [C.c_int, fp, fp, fp, i32p, fp, fp, fp, C.c_int, C.c_int, i32p, i32p, i32p, fp, fp, fp, vp]
+            f = getattr(L, f"orc_{s}_raster_forward")
+            f.argtypes = [C.c_uint32, C.c_uint32, C.c_int, C.c_int, i32p, i32p, fp, fp, fp, fp, fp, fp, i32p]
+            f = getattr(L, f"orc_{s}_raster_backward")
+            f.argtypes = [C.c_uint32, C.c_uint32, C.c_int, C.c_int, i32p, i32p, fp, fp, fp, fp, fp, i32p, fp, fp]
+            f = getattr(L, f"orc_{s}_reduce_slots")
+            f.argtypes = [C.c_int, C.c_int64, i32p, fp, fp, fp, fp, fp, fp]
+        L.orc_f32_loss.argtypes = [C.c_uint32, C.c_uint32, f32p, f32p, C.c_float, C.c_float, f32p]
+        L.orc_f32_loss.restype = C.c_double
+        L.orc_f32_project_backward_adam.argtypes = (
+            [C.c_int, f32p, f32p, C.c_float, f32p, f32p, f32p, C.c_float, C.c_float, C.c_uint32, C.c_uint32,
+             C.c_uint32, C.c_uint32, f32p, i32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p,
+             f32p, f32p, f32p] + [C.c_float] * 7)
+        L.orc_f32_adam.argtypes = [f32p, f32p, f32p, f32p, C.c_size_t] + [C.c_float] * 5
+        L.orc_f32_grad_stats.argtypes = [C.c_int, i32p, f32p, C.c_float, f32p, f32p, f32p]
+        L.orc_f32_train_step.argtypes = (
+            [C.c_int, C.c_float, f32p, f32p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint32,
+             C.c_int, C.c_int, C.c_float, C.c_uint32, C.c_uint32, f32p, f32p, f32p, C.c_float, C.c_float, vp, vp, vp,
+             f32p, f32p, C.c_float, C.c_float, C.c_float, f32p, f32p, f32p, C.c_float, vp, vp])
+        L.orc_f32_train_step.restype = C.c_float
+        L.orc_densify.argtypes = ([C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_float, C.c_float,
+                                   C.c_float, C.c_int, f32p, f32p, f32p, C.c_float, vp, C.c_int, C.c_uint32, vp, vp,
+                                   vp, vp])
+        L.orc_opacity_reset.argtypes = [f32p, f32p, f32p, C.c_int, C.c_float]
+        L.orc_radix_sort.argtypes = [u64p, i32p, C.c_uint32, C.c_int]
+        L.orc_knn3_mean_dist.argtypes = [f32p, C.c_uint32, f32p]
+        L.orc_init_gaussians.argtypes = [f32p, u8p, C.c_uint32, C.c_uint32, C.c_float, f32p, f32p, f32p, f32p]
+        L.orc_image_metrics.argtypes = [f32p, f32p, C.c_int, C.c_int, f64p]
+        L.orc_pack_rgba8.argtypes = [f32p, C.c_uint32, u8p]
+        L.orc_pcg_hash.argtypes = [C.c_uint32]
+        L.orc_pcg_hash.restype = C.c_uint32
+        L.orc_rand_uniform.argtypes = [C.c_uint32, C.c_uint32]
+        L.orc_rand_uniform.restype = C.c_float
+        L.orc_rand_normal3.argtypes = [C.c_uint32, C.c_uint32, f32p]
+
+    # ── render path (float32 or float64) ────────────────────────────────────
+    def project(self, scene, cam, dtype=np.float32, glob_scale=1.0, clip=0.01):
+        s = "f32" if dtype == np.float32 else "f64"
+        N = scene["means"].shape[0]
+        out = dict(xys=np.zeros((N, 2), dtype), depths=np.zeros(N, dtype), conics=np.zeros((N, 3), dtype),
+                   colors=np.zeros((N, 3), dtype), aabb=np.zeros((N, 2), dtype), radii=np.zeros(N, np.int32))
+        getattr(self.lib, f"orc_{s}_project")(
+            N, _c(scene["means"], dtype), _c(scene["scales"], dtype), glob_scale, _c(scene["quats"], dtype),
+            _c(cam["viewmat"], dtype), _c(cam["projmat"], dtype), cam["fx"], cam["fy"], cam["cx"], cam["cy"],
+            cam["W"], cam["H"], cam["tiles_x"], cam["tiles_y"], clip, scene["degree"], cam["dtu"],
+            _c(cam["cam_pos"], dtype), _c(scene["dc"], dtype), _c(scene["rest"], dtype), out["xys"], out["depths"],
+            out["conics"], out["colors"], out["aabb"], out["radii"])
+        return out
+
+    def bin_sort(self, proj, opac, cam, dtype=np.float32):
+        s = "f32" if dtype == np.float32 else "f64"
+        N = proj["radii"].shape[0]
+        tx, ty = cam["tiles_x"], cam["tiles_y"]
+        args = (_c(proj["xys"], dtype), _c(proj["depths"], dtype), _c(proj["aabb"], dtype), _c(proj["radii"], np.int32))
+        M = getattr(self.lib, f"orc_{s}_count_isects")(N, args[0], args[2], args[3], tx, ty, None)
+        Mc = max(M, 1)
+        out = dict(tile_start=np.zeros(tx * ty, np.int32), tile_end=np.zeros(tx * ty, np.int32),
+                   isect_gid=np.zeros(Mc, np.int32), pk_xyo=np.zeros((Mc, 3), dtype), pk_conic=np.zeros((Mc, 3), dtype),
+                   pk_rgb=np.zeros((Mc, 3), dtype), keys=np.zeros(Mc, np.uint64), M=M)
+        getattr(self.lib, f"orc_{s}_bin_sort")(
+            N, *args[:2], args[2], args[3], _c(np.ravel(opac), dtype), _c(proj["conics"], dtype),
+            _c(proj["colors"], dtype), tx, ty, out["tile_start"], out["tile_end"], out["isect_gid"], out["pk_xyo"],
+            out["pk_conic"], out["pk_rgb"], out["keys"].ctypes.data)
+        for k in ("isect_gid", "pk_xyo", "pk_conic", "pk_rgb", "keys"):
+            out[k] = out[k][:M]
+        return out
+
+    def raster_forward(self, bins, cam, bg, dtype=np.float32):
+        s = "f32" if dtype == np.float32 else "f64"
+        W, H = cam["W"], cam["H"]
+        img = np.zeros((H, W, 3), dtype)
+        fT = np.zeros((H, W), dtype)
+        fidx = np.zeros((H, W), np.int32)
+        Mc = max(bins["M"], 1)
+        getattr(self.lib, f"orc_{s}_raster_forward")(
+            W, H, cam["tiles_x"], cam["tiles_y"], bins["tile_start"], bins["tile_end"],
+            _pad(bins["pk_xyo"], Mc, dtype), _pad(bins["pk_conic"], Mc, dtype), _pad(bins["pk_rgb"], Mc, dtype),
+            _c(bg, dtype), img, fT, fidx)
+        return dict(img=img, final_T=fT, final_idx=fidx)
+
+    def render(self, scene, cam, bg, dtype=np.float32):
+        p = self.project(scene, cam, dtype)
+        b = self.bin_sort(p, scene["opac"], cam, dtype)
+        r = self.raster_forward(b, cam, bg, dtype)
+        return p, b, r
+
+    def loss(self, img, gt, ssim_weight=0.2):
+        H, W = img.shape[:2]
+        v = np.zeros((H, W, 3), np.float32)
+        tot = self.lib.orc_f32_loss(W, H, _c(img, np.float32), _c(gt, np.float32), ssim_weight,
+                                    np.float32(1.0) / np.float32(H * W * 3), v)
+        return tot, v
+
+    def raster_backward(self, bins, fwd, cam, bg, v_out, dtype=np.float32):
+        s = "f32" if dtype == np.float32 else "f64"
+        Mc = max(bins["M"], 1)
+        slots = np.zeros((Mc, 10), dtype)
+        getattr(self.lib, f"orc_{s}_raster_backward")(
+            cam["W"], cam["H"], cam["tiles_x"], cam["tiles_y"], bins["tile_start"], bins["tile_end"],
+            _pad(bins["pk_xyo"], Mc, dtype), _pad(bins["pk_conic"], Mc, dtype), _pad(bins["pk_rgb"], Mc, dtype),
+            _c(bg, dtype), _c(fwd["final_T"], dtype), _c(fwd["final_idx"], np.int32), _c(v_out, dtype), slots)
+        return slots[:bins["M"]]
+
+    def reduce_slots(self, N, bins, slots, dtype=np.float32):
+        s = "f32" if dtype == np.float32 else "f64"
+        Mc = max(bins["M"], 1)
+        out = dict(v_xy=np.zeros((N, 2), dtype), v_conic=np.zeros((N, 3), dtype), v_rgb=np.zeros((N, 3), dtype),
+                   v_opac=np.zeros(N, dtype), v_depth=np.zeros(N, dtype))
+        getattr(self.lib, f"orc_{s}_reduce_slots")(
+            N, bins["M"], _pad(bins["isect_gid"], Mc, np.int32), _pad(slots, Mc, dtype), out["v_xy"], out["v_conic"],
+            out["v_rgb"], out["v_opac"], out["v_depth"])
+        return out
+
+    def train_step(self, state, cam, bg, gt, steps, bc2s, ssim_weight=0.2, b1=0.9, b2=0.999, eps=1e-8):
+        """One training step in place on `state` (params p0..p5, m1_*, m2_*, vis/gn/ms).
+        Returns (loss, radii, image)."""
+        N = state["means"].shape[0]
+        names = ("means", "scales", "quats", "dc", "rest", "opac")
+        for n in names:
+            for pre in ("", "m1_", "m2_"):
+                state[pre + n] = _c(state[pre + n], np.float32)
+        radii = np.zeros(N, np.int32)
+        img = np.zeros((cam["H"], cam["W"], 3), np.float32)
+        H, W = cam["H"], cam["W"]
+        loss = self.lib.orc_f32_train_step(
+            N, 1.0, _c(cam["viewmat"], np.float32), _c(cam["projmat"], np.float32), cam["fx"], cam["fy"], cam["cx"],
+            cam["cy"], H, W, cam["tiles_x"], cam["tiles_y"], 0.01, state["degree"], cam["dtu"],
+            _c(cam["cam_pos"], np.float32), _c(bg, np.float32), _c(gt, np.float32), ssim_weight,
+            np.float32(1.0) / np.float32(H * W * 3), _ptrs([state[n] for n in names]),
+            _ptrs([state["m1_" + n] for n in names]), _ptrs([state["m2_" + n] for n in names]),
+            _c(steps, np.float32), _c(bc2s, np.float32), b1, b2, eps, state["vis"], state["gn"], state["ms"],
+            np.float32(1.0) / np.float32(max(H, W)), radii.ctypes.data, img.ctypes.data)
+        return loss, radii, img
+
+    def densify(self, bufs, N, cap, gn, vc, ms, half_max_dim, fr_stride, seed, grad_thr=2e-4, size_thr=0.01,
+                screen_thr=0.05, check_screen=1, cull_alpha=0.1, cull_scale=0.5, cull_screen=0.15, check_huge=0):
+        """bufs: list of 18 float32 arrays with `cap` rows, modified in place. Returns
+        (new_count, split, dup, keep)."""
+        split = np.zeros(max(N, 1), np.uint8)
+        dup = np.zeros(max(N, 1), np.uint8)
+        keep = np.zeros(max(3 * N, 1), np.uint8)
+        nn = C.c_int(0)
+        out = self.lib.orc_densify(N, cap, grad_thr, size_thr, screen_thr, check_screen, cull_alpha, cull_scale,
+                                   cull_screen, check_huge, _c(gn, np.float32), _c(vc, np.float32),
+                                   _c(ms, np.float32), half_max_dim, _ptrs(bufs), fr_stride, seed,
+                                   split.ctypes.data, dup.ctypes.data, keep.ctypes.data, C.byref(nn))
+        return out, split[:N], dup[:N], keep[:nn.value]
+
+    def opacity_reset(self, o, m1, m2, reset_logit):
+        self.lib.orc_opacity_reset(o, m1, m2, o.size, reset_logit)
+
+    def radix_sort(self, keys, vals, key_bits):
+        k = _c(keys, np.uint64).copy()
+        v = _c(vals, np.int32).copy()
+        self.lib.orc_radix_sort(k, v, k.size, key_bits)
+        return k, v
+
+    def knn3(self, pts):
+        pts = _c(pts, np.float32)
+        out = np.zeros(pts.shape[0], np.float32)
+        self.lib.orc_knn3_mean_dist(pts, pts.shape[0], out)
+        return out
+
+    def init_gaussians(self, means, rgb, seed=42, logit=None):
+        n = means.shape[0]
+        logit = np.float32(np.log(np.float32(0.1) / np.float32(0.9))) if logit is None else logit
+        s, q = np.zeros((n, 3), np.float32), np.zeros((n, 4), np.float32)
+        d, o = np.zeros((n, 3), np.float32), np.zeros((n, 1), np.float32)
+        self.lib.orc_init_gaussians(_c(means, np.float32), _c(rgb, np.uint8), n, seed, logit, s, q, d, o)
+        return s, q, d, o
+
+    def image_metrics(self, a, b):
+        out = np.zeros(3, np.float64)
+        self.lib.orc_image_metrics(_c(a, np.float32), _c(b, np.float32), a.shape[0], a.shape[1], out)
+        return out
+
+    def pack_rgba8(self, img):
+        n = img.shape[0] * img.shape[1]
+        out = np.zeros((img.shape[0], img.shape[1], 4), np.uint8)
+        self.lib.orc_pack_rgba8(_c(img, np.float32), n, out)
+        return out
+
+    def pcg_hash(self, v):
+        return self.lib.orc_pcg_hash(v)
+
+    def rand_uniform(self, seed, ctr):
+        return self.lib.orc_rand_uniform(seed, ctr)
+
+    def rand_normal3(self, seed, stream):
+        out = np.zeros(3, np.float32)
+        self.lib.orc_rand_normal3(seed, stream, out)
+        return out
+
+
+def _pad(a, n, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    if a.shape[0] >= n:
+        return a
+    pad = np.zeros((n - a.shape[0],) + a.shape[1:], dtype)
+    return np.concatenate([a, pad])
+
+
+class Reference:
+    """The reference's compiled CPU backend (core/cpu/msplat_cpu.cpp), built from the
+    mounted reference tree into oracle/_ref by oracle/Makefile."""
+
+    def __init__(self, perf: bool = False):
+        L = self.lib = _load(os.path.join("_ref", "libmsplat_ref_perf.so" if perf else "libmsplat_ref.so"))
+        L.ref_render.argtypes = ([C.c_int, f32p, f32p, f32p, f32p, f32p, C.c_int, f32p, f32p, f32p, C.c_float,
+                                  C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
+                                  f32p, f32p, f32p, vp])
+        L.ref_train_step.argtypes = ([C.c_int, vp, C.c_int, vp, vp, f32p, f32p, C.c_float, C.c_float, C.c_float,
+                                      C.c_float, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, f32p, f32p, f32p,
+                                      C.c_float, f32p, f32p, C.c_float, C.c_float, C.c_float, f32p, f32p, f32p, i32p,
+                                      vp])
+        L.ref_densify.argtypes = ([C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_float, C.c_float,
+                                   C.c_float, C.c_int, f32p, f32p, f32p, C.c_float, vp, C.c_int, C.c_uint32])
+        L.ref_opacity_reset.argtypes = [f32p, f32p, f32p, C.c_int, C.c_float]
+        L.ref_radix_sort.argtypes = [u64p, i32p, C.c_uint32, C.c_int]
+        L.ref_knn3.argtypes = [f32p, C.c_uint32, f32p]
+        L.ref_init_gaussians.argtypes = [f32p, u8p, C.c_uint32, C.c_uint32, C.c_float, f32p, f32p, f32p, f32p]
+        L.ref_image_metrics.argtypes = [f32p, f32p, C.c_int, C.c_int, f64p]
+        L.ref_pack_rgba8.argtypes = [f32p, C.c_uint32, u8p]
+
+    def render(self, scene, cam, bg):
+        N = scene["means"].shape[0]
+        img = np.zeros((cam["H"], cam["W"], 3), np.float32)
+        fT = np.zeros((cam["H"], cam["W"]), np.float32)
+        rb = sh_bases(scene["degree"]) - 1
+        rc = self.lib.ref_render(
+            N, _c(scene["means"], np.float32), _c(scene["scales"], np.float32), _c(scene["quats"], np.float32),
+            _c(scene["dc"], np.float32), _c(scene["rest"], np.float32), rb, _c(scene["opac"], np.float32),
+            _c(cam["viewmat"], np.float32), _c(cam["projmat"], np.float32), cam["fx"], cam["fy"], cam["cx"],
+            cam["cy"], cam["H"], cam["W"], 0.01, scene["degree"], cam["dtu"], _c(cam["cam_pos"], np.float32),
+            _c(bg, np.float32), img, fT.ctypes.data)
+        if rc:
+            raise RuntimeError("reference render failed")
+        return img, fT
+
+    def train_step(self, state, cam, bg, gt, steps, bc2s, ssim_weight=0.2, b1=0.9, b2=0.999, eps=1e-8):
+        N = state["means"].shape[0]
+        names = ("means", "scales", "quats", "dc", "rest", "opac")
+        for n in names:
+            for pre in ("", "m1_", "m2_"):
+                state[pre + n] = _c(state[pre + n], np.float32)
+        radii = np.zeros(N, np.int32)
+        loss = C.c_float(0)
+        rc = self.lib.ref_train_step(
+            N, _ptrs([state[n] for n in names]), sh_bases(state["degree"]) - 1,
+            _ptrs([state["m1_" + n] for n in names]), _ptrs([state["m2_" + n] for n in names]),
+            _c(cam["viewmat"], np.float32), _c(cam["projmat"], np.float32), cam["fx"], cam["fy"], cam["cx"],
+            cam["cy"], cam["H"], cam["W"], 0.01, state["degree"], cam["dtu"], _c(cam["cam_pos"], np.float32),
+            _c(bg, np.float32), _c(gt, np.float32), ssim_weight, _c(steps, np.float32), _c(bc2s, np.float32), b1, b2,
+            eps, state["vis"], state["gn"], state["ms"], radii, C.byref(loss))
+        if rc:
+            raise RuntimeError("reference train_step failed")
+        return loss.value, radii
+
+    def densify(self, bufs, N, cap, gn, vc, ms, half_max_dim, fr_stride, seed, grad_thr=2e-4, size_thr=0.01,
+                screen_thr=0.05, check_screen=1, cull_alpha=0.1, cull_scale=0.5, cull_screen=0.15, check_huge=0):
+        return self.lib.ref_densify(N, cap, grad_thr, size_thr, screen_thr, check_screen, cull_alpha, cull_scale,
+                                    cull_screen, check_huge, _c(gn, np.float32), _c(vc, np.float32),
+                                    _c(ms, np.float32), half_max_dim, _ptrs(bufs), fr_stride, seed)
+
+    def opacity_reset(self, o, m1, m2, reset_logit):
+        self.lib.ref_opacity_reset(o, m1, m2, o.size, reset_logit)
+
+    def radix_sort(self, keys, vals, key_bits):
+        k = _c(keys, np.uint64).copy()
+        v = _c(vals, np.int32).copy()
+        self.lib.ref_radix_sort(k, v, k.size, key_bits)
+        return k, v
+
+    def knn3(self, pts):
+        pts = _c(pts, np.float32)
+        out = np.zeros(pts.shape[0], np.float32)
+        self.lib.ref_knn3(pts, pts.shape[0], out)
+        return out
+
+    def init_gaussians(self, means, rgb, seed=42, logit=None):
+        n = means.shape[0]
+        logit = np.float32(np.log(np.float32(0.1) / np.float32(0.9))) if logit is None else logit
+        s, q = np.zeros((n, 3), np.float32), np.zeros((n, 4), np.float32)
+        d, o = np.zeros((n, 3), np.float32), np.zeros((n, 1), np.float32)
+        self.lib.ref_init_gaussians(_c(means, np.float32), _c(rgb, np.uint8), n, seed, logit, s, q, d, o)
+        return s, q, d, o
+
+    def image_metrics(self, a, b):
+        out = np.zeros(3, np.float64)
+        self.lib.ref_image_metrics(_c(a, np.float32), _c(b, np.float32), a.shape[0], a.shape[1], out)
+        return out
+
+    def pack_rgba8(self, img):
+        n = img.shape[0] * img.shape[1]
+        out = np.zeros((img.shape[0], img.shape[1], 4), np.uint8)
+        self.lib.ref_pack_rgba8(_c(img, np.float32), n, out)
+        return out
+
+
+def reference_available(perf: bool = False) -> bool:
+    name = "libmsplat_ref_perf.so" if perf else "libmsplat_ref.so"
+    return os.path.exists(os.path.join(_HERE, "_ref", name))

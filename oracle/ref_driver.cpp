@@ -1,0 +1,186 @@
+// Plain-C entry points over the reference CPU backend (core/metal/bindings.h as
+// implemented by core/cpu/msplat_cpu.cpp), so that Python can call the compiled
+// reference with numpy buffers. TEST INFRASTRUCTURE ONLY: built by oracle/Makefile
+// into oracle/_ref/ where /root/reference exists, used to write the golden vectors and
+// as the CPU baseline; never part of the product.
+#include "bindings.h"
+
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <vector>
+
+namespace {
+
+MTensor upload(const void *src, std::vector<int64_t> shape, DType dt) {
+    MTensor t = gpu_empty(std::move(shape), dt);
+    if (src) std::memcpy(t.data_ptr(), src, t.nbytes());
+    return t;
+}
+
+void download(void *dst, const MTensor &t) {
+    if (dst) std::memcpy(dst, t.data_ptr(), t.nbytes());
+}
+
+int fail(const std::exception &e) {
+    std::fprintf(stderr, "msplat reference: %s\n", e.what());
+    return -1;
+}
+
+struct Scene {
+    MTensor means, scales, quats, dc, rest, opac, vm, pm, bg;
+    Scene(int N, const float *m, const float *s, const float *q, const float *d, const float *r, int rest_bases,
+          const float *o, const float *viewmat, const float *projmat, const float *background) {
+        means = upload(m, {N, 3}, DType::Float32);
+        scales = upload(s, {N, 3}, DType::Float32);
+        quats = upload(q, {N, 4}, DType::Float32);
+        dc = upload(d, {N, 3}, DType::Float32);
+        rest = upload(r, {N, rest_bases > 0 ? rest_bases : 1, 3}, DType::Float32);
+        opac = upload(o, {N, 1}, DType::Float32);
+        vm = upload(viewmat, {4, 4}, DType::Float32);
+        pm = upload(projmat, {4, 4}, DType::Float32);
+        bg = upload(background, {3}, DType::Float32);
+    }
+};
+
+std::tuple<int, int, int> tiles(int W, int H) { return std::make_tuple((W + 15) / 16, (H + 15) / 16, 1); }
+
+}  // namespace
+
+extern "C" int ref_render(int N, const float *means, const float *scales, const float *quats, const float *dc,
+                          const float *rest, int rest_bases, const float *opac, const float *viewmat,
+                          const float *projmat, float fx, float fy, float cx, float cy, int H, int W, float clip,
+                          int degree, int dtu, const float *cam_pos, const float *bg, float *out_img,
+                          float *final_T) {
+    try {
+        Scene s(N, means, scales, quats, dc, rest, rest_bases, opac, viewmat, projmat, bg);
+        float cam[3] = {cam_pos[0], cam_pos[1], cam_pos[2]};
+        MTensor img = msplat_render(N, s.means, s.scales, 1.0f, s.quats, s.vm, s.pm, fx, fy, cx, cy, (unsigned)H,
+                                    (unsigned)W, tiles(W, H), clip, (unsigned)degree, (unsigned)dtu, cam, s.dc, s.rest,
+                                    s.opac, s.bg, nullptr);
+        msplat_gpu_sync();
+        download(out_img, img);
+        if (final_T) {
+            MTensor depth, fT;
+            msplat_render_aux_outputs(depth, fT);
+            download(final_T, fT);
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+// One msplat_train_step. params[6] and m1[6]/m2[6] are read and written in place
+// (means, scales, quats, dc, rest, opacity); vis/gn/ms are accumulated in place.
+extern "C" int ref_train_step(int N, float **params, int rest_bases, float **m1, float **m2, const float *viewmat,
+                              const float *projmat, float fx, float fy, float cx, float cy, int H, int W, float clip,
+                              int degree, int dtu, const float *cam_pos, const float *bg, const float *gt,
+                              float ssim_weight, const float *steps, const float *bc2s, float b1, float b2, float eps,
+                              float *vis, float *gn, float *ms, int32_t *radii_out, float *loss_out) {
+    try {
+        Scene s(N, params[0], params[1], params[2], params[3], params[4], rest_bases, params[5], viewmat, projmat, bg);
+        MTensor P[6] = {s.means, s.scales, s.quats, s.dc, s.rest, s.opac};
+        MTensor A[6], B[6];
+        for (int g = 0; g < 6; g++) {
+            A[g] = upload(m1[g], P[g].shape(), DType::Float32);
+            B[g] = upload(m2[g], P[g].shape(), DType::Float32);
+        }
+        MTensor gtt = upload(gt, {H, W, 3}, DType::Float32);
+        MTensor win = gpu_zeros({11, 11}, DType::Float32);
+        MTensor tv = upload(vis, {N}, DType::Float32), tg = upload(gn, {N}, DType::Float32),
+                tm = upload(ms, {N}, DType::Float32);
+        float ss[6], bb[6], cam[3] = {cam_pos[0], cam_pos[1], cam_pos[2]};
+        for (int g = 0; g < 6; g++) { ss[g] = steps[g]; bb[g] = bc2s[g]; }
+        auto [radii, loss] = msplat_train_step(
+            N, s.means, s.scales, 1.0f, s.quats, s.vm, s.pm, fx, fy, cx, cy, (unsigned)H, (unsigned)W, tiles(W, H), clip,
+            (unsigned)degree, (unsigned)dtu, cam, s.dc, s.rest, s.opac, s.bg, gtt, win, ssim_weight,
+            1.0f / (float)(H * W * 3), rest_bases, 6, P, A, B, ss, bb, b1, b2, eps, tv, tg, tm,
+            1.0f / (float)(H > W ? H : W), nullptr);
+        msplat_gpu_sync();
+        for (int g = 0; g < 6; g++) {
+            download(params[g], P[g]);
+            download(m1[g], A[g]);
+            download(m2[g], B[g]);
+        }
+        download(vis, tv);
+        download(gn, tg);
+        download(ms, tm);
+        download(radii_out, radii);
+        if (loss_out) *loss_out = loss;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+// msplat_densify over capacity-sized buffers: bufs[0..5] params, bufs[6..11] first
+// moments, bufs[12..17] second moments, each with `cap` rows. Returns the new count.
+extern "C" int ref_densify(int N, int cap, float grad_thr, float size_thr, float screen_thr, int check_screen,
+                           float cull_alpha, float cull_scale, float cull_screen, int check_huge, const float *gn,
+                           const float *vc, const float *ms, float half_max_dim, float **bufs, int fr_stride,
+                           uint32_t seed) {
+    try {
+        const int64_t width[6] = {3, 3, 4, 3, fr_stride, 1};
+        MTensor b[18];
+        for (int i = 0; i < 18; i++) b[i] = upload(bufs[i], {cap, width[i % 6]}, DType::Float32);
+        MTensor tg = upload(gn, {N}, DType::Float32), tv = upload(vc, {N}, DType::Float32),
+                tm = upload(ms, {N}, DType::Float32);
+        MTensor sc[8];
+        for (auto &t : sc) t = gpu_zeros({cap}, DType::Int32);
+        int out = msplat_densify(N, cap, grad_thr, size_thr, screen_thr, check_screen, cull_alpha, cull_scale,
+                                 cull_screen, check_huge, tg, tv, tm, half_max_dim, b[0], b[1], b[2], b[3], b[4], b[5],
+                                 fr_stride, b + 6, b + 12, sc[0], sc[1], sc[2], sc[3], sc[4], sc[5], sc[6], sc[7], seed);
+        for (int i = 0; i < 18; i++) download(bufs[i], b[i]);
+        return out;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+extern "C" void ref_opacity_reset(float *o, float *m1, float *m2, int n, float reset_logit) {
+    MTensor a = upload(o, {n}, DType::Float32), b = upload(m1, {n}, DType::Float32), c = upload(m2, {n}, DType::Float32);
+    msplat_opacity_reset(a, b, c, n, reset_logit);
+    download(o, a);
+    download(m1, b);
+    download(m2, c);
+}
+
+extern "C" void ref_radix_sort(uint64_t *keys, int32_t *vals, uint32_t n, int key_bits) {
+    int64_t m = n > 0 ? n : 1;
+    MTensor k = upload(nullptr, {m}, DType::Int64), v = upload(nullptr, {m}, DType::Int32);
+    std::memcpy(k.data_ptr(), keys, (size_t)n * 8);
+    std::memcpy(v.data_ptr(), vals, (size_t)n * 4);
+    msplat_radix_sort(k, v, n, key_bits);
+    std::memcpy(keys, k.data_ptr(), (size_t)n * 8);
+    std::memcpy(vals, v.data_ptr(), (size_t)n * 4);
+}
+
+extern "C" void ref_knn3(const float *pts, uint32_t n, float *out) {
+    MTensor p = upload(pts, {(int64_t)n, 3}, DType::Float32), d = upload(nullptr, {(int64_t)n}, DType::Float32);
+    msplat_knn3_mean_dist(p, n, d);
+    download(out, d);
+}
+
+extern "C" void ref_init_gaussians(const float *means, const uint8_t *rgb, uint32_t n, uint32_t seed, float logit,
+                                   float *scales, float *quats, float *dc, float *opac) {
+    int64_t m = n;
+    MTensor tm = upload(means, {m, 3}, DType::Float32), tc = upload(rgb, {m, 3}, DType::UInt8);
+    MTensor s = upload(nullptr, {m, 3}, DType::Float32), q = upload(nullptr, {m, 4}, DType::Float32),
+            d = upload(nullptr, {m, 3}, DType::Float32), o = upload(nullptr, {m, 1}, DType::Float32);
+    msplat_init_gaussians(tm, tc, n, seed, logit, s, q, d, o);
+    download(scales, s);
+    download(quats, q);
+    download(dc, d);
+    download(opac, o);
+}
+
+extern "C" void ref_image_metrics(const float *a, const float *b, int h, int w, double *out) {
+    MTensor ta = upload(a, {h, w, 3}, DType::Float32), tb = upload(b, {h, w, 3}, DType::Float32);
+    msplat_image_metrics(ta, tb, h, w, out);
+}
+
+extern "C" void ref_pack_rgba8(const float *img, uint32_t n, uint8_t *out) {
+    MTensor t = upload(img, {(int64_t)n, 3}, DType::Float32);
+    msplat_pack_rgba8(t, n, out);
+}

@@ -1,0 +1,148 @@
+"""Writes the golden vectors in this folder from the compiled reference CPU backend
+(core/cpu/msplat_cpu.cpp built by oracle/Makefile into oracle/_ref). Run where the
+reference tree is mounted:
+
+    make -C oracle && python tests/golden/make_golden.py
+
+Each .npz holds the inputs and the reference's outputs for one entry point of
+core/metal/bindings.h. Inputs are seeded synthetic scenes.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "msplat-street_amd"))
+
+from oracle import Reference  # noqa: E402
+from msplat.camera import look_at, make_camera  # noqa: E402
+from msplat.scenes import random_scene  # noqa: E402
+
+BG = np.array([0.613, 0.0101, 0.3984], np.float32)
+KEYS = ("means", "scales", "quats", "dc", "rest", "opac")
+
+
+def cams(dtu):
+    return [make_camera(look_at((2.5, 0.7, 2.9), (0.1, -0.2, 0.0)), 67, 45, 60.0, 58.0, 33.4, 22.8, dtu=dtu),
+            make_camera(look_at((-3.1, -1.2, 1.4), (0.0, 0.3, -0.2)), 40, 33, 41.0, 41.0, 19.0, 17.5, dtu=dtu)]
+
+
+def cam_fields(c, prefix):
+    return {prefix + k: np.asarray(c[k]) for k in ("viewmat", "projmat", "cam_pos", "fx", "fy", "cx", "cy", "W", "H",
+                                                   "dtu")}
+
+
+def main():
+    ref = Reference()
+    out = {}
+    # render at SH degrees 0, 1, 3 from two cameras (non-tile-multiple sizes)
+    for deg in (0, 1, 3):
+        sc = random_scene(100 + deg, 400, degree=deg, log_scale=(-3.0, 0.6))
+        d = {k: sc[k] for k in KEYS}
+        d["degree"] = np.int32(deg)
+        for i, c in enumerate(cams(deg)):
+            img, fT = ref.render(sc, c, BG)
+            d.update(cam_fields(c, f"cam{i}_"))
+            d[f"img{i}"], d[f"final_T{i}"] = img, fT
+        d["bg"] = BG
+        out[f"render_deg{deg}"] = d
+
+    # three training steps (SH degree 3 in use), zero-initialised moments
+    sc = random_scene(7, 400, degree=3, log_scale=(-3.0, 0.6))
+    state = {k: sc[k].copy() for k in KEYS}
+    for k in KEYS:
+        state["m1_" + k] = np.zeros_like(sc[k])
+        state["m2_" + k] = np.zeros_like(sc[k])
+    N = sc["means"].shape[0]
+    state.update(degree=3, vis=np.zeros(N, np.float32), gn=np.zeros(N, np.float32), ms=np.zeros(N, np.float32))
+    d = {"init_" + k: sc[k].copy() for k in KEYS}
+    lrs = np.array([1.6e-4, 5e-3, 1e-3, 2.5e-3, 1.25e-4, 5e-2], np.float32)
+    cs = cams(3)
+    rng = np.random.default_rng(11)
+    losses, steps_all, bc2_all = [], [], []
+    for t in range(3):
+        c = cs[t % 2]
+        gt = np.clip(rng.random((c["H"], c["W"], 3)), 0, 1).astype(np.float32)
+        bc1 = np.float32(1) - np.float32(0.9) ** np.float32(t + 1)
+        bc2 = np.float32(1) - np.float32(0.999) ** np.float32(t + 1)
+        steps = (lrs / bc1).astype(np.float32)
+        bc2s = np.full(6, np.sqrt(bc2), np.float32)
+        loss, radii = ref.train_step(state, c, BG, gt, steps, bc2s)
+        losses.append(loss)
+        steps_all.append(steps)
+        bc2_all.append(bc2s)
+        d[f"gt{t}"] = gt
+        d[f"radii{t}"] = radii
+    for i, c in enumerate(cs):
+        d.update(cam_fields(c, f"cam{i}_"))
+    d.update(bg=BG, losses=np.array(losses, np.float32), steps=np.array(steps_all), bc2s=np.array(bc2_all))
+    for k in KEYS:
+        for pre in ("", "m1_", "m2_"):
+            d["out_" + pre + k] = state[pre + k]
+    for k in ("vis", "gn", "ms"):
+        d["out_" + k] = state[k]
+    out["train_steps"] = d
+
+    # densify: one call with splits, duplicates and culls, huge-culling on and off
+    g = np.random.default_rng(21)
+    N, fr = 300, 9
+    cap = 3 * N
+    widths = [3, 3, 4, 3, fr, 1]
+    inputs = []
+    for i in range(18):
+        a = g.normal(0, 1, (N, widths[i % 6])).astype(np.float32)
+        if i == 1:
+            a = g.normal(-3.5, 1.5, (N, 3)).astype(np.float32)
+        if i >= 6:
+            a = (np.abs(a) * 0.01).astype(np.float32)
+        inputs.append(a)
+    vc = g.integers(0, 4, N).astype(np.float32)
+    gn = (g.random(N) * 6e-3 * vc).astype(np.float32)
+    ms = (g.random(N) * 0.3).astype(np.float32)
+    d = {f"in{i}": a for i, a in enumerate(inputs)}
+    d.update(vc=vc, gn=gn, ms=ms, N=np.int32(N), fr=np.int32(fr), seed=np.uint32((700 * 2654435761) % 2 ** 32))
+    for ch in (0, 1):
+        bufs = []
+        for i in range(18):
+            b = np.zeros((cap, widths[i % 6]), np.float32)
+            b[:N] = inputs[i]
+            bufs.append(b)
+        n = ref.densify(bufs, N, cap, gn, vc, ms, 40.0, fr, int(d["seed"]), cull_alpha=0.1, check_huge=ch)
+        d[f"count_huge{ch}"] = np.int32(n)
+        for i in range(18):
+            d[f"out{ch}_{i}"] = bufs[i][:n]
+    out["densify"] = d
+
+    # kNN and Gaussian init on a street-like cloud with duplicates
+    road = np.c_[g.uniform(-20, 20, 600), g.normal(0, 0.02, 600), g.uniform(0, 80, 600)]
+    facade = np.c_[8 + g.normal(0, 0.05, 300), g.uniform(0, 10, 300), g.uniform(0, 80, 300)]
+    pts = np.vstack([road, facade, g.normal(0, 300, (60, 3)), np.repeat(g.uniform(-1, 1, (5, 3)), 4, 0)]).astype(
+        np.float32)
+    rgb = g.integers(0, 256, (pts.shape[0], 3), dtype=np.uint8)
+    s, q, dc, o = ref.init_gaussians(pts, rgb, seed=42)
+    out["knn_init"] = dict(points=pts, rgb=rgb, knn=ref.knn3(pts), scales=s, quats=q, dc=dc, opac=o)
+
+    # radix sort (stable, low key_bits), metrics, RGBA8 packing
+    d = {}
+    for i, (n, bits, hi) in enumerate([(1, 32, 10), (1025, 32, 7), (3000, 32, 2 ** 30), (2049, 64, 3),
+                                       (1500, 45, 2 ** 45)]):
+        keys = g.integers(0, hi, n, dtype=np.uint64)
+        vals = np.arange(n, dtype=np.int32)
+        k2, v2 = ref.radix_sort(keys, vals, bits)
+        d.update({f"keys{i}": keys, f"bits{i}": np.int32(bits), f"sorted{i}": k2, f"order{i}": v2})
+    out["radix_sort"] = d
+    a = g.random((37, 53, 3), dtype=np.float32)
+    b = np.clip(a + g.normal(0, 0.05, a.shape), 0, 1).astype(np.float32)
+    img = g.uniform(-0.2, 1.2, (21, 17, 3)).astype(np.float32)
+    out["image_ops"] = dict(a=a, b=b, metrics=ref.image_metrics(a, b), img=img, rgba=ref.pack_rgba8(img))
+
+    for name, d in out.items():
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
+        print(name, os.path.getsize(os.path.join(HERE, name + ".npz")))
+
+
+if __name__ == "__main__":
+    main()
