@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Benchmark: street-scale single-view 3DGS training on one MI355X.
+
+Workload (the headline configuration): 475,115 Gaussians, SH degree 3, fp32, means on
+spheres around two capture centres 10 m apart with log-uniform depth in 2-60 m, scales
+from the exact 3-NN mean distance, 1264x832 pinhole views (fx = fy = 900) on the 20 m
+path between the centres, seeded ground-truth Gaussians rendered as targets; one view
+per step, densify/prune every 100 steps from step 500 (grad 2e-4, opacity prune 0.005),
+3000-step schedule. A step is one msplat_hip_train_step plus the trainer's schedule.
+
+    python bench.py [--steps K] [--warmup W]
+
+prints one JSON line. The timed region is K steps of the schedule after W warm-up
+steps, bracketed by stream syncs; per-stage GPU times come from HIP events on the
+training stream inside that region.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "msplat-street_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "train iter/s @475k Gaussians 1264x832; batched views/s; served p99 latency"
+N_GAUSSIANS = 475_115
+W_IMG, H_IMG, FOCAL = 1264, 832, 900.0
+N_VIEWS = 64
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+# Algorithmic bytes per unit (minimum fp32 reads + writes), per kernel class
+BYTES = {
+    "rast_fwd": (36, 20, 0),            # per intersection, per pixel, per gaussian
+    "rast_bwd": (40, 28, 40),
+    "loss_fwd_bwd": (0, 36, 0),
+    "prefix_sort_pack": (24 + 144 + 8, 0, 24),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(ctx, seed=0):
+    from msplat._abi import DeviceArray
+    from msplat.backend import DeviceGaussians, render
+    from msplat.scenes import street_cameras, street_points
+    pts, rgb = street_points(seed, N_GAUSSIANS)
+    dg = DeviceGaussians.from_points(ctx, pts, rgb, degree=3, capacity=4 * N_GAUSSIANS)
+    # ground truth: a separate seeded set, made opaque and view-dependent
+    gpts, grgb = street_points(seed + 1, N_GAUSSIANS)
+    gt = DeviceGaussians.from_points(ctx, gpts, grgb, degree=3)
+    rng = np.random.default_rng(seed + 2)
+    gt.params["opac"].upload(rng.normal(1.5, 1.0, (N_GAUSSIANS, 1)).astype(np.float32))
+    gt.params["rest"].upload(rng.normal(0, 0.05, (N_GAUSSIANS, 45)).astype(np.float32))
+    cams = street_cameras(N_VIEWS, W_IMG, H_IMG, FOCAL, FOCAL)
+    bg = (0.6130, 0.0101, 0.3984)
+    targets = []
+    for c in cams:
+        p = render(ctx, gt, dict(c, dtu=3), bg)
+        t = DeviceArray(ctx, (H_IMG, W_IMG, 3))
+        ctx.upload(t.ptr, ctx.download(p, (H_IMG, W_IMG, 3), np.float32))
+        targets.append(t)
+    return dg, gt, cams, targets
+
+
+def cpu_baseline_sample(dg, cams, targets, ctx, step_index, seconds=20.0):
+    """Times the reference CPU train step (oracle/_ref, perf build) — or the oracle when
+    the reference was not built — on this host, on the workload's own scene and view."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    threads = int(os.environ.get("MSPLAT_THREADS", "16"))
+    os.environ["MSPLAT_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    from oracle import Oracle, Reference, reference_available
+    kind = "reference" if reference_available(perf=True) else "port"
+    impl = Reference(perf=True) if kind == "reference" else Oracle()
+    full = dg.to_numpy()
+    keys = ("means", "scales", "quats", "dc", "rest", "opac")
+    st = {k: full[k].copy() for k in keys}
+    for k in keys:
+        st["m1_" + k] = np.zeros_like(full[k])
+        st["m2_" + k] = np.zeros_like(full[k])
+    n = full["means"].shape[0]
+    st.update(degree=dg.degree, vis=np.zeros(n, np.float32), gn=np.zeros(n, np.float32), ms=np.zeros(n, np.float32))
+    cam = dict(cams[0], dtu=min(step_index // 1000, 3))
+    gt = targets[0].numpy()
+    lrs = np.array([1.6e-4, 5e-3, 1e-3, 2.5e-3, 1.25e-4, 5e-2], np.float32)
+    steps, bc2 = lrs / np.float32(0.1), np.full(6, np.sqrt(np.float32(0.001)), np.float32)
+    times = []
+    t_start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        impl.train_step(st, cam, np.array([0.6130, 0.0101, 0.3984], np.float32), gt, steps, bc2)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start > seconds or len(times) >= 5:
+            break
+    per = float(np.median(times))
+    return {"value": 1.0 / per, "unit": "iter/s", "cores": threads, "kind": kind,
+            "sample": f"{len(times)} train step(s) of the same 475,115-Gaussian scene on view 0 at "
+                      f"{W_IMG}x{H_IMG} (SH degree in use {cam['dtu']}), median {per:.2f} s/step, "
+                      f"MSPLAT_THREADS={threads}, host {platform.processor() or platform.machine()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2950)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--iterations", type=int, default=3000, help="schedule length (densify window, LR decay)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from msplat._abi import Context
+    from msplat.trainer import TrainConfig, Trainer
+
+    ctx = Context(0)
+    t_setup = time.perf_counter()
+    dg, gt, cams, targets = build_workload(ctx)
+    del gt
+    ctx.sync()
+    setup_s = time.perf_counter() - t_setup
+    log(f"workload ready in {setup_s:.1f} s: {dg.n} gaussians, {len(cams)} views")
+    cfg = TrainConfig(iterations=args.iterations, sh_degree=3, warmup_length=499, cull_alpha_thresh=0.005)
+    tr = Trainer(ctx, dg, cams, targets, cfg)
+    n_start = dg.n
+    for _ in range(args.warmup):
+        tr.step()
+    ctx.sync()
+    first_timed_step = tr.step_count + 1
+    log(f"warm-up done ({args.warmup} steps); timing {args.steps} steps")
+
+    ctx.enable_stage_timing(True)
+    ctx.drain_stage_times()
+    m_total, n_total = 0, 0
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tr.step()
+        m_total += ctx.frame().num_isects
+        n_total += dg.n
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    stages = ctx.drain_stage_times()
+    ctx.enable_stage_timing(False)
+    loss = tr.step(read_loss=True)
+    log(f"timed {args.steps} steps in {elapsed:.3f} s; gaussians {n_start} -> {dg.n}")
+
+    K = args.steps
+    iters = K / elapsed
+    P = W_IMG * H_IMG
+    # dominant kernel class and its roofline
+    dom = max(stages, key=lambda k: stages[k][0])
+    dom_ms, dom_calls = stages[dom]
+    avg_ms = dom_ms / max(dom_calls, 1)
+    per_isect, per_px, per_g = BYTES.get(dom, (0, 0, 0))
+    alg_bytes = (per_isect * m_total + per_px * P * K + per_g * n_total) / max(dom_calls, 1)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(alg_bytes),
+                "stages_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in stages.items()}}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline")
+        cpu = cpu_baseline_sample(dg, cams, targets, ctx, first_timed_step, args.cpu_seconds)
+    out = {
+        "metric": METRIC, "value": round(iters, 3), "unit": "iter/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / K, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "street-scale single-view training: 475,115 SH3 Gaussians, 1264x832, one view "
+                               "per step, densify every 100 from step 500 (grad 2e-4, prune 0.005), 3000-step "
+                               "schedule",
+                   "gaussians_start": n_start, "gaussians_end": dg.n, "steps_timed": [first_timed_step,
+                                                                                      first_timed_step + K - 1],
+                   "mean_intersections_per_step": int(m_total / K), "densify_events": tr.densify_log,
+                   "final_loss": loss, "setup_s": round(setup_s, 1)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

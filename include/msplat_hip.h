@@ -165,6 +165,17 @@ int msplat_hip_stage_raster_forward(msplat_hip_ctx *ctx, const msplat_hip_camera
 int msplat_hip_stage_loss(msplat_hip_ctx *ctx, const msplat_hip_camera *cam, const float *gt, float ssim_weight,
                           float loss_inv_n, float *loss_out);
 
+/* ── stage timing (msplat_enable_gpu_timing / msplat_drain_stage_times,
+ * bindings.h:34-41) ─────────────────────────────────────────────────────────
+ * HIP events on the context's stream around each stage of render/train/backward:
+ * 0 proj_sh_fwd, 1 prefix_sort_pack, 2 rast_fwd, 3 loss_fwd_bwd, 4 rast_bwd,
+ * 5 proj_sh_bwd_adam. drain syncs, writes the accumulated milliseconds and launch
+ * counts per stage (max_stages entries each) and clears them. */
+enum { MSPLAT_NUM_STAGES = 6 };
+int msplat_hip_enable_stage_timing(msplat_hip_ctx *ctx, int enable);
+int msplat_hip_drain_stage_times(msplat_hip_ctx *ctx, double *ms, int64_t *calls, int max_stages, int *n_stages);
+const char *msplat_hip_stage_name(int stage);
+
 /* ── densification and maintenance (bindings.h:120-147) ─────────────────────── */
 
 /* msplat_densify (bindings.h:120-135): classify, split (2 children, PCG normals),

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "common.h"
 #include "kernels.h"
@@ -64,6 +65,57 @@ struct msplat_hip_ctx {
     // per tile / pixel
     Buf tile_start, image, final_T, final_idx, v_image, pix_loss, row_loss;
     Buf grad_scratch;  // stands in for NULL gradient outputs of render_backward
+    Buf aux;           // densify flags/scans/scratch, kNN and metrics workspaces
+    // stage timing
+    bool timing = false;
+    struct Mark {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;
+    std::vector<hipEvent_t> event_pool;
+    double stage_ms[MSPLAT_NUM_STAGES] = {};
+    int64_t stage_calls[MSPLAT_NUM_STAGES] = {};
+    int open_stage = -1;
+    hipEvent_t open_event = nullptr;
+
+    hipEvent_t take_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    void begin(int stage) {
+        if (!timing) return;
+        open_stage = stage;
+        open_event = take_event();
+        if (open_event) (void)hipEventRecord(open_event, stream);
+    }
+    void end() {
+        if (!timing || open_stage < 0 || !open_event) return;
+        hipEvent_t e = take_event();
+        if (!e) return;
+        (void)hipEventRecord(e, stream);
+        marks.push_back({open_stage, open_event, e});
+        open_stage = -1;
+        open_event = nullptr;
+    }
+    void collect() {  // call after a stream sync
+        for (auto &m : marks) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
+                stage_ms[m.stage] += ms;
+                stage_calls[m.stage] += 1;
+            }
+            event_pool.push_back(m.a);
+            event_pool.push_back(m.b);
+        }
+        marks.clear();
+    }
     // device scalars: [0] u64 isect total, [8] u32 total, [16] double loss sum, [24] float loss
     Buf scalars;
     uint64_t *h_total = nullptr;  // pinned
@@ -102,9 +154,11 @@ struct msplat_hip_ctx {
         Buf *all[] = {&splat_a, &splat_b, &splat_blue, &depths, &aabb, &rect, &radii, &count, &offsets, &block_sums,
                       &v_xy, &v_conic, &v_rgb, &v_opac, &keys0, &keys1, &vals0, &vals1, &slot_a, &slot_b, &slot_c,
                       &hist, &digit_total, &tile_start, &image, &final_T, &final_idx, &v_image, &pix_loss,
-                      &row_loss, &scalars, &grad_scratch};
+                      &row_loss, &scalars, &grad_scratch, &aux};
         for (Buf *b : all) release(*b);
         if (h_total) (void)hipHostFree(h_total);
+        collect();
+        for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 
@@ -184,6 +238,7 @@ int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_
     a.radii = (int32_t *)ctx->radii.p;
     a.count = (uint32_t *)ctx->count.p;
     a.block_sums = (uint32_t *)ctx->block_sums.p;
+    ctx->begin(0);
     launch_project(a, ctx->stream);
     HIPCHECK(hipGetLastError());
     return MSPLAT_OK;
@@ -195,9 +250,11 @@ int do_bin(msplat_hip_ctx *ctx) {
     const int nblocks = (n + 255) / 256;
     if (!ctx->h_total) HIPCHECK(hipHostMalloc((void **)&ctx->h_total, 64, hipHostMallocDefault));
     launch_scan_blocks((uint32_t *)ctx->block_sums.p, nblocks, ctx->d_total64(), ctx->d_total32(), ctx->stream);
+    ctx->end();
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(ctx->h_total, ctx->d_total64(), 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->collect();
     const uint64_t M = *ctx->h_total;
     if (M > 0x7FFFFFFFull)
         return set_error(MSPLAT_ERR_RANGE, "more than 2^31 tile intersections (%llu); lower the resolution",
@@ -221,6 +278,7 @@ int do_bin(msplat_hip_ctx *ctx) {
     k.offsets = (uint32_t *)ctx->offsets.p;
     k.keys = (uint64_t *)ctx->keys0.p;
     k.vals = (int32_t *)ctx->vals0.p;
+    ctx->begin(1);
     launch_keygen(k, ctx->stream);
     uint64_t *kin = (uint64_t *)ctx->keys0.p, *kout = (uint64_t *)ctx->keys1.p;
     int32_t *vin = (int32_t *)ctx->vals0.p, *vout = (int32_t *)ctx->vals1.p;
@@ -234,6 +292,7 @@ int do_bin(msplat_hip_ctx *ctx) {
     ctx->keys_sorted = kin;
     ctx->gid_sorted = vin;
     launch_tile_ranges(kin, ctx->d_total32(), (uint32_t)M, ctx->ntiles, (int32_t *)ctx->tile_start.p, ctx->stream);
+    ctx->end();
     HIPCHECK(hipGetLastError());
     return MSPLAT_OK;
 }
@@ -276,7 +335,9 @@ RasterArgs raster_args(msplat_hip_ctx *ctx) {
 int do_raster_forward(msplat_hip_ctx *ctx, const float bg[3]) {
     for (int i = 0; i < 3; i++) ctx->bg[i] = bg ? bg[i] : 0.f;
     CHECKED(ensure_pixels(ctx));
+    ctx->begin(2);
     launch_raster_forward(raster_args(ctx), ctx->stream);
+    ctx->end();
     HIPCHECK(hipGetLastError());
     return MSPLAT_OK;
 }
@@ -294,7 +355,9 @@ int do_loss(msplat_hip_ctx *ctx, const float *gt, float ssim_weight, float inv_n
     l.row_loss = (double *)ctx->row_loss.p;
     l.loss_sum = ctx->d_loss_sum();
     l.loss_mean = ctx->d_loss();
+    ctx->begin(3);
     launch_loss(l, ctx->stream);
+    ctx->end();
     HIPCHECK(hipGetLastError());
     return MSPLAT_OK;
 }
@@ -304,7 +367,9 @@ int do_raster_backward(msplat_hip_ctx *ctx) {
     CHECKED(ctx->grow(ctx->slot_a, 16 * Mc));
     CHECKED(ctx->grow(ctx->slot_b, 16 * Mc));
     CHECKED(ctx->grow(ctx->slot_c, 4 * Mc));
+    ctx->begin(4);
     launch_raster_backward(raster_args(ctx), ctx->stream);
+    ctx->end();
     HIPCHECK(hipGetLastError());
     return MSPLAT_OK;
 }
@@ -514,7 +579,9 @@ int msplat_hip_train_step(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, co
     b.gn = stats->xys_grad_norm;
     b.ms = stats->max_2d_size;
     b.inv_max_dim = inv_max_dim;
+    ctx->begin(5);
     launch_backward(b, BWD_ADAM, ctx->stream);
+    ctx->end();
     HIPCHECK(hipGetLastError());
     if (radii_out && g->num_points > 0)
         HIPCHECK(hipMemcpyAsync(radii_out, ctx->radii.p, 4 * (size_t)g->num_points, hipMemcpyDeviceToDevice,
@@ -570,7 +637,9 @@ int msplat_hip_render_backward(msplat_hip_ctx *ctx, const msplat_hip_gaussians *
     b.g_dc = pick(out->v_features_dc, 3 * N);
     b.g_rest = pick(out->v_features_rest, (nb - 1) * 3 * N);
     b.g_opac = pick(out->v_opacities, N);
+    ctx->begin(5);
     launch_backward(b, BWD_GRADS, ctx->stream);
+    ctx->end();
     HIPCHECK(hipGetLastError());
     return MSPLAT_OK;
 }
@@ -638,6 +707,186 @@ int msplat_hip_radix_sort(msplat_hip_ctx *ctx, uint64_t *keys, int32_t *vals, ui
     ctx->keys_sorted = nullptr;
     ctx->gid_sorted = nullptr;
     return MSPLAT_OK;
+}
+
+int msplat_hip_densify(msplat_hip_ctx *ctx, int32_t n, int32_t capacity, float grad_thresh, float size_thresh,
+                       float screen_thresh, int check_screen, float cull_alpha_thresh, float cull_scale_thresh,
+                       float cull_screen_size, int check_huge, const float *xys_grad_norm, const float *vis_counts,
+                       const float *max_2d_size, float half_max_dim, float *const bufs[18], int32_t fr_stride,
+                       uint32_t seed, int32_t *new_count) {
+    if (!ctx || !bufs || !new_count) return set_error(MSPLAT_ERR_ARG, "ctx, bufs or new_count is NULL");
+    if (n < 0 || fr_stride < 1) return set_error(MSPLAT_ERR_ARG, "bad gaussian count or rest stride");
+    if ((int64_t)3 * n > capacity) return set_error(MSPLAT_ERR_CAPACITY, "msplat_densify: 3*N exceeds buf_capacity");
+    for (int i = 0; i < 18; i++)
+        if (n > 0 && !bufs[i]) return set_error(MSPLAT_ERR_ARG, "densify buffer %d is NULL", i);
+    if (n == 0) {
+        *new_count = 0;
+        return MSPLAT_OK;
+    }
+    DensifyArgs a;
+    memset(&a, 0, sizeof a);
+    a.n = n;
+    a.grad_thresh = grad_thresh;
+    a.size_thresh = size_thresh;
+    a.screen_thresh = screen_thresh;
+    a.cull_alpha = cull_alpha_thresh;
+    a.cull_scale = cull_scale_thresh;
+    a.cull_screen = cull_screen_size;
+    a.half_max_dim = half_max_dim;
+    a.log_shrink = logf(1.6f);
+    a.check_screen = check_screen;
+    a.check_huge = check_huge;
+    a.seed = seed;
+    a.gn = xys_grad_norm;
+    a.vc = vis_counts;
+    a.ms = max_2d_size;
+    for (int i = 0; i < 18; i++) a.bufs[i] = bufs[i];
+    const int widths[6] = {3, 3, 4, 3, fr_stride, 1};
+    for (int i = 0; i < 6; i++) a.width[i] = widths[i];
+    const size_t N3 = 3 * (size_t)n, nb3 = (N3 + 255) / 256 + 1;
+    // workspace: flags (3 x 3N bytes), block sums (3 x nb3), ranks (3N), scalars, scratch (3N x max width)
+    const size_t maxw = (size_t)(fr_stride > 4 ? fr_stride : 4);
+    const size_t bytes = 3 * N3 + 16 + 3 * 4 * nb3 + 4 * N3 + 64 + 4 * N3 * maxw + 256;
+    CHECKED(ctx->grow(ctx->aux, bytes));
+    char *p = (char *)ctx->aux.p;
+    auto take = [&](size_t b) {
+        char *r = p;
+        p += (b + 15) & ~(size_t)15;
+        return (void *)r;
+    };
+    a.split = (uint8_t *)take(N3);
+    a.dup = (uint8_t *)take(N3);
+    a.keep = (uint8_t *)take(N3);
+    a.split_bs = (uint32_t *)take(4 * nb3);
+    a.dup_bs = (uint32_t *)take(4 * nb3);
+    a.keep_bs = (uint32_t *)take(4 * nb3);
+    a.dst = (int32_t *)take(4 * N3);
+    uint64_t *tot = (uint64_t *)take(64);
+    float *scratch = (float *)take(4 * N3 * maxw);
+    if (!ctx->h_total) HIPCHECK(hipHostMalloc((void **)&ctx->h_total, 64, hipHostMallocDefault));
+    const int nb = (n + 255) / 256;
+    launch_densify_classify(a, ctx->stream);
+    launch_scan_blocks(a.split_bs, nb, tot, (uint32_t *)(tot + 3), ctx->stream);
+    launch_scan_blocks(a.dup_bs, nb, tot + 1, (uint32_t *)(tot + 3) + 1, ctx->stream);
+    HIPCHECK(hipMemcpyAsync(ctx->h_total, tot, 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    const int n_split = (int)ctx->h_total[0], n_dup = (int)ctx->h_total[1];
+    a.n_split = n_split;
+    a.n_new = n + 2 * n_split + n_dup;
+    launch_densify_append(a, ctx->stream);
+    launch_densify_cull(a, ctx->stream);
+    const int nbn = (a.n_new + 255) / 256;
+    launch_scan_blocks(a.keep_bs, nbn, tot + 2, (uint32_t *)(tot + 3) + 2, ctx->stream);
+    launch_densify_rank(a, ctx->stream);
+    HIPCHECK(hipMemcpyAsync(ctx->h_total + 2, tot + 2, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    const int kept = (int)ctx->h_total[2];
+    for (int b = 0; b < 18; b++) {
+        const int w = widths[b % 6];
+        launch_compact_scatter(bufs[b], scratch, a.dst, a.n_new, w, ctx->stream);
+        HIPCHECK(hipMemcpyAsync(bufs[b], scratch, 4 * (size_t)kept * w, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    *new_count = kept;
+    return MSPLAT_OK;
+}
+
+int msplat_hip_opacity_reset(msplat_hip_ctx *ctx, float *opacities, float *exp_avg, float *exp_avg_sq, int32_t n,
+                             float reset_logit) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    if (n > 0 && (!opacities || !exp_avg || !exp_avg_sq)) return set_error(MSPLAT_ERR_ARG, "NULL buffer");
+    launch_opacity_reset(opacities, exp_avg, exp_avg_sq, n, reset_logit, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    return MSPLAT_OK;
+}
+
+int msplat_hip_knn3_mean_dist(msplat_hip_ctx *ctx, const float *points, uint32_t n, float *mean_dist) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    if (n == 0) return MSPLAT_OK;
+    if (!points || !mean_dist) return set_error(MSPLAT_ERR_ARG, "points or mean_dist is NULL");
+    CHECKED(ctx->grow(ctx->aux, knn_scratch_bytes(n) + 1024));
+    KnnScratch w;
+    knn_scratch_carve(ctx->aux.p, n, w);
+    knn3_launch(points, n, mean_dist, w, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    return MSPLAT_OK;
+}
+
+int msplat_hip_init_gaussians(msplat_hip_ctx *ctx, const float *means, const uint8_t *rgb, uint32_t n, uint32_t seed,
+                              float opacity_logit, float *scales, float *quats, float *features_dc,
+                              float *opacities) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    if (n == 0) return MSPLAT_OK;
+    if (!means || !rgb || !scales || !quats || !features_dc || !opacities)
+        return set_error(MSPLAT_ERR_ARG, "NULL buffer");
+    // the mean distances live past the kNN workspace
+    const size_t kb = (knn_scratch_bytes(n) + 1024 + 255) & ~(size_t)255;
+    CHECKED(ctx->grow(ctx->aux, kb + 4 * (size_t)n));
+    KnnScratch w;
+    knn_scratch_carve(ctx->aux.p, n, w);
+    float *md = (float *)((char *)ctx->aux.p + kb);
+    knn3_launch(means, n, md, w, ctx->stream);
+    launch_init_gaussians(md, rgb, n, seed, opacity_logit, scales, quats, features_dc, opacities, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    return MSPLAT_OK;
+}
+
+int msplat_hip_image_metrics(msplat_hip_ctx *ctx, const float *rendered, const float *gt, int h, int w,
+                             double out[3]) {
+    if (!ctx || !out) return set_error(MSPLAT_ERR_ARG, "ctx or out is NULL");
+    out[0] = out[1] = out[2] = 0.0;
+    if (h <= 0 || w <= 0) return MSPLAT_OK;
+    if (!rendered || !gt) return set_error(MSPLAT_ERR_ARG, "NULL image");
+    const size_t P = (size_t)h * w;
+    CHECKED(ctx->grow(ctx->aux, 4 * P * 15 + 4 * P * 3 + 8 * 3 * (size_t)h + 64));
+    float *mom = (float *)ctx->aux.p, *ssim = mom + 15 * P;
+    double *rows = (double *)(ssim + 3 * P), *res = rows + 3 * (size_t)h;  // 72 P bytes in: 8-aligned
+    launch_metrics(rendered, gt, h, w, mom, ssim, rows, res, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(out, res, 24, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    return MSPLAT_OK;
+}
+
+int msplat_hip_pack_rgba8(msplat_hip_ctx *ctx, const float *img, uint32_t n, uint8_t *out) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    if (n == 0) return MSPLAT_OK;
+    if (!img || !out) return set_error(MSPLAT_ERR_ARG, "img or out is NULL");
+    launch_pack_rgba8(img, n, out, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    return MSPLAT_OK;
+}
+
+int msplat_hip_enable_stage_timing(msplat_hip_ctx *ctx, int enable) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->collect();
+    ctx->timing = enable != 0;
+    return MSPLAT_OK;
+}
+
+int msplat_hip_drain_stage_times(msplat_hip_ctx *ctx, double *ms, int64_t *calls, int max_stages, int *n_stages) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->collect();
+    const int n = max_stages < MSPLAT_NUM_STAGES ? max_stages : MSPLAT_NUM_STAGES;
+    for (int i = 0; i < n; i++) {
+        if (ms) ms[i] = ctx->stage_ms[i];
+        if (calls) calls[i] = ctx->stage_calls[i];
+    }
+    for (int i = 0; i < MSPLAT_NUM_STAGES; i++) {
+        ctx->stage_ms[i] = 0.0;
+        ctx->stage_calls[i] = 0;
+    }
+    if (n_stages) *n_stages = MSPLAT_NUM_STAGES;
+    return MSPLAT_OK;
+}
+
+const char *msplat_hip_stage_name(int stage) {
+    static const char *names[MSPLAT_NUM_STAGES] = {"proj_sh_fwd", "prefix_sort_pack", "rast_fwd",
+                                                   "loss_fwd_bwd", "rast_bwd", "proj_sh_bwd_adam"};
+    return stage >= 0 && stage < MSPLAT_NUM_STAGES ? names[stage] : "";
 }
 
 }  // extern "C"

@@ -77,4 +77,30 @@ __device__ inline float wave_sum(float v) {
 
 __device__ inline uint64_t ballot(bool p) { return __ballot(p); }
 
+// ── counter-based RNG: PCG hash (Jarzynski & Olano 2020), 24-bit uniforms in (0, 1),
+//    Box-Muller normals (the reference's pcg_hash / rand_uniform / rand_normal3) ──
+__host__ __device__ inline uint32_t pcg_hash(uint32_t v) {
+    const uint32_t st = v * 747796405u + 2891336453u;
+    const uint32_t w = ((st >> ((st >> 28u) + 4u)) ^ st) * 277803737u;
+    return (w >> 22u) ^ w;
+}
+
+__device__ inline float rand_uniform(uint32_t seed, uint32_t ctr) {
+    const uint32_t h = pcg_hash(seed ^ pcg_hash(ctr));
+    return ((float)(h >> 8) + 0.5f) * (1.0f / 16777216.0f);
+}
+
+__device__ inline void rand_normal3(uint32_t seed, uint32_t stream, float *out) {
+    const uint32_t c = stream * 4u;
+    const float u0 = rand_uniform(seed, c), u1 = rand_uniform(seed, c + 1u), u2 = rand_uniform(seed, c + 2u),
+                u3 = rand_uniform(seed, c + 3u);
+    const float r0 = sqrtf(-2.0f * log_cr(u0)), r1 = sqrtf(-2.0f * log_cr(u2));
+    const float tau = 6.28318530717958647692f;
+    out[0] = r0 * cos_cr(tau * u1);
+    out[1] = r0 * sin_cr(tau * u1);
+    out[2] = r1 * cos_cr(tau * u3);
+}
+
+__global__ inline void fill_u32(uint32_t *p, uint32_t v) { *p = v; }
+
 }  // namespace msplat

@@ -84,6 +84,41 @@ struct BackwardArgs {
     float *v_xy, *v_conic, *v_rgb, *v_opac;
 };
 
+struct DensifyArgs {
+    int n, n_new, n_split;
+    float grad_thresh, size_thresh, screen_thresh, cull_alpha, cull_scale, cull_screen, half_max_dim, log_shrink;
+    int check_screen, check_huge;
+    uint32_t seed;
+    const float *gn, *vc, *ms;
+    float *bufs[18];
+    int width[6];
+    uint8_t *split, *dup, *keep;
+    uint32_t *split_bs, *dup_bs, *keep_bs;
+    int32_t *dst;
+};
+void launch_densify_classify(const DensifyArgs &a, hipStream_t s);
+void launch_densify_append(const DensifyArgs &a, hipStream_t s);
+void launch_densify_cull(const DensifyArgs &a, hipStream_t s);
+void launch_densify_rank(const DensifyArgs &a, hipStream_t s);
+void launch_compact_scatter(const float *src, float *scratch, const int32_t *dst, int n_new, int width, hipStream_t s);
+void launch_opacity_reset(float *o, float *m1, float *m2, int n, float reset, hipStream_t s);
+void launch_pack_rgba8(const float *img, uint32_t n, uint8_t *out, hipStream_t s);
+void launch_metrics(const float *a, const float *b, int h, int w, float *mom, float *ssim, double *rows, double *out,
+                    hipStream_t s);
+
+struct KnnScratch {
+    float *bounds, *box, *sup;
+    uint64_t *codes, *codes_tmp;
+    int32_t *order, *order_tmp;
+    float4 *sorted;
+    uint32_t *hist, *digit_total, *n_dev;
+};
+size_t knn_scratch_bytes(uint32_t n);
+void knn_scratch_carve(void *base, uint32_t n, KnnScratch &w);
+int knn3_launch(const float *pts, uint32_t n, float *mean_dist, KnnScratch &w, hipStream_t s);
+void launch_init_gaussians(const float *md, const uint8_t *rgb, uint32_t n, uint32_t seed, float logit, float *scales,
+                           float *quats, float *dc, float *opac, hipStream_t s);
+
 void launch_project(const ProjectArgs &a, hipStream_t s);
 void launch_rect_count(int n, const float4 *splat_a, const float2 *aabb, const int32_t *radii, int tx, int ty,
                        uint2 *rect, uint32_t *count, uint32_t *block_sums, hipStream_t s);

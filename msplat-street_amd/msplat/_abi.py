@@ -96,6 +96,17 @@ def lib():
             "msplat_hip_stage_raster_forward": ([vp, C.POINTER(Camera), fp], C.c_int),
             "msplat_hip_stage_loss": ([vp, C.POINTER(Camera), vp, f32, f32, fp], C.c_int),
             "msplat_hip_radix_sort": ([vp, vp, vp, u32, C.c_int], C.c_int),
+            "msplat_hip_densify": ([vp, i32, i32, f32, f32, f32, C.c_int, f32, f32, f32, C.c_int, vp, vp, vp, f32,
+                                    C.c_void_p * 18, i32, u32, C.POINTER(i32)], C.c_int),
+            "msplat_hip_opacity_reset": ([vp, vp, vp, vp, i32, f32], C.c_int),
+            "msplat_hip_knn3_mean_dist": ([vp, vp, u32, vp], C.c_int),
+            "msplat_hip_init_gaussians": ([vp, vp, vp, u32, u32, f32, vp, vp, vp, vp], C.c_int),
+            "msplat_hip_image_metrics": ([vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
+            "msplat_hip_pack_rgba8": ([vp, vp, u32, vp], C.c_int),
+            "msplat_hip_enable_stage_timing": ([vp, C.c_int], C.c_int),
+            "msplat_hip_drain_stage_times": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int,
+                                              C.POINTER(C.c_int)], C.c_int),
+            "msplat_hip_stage_name": ([C.c_int], C.c_char_p),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -137,6 +148,17 @@ class Context:
 
     def set_flags(self, flags: int):
         check(lib().msplat_hip_set_flags(self.ptr, flags))
+
+    def enable_stage_timing(self, on: bool = True):
+        check(lib().msplat_hip_enable_stage_timing(self.ptr, 1 if on else 0))
+
+    def drain_stage_times(self) -> dict:
+        """{stage name: (total ms, launches)} since the last drain (syncs)."""
+        ms = (C.c_double * 16)()
+        calls = (C.c_int64 * 16)()
+        n = C.c_int()
+        check(lib().msplat_hip_drain_stage_times(self.ptr, ms, calls, 16, C.byref(n)))
+        return {lib().msplat_hip_stage_name(i).decode(): (ms[i], calls[i]) for i in range(n.value)}
 
     def download(self, ptr, shape, dtype) -> np.ndarray:
         out = np.empty(shape, dtype)

@@ -71,6 +71,67 @@ class DeviceGaussians:
         for d in (self.vis, self.gn, self.ms):
             check(lib().msplat_hip_memset(self.ctx.ptr, C.c_void_p(d.ptr), 0, d.nbytes))
 
+    def buffers18(self):
+        return [self.params[k] for k in KEYS] + [self.m1[k] for k in KEYS] + [self.m2[k] for k in KEYS]
+
+    def ensure_capacity(self, needed: int):
+        """Grow every capacity buffer to max(needed, 2 x capacity), keeping the active
+        rows (Model::ensureCapacity, model.cpp:158-193)."""
+        if needed <= self.capacity:
+            return
+        cap = max(needed, 2 * self.capacity)
+        self.ctx.sync()
+
+        def grow(old: DeviceArray) -> DeviceArray:
+            new = DeviceArray(self.ctx, (cap,) + old.shape[1:], old.dtype)
+            nbytes = self.n * int(np.prod(old.shape[1:], dtype=np.int64)) * old.dtype.itemsize
+            if nbytes:
+                check(lib().msplat_hip_memcpy(self.ctx.ptr, C.c_void_p(new.ptr), C.c_void_p(old.ptr), nbytes, 2))
+            return new
+
+        for d in (self.params, self.m1, self.m2):
+            for k in KEYS:
+                d[k] = grow(d[k])
+        for k in ("vis", "gn", "ms", "radii"):
+            setattr(self, k, grow(getattr(self, k)))
+        self.ctx.sync()
+        self.capacity = cap
+
+    def densify(self, grad_thresh, size_thresh, screen_thresh, check_screen, cull_alpha, cull_scale, cull_screen,
+                check_huge, half_max_dim, seed) -> int:
+        bufs = (C.c_void_p * 18)(*[b.ptr for b in self.buffers18()])
+        out = C.c_int32()
+        check(lib().msplat_hip_densify(self.ctx.ptr, self.n, self.capacity, grad_thresh, size_thresh, screen_thresh,
+                                       check_screen, cull_alpha, cull_scale, cull_screen, check_huge, self.gn.ptr,
+                                       self.vis.ptr, self.ms.ptr, half_max_dim, bufs, self.widths["rest"],
+                                       int(seed) & 0xFFFFFFFF, C.byref(out)))
+        self.n = int(out.value)
+        return self.n
+
+    def opacity_reset(self, reset_logit: float):
+        check(lib().msplat_hip_opacity_reset(self.ctx.ptr, self.params["opac"].ptr, self.m1["opac"].ptr,
+                                             self.m2["opac"].ptr, self.n, reset_logit))
+
+    @classmethod
+    def from_points(cls, ctx: Context, points: np.ndarray, rgb: np.ndarray, degree: int, capacity=None, seed=42,
+                    opacity_logit=None) -> "DeviceGaussians":
+        """Model's initialisation (model.cpp:37-79): exact 3-NN log-scales, uniformly random
+        rotations, DC from the point colours, opacity logit(0.1), zero higher bands —
+        computed on the GPU (msplat_hip_init_gaussians)."""
+        n = int(points.shape[0])
+        nb = sh_bases(degree)
+        scene = dict(means=points, scales=np.zeros((n, 3), np.float32), quats=np.zeros((n, 4), np.float32),
+                     dc=np.zeros((n, 3), np.float32), rest=np.zeros((n, max(nb - 1, 1), 3), np.float32),
+                     opac=np.zeros((n, 1), np.float32), degree=degree)
+        dg = cls(ctx, scene, capacity)
+        logit = float(np.float32(np.log(np.float32(0.1) / np.float32(0.9)))) if opacity_logit is None else opacity_logit
+        drgb = DeviceArray.from_numpy(ctx, np.ascontiguousarray(rgb, np.uint8))
+        check(lib().msplat_hip_init_gaussians(ctx.ptr, dg.params["means"].ptr, drgb.ptr, n, seed, logit,
+                                              dg.params["scales"].ptr, dg.params["quats"].ptr, dg.params["dc"].ptr,
+                                              dg.params["opac"].ptr))
+        ctx.sync()
+        return dg
+
     def to_numpy(self) -> dict:
         out = {"degree": self.degree}
         for k in KEYS:

@@ -1,0 +1,128 @@
+"""Training loop over the HIP backend: the schedule of msplat's Model
+(core/src/model.cpp) — Adam bias corrections (:638-651), the means learning-rate decay
+applied after each step (:140-143), the SH degree ramp (:535), densify/prune and opacity
+reset cadence (:199-245) with capacity growth (:158-193) — driving one
+msplat_hip_train_step per view.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+
+import numpy as np
+
+from ._abi import Context, DeviceArray, check, floats, lib, make_camera_struct
+from .backend import KEYS, DeviceGaussians, adam_coeffs
+
+LR_INIT = (1.6e-4, 5e-3, 1e-3, 2.5e-3, 1.25e-4, 5e-2)  # model.cpp:100
+MEANS_LR_FINAL = 1.6e-6
+RESET_LOGIT = -1.3862943611198906  # logit(0.2)
+
+
+@dataclasses.dataclass
+class TrainConfig:
+    """The TrainingConfig fields of python/bindings.cpp:27-55 that the plain method uses,
+    plus the cull threshold that Model hard-codes to 0.1 (model.cpp:218)."""
+    iterations: int = 30000
+    sh_degree: int = 3
+    sh_degree_interval: int = 1000
+    ssim_weight: float = 0.2
+    refine_every: int = 100
+    warmup_length: int = 500
+    reset_alpha_every: int = 30
+    densify_grad_thresh: float = 0.0002
+    densify_size_thresh: float = 0.01
+    stop_screen_size_at: int = 4000
+    split_screen_size: float = 0.05
+    cull_alpha_thresh: float = 0.1
+    cull_scale_thresh: float = 0.5
+    cull_screen_size: float = 0.15
+    bg_color: tuple = (0.6130, 0.0101, 0.3984)
+    densify: bool = True
+    seed: int = 42
+
+
+class Trainer:
+    """Trains device-resident gaussians on (camera, target) pairs, one view per step."""
+
+    def __init__(self, ctx: Context, dg: DeviceGaussians, cameras, targets, config: TrainConfig):
+        self.ctx, self.dg, self.cams, self.targets, self.cfg = ctx, dg, list(cameras), list(targets), config
+        self.step_count = 0          # current_step
+        self.adam_step_count = 0
+        self.lr = list(LR_INIT)
+        self.rng = np.random.default_rng(config.seed)
+        self.order, self.order_pos = [], 0
+        self.bg = floats(config.bg_color)
+        self.last_hw = (0, 0)
+        self.stop_split_at = config.iterations // 2
+        self.densify_log = []
+        self._struct_cache = {}
+        self.stats_defined = False
+
+    def next_camera(self) -> int:
+        if self.order_pos >= len(self.order):
+            self.order = list(self.rng.permutation(len(self.cams)))
+            self.order_pos = 0
+        i = self.order[self.order_pos]
+        self.order_pos += 1
+        return int(i)
+
+    def _camera(self, i: int, dtu: int):
+        key = (i, dtu)
+        c = self._struct_cache.get(key)
+        if c is None:
+            cam = dict(self.cams[i], dtu=dtu)
+            c = self._struct_cache[key] = make_camera_struct(cam, self.dg.degree)
+        return c
+
+    def step(self, read_loss: bool = False):
+        """One iteration: train step, LR schedule, densification. Returns the loss when
+        read_loss (a sync), else None."""
+        self.step_count += 1
+        step = self.step_count
+        i = self.next_camera()
+        cam = self.cams[i]
+        dtu = min(step // max(self.cfg.sh_degree_interval, 1), self.cfg.sh_degree, self.dg.degree)
+        c = self._camera(i, dtu)
+        self.last_hw = (cam["H"], cam["W"])
+        self.adam_step_count += 1
+        steps, bc2s = adam_coeffs(self.lr, self.adam_step_count)
+        g, a, s = self.dg.struct(), self.dg.adam(steps, bc2s), self.dg.stats()
+        H, W = cam["H"], cam["W"]
+        inv_n = float(np.float32(1.0) / np.float32(H * W * 3))
+        inv_max = float(np.float32(1.0) / np.float32(max(H, W)))
+        if not self.stats_defined:
+            self.dg.reset_stats()
+            self.stats_defined = True
+        loss = C.c_float(0.0)
+        check(lib().msplat_hip_train_step(self.ctx.ptr, C.byref(g), C.byref(c), self.bg,
+                                          C.c_void_p(self.targets[i].ptr), self.cfg.ssim_weight, inv_n, C.byref(a),
+                                          C.byref(s), inv_max, C.c_void_p(self.dg.radii.ptr),
+                                          C.byref(loss) if read_loss else None))
+        self.schedulers_step(step)
+        self.after_train(step)
+        return loss.value if read_loss else None
+
+    def schedulers_step(self, step: int):
+        t = min(max(step / self.cfg.iterations, 0.0), 1.0)
+        self.lr[0] = float(np.exp(np.log(LR_INIT[0]) * (1 - t) + np.log(MEANS_LR_FINAL) * t))
+
+    def after_train(self, step: int):
+        cfg = self.cfg
+        if not cfg.densify or step % cfg.refine_every != 0 or step <= cfg.warmup_length:
+            return
+        reset_interval = cfg.reset_alpha_every * cfg.refine_every
+        do_densify = step < self.stop_split_at and step % reset_interval > len(self.cams) + cfg.refine_every
+        if do_densify:
+            before = self.dg.n
+            self.dg.ensure_capacity(3 * self.dg.n)
+            half_max_dim = 0.5 * float(max(self.last_hw))
+            check_screen = 1 if step < cfg.stop_screen_size_at else 0
+            check_huge = 1 if step > cfg.refine_every * cfg.reset_alpha_every else 0
+            new = self.dg.densify(cfg.densify_grad_thresh, cfg.densify_size_thresh, cfg.split_screen_size,
+                                  check_screen, cfg.cull_alpha_thresh, cfg.cull_scale_thresh, cfg.cull_screen_size,
+                                  check_huge, half_max_dim, (step * 2654435761) % (1 << 32))
+            self.densify_log.append((step, before, new))
+        if step < self.stop_split_at and step % reset_interval == cfg.refine_every:
+            self.dg.opacity_reset(RESET_LOGIT)
+        self.stats_defined = False  # statistics restart from zero (model.cpp:242-244)
