@@ -115,6 +115,7 @@ def main():
     ap.add_argument("--iterations", type=int, default=3000, help="schedule length (densify window, LR decay)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--bwd-ppt", type=int, default=0, help="backward pixels per thread (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -125,6 +126,9 @@ def main():
     from msplat.trainer import TrainConfig, Trainer
 
     ctx = Context(0)
+    if args.bwd_ppt:
+        from msplat._abi import TUNE_BWD_PPT
+        ctx.set_tuning(TUNE_BWD_PPT, args.bwd_ppt)
     t_setup = time.perf_counter()
     dg, gt, cams, targets = build_workload(ctx)
     del gt

@@ -154,6 +154,14 @@ int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *out);
 enum { MSPLAT_FRAME_KEEP_GRADS = 1 };
 int msplat_hip_set_flags(msplat_hip_ctx *ctx, int flags);
 
+/* Kernel variants picked per context (by the autotuner, msplat/autotune.py); no
+ * reference counterpart (the Metal backend hard-codes its launch shapes,
+ * msplat_metal.mm:431-436). MSPLAT_TUNE_BWD_PPT: pixels per thread of the backward
+ * rasterizer, 1, 2 or 4. */
+enum { MSPLAT_TUNE_BWD_PPT = 0, MSPLAT_NUM_TUNABLES = 1 };
+int msplat_hip_set_tuning(msplat_hip_ctx *ctx, int key, int value);
+int msplat_hip_get_tuning(msplat_hip_ctx *ctx, int key, int *value);
+
 /* Stage-wise entry points over the frame cache (test hooks, like the reference's
  * _core._gpu_* hooks, python/bindings.cpp:655-738). Each runs one stage of
  * msplat_render / msplat_train_step on the context's stream. */

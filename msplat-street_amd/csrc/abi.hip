@@ -57,9 +57,10 @@ struct msplat_hip_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int flags = 0;
+    int tune[MSPLAT_NUM_TUNABLES] = {4};
     // per gaussian
     Buf splat_a, splat_b, splat_blue, depths, aabb, rect, radii, count, offsets, block_sums;
-    Buf v_xy, v_conic, v_rgb, v_opac;
+    Buf v_xy, v_conic, v_rgb, v_opac, sh_scratch, big;
     // per intersection
     Buf keys0, keys1, vals0, vals1, slot_a, slot_b, slot_c, hist, digit_total;
     // per tile / pixel
@@ -152,7 +153,7 @@ struct msplat_hip_ctx {
     ~msplat_hip_ctx() {
         if (stream) (void)hipStreamSynchronize(stream);
         Buf *all[] = {&splat_a, &splat_b, &splat_blue, &depths, &aabb, &rect, &radii, &count, &offsets, &block_sums,
-                      &v_xy, &v_conic, &v_rgb, &v_opac, &keys0, &keys1, &vals0, &vals1, &slot_a, &slot_b, &slot_c,
+                      &v_xy, &v_conic, &v_rgb, &v_opac, &sh_scratch, &big, &keys0, &keys1, &vals0, &vals1, &slot_a, &slot_b, &slot_c,
                       &hist, &digit_total, &tile_start, &image, &final_T, &final_idx, &v_image, &pix_loss,
                       &row_loss, &scalars, &grad_scratch, &aux};
         for (Buf *b : all) release(*b);
@@ -163,6 +164,9 @@ struct msplat_hip_ctx {
     }
 
     uint64_t *d_total64() { return (uint64_t *)scalars.p; }
+    uint32_t *d_big_count() { return (uint32_t *)((char *)scalars.p + 32); }
+    uint32_t *big_list() { return (uint32_t *)big.p; }
+    float4 *big_a() { return (float4 *)((char *)big.p + 4 * (size_t)(n > 0 ? n : 1) + 12); }
     uint32_t *d_total32() { return (uint32_t *)((char *)scalars.p + 8); }
     double *d_loss_sum() { return (double *)((char *)scalars.p + 16); }
     float *d_loss() { return (float *)((char *)scalars.p + 24); }
@@ -214,6 +218,7 @@ int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_
     CHECKED(ctx->grow(ctx->offsets, 4 * N));
     CHECKED(ctx->grow(ctx->block_sums, 4 * (size_t)(nblocks + 1)));
     CHECKED(ctx->grow(ctx->scalars, 64));
+    CHECKED(ctx->grow(ctx->big, 40 * N + 64));  // list (4) + slot sums (16 + 16 + 4) + alignment
     ctx->n = n;
     ctx->width = cam->width;
     ctx->height = cam->height;
@@ -249,7 +254,8 @@ int do_bin(msplat_hip_ctx *ctx) {
     const int n = ctx->n;
     const int nblocks = (n + 255) / 256;
     if (!ctx->h_total) HIPCHECK(hipHostMalloc((void **)&ctx->h_total, 64, hipHostMallocDefault));
-    launch_scan_blocks((uint32_t *)ctx->block_sums.p, nblocks, ctx->d_total64(), ctx->d_total32(), ctx->stream);
+    launch_scan_blocks((uint32_t *)ctx->block_sums.p, nblocks, ctx->d_total64(), ctx->d_total32(), ctx->stream,
+                       ctx->d_big_count());
     ctx->end();
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(ctx->h_total, ctx->d_total64(), 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -278,6 +284,8 @@ int do_bin(msplat_hip_ctx *ctx) {
     k.offsets = (uint32_t *)ctx->offsets.p;
     k.keys = (uint64_t *)ctx->keys0.p;
     k.vals = (int32_t *)ctx->vals0.p;
+    k.big_list = ctx->big_list();
+    k.big_count = ctx->d_big_count();
     ctx->begin(1);
     launch_keygen(k, ctx->stream);
     uint64_t *kin = (uint64_t *)ctx->keys0.p, *kout = (uint64_t *)ctx->keys1.p;
@@ -368,7 +376,7 @@ int do_raster_backward(msplat_hip_ctx *ctx) {
     CHECKED(ctx->grow(ctx->slot_b, 16 * Mc));
     CHECKED(ctx->grow(ctx->slot_c, 4 * Mc));
     ctx->begin(4);
-    launch_raster_backward(raster_args(ctx), ctx->stream);
+    launch_raster_backward(raster_args(ctx), ctx->tune[MSPLAT_TUNE_BWD_PPT], ctx->stream);
     ctx->end();
     HIPCHECK(hipGetLastError());
     return MSPLAT_OK;
@@ -393,6 +401,18 @@ int backward_args(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const mspl
     b.slot_a = (const float4 *)ctx->slot_a.p;
     b.slot_b = (const float4 *)ctx->slot_b.p;
     b.slot_c = (const float *)ctx->slot_c.p;
+    {
+        const size_t N = (size_t)(b.n > 0 ? b.n : 1);
+        CHECKED(ctx->grow(ctx->sh_scratch, 32 * N));
+        b.sh_dir = (float4 *)ctx->sh_scratch.p;
+        b.sh_vrgb = b.sh_dir + N;
+        b.big_list = ctx->big_list();
+        b.big_count = ctx->d_big_count();
+        float4 *ba = (float4 *)(((uintptr_t)ctx->big_a() + 15) & ~(uintptr_t)15);
+        b.big_a = ba;
+        b.big_b = ba + N;
+        b.big_c = (float *)(ba + 2 * N);
+    }
     if (ctx->flags & MSPLAT_FRAME_KEEP_GRADS) {
         const size_t N = (size_t)(b.n > 0 ? b.n : 1);
         CHECKED(ctx->grow(ctx->v_xy, 8 * N));
@@ -855,6 +875,24 @@ int msplat_hip_pack_rgba8(msplat_hip_ctx *ctx, const float *img, uint32_t n, uin
     if (!img || !out) return set_error(MSPLAT_ERR_ARG, "img or out is NULL");
     launch_pack_rgba8(img, n, out, ctx->stream);
     HIPCHECK(hipGetLastError());
+    return MSPLAT_OK;
+}
+
+int msplat_hip_set_tuning(msplat_hip_ctx *ctx, int key, int value) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    if (key == MSPLAT_TUNE_BWD_PPT) {
+        if (value != 1 && value != 2 && value != 4)
+            return set_error(MSPLAT_ERR_ARG, "backward pixels per thread must be 1, 2 or 4");
+        ctx->tune[key] = value;
+        return MSPLAT_OK;
+    }
+    return set_error(MSPLAT_ERR_ARG, "unknown tunable %d", key);
+}
+
+int msplat_hip_get_tuning(msplat_hip_ctx *ctx, int key, int *value) {
+    if (!ctx || !value) return set_error(MSPLAT_ERR_ARG, "ctx or value is NULL");
+    if (key < 0 || key >= MSPLAT_NUM_TUNABLES) return set_error(MSPLAT_ERR_ARG, "unknown tunable %d", key);
+    *value = ctx->tune[key];
     return MSPLAT_OK;
 }
 

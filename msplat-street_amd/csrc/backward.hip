@@ -8,6 +8,7 @@
 //   BWD_GRADS — the parameter gradients themselves (autograd backend).
 // The slot sum runs over the gaussian's intersections in tile order, as the reference's
 // reduce_isect_grads (cpu:1034-1064). Built with -ffp-contract=off.
+#include "common.h"
 #include "kernels.h"
 #include "proj_math.h"
 
@@ -28,7 +29,12 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
     if (g >= a.n) return;
     const bool vis = a.radii[g] > 0;
     float vxy[2] = {0.f, 0.f}, vcn[3] = {0.f, 0.f, 0.f}, vrgb[3] = {0.f, 0.f, 0.f}, vop = 0.f;
-    if (vis) {
+    if (vis && a.count[g] > BIG_SLOTS) {  // reduced by big_reduce_kernel
+        const float4 A = a.big_a[g], B = a.big_b[g];
+        vxy[0] = A.x; vxy[1] = A.y; vcn[0] = A.z; vcn[1] = A.w;
+        vcn[2] = B.x; vrgb[0] = B.y; vrgb[1] = B.z; vrgb[2] = B.w;
+        vop = a.big_c[g];
+    } else if (vis) {
         const uint32_t o = a.offsets[g], c = a.count[g];
         for (uint32_t p = o; p < o + c; p++) {
             const float4 A = a.slot_a[p], B = a.slot_b[p];
@@ -43,8 +49,6 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
         a.v_opac[g] = vop;
     }
     const msplat_hip_camera &cam = a.cam;
-    const uint32_t nb = sh_bases(cam.degree);
-    const size_t rest_base = (size_t)(nb - 1) * 3 * g;
     float vm[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
     float dirv[3] = {0.f, 0.f, 0.f};
     if (vis) {
@@ -69,8 +73,9 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
         vs[0] *= es[0]; vs[1] *= es[1]; vs[2] *= es[2];
         view_direction(p, cam.cam_pos, dirv);
     }
-    float basis[15];
-    const int nbasis = vis ? sh_basis(min(cam.degrees_to_use, 3u), dirv, basis) : 0;
+    // the higher SH bands are handled per coefficient by sh_rest_kernel
+    a.sh_dir[g] = make_float4(dirv[0], dirv[1], dirv[2], vis ? 1.f : 0.f);
+    a.sh_vrgb[g] = make_float4(vrgb[0], vrgb[1], vrgb[2], 0.f);
 
     if (MODE == BWD_GRADS) {
         for (int i = 0; i < 3; i++) {
@@ -79,24 +84,16 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
             a.g_dc[3 * g + i] = vis ? SH_C0 * vrgb[i] : 0.f;
         }
         for (int i = 0; i < 4; i++) a.g_quats[4 * g + i] = vq[i];
-        for (uint32_t b = 0; b + 1 < nb; b++)
-            for (int c = 0; c < 3; c++)
-                a.g_rest[rest_base + 3 * b + c] = (int)b < nbasis ? basis[b] * vrgb[c] : 0.f;
         a.g_opac[g] = vop;
         return;
     }
 
     const float b1 = a.beta1, b2 = a.beta2, eps = a.eps;
-    if (vis) {  // SH groups: visible gaussians only
+    if (vis) {  // SH DC: visible gaussians only
         for (int c = 0; c < 3; c++) {
             const size_t i = 3 * (size_t)g + c;
             adam_step(a.dc[i], a.m1[3][i], a.m2[3][i], SH_C0 * vrgb[c], a.step[3], b1, b2, a.bc2[3], eps);
         }
-        for (int b = 0; b < nbasis; b++)
-            for (int c = 0; c < 3; c++) {
-                const size_t i = rest_base + 3 * b + c;
-                adam_step(a.rest[i], a.m1[4][i], a.m2[4][i], basis[b] * vrgb[c], a.step[4], b1, b2, a.bc2[4], eps);
-            }
     }
     // dense groups: means, scales, quats, opacity
     for (int i = 0; i < 3; i++) {
@@ -119,13 +116,82 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
     }
 }
 
+// Slot sums of the gaussians with more than BIG_SLOTS intersections, one wave each
+// (grid-stride over the device-side list; every wave exits when the list is done).
+__global__ __launch_bounds__(256) void big_reduce_kernel(BackwardArgs a) {
+#pragma clang fp contract(fast)
+    const uint32_t nbig = *a.big_count;
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * 256 + threadIdx.x) >> 6, nwaves = (gridDim.x * 256) >> 6;
+    for (uint32_t i = wave; i < nbig; i += nwaves) {
+        const uint32_t g = a.big_list[i];
+        const uint32_t o = a.offsets[g], c = a.count[g];
+        float v[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (uint32_t p = o + lane; p < o + c; p += 64) {
+            const float4 A = a.slot_a[p], B = a.slot_b[p];
+            v[0] += A.x; v[1] += A.y; v[2] += A.z; v[3] += A.w;
+            v[4] += B.x; v[5] += B.y; v[6] += B.z; v[7] += B.w;
+            v[8] += a.slot_c[p];
+        }
+        float r[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) r[q] = wave_sum(v[q]);
+        if (lane == 0) {
+            a.big_a[g] = make_float4(r[0], r[1], r[2], r[3]);
+            a.big_b[g] = make_float4(r[4], r[5], r[6], r[7]);
+            a.big_c[g] = r[8];
+        }
+    }
+}
+
+// SH bands 1..3, one thread per stored coefficient (gaussian g, basis b, channel c), so
+// that the (N, bases-1, 3) coefficients and moments are read and written coalesced.
+// Adam mode updates coefficients of visible gaussians in the bands in use (the
+// reference's fused SH Adam, cpu:1118-1137); gradient mode writes every coefficient's
+// gradient (zero outside the bands in use).
+template <int MODE>
+__global__ __launch_bounds__(256) void sh_rest_kernel(BackwardArgs a) {
+    const uint32_t nb = sh_bases(a.cam.degree);
+    const uint32_t stride = (nb - 1) * 3;
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (stride == 0 || e >= (size_t)a.n * stride) return;
+    const int g = (int)(e / stride);
+    const int j = (int)(e % stride), b = j / 3, c = j % 3;
+    const float4 D = a.sh_dir[g];
+    const int dtu = (int)min(a.cam.degrees_to_use, 3u);
+    const int nbasis = dtu >= 3 ? 15 : dtu == 2 ? 8 : dtu == 1 ? 3 : 0;
+    const bool active = D.w > 0.f && b < nbasis;
+    if (MODE == BWD_GRADS && !active) {
+        a.g_rest[e] = 0.f;
+        return;
+    }
+    if (!active) return;
+    const float d[3] = {D.x, D.y, D.z};
+    float basis[15];
+    sh_basis((uint32_t)dtu, d, basis);
+    const float4 V = a.sh_vrgb[g];
+    const float vc = c == 0 ? V.x : c == 1 ? V.y : V.z;
+    const float grad = basis[b] * vc;
+    if (MODE == BWD_GRADS) {
+        a.g_rest[e] = grad;
+        return;
+    }
+    adam_step(a.rest[e], a.m1[4][e], a.m2[4][e], grad, a.step[4], a.beta1, a.beta2, a.bc2[4], a.eps);
+}
+
 void launch_backward(const BackwardArgs &a, BackwardMode mode, hipStream_t s) {
     const int nb = (a.n + 255) / 256;
     if (nb == 0) return;
-    if (mode == BWD_ADAM)
+    const size_t ne = (size_t)a.n * (sh_bases(a.cam.degree) - 1) * 3;
+    const int nbe = (int)((ne + 255) / 256);
+    big_reduce_kernel<<<1024, 256, 0, s>>>(a);
+    if (mode == BWD_ADAM) {
         backward_kernel<BWD_ADAM><<<nb, 256, 0, s>>>(a);
-    else
+        if (nbe) sh_rest_kernel<BWD_ADAM><<<nbe, 256, 0, s>>>(a);
+    } else {
         backward_kernel<BWD_GRADS><<<nb, 256, 0, s>>>(a);
+        if (nbe) sh_rest_kernel<BWD_GRADS><<<nbe, 256, 0, s>>>(a);
+    }
 }
 
 }  // namespace msplat

@@ -110,10 +110,13 @@ __global__ __launch_bounds__(256) void rect_count_kernel(int n, const float4 *sp
 // Exclusive scan of the per-block intersection counts (one block, sequential chunks);
 // writes the block bases and the total (u64, so that > 2^31 is detected).
 __global__ __launch_bounds__(256) void scan_blocks_kernel(uint32_t *sums, int nblocks, uint64_t *total,
-                                                          uint32_t *total32) {
+                                                          uint32_t *total32, uint32_t *zero_also) {
     __shared__ uint32_t lds4[4];
     __shared__ uint64_t carry;
-    if (threadIdx.x == 0) carry = 0;
+    if (threadIdx.x == 0) {
+        carry = 0;
+        if (zero_also) *zero_also = 0;
+    }
     __syncthreads();
     for (int base = 0; base < nblocks; base += 256) {
         const int i = base + threadIdx.x;
@@ -131,25 +134,52 @@ __global__ __launch_bounds__(256) void scan_blocks_kernel(uint32_t *sums, int nb
     }
 }
 
-// Key emission: gaussian g writes its tiles in row-major order at offsets[g].
+// Key emission: gaussian g writes its tiles in row-major order at offsets[g]. The
+// workgroup emits the keys of its 256 gaussians together: intersection i of the block
+// finds its gaussian by binary search over the block's offsets (LDS), so consecutive
+// threads write consecutive keys.
 __global__ __launch_bounds__(256) void keygen_kernel(KeygenArgs a) {
     __shared__ uint32_t lds4[4];
+    __shared__ uint32_t s_off[257];
+    __shared__ uint32_t s_rect[256][2];
+    __shared__ uint32_t s_depth[256];
+    __shared__ uint32_t s_bigbase;
     const int g = blockIdx.x * 256 + threadIdx.x;
     const uint32_t c = g < a.n ? a.count[g] : 0;
     uint32_t total;
-    const uint32_t off = a.block_bases[blockIdx.x] + block_scan_u32(c, total, lds4);
-    if (g >= a.n) return;
-    a.offsets[g] = off;
-    if (c == 0) return;
-    const uint2 r = a.rect[g];
-    const int x0 = r.x & 0xFFFF, x1 = r.x >> 16, y0 = r.y & 0xFFFF, y1 = r.y >> 16;
-    const uint64_t dbits = (uint64_t)__float_as_uint(a.depths[g]);
-    uint32_t k = off;
-    for (int y = y0; y < y1; y++)
-        for (int x = x0; x < x1; x++, k++) {
-            a.keys[k] = ((uint64_t)(uint32_t)(y * a.tiles_x + x) << 32) | dbits;
-            a.vals[k] = g;
+    const uint32_t ex = block_scan_u32(c, total, lds4);
+    const uint32_t base = a.block_bases[blockIdx.x];
+    if (g < a.n) a.offsets[g] = base + ex;
+    // gaussians with many intersections join the big list: one atomic per workgroup
+    uint32_t nbig;
+    const uint32_t bex = block_scan_u32(c > BIG_SLOTS ? 1u : 0u, nbig, lds4);
+    if (threadIdx.x == 0 && nbig) s_bigbase = atomicAdd(a.big_count, nbig);
+    __syncthreads();
+    if (c > BIG_SLOTS) a.big_list[s_bigbase + bex] = (uint32_t)g;
+    s_off[threadIdx.x] = ex;
+    if (threadIdx.x == 0) s_off[256] = total;
+    if (c) {
+        const uint2 r = a.rect[g];
+        s_rect[threadIdx.x][0] = r.x;
+        s_rect[threadIdx.x][1] = r.y;
+        s_depth[threadIdx.x] = __float_as_uint(a.depths[g]);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < total; i += 256) {
+        // the last gaussian j of the block with s_off[j] <= i
+        int lo = 0, hi = 255;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= i) lo = mid;
+            else hi = mid - 1;
         }
+        const uint32_t rx = s_rect[lo][0], ry = s_rect[lo][1];
+        const int x0 = rx & 0xFFFF, x1 = rx >> 16, y0 = ry & 0xFFFF;
+        const uint32_t local = i - s_off[lo], w = (uint32_t)(x1 - x0);
+        const int ty = y0 + (int)(local / w), tx = x0 + (int)(local % w);
+        a.keys[base + i] = ((uint64_t)(uint32_t)(ty * a.tiles_x + tx) << 32) | (uint64_t)s_depth[lo];
+        a.vals[base + i] = blockIdx.x * 256 + lo;
+    }
 }
 
 // ── stable LSD radix sort, 8-bit digits ──────────────────────────────────────
@@ -275,8 +305,9 @@ void launch_rect_count(int n, const float4 *splat_a, const float2 *aabb, const i
     if (nb > 0) rect_count_kernel<<<nb, 256, 0, s>>>(n, splat_a, aabb, radii, tx, ty, rect, count, block_sums);
 }
 
-void launch_scan_blocks(uint32_t *sums, int nblocks, uint64_t *total, uint32_t *total32, hipStream_t s) {
-    scan_blocks_kernel<<<1, 256, 0, s>>>(sums, nblocks, total, total32);
+void launch_scan_blocks(uint32_t *sums, int nblocks, uint64_t *total, uint32_t *total32, hipStream_t s,
+                        uint32_t *zero_also) {
+    scan_blocks_kernel<<<1, 256, 0, s>>>(sums, nblocks, total, total32, zero_also);
 }
 
 size_t radix_hist_words(uint32_t n_max) { return (size_t)256 * ((n_max + RS_ITEMS - 1) / RS_ITEMS + 1); }

@@ -75,6 +75,22 @@ __device__ inline float wave_sum(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// Sum over the 16 lanes of each DPP row; every lane of the row ends with the row total.
+__device__ inline float row_sum16(float v) {
+    v += dpp_f<0xb1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f<0x4e>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f<0x141>(v);  // row_half_mirror
+    v += dpp_f<0x140>(v);  // row_mirror
+    return v;
+}
+
+// Completes a row_sum16: lane 63 ends with the total of the four rows' sums.
+__device__ inline float rows_to_lane63(float v) {
+    v += dpp_f<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 __device__ inline uint64_t ballot(bool p) { return __ballot(p); }
 
 // ── counter-based RNG: PCG hash (Jarzynski & Olano 2020), 24-bit uniforms in (0, 1),

@@ -23,6 +23,8 @@ struct ProjectArgs {
     uint32_t *count, *block_sums;
 };
 
+constexpr uint32_t BIG_SLOTS = 32;  // gaussians with more intersections reduce their slots per wave
+
 struct KeygenArgs {
     int n, tiles_x;
     const uint32_t *count, *block_bases;
@@ -31,6 +33,7 @@ struct KeygenArgs {
     uint32_t *offsets;
     uint64_t *keys;
     int32_t *vals;
+    uint32_t *big_list, *big_count;  // gaussians with count > BIG_SLOTS (big_count zeroed by the scan)
 };
 
 struct RasterArgs {
@@ -82,6 +85,12 @@ struct BackwardArgs {
     float *g_means, *g_scales, *g_quats, *g_dc, *g_rest, *g_opac;
     // optional per-gaussian reduced gradients (test hook)
     float *v_xy, *v_conic, *v_rgb, *v_opac;
+    // per-gaussian view direction (w = visible) and colour gradient, for the SH bands
+    float4 *sh_dir, *sh_vrgb;
+    // slot sums of the gaussians with many intersections, reduced one wave each
+    const uint32_t *big_list, *big_count;
+    float4 *big_a, *big_b;
+    float *big_c;
 };
 
 struct DensifyArgs {
@@ -122,7 +131,8 @@ void launch_init_gaussians(const float *md, const uint8_t *rgb, uint32_t n, uint
 void launch_project(const ProjectArgs &a, hipStream_t s);
 void launch_rect_count(int n, const float4 *splat_a, const float2 *aabb, const int32_t *radii, int tx, int ty,
                        uint2 *rect, uint32_t *count, uint32_t *block_sums, hipStream_t s);
-void launch_scan_blocks(uint32_t *sums, int nblocks, uint64_t *total, uint32_t *total32, hipStream_t s);
+void launch_scan_blocks(uint32_t *sums, int nblocks, uint64_t *total, uint32_t *total32, hipStream_t s,
+                        uint32_t *zero_also = nullptr);
 void launch_keygen(const KeygenArgs &a, hipStream_t s);
 void launch_radix_pass(const uint64_t *kin, const int32_t *vin, uint64_t *kout, int32_t *vout, const uint32_t *n_dev,
                        uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s);
@@ -130,7 +140,7 @@ size_t radix_hist_words(uint32_t n_max);
 void launch_tile_ranges(const uint64_t *keys, const uint32_t *n_dev, uint32_t n_max, int ntiles, int32_t *tile_start,
                         hipStream_t s);
 void launch_raster_forward(const RasterArgs &a, hipStream_t s);
-void launch_raster_backward(const RasterArgs &a, hipStream_t s);
+void launch_raster_backward(const RasterArgs &a, int pixels_per_thread, hipStream_t s);
 void launch_loss(const LossArgs &a, hipStream_t s);
 void launch_backward(const BackwardArgs &a, BackwardMode mode, hipStream_t s);
 

@@ -151,29 +151,49 @@ __global__ __launch_bounds__(256) void loss_kernel(LossArgs a) {
     }
 }
 
-// Per-row double sums of the per-pixel loss, in x order (one lane per row)
-__global__ __launch_bounds__(64) void loss_rows_kernel(const float *pix, int W, int H, double *rows) {
-    const int y = blockIdx.x * 64 + threadIdx.x;
-    if (y >= H) return;
+// Per-row double sums of the per-pixel loss in x order: the row is staged in LDS with
+// coalesced loads, then one lane adds it up sequentially (the reference's order).
+constexpr int ROW_MAX = 4096;
+
+__global__ __launch_bounds__(256) void loss_rows_kernel(const float *pix, int W, int H, double *rows) {
+    __shared__ float r[ROW_MAX];
+    const int y = blockIdx.x;
     const float *p = pix + (size_t)y * W;
     double s = 0.0;
-    for (int x = 0; x < W; x++) s += (double)p[x];
-    rows[y] = s;
+    for (int base = 0; base < W; base += ROW_MAX) {
+        const int n = min(ROW_MAX, W - base);
+        __syncthreads();
+        for (int x = threadIdx.x; x < n; x += 256) r[x] = p[base + x];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int x = 0; x < n; x++) s += (double)r[x];
+    }
+    if (threadIdx.x == 0) rows[y] = s;
 }
 
-__global__ void loss_total_kernel(const double *rows, int H, int W, double *sum, float *mean) {
-    if (threadIdx.x != 0) return;
+// Sum of the row sums in y order (staged in LDS, one lane adds)
+__global__ __launch_bounds__(256) void loss_total_kernel(const double *rows, int H, int W, double *sum, float *mean) {
+    __shared__ double r[2048];
     double s = 0.0;
-    for (int y = 0; y < H; y++) s += rows[y];
-    *sum = s;
-    *mean = (float)(s / (double)((size_t)W * H));
+    for (int base = 0; base < H; base += 2048) {
+        const int n = min(2048, H - base);
+        __syncthreads();
+        for (int y = threadIdx.x; y < n; y += 256) r[y] = rows[base + y];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int y = 0; y < n; y++) s += r[y];
+    }
+    if (threadIdx.x == 0) {
+        *sum = s;
+        *mean = (float)(s / (double)((size_t)W * H));
+    }
 }
 
 void launch_loss(const LossArgs &a, hipStream_t s) {
     dim3 grid((a.width + LT_W - 1) / LT_W, (a.height + LT_H - 1) / LT_H);
     loss_kernel<<<grid, 256, 0, s>>>(a);
-    loss_rows_kernel<<<(a.height + 63) / 64, 64, 0, s>>>(a.pix_loss, a.width, a.height, a.row_loss);
-    loss_total_kernel<<<1, 64, 0, s>>>(a.row_loss, a.height, a.width, a.loss_sum, a.loss_mean);
+    loss_rows_kernel<<<a.height, 256, 0, s>>>(a.pix_loss, a.width, a.height, a.row_loss);
+    loss_total_kernel<<<1, 256, 0, s>>>(a.row_loss, a.height, a.width, a.loss_sum, a.loss_mean);
 }
 
 }  // namespace msplat

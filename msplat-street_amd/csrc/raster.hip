@@ -50,27 +50,27 @@ __global__ __launch_bounds__(256) void raster_forward_kernel(RasterArgs a) {
             lc[threadIdx.x] = a.splat_blue[g];
         }
         __syncthreads();
-        if (!done) {
+        if (__ballot(!done) != 0) {
             const int n = min(FWD_BATCH, e - base);
             for (int j = 0; j < n; j++) {
+                // predicated compositing step: skip (out of range / too faint), accept, or
+                // terminate (transmittance would drop to T_MIN: last = k - 1, as the CPU)
                 const float4 A = la[j], B = lb[j];
                 const float dx = A.x - fx, dy = A.y - fy;
                 const float sig = 0.5f * (A.z * dx * dx + B.x * dy * dy) + A.w * dx * dy;
-                if (sig < 0.f || sig >= SIGMA_MAX) continue;
                 const float alpha = fminf(ALPHA_MAX, B.y * __expf(-sig));
-                if (alpha < ALPHA_MIN) continue;
+                const bool valid = !done && sig >= 0.f && sig < SIGMA_MAX && alpha >= ALPHA_MIN;
                 const float nT = T * (1.f - alpha);
-                if (nT <= T_MIN) {
-                    last = base + j - 1;
-                    done = true;
-                    break;
-                }
-                const float vis = alpha * T;
+                const bool term = valid && nT <= T_MIN;
+                const bool accept = valid && !term;
+                const float vis = accept ? alpha * T : 0.f;
                 cr += fmaxf(B.z + 0.5f, 0.f) * vis;
                 cg += fmaxf(B.w + 0.5f, 0.f) * vis;
                 cb += fmaxf(lc[j] + 0.5f, 0.f) * vis;
-                T = nT;
-                last = base + j;
+                T = accept ? nT : T;
+                last = term ? base + j - 1 : (accept ? base + j : last);
+                done = done || term;
+                if ((j & 7) == 7 && __ballot(!done) == 0) break;
             }
         }
     }
@@ -86,44 +86,69 @@ __global__ __launch_bounds__(256) void raster_forward_kernel(RasterArgs a) {
 
 // rasterize_backward (core/cpu/msplat_cpu.cpp:959-1031), slot values per intersection:
 // a = (v_x, v_y, v_conic.x, v_conic.y), b = (v_conic.z, v_r, v_g, v_b), c = v_logit.
-__global__ __launch_bounds__(256) void raster_backward_kernel(RasterArgs a) {
+//
+// PPT pixels per thread: 4/PPT waves per tile, wave w owning rows [4 PPT w, 4 PPT (w+1))
+// and lane l the pixels (l % 16, 4 PPT w + l / 16 + 4 k), k < PPT.
+//
+// Per pixel the back-to-front recursion keeps T and one running scalar
+//   S = <buffer + T_final bg, v_out>,  so  v_alpha = T <rgb, v_out> - S / (1 - alpha),
+// and the per-gaussian gradients are assembled after the reduction from sufficient
+// statistics of v_sigma = -alpha v_alpha:
+//   v_conic = 1/2 (sum vs dx^2, sum vs dx dy, sum vs dy^2),
+//   v_xy = (a sum vs dx + b sum vs dy, b sum vs dx + c sum vs dy),
+//   v_logit = -(1 - o) sum vs, v_rgb = sum fac v_out (channels with raw + 0.5 >= 0).
+// A lane first adds its PPT pixels; each value is then summed over the wave with DPP
+// (rows, then row broadcasts into lane 63), and lane 63 writes the wave's total to LDS,
+// to be added over the waves in fixed order when the batch's slots are written. One
+// partial per wave keeps the LDS footprint small enough for several waves per SIMD.
+constexpr int NQ = 9;  // sum vs, vs dx, vs dy, vs dx^2, vs dx dy, vs dy^2, fac vr, fac vg, fac vb
+
+template <int PPT>
+__global__ __launch_bounds__(256 / PPT) void raster_backward_kernel(RasterArgs a) {
 #pragma clang fp contract(fast)
-    __shared__ float4 la[BWD_BATCH], lb[BWD_BATCH];
-    __shared__ float lc[BWD_BATCH];
-    __shared__ uint32_t lslot[BWD_BATCH];
-    __shared__ float part[4][9][BWD_BATCH];
-    __shared__ int smax[4];
-    int tx, ty, px, py;
+    constexpr int NT = 256 / PPT, NW = NT / 64, BATCH = 128;
+    __shared__ float4 la[BATCH], lb[BATCH];
+    __shared__ float lc[BATCH];
+    __shared__ uint32_t lslot[BATCH];
+    __shared__ float part[NW][NQ][BATCH];
+    __shared__ int smax[NW];
     const int tile = blockIdx.x;
-    tile_pixel(tile, a.tiles_x, tx, ty, px, py);
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const bool inside = px < (int)a.width && py < (int)a.height;
     const int s = a.tile_start[tile], e = a.tile_start[tile + 1];
     if (s == e) return;
-    const size_t pid = (size_t)py * a.width + px;
-    float Tf = 0.f, vr = 0.f, vg = 0.f, vb = 0.f;
-    int last = s - 1;
-    if (inside) {
-        Tf = a.final_T[pid];
-        last = min(a.final_idx[pid], e - 1);
-        vr = a.v_image[3 * pid];
-        vg = a.v_image[3 * pid + 1];
-        vb = a.v_image[3 * pid + 2];
+    const int px = tx * TILE + (lane & 15);
+    float fy[PPT], T[PPT], S[PPT], vr[PPT], vg[PPT], vb[PPT];
+    int last[PPT];
+    int m = s - 1;
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        const int py = ty * TILE + 4 * PPT * w + (lane >> 4) + 4 * k;
+        fy[k] = (float)py;
+        T[k] = 0.f; S[k] = 0.f; vr[k] = 0.f; vg[k] = 0.f; vb[k] = 0.f;
+        last[k] = s - 1;
+        if (px < (int)a.width && py < (int)a.height) {
+            const size_t pid = (size_t)py * a.width + px;
+            T[k] = a.final_T[pid];
+            last[k] = min(a.final_idx[pid], e - 1);
+            vr[k] = a.v_image[3 * pid];
+            vg[k] = a.v_image[3 * pid + 1];
+            vb[k] = a.v_image[3 * pid + 2];
+            S[k] = T[k] * (a.bg[0] * vr[k] + a.bg[1] * vg[k] + a.bg[2] * vb[k]);
+        }
+        m = max(m, last[k]);
     }
     // the block's furthest contributing intersection: nothing behind it is visited
-    {
-        int m = last;
-        for (int d = 32; d >= 1; d >>= 1) m = max(m, __shfl_xor(m, d));
-        if (lane == 0) smax[w] = m;
-    }
+    for (int d = 32; d >= 1; d >>= 1) m = max(m, __shfl_xor(m, d));
+    if (lane == 0) smax[w] = m;
     __syncthreads();
-    const int kmax = max(max(smax[0], smax[1]), max(smax[2], smax[3]));
-    const float tbr = Tf * a.bg[0], tbg = Tf * a.bg[1], tbb = Tf * a.bg[2];
-    const float fx = (float)px, fy = (float)py;
-    float T = Tf, br = 0.f, bgc = 0.f, bb = 0.f;
+    int kmax = smax[0];
+#pragma unroll
+    for (int i = 1; i < NW; i++) kmax = max(kmax, smax[i]);
+    const float fx = (float)px;
 
     // slots of intersections behind every pixel's last contributor are zero
-    for (int k = kmax + 1 + threadIdx.x; k < e; k += 256) {
+    for (int k = kmax + 1 + threadIdx.x; k < e; k += NT) {
         const int g = a.gid[k];
         const uint2 r = a.rect[g];
         const int x0 = r.x & 0xFFFF, x1 = r.x >> 16, y0 = r.y & 0xFFFF;
@@ -133,80 +158,85 @@ __global__ __launch_bounds__(256) void raster_backward_kernel(RasterArgs a) {
         a.slot_c[p] = 0.f;
     }
 
-    for (int end = kmax + 1; end > s; end -= BWD_BATCH) {
-        const int start = max(s, end - BWD_BATCH);
+    for (int end = kmax + 1; end > s; end -= BATCH) {
+        const int start = max(s, end - BATCH);
         const int n = end - start;
         __syncthreads();
-        if (threadIdx.x < n) {
-            const int k = start + threadIdx.x;
-            const int g = a.gid[k];
-            la[threadIdx.x] = a.splat_a[g];
-            lb[threadIdx.x] = a.splat_b[g];
-            lc[threadIdx.x] = a.splat_blue[g];
+        for (int i = threadIdx.x; i < n; i += NT) {
+            const int g = a.gid[start + i];
+            la[i] = a.splat_a[g];
+            lb[i] = a.splat_b[g];
+            lc[i] = a.splat_blue[g];
             const uint2 r = a.rect[g];
             const int x0 = r.x & 0xFFFF, x1 = r.x >> 16, y0 = r.y & 0xFFFF;
-            lslot[threadIdx.x] = a.offsets[g] + (uint32_t)((ty - y0) * (x1 - x0) + (tx - x0));
+            lslot[i] = a.offsets[g] + (uint32_t)((ty - y0) * (x1 - x0) + (tx - x0));
         }
-        for (int i = threadIdx.x; i < 4 * 9 * BWD_BATCH; i += 256) (&part[0][0][0])[i] = 0.f;
+        for (int i = threadIdx.x; i < NW * NQ * BATCH; i += NT) (&part[0][0][0])[i] = 0.f;
         __syncthreads();
         for (int j = n - 1; j >= 0; j--) {
-            const int k = start + j;
-            float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f, g6 = 0.f, g7 = 0.f, g8 = 0.f;
+            const int kk = start + j;
+            const float4 A = la[j], B = lb[j];
+            const float cr = fmaxf(B.z + 0.5f, 0.f), cg = fmaxf(B.w + 0.5f, 0.f), cbl = fmaxf(lc[j] + 0.5f, 0.f);
+            float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f, q4 = 0.f, q5 = 0.f, q6 = 0.f, q7 = 0.f, q8 = 0.f;
             bool contrib = false;
-            if (k <= last) {
-                const float4 A = la[j], B = lb[j];
-                const float dx = A.x - fx, dy = A.y - fy;
+#pragma unroll
+            for (int k = 0; k < PPT; k++) {
+                if (__ballot(kk <= last[k]) == 0) continue;  // the whole wave is past this pixel row
+                const float dx = A.x - fx, dy = A.y - fy[k];
                 const float sig = 0.5f * (A.z * dx * dx + B.x * dy * dy) + A.w * dx * dy;
-                if (sig >= 0.f && sig < SIGMA_MAX) {
-                    const float alpha = fminf(ALPHA_MAX, B.y * __expf(-sig));
-                    if (alpha >= ALPHA_MIN) {
-                        contrib = true;
-                        const float ra = 1.f / (1.f - alpha);
-                        T *= ra;
-                        const float fac = alpha * T;
-                        const float rr = B.z + 0.5f, rg = B.w + 0.5f, rb = lc[j] + 0.5f;
-                        const float cr = fmaxf(rr, 0.f), cg = fmaxf(rg, 0.f), cb = fmaxf(rb, 0.f);
-                        const float v_alpha = (cr * T - br * ra - ra * tbr) * vr + (cg * T - bgc * ra - ra * tbg) * vg +
-                                              (cb * T - bb * ra - ra * tbb) * vb;
-                        br += cr * fac;
-                        bgc += cg * fac;
-                        bb += cb * fac;
-                        g5 = rr >= 0.f ? fac * vr : 0.f;
-                        g6 = rg >= 0.f ? fac * vg : 0.f;
-                        g7 = rb >= 0.f ? fac * vb : 0.f;
-                        if (alpha < ALPHA_MAX) {
-                            const float vs = -alpha * v_alpha;
-                            g2 = 0.5f * vs * dx * dx;
-                            g3 = 0.5f * vs * dx * dy;
-                            g4 = 0.5f * vs * dy * dy;
-                            g0 = vs * (A.z * dx + A.w * dy);
-                            g1 = vs * (A.w * dx + B.x * dy);
-                            g8 = -vs * (1.f - B.y);
-                        }
-                    }
-                }
+                const float alpha = fminf(ALPHA_MAX, B.y * __expf(-sig));
+                const bool valid = kk <= last[k] && sig >= 0.f && sig < SIGMA_MAX && alpha >= ALPHA_MIN;
+                contrib = contrib || valid;
+                // predicated: an invalid pair leaves T and S unchanged and adds zeros
+                const float ra = __builtin_amdgcn_rcpf(1.f - alpha);
+                const float Tn = T[k] * ra;
+                const float fac = valid ? alpha * Tn : 0.f;
+                const float cv = cr * vr[k] + cg * vg[k] + cbl * vb[k];
+                const float v_alpha = Tn * cv - S[k] * ra;
+                T[k] = valid ? Tn : T[k];
+                S[k] += fac * cv;
+                q6 += fac * vr[k];
+                q7 += fac * vg[k];
+                q8 += fac * vb[k];
+                const float vs = (valid && alpha < ALPHA_MAX) ? -alpha * v_alpha : 0.f;
+                const float vsx = vs * dx, vsy = vs * dy;
+                q0 += vs;
+                q1 += vsx;
+                q2 += vsy;
+                q3 += vsx * dx;
+                q4 += vsx * dy;
+                q5 += vsy * dy;
             }
             if (__ballot(contrib) != 0) {
-                const float s0 = wave_sum(g0), s1 = wave_sum(g1), s2 = wave_sum(g2), s3 = wave_sum(g3),
-                            s4 = wave_sum(g4), s5 = wave_sum(g5), s6 = wave_sum(g6), s7 = wave_sum(g7),
-                            s8 = wave_sum(g8);
-                if (lane == 0) {
-                    part[w][0][j] = s0; part[w][1][j] = s1; part[w][2][j] = s2;
-                    part[w][3][j] = s3; part[w][4][j] = s4; part[w][5][j] = s5;
-                    part[w][6][j] = s6; part[w][7][j] = s7; part[w][8][j] = s8;
+#define MSPLAT_WSUM(x) rows_to_lane63(row_sum16(x))
+                const float r0 = MSPLAT_WSUM(q0), r1 = MSPLAT_WSUM(q1), r2 = MSPLAT_WSUM(q2), r3 = MSPLAT_WSUM(q3),
+                            r4 = MSPLAT_WSUM(q4), r5 = MSPLAT_WSUM(q5), r6 = MSPLAT_WSUM(q6), r7 = MSPLAT_WSUM(q7),
+                            r8 = MSPLAT_WSUM(q8);
+#undef MSPLAT_WSUM
+                if (lane == 63) {
+                    part[w][0][j] = r0; part[w][1][j] = r1; part[w][2][j] = r2;
+                    part[w][3][j] = r3; part[w][4][j] = r4; part[w][5][j] = r5;
+                    part[w][6][j] = r6; part[w][7][j] = r7; part[w][8][j] = r8;
                 }
             }
         }
         __syncthreads();
-        if (threadIdx.x < n) {
-            const int j = threadIdx.x;
-            float v[9];
+        for (int j = threadIdx.x; j < n; j += NT) {
+            float q[NQ];
 #pragma unroll
-            for (int q = 0; q < 9; q++) v[q] = ((part[0][q][j] + part[1][q][j]) + part[2][q][j]) + part[3][q][j];
+            for (int i = 0; i < NQ; i++) {
+                float acc = part[0][i][j];
+#pragma unroll
+                for (int r = 1; r < NW; r++) acc += part[r][i][j];
+                q[i] = acc;
+            }
+            const float4 A = la[j], B = lb[j];
+            const float ox = A.z * q[1] + A.w * q[2], oy = A.w * q[1] + B.x * q[2];
+            const bool mr = B.z + 0.5f >= 0.f, mg = B.w + 0.5f >= 0.f, mb = lc[j] + 0.5f >= 0.f;
             const uint32_t p = lslot[j];
-            a.slot_a[p] = make_float4(v[0], v[1], v[2], v[3]);
-            a.slot_b[p] = make_float4(v[4], v[5], v[6], v[7]);
-            a.slot_c[p] = v[8];
+            a.slot_a[p] = make_float4(ox, oy, 0.5f * q[3], 0.5f * q[4]);
+            a.slot_b[p] = make_float4(0.5f * q[5], mr ? q[6] : 0.f, mg ? q[7] : 0.f, mb ? q[8] : 0.f);
+            a.slot_c[p] = -(1.f - B.y) * q[0];
         }
     }
 }
@@ -216,9 +246,15 @@ void launch_raster_forward(const RasterArgs &a, hipStream_t s) {
     if (nt > 0) raster_forward_kernel<<<nt, 256, 0, s>>>(a);
 }
 
-void launch_raster_backward(const RasterArgs &a, hipStream_t s) {
+void launch_raster_backward(const RasterArgs &a, int ppt, hipStream_t s) {
     const int nt = a.tiles_x * a.tiles_y;
-    if (nt > 0) raster_backward_kernel<<<nt, 256, 0, s>>>(a);
+    if (nt <= 0) return;
+    if (ppt == 4)
+        raster_backward_kernel<4><<<nt, 64, 0, s>>>(a);
+    else if (ppt == 2)
+        raster_backward_kernel<2><<<nt, 128, 0, s>>>(a);
+    else
+        raster_backward_kernel<1><<<nt, 256, 0, s>>>(a);
 }
 
 }  // namespace msplat

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libmsplat_hip.so")
 
 MSPLAT_OK = 0
 FRAME_KEEP_GRADS = 1
+TUNE_BWD_PPT = 0
 fp = C.POINTER(C.c_float)
 
 
@@ -107,6 +108,8 @@ def lib():
             "msplat_hip_drain_stage_times": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int,
                                               C.POINTER(C.c_int)], C.c_int),
             "msplat_hip_stage_name": ([C.c_int], C.c_char_p),
+            "msplat_hip_set_tuning": ([vp, C.c_int, C.c_int], C.c_int),
+            "msplat_hip_get_tuning": ([vp, C.c_int, C.POINTER(C.c_int)], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -148,6 +151,14 @@ class Context:
 
     def set_flags(self, flags: int):
         check(lib().msplat_hip_set_flags(self.ptr, flags))
+
+    def set_tuning(self, key: int, value: int):
+        check(lib().msplat_hip_set_tuning(self.ptr, key, value))
+
+    def get_tuning(self, key: int) -> int:
+        v = C.c_int()
+        check(lib().msplat_hip_get_tuning(self.ptr, key, C.byref(v)))
+        return v.value
 
     def enable_stage_timing(self, on: bool = True):
         check(lib().msplat_hip_enable_stage_timing(self.ptr, 1 if on else 0))

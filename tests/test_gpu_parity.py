@@ -166,12 +166,15 @@ def _oracle_grads(oracle, sc, cam, v_img):
     return red, vm, vs, vq
 
 
-def test_backward_gradients_match_oracle(ctx, oracle):
-    from msplat._abi import DeviceArray, FRAME_KEEP_GRADS, Grads, check, floats, lib, make_camera_struct
+@pytest.mark.parametrize("ppt", [1, 2, 4])
+def test_backward_gradients_match_oracle(ctx, oracle, ppt):
+    from msplat._abi import DeviceArray, FRAME_KEEP_GRADS, TUNE_BWD_PPT, Grads, check, floats, lib, make_camera_struct
     from msplat.backend import DeviceGaussians, render
     sc, cam = _scene_cam(seed=8, n=3000, deg=3, W=97, H=71)
     dg = DeviceGaussians(ctx, sc)
     ctx.set_flags(FRAME_KEEP_GRADS)
+    default_ppt = ctx.get_tuning(TUNE_BWD_PPT)
+    ctx.set_tuning(TUNE_BWD_PPT, ppt)
     try:
         render(ctx, dg, cam, BG)
         rng = np.random.default_rng(9)
@@ -197,11 +200,30 @@ def test_backward_gradients_match_oracle(ctx, oracle):
         report["scales"] = _rel(outs["scales"].numpy(), vs)
         report["quats"] = _rel(outs["quats"].numpy(), vq)
         report["opac"] = _rel(outs["opac"].numpy(), red["v_opac"])
-        print("\nbackward rel errors:", {k: f"{v:.2e}" for k, v in report.items()})
+        # SH bands: C0 v_rgb for dc, basis x v_rgb for the rest
+        from msplat.camera import TILE  # noqa: F401
+        d = sc["means"].astype(np.float64) - cam["cam_pos"]
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        x, y, z = d.T
+        C1, C2, C3 = 0.4886025119029199, [1.0925484305920792, -1.0925484305920792, 0.31539156525252005,
+                                          -1.0925484305920792, 0.5462742152960396], \
+            [-0.5900435899266435, 2.890611442640554, -0.4570457994644658, 0.3731763325901154, -0.4570457994644658,
+             1.445305721320277, -0.5900435899266435]
+        basis = np.stack([-C1 * y, C1 * z, -C1 * x, C2[0] * x * y, C2[1] * y * z, C2[2] * (2 * z * z - x * x - y * y),
+                          C2[3] * x * z, C2[4] * (x * x - y * y), C3[0] * y * (3 * x * x - y * y), C3[1] * x * y * z,
+                          C3[2] * y * (4 * z * z - x * x - y * y), C3[3] * z * (2 * z * z - 3 * x * x - 3 * y * y),
+                          C3[4] * x * (4 * z * z - x * x - y * y), C3[5] * z * (x * x - y * y),
+                          C3[6] * x * (x * x - 3 * y * y)], 1)
+        vis = (red["v_rgb"] != 0).any(1)[:, None, None]
+        rest_ref = np.where(vis, basis[:, :, None] * red["v_rgb"][:, None, :], 0)
+        report["dc"] = _rel(outs["dc"].numpy(), 0.28209479177387814 * red["v_rgb"])
+        report["rest"] = _rel(outs["rest"].numpy(), rest_ref)
+        print(f"\nbackward (ppt={ppt}) rel errors:", {k: f"{v:.2e}" for k, v in report.items()})
         for k, v in report.items():
             assert v < 1e-4, (k, v)
     finally:
         ctx.set_flags(0)
+        ctx.set_tuning(TUNE_BWD_PPT, default_ppt)
 
 
 def test_train_steps_match_golden(ctx):
