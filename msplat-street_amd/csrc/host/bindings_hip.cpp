@@ -36,7 +36,6 @@ struct Backend {
     std::unordered_map<const void *, Mirror> mirrors;
     MTensor out_img, radii;
     bool timing = false;
-    std::vector<double> cb_times;
     std::mutex mu;
 
     msplat_hip_ctx *get() {
