@@ -218,6 +218,56 @@ int msplat_hip_image_metrics(msplat_hip_ctx *ctx, const float *rendered, const f
 /* msplat_pack_rgba8 (bindings.h:174-175): out is a device (n,4) uint8 buffer. */
 int msplat_hip_pack_rgba8(msplat_hip_ctx *ctx, const float *img, uint32_t n, uint8_t *out);
 
+/* ── batched views (no reference counterpart: the reference renders and trains one view
+ * per call, msplat_render / msplat_train_step, bindings.h:81-118) ──────────────────
+ * Several views of one scene, of mixed resolution, in one pass: one projection of every
+ * gaussian into every view, one key sort over all views' tiles, a persistent rasterizer
+ * that pulls (view, tile) items from a device work queue, one loss pass and one
+ * backward that sums the views' gradients. Per view the results equal those of the
+ * one-view entry points (images, final T and contributor indices bit for bit;
+ * gradients within float rounding of the sum of one-view gradients).
+ *
+ * Per-pixel outputs of view v are packed at pixel offset pixel_offsets[v] (the sum of
+ * W*H over the views before it): image (3 floats per pixel) and final_T. The batch
+ * calls sync before they return. */
+#define MSPLAT_MAX_BATCH_VIEWS 64
+
+typedef struct {
+    int32_t num_views;                 /* 1..MSPLAT_MAX_BATCH_VIEWS */
+    const msplat_hip_camera *cameras;  /* host array [num_views]; same degree and glob_scale */
+    const float *backgrounds;          /* host [num_views * 3] */
+    const float *const *targets;       /* host array of device pointers (H_v, W_v, 3); training only */
+} msplat_hip_views;
+
+/* Renders every view; *out_images (device) receives the packed images and
+ * pixel_offsets (host, num_views entries, may be NULL) the views' offsets in pixels. */
+int msplat_hip_render_batch(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_views *views,
+                            float **out_images, uint64_t *pixel_offsets);
+
+/* One training step over every view: loss_v = (1 - w) L1 + w (1 - SSIM) of view v
+ * (normalised by 3 W_v H_v), gradients summed over the views, one Adam step (dense
+ * groups over every gaussian, SH groups over gaussians visible in some view), and the
+ * densification statistics updated per view (max_2d_size with 1 / max(W_v, H_v)).
+ * losses (host, num_views entries, may be NULL) receives the per-view losses. */
+int msplat_hip_train_step_batch(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_views *views,
+                                float ssim_weight, const msplat_hip_adam *adam, const msplat_hip_grad_stats *stats,
+                                float *losses);
+
+/* Gradients of sum_v <image_v, v_images_v> for the last render_batch on this context;
+ * v_images is packed like the rendered images. Gradients are summed over the views and
+ * written (not accumulated); NULL outputs are skipped. */
+int msplat_hip_render_backward_batch(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_views *views,
+                                     const float *v_images, const msplat_hip_grads *out);
+
+/* Captures msplat_hip_train_step_batch into a hipGraph and replays it (1) or runs it
+ * eagerly (0, the default). The graph is recaptured when the gaussian count, the view
+ * count, a buffer or the intersection capacity changes; poses, resolutions within the
+ * buffers, targets and Adam scalars change between replays through device memory.
+ * Needs a context with its own (non-null) stream. */
+int msplat_hip_set_batch_graph(msplat_hip_ctx *ctx, int enable);
+/* Graph statistics: captures so far and replays so far. */
+int msplat_hip_batch_graph_stats(msplat_hip_ctx *ctx, int64_t *captures, int64_t *replays);
+
 #ifdef __cplusplus
 }
 #endif

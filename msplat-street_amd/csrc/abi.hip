@@ -12,16 +12,15 @@
 #include <string>
 #include <vector>
 
-#include "common.h"
-#include "kernels.h"
+#include "context.h"
 
 using namespace msplat;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-int set_error(int code, const char *fmt, ...) {
+int msplat::set_error(int code, const char *fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -31,176 +30,7 @@ int set_error(int code, const char *fmt, ...) {
     return code;
 }
 
-#define HIPCHECK(expr)                                                                                           \
-    do {                                                                                                         \
-        hipError_t e_ = (expr);                                                                                  \
-        if (e_ != hipSuccess) return set_error(MSPLAT_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
-
-#define CHECKED(expr)               \
-    do {                            \
-        int rc_ = (expr);           \
-        if (rc_ != MSPLAT_OK) return rc_; \
-    } while (0)
-
-struct Buf {
-    void *p = nullptr;
-    size_t cap = 0;
-};
-
-__global__ void fill_u32_kernel(uint32_t *p, uint32_t v) { *p = v; }
-
-}  // namespace
-
-struct msplat_hip_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    int flags = 0;
-    int tune[MSPLAT_NUM_TUNABLES] = {4};
-    // per gaussian
-    Buf splat_a, splat_b, splat_blue, depths, aabb, rect, radii, count, offsets, block_sums;
-    Buf v_xy, v_conic, v_rgb, v_opac, sh_scratch, big;
-    // per intersection
-    Buf keys0, keys1, vals0, vals1, slot_a, slot_b, slot_c, hist, digit_total;
-    // per tile / pixel
-    Buf tile_start, image, final_T, final_idx, v_image, pix_loss, row_loss;
-    Buf grad_scratch;  // stands in for NULL gradient outputs of render_backward
-    Buf aux;           // densify flags/scans/scratch, kNN and metrics workspaces
-    // stage timing
-    bool timing = false;
-    struct Mark {
-        int stage;
-        hipEvent_t a, b;
-    };
-    std::vector<Mark> marks;
-    std::vector<hipEvent_t> event_pool;
-    double stage_ms[MSPLAT_NUM_STAGES] = {};
-    int64_t stage_calls[MSPLAT_NUM_STAGES] = {};
-    int open_stage = -1;
-    hipEvent_t open_event = nullptr;
-
-    hipEvent_t take_event() {
-        if (!event_pool.empty()) {
-            hipEvent_t e = event_pool.back();
-            event_pool.pop_back();
-            return e;
-        }
-        hipEvent_t e = nullptr;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
-        return e;
-    }
-    void begin(int stage) {
-        if (!timing) return;
-        open_stage = stage;
-        open_event = take_event();
-        if (open_event) (void)hipEventRecord(open_event, stream);
-    }
-    void end() {
-        if (!timing || open_stage < 0 || !open_event) return;
-        hipEvent_t e = take_event();
-        if (!e) return;
-        (void)hipEventRecord(e, stream);
-        marks.push_back({open_stage, open_event, e});
-        open_stage = -1;
-        open_event = nullptr;
-    }
-    void collect() {  // call after a stream sync
-        for (auto &m : marks) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
-                stage_ms[m.stage] += ms;
-                stage_calls[m.stage] += 1;
-            }
-            event_pool.push_back(m.a);
-            event_pool.push_back(m.b);
-        }
-        marks.clear();
-    }
-    // device scalars: [0] u64 isect total, [8] u32 total, [16] double loss sum, [24] float loss
-    Buf scalars;
-    uint64_t *h_total = nullptr;  // pinned
-    // frame state
-    int n = 0, ntiles = 0;
-    uint32_t width = 0, height = 0;
-    int tiles_x = 0, tiles_y = 0;
-    int64_t M = 0;
-    uint64_t *keys_sorted = nullptr;
-    int32_t *gid_sorted = nullptr;
-    float bg[3] = {0, 0, 0};
-
-    int grow(Buf &b, size_t bytes) {
-        if (bytes == 0) bytes = 16;
-        if (b.cap >= bytes) return MSPLAT_OK;
-        if (b.p) {
-            HIPCHECK(hipStreamSynchronize(stream));
-            HIPCHECK(hipFree(b.p));
-            b.p = nullptr;
-            b.cap = 0;
-        }
-        size_t cap = bytes + bytes / 4;  // headroom against regrowth
-        HIPCHECK(hipMalloc(&b.p, cap));
-        b.cap = cap;
-        return MSPLAT_OK;
-    }
-
-    void release(Buf &b) {
-        if (b.p) (void)hipFree(b.p);
-        b.p = nullptr;
-        b.cap = 0;
-    }
-
-    ~msplat_hip_ctx() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        Buf *all[] = {&splat_a, &splat_b, &splat_blue, &depths, &aabb, &rect, &radii, &count, &offsets, &block_sums,
-                      &v_xy, &v_conic, &v_rgb, &v_opac, &sh_scratch, &big, &keys0, &keys1, &vals0, &vals1, &slot_a, &slot_b, &slot_c,
-                      &hist, &digit_total, &tile_start, &image, &final_T, &final_idx, &v_image, &pix_loss,
-                      &row_loss, &scalars, &grad_scratch, &aux};
-        for (Buf *b : all) release(*b);
-        if (h_total) (void)hipHostFree(h_total);
-        collect();
-        for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
-        if (own_stream && stream) (void)hipStreamDestroy(stream);
-    }
-
-    uint64_t *d_total64() { return (uint64_t *)scalars.p; }
-    uint32_t *d_big_count() { return (uint32_t *)((char *)scalars.p + 32); }
-    uint32_t *big_list() { return (uint32_t *)big.p; }
-    float4 *big_a() { return (float4 *)((char *)big.p + 4 * (size_t)(n > 0 ? n : 1) + 12); }
-    uint32_t *d_total32() { return (uint32_t *)((char *)scalars.p + 8); }
-    double *d_loss_sum() { return (double *)((char *)scalars.p + 16); }
-    float *d_loss() { return (float *)((char *)scalars.p + 24); }
-};
-
 namespace {
-
-int check_camera(const msplat_hip_camera *c) {
-    if (!c) return set_error(MSPLAT_ERR_ARG, "camera is NULL");
-    if (c->width == 0 || c->height == 0) return set_error(MSPLAT_ERR_ARG, "image size must be positive");
-    if (c->tiles_x != (int)((c->width + TILE - 1) / TILE) || c->tiles_y != (int)((c->height + TILE - 1) / TILE))
-        return set_error(MSPLAT_ERR_ARG, "tile bounds (%d, %d) do not match %ux%u", c->tiles_x, c->tiles_y, c->width,
-                         c->height);
-    if (c->tiles_x > 0xFFFF || c->tiles_y > 0xFFFF) return set_error(MSPLAT_ERR_ARG, "image too large");
-    if (c->degree > 4) return set_error(MSPLAT_ERR_ARG, "SH degree above 4");
-    if (c->degrees_to_use > c->degree)
-        return set_error(MSPLAT_ERR_ARG, "degrees_to_use (%u) exceeds the stored SH degree (%u)", c->degrees_to_use,
-                         c->degree);
-    return MSPLAT_OK;
-}
-
-int check_gaussians(const msplat_hip_gaussians *g) {
-    if (!g) return set_error(MSPLAT_ERR_ARG, "gaussians is NULL");
-    if (g->num_points < 0) return set_error(MSPLAT_ERR_ARG, "negative gaussian count");
-    if (g->num_points > 0 && (!g->means || !g->scales || !g->quats || !g->features_dc || !g->opacities))
-        return set_error(MSPLAT_ERR_ARG, "a gaussian parameter pointer is NULL");
-    return MSPLAT_OK;
-}
-
-int tile_bits(int ntiles) {
-    int b = 0;
-    while ((1 << b) < ntiles) b++;
-    return b;
-}
 
 // projection + count scan; returns with ctx->M read back
 int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam) {
@@ -709,7 +539,7 @@ int msplat_hip_radix_sort(msplat_hip_ctx *ctx, uint64_t *keys, int32_t *vals, ui
     CHECKED(ctx->grow(ctx->digit_total, 4 * 256));
     CHECKED(ctx->grow(ctx->scalars, 64));
     uint32_t *dn = ctx->d_total32();
-    fill_u32_kernel<<<1, 1, 0, ctx->stream>>>(dn, n);
+    fill_u32<<<1, 1, 0, ctx->stream>>>(dn, n);
     uint64_t *kin = keys, *kout = (uint64_t *)ctx->keys1.p;
     int32_t *vin = vals, *vout = (int32_t *)ctx->vals1.p;
     for (int shift = 0; shift < key_bits; shift += 8) {

@@ -8,6 +8,8 @@
 //   BWD_GRADS — the parameter gradients themselves (autograd backend).
 // The slot sum runs over the gaussian's intersections in tile order, as the reference's
 // reduce_isect_grads (cpu:1034-1064). Built with -ffp-contract=off.
+#include <cstring>
+
 #include "common.h"
 #include "kernels.h"
 #include "proj_math.h"
@@ -177,6 +179,152 @@ __global__ __launch_bounds__(256) void sh_rest_kernel(BackwardArgs a) {
         return;
     }
     adam_step(a.rest[e], a.m1[4][e], a.m2[4][e], grad, a.step[4], a.beta1, a.beta2, a.bc2[4], a.eps);
+}
+
+// ── batched views ────────────────────────────────────────────────────────────
+// Gaussian g's gradient summed over the views of a batch, one thread per gaussian. Per
+// view the slot sums (or the big-list sums) give the 2D cotangents, which go through
+// the view's pixel, conic and EWA VJPs; the world-covariance cotangents are summed over
+// the views and taken through the covariance VJP once (it is linear in them), and the
+// SH band cotangents basis(dir_v) x v_rgb_v are summed per coefficient. The
+// densification statistics are updated per view as a sequence of one-view steps would.
+__global__ __launch_bounds__(256) void batch_backward_kernel(BatchBackwardArgs a) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= a.n || a.sc->overflow) return;
+    const float p[3] = {a.means[3 * g], a.means[3 * g + 1], a.means[3 * g + 2]};
+    const float es[3] = {exp_cr(a.scales[3 * g]), exp_cr(a.scales[3 * g + 1]), exp_cr(a.scales[3 * g + 2])};
+    const float q[4] = {a.quats[4 * g], a.quats[4 * g + 1], a.quats[4 * g + 2], a.quats[4 * g + 3]};
+    const float glob = a.tab->views[0].cam.glob_scale;
+    float cov[6];
+    covariance3d(es, glob, q, cov);
+    float vm[3] = {0.f, 0.f, 0.f}, vcov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, vrgb[3] = {0.f, 0.f, 0.f}, vop = 0.f;
+    float racc[45];
+#pragma unroll
+    for (int i = 0; i < 45; i++) racc[i] = 0.f;
+    bool any = false;
+    float st_vis = 0.f, st_gn = 0.f, st_ms = a.ms ? a.ms[g] : 0.f;
+    for (int v = 0; v < a.num_views; v++) {
+        const size_t r = (size_t)v * a.n + g;
+        const int32_t rad = a.radii[r];
+        if (rad <= 0) continue;
+        any = true;
+        float vxy[2] = {0.f, 0.f}, vcn[3] = {0.f, 0.f, 0.f}, vc[3] = {0.f, 0.f, 0.f}, vo = 0.f;
+        const uint32_t c = a.count[r];
+        if (c > BIG_SLOTS) {
+            const float4 A = a.big_a[r], B = a.big_b[r];
+            vxy[0] = A.x; vxy[1] = A.y; vcn[0] = A.z; vcn[1] = A.w;
+            vcn[2] = B.x; vc[0] = B.y; vc[1] = B.z; vc[2] = B.w;
+            vo = a.big_c[r];
+        } else {
+            const uint32_t o = a.offsets[r];
+            for (uint32_t k = o; k < o + c; k++) {
+                const float4 A = a.slot_a[k], B = a.slot_b[k];
+                vxy[0] += A.x; vxy[1] += A.y; vcn[0] += A.z; vcn[1] += A.w;
+                vcn[2] += B.x; vc[0] += B.y; vc[1] += B.z; vc[2] += B.w;
+                vo += a.slot_c[k];
+            }
+        }
+        const BatchView &bv = a.tab->views[v];
+        const msplat_hip_camera &cam = bv.cam;
+        float vmv[3];
+        pixel_vjp(cam.projmat, p, cam.width, cam.height, vxy[0], vxy[1], vmv);
+        const float4 SA = a.splat_a[r], SB = a.splat_b[r];
+        const float cn[3] = {SA.z, SA.w, SB.x};
+        float vc2[3], pv[3], vcv[6];
+        conic_vjp(cn, vcn, vc2);
+        const float tfx = 0.5f * (float)cam.width / cam.fx, tfy = 0.5f * (float)cam.height / cam.fy;
+        cam_space(cam.viewmat, p, pv);
+        ewa_vjp(cov, cam.viewmat, cam.fx, cam.fy, tfx, tfy, vc2, pv, vmv, vcv);
+#pragma unroll
+        for (int i = 0; i < 3; i++) vm[i] += vmv[i];
+#pragma unroll
+        for (int i = 0; i < 6; i++) vcov[i] += vcv[i];
+#pragma unroll
+        for (int i = 0; i < 3; i++) vrgb[i] += vc[i];
+        vop += vo;
+        const int dtu = (int)min(cam.degrees_to_use, 3u);
+        if (dtu > 0) {
+            float d[3], basis[15];
+            view_direction(p, cam.cam_pos, d);
+            const int nbasis = sh_basis((uint32_t)dtu, d, basis);
+#pragma unroll
+            for (int b = 0; b < 15; b++)
+                if (b < nbasis) {
+                    racc[3 * b] += basis[b] * vc[0];
+                    racc[3 * b + 1] += basis[b] * vc[1];
+                    racc[3 * b + 2] += basis[b] * vc[2];
+                }
+        }
+        st_vis += 1.0f;
+        st_gn += sqrtf(vxy[0] * vxy[0] + vxy[1] * vxy[1]);
+        st_ms = fmaxf(st_ms, (float)rad * bv.inv_max_dim);
+    }
+    float vs[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
+    if (any) {
+        cov3d_vjp(es, glob, q, vcov, vs, vq);
+        vs[0] *= es[0]; vs[1] *= es[1]; vs[2] *= es[2];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        a.g_means[3 * g + i] = vm[i];
+        a.g_scales[3 * g + i] = vs[i];
+        a.g_dc[3 * g + i] = any ? SH_C0 * vrgb[i] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) a.g_quats[4 * g + i] = vq[i];
+    a.g_opac[g] = vop;
+    if (a.rest_stride) {
+        float *gr = a.g_rest + (size_t)a.rest_stride * g;
+#pragma unroll
+        for (int i = 0; i < 45; i++)
+            if ((uint32_t)i < a.rest_stride) gr[i] = racc[i];
+        for (uint32_t i = 45; i < a.rest_stride; i++) gr[i] = 0.f;
+    }
+    a.vis_any[g] = any ? 1 : 0;
+    if (a.vis && any) {
+        a.vis[g] += st_vis;
+        a.gn[g] += st_gn;
+        a.ms[g] = st_ms;
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_adam_kernel(float *p, float *m1, float *m2, const float *grad,
+                                                        size_t count, int width, const uint8_t *vis,
+                                                        const BatchTable *tab, int group, const uint32_t *overflow) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= count || *overflow) return;
+    if (vis && !vis[e / (size_t)width]) return;
+    if (group == 4 && (uint32_t)(e % (size_t)width) >= tab->rest_active) return;
+    adam_step(p[e], m1[e], m2[e], grad[e], tab->step[group], tab->beta1, tab->beta2, tab->bc2[group], tab->eps);
+}
+
+void launch_big_reduce_records(const uint32_t *big_list, const uint32_t *big_count, const uint32_t *offsets,
+                               const uint32_t *count, const float4 *slot_a, const float4 *slot_b, const float *slot_c,
+                               float4 *big_a, float4 *big_b, float *big_c, hipStream_t s) {
+    BackwardArgs a;
+    memset(&a, 0, sizeof a);
+    a.big_list = big_list;
+    a.big_count = big_count;
+    a.offsets = offsets;
+    a.count = count;
+    a.slot_a = slot_a;
+    a.slot_b = slot_b;
+    a.slot_c = slot_c;
+    a.big_a = big_a;
+    a.big_b = big_b;
+    a.big_c = big_c;
+    big_reduce_kernel<<<1024, 256, 0, s>>>(a);
+}
+
+void launch_batch_backward(const BatchBackwardArgs &a, hipStream_t s) {
+    const int nb = (a.n + 255) / 256;
+    if (nb > 0 && a.num_views > 0) batch_backward_kernel<<<nb, 256, 0, s>>>(a);
+}
+
+void launch_batch_adam(float *p, float *m1, float *m2, const float *grad, size_t count, int width,
+                       const uint8_t *vis, const BatchTable *tab, int group, const uint32_t *overflow, hipStream_t s) {
+    const size_t nb = (count + 255) / 256;
+    if (nb > 0) batch_adam_kernel<<<(unsigned)nb, 256, 0, s>>>(p, m1, m2, grad, count, width, vis, tab, group, overflow);
 }
 
 void launch_backward(const BackwardArgs &a, BackwardMode mode, hipStream_t s) {

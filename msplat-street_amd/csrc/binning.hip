@@ -292,7 +292,185 @@ __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint64_t *keys, 
     for (int u = tp + 1; u <= t; u++) tile_start[u] = (int32_t)k;
 }
 
+// tile_ranges_kernel with the tile count read from device memory (batched frames)
+__global__ __launch_bounds__(256) void tile_ranges_dev_kernel(const uint64_t *keys, const uint32_t *n_ptr,
+                                                             const uint32_t *ntiles_ptr, int32_t *tile_start) {
+    const uint32_t n = *n_ptr;
+    const int ntiles = (int)*ntiles_ptr;
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k > n) return;
+    const int t = k < n ? (int)(keys[k] >> 32) : ntiles;
+    const int tp = k > 0 ? (int)(keys[k - 1] >> 32) : -1;
+    for (int u = tp + 1; u <= t; u++) tile_start[u] = (int32_t)k;
+}
+
+// ── batched views ────────────────────────────────────────────────────────────
+
+// Projection of gaussian g into every view of the batch: the parameters are read once
+// and record v * n + g is written per view, with the same arithmetic as project_kernel
+// (the records are bit-identical to one-view projections). Per view, the workgroup's
+// intersection count goes to block_sums[v * nbx + blockIdx.x].
+__global__ __launch_bounds__(256) void project_batch_kernel(BatchProjectArgs a) {
+    __shared__ uint32_t lds4[4];
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    const bool live = g < a.n;
+    float p[3] = {0.f, 0.f, 0.f}, s[3] = {0.f, 0.f, 0.f}, q[4] = {1.f, 0.f, 0.f, 0.f}, logit = 0.f;
+    if (live) {
+        p[0] = a.means[3 * g]; p[1] = a.means[3 * g + 1]; p[2] = a.means[3 * g + 2];
+        s[0] = exp_cr(a.scales[3 * g]); s[1] = exp_cr(a.scales[3 * g + 1]); s[2] = exp_cr(a.scales[3 * g + 2]);
+        q[0] = a.quats[4 * g]; q[1] = a.quats[4 * g + 1]; q[2] = a.quats[4 * g + 2]; q[3] = a.quats[4 * g + 3];
+        logit = a.opac[g];
+    }
+    const float opac = 1.f / (1.f + exp_cr(-logit));
+    for (int v = 0; v < a.num_views; v++) {
+        const msplat_hip_camera &cam = a.tab->views[v].cam;
+        const size_t r = (size_t)v * a.n + g;
+        uint32_t count = 0;
+        if (live) {
+            a.radii[r] = 0;
+            a.count[r] = 0;
+            float pv[3];
+            cam_space(cam.viewmat, p, pv);
+            if (pv[2] > cam.clip_thresh) {
+                float cov[6], c2[3], conic[3], radius;
+                covariance3d(s, cam.glob_scale, q, cov);
+                const float tfx = 0.5f * (float)cam.width / cam.fx, tfy = 0.5f * (float)cam.height / cam.fy;
+                ewa_cov2d(cov, cam.viewmat, cam.fx, cam.fy, tfx, tfy, pv, c2);
+                if (conic_radius(c2, conic, radius)) {
+                    float px, py;
+                    to_pixel(cam.projmat, p, cam.width, cam.height, cam.cx, cam.cy, px, py);
+                    const float ax = ceilf(3.f * sqrtf(c2[0])), ay = ceilf(3.f * sqrtf(c2[2]));
+                    int x0, y0, x1, y1;
+                    tile_rect(px, py, ax, ay, cam.tiles_x, cam.tiles_y, x0, y0, x1, y1);
+                    if ((x1 - x0) * (y1 - y0) > 0) {
+                        float d[3], col[3];
+                        view_direction(p, cam.cam_pos, d);
+                        const uint32_t nb = sh_bases(cam.degree);
+                        sh_color(cam.degrees_to_use, d, a.dc + 3 * (size_t)g, a.rest + (size_t)(nb - 1) * 3 * g, col);
+                        a.splat_a[r] = make_float4(px, py, conic[0], conic[1]);
+                        a.splat_b[r] = make_float4(conic[2], opac, col[0], col[1]);
+                        a.splat_blue[r] = col[2];
+                        a.depths[r] = pv[2];
+                        a.rect[r] = make_uint2((uint32_t)x0 | ((uint32_t)x1 << 16),
+                                               (uint32_t)y0 | ((uint32_t)y1 << 16));
+                        a.radii[r] = (int)radius;
+                        count = (uint32_t)((x1 - x0) * (y1 - y0));
+                        a.count[r] = count;
+                    }
+                }
+            }
+        }
+        uint32_t total;
+        block_scan_u32(count, total, lds4);
+        if (threadIdx.x == 0) a.block_sums[(size_t)v * a.nbx + blockIdx.x] = total;
+    }
+}
+
+// One workgroup per view: exclusive scan of the view's block sums, view total
+__global__ __launch_bounds__(256) void batch_view_scan_kernel(uint32_t *sums, int nbx, BatchScalars *sc) {
+    __shared__ uint32_t lds4[4];
+    __shared__ uint64_t carry;
+    uint32_t *row = sums + (size_t)blockIdx.x * nbx;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nbx; base += 256) {
+        const int i = base + threadIdx.x;
+        uint32_t v = i < nbx ? row[i] : 0, t;
+        const uint32_t ex = block_scan_u32(v, t, lds4);
+        const uint64_t c = carry;
+        if (i < nbx) row[i] = (uint32_t)(c + ex);
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + t;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sc->view_total[blockIdx.x] = carry > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)carry;
+}
+
+// Views' bases (in place of their totals), the intersection total and the capacity
+// check. Over capacity, total32 is 0 and overflow is set: the keys are not written and
+// every later stage sees an empty frame.
+__global__ void batch_total_kernel(int num_views, uint32_t cap, BatchScalars *sc) {
+    if (threadIdx.x != 0) return;
+    uint64_t acc = 0;
+    for (int v = 0; v < num_views; v++) {
+        const uint32_t t = sc->view_total[v];
+        sc->view_total[v] = acc > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)acc;
+        acc += t;
+    }
+    sc->total = acc;
+    const bool over = acc > (uint64_t)cap;
+    sc->overflow = over ? 1u : 0u;
+    sc->total32 = over ? 0u : (uint32_t)acc;
+    sc->big_count = 0;
+}
+
+// keygen_kernel per (block, view), keys on global tile ids, record ids as payload
+__global__ __launch_bounds__(256) void keygen_batch_kernel(BatchBinArgs a) {
+    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t s_off[257];
+    __shared__ uint32_t s_rect[256][2];
+    __shared__ uint32_t s_depth[256];
+    __shared__ uint32_t s_bigbase;
+    if (a.sc->overflow) return;
+    const int v = blockIdx.y;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    const size_t r0 = (size_t)v * a.n;
+    const uint32_t c = g < a.n ? a.count[r0 + g] : 0;
+    uint32_t total;
+    const uint32_t ex = block_scan_u32(c, total, lds4);
+    const uint32_t base = a.sc->view_total[v] + a.block_sums[(size_t)v * a.nbx + blockIdx.x];
+    if (g < a.n) a.offsets[r0 + g] = base + ex;
+    uint32_t nbig;
+    const uint32_t bex = block_scan_u32(c > BIG_SLOTS ? 1u : 0u, nbig, lds4);
+    if (threadIdx.x == 0 && nbig) s_bigbase = atomicAdd(&a.sc->big_count, nbig);
+    __syncthreads();
+    if (c > BIG_SLOTS) a.big_list[s_bigbase + bex] = (uint32_t)(r0 + g);
+    s_off[threadIdx.x] = ex;
+    if (threadIdx.x == 0) s_off[256] = total;
+    if (c) {
+        const uint2 rr = a.rect[r0 + g];
+        s_rect[threadIdx.x][0] = rr.x;
+        s_rect[threadIdx.x][1] = rr.y;
+        s_depth[threadIdx.x] = __float_as_uint(a.depths[r0 + g]);
+    }
+    __syncthreads();
+    const int tiles_x = a.tab->views[v].cam.tiles_x;
+    const uint32_t tbase = a.tab->tile_base[v];
+    for (uint32_t i = threadIdx.x; i < total; i += 256) {
+        int lo = 0, hi = 255;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t rx = s_rect[lo][0], ry = s_rect[lo][1];
+        const int x0 = rx & 0xFFFF, x1 = rx >> 16, y0 = ry & 0xFFFF;
+        const uint32_t local = i - s_off[lo], w = (uint32_t)(x1 - x0);
+        const int ty = y0 + (int)(local / w), tx = x0 + (int)(local % w);
+        const uint32_t gtile = tbase + (uint32_t)(ty * tiles_x + tx);
+        a.keys[base + i] = ((uint64_t)gtile << 32) | (uint64_t)s_depth[lo];
+        a.vals[base + i] = (int32_t)(r0 + blockIdx.x * 256 + lo);
+    }
+}
+
 // ── host launchers ───────────────────────────────────────────────────────────
+
+void launch_batch_project(const BatchProjectArgs &a, hipStream_t s) {
+    if (a.nbx > 0) project_batch_kernel<<<a.nbx, 256, 0, s>>>(a);
+}
+
+void launch_batch_bin(const BatchBinArgs &a, hipStream_t s) {
+    if (a.num_views <= 0) return;
+    if (a.nbx > 0) batch_view_scan_kernel<<<a.num_views, 256, 0, s>>>(a.block_sums, a.nbx, a.sc);
+    batch_total_kernel<<<1, 64, 0, s>>>(a.num_views, a.cap, a.sc);
+    if (a.nbx > 0) keygen_batch_kernel<<<dim3(a.nbx, a.num_views), 256, 0, s>>>(a);
+}
+
+void launch_tile_ranges_dev(const uint64_t *keys, const uint32_t *n_dev, uint32_t n_max, const uint32_t *ntiles_dev,
+                            int32_t *tile_start, hipStream_t s) {
+    const uint32_t nb = (n_max + 1 + 255) / 256;
+    tile_ranges_dev_kernel<<<nb, 256, 0, s>>>(keys, n_dev, ntiles_dev, tile_start);
+}
 
 void launch_project(const ProjectArgs &a, hipStream_t s) {
     const int nb = (a.n + 255) / 256;

@@ -128,6 +128,137 @@ int knn3_launch(const float *pts, uint32_t n, float *mean_dist, KnnScratch &w, h
 void launch_init_gaussians(const float *md, const uint8_t *rgb, uint32_t n, uint32_t seed, float logit, float *scales,
                            float *quats, float *dc, float *opac, hipStream_t s);
 
+// ── batched views ───────────────────────────────────────────────────────────
+// Several views of one scene go through one projection, one key sort over global tile
+// ids (tile_base[v] + tile), one persistent rasterizer over (view, tile) items, one
+// loss launch and one per-gaussian backward that sums over the views. Everything that
+// changes from step to step (poses, resolutions, targets, Adam scalars) is read from a
+// BatchTable in device memory, so a captured graph replays with new contents.
+// Record r = v * n + g holds gaussian g's projection in view v.
+constexpr int MAX_VIEWS = MSPLAT_MAX_BATCH_VIEWS;
+
+struct BatchView {
+    msplat_hip_camera cam;
+    const float *gt;  // (H, W, 3) target (training)
+    float bg[3];
+    float inv_n;        // loss normalisation, 1 / (3 W H)
+    float inv_max_dim;  // screen-size statistic scale
+    uint32_t pix_base;  // first pixel of the view in the packed per-pixel buffers
+};
+
+struct BatchTable {
+    int32_t num_views;
+    uint32_t total_tiles, total_pixels, total_rows, total_loss_blocks;
+    float ssim_weight;
+    float step[6], bc2[6];
+    float beta1, beta2, eps;
+    uint32_t rest_active;  // SH rest coefficients per gaussian in the bands in use (max over the views)
+    // prefix offsets over the views (num_views + 1 entries each)
+    uint32_t tile_base[MAX_VIEWS + 1], row_base[MAX_VIEWS + 1], loss_base[MAX_VIEWS + 1];
+    BatchView views[MAX_VIEWS];
+};
+
+// Device scalars of a batched frame
+struct BatchScalars {
+    uint64_t total;       // intersections
+    uint32_t total32;     // intersections, 0 when they exceed the capacity
+    uint32_t overflow;    // 1 when the intersections exceed the capacity: the step is void
+    uint32_t big_count;   // records with more than BIG_SLOTS intersections
+    uint32_t queue[2];    // persistent rasterizer work-queue heads (forward, backward)
+    uint32_t pad;
+    uint32_t view_total[MAX_VIEWS];
+    double loss_sum[MAX_VIEWS];
+    float loss_mean[MAX_VIEWS];
+};
+
+struct BatchProjectArgs {
+    const BatchTable *tab;
+    int n, num_views, nbx;  // nbx = ceil(n / 256) blocks per view
+    const float *means, *scales, *quats, *dc, *rest, *opac;
+    float4 *splat_a, *splat_b;
+    float *splat_blue, *depths;
+    uint2 *rect;
+    int32_t *radii;
+    uint32_t *count, *block_sums;  // block_sums[v * nbx + b]
+};
+
+struct BatchBinArgs {
+    const BatchTable *tab;
+    int n, num_views, nbx;
+    uint32_t cap;
+    const uint32_t *count;
+    const uint2 *rect;
+    const float *depths;
+    uint32_t *block_sums, *offsets;
+    uint64_t *keys;
+    int32_t *vals;
+    uint32_t *big_list;
+    BatchScalars *sc;
+};
+
+struct BatchLossArgs {
+    const BatchTable *tab;
+    int num_views;
+    uint32_t max_loss_blocks, max_rows;
+    const float *img;
+    float *v_img, *pix_loss;
+    double *row_loss;
+    BatchScalars *sc;
+};
+
+struct BatchBackwardArgs {
+    const BatchTable *tab;
+    int n, num_views;
+    const float *means, *scales, *quats;
+    const int32_t *radii;
+    const uint32_t *count, *offsets;
+    const float4 *splat_a, *splat_b;
+    const float4 *slot_a, *slot_b;
+    const float *slot_c;
+    const float4 *big_a, *big_b;  // slot sums of records with count > BIG_SLOTS
+    const float *big_c;
+    const BatchScalars *sc;
+    uint32_t rest_stride;  // stored (bases - 1) * 3
+    // gradients summed over the views, and whether the gaussian is visible in any view
+    float *g_means, *g_scales, *g_quats, *g_dc, *g_rest, *g_opac;
+    uint8_t *vis_any;
+    // densification statistics (null: not updated)
+    float *vis, *gn, *ms;
+};
+
+void launch_batch_project(const BatchProjectArgs &a, hipStream_t s);
+void launch_batch_bin(const BatchBinArgs &a, hipStream_t s);  // view scans, totals, keys
+void launch_tile_ranges_dev(const uint64_t *keys, const uint32_t *n_dev, uint32_t n_max, const uint32_t *ntiles_dev,
+                            int32_t *tile_start, hipStream_t s);
+struct BatchRasterArgs {
+    const BatchTable *tab;
+    int num_views;
+    uint32_t *queue;
+    const int32_t *tile_start, *gid;
+    const float4 *splat_a, *splat_b;
+    const float *splat_blue;
+    const uint2 *rect;
+    const uint32_t *offsets;
+    float *image, *final_T;
+    int32_t *final_idx;
+    const float *v_image;
+    float4 *slot_a, *slot_b;
+    float *slot_c;
+};
+int batch_raster_grid(bool backward, int ppt);  // persistent workgroups that fill the device
+void launch_batch_raster_forward(const BatchRasterArgs &a, hipStream_t s);
+void launch_batch_raster_backward(const BatchRasterArgs &a, int ppt, hipStream_t s);
+void launch_batch_loss(const BatchLossArgs &a, hipStream_t s);
+void launch_big_reduce_records(const uint32_t *big_list, const uint32_t *big_count, const uint32_t *offsets,
+                               const uint32_t *count, const float4 *slot_a, const float4 *slot_b, const float *slot_c,
+                               float4 *big_a, float4 *big_b, float *big_c, hipStream_t s);
+void launch_batch_backward(const BatchBackwardArgs &a, hipStream_t s);
+// Adam over one parameter group from summed gradients; `vis` (per gaussian, null =
+// dense) restricts it to gaussians visible in some view, and for the SH rest group to
+// the coefficients of the bands in use; void when overflow is set.
+void launch_batch_adam(float *p, float *m1, float *m2, const float *grad, size_t count, int width,
+                       const uint8_t *vis, const BatchTable *tab, int group, const uint32_t *overflow, hipStream_t s);
+
 void launch_project(const ProjectArgs &a, hipStream_t s);
 void launch_rect_count(int n, const float4 *splat_a, const float2 *aabb, const int32_t *radii, int tx, int ty,
                        uint2 *rect, uint32_t *count, uint32_t *block_sums, hipStream_t s);

@@ -23,7 +23,8 @@ __device__ constexpr float GW[11] = {0.0010283801f, 0.0075987581f, 0.0360007721f
                                      0.2660117249f, 0.2130055377f, 0.1093606895f, 0.0360007721f, 0.0075987581f,
                                      0.0010283801f};
 
-__global__ __launch_bounds__(256) void loss_kernel(LossArgs a) {
+// The loss of the 32x16 output tile at (X0, Y0); every thread of the workgroup calls it.
+__device__ inline void loss_tile(const LossArgs &a, int X0, int Y0) {
     __shared__ float in_g[IN_H][IN_W], in_r[IN_H][IN_W];
     __shared__ float mh[5][IN_H][MH_W];  // reused for the horizontal back-convolution
     __shared__ float fld[3][F_H][MH_W];
@@ -31,7 +32,6 @@ __global__ __launch_bounds__(256) void loss_kernel(LossArgs a) {
     float(*hb)[F_H][LT_W] = reinterpret_cast<float(*)[F_H][LT_W]>(&mh[0][0][0]);
 
     const int W = (int)a.width, H = (int)a.height;
-    const int X0 = blockIdx.x * LT_W, Y0 = blockIdx.y * LT_H;
     const int tid = threadIdx.x;
     const float C1 = 0.0001f, C2 = 0.0009f;
     const float w = a.ssim_weight;
@@ -151,6 +151,8 @@ __global__ __launch_bounds__(256) void loss_kernel(LossArgs a) {
     }
 }
 
+__global__ __launch_bounds__(256) void loss_kernel(LossArgs a) { loss_tile(a, blockIdx.x * LT_W, blockIdx.y * LT_H); }
+
 // Per-row double sums of the per-pixel loss in x order: the row is staged in LDS with
 // coalesced loads, then one lane adds it up sequentially (the reference's order).
 constexpr int ROW_MAX = 4096;
@@ -187,6 +189,97 @@ __global__ __launch_bounds__(256) void loss_total_kernel(const double *rows, int
         *sum = s;
         *mean = (float)(s / (double)((size_t)W * H));
     }
+}
+
+// ── batched views: one launch per stage over every view's tiles / rows ──────────
+
+__device__ inline int view_of_base(const uint32_t *base, int nv, uint32_t i) {
+    int lo = 0, hi = nv - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ inline LossArgs view_loss_args(const BatchLossArgs &b, int v) {
+    const BatchView &bv = b.tab->views[v];
+    const size_t pb = bv.pix_base;
+    LossArgs a;
+    a.width = bv.cam.width;
+    a.height = bv.cam.height;
+    a.img = b.img + 3 * pb;
+    a.gt = bv.gt;
+    a.ssim_weight = b.tab->ssim_weight;
+    a.inv_n = bv.inv_n;
+    a.v_img = b.v_img + 3 * pb;
+    a.pix_loss = b.pix_loss + pb;
+    a.row_loss = nullptr;
+    a.loss_sum = nullptr;
+    a.loss_mean = nullptr;
+    return a;
+}
+
+// grid = the capacity of loss tiles; workgroups past the batch's count exit
+__global__ __launch_bounds__(256) void loss_batch_kernel(BatchLossArgs b) {
+    const uint32_t i = blockIdx.x;
+    if (i >= b.tab->total_loss_blocks) return;
+    const int v = view_of_base(b.tab->loss_base, b.num_views, i);
+    const LossArgs a = view_loss_args(b, v);
+    const uint32_t local = i - b.tab->loss_base[v];
+    const uint32_t nbx = (a.width + LT_W - 1) / LT_W;
+    loss_tile(a, (int)(local % nbx) * LT_W, (int)(local / nbx) * LT_H);
+}
+
+__global__ __launch_bounds__(256) void loss_rows_batch_kernel(BatchLossArgs b) {
+    const uint32_t q = blockIdx.x;
+    if (q >= b.tab->total_rows) return;
+    const int v = view_of_base(b.tab->row_base, b.num_views, q);
+    const BatchView &bv = b.tab->views[v];
+    const int W = (int)bv.cam.width;
+    const int y = (int)(q - b.tab->row_base[v]);
+    __shared__ float r[ROW_MAX];
+    const float *p = b.pix_loss + bv.pix_base + (size_t)y * W;
+    double s = 0.0;
+    for (int base = 0; base < W; base += ROW_MAX) {
+        const int n = min(ROW_MAX, W - base);
+        __syncthreads();
+        for (int x = threadIdx.x; x < n; x += 256) r[x] = p[base + x];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int x = 0; x < n; x++) s += (double)r[x];
+    }
+    if (threadIdx.x == 0) b.row_loss[q] = s;
+}
+
+// one workgroup per view: the sum of its rows in y order, and the mean
+__global__ __launch_bounds__(256) void loss_total_batch_kernel(BatchLossArgs b) {
+    const int v = blockIdx.x;
+    const BatchView &bv = b.tab->views[v];
+    const int H = (int)bv.cam.height, W = (int)bv.cam.width;
+    const double *rows = b.row_loss + b.tab->row_base[v];
+    __shared__ double r[2048];
+    double s = 0.0;
+    for (int base = 0; base < H; base += 2048) {
+        const int n = min(2048, H - base);
+        __syncthreads();
+        for (int y = threadIdx.x; y < n; y += 256) r[y] = rows[base + y];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int y = 0; y < n; y++) s += r[y];
+    }
+    if (threadIdx.x == 0) {
+        b.sc->loss_sum[v] = s;
+        b.sc->loss_mean[v] = (float)(s / (double)((size_t)W * H));
+    }
+}
+
+void launch_batch_loss(const BatchLossArgs &b, hipStream_t s) {
+    if (b.num_views <= 0) return;
+    loss_batch_kernel<<<b.max_loss_blocks, 256, 0, s>>>(b);
+    loss_rows_batch_kernel<<<b.max_rows, 256, 0, s>>>(b);
+    loss_total_batch_kernel<<<b.num_views, 256, 0, s>>>(b);
 }
 
 void launch_loss(const LossArgs &a, hipStream_t s) {

@@ -1,243 +1,102 @@
-// Per-tile front-to-back alpha compositing (forward) and its backward.
-//
-// One 256-thread workgroup per 16x16 tile; each wave64 owns an 8x8 quadrant so that a
-// wave's pixels are spatially tight (better ballot skipping in the backward). Gaussian
-// records of the tile's sorted list are staged through LDS in batches.
-//
-// Backward without global atomics: per gaussian of a batch, the wave's contributions
-// are summed with DPP (wave_sum) only when some lane contributed (ballot), written to a
-// per-wave LDS partial, and the four partials are summed in wave order into the
-// intersection's gradient slot. Slots are laid out in the pre-sort order of the
-// intersections (gaussian-major, tiles row-major), so the per-gaussian reduction reads
-// each gaussian's slots contiguously and in the reference's tile order.
-#include "common.h"
-#include "kernels.h"
+// Rasterizer kernels: one workgroup per tile for a single view, and persistent
+// workgroups that pull (view, tile) items from a device work queue for a batch of views
+// of mixed resolution. Both run the tile bodies of raster_tile.h.
+#include "raster_tile.h"
 
 namespace msplat {
 
-constexpr int FWD_BATCH = 256;
-constexpr int BWD_BATCH = 128;
+namespace {
 
-__device__ inline void tile_pixel(int tile, int tiles_x, int &tx, int &ty, int &px, int &py) {
-    tx = tile % tiles_x;
-    ty = tile / tiles_x;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    px = tx * TILE + (w & 1) * 8 + (lane & 7);
-    py = ty * TILE + (w >> 1) * 8 + (lane >> 3);
+__device__ inline TileData tile_data(const RasterArgs &a) {
+    return TileData{a.tile_start, a.gid, a.splat_a, a.splat_b, a.splat_blue, a.rect, a.offsets,
+                    a.slot_a,     a.slot_b, a.slot_c};
 }
 
-// nd_rasterize_forward (core/cpu/msplat_cpu.cpp:574-626)
+__device__ inline TileView tile_view(const RasterArgs &a) {
+    return TileView{a.width, a.height, a.tiles_x, {a.bg[0], a.bg[1], a.bg[2]}, a.image, a.final_T, a.final_idx,
+                    a.v_image};
+}
+
+__device__ inline TileData tile_data(const BatchRasterArgs &a) {
+    return TileData{a.tile_start, a.gid, a.splat_a, a.splat_b, a.splat_blue, a.rect, a.offsets,
+                    a.slot_a,     a.slot_b, a.slot_c};
+}
+
+// View v of a batch: per-pixel buffers offset to the view's first pixel
+__device__ inline TileView tile_view(const BatchRasterArgs &a, int v) {
+    const BatchView &bv = a.tab->views[v];
+    const size_t pb = bv.pix_base;
+    return TileView{bv.cam.width,
+                    bv.cam.height,
+                    bv.cam.tiles_x,
+                    {bv.bg[0], bv.bg[1], bv.bg[2]},
+                    a.image ? a.image + 3 * pb : nullptr,
+                    a.final_T + pb,
+                    a.final_idx + pb,
+                    a.v_image ? a.v_image + 3 * pb : nullptr};
+}
+
+// The view of global tile t: the last v with tile_base[v] <= t
+__device__ inline int view_of(const uint32_t *tb, int nv, uint32_t t) {
+    int lo = 0, hi = nv - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tb[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Next item of the queue for the whole workgroup (one atomic per item)
+__device__ inline uint32_t next_item(uint32_t *queue, uint32_t *slot) {
+    if (threadIdx.x == 0) *slot = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint32_t it = *slot;
+    __syncthreads();
+    return it;
+}
+
+}  // namespace
+
 __global__ __launch_bounds__(256) void raster_forward_kernel(RasterArgs a) {
-#pragma clang fp contract(fast)
-    __shared__ float4 la[FWD_BATCH], lb[FWD_BATCH];
-    __shared__ float lc[FWD_BATCH];
-    int tx, ty, px, py;
-    const int tile = blockIdx.x;
-    tile_pixel(tile, a.tiles_x, tx, ty, px, py);
-    const bool inside = px < (int)a.width && py < (int)a.height;
-    const int s = a.tile_start[tile], e = a.tile_start[tile + 1];
-    const float fx = (float)px, fy = (float)py;
-    float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f;
-    int last = s - 1;
-    bool done = !inside;
-    for (int base = s; base < e; base += FWD_BATCH) {
-        if (__syncthreads_count(done) == 256) break;
-        const int k = base + threadIdx.x;
-        if (k < e) {
-            const int g = a.gid[k];
-            la[threadIdx.x] = a.splat_a[g];
-            lb[threadIdx.x] = a.splat_b[g];
-            lc[threadIdx.x] = a.splat_blue[g];
-        }
-        __syncthreads();
-        if (__ballot(!done) != 0) {
-            const int n = min(FWD_BATCH, e - base);
-            for (int j = 0; j < n; j++) {
-                // predicated compositing step: skip (out of range / too faint), accept, or
-                // terminate (transmittance would drop to T_MIN: last = k - 1, as the CPU)
-                const float4 A = la[j], B = lb[j];
-                const float dx = A.x - fx, dy = A.y - fy;
-                const float sig = 0.5f * (A.z * dx * dx + B.x * dy * dy) + A.w * dx * dy;
-                const float alpha = fminf(ALPHA_MAX, B.y * __expf(-sig));
-                const bool valid = !done && sig >= 0.f && sig < SIGMA_MAX && alpha >= ALPHA_MIN;
-                const float nT = T * (1.f - alpha);
-                const bool term = valid && nT <= T_MIN;
-                const bool accept = valid && !term;
-                const float vis = accept ? alpha * T : 0.f;
-                cr += fmaxf(B.z + 0.5f, 0.f) * vis;
-                cg += fmaxf(B.w + 0.5f, 0.f) * vis;
-                cb += fmaxf(lc[j] + 0.5f, 0.f) * vis;
-                T = accept ? nT : T;
-                last = term ? base + j - 1 : (accept ? base + j : last);
-                done = done || term;
-                if ((j & 7) == 7 && __ballot(!done) == 0) break;
-            }
-        }
-    }
-    if (inside) {
-        const size_t pid = (size_t)py * a.width + px;
-        a.final_T[pid] = T;
-        a.final_idx[pid] = last;
-        a.image[3 * pid] = fminf(fmaxf(a.bg[0] * T + cr, 0.f), 1.f);
-        a.image[3 * pid + 1] = fminf(fmaxf(a.bg[1] * T + cg, 0.f), 1.f);
-        a.image[3 * pid + 2] = fminf(fmaxf(a.bg[2] * T + cb, 0.f), 1.f);
-    }
+    forward_tile(tile_view(a), tile_data(a), blockIdx.x, blockIdx.x);
 }
-
-// rasterize_backward (core/cpu/msplat_cpu.cpp:959-1031), slot values per intersection:
-// a = (v_x, v_y, v_conic.x, v_conic.y), b = (v_conic.z, v_r, v_g, v_b), c = v_logit.
-//
-// PPT pixels per thread: 4/PPT waves per tile, wave w owning rows [4 PPT w, 4 PPT (w+1))
-// and lane l the pixels (l % 16, 4 PPT w + l / 16 + 4 k), k < PPT.
-//
-// Per pixel the back-to-front recursion keeps T and one running scalar
-//   S = <buffer + T_final bg, v_out>,  so  v_alpha = T <rgb, v_out> - S / (1 - alpha),
-// and the per-gaussian gradients are assembled after the reduction from sufficient
-// statistics of v_sigma = -alpha v_alpha:
-//   v_conic = 1/2 (sum vs dx^2, sum vs dx dy, sum vs dy^2),
-//   v_xy = (a sum vs dx + b sum vs dy, b sum vs dx + c sum vs dy),
-//   v_logit = -(1 - o) sum vs, v_rgb = sum fac v_out (channels with raw + 0.5 >= 0).
-// A lane first adds its PPT pixels; each value is then summed over the wave with DPP
-// (rows, then row broadcasts into lane 63), and lane 63 writes the wave's total to LDS,
-// to be added over the waves in fixed order when the batch's slots are written. One
-// partial per wave keeps the LDS footprint small enough for several waves per SIMD.
-constexpr int NQ = 9;  // sum vs, vs dx, vs dy, vs dx^2, vs dx dy, vs dy^2, fac vr, fac vg, fac vb
 
 template <int PPT>
 __global__ __launch_bounds__(256 / PPT) void raster_backward_kernel(RasterArgs a) {
-#pragma clang fp contract(fast)
-    constexpr int NT = 256 / PPT, NW = NT / 64, BATCH = 128;
-    __shared__ float4 la[BATCH], lb[BATCH];
-    __shared__ float lc[BATCH];
-    __shared__ uint32_t lslot[BATCH];
-    __shared__ float part[NW][NQ][BATCH];
-    __shared__ int smax[NW];
-    const int tile = blockIdx.x;
-    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int s = a.tile_start[tile], e = a.tile_start[tile + 1];
-    if (s == e) return;
-    const int px = tx * TILE + (lane & 15);
-    float fy[PPT], T[PPT], S[PPT], vr[PPT], vg[PPT], vb[PPT];
-    int last[PPT];
-    int m = s - 1;
-#pragma unroll
-    for (int k = 0; k < PPT; k++) {
-        const int py = ty * TILE + 4 * PPT * w + (lane >> 4) + 4 * k;
-        fy[k] = (float)py;
-        T[k] = 0.f; S[k] = 0.f; vr[k] = 0.f; vg[k] = 0.f; vb[k] = 0.f;
-        last[k] = s - 1;
-        if (px < (int)a.width && py < (int)a.height) {
-            const size_t pid = (size_t)py * a.width + px;
-            T[k] = a.final_T[pid];
-            last[k] = min(a.final_idx[pid], e - 1);
-            vr[k] = a.v_image[3 * pid];
-            vg[k] = a.v_image[3 * pid + 1];
-            vb[k] = a.v_image[3 * pid + 2];
-            S[k] = T[k] * (a.bg[0] * vr[k] + a.bg[1] * vg[k] + a.bg[2] * vb[k]);
-        }
-        m = max(m, last[k]);
-    }
-    // the block's furthest contributing intersection: nothing behind it is visited
-    for (int d = 32; d >= 1; d >>= 1) m = max(m, __shfl_xor(m, d));
-    if (lane == 0) smax[w] = m;
-    __syncthreads();
-    int kmax = smax[0];
-#pragma unroll
-    for (int i = 1; i < NW; i++) kmax = max(kmax, smax[i]);
-    const float fx = (float)px;
+    backward_tile<PPT>(tile_view(a), tile_data(a), blockIdx.x, blockIdx.x);
+}
 
-    // slots of intersections behind every pixel's last contributor are zero
-    for (int k = kmax + 1 + threadIdx.x; k < e; k += NT) {
-        const int g = a.gid[k];
-        const uint2 r = a.rect[g];
-        const int x0 = r.x & 0xFFFF, x1 = r.x >> 16, y0 = r.y & 0xFFFF;
-        const uint32_t p = a.offsets[g] + (uint32_t)((ty - y0) * (x1 - x0) + (tx - x0));
-        a.slot_a[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-        a.slot_b[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-        a.slot_c[p] = 0.f;
+// Persistent forward over every (view, tile) of a batch. Every workgroup loops until
+// the queue passes the batch's tile count, so the grid drains on its own.
+__global__ __launch_bounds__(256) void raster_forward_batch_kernel(BatchRasterArgs a) {
+    __shared__ uint32_t tb[MAX_VIEWS + 1];
+    __shared__ uint32_t item_slot;
+    const int nv = a.num_views;
+    for (int i = threadIdx.x; i <= nv; i += 256) tb[i] = a.tab->tile_base[i];
+    const uint32_t total = a.tab->total_tiles;
+    const TileData d = tile_data(a);
+    for (;;) {
+        const uint32_t t = next_item(a.queue, &item_slot);
+        if (t >= total) break;
+        const int v = view_of(tb, nv, t);
+        forward_tile(tile_view(a, v), d, (int)t, (int)(t - tb[v]));
     }
+}
 
-    for (int end = kmax + 1; end > s; end -= BATCH) {
-        const int start = max(s, end - BATCH);
-        const int n = end - start;
-        __syncthreads();
-        for (int i = threadIdx.x; i < n; i += NT) {
-            const int g = a.gid[start + i];
-            la[i] = a.splat_a[g];
-            lb[i] = a.splat_b[g];
-            lc[i] = a.splat_blue[g];
-            const uint2 r = a.rect[g];
-            const int x0 = r.x & 0xFFFF, x1 = r.x >> 16, y0 = r.y & 0xFFFF;
-            lslot[i] = a.offsets[g] + (uint32_t)((ty - y0) * (x1 - x0) + (tx - x0));
-        }
-        for (int i = threadIdx.x; i < NW * NQ * BATCH; i += NT) (&part[0][0][0])[i] = 0.f;
-        __syncthreads();
-        for (int j = n - 1; j >= 0; j--) {
-            const int kk = start + j;
-            const float4 A = la[j], B = lb[j];
-            const float cr = fmaxf(B.z + 0.5f, 0.f), cg = fmaxf(B.w + 0.5f, 0.f), cbl = fmaxf(lc[j] + 0.5f, 0.f);
-            float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f, q4 = 0.f, q5 = 0.f, q6 = 0.f, q7 = 0.f, q8 = 0.f;
-            bool contrib = false;
-#pragma unroll
-            for (int k = 0; k < PPT; k++) {
-                if (__ballot(kk <= last[k]) == 0) continue;  // the whole wave is past this pixel row
-                const float dx = A.x - fx, dy = A.y - fy[k];
-                const float sig = 0.5f * (A.z * dx * dx + B.x * dy * dy) + A.w * dx * dy;
-                const float alpha = fminf(ALPHA_MAX, B.y * __expf(-sig));
-                const bool valid = kk <= last[k] && sig >= 0.f && sig < SIGMA_MAX && alpha >= ALPHA_MIN;
-                contrib = contrib || valid;
-                // predicated: an invalid pair leaves T and S unchanged and adds zeros
-                const float ra = __builtin_amdgcn_rcpf(1.f - alpha);
-                const float Tn = T[k] * ra;
-                const float fac = valid ? alpha * Tn : 0.f;
-                const float cv = cr * vr[k] + cg * vg[k] + cbl * vb[k];
-                const float v_alpha = Tn * cv - S[k] * ra;
-                T[k] = valid ? Tn : T[k];
-                S[k] += fac * cv;
-                q6 += fac * vr[k];
-                q7 += fac * vg[k];
-                q8 += fac * vb[k];
-                const float vs = (valid && alpha < ALPHA_MAX) ? -alpha * v_alpha : 0.f;
-                const float vsx = vs * dx, vsy = vs * dy;
-                q0 += vs;
-                q1 += vsx;
-                q2 += vsy;
-                q3 += vsx * dx;
-                q4 += vsx * dy;
-                q5 += vsy * dy;
-            }
-            if (__ballot(contrib) != 0) {
-#define MSPLAT_WSUM(x) rows_to_lane63(row_sum16(x))
-                const float r0 = MSPLAT_WSUM(q0), r1 = MSPLAT_WSUM(q1), r2 = MSPLAT_WSUM(q2), r3 = MSPLAT_WSUM(q3),
-                            r4 = MSPLAT_WSUM(q4), r5 = MSPLAT_WSUM(q5), r6 = MSPLAT_WSUM(q6), r7 = MSPLAT_WSUM(q7),
-                            r8 = MSPLAT_WSUM(q8);
-#undef MSPLAT_WSUM
-                if (lane == 63) {
-                    part[w][0][j] = r0; part[w][1][j] = r1; part[w][2][j] = r2;
-                    part[w][3][j] = r3; part[w][4][j] = r4; part[w][5][j] = r5;
-                    part[w][6][j] = r6; part[w][7][j] = r7; part[w][8][j] = r8;
-                }
-            }
-        }
-        __syncthreads();
-        for (int j = threadIdx.x; j < n; j += NT) {
-            float q[NQ];
-#pragma unroll
-            for (int i = 0; i < NQ; i++) {
-                float acc = part[0][i][j];
-#pragma unroll
-                for (int r = 1; r < NW; r++) acc += part[r][i][j];
-                q[i] = acc;
-            }
-            const float4 A = la[j], B = lb[j];
-            const float ox = A.z * q[1] + A.w * q[2], oy = A.w * q[1] + B.x * q[2];
-            const bool mr = B.z + 0.5f >= 0.f, mg = B.w + 0.5f >= 0.f, mb = lc[j] + 0.5f >= 0.f;
-            const uint32_t p = lslot[j];
-            a.slot_a[p] = make_float4(ox, oy, 0.5f * q[3], 0.5f * q[4]);
-            a.slot_b[p] = make_float4(0.5f * q[5], mr ? q[6] : 0.f, mg ? q[7] : 0.f, mb ? q[8] : 0.f);
-            a.slot_c[p] = -(1.f - B.y) * q[0];
-        }
+template <int PPT>
+__global__ __launch_bounds__(256 / PPT) void raster_backward_batch_kernel(BatchRasterArgs a) {
+    __shared__ uint32_t tb[MAX_VIEWS + 1];
+    __shared__ uint32_t item_slot;
+    const int nv = a.num_views;
+    for (int i = threadIdx.x; i <= nv; i += 256 / PPT) tb[i] = a.tab->tile_base[i];
+    const uint32_t total = a.tab->total_tiles;
+    const TileData d = tile_data(a);
+    for (;;) {
+        const uint32_t t = next_item(a.queue + 1, &item_slot);
+        if (t >= total) break;
+        const int v = view_of(tb, nv, t);
+        backward_tile<PPT>(tile_view(a, v), d, (int)t, (int)(t - tb[v]));
     }
 }
 
@@ -255,6 +114,56 @@ void launch_raster_backward(const RasterArgs &a, int ppt, hipStream_t s) {
         raster_backward_kernel<2><<<nt, 128, 0, s>>>(a);
     else
         raster_backward_kernel<1><<<nt, 256, 0, s>>>(a);
+}
+
+namespace {
+
+int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+template <typename K>
+int resident_blocks(K kernel, int threads) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
+    return per_cu * device_cus();
+}
+
+}  // namespace
+
+int batch_raster_grid(bool backward, int ppt) {
+    static int g[4] = {0, 0, 0, 0};
+    const int i = !backward ? 0 : ppt == 4 ? 3 : ppt == 2 ? 2 : 1;
+    if (!g[i]) {
+        if (i == 0) g[i] = resident_blocks(raster_forward_batch_kernel, 256);
+        else if (i == 1) g[i] = resident_blocks(raster_backward_batch_kernel<1>, 256);
+        else if (i == 2) g[i] = resident_blocks(raster_backward_batch_kernel<2>, 128);
+        else g[i] = resident_blocks(raster_backward_batch_kernel<4>, 64);
+    }
+    return g[i];
+}
+
+void launch_batch_raster_forward(const BatchRasterArgs &a, hipStream_t s) {
+    fill_u32<<<1, 1, 0, s>>>(a.queue, 0u);
+    raster_forward_batch_kernel<<<batch_raster_grid(false, 1), 256, 0, s>>>(a);
+}
+
+void launch_batch_raster_backward(const BatchRasterArgs &a, int ppt, hipStream_t s) {
+    fill_u32<<<1, 1, 0, s>>>(a.queue + 1, 0u);
+    if (ppt == 4)
+        raster_backward_batch_kernel<4><<<batch_raster_grid(true, 4), 64, 0, s>>>(a);
+    else if (ppt == 2)
+        raster_backward_batch_kernel<2><<<batch_raster_grid(true, 2), 128, 0, s>>>(a);
+    else
+        raster_backward_batch_kernel<1><<<batch_raster_grid(true, 1), 256, 0, s>>>(a);
 }
 
 }  // namespace msplat

@@ -51,6 +51,14 @@ class Grads(C.Structure):
                 ("v_features_dc", C.c_void_p), ("v_features_rest", C.c_void_p), ("v_opacities", C.c_void_p)]
 
 
+class Views(C.Structure):
+    _fields_ = [("num_views", C.c_int32), ("cameras", C.POINTER(Camera)), ("backgrounds", C.POINTER(C.c_float)),
+                ("targets", C.POINTER(C.c_void_p))]
+
+
+MAX_BATCH_VIEWS = 64
+
+
 class Frame(C.Structure):
     _fields_ = [("num_points", C.c_int32), ("num_tiles", C.c_int32), ("num_isects", C.c_int64),
                 ("splat_a", C.c_void_p), ("splat_b", C.c_void_p), ("splat_blue", C.c_void_p),
@@ -110,6 +118,14 @@ def lib():
             "msplat_hip_stage_name": ([C.c_int], C.c_char_p),
             "msplat_hip_set_tuning": ([vp, C.c_int, C.c_int], C.c_int),
             "msplat_hip_get_tuning": ([vp, C.c_int, C.POINTER(C.c_int)], C.c_int),
+            "msplat_hip_render_batch": ([vp, C.POINTER(Gaussians), C.POINTER(Views), C.POINTER(vp),
+                                         C.POINTER(C.c_uint64)], C.c_int),
+            "msplat_hip_train_step_batch": ([vp, C.POINTER(Gaussians), C.POINTER(Views), f32, C.POINTER(Adam),
+                                             C.POINTER(GradStats), fp], C.c_int),
+            "msplat_hip_render_backward_batch": ([vp, C.POINTER(Gaussians), C.POINTER(Views), vp,
+                                                  C.POINTER(Grads)], C.c_int),
+            "msplat_hip_set_batch_graph": ([vp, C.c_int], C.c_int),
+            "msplat_hip_batch_graph_stats": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -170,6 +186,14 @@ class Context:
         n = C.c_int()
         check(lib().msplat_hip_drain_stage_times(self.ptr, ms, calls, 16, C.byref(n)))
         return {lib().msplat_hip_stage_name(i).decode(): (ms[i], calls[i]) for i in range(n.value)}
+
+    def set_batch_graph(self, on: bool = True):
+        check(lib().msplat_hip_set_batch_graph(self.ptr, 1 if on else 0))
+
+    def batch_graph_stats(self) -> tuple:
+        c, r = C.c_int64(), C.c_int64()
+        check(lib().msplat_hip_batch_graph_stats(self.ptr, C.byref(c), C.byref(r)))
+        return c.value, r.value
 
     def download(self, ptr, shape, dtype) -> np.ndarray:
         out = np.empty(shape, dtype)

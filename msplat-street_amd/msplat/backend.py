@@ -11,7 +11,8 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._abi import Adam, Context, DeviceArray, GradStats, Gaussians, check, floats, lib, make_camera_struct
+from ._abi import (Adam, Context, DeviceArray, GradStats, Gaussians, Grads, Views, check, floats, lib,
+                   make_camera_struct)
 
 KEYS = ("means", "scales", "quats", "dc", "rest", "opac")
 WIDTH = {"means": 3, "scales": 3, "quats": 4, "dc": 3, "opac": 1}
@@ -166,9 +167,9 @@ def adam_coeffs(lrs, step_count, beta1=BETA1, beta2=BETA2):
 
 
 def train_step(ctx: Context, dg: DeviceGaussians, cam: dict, bg, gt: DeviceArray, step_sizes, bc2s,
-               ssim_weight=0.2, read_loss=True) -> float | None:
+               ssim_weight=0.2, read_loss=True, beta1=BETA1, beta2=BETA2, eps=EPS) -> float | None:
     g, c = dg.struct(), make_camera_struct(cam, dg.degree)
-    a, s = dg.adam(step_sizes, bc2s), dg.stats()
+    a, s = dg.adam(step_sizes, bc2s, beta1, beta2, eps), dg.stats()
     H, W = cam["H"], cam["W"]
     inv_n = np.float32(1.0) / np.float32(H * W * 3)
     inv_max = np.float32(1.0) / np.float32(max(H, W))
@@ -177,3 +178,80 @@ def train_step(ctx: Context, dg: DeviceGaussians, cam: dict, bg, gt: DeviceArray
                                       float(inv_n), C.byref(a), C.byref(s), float(inv_max),
                                       C.c_void_p(dg.radii.ptr), C.byref(loss) if read_loss else None))
     return loss.value if read_loss else None
+
+
+class BatchViews:
+    """The C views record of a batch (cameras, backgrounds, optional target pointers);
+    keeps the ctypes arrays alive as long as it lives."""
+
+    def __init__(self, cams, degree: int, bgs, targets=None):
+        n = len(cams)
+        self.cams = list(cams)
+        self.c_cams = (_abi.Camera * n)(*[make_camera_struct(c, degree) for c in cams])
+        bg = np.asarray(bgs, np.float32).reshape(-1, 3)
+        if bg.shape[0] == 1:
+            bg = np.repeat(bg, n, 0)
+        self.c_bgs = (C.c_float * (3 * n))(*bg.reshape(-1).tolist())
+        self.c_targets = (C.c_void_p * n)(*[t.ptr if isinstance(t, DeviceArray) else t for t in targets]) \
+            if targets is not None else None
+        self.rec = Views()
+        self.rec.num_views = n
+        self.rec.cameras = self.c_cams
+        self.rec.backgrounds = self.c_bgs
+        self.rec.targets = self.c_targets if self.c_targets is not None else None
+        self.sizes = [(int(c["H"]), int(c["W"])) for c in cams]
+
+    def offsets(self):
+        out, acc = [], 0
+        for h, w in self.sizes:
+            out.append(acc)
+            acc += h * w
+        return out, acc
+
+
+def render_batch(ctx: Context, dg: DeviceGaussians, views: BatchViews) -> tuple:
+    """Renders every view (syncs); returns (device pointer of the packed images, pixel offsets)."""
+    g = dg.struct()
+    out = C.c_void_p()
+    offs = (C.c_uint64 * len(views.sizes))()
+    check(lib().msplat_hip_render_batch(ctx.ptr, C.byref(g), C.byref(views.rec), C.byref(out), offs))
+    return out.value, list(offs)
+
+
+def render_batch_numpy(ctx: Context, dg: DeviceGaussians, views: BatchViews) -> list:
+    ptr, offs = render_batch(ctx, dg, views)
+    _, total = views.offsets()
+    packed = ctx.download(ptr, (total, 3), np.float32)
+    return [packed[o:o + h * w].reshape(h, w, 3) for o, (h, w) in zip(offs, views.sizes)]
+
+
+def train_step_batch(ctx: Context, dg: DeviceGaussians, views: BatchViews, step_sizes, bc2s, ssim_weight=0.2,
+                     beta1=BETA1, beta2=BETA2, eps=EPS) -> np.ndarray:
+    """One batched step over every view (syncs); returns the per-view losses."""
+    g = dg.struct()
+    a, s = dg.adam(step_sizes, bc2s, beta1, beta2, eps), dg.stats()
+    losses = (C.c_float * len(views.sizes))()
+    check(lib().msplat_hip_train_step_batch(ctx.ptr, C.byref(g), C.byref(views.rec), ssim_weight, C.byref(a),
+                                            C.byref(s), losses))
+    return np.array(losses[:], np.float32)
+
+
+def render_backward_batch(ctx: Context, dg: DeviceGaussians, views: BatchViews, v_images: DeviceArray) -> dict:
+    """Gradients of sum_v <image_v, v_image_v> for the last render_batch, summed over the views."""
+    n = dg.n
+    out = {k: DeviceArray(ctx, (max(n, 1), dg.widths[k])) for k in KEYS}
+    gr = Grads(*[out[k].ptr for k in KEYS])
+    check(lib().msplat_hip_render_backward_batch(ctx.ptr, C.byref(dg.struct()), C.byref(views.rec),
+                                                 C.c_void_p(v_images.ptr), C.byref(gr)))
+    return {k: out[k].numpy()[:n] for k in KEYS}
+
+
+def render_backward(ctx: Context, dg: DeviceGaussians, cam: dict, bg, v_img: DeviceArray) -> dict:
+    """Gradients of <image, v_img> for the last render of `cam` on this context."""
+    n = dg.n
+    out = {k: DeviceArray(ctx, (max(n, 1), dg.widths[k])) for k in KEYS}
+    gr = Grads(*[out[k].ptr for k in KEYS])
+    g, c = dg.struct(), make_camera_struct(cam, dg.degree)
+    check(lib().msplat_hip_render_backward(ctx.ptr, C.byref(g), C.byref(c), floats(bg), C.c_void_p(v_img.ptr),
+                                           C.byref(gr)))
+    return {k: out[k].numpy()[:n] for k in KEYS}
