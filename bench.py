@@ -32,6 +32,7 @@ METRIC = "train iter/s @475k Gaussians 1264x832; batched views/s; served p99 lat
 N_GAUSSIANS = 475_115
 W_IMG, H_IMG, FOCAL = 1264, 832, 900.0
 N_VIEWS = 64
+BG = (0.6130, 0.0101, 0.3984)
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 # Algorithmic bytes per unit (minimum fp32 reads + writes), per kernel class
@@ -68,6 +69,126 @@ def build_workload(ctx, seed=0):
         ctx.upload(t.ptr, ctx.download(p, (H_IMG, W_IMG, 3), np.float32))
         targets.append(t)
     return dg, gt, cams, targets
+
+
+BATCH_SIZES = [(640, 360), (1264, 832), (1920, 1080), (2560, 1440)]
+BATCH_N = 1_000_000
+BATCH_VIEWS = 16
+BATCH_POOL = 32
+
+
+def psnr(a, b):
+    mse = float(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2))
+    return 10.0 * np.log10(1.0 / max(mse, 1e-20))
+
+
+def bench_batched(steps=8, warmup=2, seed=10):
+    """Batched views: 1,000,000 SH3 Gaussians (street recipe), 16 views per launch drawn
+    from a seeded pool of 32 (pose on the street path, resolution uniform over 640x360,
+    1264x832, 1920x1080, 2560x1440; fx = 900 W / 1264). The batched step (one
+    msplat_hip_train_step_batch, replayed from its captured graph; gradients summed over
+    the 16 views, one Adam step) against the plain form (16 msplat_hip_train_step, one
+    view per launch), and msplat_hip_render_batch against 16 msplat_hip_render, on the
+    same views. ΔPSNR compares the batched and plain renders of every pooled view."""
+    from msplat._abi import Context, DeviceArray
+    from msplat.backend import (BatchViews, DeviceGaussians, adam_coeffs, render, render_batch, render_batch_numpy,
+                                render_numpy, train_step, train_step_batch)
+    from msplat.camera import make_camera
+    from msplat.scenes import street_cameras, street_points
+    ctx = Context(0)
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    pts, rgb = street_points(seed, BATCH_N)
+    gpts, grgb = street_points(seed + 1, BATCH_N)
+    gt_set = DeviceGaussians.from_points(ctx, gpts, grgb, degree=3)
+    gt_set.params["opac"].upload(rng.normal(1.5, 1.0, (BATCH_N, 1)).astype(np.float32))
+    gt_set.params["rest"].upload(rng.normal(0, 0.05, (BATCH_N, 45)).astype(np.float32))
+    pool, targets = [], []
+    sizes = [BATCH_SIZES[i] for i in rng.integers(0, len(BATCH_SIZES), BATCH_POOL)]
+    path = street_cameras(BATCH_POOL, 64, 64)
+    for k, (W, H) in enumerate(sizes):
+        f = 900.0 * W / 1264.0
+        cam = make_camera(path[k]["cam_to_world"], W, H, f, f, dtu=3)
+        pool.append(cam)
+        t = DeviceArray(ctx, (H, W, 3))
+        ctx.upload(t.ptr, render_numpy(ctx, gt_set, cam, BG))
+        targets.append(t)
+    del gt_set
+    bg = [BG]
+    lrs = [1.6e-4, 5e-3, 1e-3, 2.5e-3, 1.25e-4, 5e-2]
+    picks = [rng.permutation(BATCH_POOL)[:BATCH_VIEWS] for _ in range(warmup + steps)]
+    mpx = [sum(pool[i]["W"] * pool[i]["H"] for i in p) / 1e6 for p in picks]
+
+    def fresh():
+        return DeviceGaussians.from_points(ctx, pts, rgb, degree=3)
+
+    # batched training, graph replays
+    dg = fresh()
+    ctx.set_batch_graph(True)
+    for t, p in enumerate(picks[:warmup], start=1):
+        s_, b_ = adam_coeffs(lrs, t)
+        train_step_batch(ctx, dg, BatchViews([pool[i] for i in p], 3, bg, [targets[i] for i in p]), s_, b_)
+    ctx.sync()
+    tb = time.perf_counter()
+    for t, p in enumerate(picks[warmup:], start=warmup + 1):
+        s_, b_ = adam_coeffs(lrs, t)
+        train_step_batch(ctx, dg, BatchViews([pool[i] for i in p], 3, bg, [targets[i] for i in p]), s_, b_)
+    ctx.sync()
+    batched_s = time.perf_counter() - tb
+    captures, replays = ctx.batch_graph_stats()
+    # plain training, one view per launch
+    dg = fresh()
+    t = 0
+    for p in picks[:warmup]:
+        for i in p:
+            t += 1
+            s_, b_ = adam_coeffs(lrs, t)
+            train_step(ctx, dg, pool[i], BG, targets[i], s_, b_, read_loss=False)
+    ctx.sync()
+    tp = time.perf_counter()
+    for p in picks[warmup:]:
+        for i in p:
+            t += 1
+            s_, b_ = adam_coeffs(lrs, t)
+            train_step(ctx, dg, pool[i], BG, targets[i], s_, b_, read_loss=False)
+    ctx.sync()
+    plain_s = time.perf_counter() - tp
+    # rendering: batched vs plain on the same views, and PSNR of both against the targets
+    bvs = [BatchViews([pool[i] for i in p], 3, bg) for p in picks]
+    for b in bvs[:warmup]:
+        render_batch(ctx, dg, b)
+    ctx.sync()
+    tr0 = time.perf_counter()
+    for b in bvs[warmup:]:
+        render_batch(ctx, dg, b)
+    ctx.sync()
+    rb_s = time.perf_counter() - tr0
+    tr1 = time.perf_counter()
+    for p in picks[warmup:]:
+        for i in p:
+            render(ctx, dg, pool[i], BG)
+    ctx.sync()
+    rp_s = time.perf_counter() - tr1
+    dpsnr = 0.0
+    whole = BatchViews(pool[:BATCH_VIEWS], 3, bg)
+    for img, cam, tgt in zip(render_batch_numpy(ctx, dg, whole), pool[:BATCH_VIEWS], targets[:BATCH_VIEWS]):
+        ref = tgt.numpy()
+        dpsnr = max(dpsnr, abs(psnr(img, ref) - psnr(render_numpy(ctx, dg, cam, BG), ref)))
+    views = BATCH_VIEWS * steps
+    out = {
+        "workload": "1,000,000 SH3 Gaussians, 16 views per launch from a pool of 32 (resolutions uniform over "
+                    "640x360/1264x832/1920x1080/2560x1440)",
+        "mean_mpx_per_launch": round(float(np.mean(mpx[warmup:])), 2),
+        "train_views_per_s": {"batched": round(views / batched_s, 2), "plain": round(views / plain_s, 2),
+                              "ratio": round(plain_s / batched_s, 3)},
+        "render_views_per_s": {"batched": round(views / rb_s, 2), "plain": round(views / rp_s, 2),
+                               "ratio": round(rp_s / rb_s, 3)},
+        "delta_psnr_db": round(dpsnr, 4),
+        "graph": {"captures": captures, "replays": replays},
+        "launches_timed": steps, "setup_s": round(tb - t0, 1),
+    }
+    ctx.close()
+    return out
 
 
 def cpu_baseline_sample(dg, cams, targets, ctx, step_index, seconds=20.0):
@@ -116,6 +237,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--bwd-ppt", type=int, default=0, help="backward pixels per thread (0: library default)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-views comparison")
+    ap.add_argument("--batched-steps", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -174,6 +297,11 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(alg_bytes),
                 "stages_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in stages.items()}}
+    batched = None
+    if not args.no_batched:
+        log("batched views: 1M gaussians, 16 views per launch")
+        batched = bench_batched(args.batched_steps)
+        log(f"batched: {batched['train_views_per_s']} train views/s, {batched['render_views_per_s']} render views/s")
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU baseline")
@@ -190,6 +318,7 @@ def main():
                    "mean_intersections_per_step": int(m_total / K), "densify_events": tr.densify_log,
                    "final_loss": loss, "setup_s": round(setup_s, 1)},
         "roofline": roofline,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

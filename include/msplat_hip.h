@@ -132,9 +132,9 @@ int msplat_hip_render_backward(msplat_hip_ctx *ctx, const msplat_hip_gaussians *
  * Device pointers to the intermediate buffers of the last render/train call, for tests
  * and tools. Per gaussian: xys (N,2) as float4 records (x, y, conic.x, conic.y) and
  * (conic.z, sigmoid(opacity), r, g) plus blue (N), depths, radii, aabb (N,2), tile
- * counts, prefix offsets. Per intersection (sorted): keys (tile<<32 | depth bits),
- * gaussian ids. tile_start (T+1): tile t owns sorted intersections
- * [tile_start[t], tile_start[t+1]). Per pixel: image, final_T, final_idx (index into
+ * counts, prefix offsets. Per intersection (sorted): keys (tile<<32 | depth bits,
+ * filled by frame_view when MSPLAT_FRAME_KEYS64 is set, else NULL), gaussian ids.
+ * tile_start (T+1): tile t owns sorted intersections [tile_start[t], tile_start[t+1]). Per pixel: image, final_T, final_idx (index into
  * the sorted intersections), dL/d(image). Per gaussian reduced gradients (N,2)/(N,3)/
  * (N,3)/(N) are filled when MSPLAT_FRAME_KEEP_GRADS is set. */
 typedef struct {
@@ -151,7 +151,8 @@ typedef struct {
     float *v_xy, *v_conic, *v_rgb, *v_opacity;
 } msplat_hip_frame;
 int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *out);
-enum { MSPLAT_FRAME_KEEP_GRADS = 1 };
+/* KEEP_GRADS: fill the reduced gradients; KEYS64: materialise isect_keys on frame_view. */
+enum { MSPLAT_FRAME_KEEP_GRADS = 1, MSPLAT_FRAME_KEYS64 = 2 };
 int msplat_hip_set_flags(msplat_hip_ctx *ctx, int flags);
 
 /* Kernel variants picked per context (by the autotuner, msplat/autotune.py); no

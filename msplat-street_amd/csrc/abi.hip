@@ -32,7 +32,7 @@ int msplat::set_error(int code, const char *fmt, ...) {
 
 namespace {
 
-// projection + count scan; returns with ctx->M read back
+// projection (per-block intersection and visible counts)
 int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam) {
     const int n = g->num_points;
     const size_t N = (size_t)(n > 0 ? n : 1);
@@ -47,7 +47,8 @@ int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_
     CHECKED(ctx->grow(ctx->count, 4 * N));
     CHECKED(ctx->grow(ctx->offsets, 4 * N));
     CHECKED(ctx->grow(ctx->block_sums, 4 * (size_t)(nblocks + 1)));
-    CHECKED(ctx->grow(ctx->scalars, 64));
+    CHECKED(ctx->grow(ctx->block_vis, 4 * (size_t)(nblocks + 1)));
+    CHECKED(ctx->grow(ctx->scalars, 128));
     CHECKED(ctx->grow(ctx->big, 40 * N + 64));  // list (4) + slot sums (16 + 16 + 4) + alignment
     ctx->n = n;
     ctx->width = cam->width;
@@ -73,63 +74,104 @@ int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_
     a.radii = (int32_t *)ctx->radii.p;
     a.count = (uint32_t *)ctx->count.p;
     a.block_sums = (uint32_t *)ctx->block_sums.p;
+    a.block_vis = (uint32_t *)ctx->block_vis.p;
     ctx->begin(0);
     launch_project(a, ctx->stream);
     HIPCHECK(hipGetLastError());
     return MSPLAT_OK;
 }
 
-// count scan, M read-back, key generation, sort, tile ranges
+// Count scans and their read-back (M intersections, V visible gaussians), then the
+// visible gaussians sorted by depth, keys emitted in that order, the stable tile sort
+// and the tile ranges.
 int do_bin(msplat_hip_ctx *ctx) {
     const int n = ctx->n;
-    const int nblocks = (n + 255) / 256;
-    if (!ctx->h_total) HIPCHECK(hipHostMalloc((void **)&ctx->h_total, 64, hipHostMallocDefault));
-    launch_scan_blocks((uint32_t *)ctx->block_sums.p, nblocks, ctx->d_total64(), ctx->d_total32(), ctx->stream,
-                       ctx->d_big_count());
+    const uint32_t nblocks = (uint32_t)((n + 255) / 256);
+    if (!ctx->h_total) HIPCHECK(hipHostMalloc((void **)&ctx->h_total, 128, hipHostMallocDefault));
+    CHECKED(ctx->grow(ctx->scan_part, 4 * 3 * scan_partial_words(nblocks > 0 ? nblocks : 1) + 64));
+    uint32_t *part = (uint32_t *)ctx->scan_part.p;
+    const size_t pw = scan_partial_words(nblocks > 0 ? nblocks : 1);
+    launch_scan_u32((uint32_t *)ctx->block_sums.p, nblocks, part,
+                    ScanOut{ctx->d_total64(), ctx->d_total32(), 0xFFFFFFFFu, nullptr, ctx->d_big_count()}, ctx->stream);
+    launch_scan_u32((uint32_t *)ctx->block_vis.p, nblocks, part + pw,
+                    ScanOut{ctx->d_vis64(), ctx->d_vis32(), 0xFFFFFFFFu, nullptr, nullptr}, ctx->stream);
     ctx->end();
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(ctx->h_total, ctx->d_total64(), 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(ctx->h_total, ctx->scalars.p, 64, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     ctx->collect();
-    const uint64_t M = *ctx->h_total;
+    const uint64_t M = ctx->h_total[0];
+    const uint32_t V = (uint32_t)ctx->h_total[5];
     if (M > 0x7FFFFFFFull)
         return set_error(MSPLAT_ERR_RANGE, "more than 2^31 tile intersections (%llu); lower the resolution",
                          (unsigned long long)M);
     ctx->M = (int64_t)M;
-    const size_t Mc = M > 0 ? M : 1;
-    CHECKED(ctx->grow(ctx->keys0, 8 * Mc));
-    CHECKED(ctx->grow(ctx->keys1, 8 * Mc));
+    const size_t Mc = M > 0 ? M : 1, Nc = n > 0 ? (size_t)n : 1;
+    const uint32_t vblocks = (V + 255) / 256;
+    CHECKED(ctx->grow(ctx->keys0, 4 * Mc));
+    CHECKED(ctx->grow(ctx->keys1, 4 * Mc));
     CHECKED(ctx->grow(ctx->vals0, 4 * Mc));
     CHECKED(ctx->grow(ctx->vals1, 4 * Mc));
-    CHECKED(ctx->grow(ctx->hist, 4 * radix_hist_words((uint32_t)Mc)));
+    CHECKED(ctx->grow(ctx->hist, 4 * radix_hist_words((uint32_t)std::max(Mc, Nc))));
     CHECKED(ctx->grow(ctx->digit_total, 4 * 256));
+    CHECKED(ctx->grow(ctx->dk0, 4 * Nc));
+    CHECKED(ctx->grow(ctx->dk1, 4 * Nc));
+    CHECKED(ctx->grow(ctx->dv0, 4 * Nc));
+    CHECKED(ctx->grow(ctx->dv1, 4 * Nc));
+    CHECKED(ctx->grow(ctx->ord_sums, 4 * (size_t)(vblocks + 1)));
     CHECKED(ctx->grow(ctx->tile_start, 4 * (size_t)(ctx->ntiles + 1)));
+    hipStream_t s = ctx->stream;
+    ctx->begin(1);
+    // visible gaussians in id order, stably sorted by depth bits
+    CompactArgs c;
+    c.n = n;
+    c.num_views = 1;
+    c.nbx = (int)nblocks;
+    c.count = (const uint32_t *)ctx->count.p;
+    c.depths = (const float *)ctx->depths.p;
+    c.block_vis = (const uint32_t *)ctx->block_vis.p;
+    c.dkeys = (uint32_t *)ctx->dk0.p;
+    c.dvals = (int32_t *)ctx->dv0.p;
+    launch_compact(c, s);
+    uint32_t *dki = (uint32_t *)ctx->dk0.p, *dko = (uint32_t *)ctx->dk1.p;
+    int32_t *dvi = (int32_t *)ctx->dv0.p, *dvo = (int32_t *)ctx->dv1.p;
+    for (int shift = 0; shift < 32; shift += 8) {
+        launch_radix_pass(dki, dvi, dko, dvo, ctx->d_vis32(), V, shift, (uint32_t *)ctx->hist.p,
+                          (uint32_t *)ctx->digit_total.p, s);
+        std::swap(dki, dko);
+        std::swap(dvi, dvo);
+    }
+    // intersection offsets in depth order, keys (tile ids), stable tile sort, ranges
+    launch_order_sums(dvi, ctx->d_vis32(), V, (const uint32_t *)ctx->count.p, (uint32_t *)ctx->ord_sums.p, s);
+    launch_scan_u32((uint32_t *)ctx->ord_sums.p, vblocks, part + 2 * pw,
+                    ScanOut{ctx->d_order_total64(), ctx->d_order_total32(), 0xFFFFFFFFu, nullptr, nullptr}, s);
     KeygenArgs k;
-    k.n = n;
-    k.tiles_x = ctx->tiles_x;
+    memset(&k, 0, sizeof k);
+    k.list = dvi;
+    k.nvis_ptr = ctx->d_vis32();
+    k.nvis_max = V;
     k.count = (const uint32_t *)ctx->count.p;
-    k.block_bases = (const uint32_t *)ctx->block_sums.p;
     k.rect = (const uint2 *)ctx->rect.p;
-    k.depths = (const float *)ctx->depths.p;
+    k.block_bases = (const uint32_t *)ctx->ord_sums.p;
     k.offsets = (uint32_t *)ctx->offsets.p;
-    k.keys = (uint64_t *)ctx->keys0.p;
+    k.keys = (uint32_t *)ctx->keys0.p;
     k.vals = (int32_t *)ctx->vals0.p;
     k.big_list = ctx->big_list();
     k.big_count = ctx->d_big_count();
-    ctx->begin(1);
-    launch_keygen(k, ctx->stream);
-    uint64_t *kin = (uint64_t *)ctx->keys0.p, *kout = (uint64_t *)ctx->keys1.p;
+    k.tiles_x = ctx->tiles_x;
+    launch_keygen(k, false, s);
+    uint32_t *kin = (uint32_t *)ctx->keys0.p, *kout = (uint32_t *)ctx->keys1.p;
     int32_t *vin = (int32_t *)ctx->vals0.p, *vout = (int32_t *)ctx->vals1.p;
-    const int bits = 32 + tile_bits(ctx->ntiles);
+    const int bits = tile_bits(ctx->ntiles);
     for (int shift = 0; shift < bits; shift += 8) {
         launch_radix_pass(kin, vin, kout, vout, ctx->d_total32(), (uint32_t)M, shift, (uint32_t *)ctx->hist.p,
-                          (uint32_t *)ctx->digit_total.p, ctx->stream);
+                          (uint32_t *)ctx->digit_total.p, s);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
     ctx->keys_sorted = kin;
     ctx->gid_sorted = vin;
-    launch_tile_ranges(kin, ctx->d_total32(), (uint32_t)M, ctx->ntiles, (int32_t *)ctx->tile_start.p, ctx->stream);
+    launch_tile_ranges(kin, ctx->d_total32(), (uint32_t)M, nullptr, ctx->ntiles, (int32_t *)ctx->tile_start.p, s);
     ctx->end();
     HIPCHECK(hipGetLastError());
     return MSPLAT_OK;
@@ -362,7 +404,8 @@ int msplat_hip_stage_bin(msplat_hip_ctx *ctx, const msplat_hip_camera *cam) {
     // outputs, so that a caller may overwrite them before binning
     launch_rect_count(ctx->n, (const float4 *)ctx->splat_a.p, (const float2 *)ctx->aabb.p,
                       (const int32_t *)ctx->radii.p, ctx->tiles_x, ctx->tiles_y, (uint2 *)ctx->rect.p,
-                      (uint32_t *)ctx->count.p, (uint32_t *)ctx->block_sums.p, ctx->stream);
+                      (uint32_t *)ctx->count.p, (uint32_t *)ctx->block_sums.p, (uint32_t *)ctx->block_vis.p,
+                      ctx->stream);
     HIPCHECK(hipGetLastError());
     return do_bin(ctx);
 }
@@ -508,7 +551,14 @@ int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *f) {
     f->radii = (int32_t *)ctx->radii.p;
     f->tile_count = (uint32_t *)ctx->count.p;
     f->offsets = (uint32_t *)ctx->offsets.p;
-    f->isect_keys = ctx->keys_sorted;
+    f->isect_keys = nullptr;
+    if ((ctx->flags & MSPLAT_FRAME_KEYS64) && ctx->keys_sorted) {
+        CHECKED(ctx->grow(ctx->keys64, 8 * (size_t)(ctx->M > 0 ? ctx->M : 1)));
+        launch_keys64(ctx->keys_sorted, ctx->gid_sorted, (const float *)ctx->depths.p, (uint32_t)ctx->M,
+                      (uint64_t *)ctx->keys64.p, ctx->stream);
+        HIPCHECK(hipGetLastError());
+        f->isect_keys = (uint64_t *)ctx->keys64.p;
+    }
     f->isect_gid = ctx->gid_sorted;
     f->tile_start = (int32_t *)ctx->tile_start.p;
     f->image = (float *)ctx->image.p;

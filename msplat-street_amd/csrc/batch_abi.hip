@@ -19,7 +19,9 @@ namespace msplat {
 
 struct BatchState {
     // per record (views x gaussians)
-    Buf splat_a, splat_b, splat_blue, depths, rect, radii, count, offsets, block_sums, big_list, big_sums;
+    Buf splat_a, splat_b, splat_blue, depths, rect, radii, count, offsets, block_sums, block_vis, big_list, big_sums;
+    // visible records sorted by depth, count sums in depth order, scan partials
+    Buf dk0, dk1, dv0, dv1, ord_sums, scan_part;
     // per intersection
     Buf keys0, keys1, vals0, vals1, slot_a, slot_b, slot_c, hist, digit_total;
     // per tile, pixel, row
@@ -50,7 +52,8 @@ struct BatchState {
 
 void destroy_batch_state(BatchState *b) {
     if (!b) return;
-    Buf *all[] = {&b->splat_a, &b->splat_b, &b->splat_blue, &b->depths,   &b->rect,     &b->radii,      &b->count,
+    Buf *all[] = {&b->block_vis, &b->dk0, &b->dk1, &b->dv0, &b->dv1, &b->ord_sums, &b->scan_part,
+                  &b->splat_a, &b->splat_b, &b->splat_blue, &b->depths,   &b->rect,     &b->radii,      &b->count,
                   &b->offsets, &b->block_sums, &b->big_list, &b->big_sums, &b->keys0, &b->keys1,    &b->vals0,
                   &b->vals1,   &b->slot_a,     &b->slot_b,   &b->slot_c,   &b->hist,  &b->digit_total, &b->tile_start,
                   &b->image,   &b->final_T,    &b->final_idx, &b->v_image, &b->pix_loss, &b->row_loss, &b->grads,
@@ -89,6 +92,17 @@ int grow(msplat_hip_ctx *ctx, BatchState *b, Buf &buf, size_t bytes) {
     void *old = buf.p;
     CHECKED(ctx->grow(buf, bytes));
     if (buf.p != old) b->generation++;
+    return MSPLAT_OK;
+}
+
+size_t record_count(BatchState *b) { return (size_t)b->views * (size_t)(b->n > 0 ? b->n : 1); }
+
+// Radix histograms for the larger of the two sorts: the depth sort over every record
+// and the tile sort over the intersection capacity.
+int grow_sort_scratch(msplat_hip_ctx *ctx, BatchState *b) {
+    const size_t n = std::max((size_t)b->cap, record_count(b));
+    CHECKED(grow(ctx, b, b->hist, 4 * radix_hist_words((uint32_t)n)));
+    CHECKED(grow(ctx, b, b->digit_total, 4 * 256));
     return MSPLAT_OK;
 }
 
@@ -179,6 +193,13 @@ int prepare(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g, c
     CHECKED(grow(ctx, b, b->count, 4 * R));
     CHECKED(grow(ctx, b, b->offsets, 4 * R));
     CHECKED(grow(ctx, b, b->block_sums, 4 * (nbx * nv + 1)));
+    CHECKED(grow(ctx, b, b->block_vis, 4 * (nbx * nv + 1)));
+    CHECKED(grow(ctx, b, b->dk0, 4 * R));
+    CHECKED(grow(ctx, b, b->dk1, 4 * R));
+    CHECKED(grow(ctx, b, b->dv0, 4 * R));
+    CHECKED(grow(ctx, b, b->dv1, 4 * R));
+    CHECKED(grow(ctx, b, b->ord_sums, 4 * ((R + 255) / 256 + 1)));
+    CHECKED(grow(ctx, b, b->scan_part, 4 * 3 * scan_partial_words((uint32_t)std::max(nbx * nv, (R + 255) / 256)) + 64));
     CHECKED(grow(ctx, b, b->big_list, 4 * R));
     CHECKED(grow(ctx, b, b->big_sums, 36 * R + 64));
     b->tiles_cap = std::max(b->tiles_cap, L.tiles);
@@ -197,6 +218,7 @@ int prepare(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g, c
     CHECKED(grow(ctx, b, b->scalars, sizeof(BatchScalars)));
     b->n = n;
     b->views = nv;
+    CHECKED(grow_sort_scratch(ctx, b));  // the record count may have grown
     (void)batch_raster_grid(false, 1);  // occupancy queries outside any capture
     (void)batch_raster_grid(true, ctx->tune[MSPLAT_TUNE_BWD_PPT]);
     HIPCHECK(hipMemcpyAsync(b->table.p, b->h_tab, sizeof(BatchTable), hipMemcpyHostToDevice, ctx->stream));
@@ -212,17 +234,15 @@ int grow_isects(msplat_hip_ctx *ctx, BatchState *b, uint64_t need) {
                          (unsigned long long)need);
     if (want <= b->cap && b->cap) return MSPLAT_OK;
     const size_t C = (size_t)want;
-    CHECKED(grow(ctx, b, b->keys0, 8 * C));
-    CHECKED(grow(ctx, b, b->keys1, 8 * C));
+    CHECKED(grow(ctx, b, b->keys0, 4 * C));
+    CHECKED(grow(ctx, b, b->keys1, 4 * C));
     CHECKED(grow(ctx, b, b->vals0, 4 * C));
     CHECKED(grow(ctx, b, b->vals1, 4 * C));
     CHECKED(grow(ctx, b, b->slot_a, 16 * C));
     CHECKED(grow(ctx, b, b->slot_b, 16 * C));
     CHECKED(grow(ctx, b, b->slot_c, 4 * C));
-    CHECKED(grow(ctx, b, b->hist, 4 * radix_hist_words((uint32_t)C)));
-    CHECKED(grow(ctx, b, b->digit_total, 4 * 256));
     b->cap = (uint32_t)C;
-    return MSPLAT_OK;
+    return grow_sort_scratch(ctx, b);
 }
 
 BatchScalars *dsc(BatchState *b) { return (BatchScalars *)b->scalars.p; }
@@ -248,26 +268,8 @@ BatchProjectArgs project_args(BatchState *b, const msplat_hip_gaussians *g) {
     a.radii = (int32_t *)b->radii.p;
     a.count = (uint32_t *)b->count.p;
     a.block_sums = (uint32_t *)b->block_sums.p;
+    a.block_vis = (uint32_t *)b->block_vis.p;
     return a;
-}
-
-BatchBinArgs bin_args(BatchState *b, int n, uint32_t cap) {
-    BatchBinArgs k;
-    k.tab = dtab(b);
-    k.n = n;
-    k.num_views = b->views;
-    k.nbx = (n + 255) / 256;
-    k.cap = cap;
-    k.count = (const uint32_t *)b->count.p;
-    k.rect = (const uint2 *)b->rect.p;
-    k.depths = (const float *)b->depths.p;
-    k.block_sums = (uint32_t *)b->block_sums.p;
-    k.offsets = (uint32_t *)b->offsets.p;
-    k.keys = (uint64_t *)b->keys0.p;
-    k.vals = (int32_t *)b->vals0.p;
-    k.big_list = (uint32_t *)b->big_list.p;
-    k.sc = dsc(b);
-    return k;
 }
 
 BatchRasterArgs raster_args(BatchState *b) {
@@ -292,25 +294,81 @@ BatchRasterArgs raster_args(BatchState *b) {
     return r;
 }
 
-size_t record_count(BatchState *b) { return (size_t)b->views * (size_t)(b->n > 0 ? b->n : 1); }
 float4 *big_a(BatchState *b) { return (float4 *)(((uintptr_t)b->big_sums.p + 15) & ~(uintptr_t)15); }
 
-// projection, count scan with the capacity check, keys, sort, ranges, forward
-void enqueue_forward(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g) {
+uint32_t *scan_parts(BatchState *b, int i) {
+    const size_t pw = scan_partial_words((uint32_t)std::max((size_t)b->views * ((b->n + 255) / 256),
+                                                            (record_count(b) + 255) / 256));
+    return (uint32_t *)b->scan_part.p + i * pw;
+}
+
+// projection and the count scans (intersections against the capacity, visible records)
+void enqueue_counts(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g, uint32_t cap) {
     hipStream_t s = ctx->stream;
     launch_batch_project(project_args(b, g), s);
-    launch_batch_bin(bin_args(b, g->num_points, b->cap), s);
-    uint64_t *kin = (uint64_t *)b->keys0.p, *kout = (uint64_t *)b->keys1.p;
+    const uint32_t L = (uint32_t)b->views * (uint32_t)((g->num_points + 255) / 256);
+    BatchScalars *sc = dsc(b);
+    launch_scan_u32((uint32_t *)b->block_sums.p, L, scan_parts(b, 0),
+                    ScanOut{&sc->total, &sc->total32, cap, &sc->overflow, &sc->big_count}, s);
+    launch_scan_u32((uint32_t *)b->block_vis.p, L, scan_parts(b, 1),
+                    ScanOut{&sc->nvis64, &sc->nvis, 0xFFFFFFFFu, nullptr, nullptr}, s);
+}
+
+// counts, visible records sorted by depth, keys in depth order, tile sort, ranges, forward
+void enqueue_forward(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g) {
+    hipStream_t s = ctx->stream;
+    enqueue_counts(ctx, b, g, b->cap);
+    BatchScalars *sc = dsc(b);
+    const uint32_t R = (uint32_t)record_count(b);
+    CompactArgs c;
+    c.n = g->num_points;
+    c.num_views = b->views;
+    c.nbx = (g->num_points + 255) / 256;
+    c.count = (const uint32_t *)b->count.p;
+    c.depths = (const float *)b->depths.p;
+    c.block_vis = (const uint32_t *)b->block_vis.p;
+    c.dkeys = (uint32_t *)b->dk0.p;
+    c.dvals = (int32_t *)b->dv0.p;
+    launch_compact(c, s);
+    uint32_t *dki = (uint32_t *)b->dk0.p, *dko = (uint32_t *)b->dk1.p;
+    int32_t *dvi = (int32_t *)b->dv0.p, *dvo = (int32_t *)b->dv1.p;
+    for (int shift = 0; shift < 32; shift += 8) {
+        launch_radix_pass(dki, dvi, dko, dvo, &sc->nvis, R, shift, (uint32_t *)b->hist.p,
+                          (uint32_t *)b->digit_total.p, s);
+        std::swap(dki, dko);
+        std::swap(dvi, dvo);
+    }
+    launch_order_sums(dvi, &sc->nvis, R, (const uint32_t *)b->count.p, (uint32_t *)b->ord_sums.p, s);
+    launch_scan_u32((uint32_t *)b->ord_sums.p, (R + 255) / 256, scan_parts(b, 2),
+                    ScanOut{nullptr, &sc->total32_b, b->cap, &sc->overflow_b, nullptr}, s);
+    KeygenArgs k;
+    memset(&k, 0, sizeof k);
+    k.list = dvi;
+    k.nvis_ptr = &sc->nvis;
+    k.nvis_max = R;
+    k.count = (const uint32_t *)b->count.p;
+    k.rect = (const uint2 *)b->rect.p;
+    k.block_bases = (const uint32_t *)b->ord_sums.p;
+    k.offsets = (uint32_t *)b->offsets.p;
+    k.keys = (uint32_t *)b->keys0.p;
+    k.vals = (int32_t *)b->vals0.p;
+    k.big_list = (uint32_t *)b->big_list.p;
+    k.big_count = &sc->big_count;
+    k.overflow = &sc->overflow;
+    k.tab = dtab(b);
+    k.n = g->num_points;
+    launch_keygen(k, true, s);
+    uint32_t *kin = (uint32_t *)b->keys0.p, *kout = (uint32_t *)b->keys1.p;
     int32_t *vin = (int32_t *)b->vals0.p, *vout = (int32_t *)b->vals1.p;
-    const int bits = 32 + tile_bits((int)b->tiles_cap);
+    const int bits = tile_bits((int)b->tiles_cap);
     for (int shift = 0; shift < bits; shift += 8) {
-        launch_radix_pass(kin, vin, kout, vout, &dsc(b)->total32, b->cap, shift, (uint32_t *)b->hist.p,
+        launch_radix_pass(kin, vin, kout, vout, &sc->total32, b->cap, shift, (uint32_t *)b->hist.p,
                           (uint32_t *)b->digit_total.p, s);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
     b->gid_sorted = vin;
-    launch_tile_ranges_dev(kin, &dsc(b)->total32, b->cap, &dtab(b)->total_tiles, (int32_t *)b->tile_start.p, s);
+    launch_tile_ranges(kin, &sc->total32, b->cap, &dtab(b)->total_tiles, 0, (int32_t *)b->tile_start.p, s);
     launch_batch_raster_forward(raster_args(b), s);
 }
 
@@ -413,8 +471,7 @@ int read_scalars(msplat_hip_ctx *ctx, BatchState *b, bool &over) {
 // Sizes the intersection buffers from a counting pass when they are not sized yet
 int ensure_capacity(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g) {
     if (b->cap) return MSPLAT_OK;
-    launch_batch_project(project_args(b, g), ctx->stream);
-    launch_batch_bin(bin_args(b, g->num_points, 0), ctx->stream);  // capacity 0: counts only
+    enqueue_counts(ctx, b, g, 0u);  // capacity 0: counts only
     HIPCHECK(hipGetLastError());
     bool over = false;
     CHECKED(read_scalars(ctx, b, over));

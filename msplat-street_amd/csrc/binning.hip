@@ -1,13 +1,22 @@
-// Projection + SH colour, (tile | depth) key generation, stable LSD radix sort and
-// per-tile range extraction.
+// Projection + SH colour, binning of the splats into 16x16 tiles, stable LSD radix
+// sort and per-tile range extraction.
 //
-// Layout in HBM (per gaussian, SoA records written once per view):
+// Layout in HBM (per record, SoA, written once per view; a record is a gaussian in one
+// view, record v * n + g in a batch of views):
 //   splat_a = (x, y, conic.x, conic.y)   float4
 //   splat_b = (conic.z, sigmoid(opacity), r, g)   float4
 //   splat_blue = b, depths, aabb (float2), rect (x0 | x1 << 16, y0 | y1 << 16), count
-// Per intersection: 64-bit keys (tile << 32 | fp32 depth bits) with the gaussian id as
-// payload, emitted in ascending gaussian order so that a stable sort reproduces the
-// reference's per-tile (depth, id) order (core/cpu/msplat_cpu.cpp:505-537).
+//
+// Binning keeps the reference's per-tile order, ascending (depth bits, gaussian id)
+// (core/cpu/msplat_cpu.cpp:505-537), with two small stable sorts in place of one sort
+// of 64-bit (tile | depth) keys:
+//   1. the visible records are compacted in id order and stably sorted by their 32-bit
+//      depth bits (4 passes over the visible records, not over the intersections);
+//   2. each record's intersections are emitted in that depth order with its tile id as
+//      a 32-bit key, and a stable sort by tile (ceil(tile bits / 8) passes) leaves every
+//      tile's intersections in (depth, id) order.
+// Records own contiguous slot ranges (offsets[r], count[r]) in emission order, tiles
+// row-major within a record, which the backward's per-record reductions read.
 #include "kernels.h"
 #include "proj_math.h"
 
@@ -36,6 +45,7 @@ __device__ inline uint32_t block_scan_u32(uint32_t v, uint32_t &total, uint32_t 
 }
 
 // ── projection + SH colour (project_and_sh, msplat_cpu.cpp:452-500) ────────────
+// Also writes the workgroup's intersection count and visible-record count.
 __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
     __shared__ uint32_t lds4[4];
     const int g = blockIdx.x * 256 + threadIdx.x;
@@ -78,16 +88,20 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
             }
         }
     }
-    uint32_t total;
+    uint32_t total, vis;
     block_scan_u32(count, total, lds4);
-    if (threadIdx.x == 0) a.block_sums[blockIdx.x] = total;
+    block_scan_u32(count > 0 ? 1u : 0u, vis, lds4);
+    if (threadIdx.x == 0) {
+        a.block_sums[blockIdx.x] = total;
+        a.block_vis[blockIdx.x] = vis;
+    }
 }
 
 // Tile rectangles and counts re-derived from a frame's projection outputs (pixel
 // centre in splat_a, aabb, radii), for binning outputs a caller injected.
 __global__ __launch_bounds__(256) void rect_count_kernel(int n, const float4 *splat_a, const float2 *aabb,
                                                         const int32_t *radii, int tx, int ty, uint2 *rect,
-                                                        uint32_t *count, uint32_t *block_sums) {
+                                                        uint32_t *count, uint32_t *block_sums, uint32_t *block_vis) {
     __shared__ uint32_t lds4[4];
     const int g = blockIdx.x * 256 + threadIdx.x;
     uint32_t c = 0;
@@ -102,13 +116,17 @@ __global__ __launch_bounds__(256) void rect_count_kernel(int n, const float4 *sp
         }
         count[g] = c;
     }
-    uint32_t total;
+    uint32_t total, vis;
     block_scan_u32(c, total, lds4);
-    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+    block_scan_u32(c > 0 ? 1u : 0u, vis, lds4);
+    if (threadIdx.x == 0) {
+        block_sums[blockIdx.x] = total;
+        block_vis[blockIdx.x] = vis;
+    }
 }
 
-// Exclusive scan of the per-block intersection counts (one block, sequential chunks);
-// writes the block bases and the total (u64, so that > 2^31 is detected).
+// Exclusive scan of the per-block counts of a small array (one block, sequential
+// chunks); writes the bases and the total (u64, so that > 2^31 is detected).
 __global__ __launch_bounds__(256) void scan_blocks_kernel(uint32_t *sums, int nblocks, uint64_t *total,
                                                           uint32_t *total32, uint32_t *zero_also) {
     __shared__ uint32_t lds4[4];
@@ -134,61 +152,92 @@ __global__ __launch_bounds__(256) void scan_blocks_kernel(uint32_t *sums, int nb
     }
 }
 
-// Key emission: gaussian g writes its tiles in row-major order at offsets[g]. The
-// workgroup emits the keys of its 256 gaussians together: intersection i of the block
-// finds its gaussian by binary search over the block's offsets (LDS), so consecutive
-// threads write consecutive keys.
-__global__ __launch_bounds__(256) void keygen_kernel(KeygenArgs a) {
+// ── multi-level exclusive scan of a u32 array (4096 entries per workgroup) ─────
+constexpr int SCAN_CHUNK = 4096;
+
+__global__ __launch_bounds__(256) void scan_partial_kernel(const uint32_t *a, uint32_t L, uint32_t *partial) {
     __shared__ uint32_t lds4[4];
-    __shared__ uint32_t s_off[257];
-    __shared__ uint32_t s_rect[256][2];
-    __shared__ uint32_t s_depth[256];
-    __shared__ uint32_t s_bigbase;
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t c = g < a.n ? a.count[g] : 0;
-    uint32_t total;
-    const uint32_t ex = block_scan_u32(c, total, lds4);
-    const uint32_t base = a.block_bases[blockIdx.x];
-    if (g < a.n) a.offsets[g] = base + ex;
-    // gaussians with many intersections join the big list: one atomic per workgroup
-    uint32_t nbig;
-    const uint32_t bex = block_scan_u32(c > BIG_SLOTS ? 1u : 0u, nbig, lds4);
-    if (threadIdx.x == 0 && nbig) s_bigbase = atomicAdd(a.big_count, nbig);
+    const uint32_t base = blockIdx.x * SCAN_CHUNK + threadIdx.x * 16;
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        if (base + i < L) s += a[base + i];
+    uint32_t t;
+    block_scan_u32(s, t, lds4);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// Scans the chunk totals; the grand total goes to total/total32. Over `cap`, total32 is
+// 0 and *overflow is set (when overflow is given).
+__global__ __launch_bounds__(256) void scan_top_kernel(uint32_t *partial, int nchunks, ScanOut o) {
+    __shared__ uint32_t lds4[4];
+    __shared__ uint64_t carry;
+    if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    if (c > BIG_SLOTS) a.big_list[s_bigbase + bex] = (uint32_t)g;
-    s_off[threadIdx.x] = ex;
-    if (threadIdx.x == 0) s_off[256] = total;
-    if (c) {
-        const uint2 r = a.rect[g];
-        s_rect[threadIdx.x][0] = r.x;
-        s_rect[threadIdx.x][1] = r.y;
-        s_depth[threadIdx.x] = __float_as_uint(a.depths[g]);
+    for (int base = 0; base < nchunks; base += 256) {
+        const int i = base + threadIdx.x;
+        uint32_t v = i < nchunks ? partial[i] : 0, t;
+        const uint32_t ex = block_scan_u32(v, t, lds4);
+        const uint64_t c = carry;
+        if (i < nchunks) partial[i] = (uint32_t)(c + ex);
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + t;
+        __syncthreads();
     }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < total; i += 256) {
-        // the last gaussian j of the block with s_off[j] <= i
-        int lo = 0, hi = 255;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= i) lo = mid;
-            else hi = mid - 1;
-        }
-        const uint32_t rx = s_rect[lo][0], ry = s_rect[lo][1];
-        const int x0 = rx & 0xFFFF, x1 = rx >> 16, y0 = ry & 0xFFFF;
-        const uint32_t local = i - s_off[lo], w = (uint32_t)(x1 - x0);
-        const int ty = y0 + (int)(local / w), tx = x0 + (int)(local % w);
-        a.keys[base + i] = ((uint64_t)(uint32_t)(ty * a.tiles_x + tx) << 32) | (uint64_t)s_depth[lo];
-        a.vals[base + i] = blockIdx.x * 256 + lo;
+    if (threadIdx.x == 0) {
+        const uint64_t tot = carry;
+        const bool over = tot > (uint64_t)o.cap;
+        if (o.total) *o.total = tot;
+        if (o.total32) *o.total32 = over ? (o.overflow ? 0u : 0xFFFFFFFFu) : (uint32_t)tot;
+        if (o.overflow) *o.overflow = over ? 1u : 0u;
+        if (o.zero_also) *o.zero_also = 0u;
     }
 }
 
-// ── stable LSD radix sort, 8-bit digits ──────────────────────────────────────
-// Each block owns RS_ITEMS consecutive elements: an upsweep histogram, a per-digit
-// scan over blocks, and a scatter that ranks elements stably within the block with
-// wave-level multi-split (ballots over the digit bits).
+__global__ __launch_bounds__(256) void scan_apply_kernel(uint32_t *a, uint32_t L, const uint32_t *partial) {
+    __shared__ uint32_t lds4[4];
+    const uint32_t base = blockIdx.x * SCAN_CHUNK + threadIdx.x * 16;
+    uint32_t v[16], s = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        v[i] = base + i < L ? a[base + i] : 0u;
+        s += v[i];
+    }
+    uint32_t t;
+    uint32_t ex = block_scan_u32(s, t, lds4) + partial[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        if (base + i < L) {
+            a[base + i] = ex;
+            ex += v[i];
+        }
+}
 
-__global__ __launch_bounds__(256) void rs_hist_kernel(const uint64_t *keys, const uint32_t *n_ptr, int shift,
-                                                     uint32_t *hist, int nblocks) {
+// ── visible records in id order, keyed by depth bits ──────────────────────────
+__global__ __launch_bounds__(256) void compact_kernel(CompactArgs a) {
+    __shared__ uint32_t lds4[4];
+    const int v = blockIdx.y;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    const size_t r = (size_t)v * a.n + g;
+    const bool vis = g < a.n && a.count[r] > 0;
+    uint32_t t;
+    const uint32_t ex = block_scan_u32(vis ? 1u : 0u, t, lds4);
+    if (vis) {
+        const uint32_t pos = a.block_vis[(size_t)v * a.nbx + blockIdx.x] + ex;
+        a.dkeys[pos] = __float_as_uint(a.depths[r]);
+        a.dvals[pos] = (int32_t)r;
+    }
+}
+
+// ── stable LSD radix sort, 8-bit digits, 32- or 64-bit keys, int32 payload ─────
+// Each workgroup owns RS_ITEMS consecutive elements: an upsweep histogram, a per-digit
+// scan over workgroups, and a scatter that ranks the elements stably within the
+// workgroup (wave multi-split over ballots of the digit bits), stages them in LDS in
+// digit order, and writes each digit's run to consecutive addresses.
+
+template <typename K>
+__global__ __launch_bounds__(256) void rs_hist_kernel(const K *keys, const uint32_t *n_ptr, int shift, uint32_t *hist,
+                                                     int nblocks) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -222,33 +271,41 @@ __global__ __launch_bounds__(256) void rs_scan_kernel(uint32_t *hist, int nblock
     if (threadIdx.x == 0) digit_total[blockIdx.x] = carry;
 }
 
-__global__ __launch_bounds__(256) void rs_scatter_kernel(const uint64_t *kin, const int32_t *vin, uint64_t *kout,
-                                                        int32_t *vout, const uint32_t *n_ptr, int shift,
-                                                        const uint32_t *hist, int nblocks,
-                                                        const uint32_t *digit_total) {
-    __shared__ uint32_t run[256];
+template <typename K>
+__global__ __launch_bounds__(256) void rs_scatter_kernel(const K *kin, const int32_t *vin, K *kout, int32_t *vout,
+                                                        const uint32_t *n_ptr, int shift, const uint32_t *hist,
+                                                        int nblocks, const uint32_t *digit_total) {
+    constexpr int R = RS_ITEMS / 256;
+    __shared__ K sk[RS_ITEMS];
+    __shared__ int32_t sv[RS_ITEMS];
+    __shared__ uint32_t gbase[256], lstart[256], run[256];
     __shared__ uint32_t wcnt[4][256];
     __shared__ uint32_t lds4[4];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    // global base of each digit for this block = exclusive digit prefix + block offset
-    {
-        uint32_t t;
-        uint32_t ex = block_scan_u32(digit_total[tid], t, lds4);
-        run[tid] = ex + hist[(size_t)tid * nblocks + blockIdx.x];
-    }
     const uint32_t n = *n_ptr;
     const uint32_t start = blockIdx.x * RS_ITEMS;
-    const uint64_t lt = (1ull << lane) - 1ull;
-    for (int r = 0; r < RS_ITEMS / 256; r++) {
+    if (start >= n) return;
+    {   // global position of this block's first element of each digit
+        uint32_t t;
+        const uint32_t ex = block_scan_u32(digit_total[tid], t, lds4);
+        gbase[tid] = ex + hist[(size_t)tid * nblocks + blockIdx.x];
+    }
+    run[tid] = 0;
+    K key[R];
+    int32_t val[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
         const uint32_t e = start + r * 256 + tid;
-        const bool valid = e < n;
-        uint64_t key = 0;
-        int32_t val = 0;
-        if (valid) {
-            key = kin[e];
-            val = vin[e];
-        }
-        const uint32_t digit = valid ? (uint32_t)(key >> shift) & 0xFF : 0;
+        key[r] = e < n ? kin[e] : K(0);
+        val[r] = e < n ? vin[e] : 0;
+    }
+    // per element: digit (bits 24..31) and rank within its digit in element order
+    uint32_t dr[R];
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const bool valid = start + r * 256 + tid < n;
+        const uint32_t digit = valid ? (uint32_t)(key[r] >> shift) & 0xFF : 0;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -263,8 +320,8 @@ __global__ __launch_bounds__(256) void rs_scatter_kernel(const uint64_t *kin, co
         if (valid && rank == 0) wcnt[w][digit] = (uint32_t)__popcll(peers);
         __syncthreads();
         {   // counts -> exclusive offsets over the four waves, per digit
-            uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
-            uint32_t base = run[tid];
+            const uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
+            const uint32_t base = run[tid];
             wcnt[0][tid] = base;
             wcnt[1][tid] = base + c0;
             wcnt[2][tid] = base + c0 + c1;
@@ -272,36 +329,125 @@ __global__ __launch_bounds__(256) void rs_scatter_kernel(const uint64_t *kin, co
             run[tid] = base + c0 + c1 + c2 + c3;
         }
         __syncthreads();
-        if (valid) {
-            const uint32_t pos = wcnt[w][digit] + rank;
-            kout[pos] = key;
-            vout[pos] = val;
-        }
+        dr[r] = (digit << 24) | (valid ? wcnt[w][digit] + rank : 0u);
         __syncthreads();
+    }
+    {   // local start of each digit: exclusive scan of the block's digit counts
+        uint32_t t;
+        lstart[tid] = block_scan_u32(run[tid], t, lds4);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (start + r * 256 + tid < n) {
+            const uint32_t p = lstart[dr[r] >> 24] + (dr[r] & 0xFFFFFF);
+            sk[p] = key[r];
+            sv[p] = val[r];
+        }
+    }
+    __syncthreads();
+    const uint32_t cnt = min((uint32_t)RS_ITEMS, n - start);
+    for (uint32_t p = tid; p < cnt; p += 256) {
+        const K k = sk[p];
+        const uint32_t d = (uint32_t)(k >> shift) & 0xFF;
+        const uint32_t o = gbase[d] + (p - lstart[d]);
+        kout[o] = k;
+        vout[o] = sv[p];
+    }
+}
+
+// ── intersections in depth order ──────────────────────────────────────────────
+
+// Per 256-entry block of the depth-ordered list: the sum of its records' counts
+// (0 for blocks past the list's device-side length).
+__global__ __launch_bounds__(256) void order_sums_kernel(const int32_t *list, const uint32_t *nvis_ptr,
+                                                        const uint32_t *count, uint32_t *sums) {
+    __shared__ uint32_t lds4[4];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t c = i < *nvis_ptr ? count[list[i]] : 0u;
+    uint32_t t;
+    block_scan_u32(c, t, lds4);
+    if (threadIdx.x == 0) sums[blockIdx.x] = t;
+}
+
+// Record list[i] writes its tiles in row-major order at offsets[r] = the exclusive
+// prefix of the counts in list order. The workgroup emits the keys of its 256 records
+// together: intersection i of the block finds its record by binary search over the
+// block's offsets (LDS), so consecutive threads write consecutive keys.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void keygen_kernel(KeygenArgs a) {
+    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t s_off[257];
+    __shared__ uint32_t s_rect[256][2];
+    __shared__ uint32_t s_tile[256][2];  // tiles_x, first global tile of the record's view
+    __shared__ int32_t s_rec[256];
+    __shared__ uint32_t s_bigbase;
+    if (a.overflow && *a.overflow) return;
+    const uint32_t i0 = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t nvis = *a.nvis_ptr;
+    const int32_t r = i0 < nvis ? a.list[i0] : -1;
+    const uint32_t c = r >= 0 ? a.count[r] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_scan_u32(c, total, lds4);
+    const uint32_t base = a.block_bases[blockIdx.x];
+    if (r >= 0) a.offsets[r] = base + ex;
+    uint32_t nbig;
+    const uint32_t bex = block_scan_u32(c > BIG_SLOTS ? 1u : 0u, nbig, lds4);
+    if (threadIdx.x == 0 && nbig) s_bigbase = atomicAdd(a.big_count, nbig);
+    __syncthreads();
+    if (c > BIG_SLOTS) a.big_list[s_bigbase + bex] = (uint32_t)r;
+    s_off[threadIdx.x] = ex;
+    if (threadIdx.x == 0) s_off[256] = total;
+    if (c) {
+        const uint2 rr = a.rect[r];
+        s_rect[threadIdx.x][0] = rr.x;
+        s_rect[threadIdx.x][1] = rr.y;
+        s_rec[threadIdx.x] = r;
+        if (BATCH) {
+            const int v = r / a.n;
+            s_tile[threadIdx.x][0] = (uint32_t)a.tab->views[v].cam.tiles_x;
+            s_tile[threadIdx.x][1] = a.tab->tile_base[v];
+        } else {
+            s_tile[threadIdx.x][0] = (uint32_t)a.tiles_x;
+            s_tile[threadIdx.x][1] = 0u;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < total; i += 256) {
+        // the last record j of the block with s_off[j] <= i
+        int lo = 0, hi = 255;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t rx = s_rect[lo][0], ry = s_rect[lo][1];
+        const int x0 = rx & 0xFFFF, x1 = rx >> 16, y0 = ry & 0xFFFF;
+        const uint32_t local = i - s_off[lo], w = (uint32_t)(x1 - x0);
+        const int ty = y0 + (int)(local / w), tx = x0 + (int)(local % w);
+        a.keys[base + i] = s_tile[lo][1] + (uint32_t)(ty * (int)s_tile[lo][0] + tx);
+        a.vals[base + i] = s_rec[lo];
     }
 }
 
 // tile_start[t] = first sorted intersection with tile >= t, for t in [0, T]
-__global__ __launch_bounds__(256) void tile_ranges_kernel(const uint64_t *keys, const uint32_t *n_ptr, int ntiles,
+__global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t *keys, const uint32_t *n_ptr,
+                                                         const uint32_t *ntiles_ptr, int ntiles_host,
                                                          int32_t *tile_start) {
     const uint32_t n = *n_ptr;
+    const int ntiles = ntiles_ptr ? (int)*ntiles_ptr : ntiles_host;
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
     if (k > n) return;
-    const int t = k < n ? (int)(keys[k] >> 32) : ntiles;
-    const int tp = k > 0 ? (int)(keys[k - 1] >> 32) : -1;
+    const int t = k < n ? (int)keys[k] : ntiles;
+    const int tp = k > 0 ? (int)keys[k - 1] : -1;
     for (int u = tp + 1; u <= t; u++) tile_start[u] = (int32_t)k;
 }
 
-// tile_ranges_kernel with the tile count read from device memory (batched frames)
-__global__ __launch_bounds__(256) void tile_ranges_dev_kernel(const uint64_t *keys, const uint32_t *n_ptr,
-                                                             const uint32_t *ntiles_ptr, int32_t *tile_start) {
-    const uint32_t n = *n_ptr;
-    const int ntiles = (int)*ntiles_ptr;
+// The 64-bit (tile << 32 | depth bits) view of the sorted intersections
+__global__ __launch_bounds__(256) void keys64_kernel(const uint32_t *tiles, const int32_t *gid, const float *depths,
+                                                    uint32_t n, uint64_t *out) {
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-    if (k > n) return;
-    const int t = k < n ? (int)(keys[k] >> 32) : ntiles;
-    const int tp = k > 0 ? (int)(keys[k - 1] >> 32) : -1;
-    for (int u = tp + 1; u <= t; u++) tile_start[u] = (int32_t)k;
+    if (k < n) out[k] = ((uint64_t)tiles[k] << 32) | (uint64_t)__float_as_uint(depths[gid[k]]);
 }
 
 // ── batched views ────────────────────────────────────────────────────────────
@@ -309,7 +455,7 @@ __global__ __launch_bounds__(256) void tile_ranges_dev_kernel(const uint64_t *ke
 // Projection of gaussian g into every view of the batch: the parameters are read once
 // and record v * n + g is written per view, with the same arithmetic as project_kernel
 // (the records are bit-identical to one-view projections). Per view, the workgroup's
-// intersection count goes to block_sums[v * nbx + blockIdx.x].
+// intersection and visible-record counts go to block_sums / block_vis[v * nbx + b].
 __global__ __launch_bounds__(256) void project_batch_kernel(BatchProjectArgs a) {
     __shared__ uint32_t lds4[4];
     const int g = blockIdx.x * 256 + threadIdx.x;
@@ -360,116 +506,25 @@ __global__ __launch_bounds__(256) void project_batch_kernel(BatchProjectArgs a) 
                 }
             }
         }
-        uint32_t total;
+        uint32_t total, vis;
         block_scan_u32(count, total, lds4);
-        if (threadIdx.x == 0) a.block_sums[(size_t)v * a.nbx + blockIdx.x] = total;
-    }
-}
-
-// One workgroup per view: exclusive scan of the view's block sums, view total
-__global__ __launch_bounds__(256) void batch_view_scan_kernel(uint32_t *sums, int nbx, BatchScalars *sc) {
-    __shared__ uint32_t lds4[4];
-    __shared__ uint64_t carry;
-    uint32_t *row = sums + (size_t)blockIdx.x * nbx;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < nbx; base += 256) {
-        const int i = base + threadIdx.x;
-        uint32_t v = i < nbx ? row[i] : 0, t;
-        const uint32_t ex = block_scan_u32(v, t, lds4);
-        const uint64_t c = carry;
-        if (i < nbx) row[i] = (uint32_t)(c + ex);
-        __syncthreads();
-        if (threadIdx.x == 0) carry = c + t;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) sc->view_total[blockIdx.x] = carry > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)carry;
-}
-
-// Views' bases (in place of their totals), the intersection total and the capacity
-// check. Over capacity, total32 is 0 and overflow is set: the keys are not written and
-// every later stage sees an empty frame.
-__global__ void batch_total_kernel(int num_views, uint32_t cap, BatchScalars *sc) {
-    if (threadIdx.x != 0) return;
-    uint64_t acc = 0;
-    for (int v = 0; v < num_views; v++) {
-        const uint32_t t = sc->view_total[v];
-        sc->view_total[v] = acc > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)acc;
-        acc += t;
-    }
-    sc->total = acc;
-    const bool over = acc > (uint64_t)cap;
-    sc->overflow = over ? 1u : 0u;
-    sc->total32 = over ? 0u : (uint32_t)acc;
-    sc->big_count = 0;
-}
-
-// keygen_kernel per (block, view), keys on global tile ids, record ids as payload
-__global__ __launch_bounds__(256) void keygen_batch_kernel(BatchBinArgs a) {
-    __shared__ uint32_t lds4[4];
-    __shared__ uint32_t s_off[257];
-    __shared__ uint32_t s_rect[256][2];
-    __shared__ uint32_t s_depth[256];
-    __shared__ uint32_t s_bigbase;
-    if (a.sc->overflow) return;
-    const int v = blockIdx.y;
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    const size_t r0 = (size_t)v * a.n;
-    const uint32_t c = g < a.n ? a.count[r0 + g] : 0;
-    uint32_t total;
-    const uint32_t ex = block_scan_u32(c, total, lds4);
-    const uint32_t base = a.sc->view_total[v] + a.block_sums[(size_t)v * a.nbx + blockIdx.x];
-    if (g < a.n) a.offsets[r0 + g] = base + ex;
-    uint32_t nbig;
-    const uint32_t bex = block_scan_u32(c > BIG_SLOTS ? 1u : 0u, nbig, lds4);
-    if (threadIdx.x == 0 && nbig) s_bigbase = atomicAdd(&a.sc->big_count, nbig);
-    __syncthreads();
-    if (c > BIG_SLOTS) a.big_list[s_bigbase + bex] = (uint32_t)(r0 + g);
-    s_off[threadIdx.x] = ex;
-    if (threadIdx.x == 0) s_off[256] = total;
-    if (c) {
-        const uint2 rr = a.rect[r0 + g];
-        s_rect[threadIdx.x][0] = rr.x;
-        s_rect[threadIdx.x][1] = rr.y;
-        s_depth[threadIdx.x] = __float_as_uint(a.depths[r0 + g]);
-    }
-    __syncthreads();
-    const int tiles_x = a.tab->views[v].cam.tiles_x;
-    const uint32_t tbase = a.tab->tile_base[v];
-    for (uint32_t i = threadIdx.x; i < total; i += 256) {
-        int lo = 0, hi = 255;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= i) lo = mid;
-            else hi = mid - 1;
+        block_scan_u32(count > 0 ? 1u : 0u, vis, lds4);
+        if (threadIdx.x == 0) {
+            a.block_sums[(size_t)v * a.nbx + blockIdx.x] = total;
+            a.block_vis[(size_t)v * a.nbx + blockIdx.x] = vis;
         }
-        const uint32_t rx = s_rect[lo][0], ry = s_rect[lo][1];
-        const int x0 = rx & 0xFFFF, x1 = rx >> 16, y0 = ry & 0xFFFF;
-        const uint32_t local = i - s_off[lo], w = (uint32_t)(x1 - x0);
-        const int ty = y0 + (int)(local / w), tx = x0 + (int)(local % w);
-        const uint32_t gtile = tbase + (uint32_t)(ty * tiles_x + tx);
-        a.keys[base + i] = ((uint64_t)gtile << 32) | (uint64_t)s_depth[lo];
-        a.vals[base + i] = (int32_t)(r0 + blockIdx.x * 256 + lo);
     }
 }
 
 // ── host launchers ───────────────────────────────────────────────────────────
 
-void launch_batch_project(const BatchProjectArgs &a, hipStream_t s) {
-    if (a.nbx > 0) project_batch_kernel<<<a.nbx, 256, 0, s>>>(a);
-}
+size_t scan_partial_words(uint32_t L) { return (size_t)(L + SCAN_CHUNK - 1) / SCAN_CHUNK + 1; }
 
-void launch_batch_bin(const BatchBinArgs &a, hipStream_t s) {
-    if (a.num_views <= 0) return;
-    if (a.nbx > 0) batch_view_scan_kernel<<<a.num_views, 256, 0, s>>>(a.block_sums, a.nbx, a.sc);
-    batch_total_kernel<<<1, 64, 0, s>>>(a.num_views, a.cap, a.sc);
-    if (a.nbx > 0) keygen_batch_kernel<<<dim3(a.nbx, a.num_views), 256, 0, s>>>(a);
-}
-
-void launch_tile_ranges_dev(const uint64_t *keys, const uint32_t *n_dev, uint32_t n_max, const uint32_t *ntiles_dev,
-                            int32_t *tile_start, hipStream_t s) {
-    const uint32_t nb = (n_max + 1 + 255) / 256;
-    tile_ranges_dev_kernel<<<nb, 256, 0, s>>>(keys, n_dev, ntiles_dev, tile_start);
+void launch_scan_u32(uint32_t *a, uint32_t L, uint32_t *partial, const ScanOut &o, hipStream_t s) {
+    const int nchunks = (int)((L + SCAN_CHUNK - 1) / SCAN_CHUNK);
+    if (nchunks > 0) scan_partial_kernel<<<nchunks, 256, 0, s>>>(a, L, partial);
+    scan_top_kernel<<<1, 256, 0, s>>>(partial, nchunks, o);
+    if (nchunks > 0) scan_apply_kernel<<<nchunks, 256, 0, s>>>(a, L, partial);
 }
 
 void launch_project(const ProjectArgs &a, hipStream_t s) {
@@ -478,9 +533,10 @@ void launch_project(const ProjectArgs &a, hipStream_t s) {
 }
 
 void launch_rect_count(int n, const float4 *splat_a, const float2 *aabb, const int32_t *radii, int tx, int ty,
-                       uint2 *rect, uint32_t *count, uint32_t *block_sums, hipStream_t s) {
+                       uint2 *rect, uint32_t *count, uint32_t *block_sums, uint32_t *block_vis, hipStream_t s) {
     const int nb = (n + 255) / 256;
-    if (nb > 0) rect_count_kernel<<<nb, 256, 0, s>>>(n, splat_a, aabb, radii, tx, ty, rect, count, block_sums);
+    if (nb > 0)
+        rect_count_kernel<<<nb, 256, 0, s>>>(n, splat_a, aabb, radii, tx, ty, rect, count, block_sums, block_vis);
 }
 
 void launch_scan_blocks(uint32_t *sums, int nblocks, uint64_t *total, uint32_t *total32, hipStream_t s,
@@ -488,26 +544,58 @@ void launch_scan_blocks(uint32_t *sums, int nblocks, uint64_t *total, uint32_t *
     scan_blocks_kernel<<<1, 256, 0, s>>>(sums, nblocks, total, total32, zero_also);
 }
 
+void launch_compact(const CompactArgs &a, hipStream_t s) {
+    if (a.nbx > 0 && a.num_views > 0) compact_kernel<<<dim3(a.nbx, a.num_views), 256, 0, s>>>(a);
+}
+
 size_t radix_hist_words(uint32_t n_max) { return (size_t)256 * ((n_max + RS_ITEMS - 1) / RS_ITEMS + 1); }
 
-void launch_keygen(const KeygenArgs &a, hipStream_t s) {
-    const int nb = (a.n + 255) / 256;
-    if (nb > 0) keygen_kernel<<<nb, 256, 0, s>>>(a);
+template <typename K>
+static void radix_pass(const K *kin, const int32_t *vin, K *kout, int32_t *vout, const uint32_t *n_dev,
+                       uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s) {
+    const int nblocks = (int)((n_max + RS_ITEMS - 1) / RS_ITEMS);
+    if (nblocks == 0) return;
+    rs_hist_kernel<K><<<nblocks, 256, 0, s>>>(kin, n_dev, shift, hist, nblocks);
+    rs_scan_kernel<<<256, 256, 0, s>>>(hist, nblocks, digit_total);
+    rs_scatter_kernel<K><<<nblocks, 256, 0, s>>>(kin, vin, kout, vout, n_dev, shift, hist, nblocks, digit_total);
 }
 
 void launch_radix_pass(const uint64_t *kin, const int32_t *vin, uint64_t *kout, int32_t *vout, const uint32_t *n_dev,
                        uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s) {
-    const int nblocks = (int)((n_max + RS_ITEMS - 1) / RS_ITEMS);
-    if (nblocks == 0) return;
-    rs_hist_kernel<<<nblocks, 256, 0, s>>>(kin, n_dev, shift, hist, nblocks);
-    rs_scan_kernel<<<256, 256, 0, s>>>(hist, nblocks, digit_total);
-    rs_scatter_kernel<<<nblocks, 256, 0, s>>>(kin, vin, kout, vout, n_dev, shift, hist, nblocks, digit_total);
+    radix_pass<uint64_t>(kin, vin, kout, vout, n_dev, n_max, shift, hist, digit_total, s);
 }
 
-void launch_tile_ranges(const uint64_t *keys, const uint32_t *n_dev, uint32_t n_max, int ntiles, int32_t *tile_start,
-                        hipStream_t s) {
+void launch_radix_pass(const uint32_t *kin, const int32_t *vin, uint32_t *kout, int32_t *vout, const uint32_t *n_dev,
+                       uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s) {
+    radix_pass<uint32_t>(kin, vin, kout, vout, n_dev, n_max, shift, hist, digit_total, s);
+}
+
+void launch_order_sums(const int32_t *list, const uint32_t *nvis_dev, uint32_t nvis_max, const uint32_t *count,
+                       uint32_t *sums, hipStream_t s) {
+    const uint32_t nb = (nvis_max + 255) / 256;
+    if (nb > 0) order_sums_kernel<<<nb, 256, 0, s>>>(list, nvis_dev, count, sums);
+}
+
+void launch_keygen(const KeygenArgs &a, bool batch, hipStream_t s) {
+    const uint32_t nb = (a.nvis_max + 255) / 256;
+    if (nb == 0) return;
+    if (batch) keygen_kernel<true><<<nb, 256, 0, s>>>(a);
+    else keygen_kernel<false><<<nb, 256, 0, s>>>(a);
+}
+
+void launch_tile_ranges(const uint32_t *keys, const uint32_t *n_dev, uint32_t n_max, const uint32_t *ntiles_dev,
+                        int ntiles, int32_t *tile_start, hipStream_t s) {
     const uint32_t nb = (n_max + 1 + 255) / 256;
-    tile_ranges_kernel<<<nb, 256, 0, s>>>(keys, n_dev, ntiles, tile_start);
+    tile_ranges_kernel<<<nb, 256, 0, s>>>(keys, n_dev, ntiles_dev, ntiles, tile_start);
+}
+
+void launch_keys64(const uint32_t *tiles, const int32_t *gid, const float *depths, uint32_t n, uint64_t *out,
+                   hipStream_t s) {
+    if (n) keys64_kernel<<<(n + 255) / 256, 256, 0, s>>>(tiles, gid, depths, n, out);
+}
+
+void launch_batch_project(const BatchProjectArgs &a, hipStream_t s) {
+    if (a.nbx > 0) project_batch_kernel<<<a.nbx, 256, 0, s>>>(a);
 }
 
 }  // namespace msplat

@@ -77,7 +77,9 @@ struct msplat_hip_ctx {
     Buf splat_a, splat_b, splat_blue, depths, aabb, rect, radii, count, offsets, block_sums;
     Buf v_xy, v_conic, v_rgb, v_opac, sh_scratch, big;
     // per intersection
-    Buf keys0, keys1, vals0, vals1, slot_a, slot_b, slot_c, hist, digit_total;
+    Buf keys0, keys1, vals0, vals1, slot_a, slot_b, slot_c, hist, digit_total, keys64;
+    // visible gaussians sorted by depth, per-block counts and scan partials
+    Buf block_vis, dk0, dk1, dv0, dv1, ord_sums, scan_part;
     // per tile / pixel
     Buf tile_start, image, final_T, final_idx, v_image, pix_loss, row_loss;
     Buf grad_scratch;  // stands in for NULL gradient outputs of render_backward
@@ -133,7 +135,8 @@ struct msplat_hip_ctx {
         }
         marks.clear();
     }
-    // device scalars: [0] u64 isect total, [8] u32 total, [16] double loss sum, [24] float loss
+    // device scalars: [0] u64 isect total, [8] u32 total, [16] double loss sum, [24] float loss,
+    // [32] u32 big count, [40] u64 / [48] u32 visible gaussians, [56] / [64] the depth-order total
     Buf scalars;
     uint64_t *h_total = nullptr;  // pinned
     // frame state
@@ -141,7 +144,7 @@ struct msplat_hip_ctx {
     uint32_t width = 0, height = 0;
     int tiles_x = 0, tiles_y = 0;
     int64_t M = 0;
-    uint64_t *keys_sorted = nullptr;
+    uint32_t *keys_sorted = nullptr;  // tile ids of the sorted intersections
     int32_t *gid_sorted = nullptr;
     float bg[3] = {0, 0, 0};
 
@@ -170,7 +173,7 @@ struct msplat_hip_ctx {
         if (stream) (void)hipStreamSynchronize(stream);
         Buf *all[] = {&splat_a, &splat_b, &splat_blue, &depths, &aabb, &rect, &radii, &count, &offsets, &block_sums,
                       &v_xy, &v_conic, &v_rgb, &v_opac, &sh_scratch, &big, &keys0, &keys1, &vals0, &vals1, &slot_a, &slot_b, &slot_c,
-                      &hist, &digit_total, &tile_start, &image, &final_T, &final_idx, &v_image, &pix_loss,
+                      &hist, &digit_total, &keys64, &block_vis, &dk0, &dk1, &dv0, &dv1, &ord_sums, &scan_part, &tile_start, &image, &final_T, &final_idx, &v_image, &pix_loss,
                       &row_loss, &scalars, &grad_scratch, &aux};
         for (Buf *b : all) release(*b);
         if (batch) msplat::destroy_batch_state(batch);
@@ -186,6 +189,10 @@ struct msplat_hip_ctx {
     float4 *big_a() { return (float4 *)((char *)big.p + 4 * (size_t)(n > 0 ? n : 1) + 12); }
     uint32_t *d_total32() { return (uint32_t *)((char *)scalars.p + 8); }
     double *d_loss_sum() { return (double *)((char *)scalars.p + 16); }
+    uint64_t *d_vis64() { return (uint64_t *)((char *)scalars.p + 40); }
+    uint32_t *d_vis32() { return (uint32_t *)((char *)scalars.p + 48); }
+    uint64_t *d_order_total64() { return (uint64_t *)((char *)scalars.p + 56); }
+    uint32_t *d_order_total32() { return (uint32_t *)((char *)scalars.p + 64); }
     float *d_loss() { return (float *)((char *)scalars.p + 24); }
 };
 

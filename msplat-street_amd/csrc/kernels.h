@@ -20,20 +20,49 @@ struct ProjectArgs {
     float2 *aabb;
     uint2 *rect;
     int32_t *radii;
-    uint32_t *count, *block_sums;
+    uint32_t *count, *block_sums, *block_vis;  // per 256-gaussian block: intersections, visible gaussians
 };
 
-constexpr uint32_t BIG_SLOTS = 32;  // gaussians with more intersections reduce their slots per wave
+constexpr uint32_t BIG_SLOTS = 32;  // records with more intersections reduce their slots per wave
 
-struct KeygenArgs {
-    int n, tiles_x;
-    const uint32_t *count, *block_bases;
-    const uint2 *rect;
+struct BatchTable;
+
+// Output of a multi-level scan: totals, and a capacity check (total > cap sets
+// *overflow and makes total32 0; without overflow, total32 saturates).
+struct ScanOut {
+    uint64_t *total;
+    uint32_t *total32;
+    uint32_t cap;
+    uint32_t *overflow;
+    uint32_t *zero_also;  // a counter cleared by the same launch (or null)
+};
+
+// Visible records (count > 0) in id order with their depth bits as sort keys
+struct CompactArgs {
+    int n, num_views, nbx;  // records r = v * n + g, nbx blocks of 256 per view
+    const uint32_t *count;
     const float *depths;
+    const uint32_t *block_vis;  // scanned visible counts per (view, block)
+    uint32_t *dkeys;
+    int32_t *dvals;
+};
+
+// Key emission over the visible records in depth order
+struct KeygenArgs {
+    const int32_t *list;       // visible records, depth order
+    const uint32_t *nvis_ptr;  // device length of the list
+    uint32_t nvis_max;         // grid bound
+    const uint32_t *count;
+    const uint2 *rect;
+    const uint32_t *block_bases;  // scanned count sums per 256 list entries
     uint32_t *offsets;
-    uint64_t *keys;
-    int32_t *vals;
-    uint32_t *big_list, *big_count;  // gaussians with count > BIG_SLOTS (big_count zeroed by the scan)
+    uint32_t *keys;  // global tile id
+    int32_t *vals;   // record id
+    uint32_t *big_list, *big_count;  // records with count > BIG_SLOTS
+    const uint32_t *overflow;        // null, or skip everything when set
+    int tiles_x;                     // one view
+    const BatchTable *tab;           // batch: record r is in view r / n
+    int n;
 };
 
 struct RasterArgs {
@@ -160,13 +189,14 @@ struct BatchTable {
 
 // Device scalars of a batched frame
 struct BatchScalars {
-    uint64_t total;       // intersections
-    uint32_t total32;     // intersections, 0 when they exceed the capacity
-    uint32_t overflow;    // 1 when the intersections exceed the capacity: the step is void
-    uint32_t big_count;   // records with more than BIG_SLOTS intersections
-    uint32_t queue[2];    // persistent rasterizer work-queue heads (forward, backward)
-    uint32_t pad;
-    uint32_t view_total[MAX_VIEWS];
+    uint64_t total;      // intersections
+    uint64_t nvis64;     // visible records
+    uint32_t total32;    // intersections, 0 when they exceed the capacity
+    uint32_t overflow;   // 1 when the intersections exceed the capacity: the step is void
+    uint32_t big_count;  // records with more than BIG_SLOTS intersections
+    uint32_t nvis;       // visible records
+    uint32_t queue[2];   // persistent rasterizer work-queue heads (forward, backward)
+    uint32_t total32_b, overflow_b;  // the same totals from the depth-order scan
     double loss_sum[MAX_VIEWS];
     float loss_mean[MAX_VIEWS];
 };
@@ -179,21 +209,7 @@ struct BatchProjectArgs {
     float *splat_blue, *depths;
     uint2 *rect;
     int32_t *radii;
-    uint32_t *count, *block_sums;  // block_sums[v * nbx + b]
-};
-
-struct BatchBinArgs {
-    const BatchTable *tab;
-    int n, num_views, nbx;
-    uint32_t cap;
-    const uint32_t *count;
-    const uint2 *rect;
-    const float *depths;
-    uint32_t *block_sums, *offsets;
-    uint64_t *keys;
-    int32_t *vals;
-    uint32_t *big_list;
-    BatchScalars *sc;
+    uint32_t *count, *block_sums, *block_vis;  // [v * nbx + b]
 };
 
 struct BatchLossArgs {
@@ -227,9 +243,6 @@ struct BatchBackwardArgs {
 };
 
 void launch_batch_project(const BatchProjectArgs &a, hipStream_t s);
-void launch_batch_bin(const BatchBinArgs &a, hipStream_t s);  // view scans, totals, keys
-void launch_tile_ranges_dev(const uint64_t *keys, const uint32_t *n_dev, uint32_t n_max, const uint32_t *ntiles_dev,
-                            int32_t *tile_start, hipStream_t s);
 struct BatchRasterArgs {
     const BatchTable *tab;
     int num_views;
@@ -261,15 +274,24 @@ void launch_batch_adam(float *p, float *m1, float *m2, const float *grad, size_t
 
 void launch_project(const ProjectArgs &a, hipStream_t s);
 void launch_rect_count(int n, const float4 *splat_a, const float2 *aabb, const int32_t *radii, int tx, int ty,
-                       uint2 *rect, uint32_t *count, uint32_t *block_sums, hipStream_t s);
+                       uint2 *rect, uint32_t *count, uint32_t *block_sums, uint32_t *block_vis, hipStream_t s);
 void launch_scan_blocks(uint32_t *sums, int nblocks, uint64_t *total, uint32_t *total32, hipStream_t s,
                         uint32_t *zero_also = nullptr);
-void launch_keygen(const KeygenArgs &a, hipStream_t s);
+size_t scan_partial_words(uint32_t L);
+void launch_scan_u32(uint32_t *a, uint32_t L, uint32_t *partial, const ScanOut &o, hipStream_t s);
+void launch_compact(const CompactArgs &a, hipStream_t s);
+void launch_order_sums(const int32_t *list, const uint32_t *nvis_dev, uint32_t nvis_max, const uint32_t *count,
+                       uint32_t *sums, hipStream_t s);
+void launch_keygen(const KeygenArgs &a, bool batch, hipStream_t s);
 void launch_radix_pass(const uint64_t *kin, const int32_t *vin, uint64_t *kout, int32_t *vout, const uint32_t *n_dev,
                        uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s);
+void launch_radix_pass(const uint32_t *kin, const int32_t *vin, uint32_t *kout, int32_t *vout, const uint32_t *n_dev,
+                       uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s);
 size_t radix_hist_words(uint32_t n_max);
-void launch_tile_ranges(const uint64_t *keys, const uint32_t *n_dev, uint32_t n_max, int ntiles, int32_t *tile_start,
-                        hipStream_t s);
+void launch_tile_ranges(const uint32_t *keys, const uint32_t *n_dev, uint32_t n_max, const uint32_t *ntiles_dev,
+                        int ntiles, int32_t *tile_start, hipStream_t s);
+void launch_keys64(const uint32_t *tiles, const int32_t *gid, const float *depths, uint32_t n, uint64_t *out,
+                   hipStream_t s);
 void launch_raster_forward(const RasterArgs &a, hipStream_t s);
 void launch_raster_backward(const RasterArgs &a, int pixels_per_thread, hipStream_t s);
 void launch_loss(const LossArgs &a, hipStream_t s);

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libmsplat_hip.so")
 
 MSPLAT_OK = 0
 FRAME_KEEP_GRADS = 1
+FRAME_KEYS64 = 2
 TUNE_BWD_PPT = 0
 fp = C.POINTER(C.c_float)
 

@@ -74,11 +74,18 @@ def test_projection_bit_exact(ctx, oracle, deg):
     assert np.mean(a != b) < 0.01 and np.abs(a - b).max() <= 1
 
 
-def test_binning_bit_exact_on_oracle_inputs(ctx, oracle):
-    """Keys, the stable radix sort and the tile ranges, fed the oracle's projection."""
-    from msplat._abi import check, lib, make_camera_struct
+@pytest.mark.parametrize("ties", [False, True])
+def test_binning_bit_exact_on_oracle_inputs(ctx, oracle, ties):
+    """Keys, the stable sorts and the tile ranges, fed the oracle's projection. With ties,
+    a third of the gaussians repeat another's mean (equal depth bits): the order of equal
+    depths is the gaussian id order."""
+    from msplat._abi import FRAME_KEYS64, check, lib, make_camera_struct
     from msplat.backend import DeviceGaussians
     sc, cam = _scene_cam(seed=5, n=4000, deg=1, W=130, H=90)
+    if ties:
+        rng = np.random.default_rng(55)
+        dst = rng.choice(4000, 1300, replace=False)
+        sc["means"][dst] = sc["means"][rng.integers(0, 4000, 1300)]
     dg = DeviceGaussians(ctx, sc)
     g, c = dg.struct(), make_camera_struct(cam, 1)
     check(lib().msplat_hip_stage_project(ctx.ptr, C.byref(g), C.byref(c)))
@@ -93,7 +100,11 @@ def test_binning_bit_exact_on_oracle_inputs(ctx, oracle):
     ctx.upload(f.aabb, ref["aabb"])
     ctx.upload(f.radii, ref["radii"])
     check(lib().msplat_hip_stage_bin(ctx.ptr, C.byref(c)))
-    f = ctx.frame()
+    ctx.set_flags(FRAME_KEYS64)
+    try:
+        f = ctx.frame()
+    finally:
+        ctx.set_flags(0)
     bins = oracle.bin_sort(ref, sc["opac"], cam)
     M = bins["M"]
     assert f.num_isects == M and M > 1000
