@@ -191,6 +191,96 @@ def bench_batched(steps=8, warmup=2, seed=10):
     return out
 
 
+SERVE_SCENES = (500_000, 1_000_000, 2_000_000, 3_000_000)
+SERVE_RATES = (200, 500, 1000, 2000)
+
+
+def cluster_cloud(seed, n, center, clusters=64, sigma=0.5, box=8.0):
+    """A Gaussian mixture of `clusters` clusters (sigma in m) with centres uniform in a
+    box around `center`, and a colour per cluster."""
+    rng = np.random.default_rng(seed)
+    cc = np.asarray(center) + rng.uniform(-box / 2, box / 2, (clusters, 3))
+    col = rng.integers(30, 226, (clusters, 3))
+    lab = rng.integers(0, clusters, n)
+    pts = (cc[lab] + rng.normal(0, sigma, (n, 3))).astype(np.float32)
+    rgb = np.clip(col[lab] + rng.integers(-25, 26, (n, 3)), 0, 255).astype(np.uint8)
+    return pts, rgb
+
+
+def bench_serving(rates=SERVE_RATES, duration=1.0, slots=2, max_views=16, seed=20):
+    """Serving: four resident SH3 scenes (0.5M, 1M, 2M, 3M Gaussians, 64-cluster clouds in
+    an 8 m box, kNN scales, opacity logits N(1.5, 1), higher SH bands N(0, 0.05)).
+    Poisson arrivals at each rate for `duration` s; pose on a sphere of radius 3-10 m
+    around the scene's centre, resolution 640x360 40% / 1280x720 30% / 1920x1080 20% /
+    3840x2160 10%. The continuous-batching server (2 slots, up to 16 views per batch,
+    HBM-budget admission) against the plain form (one msplat_hip_render per request,
+    first come first served), each followed by the RGBA8 pack."""
+    from msplat._abi import Context
+    from msplat.backend import DeviceGaussians
+    from msplat.serving import Server, latency_stats, poisson_requests, serve_plain
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(seed)
+    centers = [(60.0 * k, 0.0, 0.0) for k in range(len(SERVE_SCENES))]
+    scenes = []
+    for k, (n, c) in enumerate(zip(SERVE_SCENES, centers)):
+        pts, rgb = cluster_cloud(seed + k, n, c)
+        dg = DeviceGaussians.from_points(ctx, pts, rgb, degree=3)
+        dg.params["opac"].upload(rng.normal(1.5, 1.0, (n, 1)).astype(np.float32))
+        dg.params["rest"].upload(rng.normal(0, 0.05, (n, 45)).astype(np.float32))
+        scenes.append(dg)
+    ctx.sync()
+    setup = time.perf_counter() - t0
+    srv = Server(scenes, slots=slots, max_views=max_views, hbm_budget=200e9, background=BG)
+    warm = poisson_requests(2000.0, 0.1, centers, seed=seed + 99)
+    srv.run(warm)
+    serve_plain(scenes, poisson_requests(500.0, 0.1, centers, seed=seed + 98), background=BG)
+    out = {"workload": "4 resident SH3 scenes (0.5M/1M/2M/3M Gaussians); Poisson arrivals; 640x360 40% / "
+                       "1280x720 30% / 1920x1080 20% / 3840x2160 10%; pose radius 3-10 m",
+           "slots": slots, "max_views": max_views, "duration_s": duration, "setup_s": round(setup, 1), "rates": {}}
+    for rate in rates:
+        reqs = poisson_requests(float(rate), duration, centers, seed=seed + rate)
+        b = latency_stats(srv.run(reqs))
+        b["rejected"] = len(srv.rejected)
+        plain = latency_stats(serve_plain(scenes, poisson_requests(float(rate), duration, centers, seed=seed + rate),
+                                          background=BG))
+        key = "1920x1080"
+        out["rates"][str(rate)] = {
+            "requests": len(reqs),
+            "batched": b, "plain": plain,
+            "p99_1080p_ms": {"batched": b["by_resolution"].get(key, {}).get("p99_ms"),
+                             "plain": plain["by_resolution"].get(key, {}).get("p99_ms")},
+        }
+        log(f"serving {rate} req/s: batched {b['views_per_s']} views/s p50 {b['p50_ms']} p99 {b['p99_ms']} ms; "
+            f"plain {plain['views_per_s']} views/s p50 {plain['p50_ms']} p99 {plain['p99_ms']} ms")
+    srv.close()
+    ctx.close()
+    return out
+
+
+def tune_train_step(ctx, cams, targets, seed=5):
+    """Picks the rasterizer variants (msplat.autotune) on the headline train step, run on
+    a scratch copy of the scene (the same street recipe) so the trainer's schedule is not
+    advanced; the choice stays set on the context."""
+    from msplat import autotune
+    from msplat.backend import DeviceGaussians, adam_coeffs, train_step
+    from msplat.scenes import street_points
+    pts, rgb = street_points(seed, N_GAUSSIANS)
+    scratch = DeviceGaussians.from_points(ctx, pts, rgb, degree=3)
+    lrs = [1.6e-4, 5e-3, 1e-3, 2.5e-3, 1.25e-4, 5e-2]
+    k = [0]
+
+    def step():
+        k[0] += 1
+        s_, b_ = adam_coeffs(lrs, k[0])
+        i = k[0] % len(cams)
+        train_step(ctx, scratch, dict(cams[i], dtu=3), BG, targets[i], s_, b_, read_loss=False)
+
+    res = autotune.autotune(ctx, step, warmup=3, repeats=12)
+    del scratch
+    return res
+
+
 def cpu_baseline_sample(dg, cams, targets, ctx, step_index, seconds=20.0):
     """Times the reference CPU train step (oracle/_ref, perf build) — or the oracle when
     the reference was not built — on this host, on the workload's own scene and view."""
@@ -239,6 +329,9 @@ def main():
     ap.add_argument("--bwd-ppt", type=int, default=0, help="backward pixels per thread (0: library default)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-views comparison")
     ap.add_argument("--batched-steps", type=int, default=8)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the library's default kernel variants")
+    ap.add_argument("--serving-seconds", type=float, default=1.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -258,6 +351,10 @@ def main():
     ctx.sync()
     setup_s = time.perf_counter() - t_setup
     log(f"workload ready in {setup_s:.1f} s: {dg.n} gaussians, {len(cams)} views")
+    tuning = None
+    if not args.no_autotune and not args.bwd_ppt:
+        tuning = tune_train_step(ctx, cams, targets)
+        log("autotuned: " + str({k: tuning[k] for k in ("bwd_ppt", "fwd_batch", "bwd_batch", "ms")}))
     cfg = TrainConfig(iterations=args.iterations, sh_degree=3, warmup_length=499, cull_alpha_thresh=0.005)
     tr = Trainer(ctx, dg, cams, targets, cfg)
     n_start = dg.n
@@ -302,6 +399,10 @@ def main():
         log("batched views: 1M gaussians, 16 views per launch")
         batched = bench_batched(args.batched_steps)
         log(f"batched: {batched['train_views_per_s']} train views/s, {batched['render_views_per_s']} render views/s")
+    serving = None
+    if not args.no_serving:
+        log("serving: 4 resident scenes, Poisson arrivals")
+        serving = bench_serving(duration=args.serving_seconds)
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU baseline")
@@ -316,9 +417,11 @@ def main():
                    "gaussians_start": n_start, "gaussians_end": dg.n, "steps_timed": [first_timed_step,
                                                                                       first_timed_step + K - 1],
                    "mean_intersections_per_step": int(m_total / K), "densify_events": tr.densify_log,
-                   "final_loss": loss, "setup_s": round(setup_s, 1)},
+                   "final_loss": loss, "setup_s": round(setup_s, 1),
+                   "tuning": {k: tuning[k] for k in ("bwd_ppt", "fwd_batch", "bwd_batch")} if tuning else None},
         "roofline": roofline,
         "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -157,9 +157,13 @@ int msplat_hip_set_flags(msplat_hip_ctx *ctx, int flags);
 
 /* Kernel variants picked per context (by the autotuner, msplat/autotune.py); no
  * reference counterpart (the Metal backend hard-codes its launch shapes,
- * msplat_metal.mm:431-436). MSPLAT_TUNE_BWD_PPT: pixels per thread of the backward
- * rasterizer, 1, 2 or 4. */
-enum { MSPLAT_TUNE_BWD_PPT = 0, MSPLAT_NUM_TUNABLES = 1 };
+ * msplat_metal.mm:431-436). Every variant computes the same result.
+ *   MSPLAT_TUNE_BWD_PPT    backward rasterizer pixels per thread: 1, 2, 4 (default 4)
+ *   MSPLAT_TUNE_FWD_BATCH  forward LDS batch depth (gaussian records staged per batch):
+ *                          64, 128, 256 (default 256)
+ *   MSPLAT_TUNE_BWD_BATCH  backward LDS batch depth: 32, 64, 128 (default 128)
+ * The binning tile stays 16x16: it decides which gaussians reach a pixel. */
+enum { MSPLAT_TUNE_BWD_PPT = 0, MSPLAT_TUNE_FWD_BATCH = 1, MSPLAT_TUNE_BWD_BATCH = 2, MSPLAT_NUM_TUNABLES = 3 };
 int msplat_hip_set_tuning(msplat_hip_ctx *ctx, int key, int value);
 int msplat_hip_get_tuning(msplat_hip_ctx *ctx, int key, int *value);
 
@@ -268,6 +272,9 @@ int msplat_hip_render_backward_batch(msplat_hip_ctx *ctx, const msplat_hip_gauss
 int msplat_hip_set_batch_graph(msplat_hip_ctx *ctx, int enable);
 /* Graph statistics: captures so far and replays so far. */
 int msplat_hip_batch_graph_stats(msplat_hip_ctx *ctx, int64_t *captures, int64_t *replays);
+/* Sizes of the last batched frame on this context: tile intersections and visible
+ * records (views x gaussians that touch a tile). */
+int msplat_hip_batch_frame_info(msplat_hip_ctx *ctx, uint64_t *num_isects, uint64_t *num_visible);
 
 #ifdef __cplusplus
 }

@@ -216,7 +216,7 @@ int do_raster_forward(msplat_hip_ctx *ctx, const float bg[3]) {
     for (int i = 0; i < 3; i++) ctx->bg[i] = bg ? bg[i] : 0.f;
     CHECKED(ensure_pixels(ctx));
     ctx->begin(2);
-    launch_raster_forward(raster_args(ctx), ctx->stream);
+    launch_raster_forward(raster_args(ctx), ctx->tuning, ctx->stream);
     ctx->end();
     HIPCHECK(hipGetLastError());
     return MSPLAT_OK;
@@ -248,7 +248,7 @@ int do_raster_backward(msplat_hip_ctx *ctx) {
     CHECKED(ctx->grow(ctx->slot_b, 16 * Mc));
     CHECKED(ctx->grow(ctx->slot_c, 4 * Mc));
     ctx->begin(4);
-    launch_raster_backward(raster_args(ctx), ctx->tune[MSPLAT_TUNE_BWD_PPT], ctx->stream);
+    launch_raster_backward(raster_args(ctx), ctx->tuning, ctx->stream);
     ctx->end();
     HIPCHECK(hipGetLastError());
     return MSPLAT_OK;
@@ -760,20 +760,35 @@ int msplat_hip_pack_rgba8(msplat_hip_ctx *ctx, const float *img, uint32_t n, uin
 
 int msplat_hip_set_tuning(msplat_hip_ctx *ctx, int key, int value) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
-    if (key == MSPLAT_TUNE_BWD_PPT) {
+    switch (key) {
+    case MSPLAT_TUNE_BWD_PPT:
         if (value != 1 && value != 2 && value != 4)
             return set_error(MSPLAT_ERR_ARG, "backward pixels per thread must be 1, 2 or 4");
-        ctx->tune[key] = value;
+        ctx->tuning.bwd_ppt = value;
         return MSPLAT_OK;
+    case MSPLAT_TUNE_FWD_BATCH:
+        if (value != 64 && value != 128 && value != 256)
+            return set_error(MSPLAT_ERR_ARG, "forward LDS batch depth must be 64, 128 or 256");
+        ctx->tuning.fwd_batch = value;
+        return MSPLAT_OK;
+    case MSPLAT_TUNE_BWD_BATCH:
+        if (value != 32 && value != 64 && value != 128)
+            return set_error(MSPLAT_ERR_ARG, "backward LDS batch depth must be 32, 64 or 128");
+        ctx->tuning.bwd_batch = value;
+        return MSPLAT_OK;
+    default:
+        return set_error(MSPLAT_ERR_ARG, "unknown tunable %d", key);
     }
-    return set_error(MSPLAT_ERR_ARG, "unknown tunable %d", key);
 }
 
 int msplat_hip_get_tuning(msplat_hip_ctx *ctx, int key, int *value) {
     if (!ctx || !value) return set_error(MSPLAT_ERR_ARG, "ctx or value is NULL");
-    if (key < 0 || key >= MSPLAT_NUM_TUNABLES) return set_error(MSPLAT_ERR_ARG, "unknown tunable %d", key);
-    *value = ctx->tune[key];
-    return MSPLAT_OK;
+    switch (key) {
+    case MSPLAT_TUNE_BWD_PPT: *value = ctx->tuning.bwd_ppt; return MSPLAT_OK;
+    case MSPLAT_TUNE_FWD_BATCH: *value = ctx->tuning.fwd_batch; return MSPLAT_OK;
+    case MSPLAT_TUNE_BWD_BATCH: *value = ctx->tuning.bwd_batch; return MSPLAT_OK;
+    default: return set_error(MSPLAT_ERR_ARG, "unknown tunable %d", key);
+    }
 }
 
 int msplat_hip_enable_stage_timing(msplat_hip_ctx *ctx, int enable) {

@@ -219,8 +219,8 @@ int prepare(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g, c
     b->n = n;
     b->views = nv;
     CHECKED(grow_sort_scratch(ctx, b));  // the record count may have grown
-    (void)batch_raster_grid(false, 1);  // occupancy queries outside any capture
-    (void)batch_raster_grid(true, ctx->tune[MSPLAT_TUNE_BWD_PPT]);
+    (void)batch_raster_grid(false, ctx->tuning);  // occupancy queries outside any capture
+    (void)batch_raster_grid(true, ctx->tuning);
     HIPCHECK(hipMemcpyAsync(b->table.p, b->h_tab, sizeof(BatchTable), hipMemcpyHostToDevice, ctx->stream));
     HIPCHECK(hipEventRecord(b->tab_done, ctx->stream));
     b->tab_pending = true;
@@ -369,14 +369,14 @@ void enqueue_forward(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussi
     }
     b->gid_sorted = vin;
     launch_tile_ranges(kin, &sc->total32, b->cap, &dtab(b)->total_tiles, 0, (int32_t *)b->tile_start.p, s);
-    launch_batch_raster_forward(raster_args(b), s);
+    launch_batch_raster_forward(raster_args(b), ctx->tuning, s);
 }
 
 // raster backward, big-list sums, per-gaussian sums over the views
 void enqueue_backward(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g, float *const gout[6],
                       const msplat_hip_grad_stats *stats) {
     hipStream_t s = ctx->stream;
-    launch_batch_raster_backward(raster_args(b), ctx->tune[MSPLAT_TUNE_BWD_PPT], s);
+    launch_batch_raster_backward(raster_args(b), ctx->tuning, s);
     const size_t R = record_count(b);
     float4 *ba = big_a(b);
     launch_big_reduce_records((const uint32_t *)b->big_list.p, &dsc(b)->big_count, (const uint32_t *)b->offsets.p,
@@ -491,7 +491,8 @@ std::vector<uint64_t> graph_key(msplat_hip_ctx *ctx, BatchState *b, const msplat
                                 const msplat_hip_adam *adam, const msplat_hip_grad_stats *st) {
     std::vector<uint64_t> k = {(uint64_t)b->n,          (uint64_t)b->views,       b->cap,
                                b->tiles_cap,            b->pixels_cap,            b->rows_cap,
-                               b->loss_cap,             b->generation,            (uint64_t)ctx->tune[MSPLAT_TUNE_BWD_PPT],
+                               b->loss_cap,             b->generation,            (uint64_t)ctx->tuning.bwd_ppt,
+                               (uint64_t)ctx->tuning.fwd_batch, (uint64_t)ctx->tuning.bwd_batch,
                                b->h_tab->views[0].cam.degree};
     const void *ptrs[] = {g->means, g->scales, g->quats, g->features_dc, g->features_rest, g->opacities,
                           st->vis_counts, st->xys_grad_norm, st->max_2d_size};
@@ -635,6 +636,14 @@ int msplat_hip_batch_graph_stats(msplat_hip_ctx *ctx, int64_t *captures, int64_t
     BatchState *b = ctx->batch;
     if (captures) *captures = b ? b->captures : 0;
     if (replays) *replays = b ? b->replays : 0;
+    return MSPLAT_OK;
+}
+
+int msplat_hip_batch_frame_info(msplat_hip_ctx *ctx, uint64_t *num_isects, uint64_t *num_visible) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    BatchState *b = ctx->batch;
+    if (num_isects) *num_isects = b ? b->h_sc->total : 0;
+    if (num_visible) *num_visible = b ? b->h_sc->nvis64 : 0;
     return MSPLAT_OK;
 }
 

@@ -194,6 +194,44 @@ __global__ __launch_bounds__(256) void scan_top_kernel(uint32_t *partial, int nc
     }
 }
 
+// The whole scan in one workgroup, 4096 entries at a time (small arrays)
+__global__ __launch_bounds__(256) void scan_small_kernel(uint32_t *a, uint32_t L, ScanOut o) {
+    __shared__ uint32_t lds4[4];
+    __shared__ uint64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t chunk = 0; chunk < L; chunk += SCAN_CHUNK) {
+        const uint32_t base = chunk + threadIdx.x * 16;
+        uint32_t v[16], sum = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            v[i] = base + i < L ? a[base + i] : 0u;
+            sum += v[i];
+        }
+        uint32_t t;
+        const uint32_t ex = block_scan_u32(sum, t, lds4);
+        const uint64_t c = carry;
+        uint32_t run = (uint32_t)c + ex;
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if (base + i < L) {
+                a[base + i] = run;
+                run += v[i];
+            }
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + t;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint64_t tot = carry;
+        const bool over = tot > (uint64_t)o.cap;
+        if (o.total) *o.total = tot;
+        if (o.total32) *o.total32 = over ? (o.overflow ? 0u : 0xFFFFFFFFu) : (uint32_t)tot;
+        if (o.overflow) *o.overflow = over ? 1u : 0u;
+        if (o.zero_also) *o.zero_also = 0u;
+    }
+}
+
 __global__ __launch_bounds__(256) void scan_apply_kernel(uint32_t *a, uint32_t L, const uint32_t *partial) {
     __shared__ uint32_t lds4[4];
     const uint32_t base = blockIdx.x * SCAN_CHUNK + threadIdx.x * 16;
@@ -271,14 +309,24 @@ __global__ __launch_bounds__(256) void rs_scan_kernel(uint32_t *hist, int nblock
     if (threadIdx.x == 0) digit_total[blockIdx.x] = carry;
 }
 
+// Orders LDS accesses of the lanes of one wave (no instruction is emitted).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wave w ranks its own contiguous quarter of the workgroup's elements, 64 at a time,
+// with private per-digit counters (no workgroup barrier inside the loop); the waves'
+// counts are then combined in wave order, which keeps the sort stable.
 template <typename K>
 __global__ __launch_bounds__(256) void rs_scatter_kernel(const K *kin, const int32_t *vin, K *kout, int32_t *vout,
                                                         const uint32_t *n_ptr, int shift, const uint32_t *hist,
                                                         int nblocks, const uint32_t *digit_total) {
-    constexpr int R = RS_ITEMS / 256;
+    constexpr int R = RS_ITEMS / 256;  // rounds of 64 elements per wave
     __shared__ K sk[RS_ITEMS];
     __shared__ int32_t sv[RS_ITEMS];
-    __shared__ uint32_t gbase[256], lstart[256], run[256];
+    __shared__ uint32_t gbase[256], lstart[256];
     __shared__ uint32_t wcnt[4][256];
     __shared__ uint32_t lds4[4];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -290,21 +338,24 @@ __global__ __launch_bounds__(256) void rs_scatter_kernel(const K *kin, const int
         const uint32_t ex = block_scan_u32(digit_total[tid], t, lds4);
         gbase[tid] = ex + hist[(size_t)tid * nblocks + blockIdx.x];
     }
-    run[tid] = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) wcnt[w][lane + 64 * i] = 0;
+    const uint32_t wstart = start + (uint32_t)w * (64 * R);
     K key[R];
     int32_t val[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        const uint32_t e = start + r * 256 + tid;
+        const uint32_t e = wstart + r * 64 + lane;
         key[r] = e < n ? kin[e] : K(0);
         val[r] = e < n ? vin[e] : 0;
     }
-    // per element: digit (bits 24..31) and rank within its digit in element order
+    wave_sync();
+    // per element: digit (bits 24..31) and rank among the wave's elements of that digit
     uint32_t dr[R];
     const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        const bool valid = start + r * 256 + tid < n;
+        const bool valid = wstart + r * 64 + lane < n;
         const uint32_t digit = valid ? (uint32_t)(key[r] >> shift) & 0xFF : 0;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -313,34 +364,28 @@ __global__ __launch_bounds__(256) void rs_scatter_kernel(const K *kin, const int
             const uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
         }
-        const uint32_t rank = __popcll(peers & lt);
-#pragma unroll
-        for (int i = 0; i < 4; i++) wcnt[i][tid] = 0;
-        __syncthreads();
-        if (valid && rank == 0) wcnt[w][digit] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {   // counts -> exclusive offsets over the four waves, per digit
-            const uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
-            const uint32_t base = run[tid];
-            wcnt[0][tid] = base;
-            wcnt[1][tid] = base + c0;
-            wcnt[2][tid] = base + c0 + c1;
-            wcnt[3][tid] = base + c0 + c1 + c2;
-            run[tid] = base + c0 + c1 + c2 + c3;
-        }
-        __syncthreads();
-        dr[r] = (digit << 24) | (valid ? wcnt[w][digit] + rank : 0u);
-        __syncthreads();
+        const uint32_t before = valid ? wcnt[w][digit] : 0u;
+        wave_sync();
+        if (valid && (peers & lt) == 0) wcnt[w][digit] = before + (uint32_t)__popcll(peers);
+        wave_sync();
+        dr[r] = (digit << 24) | (before + (uint32_t)__popcll(peers & lt));
     }
-    {   // local start of each digit: exclusive scan of the block's digit counts
+    __syncthreads();
+    {   // per digit: wave offsets (in wave order) and the block's local start
+        const uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
         uint32_t t;
-        lstart[tid] = block_scan_u32(run[tid], t, lds4);
+        const uint32_t ls = block_scan_u32(c0 + c1 + c2 + c3, t, lds4);
+        lstart[tid] = ls;
+        wcnt[0][tid] = ls;
+        wcnt[1][tid] = ls + c0;
+        wcnt[2][tid] = ls + c0 + c1;
+        wcnt[3][tid] = ls + c0 + c1 + c2;
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        if (start + r * 256 + tid < n) {
-            const uint32_t p = lstart[dr[r] >> 24] + (dr[r] & 0xFFFFFF);
+        if (wstart + r * 64 + lane < n) {
+            const uint32_t p = wcnt[w][dr[r] >> 24] + (dr[r] & 0xFFFFFF);
             sk[p] = key[r];
             sv[p] = val[r];
         }
@@ -521,6 +566,10 @@ __global__ __launch_bounds__(256) void project_batch_kernel(BatchProjectArgs a) 
 size_t scan_partial_words(uint32_t L) { return (size_t)(L + SCAN_CHUNK - 1) / SCAN_CHUNK + 1; }
 
 void launch_scan_u32(uint32_t *a, uint32_t L, uint32_t *partial, const ScanOut &o, hipStream_t s) {
+    if (L <= 2 * SCAN_CHUNK) {  // one launch
+        scan_small_kernel<<<1, 256, 0, s>>>(a, L, o);
+        return;
+    }
     const int nchunks = (int)((L + SCAN_CHUNK - 1) / SCAN_CHUNK);
     if (nchunks > 0) scan_partial_kernel<<<nchunks, 256, 0, s>>>(a, L, partial);
     scan_top_kernel<<<1, 256, 0, s>>>(partial, nchunks, o);

@@ -72,7 +72,7 @@ struct msplat_hip_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int flags = 0;
-    int tune[MSPLAT_NUM_TUNABLES] = {4};
+    msplat::Tuning tuning;
     // per gaussian
     Buf splat_a, splat_b, splat_blue, depths, aabb, rect, radii, count, offsets, block_sums;
     Buf v_xy, v_conic, v_rgb, v_opac, sh_scratch, big;

@@ -11,6 +11,13 @@ namespace msplat {
 
 constexpr int RS_ITEMS = 4096;  // radix-sort elements per block (256 threads x 16)
 
+// Kernel variants of the rasterizer, picked per context (msplat_hip_set_tuning)
+struct Tuning {
+    int bwd_ppt = 4;      // backward pixels per thread: 1, 2, 4
+    int fwd_batch = 256;  // forward LDS batch depth: 64, 128, 256
+    int bwd_batch = 128;  // backward LDS batch depth: 32, 64, 128
+};
+
 struct ProjectArgs {
     int n;
     const float *means, *scales, *quats, *dc, *rest, *opac;
@@ -258,9 +265,9 @@ struct BatchRasterArgs {
     float4 *slot_a, *slot_b;
     float *slot_c;
 };
-int batch_raster_grid(bool backward, int ppt);  // persistent workgroups that fill the device
-void launch_batch_raster_forward(const BatchRasterArgs &a, hipStream_t s);
-void launch_batch_raster_backward(const BatchRasterArgs &a, int ppt, hipStream_t s);
+int batch_raster_grid(bool backward, const Tuning &t);  // persistent workgroups that fill the device
+void launch_batch_raster_forward(const BatchRasterArgs &a, const Tuning &t, hipStream_t s);
+void launch_batch_raster_backward(const BatchRasterArgs &a, const Tuning &t, hipStream_t s);
 void launch_batch_loss(const BatchLossArgs &a, hipStream_t s);
 void launch_big_reduce_records(const uint32_t *big_list, const uint32_t *big_count, const uint32_t *offsets,
                                const uint32_t *count, const float4 *slot_a, const float4 *slot_b, const float *slot_c,
@@ -292,8 +299,8 @@ void launch_tile_ranges(const uint32_t *keys, const uint32_t *n_dev, uint32_t n_
                         int ntiles, int32_t *tile_start, hipStream_t s);
 void launch_keys64(const uint32_t *tiles, const int32_t *gid, const float *depths, uint32_t n, uint64_t *out,
                    hipStream_t s);
-void launch_raster_forward(const RasterArgs &a, hipStream_t s);
-void launch_raster_backward(const RasterArgs &a, int pixels_per_thread, hipStream_t s);
+void launch_raster_forward(const RasterArgs &a, const Tuning &t, hipStream_t s);
+void launch_raster_backward(const RasterArgs &a, const Tuning &t, hipStream_t s);
 void launch_loss(const LossArgs &a, hipStream_t s);
 void launch_backward(const BackwardArgs &a, BackwardMode mode, hipStream_t s);
 

@@ -58,17 +58,19 @@ __device__ inline uint32_t next_item(uint32_t *queue, uint32_t *slot) {
 
 }  // namespace
 
+template <int FB>
 __global__ __launch_bounds__(256) void raster_forward_kernel(RasterArgs a) {
-    forward_tile(tile_view(a), tile_data(a), blockIdx.x, blockIdx.x);
+    forward_tile<FB>(tile_view(a), tile_data(a), blockIdx.x, blockIdx.x);
 }
 
-template <int PPT>
+template <int PPT, int BB>
 __global__ __launch_bounds__(256 / PPT) void raster_backward_kernel(RasterArgs a) {
-    backward_tile<PPT>(tile_view(a), tile_data(a), blockIdx.x, blockIdx.x);
+    backward_tile<PPT, BB>(tile_view(a), tile_data(a), blockIdx.x, blockIdx.x);
 }
 
 // Persistent forward over every (view, tile) of a batch. Every workgroup loops until
 // the queue passes the batch's tile count, so the grid drains on its own.
+template <int FB>
 __global__ __launch_bounds__(256) void raster_forward_batch_kernel(BatchRasterArgs a) {
     __shared__ uint32_t tb[MAX_VIEWS + 1];
     __shared__ uint32_t item_slot;
@@ -80,11 +82,11 @@ __global__ __launch_bounds__(256) void raster_forward_batch_kernel(BatchRasterAr
         const uint32_t t = next_item(a.queue, &item_slot);
         if (t >= total) break;
         const int v = view_of(tb, nv, t);
-        forward_tile(tile_view(a, v), d, (int)t, (int)(t - tb[v]));
+        forward_tile<FB>(tile_view(a, v), d, (int)t, (int)(t - tb[v]));
     }
 }
 
-template <int PPT>
+template <int PPT, int BB>
 __global__ __launch_bounds__(256 / PPT) void raster_backward_batch_kernel(BatchRasterArgs a) {
     __shared__ uint32_t tb[MAX_VIEWS + 1];
     __shared__ uint32_t item_slot;
@@ -96,24 +98,37 @@ __global__ __launch_bounds__(256 / PPT) void raster_backward_batch_kernel(BatchR
         const uint32_t t = next_item(a.queue + 1, &item_slot);
         if (t >= total) break;
         const int v = view_of(tb, nv, t);
-        backward_tile<PPT>(tile_view(a, v), d, (int)t, (int)(t - tb[v]));
+        backward_tile<PPT, BB>(tile_view(a, v), d, (int)t, (int)(t - tb[v]));
     }
 }
 
-void launch_raster_forward(const RasterArgs &a, hipStream_t s) {
-    const int nt = a.tiles_x * a.tiles_y;
-    if (nt > 0) raster_forward_kernel<<<nt, 256, 0, s>>>(a);
+// Kernel variants by (pixels per thread, LDS batch depth), picked at launch
+template <int FB>
+static void forward_one(const RasterArgs &a, int nt, hipStream_t s) {
+    raster_forward_kernel<FB><<<nt, 256, 0, s>>>(a);
 }
 
-void launch_raster_backward(const RasterArgs &a, int ppt, hipStream_t s) {
+void launch_raster_forward(const RasterArgs &a, const Tuning &t, hipStream_t s) {
     const int nt = a.tiles_x * a.tiles_y;
     if (nt <= 0) return;
-    if (ppt == 4)
-        raster_backward_kernel<4><<<nt, 64, 0, s>>>(a);
-    else if (ppt == 2)
-        raster_backward_kernel<2><<<nt, 128, 0, s>>>(a);
-    else
-        raster_backward_kernel<1><<<nt, 256, 0, s>>>(a);
+    if (t.fwd_batch == 64) forward_one<64>(a, nt, s);
+    else if (t.fwd_batch == 128) forward_one<128>(a, nt, s);
+    else forward_one<256>(a, nt, s);
+}
+
+template <int PPT>
+static void backward_ppt(const RasterArgs &a, int nt, int bb, hipStream_t s) {
+    if (bb == 32) raster_backward_kernel<PPT, 32><<<nt, 256 / PPT, 0, s>>>(a);
+    else if (bb == 64) raster_backward_kernel<PPT, 64><<<nt, 256 / PPT, 0, s>>>(a);
+    else raster_backward_kernel<PPT, 128><<<nt, 256 / PPT, 0, s>>>(a);
+}
+
+void launch_raster_backward(const RasterArgs &a, const Tuning &t, hipStream_t s) {
+    const int nt = a.tiles_x * a.tiles_y;
+    if (nt <= 0) return;
+    if (t.bwd_ppt == 4) backward_ppt<4>(a, nt, t.bwd_batch, s);
+    else if (t.bwd_ppt == 2) backward_ppt<2>(a, nt, t.bwd_batch, s);
+    else backward_ppt<1>(a, nt, t.bwd_batch, s);
 }
 
 namespace {
@@ -139,31 +154,52 @@ int resident_blocks(K kernel, int threads) {
 
 }  // namespace
 
-int batch_raster_grid(bool backward, int ppt) {
-    static int g[4] = {0, 0, 0, 0};
-    const int i = !backward ? 0 : ppt == 4 ? 3 : ppt == 2 ? 2 : 1;
-    if (!g[i]) {
-        if (i == 0) g[i] = resident_blocks(raster_forward_batch_kernel, 256);
-        else if (i == 1) g[i] = resident_blocks(raster_backward_batch_kernel<1>, 256);
-        else if (i == 2) g[i] = resident_blocks(raster_backward_batch_kernel<2>, 128);
-        else g[i] = resident_blocks(raster_backward_batch_kernel<4>, 64);
-    }
-    return g[i];
+template <int FB>
+static int fwd_grid() {
+    static int g = 0;
+    if (!g) g = resident_blocks(raster_forward_batch_kernel<FB>, 256);
+    return g;
 }
 
-void launch_batch_raster_forward(const BatchRasterArgs &a, hipStream_t s) {
+template <int PPT, int BB>
+static int bwd_grid() {
+    static int g = 0;
+    if (!g) g = resident_blocks(raster_backward_batch_kernel<PPT, BB>, 256 / PPT);
+    return g;
+}
+
+template <int PPT>
+static int bwd_grid_ppt(int bb) {
+    return bb == 32 ? bwd_grid<PPT, 32>() : bb == 64 ? bwd_grid<PPT, 64>() : bwd_grid<PPT, 128>();
+}
+
+int batch_raster_grid(bool backward, const Tuning &t) {
+    if (!backward) return t.fwd_batch == 64 ? fwd_grid<64>() : t.fwd_batch == 128 ? fwd_grid<128>() : fwd_grid<256>();
+    return t.bwd_ppt == 4 ? bwd_grid_ppt<4>(t.bwd_batch) : t.bwd_ppt == 2 ? bwd_grid_ppt<2>(t.bwd_batch)
+                                                                          : bwd_grid_ppt<1>(t.bwd_batch);
+}
+
+void launch_batch_raster_forward(const BatchRasterArgs &a, const Tuning &t, hipStream_t s) {
     fill_u32<<<1, 1, 0, s>>>(a.queue, 0u);
-    raster_forward_batch_kernel<<<batch_raster_grid(false, 1), 256, 0, s>>>(a);
+    const int g = batch_raster_grid(false, t);
+    if (t.fwd_batch == 64) raster_forward_batch_kernel<64><<<g, 256, 0, s>>>(a);
+    else if (t.fwd_batch == 128) raster_forward_batch_kernel<128><<<g, 256, 0, s>>>(a);
+    else raster_forward_batch_kernel<256><<<g, 256, 0, s>>>(a);
 }
 
-void launch_batch_raster_backward(const BatchRasterArgs &a, int ppt, hipStream_t s) {
+template <int PPT>
+static void batch_backward_ppt(const BatchRasterArgs &a, int g, int bb, hipStream_t s) {
+    if (bb == 32) raster_backward_batch_kernel<PPT, 32><<<g, 256 / PPT, 0, s>>>(a);
+    else if (bb == 64) raster_backward_batch_kernel<PPT, 64><<<g, 256 / PPT, 0, s>>>(a);
+    else raster_backward_batch_kernel<PPT, 128><<<g, 256 / PPT, 0, s>>>(a);
+}
+
+void launch_batch_raster_backward(const BatchRasterArgs &a, const Tuning &t, hipStream_t s) {
     fill_u32<<<1, 1, 0, s>>>(a.queue + 1, 0u);
-    if (ppt == 4)
-        raster_backward_batch_kernel<4><<<batch_raster_grid(true, 4), 64, 0, s>>>(a);
-    else if (ppt == 2)
-        raster_backward_batch_kernel<2><<<batch_raster_grid(true, 2), 128, 0, s>>>(a);
-    else
-        raster_backward_batch_kernel<1><<<batch_raster_grid(true, 1), 256, 0, s>>>(a);
+    const int g = batch_raster_grid(true, t);
+    if (t.bwd_ppt == 4) batch_backward_ppt<4>(a, g, t.bwd_batch, s);
+    else if (t.bwd_ppt == 2) batch_backward_ppt<2>(a, g, t.bwd_batch, s);
+    else batch_backward_ppt<1>(a, g, t.bwd_batch, s);
 }
 
 }  // namespace msplat

@@ -4,7 +4,7 @@
 //
 // Forward: 256 threads per 16x16 tile; each wave64 owns an 8x8 quadrant so that a
 // wave's pixels are spatially tight. Gaussian records of the tile's sorted list are
-// staged through LDS in batches of FWD_BATCH.
+// staged through LDS in batches (the LDS batch depth is a kernel variant).
 //
 // Backward without global atomics: per gaussian of a batch, the wave's contributions
 // are summed with DPP only when some lane contributed (ballot), written to a per-wave
@@ -18,9 +18,6 @@
 #include "kernels.h"
 
 namespace msplat {
-
-constexpr int FWD_BATCH = 256;
-constexpr int BWD_BATCH = 128;
 
 // The pixels of one view: `image`, `final_T`, `final_idx` and `v_image` point at the
 // view's first pixel.
@@ -46,11 +43,14 @@ struct TileData {
 };
 
 // nd_rasterize_forward (core/cpu/msplat_cpu.cpp:574-626) for global tile `gt`, which is
-// tile `lt` of view `v`. Every thread of the 256-thread workgroup calls it.
+// tile `lt` of view `v`. Every thread of the 256-thread workgroup calls it; FB gaussian
+// records are staged in LDS per batch (FB <= 256: one record per thread).
+template <int FB>
 __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt, int lt) {
 #pragma clang fp contract(fast)
-    __shared__ float4 la[FWD_BATCH], lb[FWD_BATCH];
-    __shared__ float lc[FWD_BATCH];
+    static_assert(FB <= 256, "one staged record per thread");
+    __shared__ float4 la[FB], lb[FB];
+    __shared__ float lc[FB];
     const int tx = lt % v.tiles_x, ty = lt / v.tiles_x;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int px = tx * TILE + (w & 1) * 8 + (lane & 7);
@@ -61,10 +61,10 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
     float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f;
     int last = s - 1;
     bool done = !inside;
-    for (int base = s; base < e; base += FWD_BATCH) {
+    for (int base = s; base < e; base += FB) {
         if (__syncthreads_count(done) == 256) break;
         const int k = base + threadIdx.x;
-        if (k < e) {
+        if (threadIdx.x < FB && k < e) {
             const int g = d.gid[k];
             la[threadIdx.x] = d.splat_a[g];
             lb[threadIdx.x] = d.splat_b[g];
@@ -72,7 +72,7 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
         }
         __syncthreads();
         if (__ballot(!done) != 0) {
-            const int n = min(FWD_BATCH, e - base);
+            const int n = min(FB, e - base);
             for (int j = 0; j < n; j++) {
                 // predicated compositing step: skip (out of range / too faint), accept, or
                 // terminate (transmittance would drop to T_MIN: last = k - 1, as the CPU)
@@ -124,10 +124,10 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
 // partial per wave keeps the LDS footprint small enough for several waves per SIMD.
 constexpr int NQ = 9;  // sum vs, vs dx, vs dy, vs dx^2, vs dx dy, vs dy^2, fac vr, fac vg, fac vb
 
-template <int PPT>
+template <int PPT, int BATCH>
 __device__ inline void backward_tile(const TileView &v, const TileData &d, int gt, int lt) {
 #pragma clang fp contract(fast)
-    constexpr int NT = 256 / PPT, NW = NT / 64, BATCH = BWD_BATCH;
+    constexpr int NT = 256 / PPT, NW = NT / 64;
     __shared__ float4 la[BATCH], lb[BATCH];
     __shared__ float lc[BATCH];
     __shared__ uint32_t lslot[BATCH];

@@ -18,6 +18,8 @@ MSPLAT_OK = 0
 FRAME_KEEP_GRADS = 1
 FRAME_KEYS64 = 2
 TUNE_BWD_PPT = 0
+TUNE_FWD_BATCH = 1
+TUNE_BWD_BATCH = 2
 fp = C.POINTER(C.c_float)
 
 
@@ -127,6 +129,7 @@ def lib():
                                                   C.POINTER(Grads)], C.c_int),
             "msplat_hip_set_batch_graph": ([vp, C.c_int], C.c_int),
             "msplat_hip_batch_graph_stats": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+            "msplat_hip_batch_frame_info": ([vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -195,6 +198,12 @@ class Context:
         c, r = C.c_int64(), C.c_int64()
         check(lib().msplat_hip_batch_graph_stats(self.ptr, C.byref(c), C.byref(r)))
         return c.value, r.value
+
+    def batch_isects(self) -> int:
+        """Tile intersections of the last batched frame on this context."""
+        m, v = C.c_uint64(), C.c_uint64()
+        check(lib().msplat_hip_batch_frame_info(self.ptr, C.byref(m), C.byref(v)))
+        return m.value
 
     def download(self, ptr, shape, dtype) -> np.ndarray:
         out = np.empty(shape, dtype)

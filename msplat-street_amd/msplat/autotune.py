@@ -1,0 +1,92 @@
+"""Autotuner for the rasterizer's kernel variants (msplat_hip_set_tuning).
+
+The tunables are the backward rasterizer's pixels per thread (the work tile of one lane)
+and the forward and backward LDS batch depths (gaussian records staged per batch).
+Every variant computes the same result (images and gradients bit for bit for the batch
+depths; the pixels-per-thread variants within float rounding of the per-wave sums), so
+the choice only moves time. The tuner times the caller's own step (a train step, a
+render, a batched step) on the context's stream: coordinate descent over the tunables,
+each tried at every value with the others at their best so far, a few timed repeats per
+trial after a warm-up, the fastest kept. A result can be cached in a JSON file keyed by
+device and workload.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+
+from ._abi import TUNE_BWD_BATCH, TUNE_BWD_PPT, TUNE_FWD_BATCH, Context
+
+SPACE = {
+    "bwd_ppt": (TUNE_BWD_PPT, (4, 2, 1)),
+    "fwd_batch": (TUNE_FWD_BATCH, (256, 128, 64)),
+    "bwd_batch": (TUNE_BWD_BATCH, (128, 64, 32)),
+}
+
+
+def current(ctx: Context) -> dict:
+    return {name: ctx.get_tuning(key) for name, (key, _) in SPACE.items()}
+
+
+def apply(ctx: Context, choice: dict):
+    for name, value in choice.items():
+        if name in SPACE:
+            ctx.set_tuning(SPACE[name][0], int(value))
+
+
+def _time(ctx: Context, step, warmup: int, repeats: int) -> float:
+    for _ in range(warmup):
+        step()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        step()
+    ctx.sync()
+    return (time.perf_counter() - t0) / repeats * 1e3
+
+
+def autotune(ctx: Context, step, warmup: int = 2, repeats: int = 8, space: dict | None = None,
+             cache: str | None = None, key: str | None = None, rounds: int = 1) -> dict:
+    """Picks the fastest variant for `step()` (a callable that enqueues one unit of the
+    workload on ctx) and leaves it set on ctx. Returns {tunable: value, ..., "ms": best
+    ms per step, "trials": [(choice, ms), ...], "cached": bool}."""
+    space = space or SPACE
+    if cache and key:
+        try:
+            with open(cache) as f:
+                hit = json.load(f).get(key)
+        except (OSError, ValueError):
+            hit = None
+        if hit:
+            apply(ctx, hit)
+            return dict(hit, cached=True)
+    best = current(ctx)
+    best_ms = _time(ctx, step, warmup, repeats)
+    trials = [(dict(best), best_ms)]
+    for _ in range(rounds):
+        for name, (tkey, values) in space.items():
+            for v in values:
+                if v == best[name]:
+                    continue
+                trial = dict(best, **{name: v})
+                apply(ctx, trial)
+                ms = _time(ctx, step, warmup, repeats)
+                trials.append((trial, ms))
+                if ms < best_ms:
+                    best, best_ms = trial, ms
+            apply(ctx, best)
+    out = dict(best, ms=round(best_ms, 4), trials=[(c, round(m, 4)) for c, m in trials], cached=False)
+    if cache and key:
+        try:
+            data = {}
+            if os.path.exists(cache):
+                with open(cache) as f:
+                    data = json.load(f)
+            data[key] = dict(best, ms=round(best_ms, 4))
+            os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+            with open(cache, "w") as f:
+                json.dump(data, f, indent=1)
+        except (OSError, ValueError):
+            pass
+    return out

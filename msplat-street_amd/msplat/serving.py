@@ -1,0 +1,266 @@
+"""Serving novel views of resident scenes with continuous batching.
+
+Trained scenes stay resident in HBM. Requests (scene, camera, resolution) arrive
+continuously; a request waits in the pending queue until a worker slot is free and the
+HBM budget admits it, and is then rendered together with the other pending requests of
+its scene in one msplat_hip_render_batch (mixed resolutions, one persistent
+rasterizer launch) plus one RGBA8 pack (msplat_pack_rgba8, core/cpu/msplat_cpu.cpp:
+1723-1734). Each slot owns a context (its own stream and buffers) and runs in its own
+thread; the library releases the GIL during its calls, so the batches of different slots
+overlap on the device, and requests that arrive while batches are in flight are admitted
+as slots and memory free up. The plain form renders one request per msplat_hip_render,
+first come first served, on one context.
+
+Admission: the resident scenes' parameters plus every slot's working set stay within the
+HBM budget; a batch's working set is estimated from its record count (views x
+gaussians), pixels and intersections (pixels x the scene's measured intersections per
+pixel).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import threading
+import time
+
+import numpy as np
+
+from ._abi import Context, DeviceArray, check, floats, lib, make_camera_struct
+from .backend import BatchViews, DeviceGaussians, render_batch
+
+REC_BYTES = 116    # per record (view x gaussian) of a batched frame
+PIX_BYTES = 40     # per pixel: image, final T and index, gradient image, loss, RGBA8
+ISECT_BYTES = 52   # per intersection: keys, ids (double-buffered), gradient slots
+PARAM_BYTES = 236  # per resident SH3 gaussian (59 floats)
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    scene: int
+    cam: dict
+    arrival: float                 # seconds from the start of the run
+    launched: float = float("nan")
+    done: float = float("nan")
+    batch_size: int = 0
+    slot: int = -1
+    image: np.ndarray | None = None  # RGBA8, kept when the server keeps images
+
+    @property
+    def pixels(self) -> int:
+        return int(self.cam["W"]) * int(self.cam["H"])
+
+
+class _Slot:
+    def __init__(self, index: int, device: int):
+        self.index = index
+        self.ctx = Context(device)
+        self.rgba = None
+        self.busy = False
+
+    def rgba_buffer(self, pixels: int) -> DeviceArray:
+        if self.rgba is None or self.rgba.nbytes < 4 * pixels:
+            self.rgba = DeviceArray(self.ctx, (int(pixels * 1.25) + 1024, 4), np.uint8)
+        return self.rgba
+
+
+class Server:
+    """Continuous-batching renderer over resident scenes (DeviceGaussians)."""
+
+    def __init__(self, scenes: list, slots: int = 2, max_views: int = 16, hbm_budget: float = 64e9,
+                 background=(0.0, 0.0, 0.0), device: int = 0, keep_images: bool = False):
+        self.scenes = scenes
+        self.max_views = int(max_views)
+        self.bg = tuple(float(v) for v in background)
+        self.keep_images = keep_images
+        self.resident = sum(s.n * PARAM_BYTES for s in scenes)
+        self.budget = float(hbm_budget)
+        if self.resident >= self.budget:
+            raise ValueError("the resident scenes alone exceed the HBM budget")
+        self.slots = [_Slot(i, device) for i in range(slots)]
+        self.share = (self.budget - self.resident) / slots
+        self.ipp = [60.0] * len(scenes)  # intersections per pixel, refined from each frame
+        self.pending: list[Request] = []
+        self.finished: list[Request] = []
+        self.rejected: list[Request] = []
+        self.cond = threading.Condition()
+        self.stop = False
+        self.t0 = 0.0
+        self.batches = 0
+        self.errors: list[str] = []
+
+    # ── admission ────────────────────────────────────────────────────────────
+    def working_set(self, reqs) -> float:
+        s = self.scenes[reqs[0].scene]
+        px = sum(r.pixels for r in reqs)
+        return len(reqs) * s.n * REC_BYTES + px * PIX_BYTES + px * self.ipp[reqs[0].scene] * ISECT_BYTES
+
+    def _take_batch(self) -> list:
+        """The oldest pending request and the later ones of its scene, in arrival order,
+        up to max_views and the slot's share of the budget."""
+        head = self.pending[0]
+        if self.working_set([head]) > self.share:
+            self.pending.pop(0)
+            self.rejected.append(head)
+            return []
+        batch = [head]
+        for r in self.pending[1:]:
+            if len(batch) >= self.max_views:
+                break
+            if r.scene == head.scene and self.working_set(batch + [r]) <= self.share:
+                batch.append(r)
+        taken = set(id(r) for r in batch)
+        self.pending = [r for r in self.pending if id(r) not in taken]
+        return batch
+
+    # ── workers ──────────────────────────────────────────────────────────────
+    def _now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def _worker(self, slot: _Slot):
+        while True:
+            with self.cond:
+                while not self.pending and not self.stop:
+                    self.cond.wait()
+                if not self.pending and self.stop:
+                    return
+                batch = self._take_batch()
+            if not batch:
+                continue
+            try:
+                self._render(slot, batch)
+            except Exception as e:  # recorded; the run reports it
+                self.errors.append(f"slot {slot.index}: {e}")
+                with self.cond:
+                    self.rejected.extend(batch)
+
+    def _render(self, slot: _Slot, batch: list):
+        scene = self.scenes[batch[0].scene]
+        t = self._now()
+        views = BatchViews([r.cam for r in batch], scene.degree, [self.bg])
+        ptr, offs = render_batch(slot.ctx, scene, views)
+        total = sum(r.pixels for r in batch)
+        rgba = slot.rgba_buffer(total)
+        check(lib().msplat_hip_pack_rgba8(slot.ctx.ptr, C.c_void_p(ptr), total, C.c_void_p(rgba.ptr)))
+        slot.ctx.sync()
+        done = self._now()
+        m = slot.ctx.batch_isects()
+        if m:
+            self.ipp[batch[0].scene] = 0.8 * self.ipp[batch[0].scene] + 0.2 * (m / max(total, 1))
+        if self.keep_images:
+            packed = slot.ctx.download(rgba.ptr, (total, 4), np.uint8)
+            for r, o in zip(batch, offs):
+                r.image = packed[o:o + r.pixels].reshape(r.cam["H"], r.cam["W"], 4).copy()
+        with self.cond:
+            for r in batch:
+                r.launched, r.done, r.batch_size, r.slot = t, done, len(batch), slot.index
+            self.finished.extend(batch)
+            self.batches += 1
+
+    # ── a run over an arrival schedule ───────────────────────────────────────
+    def run(self, requests: list) -> list:
+        """Feeds the requests at their arrival times (seconds from now) and returns them
+        completed (or in self.rejected)."""
+        threads = [threading.Thread(target=self._worker, args=(s,), daemon=True) for s in self.slots]
+        self.finished, self.rejected, self.errors, self.batches = [], [], [], 0
+        self.stop = False
+        self.t0 = time.perf_counter()
+        for th in threads:
+            th.start()
+        for r in sorted(requests, key=lambda r: r.arrival):
+            delay = r.arrival - self._now()
+            if delay > 0:
+                time.sleep(delay)
+            with self.cond:
+                self.pending.append(r)
+                self.cond.notify()
+        with self.cond:
+            self.stop = True
+            self.cond.notify_all()
+        for th in threads:
+            th.join()
+        if self.errors:
+            raise RuntimeError("; ".join(self.errors))
+        return self.finished
+
+    def close(self):
+        for s in self.slots:
+            s.ctx.close()
+
+
+def serve_plain(scenes: list, requests: list, background=(0.0, 0.0, 0.0), device: int = 0,
+                keep_images: bool = False) -> list:
+    """The plain form: one msplat_hip_render (+ RGBA8 pack) per request, first come first
+    served, on one context."""
+    ctx = Context(device)
+    bg = floats(background)
+    rgba = None
+    t0 = time.perf_counter()
+    try:
+        for r in sorted(requests, key=lambda r: r.arrival):
+            delay = r.arrival - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            scene = scenes[r.scene]
+            r.launched = time.perf_counter() - t0
+            g, c = scene.struct(), make_camera_struct(r.cam, scene.degree)
+            out = C.c_void_p()
+            check(lib().msplat_hip_render(ctx.ptr, C.byref(g), C.byref(c), bg, C.byref(out)))
+            if rgba is None or rgba.nbytes < 4 * r.pixels:
+                rgba = DeviceArray(ctx, (int(r.pixels * 1.25) + 1024, 4), np.uint8)
+            check(lib().msplat_hip_pack_rgba8(ctx.ptr, out, r.pixels, C.c_void_p(rgba.ptr)))
+            ctx.sync()
+            r.done = time.perf_counter() - t0
+            r.batch_size, r.slot = 1, 0
+            if keep_images:
+                r.image = ctx.download(rgba.ptr, (r.pixels, 4), np.uint8).reshape(r.cam["H"], r.cam["W"], 4)
+        return requests
+    finally:
+        ctx.close()
+
+
+def latency_stats(reqs: list, t_first: float | None = None) -> dict:
+    """views/s over the run and p50/p99 latency (ms), overall and per resolution."""
+    done = [r for r in reqs if r.done == r.done]
+    if not done:
+        return {"completed": 0}
+    lat = np.array([r.done - r.arrival for r in done]) * 1e3
+    start = min(r.arrival for r in done) if t_first is None else t_first
+    span = max(r.done for r in done) - start
+    out = {"completed": len(done), "views_per_s": round(len(done) / max(span, 1e-9), 2),
+           "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+           "mean_batch": round(float(np.mean([r.batch_size for r in done])), 2)}
+    by_res = {}
+    for r, l in zip(done, lat):
+        by_res.setdefault(f"{r.cam['W']}x{r.cam['H']}", []).append(l)
+    out["by_resolution"] = {k: {"n": len(v), "p50_ms": round(float(np.percentile(v, 50)), 3),
+                                "p99_ms": round(float(np.percentile(v, 99)), 3)} for k, v in sorted(by_res.items())}
+    return out
+
+
+RESOLUTIONS = [((640, 360), 0.4), ((1280, 720), 0.3), ((1920, 1080), 0.2), ((3840, 2160), 0.1)]
+
+
+def poisson_requests(rate: float, duration: float, centers: list, seed: int = 0, radius=(3.0, 10.0),
+                     resolutions=RESOLUTIONS, focal_scale: float = 0.9) -> list:
+    """Poisson arrivals at `rate` per second for `duration` seconds; each request picks a
+    scene uniformly, a pose on a sphere of radius U(radius) around the scene's center
+    looking at it, and a resolution from the weighted mix (fx = fy = focal_scale W)."""
+    from .camera import look_at, make_camera
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    sizes, weights = zip(*resolutions)
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        s = int(rng.integers(0, len(centers)))
+        d = rng.normal(0, 1, 3)
+        d /= np.linalg.norm(d)
+        c = np.asarray(centers[s], np.float64)
+        eye = c + d * rng.uniform(*radius)
+        W, H = sizes[int(rng.choice(len(sizes), p=np.asarray(weights) / sum(weights)))]
+        up = (0.0, 1.0, 0.0) if abs(d[1]) < 0.95 else (1.0, 0.0, 0.0)
+        cam = make_camera(look_at(eye, c, up), W, H, focal_scale * W, focal_scale * W, dtu=3)
+        out.append(Request(len(out), s, cam, t))
+    return out

@@ -1,0 +1,64 @@
+"""Serving with continuous batching on the GPU: every request's RGBA8 image is the same
+alone (one msplat_hip_render per request) and in the mix (batched with other requests
+of its scene, two slots in flight at once), and admission keeps each batch within the
+slot's share of the HBM budget."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BG = (0.1, 0.2, 0.3)
+SMALL = [((64, 48), 0.4), ((128, 72), 0.3), ((96, 96), 0.2), ((50, 70), 0.1)]
+
+
+def _scenes(ctx):
+    from msplat.backend import DeviceGaussians
+    from msplat.scenes import random_scene
+    out, centers = [], [(0.0, 0.0, 0.0), (20.0, 0.0, 5.0)]
+    for k, (n, c) in enumerate(zip((3000, 5000), centers)):
+        sc = random_scene(40 + k, n, degree=3, mean_range=1.2, log_scale=(-3.0, 0.4))
+        sc["means"] = (sc["means"] + np.asarray(c, np.float32)).astype(np.float32)
+        out.append(DeviceGaussians(ctx, sc))
+    ctx.sync()
+    return out, centers
+
+
+def test_results_are_the_same_alone_and_in_the_mix(ctx):
+    from msplat.serving import Server, poisson_requests, serve_plain
+    scenes, centers = _scenes(ctx)
+    reqs = poisson_requests(4000.0, 0.012, centers, seed=3, radius=(3.0, 6.0), resolutions=SMALL)
+    assert len(reqs) >= 30
+    srv = Server(scenes, slots=2, max_views=8, background=BG, keep_images=True)
+    try:
+        done = srv.run(reqs)
+    finally:
+        srv.close()
+    assert len(done) == len(reqs) and not srv.rejected
+    assert max(r.batch_size for r in done) > 1  # requests were batched
+    alone = serve_plain(scenes, [type(r)(r.rid, r.scene, r.cam, 0.0) for r in reqs], background=BG,
+                        keep_images=True)
+    by_id = {r.rid: r for r in alone}
+    for r in done:
+        np.testing.assert_array_equal(r.image, by_id[r.rid].image, err_msg=f"request {r.rid}")
+    assert any(r.image[..., :3].std() > 2 for r in done)  # scenes are visible
+
+
+def test_admission_keeps_batches_within_the_budget(ctx):
+    from msplat.serving import PARAM_BYTES, Request, Server, poisson_requests
+    scenes, centers = _scenes(ctx)
+    reqs = poisson_requests(4000.0, 0.006, centers, seed=5, radius=(3.0, 6.0), resolutions=[((96, 96), 1.0)])
+    resident = sum(s.n * PARAM_BYTES for s in scenes)
+    probe = Server(scenes, slots=1, background=BG)
+    one = max(probe.working_set([r]) for r in reqs)
+    probe.close()
+    # each slot's share admits one 96x96 view but not two, and never a 1920x1080 one
+    srv = Server(scenes, slots=2, max_views=8, hbm_budget=resident + 2 * 1.5 * one, background=BG)
+    from msplat.camera import look_at, make_camera
+    big = Request(len(reqs), 0, make_camera(look_at((0, 0, 5.0), (0, 0, 0)), 1920, 1080, 1500.0, 1500.0, dtu=3),
+                  0.001)
+    try:
+        done = srv.run(reqs + [big])
+    finally:
+        srv.close()
+    assert [r.rid for r in srv.rejected] == [big.rid]
+    assert len(done) == len(reqs) and all(r.batch_size == 1 for r in done)

@@ -1,0 +1,76 @@
+"""Kernel variants of the rasterizer give the same frame: every forward LDS batch depth
+renders the same image, final T and contributor indices bit for bit; every backward
+batch depth gives the same gradients bit for bit (per pixels-per-thread variant), and the
+autotuner leaves a valid variant set on the context."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+BG = (0.3, 0.1, 0.6)
+
+
+def _frame(ctx, dg, cam, v_img):
+    from msplat._abi import DeviceArray
+    from msplat.backend import render, render_backward
+    p = render(ctx, dg, cam, BG)
+    f = ctx.frame()
+    H, W = cam["H"], cam["W"]
+    img = ctx.download(p, (H, W, 3), np.float32)
+    fT = ctx.download(f.final_T, (H, W), np.float32)
+    fi = ctx.download(f.final_idx, (H, W), np.int32)
+    grads = render_backward(ctx, dg, cam, BG, v_img)
+    return img, fT, fi, grads
+
+
+def test_variants_give_the_same_frame(ctx):
+    from msplat._abi import TUNE_BWD_BATCH, TUNE_BWD_PPT, TUNE_FWD_BATCH, DeviceArray
+    from msplat.backend import DeviceGaussians
+    from msplat.camera import look_at, make_camera
+    from msplat.scenes import random_scene
+    sc = random_scene(31, 6000, degree=3, log_scale=(-3.4, 0.5))
+    cam = make_camera(look_at((2.4, 0.9, 2.6), (0.0, 0.0, 0.0)), 131, 77, 95.0, 97.0, dtu=3)
+    dg = DeviceGaussians(ctx, sc)
+    v_img = DeviceArray.from_numpy(ctx, (np.random.default_rng(3).normal(0, 1e-3, (77, 131, 3))).astype(np.float32))
+    saved = [ctx.get_tuning(k) for k in (TUNE_BWD_PPT, TUNE_FWD_BATCH, TUNE_BWD_BATCH)]
+    try:
+        for ppt in (1, 2, 4):
+            ref = None
+            for fb in (64, 128, 256):
+                for bb in (32, 64, 128):
+                    ctx.set_tuning(TUNE_BWD_PPT, ppt)
+                    ctx.set_tuning(TUNE_FWD_BATCH, fb)
+                    ctx.set_tuning(TUNE_BWD_BATCH, bb)
+                    got = _frame(ctx, dg, cam, v_img)
+                    if ref is None:
+                        ref = got
+                        continue
+                    for a, b in zip(got[:3], ref[:3]):
+                        np.testing.assert_array_equal(a, b, err_msg=f"ppt {ppt} fwd {fb} bwd {bb}")
+                    for k in ref[3]:
+                        np.testing.assert_array_equal(got[3][k], ref[3][k], err_msg=f"{k} ppt {ppt} fb {fb} bb {bb}")
+    finally:
+        for k, v in zip((TUNE_BWD_PPT, TUNE_FWD_BATCH, TUNE_BWD_BATCH), saved):
+            ctx.set_tuning(k, v)
+
+
+def test_autotuner_picks_a_valid_variant(ctx, tmp_path):
+    from msplat import autotune
+    from msplat.backend import DeviceGaussians, render
+    from msplat.camera import look_at, make_camera
+    from msplat.scenes import random_scene
+    sc = random_scene(32, 20000, degree=3, log_scale=(-3.6, 0.5))
+    cam = make_camera(look_at((2.4, 0.9, 2.6), (0.0, 0.0, 0.0)), 320, 240, 250.0, 250.0, dtu=3)
+    dg = DeviceGaussians(ctx, sc)
+    saved = autotune.current(ctx)
+    try:
+        cache = str(tmp_path / "tune.json")
+        res = autotune.autotune(ctx, lambda: render(ctx, dg, cam, BG), warmup=1, repeats=3, cache=cache, key="k")
+        assert not res["cached"] and len(res["trials"]) == 7
+        for name, (_, values) in autotune.SPACE.items():
+            assert res[name] in values and autotune.current(ctx)[name] == res[name]
+        again = autotune.autotune(ctx, lambda: render(ctx, dg, cam, BG), cache=cache, key="k")
+        assert again["cached"] and all(again[n] == res[n] for n in autotune.SPACE)
+    finally:
+        autotune.apply(ctx, saved)
