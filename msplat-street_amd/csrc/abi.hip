@@ -119,6 +119,7 @@ int do_bin(msplat_hip_ctx *ctx) {
     CHECKED(ctx->grow(ctx->dv0, 4 * Nc));
     CHECKED(ctx->grow(ctx->dv1, 4 * Nc));
     CHECKED(ctx->grow(ctx->ord_sums, 4 * (size_t)(vblocks + 1)));
+    CHECKED(ctx->grow(ctx->loff, 4 * Nc));
     CHECKED(ctx->grow(ctx->tile_start, 4 * (size_t)(ctx->ntiles + 1)));
     hipStream_t s = ctx->stream;
     ctx->begin(1);
@@ -154,6 +155,9 @@ int do_bin(msplat_hip_ctx *ctx) {
     k.rect = (const uint2 *)ctx->rect.p;
     k.block_bases = (const uint32_t *)ctx->ord_sums.p;
     k.offsets = (uint32_t *)ctx->offsets.p;
+    k.loff = (uint32_t *)ctx->loff.p;
+    k.m_ptr = ctx->d_total32();
+    k.m_max = (uint32_t)M;
     k.keys = (uint32_t *)ctx->keys0.p;
     k.vals = (int32_t *)ctx->vals0.p;
     k.big_list = ctx->big_list();

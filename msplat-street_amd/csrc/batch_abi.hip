@@ -21,7 +21,7 @@ struct BatchState {
     // per record (views x gaussians)
     Buf splat_a, splat_b, splat_blue, depths, rect, radii, count, offsets, block_sums, block_vis, big_list, big_sums;
     // visible records sorted by depth, count sums in depth order, scan partials
-    Buf dk0, dk1, dv0, dv1, ord_sums, scan_part;
+    Buf dk0, dk1, dv0, dv1, ord_sums, scan_part, loff;
     // per intersection
     Buf keys0, keys1, vals0, vals1, slot_a, slot_b, slot_c, hist, digit_total;
     // per tile, pixel, row
@@ -52,7 +52,7 @@ struct BatchState {
 
 void destroy_batch_state(BatchState *b) {
     if (!b) return;
-    Buf *all[] = {&b->block_vis, &b->dk0, &b->dk1, &b->dv0, &b->dv1, &b->ord_sums, &b->scan_part,
+    Buf *all[] = {&b->loff, &b->block_vis, &b->dk0, &b->dk1, &b->dv0, &b->dv1, &b->ord_sums, &b->scan_part,
                   &b->splat_a, &b->splat_b, &b->splat_blue, &b->depths,   &b->rect,     &b->radii,      &b->count,
                   &b->offsets, &b->block_sums, &b->big_list, &b->big_sums, &b->keys0, &b->keys1,    &b->vals0,
                   &b->vals1,   &b->slot_a,     &b->slot_b,   &b->slot_c,   &b->hist,  &b->digit_total, &b->tile_start,
@@ -198,6 +198,7 @@ int prepare(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g, c
     CHECKED(grow(ctx, b, b->dk1, 4 * R));
     CHECKED(grow(ctx, b, b->dv0, 4 * R));
     CHECKED(grow(ctx, b, b->dv1, 4 * R));
+    CHECKED(grow(ctx, b, b->loff, 4 * R));
     CHECKED(grow(ctx, b, b->ord_sums, 4 * ((R + 255) / 256 + 1)));
     CHECKED(grow(ctx, b, b->scan_part, 4 * 3 * scan_partial_words((uint32_t)std::max(nbx * nv, (R + 255) / 256)) + 64));
     CHECKED(grow(ctx, b, b->big_list, 4 * R));
@@ -350,6 +351,9 @@ void enqueue_forward(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussi
     k.rect = (const uint2 *)b->rect.p;
     k.block_bases = (const uint32_t *)b->ord_sums.p;
     k.offsets = (uint32_t *)b->offsets.p;
+    k.loff = (uint32_t *)b->loff.p;
+    k.m_ptr = &sc->total32;
+    k.m_max = b->cap;
     k.keys = (uint32_t *)b->keys0.p;
     k.vals = (int32_t *)b->vals0.p;
     k.big_list = (uint32_t *)b->big_list.p;

@@ -17,6 +17,8 @@
 //      tile's intersections in (depth, id) order.
 // Records own contiguous slot ranges (offsets[r], count[r]) in emission order, tiles
 // row-major within a record, which the backward's per-record reductions read.
+#include <algorithm>
+
 #include "kernels.h"
 #include "proj_math.h"
 
@@ -273,27 +275,37 @@ __global__ __launch_bounds__(256) void compact_kernel(CompactArgs a) {
 // workgroup (wave multi-split over ballots of the digit bits), stages them in LDS in
 // digit order, and writes each digit's run to consecutive addresses.
 
+// The workgroups of a pass loop over the element blocks (RS_ITEMS each) that the
+// device-side count needs, so a grid sized for a capacity does no empty work;
+// histogram column b of digit d lives at hist[d * stride + b].
+__device__ inline uint32_t rs_blocks(const uint32_t *n_ptr) { return (*n_ptr + RS_ITEMS - 1) / RS_ITEMS; }
+
 template <typename K>
 __global__ __launch_bounds__(256) void rs_hist_kernel(const K *keys, const uint32_t *n_ptr, int shift, uint32_t *hist,
-                                                     int nblocks) {
+                                                     int stride) {
     __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const uint32_t n = *n_ptr;
-    const uint32_t start = blockIdx.x * RS_ITEMS;
-    for (int r = 0; r < RS_ITEMS / 256; r++) {
-        uint32_t e = start + r * 256 + threadIdx.x;
-        if (e < n) atomicAdd(&h[(uint32_t)(keys[e] >> shift) & 0xFF], 1u);
+    const uint32_t n = *n_ptr, nb = rs_blocks(n_ptr);
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        h[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t start = b * RS_ITEMS;
+        for (int r = 0; r < RS_ITEMS / 256; r++) {
+            uint32_t e = start + r * 256 + threadIdx.x;
+            if (e < n) atomicAdd(&h[(uint32_t)(keys[e] >> shift) & 0xFF], 1u);
+        }
+        __syncthreads();
+        hist[(size_t)threadIdx.x * stride + b] = h[threadIdx.x];
+        __syncthreads();
     }
-    __syncthreads();
-    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
 // one block per digit: exclusive scan of that digit's counts over blocks, digit total
-__global__ __launch_bounds__(256) void rs_scan_kernel(uint32_t *hist, int nblocks, uint32_t *digit_total) {
+__global__ __launch_bounds__(256) void rs_scan_kernel(uint32_t *hist, int stride, const uint32_t *n_ptr,
+                                                     uint32_t *digit_total) {
     __shared__ uint32_t lds4[4];
     __shared__ uint32_t carry;
-    uint32_t *row = hist + (size_t)blockIdx.x * nblocks;
+    const int nblocks = (int)rs_blocks(n_ptr);
+    uint32_t *row = hist + (size_t)blockIdx.x * stride;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (int base = 0; base < nblocks; base += 256) {
@@ -319,10 +331,10 @@ __device__ inline void wave_sync() {
 // Wave w ranks its own contiguous quarter of the workgroup's elements, 64 at a time,
 // with private per-digit counters (no workgroup barrier inside the loop); the waves'
 // counts are then combined in wave order, which keeps the sort stable.
+// One element block of a pass; `my_base` is this thread's digit's global start for the block.
 template <typename K>
-__global__ __launch_bounds__(256) void rs_scatter_kernel(const K *kin, const int32_t *vin, K *kout, int32_t *vout,
-                                                        const uint32_t *n_ptr, int shift, const uint32_t *hist,
-                                                        int nblocks, const uint32_t *digit_total) {
+__device__ inline void rs_scatter_block(const K *kin, const int32_t *vin, K *kout, int32_t *vout, uint32_t n,
+                                        int shift, uint32_t start, uint32_t my_base) {
     constexpr int R = RS_ITEMS / 256;  // rounds of 64 elements per wave
     __shared__ K sk[RS_ITEMS];
     __shared__ int32_t sv[RS_ITEMS];
@@ -330,14 +342,7 @@ __global__ __launch_bounds__(256) void rs_scatter_kernel(const K *kin, const int
     __shared__ uint32_t wcnt[4][256];
     __shared__ uint32_t lds4[4];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t n = *n_ptr;
-    const uint32_t start = blockIdx.x * RS_ITEMS;
-    if (start >= n) return;
-    {   // global position of this block's first element of each digit
-        uint32_t t;
-        const uint32_t ex = block_scan_u32(digit_total[tid], t, lds4);
-        gbase[tid] = ex + hist[(size_t)tid * nblocks + blockIdx.x];
-    }
+    gbase[tid] = my_base;
 #pragma unroll
     for (int i = 0; i < 4; i++) wcnt[w][lane + 64 * i] = 0;
     const uint32_t wstart = start + (uint32_t)w * (64 * R);
@@ -401,77 +406,143 @@ __global__ __launch_bounds__(256) void rs_scatter_kernel(const K *kin, const int
     }
 }
 
+
+template <typename K>
+__global__ __launch_bounds__(256) void rs_scatter_kernel(const K *kin, const int32_t *vin, K *kout, int32_t *vout,
+                                                        const uint32_t *n_ptr, int shift, const uint32_t *hist,
+                                                        int stride, const uint32_t *digit_total) {
+    __shared__ uint32_t dex[256];
+    __shared__ uint32_t lds4[4];
+    {   // exclusive prefix of the digit totals
+        uint32_t t;
+        dex[threadIdx.x] = block_scan_u32(digit_total[threadIdx.x], t, lds4);
+    }
+    const uint32_t nb = rs_blocks(n_ptr);
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        __syncthreads();
+        rs_scatter_block(kin, vin, kout, vout, *n_ptr, shift, b * RS_ITEMS,
+                         dex[threadIdx.x] + hist[(size_t)threadIdx.x * stride + b]);
+    }
+}
+
 // ── intersections in depth order ──────────────────────────────────────────────
 
 // Per 256-entry block of the depth-ordered list: the sum of its records' counts
 // (0 for blocks past the list's device-side length).
 __global__ __launch_bounds__(256) void order_sums_kernel(const int32_t *list, const uint32_t *nvis_ptr,
-                                                        const uint32_t *count, uint32_t *sums) {
+                                                        const uint32_t *count, uint32_t *sums, uint32_t nchunks_max) {
     __shared__ uint32_t lds4[4];
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t c = i < *nvis_ptr ? count[list[i]] : 0u;
-    uint32_t t;
-    block_scan_u32(c, t, lds4);
-    if (threadIdx.x == 0) sums[blockIdx.x] = t;
+    const uint32_t nvis = *nvis_ptr;
+    for (uint32_t chunk = blockIdx.x; chunk < nchunks_max; chunk += gridDim.x) {
+        if (chunk * 256 >= nvis) {  // past the list: zero sums, nothing to read
+            if (threadIdx.x == 0) sums[chunk] = 0;
+            continue;
+        }
+        const uint32_t i = chunk * 256 + threadIdx.x;
+        const uint32_t c = i < nvis ? count[list[i]] : 0u;
+        uint32_t t;
+        block_scan_u32(c, t, lds4);
+        if (threadIdx.x == 0) sums[chunk] = t;
+    }
 }
 
 // Record list[i] writes its tiles in row-major order at offsets[r] = the exclusive
-// prefix of the counts in list order. The workgroup emits the keys of its 256 records
-// together: intersection i of the block finds its record by binary search over the
-// block's offsets (LDS), so consecutive threads write consecutive keys.
-template <bool BATCH>
-__global__ __launch_bounds__(256) void keygen_kernel(KeygenArgs a) {
+// prefix of the counts in list order (also kept in list order, loff[i], for the
+// emission), and joins the big list when it has more than BIG_SLOTS intersections.
+__global__ __launch_bounds__(256) void keygen_offsets_kernel(KeygenArgs a) {
     __shared__ uint32_t lds4[4];
-    __shared__ uint32_t s_off[257];
-    __shared__ uint32_t s_rect[256][2];
-    __shared__ uint32_t s_tile[256][2];  // tiles_x, first global tile of the record's view
-    __shared__ int32_t s_rec[256];
     __shared__ uint32_t s_bigbase;
     if (a.overflow && *a.overflow) return;
-    const uint32_t i0 = blockIdx.x * 256 + threadIdx.x;
     const uint32_t nvis = *a.nvis_ptr;
-    const int32_t r = i0 < nvis ? a.list[i0] : -1;
-    const uint32_t c = r >= 0 ? a.count[r] : 0u;
-    uint32_t total;
-    const uint32_t ex = block_scan_u32(c, total, lds4);
-    const uint32_t base = a.block_bases[blockIdx.x];
-    if (r >= 0) a.offsets[r] = base + ex;
-    uint32_t nbig;
-    const uint32_t bex = block_scan_u32(c > BIG_SLOTS ? 1u : 0u, nbig, lds4);
-    if (threadIdx.x == 0 && nbig) s_bigbase = atomicAdd(a.big_count, nbig);
-    __syncthreads();
-    if (c > BIG_SLOTS) a.big_list[s_bigbase + bex] = (uint32_t)r;
-    s_off[threadIdx.x] = ex;
-    if (threadIdx.x == 0) s_off[256] = total;
-    if (c) {
-        const uint2 rr = a.rect[r];
-        s_rect[threadIdx.x][0] = rr.x;
-        s_rect[threadIdx.x][1] = rr.y;
-        s_rec[threadIdx.x] = r;
-        if (BATCH) {
-            const int v = r / a.n;
-            s_tile[threadIdx.x][0] = (uint32_t)a.tab->views[v].cam.tiles_x;
-            s_tile[threadIdx.x][1] = a.tab->tile_base[v];
-        } else {
-            s_tile[threadIdx.x][0] = (uint32_t)a.tiles_x;
-            s_tile[threadIdx.x][1] = 0u;
+    for (uint32_t chunk = blockIdx.x; chunk * 256 < nvis; chunk += gridDim.x) {
+        const uint32_t i = chunk * 256 + threadIdx.x;
+        const int32_t r = i < nvis ? a.list[i] : -1;
+        const uint32_t c = r >= 0 ? a.count[r] : 0u;
+        uint32_t total;
+        const uint32_t off = a.block_bases[chunk] + block_scan_u32(c, total, lds4);
+        if (r >= 0) {
+            a.offsets[r] = off;
+            a.loff[i] = off;
         }
+        uint32_t nbig;
+        const uint32_t bex = block_scan_u32(c > BIG_SLOTS ? 1u : 0u, nbig, lds4);
+        if (threadIdx.x == 0 && nbig) s_bigbase = atomicAdd(a.big_count, nbig);
+        __syncthreads();
+        if (c > BIG_SLOTS) a.big_list[s_bigbase + bex] = (uint32_t)r;
+        __syncthreads();
     }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < total; i += 256) {
-        // the last record j of the block with s_off[j] <= i
-        int lo = 0, hi = 255;
+}
+
+// Key emission balanced over the output: workgroup b writes intersections
+// [b * EMIT_CHUNK, (b + 1) * EMIT_CHUNK). It finds the records covering its slice by
+// binary search over loff, stages their offsets and ids in LDS, and each output finds
+// its record there; consecutive threads write consecutive keys whatever the records'
+// footprints (depth order puts the large near splats next to each other).
+constexpr int EMIT_ITEMS = 8;
+constexpr int EMIT_CHUNK = 256 * EMIT_ITEMS;
+
+template <bool BATCH>
+__global__ __launch_bounds__(256) void keygen_emit_kernel(KeygenArgs a) {
+    __shared__ uint32_t s_off[EMIT_CHUNK];
+    __shared__ int32_t s_rec[EMIT_CHUNK];
+    __shared__ uint32_t s_first, s_n;
+    if (a.overflow && *a.overflow) return;
+    const uint32_t M = *a.m_ptr;
+    const uint32_t o0 = blockIdx.x * EMIT_CHUNK;
+    if (o0 >= M) return;
+    const uint32_t o1 = min(o0 + (uint32_t)EMIT_CHUNK, M);
+    if (threadIdx.x == 0) {
+        // records first..last cover [o0, o1): the last i with loff[i] <= o0, o1 - 1
+        const uint32_t nvis = *a.nvis_ptr;
+        uint32_t lo = 0, hi = nvis - 1;
         while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= i) lo = mid;
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (a.loff[mid] <= o0) lo = mid;
             else hi = mid - 1;
         }
-        const uint32_t rx = s_rect[lo][0], ry = s_rect[lo][1];
-        const int x0 = rx & 0xFFFF, x1 = rx >> 16, y0 = ry & 0xFFFF;
-        const uint32_t local = i - s_off[lo], w = (uint32_t)(x1 - x0);
+        uint32_t lo2 = lo, hi2 = nvis - 1;
+        while (lo2 < hi2) {
+            const uint32_t mid = (lo2 + hi2 + 1) >> 1;
+            if (a.loff[mid] <= o1 - 1) lo2 = mid;
+            else hi2 = mid - 1;
+        }
+        s_first = lo;
+        s_n = lo2 - lo + 1;  // <= o1 - o0: every record after the first starts inside the slice
+    }
+    __syncthreads();
+    const uint32_t first = s_first, nrec = s_n;
+    for (uint32_t j = threadIdx.x; j < nrec; j += 256) {
+        s_off[j] = a.loff[first + j];
+        s_rec[j] = a.list[first + j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < EMIT_ITEMS; k++) {
+        const uint32_t o = o0 + k * 256 + threadIdx.x;
+        if (o >= o1) break;
+        uint32_t lo = 0, hi = nrec - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= o) lo = mid;
+            else hi = mid - 1;
+        }
+        const int32_t r = s_rec[lo];
+        const uint32_t local = o - s_off[lo];
+        const uint2 rr = a.rect[r];
+        const int x0 = rr.x & 0xFFFF, x1 = rr.x >> 16, y0 = rr.y & 0xFFFF;
+        const uint32_t w = (uint32_t)(x1 - x0);
         const int ty = y0 + (int)(local / w), tx = x0 + (int)(local % w);
-        a.keys[base + i] = s_tile[lo][1] + (uint32_t)(ty * (int)s_tile[lo][0] + tx);
-        a.vals[base + i] = s_rec[lo];
+        uint32_t tiles_x, tbase;
+        if (BATCH) {
+            const int v = r / a.n;
+            tiles_x = (uint32_t)a.tab->views[v].cam.tiles_x;
+            tbase = a.tab->tile_base[v];
+        } else {
+            tiles_x = (uint32_t)a.tiles_x;
+            tbase = 0u;
+        }
+        a.keys[o] = tbase + (uint32_t)ty * tiles_x + (uint32_t)tx;
+        a.vals[o] = r;
     }
 }
 
@@ -599,14 +670,19 @@ void launch_compact(const CompactArgs &a, hipStream_t s) {
 
 size_t radix_hist_words(uint32_t n_max) { return (size_t)256 * ((n_max + RS_ITEMS - 1) / RS_ITEMS + 1); }
 
+// Grid caps of the capacity-sized launches: the workgroups loop over what the device
+// count needs.
+constexpr uint32_t HIST_GRID = 2048, SCATTER_GRID = 1024, LIST_GRID = 2048;
+
 template <typename K>
 static void radix_pass(const K *kin, const int32_t *vin, K *kout, int32_t *vout, const uint32_t *n_dev,
                        uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s) {
-    const int nblocks = (int)((n_max + RS_ITEMS - 1) / RS_ITEMS);
+    const uint32_t nblocks = (n_max + RS_ITEMS - 1) / RS_ITEMS;
     if (nblocks == 0) return;
-    rs_hist_kernel<K><<<nblocks, 256, 0, s>>>(kin, n_dev, shift, hist, nblocks);
-    rs_scan_kernel<<<256, 256, 0, s>>>(hist, nblocks, digit_total);
-    rs_scatter_kernel<K><<<nblocks, 256, 0, s>>>(kin, vin, kout, vout, n_dev, shift, hist, nblocks, digit_total);
+    rs_hist_kernel<K><<<std::min(nblocks, HIST_GRID), 256, 0, s>>>(kin, n_dev, shift, hist, (int)nblocks);
+    rs_scan_kernel<<<256, 256, 0, s>>>(hist, (int)nblocks, n_dev, digit_total);
+    rs_scatter_kernel<K><<<std::min(nblocks, SCATTER_GRID), 256, 0, s>>>(kin, vin, kout, vout, n_dev, shift, hist,
+                                                                         (int)nblocks, digit_total);
 }
 
 void launch_radix_pass(const uint64_t *kin, const int32_t *vin, uint64_t *kout, int32_t *vout, const uint32_t *n_dev,
@@ -622,14 +698,17 @@ void launch_radix_pass(const uint32_t *kin, const int32_t *vin, uint32_t *kout, 
 void launch_order_sums(const int32_t *list, const uint32_t *nvis_dev, uint32_t nvis_max, const uint32_t *count,
                        uint32_t *sums, hipStream_t s) {
     const uint32_t nb = (nvis_max + 255) / 256;
-    if (nb > 0) order_sums_kernel<<<nb, 256, 0, s>>>(list, nvis_dev, count, sums);
+    if (nb > 0) order_sums_kernel<<<std::min(nb, LIST_GRID), 256, 0, s>>>(list, nvis_dev, count, sums, nb);
 }
 
 void launch_keygen(const KeygenArgs &a, bool batch, hipStream_t s) {
     const uint32_t nb = (a.nvis_max + 255) / 256;
     if (nb == 0) return;
-    if (batch) keygen_kernel<true><<<nb, 256, 0, s>>>(a);
-    else keygen_kernel<false><<<nb, 256, 0, s>>>(a);
+    keygen_offsets_kernel<<<std::min(nb, LIST_GRID), 256, 0, s>>>(a);
+    const uint32_t ne = (a.m_max + EMIT_CHUNK - 1) / EMIT_CHUNK;
+    if (ne == 0) return;
+    if (batch) keygen_emit_kernel<true><<<ne, 256, 0, s>>>(a);
+    else keygen_emit_kernel<false><<<ne, 256, 0, s>>>(a);
 }
 
 void launch_tile_ranges(const uint32_t *keys, const uint32_t *n_dev, uint32_t n_max, const uint32_t *ntiles_dev,

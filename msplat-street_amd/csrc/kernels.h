@@ -63,6 +63,9 @@ struct KeygenArgs {
     const uint2 *rect;
     const uint32_t *block_bases;  // scanned count sums per 256 list entries
     uint32_t *offsets;
+    uint32_t *loff;         // offsets in list order (nvis_max entries)
+    const uint32_t *m_ptr;  // device intersection count
+    uint32_t m_max;         // grid bound of the emission
     uint32_t *keys;  // global tile id
     int32_t *vals;   // record id
     uint32_t *big_list, *big_count;  // records with count > BIG_SLOTS

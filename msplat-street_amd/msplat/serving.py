@@ -8,8 +8,9 @@ rasterizer launch) plus one RGBA8 pack (msplat_pack_rgba8, core/cpu/msplat_cpu.c
 1723-1734). Each slot owns a context (its own stream and buffers) and runs in its own
 thread; the library releases the GIL during its calls, so the batches of different slots
 overlap on the device, and requests that arrive while batches are in flight are admitted
-as slots and memory free up. The plain form renders one request per msplat_hip_render,
-first come first served, on one context.
+as slots and memory free up. A lone pending request takes the one-view path
+(msplat_hip_render). The plain form renders one request per msplat_hip_render, first
+come first served, on one context.
 
 Admission: the resident scenes' parameters plus every slot's working set stay within the
 HBM budget; a batch's working set is estimated from its record count (views x
@@ -68,7 +69,8 @@ class Server:
     """Continuous-batching renderer over resident scenes (DeviceGaussians)."""
 
     def __init__(self, scenes: list, slots: int = 2, max_views: int = 16, hbm_budget: float = 64e9,
-                 background=(0.0, 0.0, 0.0), device: int = 0, keep_images: bool = False):
+                 background=(0.0, 0.0, 0.0), device: int = 0, keep_images: bool = False,
+                 isects_per_pixel: float = 60.0, adapt: bool = True):
         self.scenes = scenes
         self.max_views = int(max_views)
         self.bg = tuple(float(v) for v in background)
@@ -79,7 +81,8 @@ class Server:
             raise ValueError("the resident scenes alone exceed the HBM budget")
         self.slots = [_Slot(i, device) for i in range(slots)]
         self.share = (self.budget - self.resident) / slots
-        self.ipp = [60.0] * len(scenes)  # intersections per pixel, refined from each frame
+        self.ipp = [float(isects_per_pixel)] * len(scenes)  # intersections per pixel
+        self.adapt = adapt  # refine ipp from each rendered frame
         self.pending: list[Request] = []
         self.finished: list[Request] = []
         self.rejected: list[Request] = []
@@ -137,15 +140,21 @@ class Server:
     def _render(self, slot: _Slot, batch: list):
         scene = self.scenes[batch[0].scene]
         t = self._now()
-        views = BatchViews([r.cam for r in batch], scene.degree, [self.bg])
-        ptr, offs = render_batch(slot.ctx, scene, views)
         total = sum(r.pixels for r in batch)
+        if len(batch) == 1:  # a lone request takes the one-view path (no batch set-up)
+            g, c = scene.struct(), make_camera_struct(batch[0].cam, scene.degree)
+            out = C.c_void_p()
+            check(lib().msplat_hip_render(slot.ctx.ptr, C.byref(g), C.byref(c), floats(self.bg), C.byref(out)))
+            ptr, offs = out.value, [0]
+        else:
+            views = BatchViews([r.cam for r in batch], scene.degree, [self.bg])
+            ptr, offs = render_batch(slot.ctx, scene, views)
         rgba = slot.rgba_buffer(total)
         check(lib().msplat_hip_pack_rgba8(slot.ctx.ptr, C.c_void_p(ptr), total, C.c_void_p(rgba.ptr)))
         slot.ctx.sync()
         done = self._now()
-        m = slot.ctx.batch_isects()
-        if m:
+        m = slot.ctx.frame().num_isects if len(batch) == 1 else slot.ctx.batch_isects()
+        if m and self.adapt:
             self.ipp[batch[0].scene] = 0.8 * self.ipp[batch[0].scene] + 0.2 * (m / max(total, 1))
         if self.keep_images:
             packed = slot.ctx.download(rgba.ptr, (total, 4), np.uint8)
