@@ -119,4 +119,15 @@ __device__ inline void rand_normal3(uint32_t seed, uint32_t stream, float *out) 
 
 __global__ inline void fill_u32(uint32_t *p, uint32_t v) { *p = v; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8,
+// each XCD with its own L2). This maps workgroup b to item t so that every XCD walks its
+// own contiguous chunk of [0, n) in order: neighbouring tiles (which share gaussians and
+// write neighbouring gradient slots) meet in the same L2. A bijection on [0, n).
+constexpr int NUM_XCD = 8;
+__host__ __device__ inline int xcd_item(int b, int n) {
+    const int x = b % NUM_XCD, l = b / NUM_XCD;
+    const int q = n / NUM_XCD, r = n % NUM_XCD;
+    return x * q + (x < r ? x : r) + l;
+}
+
 }  // namespace msplat

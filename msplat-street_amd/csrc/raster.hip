@@ -60,12 +60,14 @@ __device__ inline uint32_t next_item(uint32_t *queue, uint32_t *slot) {
 
 template <int FB>
 __global__ __launch_bounds__(256) void raster_forward_kernel(RasterArgs a) {
-    forward_tile<FB>(tile_view(a), tile_data(a), blockIdx.x, blockIdx.x);
+    const int t = xcd_item(blockIdx.x, gridDim.x);
+    forward_tile<FB>(tile_view(a), tile_data(a), t, t);
 }
 
 template <int PPT, int BB>
 __global__ __launch_bounds__(256 / PPT) void raster_backward_kernel(RasterArgs a) {
-    backward_tile<PPT, BB>(tile_view(a), tile_data(a), blockIdx.x, blockIdx.x);
+    const int t = xcd_item(blockIdx.x, gridDim.x);
+    backward_tile<PPT, BB>(tile_view(a), tile_data(a), t, t);
 }
 
 // Persistent forward over every (view, tile) of a batch. Every workgroup loops until

@@ -52,7 +52,7 @@ def test_admission_keeps_batches_within_the_budget(ctx):
     one = max(probe.working_set([r]) for r in reqs)
     probe.close()
     # each slot's share admits one 96x96 view but not two, and never a 1920x1080 one
-    srv = Server(scenes, slots=2, max_views=8, hbm_budget=resident + 2 * 1.5 * one, background=BG)
+    srv = Server(scenes, slots=2, max_views=8, hbm_budget=resident + 2 * 1.5 * one, background=BG, adapt=False)
     from msplat.camera import look_at, make_camera
     big = Request(len(reqs), 0, make_camera(look_at((0, 0, 5.0), (0, 0, 0)), 1920, 1080, 1500.0, 1500.0, dtu=3),
                   0.001)
