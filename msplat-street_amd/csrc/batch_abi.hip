@@ -3,8 +3,9 @@
 //
 // The frame's shape-independent inputs live in a BatchTable in device memory, uploaded
 // before each launch; every kernel of the batch reads poses, resolutions, targets and
-// Adam scalars from it, so a captured graph replays with new contents. The intersection
-// buffers have a fixed capacity: the count pass compares the total with it on the
+// Adam scalars from it, so a captured graph replays with new contents. Eager frames count
+// the intersections first and size their buffers from the read-back totals. A captured
+// step has fixed capacities instead: the count pass compares the total with them on the
 // device, and over capacity every stage sees an empty frame and the Adam kernels do
 // nothing, so the host grows the capacity and runs the step again (no partial update).
 #include <algorithm>
@@ -277,7 +278,7 @@ BatchRasterArgs raster_args(BatchState *b) {
     BatchRasterArgs r;
     r.tab = dtab(b);
     r.num_views = b->views;
-    r.queue = dsc(b)->queue;
+    r.queue = &dsc(b)->queue[0][0];
     r.tile_start = (const int32_t *)b->tile_start.p;
     r.gid = b->gid_sorted;
     r.splat_a = (const float4 *)b->splat_a.p;
@@ -315,10 +316,9 @@ void enqueue_counts(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussia
                     ScanOut{&sc->nvis64, &sc->nvis, 0xFFFFFFFFu, nullptr, nullptr}, s);
 }
 
-// counts, visible records sorted by depth, keys in depth order, tile sort, ranges, forward
-void enqueue_forward(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g) {
+// visible records sorted by depth, keys in depth order, tile sort, ranges, forward
+void enqueue_sort_and_forward(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g) {
     hipStream_t s = ctx->stream;
-    enqueue_counts(ctx, b, g, b->cap);
     BatchScalars *sc = dsc(b);
     const uint32_t R = (uint32_t)record_count(b);
     CompactArgs c;
@@ -435,10 +435,12 @@ int scratch_grads(msplat_hip_ctx *ctx, BatchState *b, int n, uint32_t rest_strid
     return MSPLAT_OK;
 }
 
+// The step after the count pass (counts: also enqueue it, against the capacity)
 void enqueue_train(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g, const msplat_hip_adam *adam,
-                   const msplat_hip_grad_stats *stats, float *const gr[6]) {
+                   const msplat_hip_grad_stats *stats, float *const gr[6], bool counts) {
     hipStream_t s = ctx->stream;
-    enqueue_forward(ctx, b, g);
+    if (counts) enqueue_counts(ctx, b, g, b->cap);
+    enqueue_sort_and_forward(ctx, b, g);
     BatchLossArgs l;
     l.tab = dtab(b);
     l.num_views = b->views;
@@ -482,6 +484,16 @@ int ensure_capacity(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussia
     return grow_isects(ctx, b, b->h_sc->total);
 }
 
+// Eager frames count first and read the totals back (one sync, as a one-view frame
+// does), so their buffers always fit and a frame never runs twice.
+int eager_counts(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g) {
+    enqueue_counts(ctx, b, g, 0xFFFFFFFFu);
+    HIPCHECK(hipGetLastError());
+    bool over = false;
+    CHECKED(read_scalars(ctx, b, over));
+    return grow_isects(ctx, b, b->h_sc->total);
+}
+
 int validate_adam(const msplat_hip_gaussians *g, const msplat_hip_adam *adam, uint32_t stride) {
     for (int i = 0; i < 6; i++) {
         if (i == 4 && stride == 0) continue;
@@ -514,7 +526,8 @@ int run_train(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g,
     float *gr[6];
     CHECKED(scratch_grads(ctx, b, g->num_points, stride, gr));
     if (!b->graph_on) {
-        enqueue_train(ctx, b, g, adam, stats, gr);
+        CHECKED(eager_counts(ctx, b, g));
+        enqueue_train(ctx, b, g, adam, stats, gr, false);
         HIPCHECK(hipGetLastError());
         return MSPLAT_OK;
     }
@@ -525,7 +538,7 @@ int run_train(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g,
         b->exec = nullptr;
         b->graph = nullptr;
         HIPCHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_train(ctx, b, g, adam, stats, gr);
+        enqueue_train(ctx, b, g, adam, stats, gr, true);
         const hipError_t launch_err = hipGetLastError();
         hipGraph_t graph = nullptr;
         const hipError_t end_err = hipStreamEndCapture(ctx->stream, &graph);
@@ -556,16 +569,10 @@ int msplat_hip_render_batch(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, 
     CHECKED(state(ctx, b));
     b->rendered = false;
     CHECKED(prepare(ctx, b, g, views, 0.f, nullptr));
-    CHECKED(ensure_capacity(ctx, b, g));
-    for (int attempt = 0; attempt < 2; attempt++) {
-        enqueue_forward(ctx, b, g);
-        HIPCHECK(hipGetLastError());
-        bool over = false;
-        CHECKED(read_scalars(ctx, b, over));
-        if (!over) break;
-        if (attempt == 1) return set_error(MSPLAT_ERR_HIP, "intersection capacity still exceeded after growth");
-        CHECKED(grow_isects(ctx, b, b->h_sc->total));
-    }
+    CHECKED(eager_counts(ctx, b, g));
+    enqueue_sort_and_forward(ctx, b, g);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
     b->rendered = true;
     b->frame_n = g->num_points;
     for (int v = 0; v < views->num_views; v++) {
@@ -588,7 +595,7 @@ int msplat_hip_train_step_batch(msplat_hip_ctx *ctx, const msplat_hip_gaussians 
     b->rendered = false;
     CHECKED(prepare(ctx, b, g, views, ssim_weight, adam));
     CHECKED(validate_adam(g, adam, (sh_bases(views->cameras[0].degree) - 1) * 3));
-    CHECKED(ensure_capacity(ctx, b, g));
+    if (b->graph_on) CHECKED(ensure_capacity(ctx, b, g));
     for (int attempt = 0; attempt < 2; attempt++) {
         CHECKED(run_train(ctx, b, g, adam, stats));
         bool over = false;

@@ -205,7 +205,7 @@ struct BatchScalars {
     uint32_t overflow;   // 1 when the intersections exceed the capacity: the step is void
     uint32_t big_count;  // records with more than BIG_SLOTS intersections
     uint32_t nvis;       // visible records
-    uint32_t queue[2];   // persistent rasterizer work-queue heads (forward, backward)
+    uint32_t queue[2][8];  // persistent rasterizer queue heads (forward, backward) per XCD chunk
     uint32_t total32_b, overflow_b;  // the same totals from the depth-order scan
     double loss_sum[MAX_VIEWS];
     float loss_mean[MAX_VIEWS];

@@ -47,13 +47,31 @@ __device__ inline int view_of(const uint32_t *tb, int nv, uint32_t t) {
     return lo;
 }
 
-// Next item of the queue for the whole workgroup (one atomic per item)
-__device__ inline uint32_t next_item(uint32_t *queue, uint32_t *slot) {
-    if (threadIdx.x == 0) *slot = atomicAdd(queue, 1u);
-    __syncthreads();
-    const uint32_t it = *slot;
-    __syncthreads();
-    return it;
+// Work queue of a persistent launch: the items [0, total) are cut into NUM_XCD
+// contiguous chunks with one head each. A workgroup drains the chunk of its own XCD
+// first (neighbouring tiles then share an L2) and then helps with the others; it
+// returns total once every chunk is drained. One atomic per item.
+struct ItemQueue {
+    uint32_t *heads;  // NUM_XCD counters, zeroed before the launch
+    uint32_t total;
+    int chunk;        // the chunk being drained, NUM_XCD when done
+    int home;
+};
+
+__device__ inline uint32_t next_item(ItemQueue &q, uint32_t *slot) {
+    for (;;) {
+        if (q.chunk >= NUM_XCD) return q.total;
+        const int c = (q.home + q.chunk) % NUM_XCD;
+        const uint32_t per = q.total / NUM_XCD, rem = q.total % NUM_XCD;
+        const uint32_t len = per + ((uint32_t)c < rem ? 1u : 0u);
+        const uint32_t first = (uint32_t)c * per + ((uint32_t)c < rem ? (uint32_t)c : rem);
+        if (threadIdx.x == 0) *slot = atomicAdd(&q.heads[c], 1u);
+        __syncthreads();
+        const uint32_t i = *slot;
+        __syncthreads();
+        if (i < len) return first + i;
+        q.chunk++;
+    }
 }
 
 }  // namespace
@@ -80,8 +98,9 @@ __global__ __launch_bounds__(256) void raster_forward_batch_kernel(BatchRasterAr
     for (int i = threadIdx.x; i <= nv; i += 256) tb[i] = a.tab->tile_base[i];
     const uint32_t total = a.tab->total_tiles;
     const TileData d = tile_data(a);
+    ItemQueue q{a.queue, total, 0, (int)(blockIdx.x % NUM_XCD)};
     for (;;) {
-        const uint32_t t = next_item(a.queue, &item_slot);
+        const uint32_t t = next_item(q, &item_slot);
         if (t >= total) break;
         const int v = view_of(tb, nv, t);
         forward_tile<FB>(tile_view(a, v), d, (int)t, (int)(t - tb[v]));
@@ -96,8 +115,9 @@ __global__ __launch_bounds__(256 / PPT) void raster_backward_batch_kernel(BatchR
     for (int i = threadIdx.x; i <= nv; i += 256 / PPT) tb[i] = a.tab->tile_base[i];
     const uint32_t total = a.tab->total_tiles;
     const TileData d = tile_data(a);
+    ItemQueue q{a.queue + NUM_XCD, total, 0, (int)(blockIdx.x % NUM_XCD)};
     for (;;) {
-        const uint32_t t = next_item(a.queue + 1, &item_slot);
+        const uint32_t t = next_item(q, &item_slot);
         if (t >= total) break;
         const int v = view_of(tb, nv, t);
         backward_tile<PPT, BB>(tile_view(a, v), d, (int)t, (int)(t - tb[v]));
@@ -181,8 +201,12 @@ int batch_raster_grid(bool backward, const Tuning &t) {
                                                                           : bwd_grid_ppt<1>(t.bwd_batch);
 }
 
+__global__ void zero_heads(uint32_t *p) {
+    if (threadIdx.x < NUM_XCD) p[threadIdx.x] = 0u;
+}
+
 void launch_batch_raster_forward(const BatchRasterArgs &a, const Tuning &t, hipStream_t s) {
-    fill_u32<<<1, 1, 0, s>>>(a.queue, 0u);
+    zero_heads<<<1, 64, 0, s>>>(a.queue);
     const int g = batch_raster_grid(false, t);
     if (t.fwd_batch == 64) raster_forward_batch_kernel<64><<<g, 256, 0, s>>>(a);
     else if (t.fwd_batch == 128) raster_forward_batch_kernel<128><<<g, 256, 0, s>>>(a);
@@ -197,7 +221,7 @@ static void batch_backward_ppt(const BatchRasterArgs &a, int g, int bb, hipStrea
 }
 
 void launch_batch_raster_backward(const BatchRasterArgs &a, const Tuning &t, hipStream_t s) {
-    fill_u32<<<1, 1, 0, s>>>(a.queue + 1, 0u);
+    zero_heads<<<1, 64, 0, s>>>(a.queue + NUM_XCD);
     const int g = batch_raster_grid(true, t);
     if (t.bwd_ppt == 4) batch_backward_ppt<4>(a, g, t.bwd_batch, s);
     else if (t.bwd_ppt == 2) batch_backward_ppt<2>(a, g, t.bwd_batch, s);

@@ -192,6 +192,8 @@ class BatchViews:
         if bg.shape[0] == 1:
             bg = np.repeat(bg, n, 0)
         self.c_bgs = (C.c_float * (3 * n))(*bg.reshape(-1).tolist())
+        # the record holds raw device pointers: keep the arrays alive as long as it lives
+        self.targets = list(targets) if targets is not None else None
         self.c_targets = (C.c_void_p * n)(*[t.ptr if isinstance(t, DeviceArray) else t for t in targets]) \
             if targets is not None else None
         self.rec = Views()

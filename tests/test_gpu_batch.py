@@ -171,18 +171,35 @@ def test_graph_replays_equal_eager_steps(ctx):
 
 
 def test_capacity_regrowth_gives_the_same_frame(ctx):
-    """a context sized on a sparse frame meets a dense one: the step over capacity is
-    redone with larger buffers and equals a fresh context's"""
+    """A graph context sized on a sparse scene meets a dense one: the captured step
+    exceeds its intersection capacity, changes nothing, and runs again with grown
+    buffers; the result equals an eager step on a fresh context, bit for bit. Eager
+    render frames size themselves from a count pass and equal too."""
     from msplat._abi import Context
-    from msplat.backend import BatchViews, DeviceGaussians, render_batch_numpy
+    from msplat.backend import BatchViews, DeviceGaussians, render_batch_numpy, train_step_batch
     sc_small, cams = _setup(seed=17, n=300)
     sc_big, _ = _setup(seed=17, n=20000)
-    c2 = Context(0)
+    g = Context(0)
+    g.set_batch_graph(True)
+    zero, ones = np.zeros(6, np.float32), np.ones(6, np.float32)
+    steps = np.full(6, 1e-3, np.float32)
     try:
-        render_batch_numpy(c2, DeviceGaussians(c2, sc_small), BatchViews(cams, 3, [BG]))
-        got = render_batch_numpy(c2, DeviceGaussians(c2, sc_big), BatchViews(cams, 3, [BG]))
-        ref = render_batch_numpy(ctx, DeviceGaussians(ctx, sc_big), BatchViews(cams, 3, [BG]))
+        small = _state(g, sc_small)
+        train_step_batch(g, small, BatchViews(cams, 3, [BG], _targets(g, cams)), steps, ones)
+        big_g = _state(g, sc_big)
+        lg = train_step_batch(g, big_g, BatchViews(cams, 3, [BG], _targets(g, cams)), steps, ones)
+        captures, replays = g.batch_graph_stats()
+        assert captures == 3 and replays == 3  # small; big over capacity; big again
+        big_e = _state(ctx, sc_big)
+        le = train_step_batch(ctx, big_e, BatchViews(cams, 3, [BG], _targets(ctx, cams)), steps, ones)
+        np.testing.assert_array_equal(lg, le)
+        se, sg = big_e.to_numpy(), big_g.to_numpy()
+        for k in se:
+            if k != "degree":
+                np.testing.assert_array_equal(sg[k], se[k], err_msg=k)
+        got = render_batch_numpy(g, big_g, BatchViews(cams, 3, [BG]))
+        ref = render_batch_numpy(ctx, big_e, BatchViews(cams, 3, [BG]))
         for a, b in zip(got, ref):
             np.testing.assert_array_equal(a, b)
     finally:
-        c2.close()
+        g.close()
