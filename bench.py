@@ -281,6 +281,33 @@ def tune_train_step(ctx, cams, targets, seed=5):
     return res
 
 
+def copy_calibration(mib=256, times=5):
+    """Copies `mib` MiB device to device `times` times through msplat_hip_memcpy: a kernel
+    of known traffic (mib MiB read + mib MiB written per dispatch) for scaling the HBM
+    counters of a profile."""
+    import ctypes as C
+    from msplat._abi import Context, DeviceArray, check, lib
+    ctx = Context(0)
+    n = mib * 1024 * 1024
+    a, b = DeviceArray(ctx, (n,), np.uint8), DeviceArray(ctx, (n,), np.uint8)
+    for _ in range(times):
+        check(lib().msplat_hip_memcpy(ctx.ptr, C.c_void_p(b.ptr), C.c_void_p(a.ptr), n, 2))
+    ctx.sync()
+    log(f"copied {times} x {mib} MiB")
+
+
+def profiled_traffic(kernel_stage, tuning):
+    """HBM bytes per launch of the dominant kernel, from the committed counter summary
+    (profiles/raster_backward_street_475k_traffic.json, FETCH_SIZE and WRITE_SIZE scaled
+    by a copy of known size) when the timed kernel is the profiled variant; else None."""
+    path = os.path.join(ROOT, "profiles", "raster_backward_street_475k_traffic.json")
+    if kernel_stage != "rast_bwd" or not os.path.exists(path):
+        return None
+    prof = json.load(open(path))
+    variant = "raster_backward_kernel<%d, %d>" % (tuning.get("bwd_ppt", 4), tuning.get("bwd_batch", 128))
+    return prof["bytes_per_launch"] if prof.get("kernel") == variant else None
+
+
 def cpu_baseline_sample(dg, cams, targets, ctx, step_index, seconds=20.0):
     """Times the reference CPU train step (oracle/_ref, perf build) — or the oracle when
     the reference was not built — on this host, on the workload's own scene and view."""
@@ -331,12 +358,18 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=8)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default kernel variants")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device five times (the known-size kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--serving-seconds", type=float, default=1.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     from msplat._abi import Context
     from msplat.trainer import TrainConfig, Trainer
@@ -390,8 +423,13 @@ def main():
     per_isect, per_px, per_g = BYTES.get(dom, (0, 0, 0))
     alg_bytes = (per_isect * m_total + per_px * P * K + per_g * n_total) / max(dom_calls, 1)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    from msplat import autotune
+    cur = autotune.current(ctx)
+    traffic = profiled_traffic(dom, cur)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": "profiles/raster_backward_street_475k_traffic.md" if traffic else None,
+                "variant": cur,
                 "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(alg_bytes),
                 "stages_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in stages.items()}}
     batched = None

@@ -36,6 +36,7 @@ struct BatchState {
     bool tab_pending = false;
     uint32_t cap = 0;  // intersection capacity
     uint32_t tiles_cap = 0, pixels_cap = 0, rows_cap = 0, loss_cap = 0;
+    uint32_t vtiles_cap = 0;  // tiles of the largest view (the local tile sort's key range)
     int n = 0, views = 0;
     uint64_t generation = 0;  // bumped when a buffer moves
     // the last rendered frame
@@ -163,6 +164,10 @@ int prepare(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g, c
         L.loss_blocks += ((c.width + 31) / 32) * ((c.height + 15) / 16);
     }
     t.tile_base[nv] = L.tiles;
+    uint32_t vt = 1;
+    for (int v = 0; v < nv; v++)
+        vt = std::max(vt, (uint32_t)(views->cameras[v].tiles_x * views->cameras[v].tiles_y));
+    t.max_view_tiles = vt;
     t.row_base[nv] = L.rows;
     t.loss_base[nv] = L.loss_blocks;
     t.total_tiles = L.tiles;
@@ -204,7 +209,8 @@ int prepare(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g, c
     CHECKED(grow(ctx, b, b->scan_part, 4 * 3 * scan_partial_words((uint32_t)std::max(nbx * nv, (R + 255) / 256)) + 64));
     CHECKED(grow(ctx, b, b->big_list, 4 * R));
     CHECKED(grow(ctx, b, b->big_sums, 36 * R + 64));
-    b->tiles_cap = std::max(b->tiles_cap, L.tiles);
+    b->tiles_cap = std::max(b->tiles_cap, std::max(L.tiles, vt * (uint32_t)nv));  // items and ranges
+    b->vtiles_cap = std::max(b->vtiles_cap, vt);
     b->pixels_cap = std::max(b->pixels_cap, L.pixels);
     b->rows_cap = std::max(b->rows_cap, L.rows);
     b->loss_cap = std::max(b->loss_cap, L.loss_blocks);
@@ -339,6 +345,11 @@ void enqueue_sort_and_forward(msplat_hip_ctx *ctx, BatchState *b, const msplat_h
         std::swap(dki, dko);
         std::swap(dvi, dvo);
     }
+    // stably by view: each view's records in depth order, views in order
+    launch_view_keys(dvi, &sc->nvis, R, g->num_points, dki, s);
+    launch_radix_pass(dki, dvi, dko, dvo, &sc->nvis, R, 0, (uint32_t *)b->hist.p, (uint32_t *)b->digit_total.p, s);
+    std::swap(dki, dko);
+    std::swap(dvi, dvo);
     launch_order_sums(dvi, &sc->nvis, R, (const uint32_t *)b->count.p, (uint32_t *)b->ord_sums.p, s);
     launch_scan_u32((uint32_t *)b->ord_sums.p, (R + 255) / 256, scan_parts(b, 2),
                     ScanOut{nullptr, &sc->total32_b, b->cap, &sc->overflow_b, nullptr}, s);
@@ -364,7 +375,7 @@ void enqueue_sort_and_forward(msplat_hip_ctx *ctx, BatchState *b, const msplat_h
     launch_keygen(k, true, s);
     uint32_t *kin = (uint32_t *)b->keys0.p, *kout = (uint32_t *)b->keys1.p;
     int32_t *vin = (int32_t *)b->vals0.p, *vout = (int32_t *)b->vals1.p;
-    const int bits = tile_bits((int)b->tiles_cap);
+    const int bits = tile_bits((int)b->vtiles_cap);
     for (int shift = 0; shift < bits; shift += 8) {
         launch_radix_pass(kin, vin, kout, vout, &sc->total32, b->cap, shift, (uint32_t *)b->hist.p,
                           (uint32_t *)b->digit_total.p, s);
@@ -372,7 +383,7 @@ void enqueue_sort_and_forward(msplat_hip_ctx *ctx, BatchState *b, const msplat_h
         std::swap(vin, vout);
     }
     b->gid_sorted = vin;
-    launch_tile_ranges(kin, &sc->total32, b->cap, &dtab(b)->total_tiles, 0, (int32_t *)b->tile_start.p, s);
+    launch_tile_ranges_views(kin, vin, &sc->total32, b->cap, g->num_points, dtab(b), (int32_t *)b->tile_start.p, s);
     launch_batch_raster_forward(raster_args(b), ctx->tuning, s);
 }
 
@@ -507,7 +518,7 @@ std::vector<uint64_t> graph_key(msplat_hip_ctx *ctx, BatchState *b, const msplat
                                 const msplat_hip_adam *adam, const msplat_hip_grad_stats *st) {
     std::vector<uint64_t> k = {(uint64_t)b->n,          (uint64_t)b->views,       b->cap,
                                b->tiles_cap,            b->pixels_cap,            b->rows_cap,
-                               b->loss_cap,             b->generation,            (uint64_t)ctx->tuning.bwd_ppt,
+                               b->loss_cap,             b->generation,            b->vtiles_cap,            (uint64_t)ctx->tuning.bwd_ppt,
                                (uint64_t)ctx->tuning.fwd_batch, (uint64_t)ctx->tuning.bwd_batch,
                                b->h_tab->views[0].cam.degree};
     const void *ptrs[] = {g->means, g->scales, g->quats, g->features_dc, g->features_rest, g->opacities,

@@ -532,16 +532,9 @@ __global__ __launch_bounds__(256) void keygen_emit_kernel(KeygenArgs a) {
         const int x0 = rr.x & 0xFFFF, x1 = rr.x >> 16, y0 = rr.y & 0xFFFF;
         const uint32_t w = (uint32_t)(x1 - x0);
         const int ty = y0 + (int)(local / w), tx = x0 + (int)(local % w);
-        uint32_t tiles_x, tbase;
-        if (BATCH) {
-            const int v = r / a.n;
-            tiles_x = (uint32_t)a.tab->views[v].cam.tiles_x;
-            tbase = a.tab->tile_base[v];
-        } else {
-            tiles_x = (uint32_t)a.tiles_x;
-            tbase = 0u;
-        }
-        a.keys[o] = tbase + (uint32_t)ty * tiles_x + (uint32_t)tx;
+        // local tile id (a batch keeps its views apart by the emission order)
+        const uint32_t tiles_x = BATCH ? (uint32_t)a.tab->views[r / a.n].cam.tiles_x : (uint32_t)a.tiles_x;
+        a.keys[o] = (uint32_t)ty * tiles_x + (uint32_t)tx;
         a.vals[o] = r;
     }
 }
@@ -557,6 +550,29 @@ __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t *keys, 
     const int t = k < n ? (int)keys[k] : ntiles;
     const int tp = k > 0 ? (int)keys[k - 1] : -1;
     for (int u = tp + 1; u <= t; u++) tile_start[u] = (int32_t)k;
+}
+
+// View id of each depth-sorted record, the key of one more stable pass that makes the
+// record order view-major (each view's records in depth order)
+__global__ __launch_bounds__(256) void view_keys_kernel(const int32_t *recs, const uint32_t *n_ptr, int n,
+                                                       uint32_t *keys) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < *n_ptr) keys[i] = (uint32_t)(recs[i] / n);
+}
+
+// Ranges of a batch sorted by local tile, with each tile's entries grouped by view:
+// pair (t, v) owns [range_start[t V + v], range_start[t V + v + 1]), V = num_views.
+__global__ __launch_bounds__(256) void tile_ranges_views_kernel(const uint32_t *keys, const int32_t *recs,
+                                                               const uint32_t *n_ptr, int n, const BatchTable *tab,
+                                                               int32_t *range_start) {
+    const uint32_t m = *n_ptr;
+    const uint32_t V = (uint32_t)tab->num_views;
+    const int total = (int)(tab->max_view_tiles * V);
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k > m) return;
+    const int c = k < m ? (int)(keys[k] * V + (uint32_t)(recs[k] / n)) : total;
+    const int cp = k > 0 ? (int)(keys[k - 1] * V + (uint32_t)(recs[k - 1] / n)) : -1;
+    for (int u = cp + 1; u <= c; u++) range_start[u] = (int32_t)k;
 }
 
 // The 64-bit (tile << 32 | depth bits) view of the sorted intersections
@@ -720,6 +736,16 @@ void launch_tile_ranges(const uint32_t *keys, const uint32_t *n_dev, uint32_t n_
 void launch_keys64(const uint32_t *tiles, const int32_t *gid, const float *depths, uint32_t n, uint64_t *out,
                    hipStream_t s) {
     if (n) keys64_kernel<<<(n + 255) / 256, 256, 0, s>>>(tiles, gid, depths, n, out);
+}
+
+void launch_view_keys(const int32_t *recs, const uint32_t *n_dev, uint32_t n_max, int n, uint32_t *keys,
+                      hipStream_t s) {
+    if (n_max) view_keys_kernel<<<(n_max + 255) / 256, 256, 0, s>>>(recs, n_dev, n, keys);
+}
+
+void launch_tile_ranges_views(const uint32_t *keys, const int32_t *recs, const uint32_t *n_dev, uint32_t n_max, int n,
+                              const BatchTable *tab, int32_t *range_start, hipStream_t s) {
+    tile_ranges_views_kernel<<<(n_max + 1 + 255) / 256, 256, 0, s>>>(keys, recs, n_dev, n, tab, range_start);
 }
 
 void launch_batch_project(const BatchProjectArgs &a, hipStream_t s) {

@@ -168,9 +168,11 @@ void launch_init_gaussians(const float *md, const uint8_t *rgb, uint32_t n, uint
                            float *quats, float *dc, float *opac, hipStream_t s);
 
 // ── batched views ───────────────────────────────────────────────────────────
-// Several views of one scene go through one projection, one key sort over global tile
-// ids (tile_base[v] + tile), one persistent rasterizer over (view, tile) items, one
-// loss launch and one per-gaussian backward that sums over the views. Everything that
+// Several views of one scene go through one projection, one depth sort of the visible
+// records (then stably by view), one sort of the intersections by local tile id (each
+// tile's range then holds the views in order, each in depth order: range index
+// tile * num_views + view), one persistent rasterizer over (view, tile) items, one loss
+// launch and one per-gaussian backward that sums over the views. Everything that
 // changes from step to step (poses, resolutions, targets, Adam scalars) is read from a
 // BatchTable in device memory, so a captured graph replays with new contents.
 // Record r = v * n + g holds gaussian g's projection in view v.
@@ -192,6 +194,7 @@ struct BatchTable {
     float step[6], bc2[6];
     float beta1, beta2, eps;
     uint32_t rest_active;  // SH rest coefficients per gaussian in the bands in use (max over the views)
+    uint32_t max_view_tiles;  // tiles of the largest view: ranges run over max_view_tiles * num_views
     // prefix offsets over the views (num_views + 1 entries each)
     uint32_t tile_base[MAX_VIEWS + 1], row_base[MAX_VIEWS + 1], loss_base[MAX_VIEWS + 1];
     BatchView views[MAX_VIEWS];
@@ -302,6 +305,12 @@ void launch_tile_ranges(const uint32_t *keys, const uint32_t *n_dev, uint32_t n_
                         int ntiles, int32_t *tile_start, hipStream_t s);
 void launch_keys64(const uint32_t *tiles, const int32_t *gid, const float *depths, uint32_t n, uint64_t *out,
                    hipStream_t s);
+// batch: view ids of the depth-sorted records (record / n) as sort keys
+void launch_view_keys(const int32_t *recs, const uint32_t *n_dev, uint32_t n_max, int n, uint32_t *keys,
+                      hipStream_t s);
+// batch: ranges over (local tile, view) pairs, index tile * num_views + view
+void launch_tile_ranges_views(const uint32_t *keys, const int32_t *recs, const uint32_t *n_dev, uint32_t n_max, int n,
+                              const BatchTable *tab, int32_t *range_start, hipStream_t s);
 void launch_raster_forward(const RasterArgs &a, const Tuning &t, hipStream_t s);
 void launch_raster_backward(const RasterArgs &a, const Tuning &t, hipStream_t s);
 void launch_loss(const LossArgs &a, hipStream_t s);

@@ -88,8 +88,9 @@ __global__ __launch_bounds__(256 / PPT) void raster_backward_kernel(RasterArgs a
     backward_tile<PPT, BB>(tile_view(a), tile_data(a), t, t);
 }
 
-// Persistent forward over every (view, tile) of a batch. Every workgroup loops until
-// the queue passes the batch's tile count, so the grid drains on its own.
+// Persistent forward over every (view, tile) of a batch; item t is tile t - tile_base[v]
+// of view v, whose range is number (tile * num_views + v). Every workgroup loops until
+// the queue is drained, so the grid drains on its own.
 template <int FB>
 __global__ __launch_bounds__(256) void raster_forward_batch_kernel(BatchRasterArgs a) {
     __shared__ uint32_t tb[MAX_VIEWS + 1];
@@ -103,7 +104,8 @@ __global__ __launch_bounds__(256) void raster_forward_batch_kernel(BatchRasterAr
         const uint32_t t = next_item(q, &item_slot);
         if (t >= total) break;
         const int v = view_of(tb, nv, t);
-        forward_tile<FB>(tile_view(a, v), d, (int)t, (int)(t - tb[v]));
+        const int lt = (int)(t - tb[v]);
+        forward_tile<FB>(tile_view(a, v), d, lt * nv + v, lt);
     }
 }
 
@@ -120,7 +122,8 @@ __global__ __launch_bounds__(256 / PPT) void raster_backward_batch_kernel(BatchR
         const uint32_t t = next_item(q, &item_slot);
         if (t >= total) break;
         const int v = view_of(tb, nv, t);
-        backward_tile<PPT, BB>(tile_view(a, v), d, (int)t, (int)(t - tb[v]));
+        const int lt = (int)(t - tb[v]);
+        backward_tile<PPT, BB>(tile_view(a, v), d, lt * nv + v, lt);
     }
 }
 

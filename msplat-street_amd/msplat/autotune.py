@@ -47,10 +47,12 @@ def _time(ctx: Context, step, warmup: int, repeats: int) -> float:
 
 
 def autotune(ctx: Context, step, warmup: int = 2, repeats: int = 8, space: dict | None = None,
-             cache: str | None = None, key: str | None = None, rounds: int = 1) -> dict:
+             cache: str | None = None, key: str | None = None, rounds: int = 1, min_gain: float = 0.02) -> dict:
     """Picks the fastest variant for `step()` (a callable that enqueues one unit of the
-    workload on ctx) and leaves it set on ctx. Returns {tunable: value, ..., "ms": best
-    ms per step, "trials": [(choice, ms), ...], "cached": bool}."""
+    workload on ctx) and leaves it set on ctx. A variant replaces the best so far only
+    when it is faster by more than `min_gain` (a fraction), so timing noise does not flip
+    the choice. Returns {tunable: value, ..., "ms": best ms per step, "trials":
+    [(choice, ms), ...], "cached": bool}."""
     space = space or SPACE
     if cache and key:
         try:
@@ -73,7 +75,7 @@ def autotune(ctx: Context, step, warmup: int = 2, repeats: int = 8, space: dict 
                 apply(ctx, trial)
                 ms = _time(ctx, step, warmup, repeats)
                 trials.append((trial, ms))
-                if ms < best_ms:
+                if ms < best_ms * (1.0 - min_gain):
                     best, best_ms = trial, ms
             apply(ctx, best)
     out = dict(best, ms=round(best_ms, 4), trials=[(c, round(m, 4)) for c, m in trials], cached=False)
