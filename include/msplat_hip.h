@@ -162,8 +162,18 @@ int msplat_hip_set_flags(msplat_hip_ctx *ctx, int flags);
  *   MSPLAT_TUNE_FWD_BATCH  forward LDS batch depth (gaussian records staged per batch):
  *                          64, 128, 256 (default 256)
  *   MSPLAT_TUNE_BWD_BATCH  backward LDS batch depth: 32, 64, 128 (default 128)
+ *   MSPLAT_TUNE_CULL       1 (default): the rasterizers skip a tile's gaussians whose
+ *                          alpha >= 1/255 ellipse (widened for float rounding) misses
+ *                          the tile; 0: every listed gaussian is visited. Skipped pairs
+ *                          would composite nothing, so both give identical results.
  * The binning tile stays 16x16: it decides which gaussians reach a pixel. */
-enum { MSPLAT_TUNE_BWD_PPT = 0, MSPLAT_TUNE_FWD_BATCH = 1, MSPLAT_TUNE_BWD_BATCH = 2, MSPLAT_NUM_TUNABLES = 3 };
+enum {
+    MSPLAT_TUNE_BWD_PPT = 0,
+    MSPLAT_TUNE_FWD_BATCH = 1,
+    MSPLAT_TUNE_BWD_BATCH = 2,
+    MSPLAT_TUNE_CULL = 3,
+    MSPLAT_NUM_TUNABLES = 4
+};
 int msplat_hip_set_tuning(msplat_hip_ctx *ctx, int key, int value);
 int msplat_hip_get_tuning(msplat_hip_ctx *ctx, int key, int *value);
 

@@ -780,6 +780,10 @@ int msplat_hip_set_tuning(msplat_hip_ctx *ctx, int key, int value) {
             return set_error(MSPLAT_ERR_ARG, "backward LDS batch depth must be 32, 64 or 128");
         ctx->tuning.bwd_batch = value;
         return MSPLAT_OK;
+    case MSPLAT_TUNE_CULL:
+        if (value != 0 && value != 1) return set_error(MSPLAT_ERR_ARG, "cull must be 0 or 1");
+        ctx->tuning.cull = value;
+        return MSPLAT_OK;
     default:
         return set_error(MSPLAT_ERR_ARG, "unknown tunable %d", key);
     }
@@ -791,6 +795,7 @@ int msplat_hip_get_tuning(msplat_hip_ctx *ctx, int key, int *value) {
     case MSPLAT_TUNE_BWD_PPT: *value = ctx->tuning.bwd_ppt; return MSPLAT_OK;
     case MSPLAT_TUNE_FWD_BATCH: *value = ctx->tuning.fwd_batch; return MSPLAT_OK;
     case MSPLAT_TUNE_BWD_BATCH: *value = ctx->tuning.bwd_batch; return MSPLAT_OK;
+    case MSPLAT_TUNE_CULL: *value = ctx->tuning.cull; return MSPLAT_OK;
     default: return set_error(MSPLAT_ERR_ARG, "unknown tunable %d", key);
     }
 }

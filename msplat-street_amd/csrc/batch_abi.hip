@@ -520,6 +520,7 @@ std::vector<uint64_t> graph_key(msplat_hip_ctx *ctx, BatchState *b, const msplat
                                b->tiles_cap,            b->pixels_cap,            b->rows_cap,
                                b->loss_cap,             b->generation,            b->vtiles_cap,            (uint64_t)ctx->tuning.bwd_ppt,
                                (uint64_t)ctx->tuning.fwd_batch, (uint64_t)ctx->tuning.bwd_batch,
+                               (uint64_t)ctx->tuning.cull,
                                b->h_tab->views[0].cam.degree};
     const void *ptrs[] = {g->means, g->scales, g->quats, g->features_dc, g->features_rest, g->opacities,
                           st->vis_counts, st->xys_grad_norm, st->max_2d_size};

@@ -16,6 +16,7 @@ struct Tuning {
     int bwd_ppt = 4;      // backward pixels per thread: 1, 2, 4
     int fwd_batch = 256;  // forward LDS batch depth: 64, 128, 256
     int bwd_batch = 128;  // backward LDS batch depth: 32, 64, 128
+    int cull = 1;         // skip a tile's gaussians whose alpha >= 1/255 ellipse misses it
 };
 
 struct ProjectArgs {
@@ -91,6 +92,7 @@ struct RasterArgs {
     const uint32_t *offsets;
     float4 *slot_a, *slot_b;
     float *slot_c;
+    int cull;  // set by the launchers from Tuning::cull
 };
 
 struct LossArgs {
@@ -270,6 +272,7 @@ struct BatchRasterArgs {
     const float *v_image;
     float4 *slot_a, *slot_b;
     float *slot_c;
+    int cull;  // set by the launchers from Tuning::cull
 };
 int batch_raster_grid(bool backward, const Tuning &t);  // persistent workgroups that fill the device
 void launch_batch_raster_forward(const BatchRasterArgs &a, const Tuning &t, hipStream_t s);

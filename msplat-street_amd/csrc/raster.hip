@@ -8,8 +8,8 @@ namespace msplat {
 namespace {
 
 __device__ inline TileData tile_data(const RasterArgs &a) {
-    return TileData{a.tile_start, a.gid, a.splat_a, a.splat_b, a.splat_blue, a.rect, a.offsets,
-                    a.slot_a,     a.slot_b, a.slot_c};
+    return TileData{a.tile_start, a.gid,    a.splat_a, a.splat_b, a.splat_blue, a.rect,
+                    a.offsets,    a.slot_a, a.slot_b,  a.slot_c,  a.cull != 0};
 }
 
 __device__ inline TileView tile_view(const RasterArgs &a) {
@@ -18,8 +18,8 @@ __device__ inline TileView tile_view(const RasterArgs &a) {
 }
 
 __device__ inline TileData tile_data(const BatchRasterArgs &a) {
-    return TileData{a.tile_start, a.gid, a.splat_a, a.splat_b, a.splat_blue, a.rect, a.offsets,
-                    a.slot_a,     a.slot_b, a.slot_c};
+    return TileData{a.tile_start, a.gid,    a.splat_a, a.splat_b, a.splat_blue, a.rect,
+                    a.offsets,    a.slot_a, a.slot_b,  a.slot_c,  a.cull != 0};
 }
 
 // View v of a batch: per-pixel buffers offset to the view's first pixel
@@ -133,7 +133,9 @@ static void forward_one(const RasterArgs &a, int nt, hipStream_t s) {
     raster_forward_kernel<FB><<<nt, 256, 0, s>>>(a);
 }
 
-void launch_raster_forward(const RasterArgs &a, const Tuning &t, hipStream_t s) {
+void launch_raster_forward(const RasterArgs &args, const Tuning &t, hipStream_t s) {
+    RasterArgs a = args;
+    a.cull = t.cull;
     const int nt = a.tiles_x * a.tiles_y;
     if (nt <= 0) return;
     if (t.fwd_batch == 64) forward_one<64>(a, nt, s);
@@ -148,7 +150,9 @@ static void backward_ppt(const RasterArgs &a, int nt, int bb, hipStream_t s) {
     else raster_backward_kernel<PPT, 128><<<nt, 256 / PPT, 0, s>>>(a);
 }
 
-void launch_raster_backward(const RasterArgs &a, const Tuning &t, hipStream_t s) {
+void launch_raster_backward(const RasterArgs &args, const Tuning &t, hipStream_t s) {
+    RasterArgs a = args;
+    a.cull = t.cull;
     const int nt = a.tiles_x * a.tiles_y;
     if (nt <= 0) return;
     if (t.bwd_ppt == 4) backward_ppt<4>(a, nt, t.bwd_batch, s);
@@ -208,7 +212,9 @@ __global__ void zero_heads(uint32_t *p) {
     if (threadIdx.x < NUM_XCD) p[threadIdx.x] = 0u;
 }
 
-void launch_batch_raster_forward(const BatchRasterArgs &a, const Tuning &t, hipStream_t s) {
+void launch_batch_raster_forward(const BatchRasterArgs &args, const Tuning &t, hipStream_t s) {
+    BatchRasterArgs a = args;
+    a.cull = t.cull;
     zero_heads<<<1, 64, 0, s>>>(a.queue);
     const int g = batch_raster_grid(false, t);
     if (t.fwd_batch == 64) raster_forward_batch_kernel<64><<<g, 256, 0, s>>>(a);
@@ -223,7 +229,9 @@ static void batch_backward_ppt(const BatchRasterArgs &a, int g, int bb, hipStrea
     else raster_backward_batch_kernel<PPT, 128><<<g, 256 / PPT, 0, s>>>(a);
 }
 
-void launch_batch_raster_backward(const BatchRasterArgs &a, const Tuning &t, hipStream_t s) {
+void launch_batch_raster_backward(const BatchRasterArgs &args, const Tuning &t, hipStream_t s) {
+    BatchRasterArgs a = args;
+    a.cull = t.cull;
     zero_heads<<<1, 64, 0, s>>>(a.queue + NUM_XCD);
     const int g = batch_raster_grid(true, t);
     if (t.bwd_ppt == 4) batch_backward_ppt<4>(a, g, t.bwd_batch, s);

@@ -40,7 +40,51 @@ struct TileData {
     const uint32_t *offsets;
     float4 *slot_a, *slot_b;
     float *slot_c;
+    bool cull;  // skip gaussians that splat_misses_tile proves invisible in the tile
 };
+
+// True when the splat composites no pixel of the tile whose top-left pixel is (x0, y0):
+// the tile lies outside the bounding box of the ellipse {0.5 d^T Q d <= L'}, with Q the
+// conic and L' above the largest sigma that passes both sigma < SIGMA_MAX and
+// opacity exp(-sigma) >= ALPHA_MIN. The pixel loops compute sigma to a relative error
+// below 16 u kappa (u = 2^-24, kappa the conic's condition number) and alpha to a few
+// ulp, so the test is made only for kappa < 1024 and L' is the exact level raised by
+// 2^-8 relative plus 2^-8 absolute; the box is widened by 0.1% plus 0.01 px for the
+// rounding of its own arithmetic. Any pair it rejects would fail the validity test in
+// every pixel, and an invalid pair changes nothing (it adds exact zeros), so skipping
+// it leaves every output bit unchanged.
+__device__ inline bool splat_misses_tile(const float4 &A, const float4 &B, float x0, float y0) {
+    const float a = A.z, b = A.w, c = B.x;
+    const float det = a * c - b * b, tr = a + c;
+    if (!(det > 0.f) || !(tr * tr < 1024.f * det)) return false;  // (a+c)^2/det = kappa + 2 + 1/kappa
+    const float L = fminf(SIGMA_MAX, __logf(B.y * (1.f / ALPHA_MIN)));
+    if (!(L > 1.f / 64.f)) return false;
+    const float k2 = 2.f * (L * (1.f + 1.f / 256.f) + 1.f / 256.f) / det;
+    const float hx = sqrtf(k2 * c) * 1.001f + 0.01f, hy = sqrtf(k2 * a) * 1.001f + 0.01f;
+    const float x1 = x0 + (float)(TILE - 1), y1 = y0 + (float)(TILE - 1);
+    return A.x + hx < x0 || A.x - hx > x1 || A.y + hy < y0 || A.y - hy > y1;
+}
+
+// Position of the calling thread's item among the workgroup's kept items (in thread
+// order) and, in `total`, their number. Every thread of the NW-wave workgroup calls it;
+// `wcount` must not be written again before every wave has returned (callers alternate
+// two buffers or pass a barrier in between).
+template <int NW>
+__device__ inline int kept_position(bool keep, int *wcount, int &total) {
+    const uint64_t bal = __ballot(keep);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) wcount[w] = __popcll(bal);
+    __syncthreads();
+    int off = 0;
+    total = 0;
+#pragma unroll
+    for (int i = 0; i < NW; i++) {
+        const int cnt = wcount[i];
+        off += i < w ? cnt : 0;
+        total += cnt;
+    }
+    return off + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+}
 
 // nd_rasterize_forward (core/cpu/msplat_cpu.cpp:574-626) for global tile `gt`, which is
 // tile `lt` of view `v`. Every thread of the 256-thread workgroup calls it; FB gaussian
@@ -51,7 +95,10 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
     static_assert(FB <= 256, "one staged record per thread");
     __shared__ float4 la[FB], lb[FB];
     __shared__ float lc[FB];
+    __shared__ int lk[FB];  // the staged record's position in the tile's list
+    __shared__ int wcount[4];
     const int tx = lt % v.tiles_x, ty = lt / v.tiles_x;
+    const float x0 = (float)(tx * TILE), y0 = (float)(ty * TILE);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int px = tx * TILE + (w & 1) * 8 + (lane & 7);
     const int py = ty * TILE + (w >> 1) * 8 + (lane >> 3);
@@ -63,16 +110,28 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
     bool done = !inside;
     for (int base = s; base < e; base += FB) {
         if (__syncthreads_count(done) == 256) break;
+        // stage the batch's records that can reach the tile, in list order
         const int k = base + threadIdx.x;
+        bool keep = false;
+        float4 A0, B0;
+        float C0 = 0.f;
         if (threadIdx.x < FB && k < e) {
             const int g = d.gid[k];
-            la[threadIdx.x] = d.splat_a[g];
-            lb[threadIdx.x] = d.splat_b[g];
-            lc[threadIdx.x] = d.splat_blue[g];
+            A0 = d.splat_a[g];
+            B0 = d.splat_b[g];
+            C0 = d.splat_blue[g];
+            keep = !(d.cull && splat_misses_tile(A0, B0, x0, y0));
+        }
+        int n;
+        const int pos = kept_position<4>(keep, wcount, n);
+        if (keep) {
+            la[pos] = A0;
+            lb[pos] = B0;
+            lc[pos] = C0;
+            lk[pos] = k;
         }
         __syncthreads();
         if (__ballot(!done) != 0) {
-            const int n = min(FB, e - base);
             for (int j = 0; j < n; j++) {
                 // predicated compositing step: skip (out of range / too faint), accept, or
                 // terminate (transmittance would drop to T_MIN: last = k - 1, as the CPU)
@@ -89,7 +148,8 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
                 cg += fmaxf(B.w + 0.5f, 0.f) * vis;
                 cb += fmaxf(lc[j] + 0.5f, 0.f) * vis;
                 T = accept ? nT : T;
-                last = term ? base + j - 1 : (accept ? base + j : last);
+                const int kj = lk[j];
+                last = term ? kj - 1 : (accept ? kj : last);
                 done = done || term;
                 if ((j & 7) == 7 && __ballot(!done) == 0) break;
             }
@@ -182,14 +242,17 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
     __shared__ float4 la[BATCH], lb[BATCH];
     __shared__ float lc[BATCH];
     __shared__ uint32_t lslot[BATCH];
+    __shared__ int lk[BATCH];  // the staged record's position in the tile's list
     __shared__ float part[NW][BATCH][NQ];
     __shared__ int smax[NW];
+    __shared__ int wcount[2][NW];
     const int tx = lt % v.tiles_x, ty = lt / v.tiles_x;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int s = d.tile_start[gt], e = d.tile_start[gt + 1];
     if (s == e) return;
     const SumLanes sl(lane);
     const int px = tx * TILE + (lane & 15);
+    const float x0 = (float)(tx * TILE), y0 = (float)(ty * TILE);
     float fy[PPT], T[PPT], S[PPT], vr[PPT], vg[PPT], vb[PPT];
     int last[PPT];
     int m = s - 1;
@@ -223,8 +286,8 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
     for (int k = kmax + 1 + threadIdx.x; k < e; k += NT) {
         const int g = d.gid[k];
         const uint2 r = d.rect[g];
-        const int x0 = r.x & 0xFFFF, x1 = r.x >> 16, y0 = r.y & 0xFFFF;
-        const uint32_t p = d.offsets[g] + (uint32_t)((ty - y0) * (x1 - x0) + (tx - x0));
+        const int rx0 = r.x & 0xFFFF, rx1 = r.x >> 16, ry0 = r.y & 0xFFFF;
+        const uint32_t p = d.offsets[g] + (uint32_t)((ty - ry0) * (rx1 - rx0) + (tx - rx0));
         d.slot_a[p] = make_float4(0.f, 0.f, 0.f, 0.f);
         d.slot_b[p] = make_float4(0.f, 0.f, 0.f, 0.f);
         d.slot_c[p] = 0.f;
@@ -234,19 +297,45 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
         const int start = max(s, end - BATCH);
         const int n = end - start;
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += NT) {
-            const int g = d.gid[start + i];
-            la[i] = d.splat_a[g];
-            lb[i] = d.splat_b[g];
-            lc[i] = d.splat_blue[g];
-            const uint2 r = d.rect[g];
-            const int x0 = r.x & 0xFFFF, x1 = r.x >> 16, y0 = r.y & 0xFFFF;
-            lslot[i] = d.offsets[g] + (uint32_t)((ty - y0) * (x1 - x0) + (tx - x0));
+        // stage the batch's records that can reach the tile, in list order; the others
+        // get their zero slot here
+        int kept = 0;
+        for (int c = 0; c < n; c += NT) {
+            const int i = c + threadIdx.x;
+            bool keep = false;
+            float4 A0, B0;
+            float C0 = 0.f;
+            uint32_t p = 0;
+            if (i < n) {
+                const int g = d.gid[start + i];
+                A0 = d.splat_a[g];
+                B0 = d.splat_b[g];
+                C0 = d.splat_blue[g];
+                const uint2 r = d.rect[g];
+                const int rx0 = r.x & 0xFFFF, rx1 = r.x >> 16, ry0 = r.y & 0xFFFF;
+                p = d.offsets[g] + (uint32_t)((ty - ry0) * (rx1 - rx0) + (tx - rx0));
+                keep = !(d.cull && splat_misses_tile(A0, B0, x0, y0));
+                if (!keep) {
+                    d.slot_a[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    d.slot_b[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    d.slot_c[p] = 0.f;
+                }
+            }
+            int total;
+            const int pos = kept + kept_position<NW>(keep, wcount[(c / NT) & 1], total);
+            if (keep) {
+                la[pos] = A0;
+                lb[pos] = B0;
+                lc[pos] = C0;
+                lslot[pos] = p;
+                lk[pos] = start + i;
+            }
+            kept += total;
         }
         for (int i = threadIdx.x; i < NW * NQ * BATCH; i += NT) (&part[0][0][0])[i] = 0.f;
         __syncthreads();
-        for (int j = n - 1; j >= 0; j--) {
-            const int kk = start + j;
+        for (int j = kept - 1; j >= 0; j--) {
+            const int kk = lk[j];
             const float4 A = la[j], B = lb[j];
             const float cr = fmaxf(B.z + 0.5f, 0.f), cg = fmaxf(B.w + 0.5f, 0.f), cbl = fmaxf(lc[j] + 0.5f, 0.f);
             float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f, q4 = 0.f, q5 = 0.f, q6 = 0.f, q7 = 0.f, q8 = 0.f;
@@ -285,7 +374,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             }
         }
         __syncthreads();
-        for (int j = threadIdx.x; j < n; j += NT) {
+        for (int j = threadIdx.x; j < kept; j += NT) {
             float q[NQ];
 #pragma unroll
             for (int i = 0; i < NQ; i++) {

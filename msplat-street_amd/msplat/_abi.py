@@ -20,6 +20,7 @@ FRAME_KEYS64 = 2
 TUNE_BWD_PPT = 0
 TUNE_FWD_BATCH = 1
 TUNE_BWD_BATCH = 2
+TUNE_CULL = 3
 fp = C.POINTER(C.c_float)
 
 
