@@ -187,9 +187,12 @@ int ensure_pixels(msplat_hip_ctx *ctx) {
     CHECKED(ctx->grow(ctx->final_T, 4 * P));
     CHECKED(ctx->grow(ctx->final_idx, 4 * P));
     CHECKED(ctx->grow(ctx->v_image, 12 * P));
-    CHECKED(ctx->grow(ctx->pix_loss, 4 * P));
-    CHECKED(ctx->grow(ctx->row_loss, 8 * (size_t)ctx->height));
+    CHECKED(ctx->grow(ctx->row_loss, 24 * (size_t)ctx->height));
     return MSPLAT_OK;
+}
+
+int ensure_loss_scratch(msplat_hip_ctx *ctx) {
+    return ctx->grow(ctx->loss_scratch, 4 * (size_t)LOSS_PLANES * ctx->width * ctx->height);
 }
 
 RasterArgs raster_args(msplat_hip_ctx *ctx) {
@@ -235,7 +238,8 @@ int do_loss(msplat_hip_ctx *ctx, const float *gt, float ssim_weight, float inv_n
     l.ssim_weight = ssim_weight;
     l.inv_n = inv_n;
     l.v_img = (float *)ctx->v_image.p;
-    l.pix_loss = (float *)ctx->pix_loss.p;
+    CHECKED(ensure_loss_scratch(ctx));
+    l.scratch = (float *)ctx->loss_scratch.p;
     l.row_loss = (double *)ctx->row_loss.p;
     l.loss_sum = ctx->d_loss_sum();
     l.loss_mean = ctx->d_loss();
@@ -569,6 +573,7 @@ int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *f) {
     f->final_T = (float *)ctx->final_T.p;
     f->final_idx = (int32_t *)ctx->final_idx.p;
     f->v_image = (float *)ctx->v_image.p;
+    f->loss_sum = ctx->scalars.p ? ctx->d_loss_sum() : nullptr;
     f->v_xy = (float *)ctx->v_xy.p;
     f->v_conic = (float *)ctx->v_conic.p;
     f->v_rgb = (float *)ctx->v_rgb.p;

@@ -26,7 +26,7 @@ struct BatchState {
     // per intersection
     Buf keys0, keys1, vals0, vals1, slot_a, slot_b, slot_c, hist, digit_total;
     // per tile, pixel, row
-    Buf tile_start, image, final_T, final_idx, v_image, pix_loss, row_loss;
+    Buf tile_start, image, final_T, final_idx, v_image, row_loss, loss_scratch;
     // summed gradients (per gaussian), visibility in any view
     Buf grads, vis_any;
     Buf table, scalars;
@@ -58,7 +58,7 @@ void destroy_batch_state(BatchState *b) {
                   &b->splat_a, &b->splat_b, &b->splat_blue, &b->depths,   &b->rect,     &b->radii,      &b->count,
                   &b->offsets, &b->block_sums, &b->big_list, &b->big_sums, &b->keys0, &b->keys1,    &b->vals0,
                   &b->vals1,   &b->slot_a,     &b->slot_b,   &b->slot_c,   &b->hist,  &b->digit_total, &b->tile_start,
-                  &b->image,   &b->final_T,    &b->final_idx, &b->v_image, &b->pix_loss, &b->row_loss, &b->grads,
+                  &b->image,   &b->final_T,    &b->final_idx, &b->v_image, &b->row_loss, &b->loss_scratch, &b->grads,
                   &b->vis_any, &b->table,      &b->scalars};
     for (Buf *x : all)
         if (x->p) (void)hipFree(x->p);
@@ -161,7 +161,7 @@ int prepare(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g, c
         L.pixels = (uint32_t)pix;
         L.tiles += (uint32_t)(c.tiles_x * c.tiles_y);
         L.rows += c.height;
-        L.loss_blocks += ((c.width + 31) / 32) * ((c.height + 15) / 16);
+        L.loss_blocks += loss_blocks(c.width, c.height);
     }
     t.tile_base[nv] = L.tiles;
     uint32_t vt = 1;
@@ -220,8 +220,8 @@ int prepare(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g, c
     CHECKED(grow(ctx, b, b->final_T, 4 * P));
     CHECKED(grow(ctx, b, b->final_idx, 4 * P));
     CHECKED(grow(ctx, b, b->v_image, 12 * P));
-    CHECKED(grow(ctx, b, b->pix_loss, 4 * P));
-    CHECKED(grow(ctx, b, b->row_loss, 8 * ((size_t)b->rows_cap + 1)));
+    CHECKED(grow(ctx, b, b->row_loss, 24 * ((size_t)b->rows_cap + 1)));
+    if (adam) CHECKED(grow(ctx, b, b->loss_scratch, 4 * (size_t)LOSS_PLANES * P));  // training only
     CHECKED(grow(ctx, b, b->table, sizeof(BatchTable)));
     CHECKED(grow(ctx, b, b->scalars, sizeof(BatchScalars)));
     b->n = n;
@@ -459,7 +459,7 @@ void enqueue_train(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussian
     l.max_rows = b->rows_cap;
     l.img = (const float *)b->image.p;
     l.v_img = (float *)b->v_image.p;
-    l.pix_loss = (float *)b->pix_loss.p;
+    l.scratch = (float *)b->loss_scratch.p;
     l.row_loss = (double *)b->row_loss.p;
     l.sc = dsc(b);
     launch_batch_loss(l, s);

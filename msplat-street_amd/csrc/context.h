@@ -81,7 +81,7 @@ struct msplat_hip_ctx {
     // visible gaussians sorted by depth, per-block counts and scan partials
     Buf block_vis, dk0, dk1, dv0, dv1, ord_sums, scan_part, loff;
     // per tile / pixel
-    Buf tile_start, image, final_T, final_idx, v_image, pix_loss, row_loss;
+    Buf tile_start, image, final_T, final_idx, v_image, row_loss, loss_scratch;
     Buf grad_scratch;  // stands in for NULL gradient outputs of render_backward
     Buf aux;           // densify flags/scans/scratch, kNN and metrics workspaces
     msplat::BatchState *batch = nullptr;  // batched-view frame and graph (created on first use)
@@ -173,8 +173,8 @@ struct msplat_hip_ctx {
         if (stream) (void)hipStreamSynchronize(stream);
         Buf *all[] = {&splat_a, &splat_b, &splat_blue, &depths, &aabb, &rect, &radii, &count, &offsets, &block_sums,
                       &v_xy, &v_conic, &v_rgb, &v_opac, &sh_scratch, &big, &keys0, &keys1, &vals0, &vals1, &slot_a, &slot_b, &slot_c,
-                      &hist, &digit_total, &keys64, &block_vis, &dk0, &dk1, &dv0, &dv1, &ord_sums, &scan_part, &loff, &tile_start, &image, &final_T, &final_idx, &v_image, &pix_loss,
-                      &row_loss, &scalars, &grad_scratch, &aux};
+                      &hist, &digit_total, &keys64, &block_vis, &dk0, &dk1, &dv0, &dv1, &ord_sums, &scan_part, &loff, &tile_start, &image, &final_T, &final_idx, &v_image,
+                      &row_loss, &loss_scratch, &scalars, &grad_scratch, &aux};
         for (Buf *b : all) release(*b);
         if (batch) msplat::destroy_batch_state(batch);
         if (h_total) (void)hipHostFree(h_total);

@@ -95,12 +95,21 @@ struct RasterArgs {
     int cull;  // set by the launchers from Tuning::cull
 };
 
+// Loss blocks: 8 rows x 118 columns; LOSS_PLANES floats of scratch per pixel (15
+// horizontal SSIM moments, 9 back-convolved derivative fields, 3 SSIM and 3 L1 terms)
+constexpr int LOSS_BLOCK_ROWS = 8, LOSS_BLOCK_COLS = 118, LOSS_PLANES = 30;
+inline uint32_t loss_blocks(uint32_t W, uint32_t H) {
+    return ((W + LOSS_BLOCK_COLS - 1) / LOSS_BLOCK_COLS) * ((H + LOSS_BLOCK_ROWS - 1) / LOSS_BLOCK_ROWS);
+}
+
 struct LossArgs {
     uint32_t width, height;
     const float *img, *gt;
     float ssim_weight, inv_n;
-    float *v_img, *pix_loss;
-    double *row_loss, *loss_sum;
+    float *v_img;
+    float *scratch;     // LOSS_PLANES planes of W x H floats
+    double *row_loss;   // per row: sum, absolute sum, ulp exponent
+    double *loss_sum;
     float *loss_mean;  // loss_sum / (W H), float
 };
 
@@ -232,8 +241,9 @@ struct BatchLossArgs {
     int num_views;
     uint32_t max_loss_blocks, max_rows;
     const float *img;
-    float *v_img, *pix_loss;
-    double *row_loss;
+    float *v_img;
+    float *scratch;    // LOSS_PLANES floats per pixel, view v's planes at LOSS_PLANES pix_base
+    double *row_loss;  // 3 doubles per row
     BatchScalars *sc;
 };
 

@@ -1,194 +1,335 @@
-// Fused L1 + SSIM loss forward and backward (ssim_h_fwd, ssim_fused_v_fwd_h_bwd and
-// ssim_v_bwd of core/cpu/msplat_cpu.cpp:669-785 in one pass over the image).
-//
-// One 256-thread workgroup per 32x16 output tile. Per channel, the tile's input window
-// (tile + 10 px halo, zero outside the image) is staged in LDS; horizontal 11-tap
-// moments, the vertical pass with SSIM and its derivative fields (f1, f2, f3 on the
-// tile + 5 px), and both back-convolutions are computed from LDS. HBM traffic is the
-// minimum: render and target read once, dL/d(render) and a per-pixel loss written once.
+// L1 + SSIM loss forward and backward (ssim_h_fwd, ssim_fused_v_fwd_h_bwd and
+// ssim_v_bwd of core/cpu/msplat_cpu.cpp:669-785), as separable passes over loss blocks
+// of 8 rows x 118 columns (128 threads: the block's columns plus 5 on each side):
+//   ssim_h     one row of a block per workgroup: horizontal 11-tap moments of every
+//              pixel and channel -> planes 0..14;
+//   ssim_v     one channel of a block per workgroup: vertical pass, SSIM and its
+//              derivative fields (on the block's columns plus the 5-column halo,
+//              exchanged through LDS), their horizontal back-convolution -> planes
+//              15..23, and the channel's SSIM and |target - render| -> planes 24..29;
+//   ssim_vb    one channel of a block per workgroup: vertical back-convolution and
+//              dL/d(render);
+//   rows       the per-pixel loss (channels added in order) and its sum per row;
+//   total      the sum over rows.
+// In ssim_v and ssim_vb a thread owns one column and slides down the block's rows:
+// each input row it loads is added into every output row whose window holds it, so a
+// vertical sum costs one load per input row. The planes (120 B per pixel) pass through
+// L2 and the memory-side cache between the passes.
 // Every sum runs in the reference's tap order with unfused multiply-adds, so the
-// gradient and the per-pixel loss are bit-identical to the CPU restatement; the loss
-// total is accumulated in double per row (x order) and over rows (y order) as there.
+// gradient and the per-pixel loss are bit-identical to the CPU restatement, and the
+// loss total equals the reference's double sum per row (x order) and over rows (y
+// order): see row_sum.
 #include "common.h"
 #include "kernels.h"
 
 namespace msplat {
 
-constexpr int LT_W = 32, LT_H = 16;
-constexpr int IN_W = LT_W + 20, IN_H = LT_H + 20;  // input window
-constexpr int MH_W = LT_W + 10;                     // moments / field columns
-constexpr int F_H = LT_H + 10;                      // field rows
+constexpr int HW = 5;                            // half window
+constexpr int LB_THREADS = LOSS_BLOCK_COLS + 2 * HW;
+constexpr int LB_IN = LOSS_BLOCK_ROWS + 2 * HW;  // input rows of a block's vertical sums
+constexpr int PL_HB = 15, PL_SSIM = 24, PL_L1 = 27;  // first planes of the fields, SSIM, L1
+static_assert(LB_THREADS == 128, "two waves per loss block");
 
 __device__ constexpr float GW[11] = {0.0010283801f, 0.0075987581f, 0.0360007721f, 0.1093606895f, 0.2130055377f,
                                      0.2660117249f, 0.2130055377f, 0.1093606895f, 0.0360007721f, 0.0075987581f,
                                      0.0010283801f};
 
-// The loss of the 32x16 output tile at (X0, Y0); every thread of the workgroup calls it.
-__device__ inline void loss_tile(const LossArgs &a, int X0, int Y0) {
-    __shared__ float in_g[IN_H][IN_W], in_r[IN_H][IN_W];
-    __shared__ float mh[5][IN_H][MH_W];  // reused for the horizontal back-convolution
-    __shared__ float fld[3][F_H][MH_W];
-    __shared__ float acc_s[LT_H * LT_W], acc_l[LT_H * LT_W];
-    float(*hb)[F_H][LT_W] = reinterpret_cast<float(*)[F_H][LT_W]>(&mh[0][0][0]);
-
-    const int W = (int)a.width, H = (int)a.height;
-    const int tid = threadIdx.x;
-    const float C1 = 0.0001f, C2 = 0.0009f;
-    const float w = a.ssim_weight;
-
-    for (int i = tid; i < LT_H * LT_W; i += 256) {
-        acc_s[i] = 0.f;
-        acc_l[i] = 0.f;
+// ssim_h_fwd for row y, columns [X0, X0 + 118): the row's pixels from X0 - 5 to
+// X0 + 122 are staged in LDS (zero outside the image), then thread t adds the 11 taps of
+// column X0 + t per channel. Plane 5c + q holds moment q (mu_x, mu_y, E[x^2], E[y^2],
+// E[xy]; x = target, y = render) of channel c.
+__device__ inline void ssim_h_row(const LossArgs &a, int X0, int y) {
+    __shared__ float sg[3 * LB_THREADS], sr[3 * LB_THREADS];
+    const int W = (int)a.width, H = (int)a.height, t = threadIdx.x;
+    if (y >= H) return;
+    const size_t P = (size_t)W * H;
+    for (int i = t; i < 3 * LB_THREADS; i += LB_THREADS) {
+        const int x = X0 - HW + i / 3;
+        float gv = 0.f, rv = 0.f;
+        if (x >= 0 && x < W) {
+            const size_t idx = ((size_t)y * W + x) * 3 + i % 3;
+            gv = a.gt[idx];
+            rv = a.img[idx];
+        }
+        sg[i] = gv;
+        sr[i] = rv;
     }
+    __syncthreads();
+    const int x = X0 + t;
+    if (t >= LOSS_BLOCK_COLS || x >= W) return;
     for (int c = 0; c < 3; c++) {
-        // input window of channel c
-        for (int i = tid; i < IN_H * IN_W; i += 256) {
-            const int r = i / IN_W, col = i % IN_W;
-            const int y = Y0 - 10 + r, x = X0 - 10 + col;
-            float gv = 0.f, rv = 0.f;
-            if (x >= 0 && x < W && y >= 0 && y < H) {
-                const size_t idx = ((size_t)y * W + x) * 3 + c;
-                gv = a.gt[idx];
-                rv = a.img[idx];
-            }
-            in_g[r][col] = gv;
-            in_r[r][col] = rv;
-        }
-        __syncthreads();
-        // horizontal moments mu_x, mu_y, E[x^2], E[y^2], E[xy] (x = target, y = render)
-        for (int i = tid; i < IN_H * MH_W; i += 256) {
-            const int r = i / MH_W, col = i % MH_W;
-            float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f;
+        float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f;
 #pragma unroll
-            for (int k = 0; k < 11; k++) {
-                const float gv = in_g[r][col + k], rv = in_r[r][col + k], g = GW[k];
-                m0 += g * gv;
-                m1 += g * rv;
-                m2 += g * gv * gv;
-                m3 += g * rv * rv;
-                m4 += g * gv * rv;
-            }
-            mh[0][r][col] = m0; mh[1][r][col] = m1; mh[2][r][col] = m2; mh[3][r][col] = m3; mh[4][r][col] = m4;
+        for (int k = 0; k < 11; k++) {
+            const float gv = sg[3 * (t + k) + c], rv = sr[3 * (t + k) + c], g = GW[k];
+            m0 += g * gv;
+            m1 += g * rv;
+            m2 += g * gv * gv;
+            m3 += g * rv * rv;
+            m4 += g * gv * rv;
         }
-        __syncthreads();
-        // vertical pass, SSIM, derivative fields
-        for (int i = tid; i < F_H * MH_W; i += 256) {
-            const int fr = i / MH_W, col = i % MH_W;
-            const int y = Y0 - 5 + fr, x = X0 - 5 + col;
-            float f1 = 0.f, f2 = 0.f, f3 = 0.f;
-            if (x >= 0 && x < W && y >= 0 && y < H) {
-                float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f;
-#pragma unroll
-                for (int k = 0; k < 11; k++) {
-                    const int sy = y - 5 + k;
-                    if (sy < 0 || sy >= H) continue;
-                    const float g = GW[k];
-                    s0 += g * mh[0][fr + k][col];
-                    s1 += g * mh[1][fr + k][col];
-                    s2 += g * mh[2][fr + k][col];
-                    s3 += g * mh[3][fr + k][col];
-                    s4 += g * mh[4][fr + k][col];
-                }
-                const float mx = s0, my = s1;
-                const float vx = s2 - mx * mx, vy = s3 - my * my, cxy = s4 - mx * my;
-                const float A = 2.0f * mx * my + C1, B = 2.0f * cxy + C2;
-                const float Cd = mx * mx + my * my + C1, D = vx + vy + C2;
-                const float iCD = 1.0f / (Cd * D);
-                const float dmu = 2.0f * B * (mx * Cd - A * my) / (Cd * Cd * D);
-                const float dvy = -A * B * iCD / D, dcv = 2.0f * A * iCD;
-                f1 = dmu - 2.0f * my * dvy - mx * dcv;
-                f2 = 2.0f * dvy;
-                f3 = dcv;
-                const int oy = fr - 5, ox = col - 5;
-                if (oy >= 0 && oy < LT_H && ox >= 0 && ox < LT_W) {
-                    const int o = oy * LT_W + ox;
-                    acc_s[o] += (A * B) / (Cd * D);
-                    acc_l[o] += fabsf(in_g[fr + 5][col + 5] - in_r[fr + 5][col + 5]);
-                }
-            }
-            fld[0][fr][col] = f1; fld[1][fr][col] = f2; fld[2][fr][col] = f3;
-        }
-        __syncthreads();
-        // horizontal back-convolution of the fields
-        for (int i = tid; i < F_H * LT_W; i += 256) {
-            const int fr = i / LT_W, ox = i % LT_W;
-            float h1 = 0.f, h2 = 0.f, h3 = 0.f;
-#pragma unroll
-            for (int d = 0; d < 11; d++) {
-                const float g = GW[10 - d];
-                h1 += g * fld[0][fr][ox + d];
-                h2 += g * fld[1][fr][ox + d];
-                h3 += g * fld[2][fr][ox + d];
-            }
-            hb[0][fr][ox] = h1; hb[1][fr][ox] = h2; hb[2][fr][ox] = h3;
-        }
-        __syncthreads();
-        // vertical back-convolution and the image gradient
-        for (int i = tid; i < LT_H * LT_W; i += 256) {
-            const int oy = i / LT_W, ox = i % LT_W;
-            const int y = Y0 + oy, x = X0 + ox;
-            if (x >= W || y >= H) continue;
-            float a1 = 0.f, a2 = 0.f, a3 = 0.f;
-#pragma unroll
-            for (int d = 0; d < 11; d++) {
-                const int sy = y - 5 + d;
-                if (sy < 0 || sy >= H) continue;
-                const float g = GW[10 - d];
-                a1 += g * hb[0][oy + d][ox];
-                a2 += g * hb[1][oy + d][ox];
-                a3 += g * hb[2][oy + d][ox];
-            }
-            const float rv = in_r[oy + 10][ox + 10], gv = in_g[oy + 10][ox + 10];
-            const float vs = a1 + rv * a2 + gv * a3;
-            const float vl = (gv > rv) ? -1.0f : ((gv < rv) ? 1.0f : 0.0f);
-            a.v_img[((size_t)y * W + x) * 3 + c] = a.inv_n * (-w * vs + (1.0f - w) * vl);
-        }
-        __syncthreads();
-    }
-    for (int i = tid; i < LT_H * LT_W; i += 256) {
-        const int y = Y0 + i / LT_W, x = X0 + i % LT_W;
-        if (x < W && y < H) a.pix_loss[(size_t)y * W + x] = w * (1.0f - acc_s[i] / 3.0f) + (1.0f - w) * acc_l[i] / 3.0f;
+        float *o = a.scratch + (size_t)(5 * c) * P + (size_t)y * W + x;
+        o[0] = m0;
+        o[P] = m1;
+        o[2 * P] = m2;
+        o[3 * P] = m3;
+        o[4 * P] = m4;
     }
 }
 
-__global__ __launch_bounds__(256) void loss_kernel(LossArgs a) { loss_tile(a, blockIdx.x * LT_W, blockIdx.y * LT_H); }
-
-// Per-row double sums of the per-pixel loss in x order: the row is staged in LDS with
-// coalesced loads, then one lane adds it up sequentially (the reference's order).
-constexpr int ROW_MAX = 4096;
-
-__global__ __launch_bounds__(256) void loss_rows_kernel(const float *pix, int W, int H, double *rows) {
-    __shared__ float r[ROW_MAX];
-    const int y = blockIdx.x;
-    const float *p = pix + (size_t)y * W;
-    double s = 0.0;
-    for (int base = 0; base < W; base += ROW_MAX) {
-        const int n = min(ROW_MAX, W - base);
-        __syncthreads();
-        for (int x = threadIdx.x; x < n; x += 256) r[x] = p[base + x];
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (int x = 0; x < n; x++) s += (double)r[x];
+// ssim_fused_v_fwd_h_bwd for channel c of the block: thread t owns field column
+// xf = X0 - 5 + t. It slides down input rows Y0 - 5 .. Y0 + 12 (rows outside the image
+// are skipped, as there), forms SSIM and the fields f1, f2, f3 of its 8 rows (zero
+// outside the image), and threads 5..122 then back-convolve the fields of their columns
+// from LDS. The same threads store their pixels' SSIM and L1 terms of channel c.
+__device__ inline void ssim_v_block(const LossArgs &a, int X0, int Y0, int c) {
+    __shared__ float fld[3][LOSS_BLOCK_ROWS][LB_THREADS];
+    const int W = (int)a.width, H = (int)a.height, t = threadIdx.x;
+    const size_t P = (size_t)W * H;
+    const int xf = X0 - HW + t;
+    const bool fcol = xf >= 0 && xf < W;
+    const bool out = t >= HW && t < HW + LOSS_BLOCK_COLS && xf < W;
+    const float C1 = 0.0001f, C2 = 0.0009f;
+    float sm[LOSS_BLOCK_ROWS][5];
+#pragma unroll
+    for (int r = 0; r < LOSS_BLOCK_ROWS; r++)
+#pragma unroll
+        for (int q = 0; q < 5; q++) sm[r][q] = 0.f;
+    if (fcol) {
+        const float *m = a.scratch + (size_t)(5 * c) * P + xf;
+#pragma unroll
+        for (int j = 0; j < LB_IN; j++) {
+            const int iy = Y0 - HW + j;
+            if (iy < 0 || iy >= H) continue;
+            const float *mr = m + (size_t)iy * W;
+            const float v0 = mr[0], v1 = mr[P], v2 = mr[2 * P], v3 = mr[3 * P], v4 = mr[4 * P];
+#pragma unroll
+            for (int r = 0; r < LOSS_BLOCK_ROWS; r++) {
+                const int k = j - r;  // the tap of input row iy in output row Y0 + r
+                if (k < 0 || k > 10) continue;
+                const float g = GW[k];
+                sm[r][0] += g * v0;
+                sm[r][1] += g * v1;
+                sm[r][2] += g * v2;
+                sm[r][3] += g * v3;
+                sm[r][4] += g * v4;
+            }
+        }
     }
-    if (threadIdx.x == 0) rows[y] = s;
+#pragma unroll
+    for (int r = 0; r < LOSS_BLOCK_ROWS; r++) {
+        const int y = Y0 + r;
+        float f1 = 0.f, f2 = 0.f, f3 = 0.f;
+        if (fcol && y < H) {
+            const float mx = sm[r][0], my = sm[r][1];
+            const float vx = sm[r][2] - mx * mx, vy = sm[r][3] - my * my, cxy = sm[r][4] - mx * my;
+            const float A = 2.0f * mx * my + C1, B = 2.0f * cxy + C2;
+            const float Cd = mx * mx + my * my + C1, D = vx + vy + C2;
+            const float iCD = 1.0f / (Cd * D);
+            const float dmu = 2.0f * B * (mx * Cd - A * my) / (Cd * Cd * D);
+            const float dvy = -A * B * iCD / D, dcv = 2.0f * A * iCD;
+            f1 = dmu - 2.0f * my * dvy - mx * dcv;
+            f2 = 2.0f * dvy;
+            f3 = dcv;
+            if (out) {
+                const size_t p = (size_t)y * W + xf, i = 3 * p + c;
+                a.scratch[(size_t)(PL_SSIM + c) * P + p] = (A * B) / (Cd * D);
+                a.scratch[(size_t)(PL_L1 + c) * P + p] = fabsf(a.gt[i] - a.img[i]);
+            }
+        }
+        fld[0][r][t] = f1;
+        fld[1][r][t] = f2;
+        fld[2][r][t] = f3;
+    }
+    __syncthreads();
+    if (!out) return;
+    float *hb = a.scratch + (size_t)(PL_HB + 3 * c) * P + xf;
+    for (int r = 0; r < LOSS_BLOCK_ROWS; r++) {
+        const int y = Y0 + r;
+        if (y >= H) break;
+        float h1 = 0.f, h2 = 0.f, h3 = 0.f;
+#pragma unroll
+        for (int d = 0; d < 11; d++) {
+            const float g = GW[10 - d];
+            h1 += g * fld[0][r][t - HW + d];
+            h2 += g * fld[1][r][t - HW + d];
+            h3 += g * fld[2][r][t - HW + d];
+        }
+        float *o = hb + (size_t)y * W;
+        o[0] = h1;
+        o[P] = h2;
+        o[2 * P] = h3;
+    }
 }
 
-// Sum of the row sums in y order (staged in LDS, one lane adds)
-__global__ __launch_bounds__(256) void loss_total_kernel(const double *rows, int H, int W, double *sum, float *mean) {
-    __shared__ double r[2048];
-    double s = 0.0;
-    for (int base = 0; base < H; base += 2048) {
-        const int n = min(2048, H - base);
-        __syncthreads();
-        for (int y = threadIdx.x; y < n; y += 256) r[y] = rows[base + y];
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (int y = 0; y < n; y++) s += r[y];
+// ssim_v_bwd for channel c of the block: thread t owns column X0 + t and slides down the
+// back-convolved fields of rows Y0 - 5 .. Y0 + 12 (rows outside the image skipped).
+__device__ inline void ssim_vb_block(const LossArgs &a, int X0, int Y0, int c) {
+    const int W = (int)a.width, H = (int)a.height, t = threadIdx.x;
+    const size_t P = (size_t)W * H;
+    const int x = X0 + t;
+    if (t >= LOSS_BLOCK_COLS || x >= W) return;
+    const float w = a.ssim_weight;
+    float cf[LOSS_BLOCK_ROWS][3];
+#pragma unroll
+    for (int r = 0; r < LOSS_BLOCK_ROWS; r++) cf[r][0] = cf[r][1] = cf[r][2] = 0.f;
+    const float *hb = a.scratch + (size_t)(PL_HB + 3 * c) * P + x;
+#pragma unroll
+    for (int j = 0; j < LB_IN; j++) {
+        const int iy = Y0 - HW + j;
+        if (iy < 0 || iy >= H) continue;
+        const float *hr = hb + (size_t)iy * W;
+        const float d0 = hr[0], d1 = hr[P], d2 = hr[2 * P];
+#pragma unroll
+        for (int r = 0; r < LOSS_BLOCK_ROWS; r++) {
+            const int dy = j - r;  // the tap of row iy in output row Y0 + r
+            if (dy < 0 || dy > 10) continue;
+            const float g = GW[10 - dy];
+            cf[r][0] += g * d0;
+            cf[r][1] += g * d1;
+            cf[r][2] += g * d2;
+        }
     }
-    if (threadIdx.x == 0) {
+    for (int r = 0; r < LOSS_BLOCK_ROWS; r++) {
+        const int y = Y0 + r;
+        if (y >= H) break;
+        const size_t i = ((size_t)y * W + x) * 3 + c;
+        const float rv = a.img[i], gv = a.gt[i];
+        const float vs = cf[r][0] + rv * cf[r][1] + gv * cf[r][2];
+        const float vl = (gv > rv) ? -1.0f : ((gv < rv) ? 1.0f : 0.0f);
+        a.v_img[i] = a.inv_n * (-w * vs + (1.0f - w) * vl);
+    }
+}
+
+__global__ __launch_bounds__(LB_THREADS) void ssim_h_kernel(LossArgs a) {
+    ssim_h_row(a, blockIdx.x * LOSS_BLOCK_COLS, blockIdx.y);
+}
+__global__ __launch_bounds__(LB_THREADS) void ssim_v_kernel(LossArgs a) {
+    ssim_v_block(a, blockIdx.x * LOSS_BLOCK_COLS, blockIdx.y * LOSS_BLOCK_ROWS, blockIdx.z);
+}
+__global__ __launch_bounds__(LB_THREADS) void ssim_vb_kernel(LossArgs a) {
+    ssim_vb_block(a, blockIdx.x * LOSS_BLOCK_COLS, blockIdx.y * LOSS_BLOCK_ROWS, blockIdx.z);
+}
+
+// The reference's per-pixel loss: the channels' SSIM and L1 terms added in channel
+// order from zero, then w (1 - ssim / 3) + (1 - w) l1 / 3 in float.
+__device__ inline float pixel_loss(const float *pl, size_t P, size_t p, float w) {
+    float s = 0.f, l = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        s += pl[(size_t)(PL_SSIM + c) * P + p];
+        l += pl[(size_t)(PL_L1 + c) * P + p];
+    }
+    return w * (1.0f - s / 3.0f) + (1.0f - w) * l / 3.0f;
+}
+
+// Exponent e of the ulp of a float (its value is a multiple of 2^e); zero gives
+// EXACT_NONE, which constrains nothing.
+constexpr int EXACT_NONE = 1 << 20, EXACT_FAIL = -(1 << 20);
+__device__ inline int ulp_exp(float v) {
+    const uint32_t b = __float_as_uint(v) & 0x7FFFFFFFu;
+    if (b == 0) return EXACT_NONE;
+    const int e = (int)(b >> 23);
+    return e == 0 ? -149 : e - 150;
+}
+
+// Whether every partial sum of values that are multiples of 2^emin, with absolute sum
+// `abs_sum` (itself summed with rounding, hence the factor-2 margin), is exactly
+// representable in double: |partial| <= abs_sum < 2^(emin + 52).
+__device__ inline bool sums_exact(int emin, double abs_sum) {
+    if (emin == EXACT_NONE) return true;
+    if (emin <= EXACT_FAIL / 2 || emin + 51 > 1000) return false;
+    return abs_sum <= ldexp(1.0, emin + 51);
+}
+
+__device__ inline double wave_sum_d(double v) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    return v;
+}
+__device__ inline int wave_min_i(int v) {
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d));
+    return v;
+}
+
+// Row y's loss sum as the reference forms it: a double accumulated over the row's
+// pixel losses in x order. When every partial sum is exactly representable
+// (sums_exact), each of the reference's additions is exact, so its result is the exact
+// sum, which any order of exact additions also gives: the 64 lanes add strided values
+// and the wave adds the lane sums. Otherwise lane 0 adds the values in order. Writes
+// the sum, the absolute sum and the ulp exponent (EXACT_FAIL when the row was not
+// exact) for the total. One wave per row.
+__device__ inline void row_sum(const LossArgs &a, int y, double *out) {
+    __shared__ float buf[4096];
+    const int W = (int)a.width, lane = threadIdx.x;
+    const size_t P = (size_t)W * a.height, base = (size_t)y * W;
+    double s = 0.0, as = 0.0;
+    int emin = EXACT_NONE;
+    for (int x = lane; x < W; x += 64) {
+        const float v = pixel_loss(a.scratch, P, base + x, a.ssim_weight);
+        s += (double)v;
+        as += fabs((double)v);
+        emin = min(emin, ulp_exp(v));
+        if (x < 4096) buf[x] = v;
+    }
+    s = wave_sum_d(s);
+    as = wave_sum_d(as);
+    emin = wave_min_i(emin);
+    if (!sums_exact(emin, as)) {  // uniform over the wave: lane 0 adds in x order
+        __syncthreads();             // the other lanes' values are in buf
+        s = 0.0;
+        if (lane == 0)
+            for (int x = 0; x < W; x++)
+                s += (double)(x < 4096 ? buf[x] : pixel_loss(a.scratch, P, base + x, a.ssim_weight));
+        emin = EXACT_FAIL;
+    }
+    if (lane == 0) {
+        out[3 * y] = s;
+        out[3 * y + 1] = as;
+        out[3 * y + 2] = (double)emin;
+    }
+}
+
+__global__ __launch_bounds__(64) void loss_rows_kernel(LossArgs a) { row_sum(a, blockIdx.x, a.row_loss); }
+
+// The sum of the row sums as the reference forms it (a double accumulated in y order):
+// exact, and so computed as a tree, when every row was exact and the rows together
+// pass sums_exact; otherwise one lane adds the row sums in order.
+__device__ inline void total_sum(const double *rows, int H, size_t npix, double *sum, float *mean) {
+    __shared__ double ws[4], was[4];
+    __shared__ int we[4];
+    __shared__ int exact;
+    const int t = threadIdx.x;
+    double s = 0.0, as = 0.0;
+    int emin = EXACT_NONE;
+    for (int y = t; y < H; y += 256) {
+        s += rows[3 * y];
+        as += rows[3 * y + 1];
+        emin = min(emin, (int)rows[3 * y + 2]);
+    }
+    s = wave_sum_d(s);
+    as = wave_sum_d(as);
+    emin = wave_min_i(emin);
+    if ((t & 63) == 0) {
+        ws[t >> 6] = s;
+        was[t >> 6] = as;
+        we[t >> 6] = emin;
+    }
+    __syncthreads();
+    if (t == 0) {
+        s = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+        as = ((was[0] + was[1]) + was[2]) + was[3];
+        emin = min(min(we[0], we[1]), min(we[2], we[3]));
+        if (!sums_exact(emin, as)) {
+            s = 0.0;
+            for (int y = 0; y < H; y++) s += rows[3 * y];
+        }
         *sum = s;
-        *mean = (float)(s / (double)((size_t)W * H));
+        *mean = (float)(s / (double)npix);
     }
+}
+
+__global__ __launch_bounds__(256) void loss_total_kernel(const double *rows, int H, int W, double *sum, float *mean) {
+    total_sum(rows, H, (size_t)W * H, sum, mean);
 }
 
 // ── batched views: one launch per stage over every view's tiles / rows ──────────
@@ -214,78 +355,76 @@ __device__ inline LossArgs view_loss_args(const BatchLossArgs &b, int v) {
     a.ssim_weight = b.tab->ssim_weight;
     a.inv_n = bv.inv_n;
     a.v_img = b.v_img + 3 * pb;
-    a.pix_loss = b.pix_loss + pb;
-    a.row_loss = nullptr;
+    a.scratch = b.scratch + (size_t)LOSS_PLANES * pb;
+    a.row_loss = b.row_loss + 3 * (size_t)b.tab->row_base[v];
     a.loss_sum = nullptr;
     a.loss_mean = nullptr;
     return a;
 }
 
-// grid = the capacity of loss tiles; workgroups past the batch's count exit
-__global__ __launch_bounds__(256) void loss_batch_kernel(BatchLossArgs b) {
+// Loss block i of the batch: its view and the block's corner. Grids cover the capacity
+// of loss blocks; blocks past the batch's count exit.
+__device__ inline bool batch_loss_block(const BatchLossArgs &b, LossArgs &a, int &X0, int &Y0) {
     const uint32_t i = blockIdx.x;
-    if (i >= b.tab->total_loss_blocks) return;
+    if (i >= b.tab->total_loss_blocks) return false;
     const int v = view_of_base(b.tab->loss_base, b.num_views, i);
-    const LossArgs a = view_loss_args(b, v);
+    a = view_loss_args(b, v);
     const uint32_t local = i - b.tab->loss_base[v];
-    const uint32_t nbx = (a.width + LT_W - 1) / LT_W;
-    loss_tile(a, (int)(local % nbx) * LT_W, (int)(local / nbx) * LT_H);
+    const uint32_t nbx = (a.width + LOSS_BLOCK_COLS - 1) / LOSS_BLOCK_COLS;
+    X0 = (int)(local % nbx) * LOSS_BLOCK_COLS;
+    Y0 = (int)(local / nbx) * LOSS_BLOCK_ROWS;
+    return true;
 }
 
-__global__ __launch_bounds__(256) void loss_rows_batch_kernel(BatchLossArgs b) {
+__global__ __launch_bounds__(LB_THREADS) void ssim_h_batch_kernel(BatchLossArgs b) {
+    LossArgs a;
+    int X0, Y0;
+    if (batch_loss_block(b, a, X0, Y0)) ssim_h_row(a, X0, Y0 + (int)blockIdx.y);
+}
+__global__ __launch_bounds__(LB_THREADS) void ssim_v_batch_kernel(BatchLossArgs b) {
+    LossArgs a;
+    int X0, Y0;
+    if (batch_loss_block(b, a, X0, Y0)) ssim_v_block(a, X0, Y0, blockIdx.y);
+}
+__global__ __launch_bounds__(LB_THREADS) void ssim_vb_batch_kernel(BatchLossArgs b) {
+    LossArgs a;
+    int X0, Y0;
+    if (batch_loss_block(b, a, X0, Y0)) ssim_vb_block(a, X0, Y0, blockIdx.y);
+}
+
+__global__ __launch_bounds__(64) void loss_rows_batch_kernel(BatchLossArgs b) {
     const uint32_t q = blockIdx.x;
     if (q >= b.tab->total_rows) return;
     const int v = view_of_base(b.tab->row_base, b.num_views, q);
-    const BatchView &bv = b.tab->views[v];
-    const int W = (int)bv.cam.width;
-    const int y = (int)(q - b.tab->row_base[v]);
-    __shared__ float r[ROW_MAX];
-    const float *p = b.pix_loss + bv.pix_base + (size_t)y * W;
-    double s = 0.0;
-    for (int base = 0; base < W; base += ROW_MAX) {
-        const int n = min(ROW_MAX, W - base);
-        __syncthreads();
-        for (int x = threadIdx.x; x < n; x += 256) r[x] = p[base + x];
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (int x = 0; x < n; x++) s += (double)r[x];
-    }
-    if (threadIdx.x == 0) b.row_loss[q] = s;
+    const LossArgs a = view_loss_args(b, v);
+    row_sum(a, (int)(q - b.tab->row_base[v]), a.row_loss);
 }
 
-// one workgroup per view: the sum of its rows in y order, and the mean
+// one workgroup per view: the sum of its rows, and the mean
 __global__ __launch_bounds__(256) void loss_total_batch_kernel(BatchLossArgs b) {
     const int v = blockIdx.x;
     const BatchView &bv = b.tab->views[v];
-    const int H = (int)bv.cam.height, W = (int)bv.cam.width;
-    const double *rows = b.row_loss + b.tab->row_base[v];
-    __shared__ double r[2048];
-    double s = 0.0;
-    for (int base = 0; base < H; base += 2048) {
-        const int n = min(2048, H - base);
-        __syncthreads();
-        for (int y = threadIdx.x; y < n; y += 256) r[y] = rows[base + y];
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (int y = 0; y < n; y++) s += r[y];
-    }
-    if (threadIdx.x == 0) {
-        b.sc->loss_sum[v] = s;
-        b.sc->loss_mean[v] = (float)(s / (double)((size_t)W * H));
-    }
+    total_sum(b.row_loss + 3 * (size_t)b.tab->row_base[v], (int)bv.cam.height,
+              (size_t)bv.cam.width * bv.cam.height, &b.sc->loss_sum[v], &b.sc->loss_mean[v]);
 }
 
 void launch_batch_loss(const BatchLossArgs &b, hipStream_t s) {
-    if (b.num_views <= 0) return;
-    loss_batch_kernel<<<b.max_loss_blocks, 256, 0, s>>>(b);
-    loss_rows_batch_kernel<<<b.max_rows, 256, 0, s>>>(b);
+    if (b.num_views <= 0 || b.max_loss_blocks == 0) return;
+    ssim_h_batch_kernel<<<dim3(b.max_loss_blocks, LOSS_BLOCK_ROWS), LB_THREADS, 0, s>>>(b);
+    ssim_v_batch_kernel<<<dim3(b.max_loss_blocks, 3), LB_THREADS, 0, s>>>(b);
+    ssim_vb_batch_kernel<<<dim3(b.max_loss_blocks, 3), LB_THREADS, 0, s>>>(b);
+    loss_rows_batch_kernel<<<b.max_rows, 64, 0, s>>>(b);
     loss_total_batch_kernel<<<b.num_views, 256, 0, s>>>(b);
 }
 
 void launch_loss(const LossArgs &a, hipStream_t s) {
-    dim3 grid((a.width + LT_W - 1) / LT_W, (a.height + LT_H - 1) / LT_H);
-    loss_kernel<<<grid, 256, 0, s>>>(a);
-    loss_rows_kernel<<<a.height, 256, 0, s>>>(a.pix_loss, a.width, a.height, a.row_loss);
+    if (a.width == 0 || a.height == 0) return;
+    const uint32_t nbx = (a.width + LOSS_BLOCK_COLS - 1) / LOSS_BLOCK_COLS;
+    const uint32_t nby = (a.height + LOSS_BLOCK_ROWS - 1) / LOSS_BLOCK_ROWS;
+    ssim_h_kernel<<<dim3(nbx, a.height), LB_THREADS, 0, s>>>(a);
+    ssim_v_kernel<<<dim3(nbx, nby, 3), LB_THREADS, 0, s>>>(a);
+    ssim_vb_kernel<<<dim3(nbx, nby, 3), LB_THREADS, 0, s>>>(a);
+    loss_rows_kernel<<<a.height, 64, 0, s>>>(a);
     loss_total_kernel<<<1, 256, 0, s>>>(a.row_loss, a.height, a.width, a.loss_sum, a.loss_mean);
 }
 

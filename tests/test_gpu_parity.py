@@ -137,27 +137,40 @@ def test_render_matches_oracle(ctx, oracle, deg, W, H):
     assert np.mean(fidx == fwd["final_idx"]) > 0.999
 
 
-def test_loss_bit_exact(ctx, oracle):
+# sizes around the loss kernels' 118-column x 8-row blocks and their 5-pixel halos
+LOSS_SIZES = [(75, 53), (250, 37), (118, 16), (119, 9), (236, 8), (1, 1), (3, 300), (400, 5)]
+
+
+@pytest.mark.parametrize("W,H", LOSS_SIZES)
+def test_loss_bit_exact(ctx, oracle, W, H):
     from msplat._abi import DeviceArray, check, lib, make_camera_struct
     from msplat.backend import DeviceGaussians, render_numpy
-    sc, cam = _scene_cam(seed=3, n=2000, deg=1, W=75, H=53)
+    sc, cam = _scene_cam(seed=3, n=2000, deg=1, W=W, H=H)
     dg = DeviceGaussians(ctx, sc)
     render_numpy(ctx, dg, cam, BG)
-    rng = np.random.default_rng(4)
-    img = rng.random((53, 75, 3), dtype=np.float32)
+    rng = np.random.default_rng(4 + W * H)
+    img = rng.random((H, W, 3), dtype=np.float32)
     gt = np.clip(img + rng.normal(0, 0.1, img.shape), 0, 1).astype(np.float32)
+    gt[::7, ::5] = img[::7, ::5]  # equal pixels: the L1 subgradient's zero branch
+    # a target equal to the render on the left half: pixel losses of a few ulp there,
+    # whose row sums the reference rounds (the kernels' in-order path)
+    gt_half = gt.copy()
+    gt_half[:, : (W + 1) // 2] = img[:, : (W + 1) // 2]
     f = ctx.frame()
     ctx.upload(f.image, img)
-    dgt = DeviceArray.from_numpy(ctx, gt)
     c = make_camera_struct(cam, 1)
-    for w in (0.2, 0.0, 1.0):
-        loss = C.c_float()
-        inv_n = float(np.float32(1) / np.float32(img.size))
-        check(lib().msplat_hip_stage_loss(ctx.ptr, C.byref(c), C.c_void_p(dgt.ptr), w, inv_n, C.byref(loss)))
-        v = ctx.download(ctx.frame().v_image, img.shape, np.float32)
-        tot, vref = oracle.loss(img, gt, w)
-        np.testing.assert_array_equal(v, vref)
-        assert np.float32(loss.value) == np.float32(tot / (53 * 75))
+    for target in (gt, gt_half):
+        dgt = DeviceArray.from_numpy(ctx, target)
+        for w in (0.2, 0.0, 1.0):
+            loss = C.c_float()
+            inv_n = float(np.float32(1) / np.float32(img.size))
+            check(lib().msplat_hip_stage_loss(ctx.ptr, C.byref(c), C.c_void_p(dgt.ptr), w, inv_n, C.byref(loss)))
+            v = ctx.download(ctx.frame().v_image, img.shape, np.float32)
+            tot, vref = oracle.loss(img, target, w)
+            np.testing.assert_array_equal(v, vref, err_msg=f"{W}x{H} w {w}")
+            dsum = ctx.download(ctx.frame().loss_sum, (1,), np.float64)[0]
+            assert dsum == tot, (W, H, w, dsum, tot)  # the double sum, bit for bit
+            assert np.float32(loss.value) == np.float32(tot / (H * W)), (W, H, w, loss.value, tot / (H * W))
 
 
 def _oracle_grads(oracle, sc, cam, v_img):
