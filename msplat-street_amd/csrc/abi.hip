@@ -83,36 +83,18 @@ int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_
 
 // Count scans and their read-back (M intersections, V visible gaussians), then the
 // visible gaussians sorted by depth, keys emitted in that order, the stable tile sort
-// and the tile ranges.
+// and the tile ranges. The host waits for the counts only after the depth sort and
+// the offsets are enqueued (those run on device-side counts against the gaussian
+// capacity), so the device sorts while the host reads M and sizes the key buffers.
 int do_bin(msplat_hip_ctx *ctx) {
     const int n = ctx->n;
     const uint32_t nblocks = (uint32_t)((n + 255) / 256);
     if (!ctx->h_total) HIPCHECK(hipHostMalloc((void **)&ctx->h_total, 128, hipHostMallocDefault));
-    CHECKED(ctx->grow(ctx->scan_part, 4 * 3 * scan_partial_words(nblocks > 0 ? nblocks : 1) + 64));
-    uint32_t *part = (uint32_t *)ctx->scan_part.p;
-    const size_t pw = scan_partial_words(nblocks > 0 ? nblocks : 1);
-    launch_scan_u32((uint32_t *)ctx->block_sums.p, nblocks, part,
-                    ScanOut{ctx->d_total64(), ctx->d_total32(), 0xFFFFFFFFu, nullptr, ctx->d_big_count()}, ctx->stream);
-    launch_scan_u32((uint32_t *)ctx->block_vis.p, nblocks, part + pw,
-                    ScanOut{ctx->d_vis64(), ctx->d_vis32(), 0xFFFFFFFFu, nullptr, nullptr}, ctx->stream);
-    ctx->end();
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(ctx->h_total, ctx->scalars.p, 64, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHECK(hipStreamSynchronize(ctx->stream));
-    ctx->collect();
-    const uint64_t M = ctx->h_total[0];
-    const uint32_t V = (uint32_t)ctx->h_total[5];
-    if (M > 0x7FFFFFFFull)
-        return set_error(MSPLAT_ERR_RANGE, "more than 2^31 tile intersections (%llu); lower the resolution",
-                         (unsigned long long)M);
-    ctx->M = (int64_t)M;
-    const size_t Mc = M > 0 ? M : 1, Nc = n > 0 ? (size_t)n : 1;
-    const uint32_t vblocks = (V + 255) / 256;
-    CHECKED(ctx->grow(ctx->keys0, 4 * Mc));
-    CHECKED(ctx->grow(ctx->keys1, 4 * Mc));
-    CHECKED(ctx->grow(ctx->vals0, 4 * Mc));
-    CHECKED(ctx->grow(ctx->vals1, 4 * Mc));
-    CHECKED(ctx->grow(ctx->hist, 4 * radix_hist_words((uint32_t)std::max(Mc, Nc))));
+    if (!ctx->counts_ready) HIPCHECK(hipEventCreateWithFlags(&ctx->counts_ready, hipEventDisableTiming));
+    const size_t Nc = n > 0 ? (size_t)n : 1;
+    const uint32_t vblocks = (uint32_t)((Nc + 255) / 256);  // depth-ordered list blocks, at most
+    CHECKED(ctx->grow(ctx->scan_part, 4 * 3 * scan_partial_words(std::max(nblocks, vblocks)) + 64));
+    CHECKED(ctx->grow(ctx->hist, 4 * radix_hist_words((uint32_t)std::max<size_t>(Nc, ctx->M > 0 ? ctx->M : 1))));
     CHECKED(ctx->grow(ctx->digit_total, 4 * 256));
     CHECKED(ctx->grow(ctx->dk0, 4 * Nc));
     CHECKED(ctx->grow(ctx->dk1, 4 * Nc));
@@ -121,7 +103,16 @@ int do_bin(msplat_hip_ctx *ctx) {
     CHECKED(ctx->grow(ctx->ord_sums, 4 * (size_t)(vblocks + 1)));
     CHECKED(ctx->grow(ctx->loff, 4 * Nc));
     CHECKED(ctx->grow(ctx->tile_start, 4 * (size_t)(ctx->ntiles + 1)));
+    uint32_t *part = (uint32_t *)ctx->scan_part.p;
+    const size_t pw = scan_partial_words(std::max(nblocks, vblocks));
     hipStream_t s = ctx->stream;
+    launch_scan_u32((uint32_t *)ctx->block_sums.p, nblocks, part,
+                    ScanOut{ctx->d_total64(), ctx->d_total32(), 0xFFFFFFFFu, nullptr, ctx->d_big_count()}, s);
+    launch_scan_u32((uint32_t *)ctx->block_vis.p, nblocks, part + pw,
+                    ScanOut{ctx->d_vis64(), ctx->d_vis32(), 0xFFFFFFFFu, nullptr, nullptr}, s);
+    ctx->end();
+    HIPCHECK(hipMemcpyAsync(ctx->h_total, ctx->scalars.p, 64, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipEventRecord(ctx->counts_ready, s));
     ctx->begin(1);
     // visible gaussians in id order, stably sorted by depth bits
     CompactArgs c;
@@ -137,33 +128,50 @@ int do_bin(msplat_hip_ctx *ctx) {
     uint32_t *dki = (uint32_t *)ctx->dk0.p, *dko = (uint32_t *)ctx->dk1.p;
     int32_t *dvi = (int32_t *)ctx->dv0.p, *dvo = (int32_t *)ctx->dv1.p;
     for (int shift = 0; shift < 32; shift += 8) {
-        launch_radix_pass(dki, dvi, dko, dvo, ctx->d_vis32(), V, shift, (uint32_t *)ctx->hist.p,
+        launch_radix_pass(dki, dvi, dko, dvo, ctx->d_vis32(), (uint32_t)Nc, shift, (uint32_t *)ctx->hist.p,
                           (uint32_t *)ctx->digit_total.p, s);
         std::swap(dki, dko);
         std::swap(dvi, dvo);
     }
-    // intersection offsets in depth order, keys (tile ids), stable tile sort, ranges
-    launch_order_sums(dvi, ctx->d_vis32(), V, (const uint32_t *)ctx->count.p, (uint32_t *)ctx->ord_sums.p, s);
+    // intersection offsets in depth order
+    launch_order_sums(dvi, ctx->d_vis32(), (uint32_t)Nc, (const uint32_t *)ctx->count.p, (uint32_t *)ctx->ord_sums.p,
+                      s);
     launch_scan_u32((uint32_t *)ctx->ord_sums.p, vblocks, part + 2 * pw,
                     ScanOut{ctx->d_order_total64(), ctx->d_order_total32(), 0xFFFFFFFFu, nullptr, nullptr}, s);
     KeygenArgs k;
     memset(&k, 0, sizeof k);
     k.list = dvi;
     k.nvis_ptr = ctx->d_vis32();
-    k.nvis_max = V;
+    k.nvis_max = (uint32_t)Nc;
     k.count = (const uint32_t *)ctx->count.p;
     k.rect = (const uint2 *)ctx->rect.p;
     k.block_bases = (const uint32_t *)ctx->ord_sums.p;
     k.offsets = (uint32_t *)ctx->offsets.p;
     k.loff = (uint32_t *)ctx->loff.p;
     k.m_ptr = ctx->d_total32();
-    k.m_max = (uint32_t)M;
-    k.keys = (uint32_t *)ctx->keys0.p;
-    k.vals = (int32_t *)ctx->vals0.p;
     k.big_list = ctx->big_list();
     k.big_count = ctx->d_big_count();
     k.tiles_x = ctx->tiles_x;
-    launch_keygen(k, false, s);
+    launch_keygen_offsets(k, s);
+    HIPCHECK(hipGetLastError());
+    // the counts: size the key buffers, then keys (tile ids), stable tile sort, ranges
+    HIPCHECK(hipEventSynchronize(ctx->counts_ready));
+    ctx->collect();
+    const uint64_t M = ctx->h_total[0];
+    if (M > 0x7FFFFFFFull)
+        return set_error(MSPLAT_ERR_RANGE, "more than 2^31 tile intersections (%llu); lower the resolution",
+                         (unsigned long long)M);
+    ctx->M = (int64_t)M;
+    const size_t Mc = M > 0 ? M : 1;
+    CHECKED(ctx->grow(ctx->keys0, 4 * Mc));
+    CHECKED(ctx->grow(ctx->keys1, 4 * Mc));
+    CHECKED(ctx->grow(ctx->vals0, 4 * Mc));
+    CHECKED(ctx->grow(ctx->vals1, 4 * Mc));
+    CHECKED(ctx->grow(ctx->hist, 4 * radix_hist_words((uint32_t)std::max(Mc, Nc))));
+    k.m_max = (uint32_t)M;
+    k.keys = (uint32_t *)ctx->keys0.p;
+    k.vals = (int32_t *)ctx->vals0.p;
+    launch_keygen_emit(k, false, s);
     uint32_t *kin = (uint32_t *)ctx->keys0.p, *kout = (uint32_t *)ctx->keys1.p;
     int32_t *vin = (int32_t *)ctx->vals0.p, *vout = (int32_t *)ctx->vals1.p;
     const int bits = tile_bits(ctx->ntiles);

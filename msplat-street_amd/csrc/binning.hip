@@ -270,26 +270,28 @@ __global__ __launch_bounds__(256) void compact_kernel(CompactArgs a) {
 }
 
 // ── stable LSD radix sort, 8-bit digits, 32- or 64-bit keys, int32 payload ─────
-// Each workgroup owns RS_ITEMS consecutive elements: an upsweep histogram, a per-digit
-// scan over workgroups, and a scatter that ranks the elements stably within the
-// workgroup (wave multi-split over ballots of the digit bits), stages them in LDS in
-// digit order, and writes each digit's run to consecutive addresses.
+// Each workgroup owns an element block of ITEMS consecutive elements (4096 for large
+// sorts, 1024 for small ones, so that they still spread over the device): an upsweep
+// histogram, a per-digit scan over blocks, and a scatter that ranks the elements stably
+// within the block (wave multi-split over ballots of the digit bits), stages them in
+// LDS in digit order, and writes each digit's run to consecutive addresses.
 
-// The workgroups of a pass loop over the element blocks (RS_ITEMS each) that the
-// device-side count needs, so a grid sized for a capacity does no empty work;
-// histogram column b of digit d lives at hist[d * stride + b].
-__device__ inline uint32_t rs_blocks(const uint32_t *n_ptr) { return (*n_ptr + RS_ITEMS - 1) / RS_ITEMS; }
+// The workgroups of a pass loop over the element blocks that the device-side count
+// needs, so a grid sized for a capacity does no empty work; histogram column b of
+// digit d lives at hist[d * stride + b].
+template <int ITEMS>
+__device__ inline uint32_t rs_blocks(const uint32_t *n_ptr) { return (*n_ptr + ITEMS - 1) / ITEMS; }
 
-template <typename K>
+template <typename K, int ITEMS>
 __global__ __launch_bounds__(256) void rs_hist_kernel(const K *keys, const uint32_t *n_ptr, int shift, uint32_t *hist,
                                                      int stride) {
     __shared__ uint32_t h[256];
-    const uint32_t n = *n_ptr, nb = rs_blocks(n_ptr);
+    const uint32_t n = *n_ptr, nb = rs_blocks<ITEMS>(n_ptr);
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
         h[threadIdx.x] = 0;
         __syncthreads();
-        const uint32_t start = b * RS_ITEMS;
-        for (int r = 0; r < RS_ITEMS / 256; r++) {
+        const uint32_t start = b * ITEMS;
+        for (int r = 0; r < ITEMS / 256; r++) {
             uint32_t e = start + r * 256 + threadIdx.x;
             if (e < n) atomicAdd(&h[(uint32_t)(keys[e] >> shift) & 0xFF], 1u);
         }
@@ -300,11 +302,12 @@ __global__ __launch_bounds__(256) void rs_hist_kernel(const K *keys, const uint3
 }
 
 // one block per digit: exclusive scan of that digit's counts over blocks, digit total
+template <int ITEMS>
 __global__ __launch_bounds__(256) void rs_scan_kernel(uint32_t *hist, int stride, const uint32_t *n_ptr,
                                                      uint32_t *digit_total) {
     __shared__ uint32_t lds4[4];
     __shared__ uint32_t carry;
-    const int nblocks = (int)rs_blocks(n_ptr);
+    const int nblocks = (int)rs_blocks<ITEMS>(n_ptr);
     uint32_t *row = hist + (size_t)blockIdx.x * stride;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
@@ -332,12 +335,12 @@ __device__ inline void wave_sync() {
 // with private per-digit counters (no workgroup barrier inside the loop); the waves'
 // counts are then combined in wave order, which keeps the sort stable.
 // One element block of a pass; `my_base` is this thread's digit's global start for the block.
-template <typename K>
+template <typename K, int ITEMS>
 __device__ inline void rs_scatter_block(const K *kin, const int32_t *vin, K *kout, int32_t *vout, uint32_t n,
                                         int shift, uint32_t start, uint32_t my_base) {
-    constexpr int R = RS_ITEMS / 256;  // rounds of 64 elements per wave
-    __shared__ K sk[RS_ITEMS];
-    __shared__ int32_t sv[RS_ITEMS];
+    constexpr int R = ITEMS / 256;  // rounds of 64 elements per wave
+    __shared__ K sk[ITEMS];
+    __shared__ int32_t sv[ITEMS];
     __shared__ uint32_t gbase[256], lstart[256];
     __shared__ uint32_t wcnt[4][256];
     __shared__ uint32_t lds4[4];
@@ -396,7 +399,7 @@ __device__ inline void rs_scatter_block(const K *kin, const int32_t *vin, K *kou
         }
     }
     __syncthreads();
-    const uint32_t cnt = min((uint32_t)RS_ITEMS, n - start);
+    const uint32_t cnt = min((uint32_t)ITEMS, n - start);
     for (uint32_t p = tid; p < cnt; p += 256) {
         const K k = sk[p];
         const uint32_t d = (uint32_t)(k >> shift) & 0xFF;
@@ -407,7 +410,7 @@ __device__ inline void rs_scatter_block(const K *kin, const int32_t *vin, K *kou
 }
 
 
-template <typename K>
+template <typename K, int ITEMS>
 __global__ __launch_bounds__(256) void rs_scatter_kernel(const K *kin, const int32_t *vin, K *kout, int32_t *vout,
                                                         const uint32_t *n_ptr, int shift, const uint32_t *hist,
                                                         int stride, const uint32_t *digit_total) {
@@ -417,11 +420,11 @@ __global__ __launch_bounds__(256) void rs_scatter_kernel(const K *kin, const int
         uint32_t t;
         dex[threadIdx.x] = block_scan_u32(digit_total[threadIdx.x], t, lds4);
     }
-    const uint32_t nb = rs_blocks(n_ptr);
+    const uint32_t nb = rs_blocks<ITEMS>(n_ptr);
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
         __syncthreads();
-        rs_scatter_block(kin, vin, kout, vout, *n_ptr, shift, b * RS_ITEMS,
-                         dex[threadIdx.x] + hist[(size_t)threadIdx.x * stride + b]);
+        rs_scatter_block<K, ITEMS>(kin, vin, kout, vout, *n_ptr, shift, b * ITEMS,
+                                   dex[threadIdx.x] + hist[(size_t)threadIdx.x * stride + b]);
     }
 }
 
@@ -684,21 +687,35 @@ void launch_compact(const CompactArgs &a, hipStream_t s) {
     if (a.nbx > 0 && a.num_views > 0) compact_kernel<<<dim3(a.nbx, a.num_views), 256, 0, s>>>(a);
 }
 
-size_t radix_hist_words(uint32_t n_max) { return (size_t)256 * ((n_max + RS_ITEMS - 1) / RS_ITEMS + 1); }
+// Element blocks of a pass: RS_ITEMS elements from RS_LARGE elements up, RS_ITEMS_SMALL
+// below (the histogram words hold the smaller blocks' count either way).
+constexpr int RS_ITEMS_SMALL = 1024;
+constexpr uint32_t RS_LARGE = 2u << 20;
+
+size_t radix_hist_words(uint32_t n_max) {
+    return (size_t)256 * ((n_max + RS_ITEMS_SMALL - 1) / RS_ITEMS_SMALL + 1);
+}
 
 // Grid caps of the capacity-sized launches: the workgroups loop over what the device
 // count needs.
 constexpr uint32_t HIST_GRID = 2048, SCATTER_GRID = 1024, LIST_GRID = 2048;
 
+template <typename K, int ITEMS>
+static void radix_pass_items(const K *kin, const int32_t *vin, K *kout, int32_t *vout, const uint32_t *n_dev,
+                             uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s) {
+    const uint32_t nblocks = (n_max + ITEMS - 1) / ITEMS;
+    if (nblocks == 0) return;
+    rs_hist_kernel<K, ITEMS><<<std::min(nblocks, HIST_GRID), 256, 0, s>>>(kin, n_dev, shift, hist, (int)nblocks);
+    rs_scan_kernel<ITEMS><<<256, 256, 0, s>>>(hist, (int)nblocks, n_dev, digit_total);
+    rs_scatter_kernel<K, ITEMS><<<std::min(nblocks, SCATTER_GRID), 256, 0, s>>>(kin, vin, kout, vout, n_dev, shift,
+                                                                                hist, (int)nblocks, digit_total);
+}
+
 template <typename K>
 static void radix_pass(const K *kin, const int32_t *vin, K *kout, int32_t *vout, const uint32_t *n_dev,
                        uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s) {
-    const uint32_t nblocks = (n_max + RS_ITEMS - 1) / RS_ITEMS;
-    if (nblocks == 0) return;
-    rs_hist_kernel<K><<<std::min(nblocks, HIST_GRID), 256, 0, s>>>(kin, n_dev, shift, hist, (int)nblocks);
-    rs_scan_kernel<<<256, 256, 0, s>>>(hist, (int)nblocks, n_dev, digit_total);
-    rs_scatter_kernel<K><<<std::min(nblocks, SCATTER_GRID), 256, 0, s>>>(kin, vin, kout, vout, n_dev, shift, hist,
-                                                                         (int)nblocks, digit_total);
+    if (n_max >= RS_LARGE) radix_pass_items<K, RS_ITEMS>(kin, vin, kout, vout, n_dev, n_max, shift, hist, digit_total, s);
+    else radix_pass_items<K, RS_ITEMS_SMALL>(kin, vin, kout, vout, n_dev, n_max, shift, hist, digit_total, s);
 }
 
 void launch_radix_pass(const uint64_t *kin, const int32_t *vin, uint64_t *kout, int32_t *vout, const uint32_t *n_dev,
@@ -717,14 +734,21 @@ void launch_order_sums(const int32_t *list, const uint32_t *nvis_dev, uint32_t n
     if (nb > 0) order_sums_kernel<<<std::min(nb, LIST_GRID), 256, 0, s>>>(list, nvis_dev, count, sums, nb);
 }
 
-void launch_keygen(const KeygenArgs &a, bool batch, hipStream_t s) {
+void launch_keygen_offsets(const KeygenArgs &a, hipStream_t s) {
     const uint32_t nb = (a.nvis_max + 255) / 256;
-    if (nb == 0) return;
-    keygen_offsets_kernel<<<std::min(nb, LIST_GRID), 256, 0, s>>>(a);
+    if (nb > 0) keygen_offsets_kernel<<<std::min(nb, LIST_GRID), 256, 0, s>>>(a);
+}
+
+void launch_keygen_emit(const KeygenArgs &a, bool batch, hipStream_t s) {
     const uint32_t ne = (a.m_max + EMIT_CHUNK - 1) / EMIT_CHUNK;
-    if (ne == 0) return;
+    if (a.nvis_max == 0 || ne == 0) return;
     if (batch) keygen_emit_kernel<true><<<ne, 256, 0, s>>>(a);
     else keygen_emit_kernel<false><<<ne, 256, 0, s>>>(a);
+}
+
+void launch_keygen(const KeygenArgs &a, bool batch, hipStream_t s) {
+    launch_keygen_offsets(a, s);
+    launch_keygen_emit(a, batch, s);
 }
 
 void launch_tile_ranges(const uint32_t *keys, const uint32_t *n_dev, uint32_t n_max, const uint32_t *ntiles_dev,

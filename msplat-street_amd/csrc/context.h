@@ -123,8 +123,13 @@ struct msplat_hip_ctx {
         open_stage = -1;
         open_event = nullptr;
     }
-    void collect() {  // call after a stream sync
+    void collect() {  // stages whose end event has completed; the others stay open
+        std::vector<Mark> open;
         for (auto &m : marks) {
+            if (hipEventQuery(m.b) != hipSuccess) {
+                open.push_back(m);
+                continue;
+            }
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
                 stage_ms[m.stage] += ms;
@@ -133,12 +138,13 @@ struct msplat_hip_ctx {
             event_pool.push_back(m.a);
             event_pool.push_back(m.b);
         }
-        marks.clear();
+        marks.swap(open);
     }
     // device scalars: [0] u64 isect total, [8] u32 total, [16] double loss sum, [24] float loss,
     // [32] u32 big count, [40] u64 / [48] u32 visible gaussians, [56] / [64] the depth-order total
     Buf scalars;
     uint64_t *h_total = nullptr;  // pinned
+    hipEvent_t counts_ready = nullptr;  // h_total holds this frame's counts
     // frame state
     int n = 0, ntiles = 0;
     uint32_t width = 0, height = 0;
@@ -178,6 +184,7 @@ struct msplat_hip_ctx {
         for (Buf *b : all) release(*b);
         if (batch) msplat::destroy_batch_state(batch);
         if (h_total) (void)hipHostFree(h_total);
+        if (counts_ready) (void)hipEventDestroy(counts_ready);
         collect();
         for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
         if (own_stream && stream) (void)hipStreamDestroy(stream);

@@ -308,6 +308,8 @@ void launch_scan_u32(uint32_t *a, uint32_t L, uint32_t *partial, const ScanOut &
 void launch_compact(const CompactArgs &a, hipStream_t s);
 void launch_order_sums(const int32_t *list, const uint32_t *nvis_dev, uint32_t nvis_max, const uint32_t *count,
                        uint32_t *sums, hipStream_t s);
+void launch_keygen_offsets(const KeygenArgs &a, hipStream_t s);  // offsets, loff, big list
+void launch_keygen_emit(const KeygenArgs &a, bool batch, hipStream_t s);  // the keys
 void launch_keygen(const KeygenArgs &a, bool batch, hipStream_t s);
 void launch_radix_pass(const uint64_t *kin, const int32_t *vin, uint64_t *kout, int32_t *vout, const uint32_t *n_dev,
                        uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s);
