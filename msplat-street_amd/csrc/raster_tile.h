@@ -65,6 +65,13 @@ __device__ inline bool splat_misses_tile(const float4 &A, const float4 &B, float
     return A.x + hx < x0 || A.x - hx > x1 || A.y + hy < y0 || A.y - hy > y1;
 }
 
+// sigma = 1/2 (a dx^2 + c dy^2) + b dx dy in one fixed evaluation order, shared by the
+// forward and backward loops so that both make the same validity decisions. With
+// adx2 = (a dx) dx and bdx = b dx, which the backward hoists out of its pixel rows.
+__device__ inline float splat_sigma(float adx2, float bdx, float c, float dy) {
+    return __builtin_fmaf(bdx, dy, 0.5f * __builtin_fmaf(c * dy, dy, adx2));
+}
+
 // Position of the calling thread's item among the workgroup's kept items (in thread
 // order) and, in `total`, their number. Every thread of the NW-wave workgroup calls it;
 // `wcount` must not be written again before every wave has returned (callers alternate
@@ -137,7 +144,7 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
                 // terminate (transmittance would drop to T_MIN: last = k - 1, as the CPU)
                 const float4 A = la[j], B = lb[j];
                 const float dx = A.x - fx, dy = A.y - fy;
-                const float sig = 0.5f * (A.z * dx * dx + B.x * dy * dy) + A.w * dx * dy;
+                const float sig = splat_sigma((A.z * dx) * dx, A.w * dx, B.x, dy);
                 const float alpha = fminf(ALPHA_MAX, B.y * __expf(-sig));
                 const bool valid = !done && sig >= 0.f && sig < SIGMA_MAX && alpha >= ALPHA_MIN;
                 const float nT = T * (1.f - alpha);
@@ -338,13 +345,16 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             const int kk = lk[j];
             const float4 A = la[j], B = lb[j];
             const float cr = fmaxf(B.z + 0.5f, 0.f), cg = fmaxf(B.w + 0.5f, 0.f), cbl = fmaxf(lc[j] + 0.5f, 0.f);
-            float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f, q4 = 0.f, q5 = 0.f, q6 = 0.f, q7 = 0.f, q8 = 0.f;
+            // the lane's pixels share a column: dx and its products are per gaussian, and
+            // the dx-weighted sums follow from q0 and q2 after the rows
+            const float dx = A.x - fx;
+            const float adx2 = (A.z * dx) * dx, bdx = A.w * dx;
+            float q0 = 0.f, q2 = 0.f, q5 = 0.f, q6 = 0.f, q7 = 0.f, q8 = 0.f;
             bool contrib = false;
 #pragma unroll
             for (int k = 0; k < PPT; k++) {
-                if (__ballot(kk <= last[k]) == 0) continue;  // the whole wave is past this pixel row
-                const float dx = A.x - fx, dy = A.y - fy[k];
-                const float sig = 0.5f * (A.z * dx * dx + B.x * dy * dy) + A.w * dx * dy;
+                const float dy = A.y - fy[k];
+                const float sig = splat_sigma(adx2, bdx, B.x, dy);
                 const float alpha = fminf(ALPHA_MAX, B.y * __expf(-sig));
                 const bool valid = kk <= last[k] && sig >= 0.f && sig < SIGMA_MAX && alpha >= ALPHA_MIN;
                 contrib = contrib || valid;
@@ -360,14 +370,12 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
                 q7 += fac * vg[k];
                 q8 += fac * vb[k];
                 const float vs = (valid && alpha < ALPHA_MAX) ? -alpha * v_alpha : 0.f;
-                const float vsx = vs * dx, vsy = vs * dy;
+                const float vsy = vs * dy;
                 q0 += vs;
-                q1 += vsx;
                 q2 += vsy;
-                q3 += vsx * dx;
-                q4 += vsx * dy;
                 q5 += vsy * dy;
             }
+            const float q1 = q0 * dx, q3 = q1 * dx, q4 = q2 * dx;
             if (__ballot(contrib) != 0) {
                 const float t = wave_sums9(sl, q0, q1, q2, q3, q4, q5, q6, q7, q8);
                 if (lane < NQ) part[w][j][sl.q] = t;
