@@ -25,6 +25,8 @@ __device__ inline void adam_step(float &p, float &m1, float &m2, float g, float 
     m2 = v;
 }
 
+constexpr int BIG_REDUCE_GRID = 2048;  // 8 waves per SIMD
+
 template <int MODE>
 __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
     const int g = blockIdx.x * 256 + threadIdx.x;
@@ -36,9 +38,26 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
         vxy[0] = A.x; vxy[1] = A.y; vcn[0] = A.z; vcn[1] = A.w;
         vcn[2] = B.x; vrgb[0] = B.y; vrgb[1] = B.z; vrgb[2] = B.w;
         vop = a.big_c[g];
-    } else if (vis) {
-        const uint32_t o = a.offsets[g], c = a.count[g];
-        for (uint32_t p = o; p < o + c; p++) {
+    } else if (vis) {  // in tile order, four slots' loads in flight at a time
+        const uint32_t o = a.offsets[g], end = o + a.count[g];
+        uint32_t p = o;
+        for (; p + 4 <= end; p += 4) {
+            float4 A[4], B[4];
+            float Cs[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                A[u] = a.slot_a[p + u];
+                B[u] = a.slot_b[p + u];
+                Cs[u] = a.slot_c[p + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                vxy[0] += A[u].x; vxy[1] += A[u].y; vcn[0] += A[u].z; vcn[1] += A[u].w;
+                vcn[2] += B[u].x; vrgb[0] += B[u].y; vrgb[1] += B[u].z; vrgb[2] += B[u].w;
+                vop += Cs[u];
+            }
+        }
+        for (; p < end; p++) {
             const float4 A = a.slot_a[p], B = a.slot_b[p];
             vxy[0] += A.x; vxy[1] += A.y; vcn[0] += A.z; vcn[1] += A.w;
             vcn[2] += B.x; vrgb[0] += B.y; vrgb[1] += B.z; vrgb[2] += B.w;
@@ -120,28 +139,44 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
 
 // Slot sums of the gaussians with more than BIG_SLOTS intersections, one wave each
 // (grid-stride over the device-side list; every wave exits when the list is done).
+// Lane l adds slots l, l + 64, ... in order, four rounds of loads in flight at a time,
+// and the nine lane sums are added over the wave by wave_sums9.
 __global__ __launch_bounds__(256) void big_reduce_kernel(BackwardArgs a) {
-#pragma clang fp contract(fast)
     const uint32_t nbig = *a.big_count;
     const int lane = threadIdx.x & 63;
+    const SumLanes sl(lane);
     const uint32_t wave = (blockIdx.x * 256 + threadIdx.x) >> 6, nwaves = (gridDim.x * 256) >> 6;
     for (uint32_t i = wave; i < nbig; i += nwaves) {
         const uint32_t g = a.big_list[i];
-        const uint32_t o = a.offsets[g], c = a.count[g];
+        const uint32_t o = a.offsets[g], end = o + a.count[g];
         float v[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        for (uint32_t p = o + lane; p < o + c; p += 64) {
+        uint32_t p = o + lane;
+        for (; p + 192 < end; p += 256) {
+            float4 A[4], B[4];
+            float Cs[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                A[u] = a.slot_a[p + 64 * u];
+                B[u] = a.slot_b[p + 64 * u];
+                Cs[u] = a.slot_c[p + 64 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                v[0] += A[u].x; v[1] += A[u].y; v[2] += A[u].z; v[3] += A[u].w;
+                v[4] += B[u].x; v[5] += B[u].y; v[6] += B[u].z; v[7] += B[u].w;
+                v[8] += Cs[u];
+            }
+        }
+        for (; p < end; p += 64) {
             const float4 A = a.slot_a[p], B = a.slot_b[p];
             v[0] += A.x; v[1] += A.y; v[2] += A.z; v[3] += A.w;
             v[4] += B.x; v[5] += B.y; v[6] += B.z; v[7] += B.w;
             v[8] += a.slot_c[p];
         }
-        float r[9];
-#pragma unroll
-        for (int q = 0; q < 9; q++) r[q] = wave_sum(v[q]);
-        if (lane == 0) {
-            a.big_a[g] = make_float4(r[0], r[1], r[2], r[3]);
-            a.big_b[g] = make_float4(r[4], r[5], r[6], r[7]);
-            a.big_c[g] = r[8];
+        const float t = wave_sums9(sl, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]);
+        if (lane < 9) {
+            float *dst = sl.q < 4 ? &a.big_a[g].x + sl.q : sl.q < 8 ? &a.big_b[g].x + (sl.q - 4) : &a.big_c[g];
+            *dst = t;
         }
     }
 }
@@ -313,7 +348,7 @@ void launch_big_reduce_records(const uint32_t *big_list, const uint32_t *big_cou
     a.big_a = big_a;
     a.big_b = big_b;
     a.big_c = big_c;
-    big_reduce_kernel<<<1024, 256, 0, s>>>(a);
+    big_reduce_kernel<<<BIG_REDUCE_GRID, 256, 0, s>>>(a);
 }
 
 void launch_batch_backward(const BatchBackwardArgs &a, hipStream_t s) {
@@ -332,7 +367,7 @@ void launch_backward(const BackwardArgs &a, BackwardMode mode, hipStream_t s) {
     if (nb == 0) return;
     const size_t ne = (size_t)a.n * (sh_bases(a.cam.degree) - 1) * 3;
     const int nbe = (int)((ne + 255) / 256);
-    big_reduce_kernel<<<1024, 256, 0, s>>>(a);
+    big_reduce_kernel<<<BIG_REDUCE_GRID, 256, 0, s>>>(a);
     if (mode == BWD_ADAM) {
         backward_kernel<BWD_ADAM><<<nb, 256, 0, s>>>(a);
         if (nbe) sh_rest_kernel<BWD_ADAM><<<nbe, 256, 0, s>>>(a);

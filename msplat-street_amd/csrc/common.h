@@ -91,6 +91,57 @@ __device__ inline float rows_to_lane63(float v) {
     return v;
 }
 
+// Lane roles of wave_sums9 within a row of 16 lanes (r = lane & 15): at butterfly step
+// s (partners r^1, r^2, then the half-row and row mirrors r^7, r^15) a lane keeps the
+// first value of each pair when its bit is clear and the second when set. The bits are
+// b0^b2, b1^b2, b2^b3 of r: partners differ in the bit of their own step and agree in
+// the bits of the steps before and after it, so both hold the same value when they add.
+struct SumLanes {
+    bool s1, s2, s3;
+    int q;  // value whose wave total this lane holds at the end: 4 s3 + 2 s2 + s1 (r < 8), 8 (r = 8)
+    __device__ explicit SumLanes(int lane) {
+        const int r = lane & 15;
+        s1 = ((r ^ (r >> 2)) & 1) != 0;
+        s2 = (((r >> 1) ^ (r >> 2)) & 1) != 0;
+        s3 = (((r >> 2) ^ (r >> 3)) & 1) != 0;
+        q = r < 8 ? 4 * (int)s3 + 2 * (int)s2 + (int)s1 : 8;
+    }
+};
+
+// keep + partner's copy of the kept value (the partner sends the value it does not keep)
+template <int CTRL>
+__device__ inline float pair_step(float a, float b, bool second) {
+    const float keep = second ? b : a, send = second ? a : b;
+    return keep + dpp_f<CTRL>(send);
+}
+
+// x(row) + x(row ^ 1), then + the same over rows ^ 2: every lane ends with the sum over
+// the four rows of its position's value
+__device__ inline float sum_rows(float x) {
+    const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    const float y = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+    const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(y), __float_as_uint(y), false, false);
+    return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+}
+
+// Wave totals of nine per-lane values: lane l < 8 returns the total of value
+// SumLanes(l).q, lane 8 the total of q8. Each value is added in the order of a plain
+// per-value butterfly (partners r^1, r^2, r^7, r^15, then rows r^1, r^2), with a third
+// of its DPP operations.
+__device__ inline float wave_sums9(const SumLanes &L, float q0, float q1, float q2, float q3, float q4, float q5,
+                                   float q6, float q7, float q8) {
+    const float p0 = pair_step<0xb1>(q0, q1, L.s1), p1 = pair_step<0xb1>(q2, q3, L.s1);
+    const float p2 = pair_step<0xb1>(q4, q5, L.s1), p3 = pair_step<0xb1>(q6, q7, L.s1);
+    float e = q8 + dpp_f<0xb1>(q8);                              // quad_perm [1,0,3,2]
+    const float r0 = pair_step<0x4e>(p0, p1, L.s2), r1 = pair_step<0x4e>(p2, p3, L.s2);
+    e += dpp_f<0x4e>(e);                                         // quad_perm [2,3,0,1]
+    float u = pair_step<0x141>(r0, r1, L.s3);
+    e += dpp_f<0x141>(e);                                        // row_half_mirror
+    u += dpp_f<0x140>(u);                                        // row_mirror
+    e += dpp_f<0x140>(e);
+    return sum_rows(L.q == 8 ? e : u);  // lane position r holds the same value in every row
+}
+
 __device__ inline uint64_t ballot(bool p) { return __ballot(p); }
 
 // ── counter-based RNG: PCG hash (Jarzynski & Olano 2020), 24-bit uniforms in (0, 1),
