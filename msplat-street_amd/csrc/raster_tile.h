@@ -131,34 +131,48 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
         }
         int n;
         const int pos = kept_position<4>(keep, wcount, n);
-        if (keep) {
+        if (keep) {  // colours stored clamped: max(raw + 0.5, 0), as composited
             la[pos] = A0;
-            lb[pos] = B0;
-            lc[pos] = C0;
+            lb[pos] = make_float4(B0.x, B0.y, fmaxf(B0.z + 0.5f, 0.f), fmaxf(B0.w + 0.5f, 0.f));
+            lc[pos] = fmaxf(C0 + 0.5f, 0.f);
             lk[pos] = k;
+        }
+        // pad to a multiple of 8 with records of zero opacity (alpha 0 < ALPHA_MIN: they
+        // composite nothing), so that the loop below runs whole groups of 8
+        const int n8 = (n + 7) & ~7;
+        if ((int)threadIdx.x >= n && (int)threadIdx.x < n8) {
+            la[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+            lb[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+            lc[threadIdx.x] = 0.f;
+            lk[threadIdx.x] = 0;
         }
         __syncthreads();
         if (__ballot(!done) != 0) {
-            for (int j = 0; j < n; j++) {
-                // predicated compositing step: skip (out of range / too faint), accept, or
-                // terminate (transmittance would drop to T_MIN: last = k - 1, as the CPU)
-                const float4 A = la[j], B = lb[j];
-                const float dx = A.x - fx, dy = A.y - fy;
-                const float sig = splat_sigma((A.z * dx) * dx, A.w * dx, B.x, dy);
-                const float alpha = fminf(ALPHA_MAX, B.y * __expf(-sig));
-                const bool valid = !done && sig >= 0.f && sig < SIGMA_MAX && alpha >= ALPHA_MIN;
-                const float nT = T * (1.f - alpha);
-                const bool term = valid && nT <= T_MIN;
-                const bool accept = valid && !term;
-                const float vis = accept ? alpha * T : 0.f;
-                cr += fmaxf(B.z + 0.5f, 0.f) * vis;
-                cg += fmaxf(B.w + 0.5f, 0.f) * vis;
-                cb += fmaxf(lc[j] + 0.5f, 0.f) * vis;
-                T = accept ? nT : T;
-                const int kj = lk[j];
-                last = term ? kj - 1 : (accept ? kj : last);
-                done = done || term;
-                if ((j & 7) == 7 && __ballot(!done) == 0) break;
+            for (int j0 = 0; j0 < n8; j0 += 8) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    // predicated compositing step: skip (out of range / too faint), accept,
+                    // or terminate (transmittance would drop to T_MIN: last = k - 1, as the
+                    // CPU)
+                    const int j = j0 + u;
+                    const float4 A = la[j], B = lb[j];
+                    const float dx = A.x - fx, dy = A.y - fy;
+                    const float sig = splat_sigma((A.z * dx) * dx, A.w * dx, B.x, dy);
+                    const float alpha = fminf(ALPHA_MAX, B.y * __expf(-sig));
+                    const bool valid = !done && sig >= 0.f && sig < SIGMA_MAX && alpha >= ALPHA_MIN;
+                    const float nT = T * (1.f - alpha);
+                    const bool term = valid && nT <= T_MIN;
+                    const bool accept = valid && !term;
+                    const float vis = accept ? alpha * T : 0.f;
+                    cr += B.z * vis;
+                    cg += B.w * vis;
+                    cb += lc[j] * vis;
+                    T = accept ? nT : T;
+                    const int kj = lk[j];
+                    last = term ? kj - 1 : (accept ? kj : last);
+                    done = done || term;
+                }
+                if (__ballot(!done) == 0) break;
             }
         }
     }
