@@ -76,8 +76,10 @@ __device__ inline uint32_t next_item(ItemQueue &q, uint32_t *slot) {
 
 }  // namespace
 
+// Forward kernels are capped at 64 VGPRs (8 waves per SIMD): the unrolled compositing
+// loop would otherwise hold more registers than the extra waves are worth.
 template <int FB>
-__global__ __launch_bounds__(256) void raster_forward_kernel(RasterArgs a) {
+__global__ __launch_bounds__(256, 8) void raster_forward_kernel(RasterArgs a) {
     const int t = xcd_item(blockIdx.x, gridDim.x);
     forward_tile<FB>(tile_view(a), tile_data(a), t, t);
 }
@@ -92,7 +94,7 @@ __global__ __launch_bounds__(256 / PPT) void raster_backward_kernel(RasterArgs a
 // of view v, whose range is number (tile * num_views + v). Every workgroup loops until
 // the queue is drained, so the grid drains on its own.
 template <int FB>
-__global__ __launch_bounds__(256) void raster_forward_batch_kernel(BatchRasterArgs a) {
+__global__ __launch_bounds__(256, 8) void raster_forward_batch_kernel(BatchRasterArgs a) {
     __shared__ uint32_t tb[MAX_VIEWS + 1];
     __shared__ uint32_t item_slot;
     const int nv = a.num_views;
