@@ -212,7 +212,8 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
     __shared__ float4 la[BATCH], lb[BATCH];
     __shared__ float lc[BATCH];
     __shared__ uint32_t lslot[BATCH];
-    __shared__ int lk[BATCH];  // the staged record's position in the tile's list
+    __shared__ int lk[BATCH];       // the staged record's position in the tile's list
+    __shared__ uint8_t lmask[BATCH];  // bit c: raw colour c + 0.5 >= 0 (its gradient passes)
     __shared__ float part[NW][BATCH][NQ];
     __shared__ int smax[NW];
     __shared__ int wcount[2][NW];
@@ -293,10 +294,12 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             }
             int total;
             const int pos = kept + kept_position<NW>(keep, wcount[(c / NT) & 1], total);
-            if (keep) {
+            if (keep) {  // colours stored clamped, max(raw + 0.5, 0), as composited
                 la[pos] = A0;
-                lb[pos] = B0;
-                lc[pos] = C0;
+                lb[pos] = make_float4(B0.x, B0.y, fmaxf(B0.z + 0.5f, 0.f), fmaxf(B0.w + 0.5f, 0.f));
+                lc[pos] = fmaxf(C0 + 0.5f, 0.f);
+                lmask[pos] = (uint8_t)((B0.z + 0.5f >= 0.f ? 1 : 0) | (B0.w + 0.5f >= 0.f ? 2 : 0) |
+                                       (C0 + 0.5f >= 0.f ? 4 : 0));
                 lslot[pos] = p;
                 lk[pos] = start + i;
             }
@@ -307,7 +310,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
         for (int j = kept - 1; j >= 0; j--) {
             const int kk = lk[j];
             const float4 A = la[j], B = lb[j];
-            const float cr = fmaxf(B.z + 0.5f, 0.f), cg = fmaxf(B.w + 0.5f, 0.f), cbl = fmaxf(lc[j] + 0.5f, 0.f);
+            const float cr = B.z, cg = B.w, cbl = lc[j];
             // the lane's pixels share a column: dx and its products are per gaussian, and
             // the dx-weighted sums follow from q0 and q2 after the rows
             const float dx = A.x - fx;
@@ -356,7 +359,8 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             }
             const float4 A = la[j], B = lb[j];
             const float ox = A.z * q[1] + A.w * q[2], oy = A.w * q[1] + B.x * q[2];
-            const bool mr = B.z + 0.5f >= 0.f, mg = B.w + 0.5f >= 0.f, mb = lc[j] + 0.5f >= 0.f;
+            const uint32_t cm = lmask[j];
+            const bool mr = cm & 1u, mg = cm & 2u, mb = cm & 4u;
             const uint32_t p = lslot[j];
             d.slot_a[p] = make_float4(ox, oy, 0.5f * q[3], 0.5f * q[4]);
             d.slot_b[p] = make_float4(0.5f * q[5], mr ? q[6] : 0.f, mg ? q[7] : 0.f, mb ? q[8] : 0.f);
