@@ -299,13 +299,14 @@ def copy_calibration(mib=256, times=5):
 def profiled_traffic(kernel_stage, tuning):
     """HBM bytes per launch of the dominant kernel, from the committed counter summary
     (profiles/raster_backward_street_475k_traffic.json, FETCH_SIZE and WRITE_SIZE scaled
-    by a copy of known size) when the timed kernel is the profiled variant; else None."""
+    by a copy of known size, per kernel variant) for the timed variant; else None."""
     path = os.path.join(ROOT, "profiles", "raster_backward_street_475k_traffic.json")
     if kernel_stage != "rast_bwd" or not os.path.exists(path):
         return None
     prof = json.load(open(path))
     variant = "raster_backward_kernel<%d, %d>" % (tuning.get("bwd_ppt", 4), tuning.get("bwd_batch", 128))
-    return prof["bytes_per_launch"] if prof.get("kernel") == variant else None
+    entry = prof.get("variants", {}).get(variant)
+    return entry["bytes_per_launch"] if entry else None
 
 
 def cpu_baseline_sample(dg, cams, targets, ctx, step_index, seconds=20.0):
@@ -354,6 +355,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--bwd-ppt", type=int, default=0, help="backward pixels per thread (0: library default)")
+    ap.add_argument("--bwd-batch", type=int, default=0, help="backward LDS batch depth (0: library default)")
+    ap.add_argument("--fwd-batch", type=int, default=0, help="forward LDS batch depth (0: library default)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-views comparison")
     ap.add_argument("--batched-steps", type=int, default=8)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
@@ -375,9 +378,13 @@ def main():
     from msplat.trainer import TrainConfig, Trainer
 
     ctx = Context(0)
-    if args.bwd_ppt:
-        from msplat._abi import TUNE_BWD_PPT
-        ctx.set_tuning(TUNE_BWD_PPT, args.bwd_ppt)
+    pinned = args.bwd_ppt or args.bwd_batch or args.fwd_batch  # a fixed variant: no autotuning
+    if pinned:
+        from msplat._abi import TUNE_BWD_BATCH, TUNE_BWD_PPT, TUNE_FWD_BATCH
+        for key, value in ((TUNE_BWD_PPT, args.bwd_ppt), (TUNE_BWD_BATCH, args.bwd_batch),
+                           (TUNE_FWD_BATCH, args.fwd_batch)):
+            if value:
+                ctx.set_tuning(key, value)
     t_setup = time.perf_counter()
     dg, gt, cams, targets = build_workload(ctx)
     del gt
@@ -385,7 +392,7 @@ def main():
     setup_s = time.perf_counter() - t_setup
     log(f"workload ready in {setup_s:.1f} s: {dg.n} gaussians, {len(cams)} views")
     tuning = None
-    if not args.no_autotune and not args.bwd_ppt:
+    if not args.no_autotune and not pinned:
         tuning = tune_train_step(ctx, cams, targets)
         log("autotuned: " + str({k: tuning[k] for k in ("bwd_ppt", "fwd_batch", "bwd_batch", "ms")}))
     cfg = TrainConfig(iterations=args.iterations, sh_degree=3, warmup_length=499, cull_alpha_thresh=0.005)

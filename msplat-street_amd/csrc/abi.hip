@@ -133,6 +133,7 @@ int do_bin(msplat_hip_ctx *ctx) {
         std::swap(dki, dko);
         std::swap(dvi, dvo);
     }
+    ctx->vis_sorted = dvi;
     // intersection offsets in depth order
     launch_order_sums(dvi, ctx->d_vis32(), (uint32_t)Nc, (const uint32_t *)ctx->count.p, (uint32_t *)ctx->ord_sums.p,
                       s);
@@ -294,6 +295,8 @@ int backward_args(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const mspl
         CHECKED(ctx->grow(ctx->sh_scratch, 32 * N));
         b.sh_dir = (float4 *)ctx->sh_scratch.p;
         b.sh_vrgb = b.sh_dir + N;
+        b.vis_list = ctx->vis_sorted;
+        b.vis_count = ctx->d_vis32();
         b.big_list = ctx->big_list();
         b.big_count = ctx->d_big_count();
         float4 *ba = (float4 *)(((uintptr_t)ctx->big_a() + 15) & ~(uintptr_t)15);

@@ -216,6 +216,32 @@ __global__ __launch_bounds__(256) void sh_rest_kernel(BackwardArgs a) {
     adam_step(a.rest[e], a.m1[4][e], a.m2[4][e], grad, a.step[4], a.beta1, a.beta2, a.bc2[4], a.eps);
 }
 
+// Adam on SH bands 1..3 of the visible gaussians only: one thread per coefficient of
+// the frame's visible list (grid-stride over its device-side count), so the many
+// gaussians outside the view, whose coefficients the fused SH Adam leaves alone, cost
+// no threads. A gaussian's coefficients are contiguous, so its threads stay coalesced.
+__global__ __launch_bounds__(256) void sh_rest_adam_kernel(BackwardArgs a) {
+    const uint32_t stride = (sh_bases(a.cam.degree) - 1) * 3;
+    const int dtu = (int)min(a.cam.degrees_to_use, 3u);
+    const uint32_t nbasis = dtu >= 3 ? 15 : dtu == 2 ? 8 : dtu == 1 ? 3 : 0;
+    const size_t total = (size_t)*a.vis_count * stride;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+        const uint32_t i = (uint32_t)(e / stride), j = (uint32_t)(e % stride), b = j / 3, c = j % 3;
+        if (b >= nbasis) continue;
+        const int g = a.vis_list[i];
+        const float4 D = a.sh_dir[g];
+        const float d[3] = {D.x, D.y, D.z};
+        float basis[15];
+        sh_basis((uint32_t)dtu, d, basis);
+        const float4 V = a.sh_vrgb[g];
+        const float vc = c == 0 ? V.x : c == 1 ? V.y : V.z;
+        const size_t k = (size_t)g * stride + j;
+        adam_step(a.rest[k], a.m1[4][k], a.m2[4][k], basis[b] * vc, a.step[4], a.beta1, a.beta2, a.bc2[4], a.eps);
+    }
+}
+
+constexpr int SH_ADAM_GRID = 4096;
+
 // ── batched views ────────────────────────────────────────────────────────────
 // Gaussian g's gradient summed over the views of a batch, one thread per gaussian. Per
 // view the slot sums (or the big-list sums) give the 2D cotangents, which go through
@@ -370,7 +396,7 @@ void launch_backward(const BackwardArgs &a, BackwardMode mode, hipStream_t s) {
     big_reduce_kernel<<<BIG_REDUCE_GRID, 256, 0, s>>>(a);
     if (mode == BWD_ADAM) {
         backward_kernel<BWD_ADAM><<<nb, 256, 0, s>>>(a);
-        if (nbe) sh_rest_kernel<BWD_ADAM><<<nbe, 256, 0, s>>>(a);
+        if (nbe) sh_rest_adam_kernel<<<std::min(nbe, SH_ADAM_GRID), 256, 0, s>>>(a);
     } else {
         backward_kernel<BWD_GRADS><<<nb, 256, 0, s>>>(a);
         if (nbe) sh_rest_kernel<BWD_GRADS><<<nbe, 256, 0, s>>>(a);

@@ -84,6 +84,7 @@ struct msplat_hip_ctx {
     Buf tile_start, image, final_T, final_idx, v_image, row_loss, loss_scratch;
     Buf grad_scratch;  // stands in for NULL gradient outputs of render_backward
     Buf aux;           // densify flags/scans/scratch, kNN and metrics workspaces
+    int32_t *vis_sorted = nullptr;  // visible gaussians in depth order (the frame's list)
     msplat::BatchState *batch = nullptr;  // batched-view frame and graph (created on first use)
     // stage timing
     bool timing = false;

@@ -141,6 +141,9 @@ struct BackwardArgs {
     const uint32_t *big_list, *big_count;
     float4 *big_a, *big_b;
     float *big_c;
+    // the visible gaussians (depth order) and their device-side count, for the SH Adam
+    const int32_t *vis_list;
+    const uint32_t *vis_count;
 };
 
 struct DensifyArgs {
