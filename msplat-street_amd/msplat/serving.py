@@ -2,15 +2,18 @@
 
 Trained scenes stay resident in HBM. Requests (scene, camera, resolution) arrive
 continuously; a request waits in the pending queue until a worker slot is free and the
-HBM budget admits it, and is then rendered together with the other pending requests of
-its scene in one msplat_hip_render_batch (mixed resolutions, one persistent
-rasterizer launch) plus one RGBA8 pack (msplat_pack_rgba8, core/cpu/msplat_cpu.cpp:
-1723-1734). Each slot owns a context (its own stream and buffers) and runs in its own
-thread; the library releases the GIL during its calls, so the batches of different slots
-overlap on the device, and requests that arrive while batches are in flight are admitted
-as slots and memory free up. A lone pending request takes the one-view path
-(msplat_hip_render). The plain form renders one request per msplat_hip_render, first
-come first served, on one context.
+HBM budget admits it. Each slot owns a context (its own stream and buffers) and runs in
+its own thread; the library releases the GIL during its calls, so the renders of
+different slots overlap on the device, and requests that arrive meanwhile are admitted
+as slots and memory free up. A free slot renders the oldest request alone
+(msplat_hip_render) while the queue is short; once more than `batch_depth` requests are
+pending, it renders the oldest together with later pending requests of its scene in one
+msplat_hip_render_batch (mixed resolutions, one persistent rasterizer launch), up to
+`max_batch_pixels` in all. A batch costs about what its views cost one by one, except
+that small views fill the device better together; so batches are kept short (every
+request of a batch completes when the batch does) and form only when requests queue. Every render ends with one RGBA8
+pack (msplat_pack_rgba8, core/cpu/msplat_cpu.cpp:1723-1734). The plain form renders one
+request per msplat_hip_render, first come first served, on one context.
 
 Admission: the resident scenes' parameters plus every slot's working set stay within the
 HBM budget; a batch's working set is estimated from its record count (views x
@@ -70,9 +73,14 @@ class Server:
 
     def __init__(self, scenes: list, slots: int = 2, max_views: int = 16, hbm_budget: float = 64e9,
                  background=(0.0, 0.0, 0.0), device: int = 0, keep_images: bool = False,
-                 isects_per_pixel: float = 60.0, adapt: bool = True):
+                 isects_per_pixel: float = 60.0, adapt: bool = True, batch_depth: int | None = None,
+                 max_batch_pixels: float = 2.1e6):
         self.scenes = scenes
         self.max_views = int(max_views)
+        # pending requests beyond which a free slot batches (default: 2 per slot), and the
+        # pixels a batch may hold in all (default: about one 1920x1080 view)
+        self.batch_depth = 2 * slots if batch_depth is None else int(batch_depth)
+        self.max_batch_pixels = float(max_batch_pixels)
         self.bg = tuple(float(v) for v in background)
         self.keep_images = keep_images
         self.resident = sum(s.n * PARAM_BYTES for s in scenes)
@@ -99,19 +107,26 @@ class Server:
         return len(reqs) * s.n * REC_BYTES + px * PIX_BYTES + px * self.ipp[reqs[0].scene] * ISECT_BYTES
 
     def _take_batch(self) -> list:
-        """The oldest pending request and the later ones of its scene, in arrival order,
-        up to max_views and the slot's share of the budget."""
+        """The oldest pending request; when more than batch_depth are pending, also the
+        later ones of its scene, in arrival order, up to max_views, max_batch_pixels and
+        the slot's share of the budget."""
         head = self.pending[0]
         if self.working_set([head]) > self.share:
             self.pending.pop(0)
             self.rejected.append(head)
             return []
         batch = [head]
+        if len(self.pending) <= self.batch_depth:
+            self.pending.pop(0)
+            return batch
+        pixels = head.pixels
         for r in self.pending[1:]:
             if len(batch) >= self.max_views:
                 break
-            if r.scene == head.scene and self.working_set(batch + [r]) <= self.share:
+            if (r.scene == head.scene and pixels + r.pixels <= self.max_batch_pixels
+                    and self.working_set(batch + [r]) <= self.share):
                 batch.append(r)
+                pixels += r.pixels
         taken = set(id(r) for r in batch)
         self.pending = [r for r in self.pending if id(r) not in taken]
         return batch
