@@ -42,6 +42,7 @@ BYTES = {
     "loss_fwd_bwd": (0, 36, 0),
     "prefix_sort_pack": (24 + 144 + 8, 0, 24),
 }
+DOMINANT = "rast_bwd"  # the train step's longest stage (the raster backward)
 
 
 def log(msg):
@@ -365,6 +366,8 @@ def main():
                     help="only copy 256 MiB device to device five times (the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--serving-seconds", type=float, default=1.0)
+    ap.add_argument("--stage-steps", type=int, default=200,
+                    help="steps after the timed ones that time every stage for the breakdown")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -404,7 +407,11 @@ def main():
     first_timed_step = tr.step_count + 1
     log(f"warm-up done ({args.warmup} steps); timing {args.steps} steps")
 
+    # The timed steps record events around the dominant stage only: an event between two
+    # kernels costs the stream a few microseconds, and six stages' worth would take
+    # several percent off the step rate.
     ctx.enable_stage_timing(True)
+    ctx.set_stage_mask((DOMINANT,))
     ctx.drain_stage_times()
     m_total, n_total = 0, 0
     ctx.sync()
@@ -415,6 +422,11 @@ def main():
         n_total += dg.n
     ctx.sync()
     elapsed = time.perf_counter() - t0
+    timed = ctx.drain_stage_times()
+    # stage breakdown over a separate window of steps that times every stage
+    ctx.set_stage_mask(None)
+    for _ in range(args.stage_steps):
+        tr.step()
     stages = ctx.drain_stage_times()
     ctx.enable_stage_timing(False)
     loss = tr.step(read_loss=True)
@@ -423,9 +435,12 @@ def main():
     K = args.steps
     iters = K / elapsed
     P = W_IMG * H_IMG
-    # dominant kernel class and its roofline
-    dom = max(stages, key=lambda k: stages[k][0])
-    dom_ms, dom_calls = stages[dom]
+    # dominant kernel class and its roofline, from the events of the timed steps
+    dom = DOMINANT
+    heaviest = max(stages, key=lambda k: stages[k][0])
+    if heaviest != dom:
+        log(f"note: the stage breakdown's heaviest stage is {heaviest}, not {dom}")
+    dom_ms, dom_calls = timed[dom]
     avg_ms = dom_ms / max(dom_calls, 1)
     per_isect, per_px, per_g = BYTES.get(dom, (0, 0, 0))
     alg_bytes = (per_isect * m_total + per_px * P * K + per_g * n_total) / max(dom_calls, 1)
@@ -438,7 +453,8 @@ def main():
                 "traffic_source": "profiles/raster_backward_street_475k_traffic.md" if traffic else None,
                 "variant": cur,
                 "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(alg_bytes),
-                "stages_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in stages.items()}}
+                "stages_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in stages.items()},
+                "stages_window_steps": args.stage_steps}
     batched = None
     if not args.no_batched:
         log("batched views: 1M gaussians, 16 views per launch")

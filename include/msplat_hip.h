@@ -199,6 +199,11 @@ int msplat_hip_stage_loss(msplat_hip_ctx *ctx, const msplat_hip_camera *cam, con
 enum { MSPLAT_NUM_STAGES = 6 };
 int msplat_hip_enable_stage_timing(msplat_hip_ctx *ctx, int enable);
 int msplat_hip_drain_stage_times(msplat_hip_ctx *ctx, double *ms, int64_t *calls, int max_stages, int *n_stages);
+/* Times only the stages whose bit (1 << stage) is set in mask (default: every
+ * stage). Each timed stage records two events, and an event between two kernels
+ * costs the stream some microseconds, so a caller measuring whole-step throughput
+ * times the one stage it needs. No counterpart in the reference. */
+int msplat_hip_set_stage_mask(msplat_hip_ctx *ctx, uint32_t mask);
 const char *msplat_hip_stage_name(int stage);
 
 /* ── densification and maintenance (bindings.h:120-147) ─────────────────────── */

@@ -824,6 +824,13 @@ int msplat_hip_enable_stage_timing(msplat_hip_ctx *ctx, int enable) {
     return MSPLAT_OK;
 }
 
+int msplat_hip_set_stage_mask(msplat_hip_ctx *ctx, uint32_t mask) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    if (mask >> MSPLAT_NUM_STAGES) return set_error(MSPLAT_ERR_ARG, "stage mask 0x%x names no stage", mask);
+    ctx->stage_mask = mask;
+    return MSPLAT_OK;
+}
+
 int msplat_hip_drain_stage_times(msplat_hip_ctx *ctx, double *ms, int64_t *calls, int max_stages, int *n_stages) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
     HIPCHECK(hipStreamSynchronize(ctx->stream));

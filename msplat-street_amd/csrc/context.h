@@ -88,6 +88,7 @@ struct msplat_hip_ctx {
     msplat::BatchState *batch = nullptr;  // batched-view frame and graph (created on first use)
     // stage timing
     bool timing = false;
+    uint32_t stage_mask = ~0u;  // stages timed while timing is on (bit s = stage s)
     struct Mark {
         int stage;
         hipEvent_t a, b;
@@ -110,7 +111,7 @@ struct msplat_hip_ctx {
         return e;
     }
     void begin(int stage) {
-        if (!timing) return;
+        if (!timing || !((stage_mask >> stage) & 1u)) return;
         open_stage = stage;
         open_event = take_event();
         if (open_event) (void)hipEventRecord(open_event, stream);

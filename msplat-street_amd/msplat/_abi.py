@@ -21,6 +21,8 @@ TUNE_BWD_PPT = 0
 TUNE_FWD_BATCH = 1
 TUNE_BWD_BATCH = 2
 TUNE_CULL = 3
+# stage timing: stage s is bit (1 << s) of a stage mask
+STAGE_NAMES = ("proj_sh_fwd", "prefix_sort_pack", "rast_fwd", "loss_fwd_bwd", "rast_bwd", "proj_sh_bwd_adam")
 fp = C.POINTER(C.c_float)
 
 
@@ -118,6 +120,7 @@ def lib():
             "msplat_hip_image_metrics": ([vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
             "msplat_hip_pack_rgba8": ([vp, vp, u32, vp], C.c_int),
             "msplat_hip_enable_stage_timing": ([vp, C.c_int], C.c_int),
+            "msplat_hip_set_stage_mask": ([vp, C.c_uint32], C.c_int),
             "msplat_hip_drain_stage_times": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int,
                                               C.POINTER(C.c_int)], C.c_int),
             "msplat_hip_stage_name": ([C.c_int], C.c_char_p),
@@ -184,6 +187,11 @@ class Context:
 
     def enable_stage_timing(self, on: bool = True):
         check(lib().msplat_hip_enable_stage_timing(self.ptr, 1 if on else 0))
+
+    def set_stage_mask(self, stages=None):
+        """Time only the named stages (None: every stage)."""
+        mask = (1 << len(STAGE_NAMES)) - 1 if stages is None else sum(1 << STAGE_NAMES.index(s) for s in stages)
+        check(lib().msplat_hip_set_stage_mask(self.ptr, mask))
 
     def drain_stage_times(self) -> dict:
         """{stage name: (total ms, launches)} since the last drain (syncs)."""

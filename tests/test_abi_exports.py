@@ -43,6 +43,8 @@ def test_ctypes_binding_covers_the_header():
     unbound = [n for n in declared() if getattr(lib, n).argtypes is None]
     assert not unbound, unbound
     assert lib.msplat_hip_stage_name(0) == b"proj_sh_fwd"
+    # the stage mask's bit order is the library's stage numbering
+    assert [lib.msplat_hip_stage_name(i).decode() for i in range(len(_abi.STAGE_NAMES))] == list(_abi.STAGE_NAMES)
 
 
 REF = "/root/reference"

@@ -74,3 +74,35 @@ def test_autotuner_picks_a_valid_variant(ctx, tmp_path):
         assert again["cached"] and all(again[n] == res[n] for n in autotune.SPACE)
     finally:
         autotune.apply(ctx, saved)
+
+
+def test_stage_mask_times_only_the_named_stages(ctx):
+    from msplat._abi import STAGE_NAMES, DeviceArray
+    from msplat.backend import DeviceGaussians, render, render_backward
+    from msplat.camera import look_at, make_camera
+    from msplat.scenes import random_scene
+    sc = random_scene(33, 3000, degree=3, log_scale=(-3.4, 0.5))
+    cam = make_camera(look_at((2.4, 0.9, 2.6), (0.0, 0.0, 0.0)), 96, 64, 80.0, 80.0, dtu=3)
+    dg = DeviceGaussians(ctx, sc)
+    v_img = DeviceArray.from_numpy(ctx, np.full((64, 96, 3), 1e-3, np.float32))
+    try:
+        ctx.enable_stage_timing(True)
+        for stages in (None, ("rast_bwd",), ("prefix_sort_pack", "rast_fwd")):
+            ctx.set_stage_mask(stages)
+            ctx.drain_stage_times()
+            for _ in range(3):
+                render(ctx, dg, cam, BG)
+                render_backward(ctx, dg, cam, BG, v_img)
+            st = ctx.drain_stage_times()
+            want = set(STAGE_NAMES if stages is None else stages) - {"loss_fwd_bwd"}  # no loss stage here
+            for name, (ms, calls) in st.items():
+                if name in want:
+                    assert calls == 3 and ms > 0, (stages, name, st[name])
+                else:
+                    assert calls == 0 and ms == 0, (stages, name, st[name])
+        with pytest.raises(RuntimeError):
+            from msplat._abi import check, lib
+            check(lib().msplat_hip_set_stage_mask(ctx.ptr, 1 << len(STAGE_NAMES)))
+    finally:
+        ctx.set_stage_mask(None)
+        ctx.enable_stage_timing(False)
