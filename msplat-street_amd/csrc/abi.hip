@@ -225,6 +225,7 @@ RasterArgs raster_args(msplat_hip_ctx *ctx) {
     r.slot_a = (float4 *)ctx->slot_a.p;
     r.slot_b = (float4 *)ctx->slot_b.p;
     r.slot_c = (float *)ctx->slot_c.p;
+    r.slot_f = (uint8_t *)ctx->slot_f.p;
     return r;
 }
 
@@ -264,6 +265,7 @@ int do_raster_backward(msplat_hip_ctx *ctx) {
     CHECKED(ctx->grow(ctx->slot_a, 16 * Mc));
     CHECKED(ctx->grow(ctx->slot_b, 16 * Mc));
     CHECKED(ctx->grow(ctx->slot_c, 4 * Mc));
+    CHECKED(ctx->grow(ctx->slot_f, Mc));
     ctx->begin(4);
     launch_raster_backward(raster_args(ctx), ctx->tuning, ctx->stream);
     ctx->end();
@@ -290,6 +292,7 @@ int backward_args(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const mspl
     b.slot_a = (const float4 *)ctx->slot_a.p;
     b.slot_b = (const float4 *)ctx->slot_b.p;
     b.slot_c = (const float *)ctx->slot_c.p;
+    b.slot_f = (const uint8_t *)ctx->slot_f.p;
     {
         const size_t N = (size_t)(b.n > 0 ? b.n : 1);
         CHECKED(ctx->grow(ctx->sh_scratch, 32 * N));

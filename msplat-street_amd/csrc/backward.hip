@@ -7,7 +7,8 @@
 //               parameters; or
 //   BWD_GRADS — the parameter gradients themselves (autograd backend).
 // The slot sum runs over the gaussian's intersections in tile order, as the reference's
-// reduce_isect_grads (cpu:1034-1064). Built with -ffp-contract=off.
+// reduce_isect_grads (cpu:1034-1064); slots flagged zero by the rasterizer are not read
+// (they add exact zeros). Built with -ffp-contract=off.
 #include <cstring>
 
 #include "common.h"
@@ -27,6 +28,20 @@ __device__ inline void adam_step(float &p, float &m1, float &m2, float g, float 
 
 constexpr int BIG_REDUCE_GRID = 2048;  // 8 waves per SIMD
 
+// Slot p's nine values, or zeros when its flag says they are all zero (not read)
+__device__ inline void load_slot(const float4 *sa, const float4 *sb, const float *sc, const uint8_t *sf, uint32_t p,
+                                 float4 &A, float4 &B, float &C) {
+    if (sf[p]) {
+        A = sa[p];
+        B = sb[p];
+        C = sc[p];
+    } else {
+        A = make_float4(0.f, 0.f, 0.f, 0.f);
+        B = A;
+        C = 0.f;
+    }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
     const int g = blockIdx.x * 256 + threadIdx.x;
@@ -45,11 +60,7 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
             float4 A[4], B[4];
             float Cs[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                A[u] = a.slot_a[p + u];
-                B[u] = a.slot_b[p + u];
-                Cs[u] = a.slot_c[p + u];
-            }
+            for (int u = 0; u < 4; u++) load_slot(a.slot_a, a.slot_b, a.slot_c, a.slot_f, p + u, A[u], B[u], Cs[u]);
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 vxy[0] += A[u].x; vxy[1] += A[u].y; vcn[0] += A[u].z; vcn[1] += A[u].w;
@@ -58,10 +69,12 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
             }
         }
         for (; p < end; p++) {
-            const float4 A = a.slot_a[p], B = a.slot_b[p];
+            float4 A, B;
+            float Cs;
+            load_slot(a.slot_a, a.slot_b, a.slot_c, a.slot_f, p, A, B, Cs);
             vxy[0] += A.x; vxy[1] += A.y; vcn[0] += A.z; vcn[1] += A.w;
             vcn[2] += B.x; vrgb[0] += B.y; vrgb[1] += B.z; vrgb[2] += B.w;
-            vop += a.slot_c[p];
+            vop += Cs;
         }
     }
     if (a.v_xy) {
@@ -155,11 +168,8 @@ __global__ __launch_bounds__(256) void big_reduce_kernel(BackwardArgs a) {
             float4 A[4], B[4];
             float Cs[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                A[u] = a.slot_a[p + 64 * u];
-                B[u] = a.slot_b[p + 64 * u];
-                Cs[u] = a.slot_c[p + 64 * u];
-            }
+            for (int u = 0; u < 4; u++)
+                load_slot(a.slot_a, a.slot_b, a.slot_c, a.slot_f, p + 64 * u, A[u], B[u], Cs[u]);
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 v[0] += A[u].x; v[1] += A[u].y; v[2] += A[u].z; v[3] += A[u].w;
@@ -168,10 +178,12 @@ __global__ __launch_bounds__(256) void big_reduce_kernel(BackwardArgs a) {
             }
         }
         for (; p < end; p += 64) {
-            const float4 A = a.slot_a[p], B = a.slot_b[p];
+            float4 A, B;
+            float Cs;
+            load_slot(a.slot_a, a.slot_b, a.slot_c, a.slot_f, p, A, B, Cs);
             v[0] += A.x; v[1] += A.y; v[2] += A.z; v[3] += A.w;
             v[4] += B.x; v[5] += B.y; v[6] += B.z; v[7] += B.w;
-            v[8] += a.slot_c[p];
+            v[8] += Cs;
         }
         const float t = wave_sums9(sl, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]);
         if (lane < 9) {
@@ -279,10 +291,12 @@ __global__ __launch_bounds__(256) void batch_backward_kernel(BatchBackwardArgs a
         } else {
             const uint32_t o = a.offsets[r];
             for (uint32_t k = o; k < o + c; k++) {
-                const float4 A = a.slot_a[k], B = a.slot_b[k];
+                float4 A, B;
+                float Cs;
+                load_slot(a.slot_a, a.slot_b, a.slot_c, a.slot_f, k, A, B, Cs);
                 vxy[0] += A.x; vxy[1] += A.y; vcn[0] += A.z; vcn[1] += A.w;
                 vcn[2] += B.x; vc[0] += B.y; vc[1] += B.z; vc[2] += B.w;
-                vo += a.slot_c[k];
+                vo += Cs;
             }
         }
         const BatchView &bv = a.tab->views[v];
@@ -361,7 +375,7 @@ __global__ __launch_bounds__(256) void batch_adam_kernel(float *p, float *m1, fl
 
 void launch_big_reduce_records(const uint32_t *big_list, const uint32_t *big_count, const uint32_t *offsets,
                                const uint32_t *count, const float4 *slot_a, const float4 *slot_b, const float *slot_c,
-                               float4 *big_a, float4 *big_b, float *big_c, hipStream_t s) {
+                               const uint8_t *slot_f, float4 *big_a, float4 *big_b, float *big_c, hipStream_t s) {
     BackwardArgs a;
     memset(&a, 0, sizeof a);
     a.big_list = big_list;
@@ -371,6 +385,7 @@ void launch_big_reduce_records(const uint32_t *big_list, const uint32_t *big_cou
     a.slot_a = slot_a;
     a.slot_b = slot_b;
     a.slot_c = slot_c;
+    a.slot_f = slot_f;
     a.big_a = big_a;
     a.big_b = big_b;
     a.big_c = big_c;

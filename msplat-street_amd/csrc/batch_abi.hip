@@ -24,7 +24,7 @@ struct BatchState {
     // visible records sorted by depth, count sums in depth order, scan partials
     Buf dk0, dk1, dv0, dv1, ord_sums, scan_part, loff;
     // per intersection
-    Buf keys0, keys1, vals0, vals1, slot_a, slot_b, slot_c, hist, digit_total;
+    Buf keys0, keys1, vals0, vals1, slot_a, slot_b, slot_c, slot_f, hist, digit_total;
     // per tile, pixel, row
     Buf tile_start, image, final_T, final_idx, v_image, row_loss, loss_scratch;
     // summed gradients (per gaussian), visibility in any view
@@ -57,7 +57,7 @@ void destroy_batch_state(BatchState *b) {
     Buf *all[] = {&b->loff, &b->block_vis, &b->dk0, &b->dk1, &b->dv0, &b->dv1, &b->ord_sums, &b->scan_part,
                   &b->splat_a, &b->splat_b, &b->splat_blue, &b->depths,   &b->rect,     &b->radii,      &b->count,
                   &b->offsets, &b->block_sums, &b->big_list, &b->big_sums, &b->keys0, &b->keys1,    &b->vals0,
-                  &b->vals1,   &b->slot_a,     &b->slot_b,   &b->slot_c,   &b->hist,  &b->digit_total, &b->tile_start,
+                  &b->vals1,   &b->slot_a,     &b->slot_b,   &b->slot_c,   &b->slot_f, &b->hist,  &b->digit_total, &b->tile_start,
                   &b->image,   &b->final_T,    &b->final_idx, &b->v_image, &b->row_loss, &b->loss_scratch, &b->grads,
                   &b->vis_any, &b->table,      &b->scalars};
     for (Buf *x : all)
@@ -249,6 +249,7 @@ int grow_isects(msplat_hip_ctx *ctx, BatchState *b, uint64_t need) {
     CHECKED(grow(ctx, b, b->slot_a, 16 * C));
     CHECKED(grow(ctx, b, b->slot_b, 16 * C));
     CHECKED(grow(ctx, b, b->slot_c, 4 * C));
+    CHECKED(grow(ctx, b, b->slot_f, C));
     b->cap = (uint32_t)C;
     return grow_sort_scratch(ctx, b);
 }
@@ -299,6 +300,7 @@ BatchRasterArgs raster_args(BatchState *b) {
     r.slot_a = (float4 *)b->slot_a.p;
     r.slot_b = (float4 *)b->slot_b.p;
     r.slot_c = (float *)b->slot_c.p;
+    r.slot_f = (uint8_t *)b->slot_f.p;
     return r;
 }
 
@@ -396,7 +398,7 @@ void enqueue_backward(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gauss
     float4 *ba = big_a(b);
     launch_big_reduce_records((const uint32_t *)b->big_list.p, &dsc(b)->big_count, (const uint32_t *)b->offsets.p,
                               (const uint32_t *)b->count.p, (const float4 *)b->slot_a.p, (const float4 *)b->slot_b.p,
-                              (const float *)b->slot_c.p, ba, ba + R, (float *)(ba + 2 * R), s);
+                              (const float *)b->slot_c.p, (const uint8_t *)b->slot_f.p, ba, ba + R, (float *)(ba + 2 * R), s);
     BatchBackwardArgs a;
     a.tab = dtab(b);
     a.n = g->num_points;
@@ -412,6 +414,7 @@ void enqueue_backward(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gauss
     a.slot_a = (const float4 *)b->slot_a.p;
     a.slot_b = (const float4 *)b->slot_b.p;
     a.slot_c = (const float *)b->slot_c.p;
+    a.slot_f = (const uint8_t *)b->slot_f.p;
     a.big_a = ba;
     a.big_b = ba + R;
     a.big_c = (const float *)(ba + 2 * R);

@@ -77,7 +77,7 @@ struct msplat_hip_ctx {
     Buf splat_a, splat_b, splat_blue, depths, aabb, rect, radii, count, offsets, block_sums;
     Buf v_xy, v_conic, v_rgb, v_opac, sh_scratch, big;
     // per intersection
-    Buf keys0, keys1, vals0, vals1, slot_a, slot_b, slot_c, hist, digit_total, keys64;
+    Buf keys0, keys1, vals0, vals1, slot_a, slot_b, slot_c, slot_f, hist, digit_total, keys64;
     // visible gaussians sorted by depth, per-block counts and scan partials
     Buf block_vis, dk0, dk1, dv0, dv1, ord_sums, scan_part, loff;
     // per tile / pixel
@@ -180,7 +180,7 @@ struct msplat_hip_ctx {
     ~msplat_hip_ctx() {
         if (stream) (void)hipStreamSynchronize(stream);
         Buf *all[] = {&splat_a, &splat_b, &splat_blue, &depths, &aabb, &rect, &radii, &count, &offsets, &block_sums,
-                      &v_xy, &v_conic, &v_rgb, &v_opac, &sh_scratch, &big, &keys0, &keys1, &vals0, &vals1, &slot_a, &slot_b, &slot_c,
+                      &v_xy, &v_conic, &v_rgb, &v_opac, &sh_scratch, &big, &keys0, &keys1, &vals0, &vals1, &slot_a, &slot_b, &slot_c, &slot_f,
                       &hist, &digit_total, &keys64, &block_vis, &dk0, &dk1, &dv0, &dv1, &ord_sums, &scan_part, &loff, &tile_start, &image, &final_T, &final_idx, &v_image,
                       &row_loss, &loss_scratch, &scalars, &grad_scratch, &aux};
         for (Buf *b : all) release(*b);

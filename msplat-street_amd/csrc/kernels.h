@@ -92,6 +92,7 @@ struct RasterArgs {
     const uint32_t *offsets;
     float4 *slot_a, *slot_b;
     float *slot_c;
+    uint8_t *slot_f;  // per slot: 1 = written, 0 = all nine values zero (left unwritten)
     int cull;  // set by the launchers from Tuning::cull
 };
 
@@ -125,6 +126,7 @@ struct BackwardArgs {
     const float4 *splat_a, *splat_b;
     const float4 *slot_a, *slot_b;
     const float *slot_c;
+    const uint8_t *slot_f;  // slots with a zero flag are zero and not read
     // Adam mode
     float *m1[6], *m2[6];
     float step[6], bc2[6];
@@ -259,6 +261,7 @@ struct BatchBackwardArgs {
     const float4 *splat_a, *splat_b;
     const float4 *slot_a, *slot_b;
     const float *slot_c;
+    const uint8_t *slot_f;  // slots with a zero flag are zero and not read
     const float4 *big_a, *big_b;  // slot sums of records with count > BIG_SLOTS
     const float *big_c;
     const BatchScalars *sc;
@@ -285,6 +288,7 @@ struct BatchRasterArgs {
     const float *v_image;
     float4 *slot_a, *slot_b;
     float *slot_c;
+    uint8_t *slot_f;  // per slot: 1 = written, 0 = all nine values zero (left unwritten)
     int cull;  // set by the launchers from Tuning::cull
 };
 int batch_raster_grid(bool backward, const Tuning &t);  // persistent workgroups that fill the device
@@ -293,7 +297,7 @@ void launch_batch_raster_backward(const BatchRasterArgs &a, const Tuning &t, hip
 void launch_batch_loss(const BatchLossArgs &a, hipStream_t s);
 void launch_big_reduce_records(const uint32_t *big_list, const uint32_t *big_count, const uint32_t *offsets,
                                const uint32_t *count, const float4 *slot_a, const float4 *slot_b, const float *slot_c,
-                               float4 *big_a, float4 *big_b, float *big_c, hipStream_t s);
+                               const uint8_t *slot_f, float4 *big_a, float4 *big_b, float *big_c, hipStream_t s);
 void launch_batch_backward(const BatchBackwardArgs &a, hipStream_t s);
 // Adam over one parameter group from summed gradients; `vis` (per gaussian, null =
 // dense) restricts it to gaussians visible in some view, and for the SH rest group to

@@ -40,6 +40,7 @@ struct TileData {
     const uint32_t *offsets;
     float4 *slot_a, *slot_b;
     float *slot_c;
+    uint8_t *slot_f;  // 1: the slot's nine values were written; 0: they are all zero
     bool cull;  // skip gaussians that splat_misses_tile proves invisible in the tile
 };
 
@@ -203,6 +204,11 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
 // (wave_sums9) and lanes 0..8 write the wave's totals to LDS, to be added over the waves
 // in fixed order when the batch's slots are written. One partial per wave keeps the LDS
 // footprint small enough for several waves per SIMD.
+// A slot whose nine values are all zero (a record culled from the tile, behind every
+// pixel's last contributor, or valid in no pixel) only gets its flag cleared: its
+// floats are left as they are, and the slot sums skip it. Skipping a zero term changes
+// no sum: the sums start at +0, and adding +0 or -0 to +0 or to a nonzero value returns
+// it unchanged.
 constexpr int NQ = 9;  // sum vs, vs dx, vs dy, vs dx^2, vs dx dy, vs dy^2, fac vr, fac vg, fac vb
 
 template <int PPT, int BATCH>
@@ -214,6 +220,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
     __shared__ uint32_t lslot[BATCH];
     __shared__ int lk[BATCH];       // the staged record's position in the tile's list
     __shared__ uint8_t lmask[BATCH];  // bit c: raw colour c + 0.5 >= 0 (its gradient passes)
+    __shared__ uint8_t ltouch[BATCH];  // some wave had a valid pair with the record
     __shared__ float part[NW][BATCH][NQ];
     __shared__ int smax[NW];
     __shared__ int wcount[2][NW];
@@ -258,10 +265,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
         const int g = d.gid[k];
         const uint2 r = d.rect[g];
         const int rx0 = r.x & 0xFFFF, rx1 = r.x >> 16, ry0 = r.y & 0xFFFF;
-        const uint32_t p = d.offsets[g] + (uint32_t)((ty - ry0) * (rx1 - rx0) + (tx - rx0));
-        d.slot_a[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-        d.slot_b[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-        d.slot_c[p] = 0.f;
+        d.slot_f[d.offsets[g] + (uint32_t)((ty - ry0) * (rx1 - rx0) + (tx - rx0))] = 0;
     }
 
     for (int end = kmax + 1; end > s; end -= BATCH) {
@@ -286,11 +290,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
                 const int rx0 = r.x & 0xFFFF, rx1 = r.x >> 16, ry0 = r.y & 0xFFFF;
                 p = d.offsets[g] + (uint32_t)((ty - ry0) * (rx1 - rx0) + (tx - rx0));
                 keep = !(d.cull && splat_misses_tile(A0, B0, x0, y0));
-                if (!keep) {
-                    d.slot_a[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    d.slot_b[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    d.slot_c[p] = 0.f;
-                }
+                if (!keep) d.slot_f[p] = 0;
             }
             int total;
             const int pos = kept + kept_position<NW>(keep, wcount[(c / NT) & 1], total);
@@ -302,6 +302,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
                                        (C0 + 0.5f >= 0.f ? 4 : 0));
                 lslot[pos] = p;
                 lk[pos] = start + i;
+                ltouch[pos] = 0;
             }
             kept += total;
         }
@@ -345,10 +346,16 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             if (__ballot(contrib) != 0) {
                 const float t = wave_sums9(sl, q0, q1, q2, q3, q4, q5, q6, q7, q8);
                 if (lane < NQ) part[w][j][sl.q] = t;
+                if (lane == 0) ltouch[j] = 1;
             }
         }
         __syncthreads();
         for (int j = threadIdx.x; j < kept; j += NT) {
+            const uint32_t p = lslot[j];
+            if (!ltouch[j]) {  // every wave's partials are zero
+                d.slot_f[p] = 0;
+                continue;
+            }
             float q[NQ];
 #pragma unroll
             for (int i = 0; i < NQ; i++) {
@@ -361,10 +368,10 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             const float ox = A.z * q[1] + A.w * q[2], oy = A.w * q[1] + B.x * q[2];
             const uint32_t cm = lmask[j];
             const bool mr = cm & 1u, mg = cm & 2u, mb = cm & 4u;
-            const uint32_t p = lslot[j];
             d.slot_a[p] = make_float4(ox, oy, 0.5f * q[3], 0.5f * q[4]);
             d.slot_b[p] = make_float4(0.5f * q[5], mr ? q[6] : 0.f, mg ? q[7] : 0.f, mb ? q[8] : 0.f);
             d.slot_c[p] = -(1.f - B.y) * q[0];
+            d.slot_f[p] = 1;
         }
     }
 }
