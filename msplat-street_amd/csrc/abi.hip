@@ -169,10 +169,13 @@ int do_bin(msplat_hip_ctx *ctx) {
     CHECKED(ctx->grow(ctx->vals0, 4 * Mc));
     CHECKED(ctx->grow(ctx->vals1, 4 * Mc));
     CHECKED(ctx->grow(ctx->hist, 4 * radix_hist_words((uint32_t)std::max(Mc, Nc))));
+    CHECKED(ctx->grow(ctx->slot_f, (Mc + 3) & ~(size_t)3));
     k.m_max = (uint32_t)M;
     k.keys = (uint32_t *)ctx->keys0.p;
     k.vals = (int32_t *)ctx->vals0.p;
+    k.slot_f = (uint8_t *)ctx->slot_f.p;
     launch_keygen_emit(k, false, s);
+    ctx->slot_f_clean = true;
     uint32_t *kin = (uint32_t *)ctx->keys0.p, *kout = (uint32_t *)ctx->keys1.p;
     int32_t *vin = (int32_t *)ctx->vals0.p, *vout = (int32_t *)ctx->vals1.p;
     const int bits = tile_bits(ctx->ntiles);
@@ -226,6 +229,7 @@ RasterArgs raster_args(msplat_hip_ctx *ctx) {
     r.slot_b = (float4 *)ctx->slot_b.p;
     r.slot_c = (float *)ctx->slot_c.p;
     r.slot_f = (uint8_t *)ctx->slot_f.p;
+    r.flags_zero = 0;
     return r;
 }
 
@@ -265,10 +269,20 @@ int do_raster_backward(msplat_hip_ctx *ctx) {
     CHECKED(ctx->grow(ctx->slot_a, 16 * Mc));
     CHECKED(ctx->grow(ctx->slot_b, 16 * Mc));
     CHECKED(ctx->grow(ctx->slot_c, 4 * Mc));
-    CHECKED(ctx->grow(ctx->slot_f, Mc));
+    CHECKED(ctx->grow(ctx->slot_f, (Mc + 3) & ~(size_t)3));
+    // the zero flags of the frame's slots: cleared by its key emission, or here when an
+    // earlier backward of the same frame has set some
+    if (!ctx->slot_f_clean) {
+        const size_t words = (Mc + 3) / 4;
+        zero_u32<<<(unsigned)std::min<size_t>((words + 255) / 256, 2048), 256, 0, ctx->stream>>>(
+            (uint32_t *)ctx->slot_f.p, words);
+    }
+    RasterArgs r = raster_args(ctx);
+    r.flags_zero = 1;
     ctx->begin(4);
-    launch_raster_backward(raster_args(ctx), ctx->tuning, ctx->stream);
+    launch_raster_backward(r, ctx->tuning, ctx->stream);
     ctx->end();
+    ctx->slot_f_clean = false;
     HIPCHECK(hipGetLastError());
     return MSPLAT_OK;
 }

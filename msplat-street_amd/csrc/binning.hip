@@ -539,6 +539,7 @@ __global__ __launch_bounds__(256) void keygen_emit_kernel(KeygenArgs a) {
         const uint32_t tiles_x = BATCH ? (uint32_t)a.tab->views[r / a.n].cam.tiles_x : (uint32_t)a.tiles_x;
         a.keys[o] = (uint32_t)ty * tiles_x + (uint32_t)tx;
         a.vals[o] = r;
+        if (a.slot_f) a.slot_f[o] = 0;  // output o is the record's slot o (emission order)
     }
 }
 

@@ -170,6 +170,11 @@ __device__ inline void rand_normal3(uint32_t seed, uint32_t stream, float *out) 
 
 __global__ inline void fill_u32(uint32_t *p, uint32_t v) { *p = v; }
 
+// p[0, n) = 0, grid-stride
+__global__ inline void zero_u32(uint32_t *p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0u;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8,
 // each XCD with its own L2). This maps workgroup b to item t so that every XCD walks its
 // own contiguous chunk of [0, n) in order: neighbouring tiles (which share gaussians and

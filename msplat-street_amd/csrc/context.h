@@ -152,6 +152,7 @@ struct msplat_hip_ctx {
     uint32_t width = 0, height = 0;
     int tiles_x = 0, tiles_y = 0;
     int64_t M = 0;
+    bool slot_f_clean = false;  // slot_f[0, M) is all zero: set by the key emission, cleared by a raster backward
     uint32_t *keys_sorted = nullptr;  // tile ids of the sorted intersections
     int32_t *gid_sorted = nullptr;
     float bg[3] = {0, 0, 0};

@@ -70,6 +70,7 @@ struct KeygenArgs {
     uint32_t *keys;  // global tile id
     int32_t *vals;   // record id
     uint32_t *big_list, *big_count;  // records with count > BIG_SLOTS
+    uint8_t *slot_f;                 // null, or the slots' zero flags, cleared by the emission
     const uint32_t *overflow;        // null, or skip everything when set
     int tiles_x;                     // one view
     const BatchTable *tab;           // batch: record r is in view r / n
@@ -93,6 +94,7 @@ struct RasterArgs {
     float4 *slot_a, *slot_b;
     float *slot_c;
     uint8_t *slot_f;  // per slot: 1 = written, 0 = all nine values zero (left unwritten)
+    int flags_zero;   // every slot_f entry is 0 on entry: only nonzero slots are flagged
     int cull;  // set by the launchers from Tuning::cull
 };
 

@@ -41,6 +41,7 @@ struct TileData {
     float4 *slot_a, *slot_b;
     float *slot_c;
     uint8_t *slot_f;  // 1: the slot's nine values were written; 0: they are all zero
+    bool flags_zero;  // slot_f is all zero on entry, so zero slots need no write
     bool cull;  // skip gaussians that splat_misses_tile proves invisible in the tile
 };
 
@@ -261,7 +262,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
     const float fx = (float)px;
 
     // slots of intersections behind every pixel's last contributor are zero
-    for (int k = kmax + 1 + threadIdx.x; k < e; k += NT) {
+    for (int k = kmax + 1 + threadIdx.x; !d.flags_zero && k < e; k += NT) {
         const int g = d.gid[k];
         const uint2 r = d.rect[g];
         const int rx0 = r.x & 0xFFFF, rx1 = r.x >> 16, ry0 = r.y & 0xFFFF;
@@ -286,11 +287,13 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
                 A0 = d.splat_a[g];
                 B0 = d.splat_b[g];
                 C0 = d.splat_blue[g];
-                const uint2 r = d.rect[g];
-                const int rx0 = r.x & 0xFFFF, rx1 = r.x >> 16, ry0 = r.y & 0xFFFF;
-                p = d.offsets[g] + (uint32_t)((ty - ry0) * (rx1 - rx0) + (tx - rx0));
                 keep = !(d.cull && splat_misses_tile(A0, B0, x0, y0));
-                if (!keep) d.slot_f[p] = 0;
+                if (keep || !d.flags_zero) {
+                    const uint2 r = d.rect[g];
+                    const int rx0 = r.x & 0xFFFF, rx1 = r.x >> 16, ry0 = r.y & 0xFFFF;
+                    p = d.offsets[g] + (uint32_t)((ty - ry0) * (rx1 - rx0) + (tx - rx0));
+                    if (!keep) d.slot_f[p] = 0;
+                }
             }
             int total;
             const int pos = kept + kept_position<NW>(keep, wcount[(c / NT) & 1], total);
@@ -353,7 +356,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
         for (int j = threadIdx.x; j < kept; j += NT) {
             const uint32_t p = lslot[j];
             if (!ltouch[j]) {  // every wave's partials are zero
-                d.slot_f[p] = 0;
+                if (!d.flags_zero) d.slot_f[p] = 0;
                 continue;
             }
             float q[NQ];

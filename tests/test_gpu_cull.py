@@ -97,3 +97,30 @@ def test_cull_changes_no_bit_batched(ctx):
             np.testing.assert_array_equal(res[0][1][k], res[1][1][k], err_msg=k)
     finally:
         ctx.set_tuning(TUNE_CULL, saved)
+
+
+@pytest.mark.gpu
+def test_repeated_backward_of_one_frame(ctx):
+    """A slot left zero by one backward is flagged zero again by the next backward of the
+    same frame: two backward passes of one render, with cotangents zero in different
+    halves of the image, give the gradients of a fresh render and backward each."""
+    from msplat._abi import DeviceArray
+    from msplat.backend import DeviceGaussians, render, render_backward
+    sc, cam = _scene()
+    dg = DeviceGaussians(ctx, sc)
+    rng = np.random.default_rng(5)
+    va = rng.normal(0, 1e-3, (H, W, 3)).astype(np.float32)
+    vb = va.copy()
+    va[:, W // 2:] = 0.0  # the first backward sees the left half only, the second the right half
+    vb[:, : W // 2] = 0.0
+    da, db = DeviceArray.from_numpy(ctx, va), DeviceArray.from_numpy(ctx, vb)
+    fresh = []
+    for d in (da, db):
+        render(ctx, dg, cam, BG)
+        fresh.append(render_backward(ctx, dg, cam, BG, d))
+    render(ctx, dg, cam, BG)
+    again = [render_backward(ctx, dg, cam, BG, d) for d in (da, db, da)]
+    for got, want in zip(again, fresh + fresh[:1]):
+        for k in want:
+            assert np.abs(want[k]).max() > 0, k
+            np.testing.assert_array_equal(got[k], want[k], err_msg=k)
