@@ -41,6 +41,7 @@ struct BatchState {
     uint64_t generation = 0;  // bumped when a buffer moves
     // the last rendered frame
     bool rendered = false;
+    bool slot_f_clean = false;  // slot_f of the last frame is all zero (its key emission ran, no backward yet)
     int frame_n = 0;
     uint32_t frame_w[MAX_VIEWS] = {}, frame_h[MAX_VIEWS] = {};
     int32_t *gid_sorted = nullptr;
@@ -249,7 +250,7 @@ int grow_isects(msplat_hip_ctx *ctx, BatchState *b, uint64_t need) {
     CHECKED(grow(ctx, b, b->slot_a, 16 * C));
     CHECKED(grow(ctx, b, b->slot_b, 16 * C));
     CHECKED(grow(ctx, b, b->slot_c, 4 * C));
-    CHECKED(grow(ctx, b, b->slot_f, C));
+    CHECKED(grow(ctx, b, b->slot_f, (C + 3) & ~(uint64_t)3));
     b->cap = (uint32_t)C;
     return grow_sort_scratch(ctx, b);
 }
@@ -301,6 +302,7 @@ BatchRasterArgs raster_args(BatchState *b) {
     r.slot_b = (float4 *)b->slot_b.p;
     r.slot_c = (float *)b->slot_c.p;
     r.slot_f = (uint8_t *)b->slot_f.p;
+    r.flags_zero = 0;
     return r;
 }
 
@@ -374,6 +376,7 @@ void enqueue_sort_and_forward(msplat_hip_ctx *ctx, BatchState *b, const msplat_h
     k.overflow = &sc->overflow;
     k.tab = dtab(b);
     k.n = g->num_points;
+    k.slot_f = (uint8_t *)b->slot_f.p;
     launch_keygen(k, true, s);
     uint32_t *kin = (uint32_t *)b->keys0.p, *kout = (uint32_t *)b->keys1.p;
     int32_t *vin = (int32_t *)b->vals0.p, *vout = (int32_t *)b->vals1.p;
@@ -389,11 +392,19 @@ void enqueue_sort_and_forward(msplat_hip_ctx *ctx, BatchState *b, const msplat_h
     launch_batch_raster_forward(raster_args(b), ctx->tuning, s);
 }
 
-// raster backward, big-list sums, per-gaussian sums over the views
+// raster backward, big-list sums, per-gaussian sums over the views. The slots' zero flags
+// are all zero after the frame's key emission; a later backward of the same frame
+// (clear_flags) zeroes them first.
 void enqueue_backward(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussians *g, float *const gout[6],
-                      const msplat_hip_grad_stats *stats) {
+                      const msplat_hip_grad_stats *stats, bool clear_flags) {
     hipStream_t s = ctx->stream;
-    launch_batch_raster_backward(raster_args(b), ctx->tuning, s);
+    if (clear_flags) {
+        const size_t words = (b->cap + 3) / 4;
+        if (words) zero_u32<<<(unsigned)std::min<size_t>((words + 255) / 256, 2048), 256, 0, s>>>((uint32_t *)b->slot_f.p, words);
+    }
+    BatchRasterArgs r = raster_args(b);
+    r.flags_zero = 1;
+    launch_batch_raster_backward(r, ctx->tuning, s);
     const size_t R = record_count(b);
     float4 *ba = big_a(b);
     launch_big_reduce_records((const uint32_t *)b->big_list.p, &dsc(b)->big_count, (const uint32_t *)b->offsets.p,
@@ -466,7 +477,7 @@ void enqueue_train(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gaussian
     l.row_loss = (double *)b->row_loss.p;
     l.sc = dsc(b);
     launch_batch_loss(l, s);
-    enqueue_backward(ctx, b, g, gr, stats);
+    enqueue_backward(ctx, b, g, gr, stats, false);
     const size_t N = (size_t)g->num_points;
     const uint32_t stride = (sh_bases(b->h_tab->views[0].cam.degree) - 1) * 3;
     float *params[6] = {g->means, g->scales, g->quats, g->features_dc, g->features_rest, g->opacities};
@@ -589,6 +600,7 @@ int msplat_hip_render_batch(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, 
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     b->rendered = true;
+    b->slot_f_clean = true;
     b->frame_n = g->num_points;
     for (int v = 0; v < views->num_views; v++) {
         b->frame_w[v] = views->cameras[v].width;
@@ -643,7 +655,8 @@ int msplat_hip_render_backward_batch(msplat_hip_ctx *ctx, const msplat_hip_gauss
                       out->v_opacities};
     float *gr[6];
     for (int i = 0; i < 6; i++) gr[i] = user[i] ? user[i] : scratch[i];
-    enqueue_backward(ctx, b, g, gr, nullptr);
+    enqueue_backward(ctx, b, g, gr, nullptr, !b->slot_f_clean);
+    b->slot_f_clean = false;
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     return MSPLAT_OK;

@@ -291,6 +291,7 @@ struct BatchRasterArgs {
     float4 *slot_a, *slot_b;
     float *slot_c;
     uint8_t *slot_f;  // per slot: 1 = written, 0 = all nine values zero (left unwritten)
+    int flags_zero;   // every slot_f entry is 0 on entry: only nonzero slots are flagged
     int cull;  // set by the launchers from Tuning::cull
 };
 int batch_raster_grid(bool backward, const Tuning &t);  // persistent workgroups that fill the device

@@ -68,6 +68,33 @@ def test_render_backward_batch_is_the_sum_of_views(ctx):
         print(f"{k}: max error {err:.2e} of the largest")
 
 
+
+def test_repeated_backward_batch_of_one_frame(ctx):
+    """Two backward passes of one batched render, with cotangents zero in different
+    views, give the gradients of a fresh render and backward each (the second pass
+    re-flags the slots the first one wrote)."""
+    from msplat._abi import DeviceArray
+    from msplat.backend import BatchViews, DeviceGaussians, render_backward_batch, render_batch
+    sc, cams = _setup(seed=7)
+    dg = DeviceGaussians(ctx, sc)
+    bv = BatchViews(cams, 3, [BG])
+    rng = np.random.default_rng(8)
+    v = [(rng.normal(0, 1, (c["H"], c["W"], 3)) * 1e-3).astype(np.float32) for c in cams]
+    half = len(cams) // 2
+    va = np.concatenate([(x if i < half else 0 * x).reshape(-1, 3) for i, x in enumerate(v)])
+    vb = np.concatenate([(0 * x if i < half else x).reshape(-1, 3) for i, x in enumerate(v)])
+    da, db = DeviceArray.from_numpy(ctx, va), DeviceArray.from_numpy(ctx, vb)
+    fresh = []
+    for d in (da, db):
+        render_batch(ctx, dg, bv)
+        fresh.append(render_backward_batch(ctx, dg, bv, d))
+    render_batch(ctx, dg, bv)
+    again = [render_backward_batch(ctx, dg, bv, d) for d in (da, db, da)]
+    for got, want in zip(again, fresh + fresh[:1]):
+        for k in want:
+            assert np.abs(want[k]).max() > 0, k
+            np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+
 def _state(ctx, sc):
     from msplat.backend import DeviceGaussians
     return DeviceGaussians(ctx, sc)
