@@ -254,17 +254,21 @@ __device__ inline int wave_min_i(int v) {
 // Row y's loss sum as the reference forms it: a double accumulated over the row's
 // pixel losses in x order. When every partial sum is exactly representable
 // (sums_exact), each of the reference's additions is exact, so its result is the exact
-// sum, which any order of exact additions also gives: the 64 lanes add strided values
-// and the wave adds the lane sums. Otherwise lane 0 adds the values in order. Writes
-// the sum, the absolute sum and the ulp exponent (EXACT_FAIL when the row was not
-// exact) for the total. One wave per row.
+// sum, which any order of exact additions also gives: the 256 threads add strided
+// values, each wave adds its lanes' sums and thread 0 adds the waves'. Otherwise
+// thread 0 adds the values in order. Writes the sum, the absolute sum and the ulp
+// exponent (EXACT_FAIL when the row was not exact) for the total. One workgroup of
+// ROW_THREADS per row.
+constexpr int ROW_THREADS = 256;
 __device__ inline void row_sum(const LossArgs &a, int y, double *out) {
     __shared__ float buf[4096];
-    const int W = (int)a.width, lane = threadIdx.x;
+    __shared__ double ws[ROW_THREADS / 64], was[ROW_THREADS / 64];
+    __shared__ int we[ROW_THREADS / 64];
+    const int W = (int)a.width, t = threadIdx.x;
     const size_t P = (size_t)W * a.height, base = (size_t)y * W;
     double s = 0.0, as = 0.0;
     int emin = EXACT_NONE;
-    for (int x = lane; x < W; x += 64) {
+    for (int x = t; x < W; x += ROW_THREADS) {
         const float v = pixel_loss(a.scratch, P, base + x, a.ssim_weight);
         s += (double)v;
         as += fabs((double)v);
@@ -274,22 +278,32 @@ __device__ inline void row_sum(const LossArgs &a, int y, double *out) {
     s = wave_sum_d(s);
     as = wave_sum_d(as);
     emin = wave_min_i(emin);
-    if (!sums_exact(emin, as)) {  // uniform over the wave: lane 0 adds in x order
-        __syncthreads();             // the other lanes' values are in buf
-        s = 0.0;
-        if (lane == 0)
+    if ((t & 63) == 0) {
+        ws[t >> 6] = s;
+        was[t >> 6] = as;
+        we[t >> 6] = emin;
+    }
+    __syncthreads();  // the wave sums, and every thread's values in buf
+    if (t == 0) {
+        s = ws[0];
+        as = was[0];
+        emin = we[0];
+        for (int i = 1; i < ROW_THREADS / 64; i++) {
+            s += ws[i];
+            as += was[i];
+            emin = min(emin, we[i]);
+        }
+        if (!sums_exact(emin, as)) {  // thread 0 adds in x order
+            s = 0.0;
             for (int x = 0; x < W; x++)
                 s += (double)(x < 4096 ? buf[x] : pixel_loss(a.scratch, P, base + x, a.ssim_weight));
-        emin = EXACT_FAIL;
-    }
-    if (lane == 0) {
+            emin = EXACT_FAIL;
+        }
         out[3 * y] = s;
         out[3 * y + 1] = as;
         out[3 * y + 2] = (double)emin;
     }
 }
-
-__global__ __launch_bounds__(64) void loss_rows_kernel(LossArgs a) { row_sum(a, blockIdx.x, a.row_loss); }
 
 // The sum of the row sums as the reference forms it (a double accumulated in y order):
 // exact, and so computed as a tree, when every row was exact and the rows together
@@ -327,6 +341,8 @@ __device__ inline void total_sum(const double *rows, int H, size_t npix, double 
         *mean = (float)(s / (double)npix);
     }
 }
+
+__global__ __launch_bounds__(ROW_THREADS) void loss_rows_kernel(LossArgs a) { row_sum(a, blockIdx.x, a.row_loss); }
 
 __global__ __launch_bounds__(256) void loss_total_kernel(const double *rows, int H, int W, double *sum, float *mean) {
     total_sum(rows, H, (size_t)W * H, sum, mean);
@@ -392,7 +408,7 @@ __global__ __launch_bounds__(LB_THREADS) void ssim_vb_batch_kernel(BatchLossArgs
     if (batch_loss_block(b, a, X0, Y0)) ssim_vb_block(a, X0, Y0, blockIdx.y);
 }
 
-__global__ __launch_bounds__(64) void loss_rows_batch_kernel(BatchLossArgs b) {
+__global__ __launch_bounds__(ROW_THREADS) void loss_rows_batch_kernel(BatchLossArgs b) {
     const uint32_t q = blockIdx.x;
     if (q >= b.tab->total_rows) return;
     const int v = view_of_base(b.tab->row_base, b.num_views, q);
@@ -413,7 +429,7 @@ void launch_batch_loss(const BatchLossArgs &b, hipStream_t s) {
     ssim_h_batch_kernel<<<dim3(b.max_loss_blocks, LOSS_BLOCK_ROWS), LB_THREADS, 0, s>>>(b);
     ssim_v_batch_kernel<<<dim3(b.max_loss_blocks, 3), LB_THREADS, 0, s>>>(b);
     ssim_vb_batch_kernel<<<dim3(b.max_loss_blocks, 3), LB_THREADS, 0, s>>>(b);
-    loss_rows_batch_kernel<<<b.max_rows, 64, 0, s>>>(b);
+    loss_rows_batch_kernel<<<b.max_rows, ROW_THREADS, 0, s>>>(b);
     loss_total_batch_kernel<<<b.num_views, 256, 0, s>>>(b);
 }
 
@@ -424,7 +440,7 @@ void launch_loss(const LossArgs &a, hipStream_t s) {
     ssim_h_kernel<<<dim3(nbx, a.height), LB_THREADS, 0, s>>>(a);
     ssim_v_kernel<<<dim3(nbx, nby, 3), LB_THREADS, 0, s>>>(a);
     ssim_vb_kernel<<<dim3(nbx, nby, 3), LB_THREADS, 0, s>>>(a);
-    loss_rows_kernel<<<a.height, 64, 0, s>>>(a);
+    loss_rows_kernel<<<a.height, ROW_THREADS, 0, s>>>(a);
     loss_total_kernel<<<1, 256, 0, s>>>(a.row_loss, a.height, a.width, a.loss_sum, a.loss_mean);
 }
 
