@@ -28,17 +28,62 @@ __device__ inline void adam_step(float &p, float &m1, float &m2, float g, float 
 
 constexpr int BIG_REDUCE_GRID = 2048;  // 8 waves per SIMD
 
-// Slot p's nine values, or zeros when its flag says they are all zero (not read)
-__device__ inline void load_slot(const float4 *sa, const float4 *sb, const float *sc, const uint8_t *sf, uint32_t p,
-                                 float4 &A, float4 &B, float &C) {
-    if (sf[p]) {
-        A = sa[p];
-        B = sb[p];
-        C = sc[p];
-    } else {
-        A = make_float4(0.f, 0.f, 0.f, 0.f);
-        B = A;
-        C = 0.f;
+// The zero flags of the c <= BIG_SLOTS slots from o as a mask (bit k: slot o + k was
+// written), from the at most nine aligned words that hold them. The words also hold
+// bytes outside the range (other records' flags, or never-written bytes past the
+// frame's last slot), so each byte is tested for nonzero on its own (bit 7 of
+// ((x & 0x7F) + 0x7F) | x, no carry between bytes) before the four bits are gathered.
+// The flag buffers are allocated in whole words.
+__device__ inline uint64_t slot_mask(const uint8_t *sf, uint32_t o, uint32_t c) {
+    if (c == 0) return 0;  // no slots (and o may be stale)
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(sf) + (o >> 2);
+    const uint32_t nw = ((o & 3u) + c + 3u) >> 2;
+    uint64_t m = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < (BIG_SLOTS + 3 + 3) / 4; j++) {
+        if (j < nw) {
+            const uint32_t v = w[j];
+            const uint32_t nz = ((((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u) >> 7;
+            m |= (uint64_t)((nz | (nz >> 7) | (nz >> 14) | (nz >> 21)) & 0xFu) << (4 * j);
+        }
+    }
+    return (m >> (o & 3u)) & ((1ull << c) - 1ull);
+}
+
+// Sum of the written slots among o + (set bits of m), in increasing order, four slots'
+// loads in flight at a time (padding adds zeros). The sum equals the plain in-order sum
+// over every slot of the range: the others are zero.
+__device__ inline void sum_slots(const float4 *sa, const float4 *sb, const float *sc, uint32_t o, uint64_t m,
+                                 float v[9]) {
+    while (m) {
+        uint32_t k[4];
+        bool h[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            h[u] = m != 0;
+            k[u] = h[u] ? (uint32_t)__builtin_ctzll(m) : 0u;
+            m &= m - 1ull;
+        }
+        float4 A[4], B[4];
+        float Cs[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (h[u]) {
+                A[u] = sa[o + k[u]];
+                B[u] = sb[o + k[u]];
+                Cs[u] = sc[o + k[u]];
+            } else {
+                A[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                B[u] = A[u];
+                Cs[u] = 0.f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            v[0] += A[u].x; v[1] += A[u].y; v[2] += A[u].z; v[3] += A[u].w;
+            v[4] += B[u].x; v[5] += B[u].y; v[6] += B[u].z; v[7] += B[u].w;
+            v[8] += Cs[u];
+        }
     }
 }
 
@@ -53,29 +98,13 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
         vxy[0] = A.x; vxy[1] = A.y; vcn[0] = A.z; vcn[1] = A.w;
         vcn[2] = B.x; vrgb[0] = B.y; vrgb[1] = B.z; vrgb[2] = B.w;
         vop = a.big_c[g];
-    } else if (vis) {  // in tile order, four slots' loads in flight at a time
-        const uint32_t o = a.offsets[g], end = o + a.count[g];
-        uint32_t p = o;
-        for (; p + 4 <= end; p += 4) {
-            float4 A[4], B[4];
-            float Cs[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) load_slot(a.slot_a, a.slot_b, a.slot_c, a.slot_f, p + u, A[u], B[u], Cs[u]);
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                vxy[0] += A[u].x; vxy[1] += A[u].y; vcn[0] += A[u].z; vcn[1] += A[u].w;
-                vcn[2] += B[u].x; vrgb[0] += B[u].y; vrgb[1] += B[u].z; vrgb[2] += B[u].w;
-                vop += Cs[u];
-            }
-        }
-        for (; p < end; p++) {
-            float4 A, B;
-            float Cs;
-            load_slot(a.slot_a, a.slot_b, a.slot_c, a.slot_f, p, A, B, Cs);
-            vxy[0] += A.x; vxy[1] += A.y; vcn[0] += A.z; vcn[1] += A.w;
-            vcn[2] += B.x; vrgb[0] += B.y; vrgb[1] += B.z; vrgb[2] += B.w;
-            vop += Cs;
-        }
+    } else if (vis) {  // the written slots, in tile order
+        const uint32_t o = a.offsets[g];
+        float v[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        sum_slots(a.slot_a, a.slot_b, a.slot_c, o, slot_mask(a.slot_f, o, a.count[g]), v);
+        vxy[0] = v[0]; vxy[1] = v[1]; vcn[0] = v[2]; vcn[1] = v[3];
+        vcn[2] = v[4]; vrgb[0] = v[5]; vrgb[1] = v[6]; vrgb[2] = v[7];
+        vop = v[8];
     }
     if (a.v_xy) {
         a.v_xy[2 * g] = vxy[0]; a.v_xy[2 * g + 1] = vxy[1];
@@ -152,24 +181,37 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
 
 // Slot sums of the gaussians with more than BIG_SLOTS intersections, one wave each
 // (grid-stride over the device-side list; every wave exits when the list is done).
-// Lane l adds slots l, l + 64, ... in order, four rounds of loads in flight at a time,
-// and the nine lane sums are added over the wave by wave_sums9.
+// Lane l adds the written slots among l, l + 64, ... in order, in rounds of four slots
+// whose flag and value loads are in flight together; the nine lane sums are added over
+// the wave by wave_sums9. The next record's id is loaded while this one is summed.
 __global__ __launch_bounds__(256) void big_reduce_kernel(BackwardArgs a) {
     const uint32_t nbig = *a.big_count;
     const int lane = threadIdx.x & 63;
     const SumLanes sl(lane);
     const uint32_t wave = (blockIdx.x * 256 + threadIdx.x) >> 6, nwaves = (gridDim.x * 256) >> 6;
+    uint32_t g = wave < nbig ? a.big_list[wave] : 0u;
     for (uint32_t i = wave; i < nbig; i += nwaves) {
-        const uint32_t g = a.big_list[i];
         const uint32_t o = a.offsets[g], end = o + a.count[g];
+        const uint32_t gn = i + nwaves < nbig ? a.big_list[i + nwaves] : 0u;
         float v[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        uint32_t p = o + lane;
-        for (; p + 192 < end; p += 256) {
+        for (uint32_t p0 = o + lane; p0 < end; p0 += 256) {
+            bool f[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) f[u] = p0 + 64 * u < end && a.slot_f[p0 + 64 * u] != 0;
             float4 A[4], B[4];
             float Cs[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-                load_slot(a.slot_a, a.slot_b, a.slot_c, a.slot_f, p + 64 * u, A[u], B[u], Cs[u]);
+            for (int u = 0; u < 4; u++) {
+                if (f[u]) {
+                    A[u] = a.slot_a[p0 + 64 * u];
+                    B[u] = a.slot_b[p0 + 64 * u];
+                    Cs[u] = a.slot_c[p0 + 64 * u];
+                } else {
+                    A[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    B[u] = A[u];
+                    Cs[u] = 0.f;
+                }
+            }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 v[0] += A[u].x; v[1] += A[u].y; v[2] += A[u].z; v[3] += A[u].w;
@@ -177,19 +219,12 @@ __global__ __launch_bounds__(256) void big_reduce_kernel(BackwardArgs a) {
                 v[8] += Cs[u];
             }
         }
-        for (; p < end; p += 64) {
-            float4 A, B;
-            float Cs;
-            load_slot(a.slot_a, a.slot_b, a.slot_c, a.slot_f, p, A, B, Cs);
-            v[0] += A.x; v[1] += A.y; v[2] += A.z; v[3] += A.w;
-            v[4] += B.x; v[5] += B.y; v[6] += B.z; v[7] += B.w;
-            v[8] += Cs;
-        }
         const float t = wave_sums9(sl, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]);
         if (lane < 9) {
             float *dst = sl.q < 4 ? &a.big_a[g].x + sl.q : sl.q < 8 ? &a.big_b[g].x + (sl.q - 4) : &a.big_c[g];
             *dst = t;
         }
+        g = gn;
     }
 }
 
@@ -290,14 +325,11 @@ __global__ __launch_bounds__(256) void batch_backward_kernel(BatchBackwardArgs a
             vo = a.big_c[r];
         } else {
             const uint32_t o = a.offsets[r];
-            for (uint32_t k = o; k < o + c; k++) {
-                float4 A, B;
-                float Cs;
-                load_slot(a.slot_a, a.slot_b, a.slot_c, a.slot_f, k, A, B, Cs);
-                vxy[0] += A.x; vxy[1] += A.y; vcn[0] += A.z; vcn[1] += A.w;
-                vcn[2] += B.x; vc[0] += B.y; vc[1] += B.z; vc[2] += B.w;
-                vo += Cs;
-            }
+            float s9[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            sum_slots(a.slot_a, a.slot_b, a.slot_c, o, slot_mask(a.slot_f, o, c), s9);
+            vxy[0] = s9[0]; vxy[1] = s9[1]; vcn[0] = s9[2]; vcn[1] = s9[3];
+            vcn[2] = s9[4]; vc[0] = s9[5]; vc[1] = s9[6]; vc[2] = s9[7];
+            vo = s9[8];
         }
         const BatchView &bv = a.tab->views[v];
         const msplat_hip_camera &cam = bv.cam;
