@@ -106,10 +106,10 @@ int do_bin(msplat_hip_ctx *ctx) {
     uint32_t *part = (uint32_t *)ctx->scan_part.p;
     const size_t pw = scan_partial_words(std::max(nblocks, vblocks));
     hipStream_t s = ctx->stream;
-    launch_scan_u32((uint32_t *)ctx->block_sums.p, nblocks, part,
-                    ScanOut{ctx->d_total64(), ctx->d_total32(), 0xFFFFFFFFu, nullptr, ctx->d_big_count()}, s);
-    launch_scan_u32((uint32_t *)ctx->block_vis.p, nblocks, part + pw,
-                    ScanOut{ctx->d_vis64(), ctx->d_vis32(), 0xFFFFFFFFu, nullptr, nullptr}, s);
+    launch_scan_u32_pair((uint32_t *)ctx->block_sums.p,
+                         ScanOut{ctx->d_total64(), ctx->d_total32(), 0xFFFFFFFFu, nullptr, ctx->d_big_count()},
+                         (uint32_t *)ctx->block_vis.p, ScanOut{ctx->d_vis64(), ctx->d_vis32(), 0xFFFFFFFFu, nullptr, nullptr},
+                         nblocks, part, part + pw, s);
     ctx->end();
     HIPCHECK(hipMemcpyAsync(ctx->h_total, ctx->scalars.p, 64, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipEventRecord(ctx->counts_ready, s));
@@ -137,8 +137,6 @@ int do_bin(msplat_hip_ctx *ctx) {
     // intersection offsets in depth order
     launch_order_sums(dvi, ctx->d_vis32(), (uint32_t)Nc, (const uint32_t *)ctx->count.p, (uint32_t *)ctx->ord_sums.p,
                       s);
-    launch_scan_u32((uint32_t *)ctx->ord_sums.p, vblocks, part + 2 * pw,
-                    ScanOut{ctx->d_order_total64(), ctx->d_order_total32(), 0xFFFFFFFFu, nullptr, nullptr}, s);
     KeygenArgs k;
     memset(&k, 0, sizeof k);
     k.list = dvi;
@@ -146,7 +144,15 @@ int do_bin(msplat_hip_ctx *ctx) {
     k.nvis_max = (uint32_t)Nc;
     k.count = (const uint32_t *)ctx->count.p;
     k.rect = (const uint2 *)ctx->rect.p;
-    k.block_bases = (const uint32_t *)ctx->ord_sums.p;
+    // up to 4096 chunks the key generation adds the preceding chunk sums itself (a few
+    // loads per thread); past that, one scan launch is cheaper than the quadratic reads
+    if (vblocks <= 4096) {
+        k.chunk_sums = (const uint32_t *)ctx->ord_sums.p;
+    } else {
+        launch_scan_u32((uint32_t *)ctx->ord_sums.p, vblocks, part + 2 * pw,
+                        ScanOut{nullptr, nullptr, 0xFFFFFFFFu, nullptr, nullptr}, s);
+        k.block_bases = (const uint32_t *)ctx->ord_sums.p;
+    }
     k.offsets = (uint32_t *)ctx->offsets.p;
     k.loff = (uint32_t *)ctx->loff.p;
     k.m_ptr = ctx->d_total32();

@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void scan_top_kernel(uint32_t *partial, int nc
 }
 
 // The whole scan in one workgroup, 4096 entries at a time (small arrays)
-__global__ __launch_bounds__(256) void scan_small_kernel(uint32_t *a, uint32_t L, ScanOut o) {
+__device__ inline void scan_small(uint32_t *a, uint32_t L, const ScanOut &o) {
     __shared__ uint32_t lds4[4];
     __shared__ uint64_t carry;
     if (threadIdx.x == 0) carry = 0;
@@ -232,6 +232,15 @@ __global__ __launch_bounds__(256) void scan_small_kernel(uint32_t *a, uint32_t L
         if (o.overflow) *o.overflow = over ? 1u : 0u;
         if (o.zero_also) *o.zero_also = 0u;
     }
+}
+
+__global__ __launch_bounds__(256) void scan_small_kernel(uint32_t *a, uint32_t L, ScanOut o) { scan_small(a, L, o); }
+
+// Two small scans of the same length in one launch, one workgroup each
+__global__ __launch_bounds__(256) void scan_small_pair_kernel(uint32_t *a0, ScanOut o0, uint32_t *a1, ScanOut o1,
+                                                              uint32_t L) {
+    if (blockIdx.x == 0) scan_small(a0, L, o0);
+    else scan_small(a1, L, o1);
 }
 
 __global__ __launch_bounds__(256) void scan_apply_kernel(uint32_t *a, uint32_t L, const uint32_t *partial) {
@@ -462,7 +471,15 @@ __global__ __launch_bounds__(256) void keygen_offsets_kernel(KeygenArgs a) {
         const int32_t r = i < nvis ? a.list[i] : -1;
         const uint32_t c = r >= 0 ? a.count[r] : 0u;
         uint32_t total;
-        const uint32_t off = a.block_bases[chunk] + block_scan_u32(c, total, lds4);
+        uint32_t base;
+        if (a.chunk_sums) {  // the sums of the chunks before this one, added here
+            uint32_t part = 0;
+            for (uint32_t q = threadIdx.x; q < chunk; q += 256) part += a.chunk_sums[q];
+            block_scan_u32(part, base, lds4);
+        } else {
+            base = a.block_bases[chunk];
+        }
+        const uint32_t off = base + block_scan_u32(c, total, lds4);
         if (r >= 0) {
             a.offsets[r] = off;
             a.loff[i] = off;
@@ -665,6 +682,16 @@ void launch_scan_u32(uint32_t *a, uint32_t L, uint32_t *partial, const ScanOut &
     if (nchunks > 0) scan_partial_kernel<<<nchunks, 256, 0, s>>>(a, L, partial);
     scan_top_kernel<<<1, 256, 0, s>>>(partial, nchunks, o);
     if (nchunks > 0) scan_apply_kernel<<<nchunks, 256, 0, s>>>(a, L, partial);
+}
+
+void launch_scan_u32_pair(uint32_t *a0, const ScanOut &o0, uint32_t *a1, const ScanOut &o1, uint32_t L,
+                          uint32_t *partial0, uint32_t *partial1, hipStream_t s) {
+    if (L <= 2 * SCAN_CHUNK) {
+        scan_small_pair_kernel<<<2, 256, 0, s>>>(a0, o0, a1, o1, L);
+        return;
+    }
+    launch_scan_u32(a0, L, partial0, o0, s);
+    launch_scan_u32(a1, L, partial1, o1, s);
 }
 
 void launch_project(const ProjectArgs &a, hipStream_t s) {

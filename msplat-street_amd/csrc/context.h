@@ -143,7 +143,7 @@ struct msplat_hip_ctx {
         marks.swap(open);
     }
     // device scalars: [0] u64 isect total, [8] u32 total, [16] double loss sum, [24] float loss,
-    // [32] u32 big count, [40] u64 / [48] u32 visible gaussians, [56] / [64] the depth-order total
+    // [32] u32 big count, [40] u64 / [48] u32 visible gaussians
     Buf scalars;
     uint64_t *h_total = nullptr;  // pinned
     hipEvent_t counts_ready = nullptr;  // h_total holds this frame's counts
@@ -201,8 +201,6 @@ struct msplat_hip_ctx {
     double *d_loss_sum() { return (double *)((char *)scalars.p + 16); }
     uint64_t *d_vis64() { return (uint64_t *)((char *)scalars.p + 40); }
     uint32_t *d_vis32() { return (uint32_t *)((char *)scalars.p + 48); }
-    uint64_t *d_order_total64() { return (uint64_t *)((char *)scalars.p + 56); }
-    uint32_t *d_order_total32() { return (uint32_t *)((char *)scalars.p + 64); }
     float *d_loss() { return (float *)((char *)scalars.p + 24); }
 };
 

@@ -62,7 +62,8 @@ struct KeygenArgs {
     uint32_t nvis_max;         // grid bound
     const uint32_t *count;
     const uint2 *rect;
-    const uint32_t *block_bases;  // scanned count sums per 256 list entries
+    const uint32_t *block_bases;  // scanned count sums per 256 list entries, or
+    const uint32_t *chunk_sums;   // (when not null) the sums themselves, unscanned
     uint32_t *offsets;
     uint32_t *loff;         // offsets in list order (nvis_max entries)
     const uint32_t *m_ptr;  // device intersection count
@@ -315,6 +316,9 @@ void launch_scan_blocks(uint32_t *sums, int nblocks, uint64_t *total, uint32_t *
                         uint32_t *zero_also = nullptr);
 size_t scan_partial_words(uint32_t L);
 void launch_scan_u32(uint32_t *a, uint32_t L, uint32_t *partial, const ScanOut &o, hipStream_t s);
+// two scans of length L (one launch when L is small)
+void launch_scan_u32_pair(uint32_t *a0, const ScanOut &o0, uint32_t *a1, const ScanOut &o1, uint32_t L,
+                          uint32_t *partial0, uint32_t *partial1, hipStream_t s);
 void launch_compact(const CompactArgs &a, hipStream_t s);
 void launch_order_sums(const int32_t *list, const uint32_t *nvis_dev, uint32_t nvis_max, const uint32_t *count,
                        uint32_t *sums, hipStream_t s);
