@@ -38,11 +38,17 @@ FP32_PEAK_TFLOPS = 157.3
 # Algorithmic bytes per unit (minimum fp32 reads + writes), per kernel class
 BYTES = {
     "rast_fwd": (36, 20, 0),            # per intersection, per pixel, per gaussian
-    "rast_bwd": (40, 28, 40),
+    "rast_bwd": (4, 28, 40),            # + SLOT_BYTES per written gradient slot
     "loss_fwd_bwd": (0, 36, 0),
     "prefix_sort_pack": (24 + 144 + 8, 0, 24),
 }
 DOMINANT = "rast_bwd"  # the train step's longest stage (the raster backward)
+SLOT_BYTES = 37  # a written gradient slot: nine floats and its flag (all-zero slots are only flagged)
+
+
+def slots_written(ctx) -> int:
+    """The context's running count of gradient slots written by the raster backward."""
+    return int(ctx.download(ctx.frame().slots_written, (1,), np.uint64)[0])
 
 
 def log(msg):
@@ -415,6 +421,7 @@ def main():
     ctx.drain_stage_times()
     m_total, n_total = 0, 0
     ctx.sync()
+    written0 = slots_written(ctx)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         tr.step()
@@ -422,6 +429,7 @@ def main():
         n_total += dg.n
     ctx.sync()
     elapsed = time.perf_counter() - t0
+    written = slots_written(ctx) - written0
     timed = ctx.drain_stage_times()
     # stage breakdown over a separate window of steps that times every stage
     ctx.set_stage_mask(None)
@@ -443,7 +451,8 @@ def main():
     dom_ms, dom_calls = timed[dom]
     avg_ms = dom_ms / max(dom_calls, 1)
     per_isect, per_px, per_g = BYTES.get(dom, (0, 0, 0))
-    alg_bytes = (per_isect * m_total + per_px * P * K + per_g * n_total) / max(dom_calls, 1)
+    per_slot = SLOT_BYTES if dom == "rast_bwd" else 0
+    alg_bytes = (per_isect * m_total + per_px * P * K + per_g * n_total + per_slot * written) / max(dom_calls, 1)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     from msplat import autotune
     cur = autotune.current(ctx)
@@ -453,6 +462,8 @@ def main():
                 "traffic_source": "profiles/raster_backward_street_475k_traffic.md" if traffic else None,
                 "variant": cur,
                 "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(alg_bytes),
+                "intersections_per_launch": int(m_total / max(dom_calls, 1)),
+                "slots_written_per_launch": int(written / max(dom_calls, 1)) if per_slot else None,
                 "stages_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in stages.items()},
                 "stages_window_steps": args.stage_steps}
     batched = None

@@ -137,7 +137,10 @@ int msplat_hip_render_backward(msplat_hip_ctx *ctx, const msplat_hip_gaussians *
  * tile_start (T+1): tile t owns sorted intersections [tile_start[t], tile_start[t+1]). Per pixel: image, final_T, final_idx (index into
  * the sorted intersections), dL/d(image). Per gaussian reduced gradients (N,2)/(N,3)/
  * (N,3)/(N) are filled when MSPLAT_FRAME_KEEP_GRADS is set. loss_sum: the last loss's
- * double sum of the per-pixel losses (one double, device memory). */
+ * double sum of the per-pixel losses (one double, device memory). slots_written: a
+ * running count (one uint64, device memory) of the gradient slots the one-view raster
+ * backward has written since the context was created; the other slots of its frames
+ * were all-zero and only flagged. */
 typedef struct {
     int32_t num_points, num_tiles;
     int64_t num_isects;
@@ -151,6 +154,7 @@ typedef struct {
     int32_t *final_idx;
     float *v_xy, *v_conic, *v_rgb, *v_opacity;
     double *loss_sum;
+    uint64_t *slots_written;
 } msplat_hip_frame;
 int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *out);
 /* KEEP_GRADS: fill the reduced gradients; KEYS64: materialise isect_keys on frame_view. */

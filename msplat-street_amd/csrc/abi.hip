@@ -48,7 +48,10 @@ int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_
     CHECKED(ctx->grow(ctx->offsets, 4 * N));
     CHECKED(ctx->grow(ctx->block_sums, 4 * (size_t)(nblocks + 1)));
     CHECKED(ctx->grow(ctx->block_vis, 4 * (size_t)(nblocks + 1)));
-    CHECKED(ctx->grow(ctx->scalars, 128));
+    if (!ctx->scalars.p) {
+        CHECKED(ctx->grow(ctx->scalars, 128));
+        zero_u32<<<1, 256, 0, ctx->stream>>>((uint32_t *)ctx->scalars.p, 32);
+    }
     CHECKED(ctx->grow(ctx->big, 40 * N + 64));  // list (4) + slot sums (16 + 16 + 4) + alignment
     ctx->n = n;
     ctx->width = cam->width;
@@ -236,6 +239,7 @@ RasterArgs raster_args(msplat_hip_ctx *ctx) {
     r.slot_c = (float *)ctx->slot_c.p;
     r.slot_f = (uint8_t *)ctx->slot_f.p;
     r.flags_zero = 0;
+    r.slots_written = nullptr;
     return r;
 }
 
@@ -285,6 +289,7 @@ int do_raster_backward(msplat_hip_ctx *ctx) {
     }
     RasterArgs r = raster_args(ctx);
     r.flags_zero = 1;
+    r.slots_written = ctx->d_slots_written();
     ctx->begin(4);
     launch_raster_backward(r, ctx->tuning, ctx->stream);
     ctx->end();
@@ -608,6 +613,7 @@ int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *f) {
     f->final_idx = (int32_t *)ctx->final_idx.p;
     f->v_image = (float *)ctx->v_image.p;
     f->loss_sum = ctx->scalars.p ? ctx->d_loss_sum() : nullptr;
+    f->slots_written = ctx->scalars.p ? (uint64_t *)ctx->d_slots_written() : nullptr;
     f->v_xy = (float *)ctx->v_xy.p;
     f->v_conic = (float *)ctx->v_conic.p;
     f->v_rgb = (float *)ctx->v_rgb.p;

@@ -143,7 +143,8 @@ struct msplat_hip_ctx {
         marks.swap(open);
     }
     // device scalars: [0] u64 isect total, [8] u32 total, [16] double loss sum, [24] float loss,
-    // [32] u32 big count, [40] u64 / [48] u32 visible gaussians
+    // [32] u32 big count, [40] u64 / [48] u32 visible gaussians, [56] u64 running count of
+    // written gradient slots (zeroed at allocation)
     Buf scalars;
     uint64_t *h_total = nullptr;  // pinned
     hipEvent_t counts_ready = nullptr;  // h_total holds this frame's counts
@@ -202,5 +203,6 @@ struct msplat_hip_ctx {
     uint64_t *d_vis64() { return (uint64_t *)((char *)scalars.p + 40); }
     uint32_t *d_vis32() { return (uint32_t *)((char *)scalars.p + 48); }
     float *d_loss() { return (float *)((char *)scalars.p + 24); }
+    unsigned long long *d_slots_written() { return (unsigned long long *)((char *)scalars.p + 56); }
 };
 

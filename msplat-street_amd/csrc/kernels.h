@@ -96,6 +96,7 @@ struct RasterArgs {
     float *slot_c;
     uint8_t *slot_f;  // per slot: 1 = written, 0 = all nine values zero (left unwritten)
     int flags_zero;   // every slot_f entry is 0 on entry: only nonzero slots are flagged
+    unsigned long long *slots_written;  // null, or a running count of written slots
     int cull;  // set by the launchers from Tuning::cull
 };
 

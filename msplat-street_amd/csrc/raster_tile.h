@@ -42,6 +42,7 @@ struct TileData {
     float *slot_c;
     uint8_t *slot_f;  // 1: the slot's nine values were written; 0: they are all zero
     bool flags_zero;  // slot_f is all zero on entry, so zero slots need no write
+    unsigned long long *slots_written;  // null, or a running count of written slots
     bool cull;  // skip gaussians that splat_misses_tile proves invisible in the tile
 };
 
@@ -269,6 +270,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
         d.slot_f[d.offsets[g] + (uint32_t)((ty - ry0) * (rx1 - rx0) + (tx - rx0))] = 0;
     }
 
+    uint32_t nwritten = 0;  // slots this thread writes (counted when slots_written is given)
     for (int end = kmax + 1; end > s; end -= BATCH) {
         const int start = max(s, end - BATCH);
         const int n = end - start;
@@ -359,6 +361,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
                 if (!d.flags_zero) d.slot_f[p] = 0;
                 continue;
             }
+            nwritten++;
             float q[NQ];
 #pragma unroll
             for (int i = 0; i < NQ; i++) {
@@ -376,6 +379,10 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             d.slot_c[p] = -(1.f - B.y) * q[0];
             d.slot_f[p] = 1;
         }
+    }
+    if (d.slots_written) {  // one atomic per wave
+        for (int dd = 32; dd >= 1; dd >>= 1) nwritten += __shfl_xor(nwritten, dd);
+        if (lane == 0 && nwritten) atomicAdd(d.slots_written, (unsigned long long)nwritten);
     }
 }
 

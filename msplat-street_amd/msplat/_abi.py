@@ -73,7 +73,7 @@ class Frame(C.Structure):
                 ("tile_start", C.c_void_p), ("image", C.c_void_p), ("final_T", C.c_void_p),
                 ("v_image", C.c_void_p), ("final_idx", C.c_void_p), ("v_xy", C.c_void_p),
                 ("v_conic", C.c_void_p), ("v_rgb", C.c_void_p), ("v_opacity", C.c_void_p),
-                ("loss_sum", C.c_void_p)]
+                ("loss_sum", C.c_void_p), ("slots_written", C.c_void_p)]
 
 
 _lib = None

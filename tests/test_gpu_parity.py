@@ -295,3 +295,27 @@ def test_radix_sort_matches_numpy(ctx, n, bits, hi):
     order = np.argsort(keys & np.uint64((1 << bits) - 1 if bits < 64 else 0xFFFFFFFFFFFFFFFF), kind="stable")
     np.testing.assert_array_equal(dk.numpy(), keys[order])
     np.testing.assert_array_equal(dv.numpy(), order)
+
+
+def test_written_slot_count_matches_oracle(ctx, oracle):
+    """The raster backward writes (and counts) exactly the slots whose gradient is not
+    all zero; the others are only flagged. With a cotangent that is nonzero everywhere,
+    a slot is nonzero exactly when its pair composites some pixel, so the count equals
+    the number of nonzero rows of the oracle's slots."""
+    from msplat._abi import DeviceArray
+    from msplat.backend import DeviceGaussians, render, render_backward
+    sc, cam = _scene_cam(seed=21, n=4000, deg=3, W=97, H=71)
+    dg = DeviceGaussians(ctx, sc)
+    rng = np.random.default_rng(22)
+    v_img = (rng.normal(0, 1, (71, 97, 3)) * 1e-3).astype(np.float32)
+    render(ctx, dg, cam, BG)
+    f = ctx.frame()
+    before = int(ctx.download(f.slots_written, (1,), np.uint64)[0])
+    render_backward(ctx, dg, cam, BG, DeviceArray.from_numpy(ctx, v_img))
+    written = int(ctx.download(ctx.frame().slots_written, (1,), np.uint64)[0]) - before
+    _, b, fwd = oracle.render(sc, cam, BG)
+    slots = oracle.raster_backward(b, fwd, cam, BG, v_img)
+    nonzero = int(np.any(slots != 0, axis=1).sum())
+    assert b["M"] == ctx.frame().num_isects
+    assert 0 < nonzero < b["M"]
+    assert written == nonzero, (written, nonzero, b["M"])
