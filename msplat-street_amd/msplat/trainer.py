@@ -77,7 +77,8 @@ class Trainer:
 
     def step(self, read_loss: bool = False):
         """One iteration: train step, LR schedule, densification. Returns the loss when
-        read_loss (a sync), else None."""
+        read_loss (a sync), else None. The step's radii are not copied out: the fused
+        densification statistics already use them (they stay in ctx.frame().radii)."""
         self.step_count += 1
         step = self.step_count
         i = self.next_camera()
@@ -97,7 +98,7 @@ class Trainer:
         loss = C.c_float(0.0)
         check(lib().msplat_hip_train_step(self.ctx.ptr, C.byref(g), C.byref(c), self.bg,
                                           C.c_void_p(self.targets[i].ptr), self.cfg.ssim_weight, inv_n, C.byref(a),
-                                          C.byref(s), inv_max, C.c_void_p(self.dg.radii.ptr),
+                                          C.byref(s), inv_max, None,
                                           C.byref(loss) if read_loss else None))
         self.schedulers_step(step)
         self.after_train(step)
