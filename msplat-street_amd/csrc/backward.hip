@@ -91,6 +91,29 @@ template <int MODE>
 __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g >= a.n) return;
+    // the dense Adam groups' parameters and moments, loaded before the slot sums so that
+    // their loads are in flight during the visible path's dependent loads
+    float dp[11], d1[11], d2[11];
+    if (MODE == BWD_ADAM) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            dp[i] = a.means_rw[3 * (size_t)g + i];
+            d1[i] = a.m1[0][3 * (size_t)g + i];
+            d2[i] = a.m2[0][3 * (size_t)g + i];
+            dp[3 + i] = a.scales[3 * (size_t)g + i];
+            d1[3 + i] = a.m1[1][3 * (size_t)g + i];
+            d2[3 + i] = a.m2[1][3 * (size_t)g + i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            dp[6 + i] = a.quats[4 * (size_t)g + i];
+            d1[6 + i] = a.m1[2][4 * (size_t)g + i];
+            d2[6 + i] = a.m2[2][4 * (size_t)g + i];
+        }
+        dp[10] = a.opac[g];
+        d1[10] = a.m1[5][g];
+        d2[10] = a.m2[5][g];
+    }
     const bool vis = a.radii[g] > 0;
     float vxy[2] = {0.f, 0.f}, vcn[3] = {0.f, 0.f, 0.f}, vrgb[3] = {0.f, 0.f, 0.f}, vop = 0.f;
     if (vis && a.count[g] > BIG_SLOTS) {  // reduced by big_reduce_kernel
@@ -158,20 +181,31 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
             adam_step(a.dc[i], a.m1[3][i], a.m2[3][i], SH_C0 * vrgb[c], a.step[3], b1, b2, a.bc2[3], eps);
         }
     }
-    // dense groups: means, scales, quats, opacity
+    // dense groups: means, scales, quats, opacity (from the values loaded at the start)
+#pragma unroll
     for (int i = 0; i < 3; i++) {
         const size_t k = 3 * (size_t)g + i;
-        adam_step(a.means_rw[k], a.m1[0][k], a.m2[0][k], vm[i], a.step[0], b1, b2, a.bc2[0], eps);
+        adam_step(dp[i], d1[i], d2[i], vm[i], a.step[0], b1, b2, a.bc2[0], eps);
+        a.means_rw[k] = dp[i];
+        a.m1[0][k] = d1[i];
+        a.m2[0][k] = d2[i];
+        adam_step(dp[3 + i], d1[3 + i], d2[3 + i], vs[i], a.step[1], b1, b2, a.bc2[1], eps);
+        a.scales[k] = dp[3 + i];
+        a.m1[1][k] = d1[3 + i];
+        a.m2[1][k] = d2[3 + i];
     }
-    for (int i = 0; i < 3; i++) {
-        const size_t k = 3 * (size_t)g + i;
-        adam_step(a.scales[k], a.m1[1][k], a.m2[1][k], vs[i], a.step[1], b1, b2, a.bc2[1], eps);
-    }
+#pragma unroll
     for (int i = 0; i < 4; i++) {
         const size_t k = 4 * (size_t)g + i;
-        adam_step(a.quats[k], a.m1[2][k], a.m2[2][k], vq[i], a.step[2], b1, b2, a.bc2[2], eps);
+        adam_step(dp[6 + i], d1[6 + i], d2[6 + i], vq[i], a.step[2], b1, b2, a.bc2[2], eps);
+        a.quats[k] = dp[6 + i];
+        a.m1[2][k] = d1[6 + i];
+        a.m2[2][k] = d2[6 + i];
     }
-    adam_step(a.opac[g], a.m1[5][g], a.m2[5][g], vop, a.step[5], b1, b2, a.bc2[5], eps);
+    adam_step(dp[10], d1[10], d2[10], vop, a.step[5], b1, b2, a.bc2[5], eps);
+    a.opac[g] = dp[10];
+    a.m1[5][g] = d1[10];
+    a.m2[5][g] = d2[10];
     if (vis) {
         a.vis[g] += 1.0f;
         a.gn[g] += sqrtf(vxy[0] * vxy[0] + vxy[1] * vxy[1]);
