@@ -159,9 +159,12 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
         vs[0] *= es[0]; vs[1] *= es[1]; vs[2] *= es[2];
         view_direction(p, cam.cam_pos, dirv);
     }
-    // the higher SH bands are handled per coefficient by sh_rest_kernel
-    a.sh_dir[g] = make_float4(dirv[0], dirv[1], dirv[2], vis ? 1.f : 0.f);
-    a.sh_vrgb[g] = make_float4(vrgb[0], vrgb[1], vrgb[2], 0.f);
+    // the higher SH bands are handled per coefficient by sh_rest_kernel (every gaussian,
+    // gradient mode) or sh_rest_adam_kernel (the visible list only, Adam mode)
+    if (MODE == BWD_GRADS || vis) {
+        a.sh_dir[g] = make_float4(dirv[0], dirv[1], dirv[2], vis ? 1.f : 0.f);
+        a.sh_vrgb[g] = make_float4(vrgb[0], vrgb[1], vrgb[2], 0.f);
+    }
 
     if (MODE == BWD_GRADS) {
         for (int i = 0; i < 3; i++) {
