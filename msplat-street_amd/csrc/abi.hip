@@ -187,12 +187,17 @@ int do_bin(msplat_hip_ctx *ctx) {
     ctx->slot_f_clean = true;
     uint32_t *kin = (uint32_t *)ctx->keys0.p, *kout = (uint32_t *)ctx->keys1.p;
     int32_t *vin = (int32_t *)ctx->vals0.p, *vout = (int32_t *)ctx->vals1.p;
+    // The first pass takes the bits left over by whole bytes (the last pass takes 8), so
+    // that it has fewer, longer digit runs: its input is in depth order, the least
+    // coalesced scatter of the sort.
     const int bits = tile_bits(ctx->ntiles);
-    for (int shift = 0; shift < bits; shift += 8) {
-        launch_radix_pass(kin, vin, kout, vout, ctx->d_total32(), (uint32_t)M, shift, (uint32_t *)ctx->hist.p,
-                          (uint32_t *)ctx->digit_total.p, s);
+    for (int done = 0; done < bits;) {
+        const int width = done == 0 && bits % 8 ? bits % 8 : 8;
+        launch_radix_pass(kin, vin, kout, vout, ctx->d_total32(), (uint32_t)M, done, (uint32_t *)ctx->hist.p,
+                          (uint32_t *)ctx->digit_total.p, s, (1u << width) - 1u);
         std::swap(kin, kout);
         std::swap(vin, vout);
+        done += width;
     }
     ctx->keys_sorted = kin;
     ctx->gid_sorted = vin;

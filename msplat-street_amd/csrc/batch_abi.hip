@@ -381,11 +381,13 @@ void enqueue_sort_and_forward(msplat_hip_ctx *ctx, BatchState *b, const msplat_h
     uint32_t *kin = (uint32_t *)b->keys0.p, *kout = (uint32_t *)b->keys1.p;
     int32_t *vin = (int32_t *)b->vals0.p, *vout = (int32_t *)b->vals1.p;
     const int bits = tile_bits((int)b->vtiles_cap);
-    for (int shift = 0; shift < bits; shift += 8) {
-        launch_radix_pass(kin, vin, kout, vout, &sc->total32, b->cap, shift, (uint32_t *)b->hist.p,
-                          (uint32_t *)b->digit_total.p, s);
+    for (int done = 0; done < bits;) {  // narrow first pass, as for one view (abi.hip)
+        const int width = done == 0 && bits % 8 ? bits % 8 : 8;
+        launch_radix_pass(kin, vin, kout, vout, &sc->total32, b->cap, done, (uint32_t *)b->hist.p,
+                          (uint32_t *)b->digit_total.p, s, (1u << width) - 1u);
         std::swap(kin, kout);
         std::swap(vin, vout);
+        done += width;
     }
     b->gid_sorted = vin;
     launch_tile_ranges_views(kin, vin, &sc->total32, b->cap, g->num_points, dtab(b), (int32_t *)b->tile_start.p, s);

@@ -292,8 +292,8 @@ template <int ITEMS>
 __device__ inline uint32_t rs_blocks(const uint32_t *n_ptr) { return (*n_ptr + ITEMS - 1) / ITEMS; }
 
 template <typename K, int ITEMS>
-__global__ __launch_bounds__(256) void rs_hist_kernel(const K *keys, const uint32_t *n_ptr, int shift, uint32_t *hist,
-                                                     int stride) {
+__global__ __launch_bounds__(256) void rs_hist_kernel(const K *keys, const uint32_t *n_ptr, int shift, uint32_t mask,
+                                                     uint32_t *hist, int stride) {
     __shared__ uint32_t h[256];
     const uint32_t n = *n_ptr, nb = rs_blocks<ITEMS>(n_ptr);
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(256) void rs_hist_kernel(const K *keys, const uint3
         const uint32_t start = b * ITEMS;
         for (int r = 0; r < ITEMS / 256; r++) {
             uint32_t e = start + r * 256 + threadIdx.x;
-            if (e < n) atomicAdd(&h[(uint32_t)(keys[e] >> shift) & 0xFF], 1u);
+            if (e < n) atomicAdd(&h[(uint32_t)(keys[e] >> shift) & mask], 1u);
         }
         __syncthreads();
         hist[(size_t)threadIdx.x * stride + b] = h[threadIdx.x];
@@ -346,7 +346,7 @@ __device__ inline void wave_sync() {
 // One element block of a pass; `my_base` is this thread's digit's global start for the block.
 template <typename K, int ITEMS>
 __device__ inline void rs_scatter_block(const K *kin, const int32_t *vin, K *kout, int32_t *vout, uint32_t n,
-                                        int shift, uint32_t start, uint32_t my_base) {
+                                        int shift, uint32_t mask, uint32_t start, uint32_t my_base) {
     constexpr int R = ITEMS / 256;  // rounds of 64 elements per wave
     __shared__ K sk[ITEMS];
     __shared__ int32_t sv[ITEMS];
@@ -373,7 +373,7 @@ __device__ inline void rs_scatter_block(const K *kin, const int32_t *vin, K *kou
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const bool valid = wstart + r * 64 + lane < n;
-        const uint32_t digit = valid ? (uint32_t)(key[r] >> shift) & 0xFF : 0;
+        const uint32_t digit = valid ? (uint32_t)(key[r] >> shift) & mask : 0;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -411,7 +411,7 @@ __device__ inline void rs_scatter_block(const K *kin, const int32_t *vin, K *kou
     const uint32_t cnt = min((uint32_t)ITEMS, n - start);
     for (uint32_t p = tid; p < cnt; p += 256) {
         const K k = sk[p];
-        const uint32_t d = (uint32_t)(k >> shift) & 0xFF;
+        const uint32_t d = (uint32_t)(k >> shift) & mask;
         const uint32_t o = gbase[d] + (p - lstart[d]);
         kout[o] = k;
         vout[o] = sv[p];
@@ -421,8 +421,8 @@ __device__ inline void rs_scatter_block(const K *kin, const int32_t *vin, K *kou
 
 template <typename K, int ITEMS>
 __global__ __launch_bounds__(256) void rs_scatter_kernel(const K *kin, const int32_t *vin, K *kout, int32_t *vout,
-                                                        const uint32_t *n_ptr, int shift, const uint32_t *hist,
-                                                        int stride, const uint32_t *digit_total) {
+                                                        const uint32_t *n_ptr, int shift, uint32_t mask,
+                                                        const uint32_t *hist, int stride, const uint32_t *digit_total) {
     __shared__ uint32_t dex[256];
     __shared__ uint32_t lds4[4];
     {   // exclusive prefix of the digit totals
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(256) void rs_scatter_kernel(const K *kin, const int
     const uint32_t nb = rs_blocks<ITEMS>(n_ptr);
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
         __syncthreads();
-        rs_scatter_block<K, ITEMS>(kin, vin, kout, vout, *n_ptr, shift, b * ITEMS,
+        rs_scatter_block<K, ITEMS>(kin, vin, kout, vout, *n_ptr, shift, mask, b * ITEMS,
                                    dex[threadIdx.x] + hist[(size_t)threadIdx.x * stride + b]);
     }
 }
@@ -730,30 +730,32 @@ constexpr uint32_t HIST_GRID = 2048, SCATTER_GRID = 1024, LIST_GRID = 2048;
 
 template <typename K, int ITEMS>
 static void radix_pass_items(const K *kin, const int32_t *vin, K *kout, int32_t *vout, const uint32_t *n_dev,
-                             uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s) {
+                             uint32_t n_max, int shift, uint32_t mask, uint32_t *hist, uint32_t *digit_total,
+                             hipStream_t s) {
     const uint32_t nblocks = (n_max + ITEMS - 1) / ITEMS;
     if (nblocks == 0) return;
-    rs_hist_kernel<K, ITEMS><<<std::min(nblocks, HIST_GRID), 256, 0, s>>>(kin, n_dev, shift, hist, (int)nblocks);
+    rs_hist_kernel<K, ITEMS><<<std::min(nblocks, HIST_GRID), 256, 0, s>>>(kin, n_dev, shift, mask, hist, (int)nblocks);
     rs_scan_kernel<ITEMS><<<256, 256, 0, s>>>(hist, (int)nblocks, n_dev, digit_total);
     rs_scatter_kernel<K, ITEMS><<<std::min(nblocks, SCATTER_GRID), 256, 0, s>>>(kin, vin, kout, vout, n_dev, shift,
-                                                                                hist, (int)nblocks, digit_total);
+                                                                                mask, hist, (int)nblocks, digit_total);
 }
 
 template <typename K>
 static void radix_pass(const K *kin, const int32_t *vin, K *kout, int32_t *vout, const uint32_t *n_dev,
-                       uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s) {
-    if (n_max >= RS_LARGE) radix_pass_items<K, RS_ITEMS>(kin, vin, kout, vout, n_dev, n_max, shift, hist, digit_total, s);
-    else radix_pass_items<K, RS_ITEMS_SMALL>(kin, vin, kout, vout, n_dev, n_max, shift, hist, digit_total, s);
+                       uint32_t n_max, int shift, uint32_t mask, uint32_t *hist, uint32_t *digit_total, hipStream_t s) {
+    if (n_max >= RS_LARGE)
+        radix_pass_items<K, RS_ITEMS>(kin, vin, kout, vout, n_dev, n_max, shift, mask, hist, digit_total, s);
+    else radix_pass_items<K, RS_ITEMS_SMALL>(kin, vin, kout, vout, n_dev, n_max, shift, mask, hist, digit_total, s);
 }
 
 void launch_radix_pass(const uint64_t *kin, const int32_t *vin, uint64_t *kout, int32_t *vout, const uint32_t *n_dev,
-                       uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s) {
-    radix_pass<uint64_t>(kin, vin, kout, vout, n_dev, n_max, shift, hist, digit_total, s);
+                       uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s, uint32_t mask) {
+    radix_pass<uint64_t>(kin, vin, kout, vout, n_dev, n_max, shift, mask, hist, digit_total, s);
 }
 
 void launch_radix_pass(const uint32_t *kin, const int32_t *vin, uint32_t *kout, int32_t *vout, const uint32_t *n_dev,
-                       uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s) {
-    radix_pass<uint32_t>(kin, vin, kout, vout, n_dev, n_max, shift, hist, digit_total, s);
+                       uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s, uint32_t mask) {
+    radix_pass<uint32_t>(kin, vin, kout, vout, n_dev, n_max, shift, mask, hist, digit_total, s);
 }
 
 void launch_order_sums(const int32_t *list, const uint32_t *nvis_dev, uint32_t nvis_max, const uint32_t *count,

@@ -326,10 +326,13 @@ void launch_order_sums(const int32_t *list, const uint32_t *nvis_dev, uint32_t n
 void launch_keygen_offsets(const KeygenArgs &a, hipStream_t s);  // offsets, loff, big list
 void launch_keygen_emit(const KeygenArgs &a, bool batch, hipStream_t s);  // the keys
 void launch_keygen(const KeygenArgs &a, bool batch, hipStream_t s);
+// One stable pass over the digit (key >> shift) & mask (mask at most 0xFF)
 void launch_radix_pass(const uint64_t *kin, const int32_t *vin, uint64_t *kout, int32_t *vout, const uint32_t *n_dev,
-                       uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s);
+                       uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s,
+                       uint32_t mask = 0xFF);
 void launch_radix_pass(const uint32_t *kin, const int32_t *vin, uint32_t *kout, int32_t *vout, const uint32_t *n_dev,
-                       uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s);
+                       uint32_t n_max, int shift, uint32_t *hist, uint32_t *digit_total, hipStream_t s,
+                       uint32_t mask = 0xFF);
 size_t radix_hist_words(uint32_t n_max);
 void launch_tile_ranges(const uint32_t *keys, const uint32_t *n_dev, uint32_t n_max, const uint32_t *ntiles_dev,
                         int ntiles, int32_t *tile_start, hipStream_t s);
