@@ -460,6 +460,10 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": "profiles/raster_backward_street_475k_traffic.md" if traffic else None,
+                "note": ("the raster backward is bound by its per-pixel VALU work, not HBM: its VALU instructions "
+                         "take 82% of its time (profiles/raster_backward_street_475k_traffic.md); all-zero gradient "
+                         "slots are flagged, not written, so its algorithmic bytes count written slots only")
+                if dom == "rast_bwd" else None,
                 "variant": cur,
                 "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(alg_bytes),
                 "intersections_per_launch": int(m_total / max(dom_calls, 1)),
