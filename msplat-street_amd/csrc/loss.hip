@@ -163,25 +163,26 @@ __device__ inline void ssim_v_block(const LossArgs &a, int X0, int Y0, int c) {
 }
 
 // ssim_v_bwd for channel c of the block: thread t owns column X0 + t and slides down the
-// back-convolved fields of rows Y0 - 5 .. Y0 + 12 (rows outside the image skipped).
+// back-convolved fields of rows Y0 - 5 .. Y0 + ROWS + 4 (rows outside the image skipped).
+template <int ROWS = LOSS_BLOCK_ROWS>
 __device__ inline void ssim_vb_block(const LossArgs &a, int X0, int Y0, int c) {
     const int W = (int)a.width, H = (int)a.height, t = threadIdx.x;
     const size_t P = (size_t)W * H;
     const int x = X0 + t;
     if (t >= LOSS_BLOCK_COLS || x >= W) return;
     const float w = a.ssim_weight;
-    float cf[LOSS_BLOCK_ROWS][3];
+    float cf[ROWS][3];
 #pragma unroll
-    for (int r = 0; r < LOSS_BLOCK_ROWS; r++) cf[r][0] = cf[r][1] = cf[r][2] = 0.f;
+    for (int r = 0; r < ROWS; r++) cf[r][0] = cf[r][1] = cf[r][2] = 0.f;
     const float *hb = a.scratch + (size_t)(PL_HB + 3 * c) * P + x;
 #pragma unroll
-    for (int j = 0; j < LB_IN; j++) {
+    for (int j = 0; j < ROWS + 2 * HW; j++) {
         const int iy = Y0 - HW + j;
         if (iy < 0 || iy >= H) continue;
         const float *hr = hb + (size_t)iy * W;
         const float d0 = hr[0], d1 = hr[P], d2 = hr[2 * P];
 #pragma unroll
-        for (int r = 0; r < LOSS_BLOCK_ROWS; r++) {
+        for (int r = 0; r < ROWS; r++) {
             const int dy = j - r;  // the tap of row iy in output row Y0 + r
             if (dy < 0 || dy > 10) continue;
             const float g = GW[10 - dy];
@@ -190,7 +191,7 @@ __device__ inline void ssim_vb_block(const LossArgs &a, int X0, int Y0, int c) {
             cf[r][2] += g * d2;
         }
     }
-    for (int r = 0; r < LOSS_BLOCK_ROWS; r++) {
+    for (int r = 0; r < ROWS; r++) {
         const int y = Y0 + r;
         if (y >= H) break;
         const size_t i = ((size_t)y * W + x) * 3 + c;
@@ -201,6 +202,11 @@ __device__ inline void ssim_vb_block(const LossArgs &a, int X0, int Y0, int c) {
     }
 }
 
+// The one-view vertical back-convolution runs taller blocks than the other passes: it
+// needs no column halo or LDS, so more rows per thread only cost registers, and its
+// 5-row halos are re-read less often.
+constexpr int VB_ROWS = 16;
+
 __global__ __launch_bounds__(LB_THREADS) void ssim_h_kernel(LossArgs a) {
     ssim_h_row(a, blockIdx.x * LOSS_BLOCK_COLS, blockIdx.y);
 }
@@ -208,7 +214,7 @@ __global__ __launch_bounds__(LB_THREADS) void ssim_v_kernel(LossArgs a) {
     ssim_v_block(a, blockIdx.x * LOSS_BLOCK_COLS, blockIdx.y * LOSS_BLOCK_ROWS, blockIdx.z);
 }
 __global__ __launch_bounds__(LB_THREADS) void ssim_vb_kernel(LossArgs a) {
-    ssim_vb_block(a, blockIdx.x * LOSS_BLOCK_COLS, blockIdx.y * LOSS_BLOCK_ROWS, blockIdx.z);
+    ssim_vb_block<VB_ROWS>(a, blockIdx.x * LOSS_BLOCK_COLS, blockIdx.y * VB_ROWS, blockIdx.z);
 }
 
 // The reference's per-pixel loss: the channels' SSIM and L1 terms added in channel
@@ -439,7 +445,7 @@ void launch_loss(const LossArgs &a, hipStream_t s) {
     const uint32_t nby = (a.height + LOSS_BLOCK_ROWS - 1) / LOSS_BLOCK_ROWS;
     ssim_h_kernel<<<dim3(nbx, a.height), LB_THREADS, 0, s>>>(a);
     ssim_v_kernel<<<dim3(nbx, nby, 3), LB_THREADS, 0, s>>>(a);
-    ssim_vb_kernel<<<dim3(nbx, nby, 3), LB_THREADS, 0, s>>>(a);
+    ssim_vb_kernel<<<dim3(nbx, (a.height + VB_ROWS - 1) / VB_ROWS, 3), LB_THREADS, 0, s>>>(a);
     loss_rows_kernel<<<a.height, ROW_THREADS, 0, s>>>(a);
     loss_total_kernel<<<1, 256, 0, s>>>(a.row_loss, a.height, a.width, a.loss_sum, a.loss_mean);
 }
