@@ -101,8 +101,8 @@ struct RasterArgs {
 };
 
 // Loss blocks: 8 rows x 118 columns; LOSS_PLANES floats of scratch per pixel (15
-// horizontal SSIM moments, 9 back-convolved derivative fields, 3 SSIM and 3 L1 terms)
-constexpr int LOSS_BLOCK_ROWS = 8, LOSS_BLOCK_COLS = 118, LOSS_PLANES = 30;
+// horizontal SSIM moments, 9 back-convolved derivative fields, 3 SSIM terms)
+constexpr int LOSS_BLOCK_ROWS = 8, LOSS_BLOCK_COLS = 118, LOSS_PLANES = 27;
 inline uint32_t loss_blocks(uint32_t W, uint32_t H) {
     return ((W + LOSS_BLOCK_COLS - 1) / LOSS_BLOCK_COLS) * ((H + LOSS_BLOCK_ROWS - 1) / LOSS_BLOCK_ROWS);
 }

@@ -6,14 +6,15 @@
 //   ssim_v     one channel of a block per workgroup: vertical pass, SSIM and its
 //              derivative fields (on the block's columns plus the 5-column halo,
 //              exchanged through LDS), their horizontal back-convolution -> planes
-//              15..23, and the channel's SSIM and |target - render| -> planes 24..29;
+//              15..23, and the channel's SSIM -> planes 24..26;
 //   ssim_vb    one channel of a block per workgroup: vertical back-convolution and
 //              dL/d(render);
-//   rows       the per-pixel loss (channels added in order) and its sum per row;
+//   rows       the per-pixel loss (channels' SSIM planes and |target - render| from the
+//              images, added in channel order) and its sum per row;
 //   total      the sum over rows.
 // In ssim_v and ssim_vb a thread owns one column and slides down the block's rows:
 // each input row it loads is added into every output row whose window holds it, so a
-// vertical sum costs one load per input row. The planes (120 B per pixel) pass through
+// vertical sum costs one load per input row. The planes (108 B per pixel) pass through
 // L2 and the memory-side cache between the passes.
 // Every sum runs in the reference's tap order with unfused multiply-adds, so the
 // gradient and the per-pixel loss are bit-identical to the CPU restatement, and the
@@ -27,7 +28,7 @@ namespace msplat {
 constexpr int HW = 5;                            // half window
 constexpr int LB_THREADS = LOSS_BLOCK_COLS + 2 * HW;
 constexpr int LB_IN = LOSS_BLOCK_ROWS + 2 * HW;  // input rows of a block's vertical sums
-constexpr int PL_HB = 15, PL_SSIM = 24, PL_L1 = 27;  // first planes of the fields, SSIM, L1
+constexpr int PL_HB = 15, PL_SSIM = 24;  // first planes of the fields and of SSIM
 static_assert(LB_THREADS == 128, "two waves per loss block");
 
 __device__ constexpr float GW[11] = {0.0010283801f, 0.0075987581f, 0.0360007721f, 0.1093606895f, 0.2130055377f,
@@ -81,7 +82,7 @@ __device__ inline void ssim_h_row(const LossArgs &a, int X0, int y) {
 // xf = X0 - 5 + t. It slides down input rows Y0 - 5 .. Y0 + 12 (rows outside the image
 // are skipped, as there), forms SSIM and the fields f1, f2, f3 of its 8 rows (zero
 // outside the image), and threads 5..122 then back-convolve the fields of their columns
-// from LDS. The same threads store their pixels' SSIM and L1 terms of channel c.
+// from LDS. The same threads store their pixels' SSIM of channel c.
 __device__ inline void ssim_v_block(const LossArgs &a, int X0, int Y0, int c) {
     __shared__ float fld[3][LOSS_BLOCK_ROWS][LB_THREADS];
     const int W = (int)a.width, H = (int)a.height, t = threadIdx.x;
@@ -131,11 +132,7 @@ __device__ inline void ssim_v_block(const LossArgs &a, int X0, int Y0, int c) {
             f1 = dmu - 2.0f * my * dvy - mx * dcv;
             f2 = 2.0f * dvy;
             f3 = dcv;
-            if (out) {
-                const size_t p = (size_t)y * W + xf, i = 3 * p + c;
-                a.scratch[(size_t)(PL_SSIM + c) * P + p] = (A * B) / (Cd * D);
-                a.scratch[(size_t)(PL_L1 + c) * P + p] = fabsf(a.gt[i] - a.img[i]);
-            }
+            if (out) a.scratch[(size_t)(PL_SSIM + c) * P + (size_t)y * W + xf] = (A * B) / (Cd * D);
         }
         fld[0][r][t] = f1;
         fld[1][r][t] = f2;
@@ -213,19 +210,29 @@ __global__ __launch_bounds__(LB_THREADS) void ssim_h_kernel(LossArgs a) {
 __global__ __launch_bounds__(LB_THREADS) void ssim_v_kernel(LossArgs a) {
     ssim_v_block(a, blockIdx.x * LOSS_BLOCK_COLS, blockIdx.y * LOSS_BLOCK_ROWS, blockIdx.z);
 }
+// The three channel workgroups of a block read and write the same interleaved pixels:
+// workgroup ids 8 apart, so they run on one XCD, close in time, and share its L2.
+// Block q, channel c is workgroup (q / 8) * 24 + c * 8 + q % 8; ids past the blocks exit.
 __global__ __launch_bounds__(LB_THREADS) void ssim_vb_kernel(LossArgs a) {
-    ssim_vb_block<VB_ROWS>(a, blockIdx.x * LOSS_BLOCK_COLS, blockIdx.y * VB_ROWS, blockIdx.z);
+    const uint32_t L = blockIdx.x, rest = L / NUM_XCD;
+    const uint32_t c = rest % 3, q = (rest / 3) * NUM_XCD + L % NUM_XCD;
+    const uint32_t nbx = (a.width + LOSS_BLOCK_COLS - 1) / LOSS_BLOCK_COLS;
+    const uint32_t nby = (a.height + VB_ROWS - 1) / VB_ROWS;
+    if (q >= nbx * nby) return;
+    ssim_vb_block<VB_ROWS>(a, (int)(q % nbx) * LOSS_BLOCK_COLS, (int)(q / nbx) * VB_ROWS, (int)c);
 }
 
 // The reference's per-pixel loss: the channels' SSIM and L1 terms added in channel
-// order from zero, then w (1 - ssim / 3) + (1 - w) l1 / 3 in float.
-__device__ inline float pixel_loss(const float *pl, size_t P, size_t p, float w) {
+// order from zero, then w (1 - ssim / 3) + (1 - w) l1 / 3 in float. The L1 terms come
+// from the images (one contiguous 12-byte pixel each), the SSIM terms from their planes.
+__device__ inline float pixel_loss(const LossArgs &a, size_t P, size_t p) {
     float s = 0.f, l = 0.f;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        s += pl[(size_t)(PL_SSIM + c) * P + p];
-        l += pl[(size_t)(PL_L1 + c) * P + p];
+        s += a.scratch[(size_t)(PL_SSIM + c) * P + p];
+        l += fabsf(a.gt[3 * p + c] - a.img[3 * p + c]);
     }
+    const float w = a.ssim_weight;
     return w * (1.0f - s / 3.0f) + (1.0f - w) * l / 3.0f;
 }
 
@@ -275,7 +282,7 @@ __device__ inline void row_sum(const LossArgs &a, int y, double *out) {
     double s = 0.0, as = 0.0;
     int emin = EXACT_NONE;
     for (int x = t; x < W; x += ROW_THREADS) {
-        const float v = pixel_loss(a.scratch, P, base + x, a.ssim_weight);
+        const float v = pixel_loss(a, P, base + x);
         s += (double)v;
         as += fabs((double)v);
         emin = min(emin, ulp_exp(v));
@@ -302,7 +309,7 @@ __device__ inline void row_sum(const LossArgs &a, int y, double *out) {
         if (!sums_exact(emin, as)) {  // thread 0 adds in x order
             s = 0.0;
             for (int x = 0; x < W; x++)
-                s += (double)(x < 4096 ? buf[x] : pixel_loss(a.scratch, P, base + x, a.ssim_weight));
+                s += (double)(x < 4096 ? buf[x] : pixel_loss(a, P, base + x));
             emin = EXACT_FAIL;
         }
         out[3 * y] = s;
@@ -445,7 +452,8 @@ void launch_loss(const LossArgs &a, hipStream_t s) {
     const uint32_t nby = (a.height + LOSS_BLOCK_ROWS - 1) / LOSS_BLOCK_ROWS;
     ssim_h_kernel<<<dim3(nbx, a.height), LB_THREADS, 0, s>>>(a);
     ssim_v_kernel<<<dim3(nbx, nby, 3), LB_THREADS, 0, s>>>(a);
-    ssim_vb_kernel<<<dim3(nbx, (a.height + VB_ROWS - 1) / VB_ROWS, 3), LB_THREADS, 0, s>>>(a);
+    const uint32_t nvb = nbx * ((a.height + VB_ROWS - 1) / VB_ROWS);
+    ssim_vb_kernel<<<3 * ((nvb + NUM_XCD - 1) / NUM_XCD) * NUM_XCD, LB_THREADS, 0, s>>>(a);
     loss_rows_kernel<<<a.height, ROW_THREADS, 0, s>>>(a);
     loss_total_kernel<<<1, 256, 0, s>>>(a.row_loss, a.height, a.width, a.loss_sum, a.loss_mean);
 }
