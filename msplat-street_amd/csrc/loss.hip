@@ -393,8 +393,8 @@ __device__ inline LossArgs view_loss_args(const BatchLossArgs &b, int v) {
 
 // Loss block i of the batch: its view and the block's corner. Grids cover the capacity
 // of loss blocks; blocks past the batch's count exit.
-__device__ inline bool batch_loss_block(const BatchLossArgs &b, LossArgs &a, int &X0, int &Y0) {
-    const uint32_t i = blockIdx.x;
+__device__ inline bool batch_loss_block(const BatchLossArgs &b, LossArgs &a, int &X0, int &Y0,
+                                        uint32_t i = blockIdx.x) {
     if (i >= b.tab->total_loss_blocks) return false;
     const int v = view_of_base(b.tab->loss_base, b.num_views, i);
     a = view_loss_args(b, v);
@@ -415,10 +415,13 @@ __global__ __launch_bounds__(LB_THREADS) void ssim_v_batch_kernel(BatchLossArgs 
     int X0, Y0;
     if (batch_loss_block(b, a, X0, Y0)) ssim_v_block(a, X0, Y0, blockIdx.y);
 }
+// channel workgroups of a block on one XCD, as in ssim_vb_kernel
 __global__ __launch_bounds__(LB_THREADS) void ssim_vb_batch_kernel(BatchLossArgs b) {
+    const uint32_t L = blockIdx.x, rest = L / NUM_XCD;
+    const uint32_t c = rest % 3, q = (rest / 3) * NUM_XCD + L % NUM_XCD;
     LossArgs a;
     int X0, Y0;
-    if (batch_loss_block(b, a, X0, Y0)) ssim_vb_block(a, X0, Y0, blockIdx.y);
+    if (batch_loss_block(b, a, X0, Y0, q)) ssim_vb_block(a, X0, Y0, (int)c);
 }
 
 __global__ __launch_bounds__(ROW_THREADS) void loss_rows_batch_kernel(BatchLossArgs b) {
@@ -441,7 +444,7 @@ void launch_batch_loss(const BatchLossArgs &b, hipStream_t s) {
     if (b.num_views <= 0 || b.max_loss_blocks == 0) return;
     ssim_h_batch_kernel<<<dim3(b.max_loss_blocks, LOSS_BLOCK_ROWS), LB_THREADS, 0, s>>>(b);
     ssim_v_batch_kernel<<<dim3(b.max_loss_blocks, 3), LB_THREADS, 0, s>>>(b);
-    ssim_vb_batch_kernel<<<dim3(b.max_loss_blocks, 3), LB_THREADS, 0, s>>>(b);
+    ssim_vb_batch_kernel<<<3 * ((b.max_loss_blocks + NUM_XCD - 1) / NUM_XCD) * NUM_XCD, LB_THREADS, 0, s>>>(b);
     loss_rows_batch_kernel<<<b.max_rows, ROW_THREADS, 0, s>>>(b);
     loss_total_batch_kernel<<<b.num_views, 256, 0, s>>>(b);
 }
