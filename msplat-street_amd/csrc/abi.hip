@@ -243,7 +243,6 @@ RasterArgs raster_args(msplat_hip_ctx *ctx) {
     r.slot_b = (float4 *)ctx->slot_b.p;
     r.slot_c = (float *)ctx->slot_c.p;
     r.slot_f = (uint8_t *)ctx->slot_f.p;
-    r.flags_zero = 0;
     r.slots_written = nullptr;
     return r;
 }
@@ -293,7 +292,6 @@ int do_raster_backward(msplat_hip_ctx *ctx) {
             (uint32_t *)ctx->slot_f.p, words);
     }
     RasterArgs r = raster_args(ctx);
-    r.flags_zero = 1;
     r.slots_written = ctx->d_slots_written();
     ctx->begin(4);
     launch_raster_backward(r, ctx->tuning, ctx->stream);

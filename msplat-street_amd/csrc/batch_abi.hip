@@ -302,7 +302,6 @@ BatchRasterArgs raster_args(BatchState *b) {
     r.slot_b = (float4 *)b->slot_b.p;
     r.slot_c = (float *)b->slot_c.p;
     r.slot_f = (uint8_t *)b->slot_f.p;
-    r.flags_zero = 0;
     return r;
 }
 
@@ -404,9 +403,7 @@ void enqueue_backward(msplat_hip_ctx *ctx, BatchState *b, const msplat_hip_gauss
         const size_t words = (b->cap + 3) / 4;
         if (words) zero_u32<<<(unsigned)std::min<size_t>((words + 255) / 256, 2048), 256, 0, s>>>((uint32_t *)b->slot_f.p, words);
     }
-    BatchRasterArgs r = raster_args(b);
-    r.flags_zero = 1;
-    launch_batch_raster_backward(r, ctx->tuning, s);
+    launch_batch_raster_backward(raster_args(b), ctx->tuning, s);
     const size_t R = record_count(b);
     float4 *ba = big_a(b);
     launch_big_reduce_records((const uint32_t *)b->big_list.p, &dsc(b)->big_count, (const uint32_t *)b->offsets.p,

@@ -94,8 +94,8 @@ struct RasterArgs {
     const uint32_t *offsets;
     float4 *slot_a, *slot_b;
     float *slot_c;
-    uint8_t *slot_f;  // per slot: 1 = written, 0 = all nine values zero (left unwritten)
-    int flags_zero;   // every slot_f entry is 0 on entry: only nonzero slots are flagged
+    uint8_t *slot_f;  // per slot: 1 = written; 0 = all nine values zero (left unwritten), and
+                      // all 0 when the raster backward starts
     unsigned long long *slots_written;  // null, or a running count of written slots
     int cull;  // set by the launchers from Tuning::cull
 };
@@ -292,8 +292,8 @@ struct BatchRasterArgs {
     const float *v_image;
     float4 *slot_a, *slot_b;
     float *slot_c;
-    uint8_t *slot_f;  // per slot: 1 = written, 0 = all nine values zero (left unwritten)
-    int flags_zero;   // every slot_f entry is 0 on entry: only nonzero slots are flagged
+    uint8_t *slot_f;  // per slot: 1 = written; 0 = all nine values zero (left unwritten), and
+                      // all 0 when the raster backward starts
     int cull;  // set by the launchers from Tuning::cull
 };
 int batch_raster_grid(bool backward, const Tuning &t);  // persistent workgroups that fill the device

@@ -9,8 +9,7 @@ namespace {
 
 __device__ inline TileData tile_data(const RasterArgs &a) {
     return TileData{a.tile_start, a.gid,    a.splat_a, a.splat_b, a.splat_blue,     a.rect,
-                    a.offsets,    a.slot_a, a.slot_b,  a.slot_c,  a.slot_f, a.flags_zero != 0, a.slots_written,
-                    a.cull != 0};
+                    a.offsets,    a.slot_a, a.slot_b,  a.slot_c,  a.slot_f, a.slots_written, a.cull != 0};
 }
 
 __device__ inline TileView tile_view(const RasterArgs &a) {
@@ -20,8 +19,7 @@ __device__ inline TileView tile_view(const RasterArgs &a) {
 
 __device__ inline TileData tile_data(const BatchRasterArgs &a) {
     return TileData{a.tile_start, a.gid,    a.splat_a, a.splat_b, a.splat_blue, a.rect,
-                    a.offsets,    a.slot_a, a.slot_b,  a.slot_c,  a.slot_f, a.flags_zero != 0, nullptr,
-                    a.cull != 0};
+                    a.offsets,    a.slot_a, a.slot_b,  a.slot_c,  a.slot_f, nullptr,        a.cull != 0};
 }
 
 // View v of a batch: per-pixel buffers offset to the view's first pixel

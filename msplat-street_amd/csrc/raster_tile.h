@@ -40,8 +40,7 @@ struct TileData {
     const uint32_t *offsets;
     float4 *slot_a, *slot_b;
     float *slot_c;
-    uint8_t *slot_f;  // 1: the slot's nine values were written; 0: they are all zero
-    bool flags_zero;  // slot_f is all zero on entry, so zero slots need no write
+    uint8_t *slot_f;  // 1: the slot's nine values were written; 0 (set before the launch): all zero
     unsigned long long *slots_written;  // null, or a running count of written slots
     bool cull;  // skip gaussians that splat_misses_tile proves invisible in the tile
 };
@@ -206,11 +205,12 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
 // (wave_sums9) and lanes 0..8 write the wave's totals to LDS, to be added over the waves
 // in fixed order when the batch's slots are written. One partial per wave keeps the LDS
 // footprint small enough for several waves per SIMD.
-// A slot whose nine values are all zero (a record culled from the tile, behind every
-// pixel's last contributor, or valid in no pixel) only gets its flag cleared: its
-// floats are left as they are, and the slot sums skip it. Skipping a zero term changes
-// no sum: the sums start at +0, and adding +0 or -0 to +0 or to a nonzero value returns
-// it unchanged.
+// Every slot flag is zero when the kernel starts (the key emission clears them, or a
+// fill before a second backward of the same frame). A slot whose nine values are all
+// zero (a record culled from the tile, behind every pixel's last contributor, or valid
+// in no pixel) is not touched: its floats stay as they are, and the slot sums skip it.
+// Skipping a zero term changes no sum: the sums start at +0, and adding +0 or -0 to +0
+// or to a nonzero value returns it unchanged.
 constexpr int NQ = 9;  // sum vs, vs dx, vs dy, vs dx^2, vs dx dy, vs dy^2, fac vr, fac vg, fac vb
 
 template <int PPT, int BATCH>
@@ -262,21 +262,12 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
     for (int i = 1; i < NW; i++) kmax = max(kmax, smax[i]);
     const float fx = (float)px;
 
-    // slots of intersections behind every pixel's last contributor are zero
-    for (int k = kmax + 1 + threadIdx.x; !d.flags_zero && k < e; k += NT) {
-        const int g = d.gid[k];
-        const uint2 r = d.rect[g];
-        const int rx0 = r.x & 0xFFFF, rx1 = r.x >> 16, ry0 = r.y & 0xFFFF;
-        d.slot_f[d.offsets[g] + (uint32_t)((ty - ry0) * (rx1 - rx0) + (tx - rx0))] = 0;
-    }
-
     uint32_t nwritten = 0;  // slots this thread writes (counted when slots_written is given)
     for (int end = kmax + 1; end > s; end -= BATCH) {
         const int start = max(s, end - BATCH);
         const int n = end - start;
         __syncthreads();
-        // stage the batch's records that can reach the tile, in list order; the others
-        // get their zero slot here
+        // stage the batch's records that can reach the tile, in list order
         int kept = 0;
         for (int c = 0; c < n; c += NT) {
             const int i = c + threadIdx.x;
@@ -290,11 +281,10 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
                 B0 = d.splat_b[g];
                 C0 = d.splat_blue[g];
                 keep = !(d.cull && splat_misses_tile(A0, B0, x0, y0));
-                if (keep || !d.flags_zero) {
+                if (keep) {
                     const uint2 r = d.rect[g];
                     const int rx0 = r.x & 0xFFFF, rx1 = r.x >> 16, ry0 = r.y & 0xFFFF;
                     p = d.offsets[g] + (uint32_t)((ty - ry0) * (rx1 - rx0) + (tx - rx0));
-                    if (!keep) d.slot_f[p] = 0;
                 }
             }
             int total;
@@ -357,10 +347,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
         __syncthreads();
         for (int j = threadIdx.x; j < kept; j += NT) {
             const uint32_t p = lslot[j];
-            if (!ltouch[j]) {  // every wave's partials are zero
-                if (!d.flags_zero) d.slot_f[p] = 0;
-                continue;
-            }
+            if (!ltouch[j]) continue;  // every wave's partials are zero
             nwritten++;
             float q[NQ];
 #pragma unroll
