@@ -10,7 +10,8 @@
 //   ssim_vb    one channel of a block per workgroup: vertical back-convolution and
 //              dL/d(render);
 //   rows       the per-pixel loss (channels' SSIM planes and |target - render| from the
-//              images, added in channel order) and its sum per row;
+//              images, added in channel order) and its sum per row (for one view in
+//              the same launch as ssim_vb);
 //   total      the sum over rows.
 // In ssim_v and ssim_vb a thread owns one column and slides down the block's rows:
 // each input row it loads is added into every output row whose window holds it, so a
@@ -212,9 +213,9 @@ __global__ __launch_bounds__(LB_THREADS) void ssim_v_kernel(LossArgs a) {
 }
 // The three channel workgroups of a block read and write the same interleaved pixels:
 // workgroup ids 8 apart, so they run on one XCD, close in time, and share its L2.
-// Block q, channel c is workgroup (q / 8) * 24 + c * 8 + q % 8; ids past the blocks exit.
-__global__ __launch_bounds__(LB_THREADS) void ssim_vb_kernel(LossArgs a) {
-    const uint32_t L = blockIdx.x, rest = L / NUM_XCD;
+// Block q, channel c is item (q / 8) * 24 + c * 8 + q % 8; items past the blocks exit.
+__device__ inline void ssim_vb_item(const LossArgs &a, uint32_t L) {
+    const uint32_t rest = L / NUM_XCD;
     const uint32_t c = rest % 3, q = (rest / 3) * NUM_XCD + L % NUM_XCD;
     const uint32_t nbx = (a.width + LOSS_BLOCK_COLS - 1) / LOSS_BLOCK_COLS;
     const uint32_t nby = (a.height + VB_ROWS - 1) / VB_ROWS;
@@ -267,26 +268,25 @@ __device__ inline int wave_min_i(int v) {
 // Row y's loss sum as the reference forms it: a double accumulated over the row's
 // pixel losses in x order. When every partial sum is exactly representable
 // (sums_exact), each of the reference's additions is exact, so its result is the exact
-// sum, which any order of exact additions also gives: the 256 threads add strided
+// sum, which any order of exact additions also gives: the THREADS threads add strided
 // values, each wave adds its lanes' sums and thread 0 adds the waves'. Otherwise
-// thread 0 adds the values in order. Writes the sum, the absolute sum and the ulp
-// exponent (EXACT_FAIL when the row was not exact) for the total. One workgroup of
-// ROW_THREADS per row.
+// thread 0 recomputes the values and adds them in order (the rare path). Writes the sum,
+// the absolute sum and the ulp exponent (EXACT_FAIL when the row was not exact) for the
+// total. One workgroup per row.
 constexpr int ROW_THREADS = 256;
+template <int THREADS = ROW_THREADS>
 __device__ inline void row_sum(const LossArgs &a, int y, double *out) {
-    __shared__ float buf[4096];
-    __shared__ double ws[ROW_THREADS / 64], was[ROW_THREADS / 64];
-    __shared__ int we[ROW_THREADS / 64];
+    __shared__ double ws[THREADS / 64], was[THREADS / 64];
+    __shared__ int we[THREADS / 64];
     const int W = (int)a.width, t = threadIdx.x;
     const size_t P = (size_t)W * a.height, base = (size_t)y * W;
     double s = 0.0, as = 0.0;
     int emin = EXACT_NONE;
-    for (int x = t; x < W; x += ROW_THREADS) {
+    for (int x = t; x < W; x += THREADS) {
         const float v = pixel_loss(a, P, base + x);
         s += (double)v;
         as += fabs((double)v);
         emin = min(emin, ulp_exp(v));
-        if (x < 4096) buf[x] = v;
     }
     s = wave_sum_d(s);
     as = wave_sum_d(as);
@@ -296,12 +296,12 @@ __device__ inline void row_sum(const LossArgs &a, int y, double *out) {
         was[t >> 6] = as;
         we[t >> 6] = emin;
     }
-    __syncthreads();  // the wave sums, and every thread's values in buf
+    __syncthreads();  // the wave sums
     if (t == 0) {
         s = ws[0];
         as = was[0];
         emin = we[0];
-        for (int i = 1; i < ROW_THREADS / 64; i++) {
+        for (int i = 1; i < THREADS / 64; i++) {
             s += ws[i];
             as += was[i];
             emin = min(emin, we[i]);
@@ -309,7 +309,7 @@ __device__ inline void row_sum(const LossArgs &a, int y, double *out) {
         if (!sums_exact(emin, as)) {  // thread 0 adds in x order
             s = 0.0;
             for (int x = 0; x < W; x++)
-                s += (double)(x < 4096 ? buf[x] : pixel_loss(a, P, base + x));
+                s += (double)pixel_loss(a, P, base + x);
             emin = EXACT_FAIL;
         }
         out[3 * y] = s;
@@ -355,7 +355,13 @@ __device__ inline void total_sum(const double *rows, int H, size_t npix, double 
     }
 }
 
-__global__ __launch_bounds__(ROW_THREADS) void loss_rows_kernel(LossArgs a) { row_sum(a, blockIdx.x, a.row_loss); }
+// One view: the row sums and the vertical back-convolution in one launch. They read
+// different planes and neither waits for the other, so the latency-bound row sums (the
+// first H workgroups) overlap the back-convolution's traffic.
+__global__ __launch_bounds__(LB_THREADS) void ssim_vb_rows_kernel(LossArgs a) {
+    if (blockIdx.x < a.height) row_sum<LB_THREADS>(a, (int)blockIdx.x, a.row_loss);
+    else ssim_vb_item(a, blockIdx.x - a.height);
+}
 
 __global__ __launch_bounds__(256) void loss_total_kernel(const double *rows, int H, int W, double *sum, float *mean) {
     total_sum(rows, H, (size_t)W * H, sum, mean);
@@ -415,7 +421,7 @@ __global__ __launch_bounds__(LB_THREADS) void ssim_v_batch_kernel(BatchLossArgs 
     int X0, Y0;
     if (batch_loss_block(b, a, X0, Y0)) ssim_v_block(a, X0, Y0, blockIdx.y);
 }
-// channel workgroups of a block on one XCD, as in ssim_vb_kernel
+// channel workgroups of a block on one XCD, as in ssim_vb_item
 __global__ __launch_bounds__(LB_THREADS) void ssim_vb_batch_kernel(BatchLossArgs b) {
     const uint32_t L = blockIdx.x, rest = L / NUM_XCD;
     const uint32_t c = rest % 3, q = (rest / 3) * NUM_XCD + L % NUM_XCD;
@@ -456,8 +462,7 @@ void launch_loss(const LossArgs &a, hipStream_t s) {
     ssim_h_kernel<<<dim3(nbx, a.height), LB_THREADS, 0, s>>>(a);
     ssim_v_kernel<<<dim3(nbx, nby, 3), LB_THREADS, 0, s>>>(a);
     const uint32_t nvb = nbx * ((a.height + VB_ROWS - 1) / VB_ROWS);
-    ssim_vb_kernel<<<3 * ((nvb + NUM_XCD - 1) / NUM_XCD) * NUM_XCD, LB_THREADS, 0, s>>>(a);
-    loss_rows_kernel<<<a.height, ROW_THREADS, 0, s>>>(a);
+    ssim_vb_rows_kernel<<<a.height + 3 * ((nvb + NUM_XCD - 1) / NUM_XCD) * NUM_XCD, LB_THREADS, 0, s>>>(a);
     loss_total_kernel<<<1, 256, 0, s>>>(a.row_loss, a.height, a.width, a.loss_sum, a.loss_mean);
 }
 
