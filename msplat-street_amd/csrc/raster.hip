@@ -164,14 +164,16 @@ void launch_raster_backward(const RasterArgs &args, const Tuning &t, hipStream_t
 
 namespace {
 
+// Function-local statics: initialised once, thread-safely, on first use (the render
+// server launches frames from several host threads).
 int device_cus() {
-    static int cus = 0;
-    if (!cus) {
-        int dev = 0;
+    static const int cus = [] {
+        int dev = 0, n = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-    }
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        return n;
+    }();
     return cus;
 }
 
@@ -187,15 +189,13 @@ int resident_blocks(K kernel, int threads) {
 
 template <int FB>
 static int fwd_grid() {
-    static int g = 0;
-    if (!g) g = resident_blocks(raster_forward_batch_kernel<FB>, 256);
+    static const int g = resident_blocks(raster_forward_batch_kernel<FB>, 256);
     return g;
 }
 
 template <int PPT, int BB>
 static int bwd_grid() {
-    static int g = 0;
-    if (!g) g = resident_blocks(raster_backward_batch_kernel<PPT, BB>, 256 / PPT);
+    static const int g = resident_blocks(raster_backward_batch_kernel<PPT, BB>, 256 / PPT);
     return g;
 }
 
