@@ -644,9 +644,12 @@ int msplat_hip_radix_sort(msplat_hip_ctx *ctx, uint64_t *keys, int32_t *vals, ui
     fill_u32<<<1, 1, 0, ctx->stream>>>(dn, n);
     uint64_t *kin = keys, *kout = (uint64_t *)ctx->keys1.p;
     int32_t *vin = vals, *vout = (int32_t *)ctx->vals1.p;
+    // only the low key_bits order the keys (higher bits ride along): the last pass is
+    // narrowed to the bits left
     for (int shift = 0; shift < key_bits; shift += 8) {
+        const int width = std::min(8, key_bits - shift);
         launch_radix_pass(kin, vin, kout, vout, dn, n, shift, (uint32_t *)ctx->hist.p, (uint32_t *)ctx->digit_total.p,
-                          ctx->stream);
+                          ctx->stream, (1u << width) - 1u);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }

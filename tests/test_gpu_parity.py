@@ -283,9 +283,12 @@ def test_train_steps_match_golden(ctx):
 
 @pytest.mark.parametrize("n,bits,hi", [(1, 32, 10), (5, 32, 4), (1023, 32, 2 ** 30), (1024, 32, 2 ** 30),
                                        (1025, 32, 7), (70_000, 32, 2 ** 30), (5_000, 64, 2 ** 63), (2_049, 64, 3),
-                                       (300_000, 45, 2 ** 45)])
+                                       (300_000, 45, 2 ** 45), (50_000, 30, 2 ** 34), (3_000, 45, 2 ** 63),
+                                       (4_096, 5, 2 ** 12)])
 def test_radix_sort_matches_numpy(ctx, n, bits, hi):
-    """The reference's own radix-sort cases (tests/test_msplat.py:429-441) and a large one."""
+    """The reference's own radix-sort cases (tests/test_msplat.py:429-441), a large one, and
+    keys with bits set above key_bits, which the reference ignores when it orders them
+    (core/cpu/msplat_cpu.cpp:1484-1486: the comparison masks both keys)."""
     from msplat._abi import DeviceArray, check, lib
     rng = np.random.default_rng(n + bits)
     keys = rng.integers(0, hi, n, dtype=np.uint64)
