@@ -120,6 +120,24 @@ def test_knn_matches_bruteforce_and_oracle(ctx, oracle, kind):
     np.testing.assert_array_equal(got, oracle.knn3(pts))
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 6])
+def test_knn_tiny_clouds(ctx, oracle, n):
+    """Fewer than three neighbours: the mean runs over those there are, and a lone point
+    gets 1.0 (core/cpu/msplat_cpu.cpp:1572-1576). n = 6 repeats one point: all zero."""
+    from msplat._abi import DeviceArray, check, lib
+    rng = np.random.default_rng(n)
+    pts = rng.uniform(-1, 1, (n, 3)).astype(np.float32) if n < 6 else np.repeat(
+        rng.uniform(-1, 1, (1, 3)).astype(np.float32), n, axis=0)
+    dp = DeviceArray.from_numpy(ctx, pts)
+    out = DeviceArray(ctx, (n,))
+    check(lib().msplat_hip_knn3_mean_dist(ctx.ptr, dp.ptr, n, out.ptr))
+    np.testing.assert_array_equal(out.numpy(), oracle.knn3(pts))
+    if n == 1:
+        assert out.numpy()[0] == 1.0
+    if n == 6:
+        assert (out.numpy() == 0.0).all()
+
+
 def test_knn_large_street_cloud(ctx, oracle):
     """The street recipe at 200k points: bit-exact against the oracle's grid search."""
     from msplat._abi import DeviceArray, check, lib

@@ -281,6 +281,52 @@ def test_train_steps_match_golden(ctx):
         assert v < 1e-4, (k, v)
 
 
+def test_train_steps_with_an_empty_frame_match_oracle(ctx, oracle):
+    """A camera that sees no Gaussian (all behind it: no tile intersection), then one that
+    sees them, then the empty one again. The empty frame renders the background, and
+    Adam still runs on every Gaussian with zero gradients (the moments decay and the
+    parameters move by the decayed first moment)."""
+    from msplat._abi import DeviceArray
+    from msplat.backend import DeviceGaussians, train_step
+    from msplat.camera import look_at, make_camera
+    sc, cam = _scene_cam(seed=31, n=2000, deg=3, W=97, H=71)
+    away = make_camera(look_at((0.0, 0.0, 5.0), (0.0, 0.0, 10.0)), 97, 71, 80.0, 83.0, 47.3, 36.4, dtu=3)
+    dg = DeviceGaussians(ctx, sc)
+    st = {k: np.ascontiguousarray(sc[k], np.float32).copy() for k in KEYS}
+    for k in KEYS:
+        st["m1_" + k] = np.zeros_like(st[k])
+        st["m2_" + k] = np.zeros_like(st[k])
+    n = sc["means"].shape[0]
+    st.update(degree=3, vis=np.zeros(n, np.float32), gn=np.zeros(n, np.float32), ms=np.zeros(n, np.float32))
+    rng = np.random.default_rng(32)
+    steps = np.array([1.6e-4, 5e-3, 1e-3, 2.5e-3, 1.25e-4, 5e-2], np.float32)
+    for t, c in enumerate((away, cam, away)):
+        gt_np = rng.uniform(0, 1, (71, 97, 3)).astype(np.float32)
+        bc2 = np.full(6, np.float32(np.sqrt(1 - 0.999 ** (t + 1))), np.float32)
+        loss = train_step(ctx, dg, c, BG, DeviceArray.from_numpy(ctx, gt_np), steps, bc2)
+        ref_loss, ref_radii, ref_img = oracle.train_step(st, c, BG, gt_np, steps, bc2)
+        f = ctx.frame()
+        assert (f.num_isects == 0) == (c is away), (t, f.num_isects)
+        assert abs(loss - ref_loss) <= 1e-6 * abs(ref_loss), (t, loss, ref_loss)
+        radii = dg.radii.numpy()[:n]
+        if c is away:
+            assert not radii.any()
+            np.testing.assert_array_equal(ref_img, np.broadcast_to(BG, ref_img.shape))
+            np.testing.assert_array_equal(ctx.download(f.image, (71, 97, 3), np.float32), ref_img)
+        else:
+            assert np.mean(radii == ref_radii) > 0.995
+    got = dg.to_numpy()
+    errs = {}
+    for k in KEYS:
+        for pre in ("", "m1_", "m2_"):
+            errs[pre + k] = _rel(got[pre + k], st[pre + k].reshape(got[pre + k].shape))
+    for k in ("vis", "gn", "ms"):
+        errs[k] = _rel(got[k], st[k])
+    print("\nempty-frame train-step rel errors:", {k: f"{v:.1e}" for k, v in errs.items()})
+    for k, v in errs.items():
+        assert v < 1e-4, (k, v)
+
+
 @pytest.mark.parametrize("n,bits,hi", [(1, 32, 10), (5, 32, 4), (1023, 32, 2 ** 30), (1024, 32, 2 ** 30),
                                        (1025, 32, 7), (70_000, 32, 2 ** 30), (5_000, 64, 2 ** 63), (2_049, 64, 3),
                                        (300_000, 45, 2 ** 45), (50_000, 30, 2 ** 34), (3_000, 45, 2 ** 63),

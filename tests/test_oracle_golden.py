@@ -217,6 +217,21 @@ def test_knn_bruteforce(oracle):
     np.testing.assert_allclose(oracle.knn3(pts), ref, rtol=1e-5, atol=1e-6)
 
 
+def _knn_tiny_expected(pts):
+    """Mean of the up-to-3 nearest distances; 1.0 for a point with no neighbour
+    (core/cpu/msplat_cpu.cpp:1572-1576)."""
+    d2 = ((pts[:, None, :].astype(np.float64) - pts[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(d2, np.inf)
+    k = min(3, pts.shape[0] - 1)
+    return np.sqrt(np.sort(d2, axis=1)[:, :k]).mean(axis=1) if k else np.ones(pts.shape[0])
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4])
+def test_knn_fewer_than_three_neighbours(oracle, n):
+    pts = np.random.default_rng(n).uniform(-1, 1, (n, 3)).astype(np.float32)
+    np.testing.assert_allclose(oracle.knn3(pts), _knn_tiny_expected(pts), rtol=1e-6, atol=1e-7)
+
+
 def test_render_double_agrees_with_float(oracle):
     """The FP64 renderer agrees with the FP32 one to float rounding on a smooth scene."""
     from msplat.scenes import random_scene
