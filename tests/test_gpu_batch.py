@@ -167,6 +167,37 @@ def test_one_view_m1_reference_uses_beta1_one_minus(ctx):
     np.testing.assert_array_equal(a.to_numpy()["m1_means"], b.to_numpy()["m1_means"])
 
 
+def test_batch_with_an_empty_view(ctx):
+    """A view that sees no Gaussian (all behind its camera) inside a batch: its image is
+    the background, the other views render as alone, and the batch's gradients are the
+    sum over the views that see something."""
+    from msplat.backend import BatchViews, render_batch_numpy, render_numpy, train_step, train_step_batch
+    from msplat.camera import look_at, make_camera
+    sc, cams = _setup(seed=13, sizes=[(97, 71), (64, 48)])
+    away = make_camera(look_at((0.0, 0.0, 5.0), (0.0, 0.0, 10.0)), 80, 60, 70.0, 70.0, dtu=3)
+    views = [cams[0], away, cams[1]]
+    dg = _state(ctx, sc)
+    imgs = render_batch_numpy(ctx, dg, BatchViews(views, 3, [BG]))
+    for cam, img in zip(views, imgs):
+        np.testing.assert_array_equal(img, render_numpy(ctx, dg, cam, BG))
+    np.testing.assert_array_equal(imgs[1], np.broadcast_to(np.float32(BG), imgs[1].shape))
+    gts = _targets(ctx, views, seed=14)
+    zero, ones = np.zeros(6, np.float32), np.ones(6, np.float32)
+    m1_sum, losses = {}, []
+    for cam, gt in zip(views, gts):
+        one = _state(ctx, sc)
+        losses.append(train_step(ctx, one, cam, BG, gt, zero, ones, ssim_weight=0.2, beta1=0.0))
+        st = one.to_numpy()
+        for k in ("means", "scales", "quats", "dc", "rest", "opac"):
+            m1_sum[k] = m1_sum.get(k, 0) + st["m1_" + k].astype(np.float64)
+    bat = _state(ctx, sc)
+    lb = train_step_batch(ctx, bat, BatchViews(views, 3, [BG], gts), zero, ones, ssim_weight=0.2, beta1=0.0)
+    np.testing.assert_array_equal(lb, np.array(losses, np.float32))
+    got = bat.to_numpy()
+    for k in m1_sum:
+        _close(got["m1_" + k].astype(np.float64), m1_sum[k], 2e-5)
+
+
 def test_graph_replays_equal_eager_steps(ctx):
     """Captured step replayed with new poses, resolutions (within the buffers) and Adam
     scalars, with eager work and syncs between replays, equals the eager steps bit for bit."""
