@@ -327,6 +327,28 @@ def test_train_steps_with_an_empty_frame_match_oracle(ctx, oracle):
         assert v < 1e-4, (k, v)
 
 
+def test_train_step_without_gaussians_matches_oracle(ctx, oracle):
+    """Zero Gaussians: the frame is the background and the loss is the oracle's."""
+    from msplat._abi import DeviceArray
+    from msplat.backend import DeviceGaussians, render_numpy, train_step
+    from msplat.scenes import random_scene
+    _, cam = _scene_cam(seed=1, n=1, deg=3)
+    sc = random_scene(33, 0, degree=3)
+    dg = DeviceGaussians(ctx, sc)
+    np.testing.assert_array_equal(render_numpy(ctx, dg, cam, BG), np.broadcast_to(BG, (71, 97, 3)))
+    gt_np = np.random.default_rng(34).uniform(0, 1, (71, 97, 3)).astype(np.float32)
+    st = {k: np.ascontiguousarray(sc[k], np.float32).copy() for k in KEYS}
+    for k in KEYS:
+        st["m1_" + k] = np.zeros_like(st[k])
+        st["m2_" + k] = np.zeros_like(st[k])
+    st.update(degree=3, vis=np.zeros(0, np.float32), gn=np.zeros(0, np.float32), ms=np.zeros(0, np.float32))
+    steps, bc2 = np.full(6, 1e-3, np.float32), np.full(6, 0.5, np.float32)
+    loss = train_step(ctx, dg, cam, BG, DeviceArray.from_numpy(ctx, gt_np), steps, bc2)
+    ref_loss, _, _ = oracle.train_step(st, cam, BG, gt_np, steps, bc2)
+    assert ctx.frame().num_isects == 0
+    assert loss == np.float32(ref_loss), (loss, ref_loss)
+
+
 @pytest.mark.parametrize("n,bits,hi", [(1, 32, 10), (5, 32, 4), (1023, 32, 2 ** 30), (1024, 32, 2 ** 30),
                                        (1025, 32, 7), (70_000, 32, 2 ** 30), (5_000, 64, 2 ** 63), (2_049, 64, 3),
                                        (300_000, 45, 2 ** 45), (50_000, 30, 2 ** 34), (3_000, 45, 2 ** 63),
