@@ -157,7 +157,8 @@ typedef struct {
     uint64_t *slots_written;
 } msplat_hip_frame;
 int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *out);
-/* KEEP_GRADS: fill the reduced gradients; KEYS64: materialise isect_keys on frame_view. */
+/* KEEP_GRADS: fill the reduced gradients; KEYS64: materialise isect_keys on frame_view
+ * (and isect_gid, when the tile sort packed the ids into its keys). */
 enum { MSPLAT_FRAME_KEEP_GRADS = 1, MSPLAT_FRAME_KEYS64 = 2 };
 int msplat_hip_set_flags(msplat_hip_ctx *ctx, int flags);
 
@@ -172,13 +173,18 @@ int msplat_hip_set_flags(msplat_hip_ctx *ctx, int flags);
  *                          alpha >= 1/255 ellipse (widened for float rounding) misses
  *                          the tile; 0: every listed gaussian is visited. Skipped pairs
  *                          would composite nothing, so both give identical results.
+ *   MSPLAT_TUNE_PACK_KEYS  1 (default): when the tile-id and gaussian-id bits fit in 32,
+ *                          the one-view tile sort moves (tile << id bits | id) keys alone
+ *                          instead of tile keys plus id values (same order, half the
+ *                          bytes); 0: always keys plus values.
  * The binning tile stays 16x16: it decides which gaussians reach a pixel. */
 enum {
     MSPLAT_TUNE_BWD_PPT = 0,
     MSPLAT_TUNE_FWD_BATCH = 1,
     MSPLAT_TUNE_BWD_BATCH = 2,
     MSPLAT_TUNE_CULL = 3,
-    MSPLAT_NUM_TUNABLES = 4
+    MSPLAT_TUNE_PACK_KEYS = 4,
+    MSPLAT_NUM_TUNABLES = 5
 };
 int msplat_hip_set_tuning(msplat_hip_ctx *ctx, int key, int value);
 int msplat_hip_get_tuning(msplat_hip_ctx *ctx, int key, int *value);

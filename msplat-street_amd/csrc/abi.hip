@@ -179,29 +179,37 @@ int do_bin(msplat_hip_ctx *ctx) {
     CHECKED(ctx->grow(ctx->vals1, 4 * Mc));
     CHECKED(ctx->grow(ctx->hist, 4 * radix_hist_words((uint32_t)std::max(Mc, Nc))));
     CHECKED(ctx->grow(ctx->slot_f, (Mc + 3) & ~(size_t)3));
+    // (tile << gbits) | id fits in 32 bits: the sort moves the keys alone
+    const int bits = tile_bits(ctx->ntiles);
+    int gbits = 1;
+    while (gbits < 31 && (1u << gbits) < (uint32_t)Nc) gbits++;
+    const bool packed = ctx->tuning.pack_keys && gbits + bits <= 32;
     k.m_max = (uint32_t)M;
     k.keys = (uint32_t *)ctx->keys0.p;
-    k.vals = (int32_t *)ctx->vals0.p;
+    k.vals = packed ? nullptr : (int32_t *)ctx->vals0.p;
+    k.gid_bits = packed ? (uint32_t)gbits : 0u;
     k.slot_f = (uint8_t *)ctx->slot_f.p;
     launch_keygen_emit(k, false, s);
     ctx->slot_f_clean = true;
     uint32_t *kin = (uint32_t *)ctx->keys0.p, *kout = (uint32_t *)ctx->keys1.p;
-    int32_t *vin = (int32_t *)ctx->vals0.p, *vout = (int32_t *)ctx->vals1.p;
+    int32_t *vin = packed ? nullptr : (int32_t *)ctx->vals0.p, *vout = packed ? nullptr : (int32_t *)ctx->vals1.p;
+    const int shift0 = packed ? gbits : 0;
     // The first pass takes the bits left over by whole bytes (the last pass takes 8), so
     // that it has fewer, longer digit runs: its input is in depth order, the least
     // coalesced scatter of the sort.
-    const int bits = tile_bits(ctx->ntiles);
     for (int done = 0; done < bits;) {
         const int width = done == 0 && bits % 8 ? bits % 8 : 8;
-        launch_radix_pass(kin, vin, kout, vout, ctx->d_total32(), (uint32_t)M, done, (uint32_t *)ctx->hist.p,
-                          (uint32_t *)ctx->digit_total.p, s, (1u << width) - 1u);
+        launch_radix_pass(kin, vin, kout, vout, ctx->d_total32(), (uint32_t)M, shift0 + done,
+                          (uint32_t *)ctx->hist.p, (uint32_t *)ctx->digit_total.p, s, (1u << width) - 1u);
         std::swap(kin, kout);
         std::swap(vin, vout);
         done += width;
     }
     ctx->keys_sorted = kin;
     ctx->gid_sorted = vin;
-    launch_tile_ranges(kin, ctx->d_total32(), (uint32_t)M, nullptr, ctx->ntiles, (int32_t *)ctx->tile_start.p, s);
+    ctx->sort_gid_bits = shift0;
+    launch_tile_ranges(kin, ctx->d_total32(), (uint32_t)M, nullptr, ctx->ntiles, (int32_t *)ctx->tile_start.p, s,
+                       shift0);
     ctx->end();
     HIPCHECK(hipGetLastError());
     return MSPLAT_OK;
@@ -228,7 +236,8 @@ RasterArgs raster_args(msplat_hip_ctx *ctx) {
     r.tiles_x = ctx->tiles_x;
     r.tiles_y = ctx->tiles_y;
     r.tile_start = (const int32_t *)ctx->tile_start.p;
-    r.gid = ctx->gid_sorted;
+    r.gid = ctx->sort_gid_bits ? (const int32_t *)ctx->keys_sorted : ctx->gid_sorted;
+    r.gid_mask = ctx->sort_gid_bits ? (1u << ctx->sort_gid_bits) - 1u : 0xFFFFFFFFu;
     r.splat_a = (const float4 *)ctx->splat_a.p;
     r.splat_b = (const float4 *)ctx->splat_b.p;
     r.splat_blue = (const float *)ctx->splat_blue.p;
@@ -602,14 +611,23 @@ int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *f) {
     f->tile_count = (uint32_t *)ctx->count.p;
     f->offsets = (uint32_t *)ctx->offsets.p;
     f->isect_keys = nullptr;
+    f->isect_gid = ctx->gid_sorted;
     if ((ctx->flags & MSPLAT_FRAME_KEYS64) && ctx->keys_sorted) {
+        const uint32_t *tiles = ctx->keys_sorted;
+        if (ctx->sort_gid_bits) {  // split the packed keys into the key buffer not holding them, and vals0
+            uint32_t *spare = ctx->keys_sorted == (uint32_t *)ctx->keys0.p ? (uint32_t *)ctx->keys1.p
+                                                                           : (uint32_t *)ctx->keys0.p;
+            launch_unpack_keys(ctx->keys_sorted, (uint32_t)ctx->M, ctx->sort_gid_bits, spare,
+                               (int32_t *)ctx->vals0.p, ctx->stream);
+            tiles = spare;
+            f->isect_gid = (int32_t *)ctx->vals0.p;
+        }
         CHECKED(ctx->grow(ctx->keys64, 8 * (size_t)(ctx->M > 0 ? ctx->M : 1)));
-        launch_keys64(ctx->keys_sorted, ctx->gid_sorted, (const float *)ctx->depths.p, (uint32_t)ctx->M,
+        launch_keys64(tiles, f->isect_gid, (const float *)ctx->depths.p, (uint32_t)ctx->M,
                       (uint64_t *)ctx->keys64.p, ctx->stream);
         HIPCHECK(hipGetLastError());
         f->isect_keys = (uint64_t *)ctx->keys64.p;
     }
-    f->isect_gid = ctx->gid_sorted;
     f->tile_start = (int32_t *)ctx->tile_start.p;
     f->image = (float *)ctx->image.p;
     f->final_T = (float *)ctx->final_T.p;
@@ -661,6 +679,7 @@ int msplat_hip_radix_sort(msplat_hip_ctx *ctx, uint64_t *keys, int32_t *vals, ui
     // the frame's sorted intersections lived in keys1/vals1; they are no longer valid
     ctx->keys_sorted = nullptr;
     ctx->gid_sorted = nullptr;
+    ctx->sort_gid_bits = 0;
     return MSPLAT_OK;
 }
 
@@ -835,6 +854,10 @@ int msplat_hip_set_tuning(msplat_hip_ctx *ctx, int key, int value) {
         if (value != 0 && value != 1) return set_error(MSPLAT_ERR_ARG, "cull must be 0 or 1");
         ctx->tuning.cull = value;
         return MSPLAT_OK;
+    case MSPLAT_TUNE_PACK_KEYS:
+        if (value != 0 && value != 1) return set_error(MSPLAT_ERR_ARG, "pack_keys must be 0 or 1");
+        ctx->tuning.pack_keys = value;
+        return MSPLAT_OK;
     default:
         return set_error(MSPLAT_ERR_ARG, "unknown tunable %d", key);
     }
@@ -847,6 +870,7 @@ int msplat_hip_get_tuning(msplat_hip_ctx *ctx, int key, int *value) {
     case MSPLAT_TUNE_FWD_BATCH: *value = ctx->tuning.fwd_batch; return MSPLAT_OK;
     case MSPLAT_TUNE_BWD_BATCH: *value = ctx->tuning.bwd_batch; return MSPLAT_OK;
     case MSPLAT_TUNE_CULL: *value = ctx->tuning.cull; return MSPLAT_OK;
+    case MSPLAT_TUNE_PACK_KEYS: *value = ctx->tuning.pack_keys; return MSPLAT_OK;
     default: return set_error(MSPLAT_ERR_ARG, "unknown tunable %d", key);
     }
 }

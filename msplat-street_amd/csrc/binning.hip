@@ -360,11 +360,12 @@ __device__ inline void rs_scatter_block(const K *kin, const int32_t *vin, K *kou
     const uint32_t wstart = start + (uint32_t)w * (64 * R);
     K key[R];
     int32_t val[R];
+    const bool has_vals = vin != nullptr;  // null: the keys carry their payload
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const uint32_t e = wstart + r * 64 + lane;
         key[r] = e < n ? kin[e] : K(0);
-        val[r] = e < n ? vin[e] : 0;
+        val[r] = has_vals && e < n ? vin[e] : 0;
     }
     wave_sync();
     // per element: digit (bits 24..31) and rank among the wave's elements of that digit
@@ -404,7 +405,7 @@ __device__ inline void rs_scatter_block(const K *kin, const int32_t *vin, K *kou
         if (wstart + r * 64 + lane < n) {
             const uint32_t p = wcnt[w][dr[r] >> 24] + (dr[r] & 0xFFFFFF);
             sk[p] = key[r];
-            sv[p] = val[r];
+            if (has_vals) sv[p] = val[r];
         }
     }
     __syncthreads();
@@ -414,7 +415,7 @@ __device__ inline void rs_scatter_block(const K *kin, const int32_t *vin, K *kou
         const uint32_t d = (uint32_t)(k >> shift) & mask;
         const uint32_t o = gbase[d] + (p - lstart[d]);
         kout[o] = k;
-        vout[o] = sv[p];
+        if (has_vals) vout[o] = sv[p];
     }
 }
 
@@ -554,8 +555,13 @@ __global__ __launch_bounds__(256) void keygen_emit_kernel(KeygenArgs a) {
         const int ty = y0 + (int)(local / w), tx = x0 + (int)(local % w);
         // local tile id (a batch keeps its views apart by the emission order)
         const uint32_t tiles_x = BATCH ? (uint32_t)a.tab->views[r / a.n].cam.tiles_x : (uint32_t)a.tiles_x;
-        a.keys[o] = (uint32_t)ty * tiles_x + (uint32_t)tx;
-        a.vals[o] = r;
+        const uint32_t tile = (uint32_t)ty * tiles_x + (uint32_t)tx;
+        if (!BATCH && a.gid_bits) {
+            a.keys[o] = (tile << a.gid_bits) | (uint32_t)r;
+        } else {
+            a.keys[o] = tile;
+            a.vals[o] = r;
+        }
         if (a.slot_f) a.slot_f[o] = 0;  // output o is the record's slot o (emission order)
     }
 }
@@ -563,13 +569,13 @@ __global__ __launch_bounds__(256) void keygen_emit_kernel(KeygenArgs a) {
 // tile_start[t] = first sorted intersection with tile >= t, for t in [0, T]
 __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t *keys, const uint32_t *n_ptr,
                                                          const uint32_t *ntiles_ptr, int ntiles_host,
-                                                         int32_t *tile_start) {
+                                                         int32_t *tile_start, int shift) {
     const uint32_t n = *n_ptr;
     const int ntiles = ntiles_ptr ? (int)*ntiles_ptr : ntiles_host;
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
     if (k > n) return;
-    const int t = k < n ? (int)keys[k] : ntiles;
-    const int tp = k > 0 ? (int)keys[k - 1] : -1;
+    const int t = k < n ? (int)(keys[k] >> shift) : ntiles;
+    const int tp = k > 0 ? (int)(keys[k - 1] >> shift) : -1;
     for (int u = tp + 1; u <= t; u++) tile_start[u] = (int32_t)k;
 }
 
@@ -594,6 +600,15 @@ __global__ __launch_bounds__(256) void tile_ranges_views_kernel(const uint32_t *
     const int c = k < m ? (int)(keys[k] * V + (uint32_t)(recs[k] / n)) : total;
     const int cp = k > 0 ? (int)(keys[k - 1] * V + (uint32_t)(recs[k - 1] / n)) : -1;
     for (int u = cp + 1; u <= c; u++) range_start[u] = (int32_t)k;
+}
+
+__global__ __launch_bounds__(256) void unpack_keys_kernel(const uint32_t *packed, uint32_t n, int shift,
+                                                         uint32_t *tiles, int32_t *gid) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t v = packed[k];
+    tiles[k] = v >> shift;
+    gid[k] = (int32_t)(v & ((1u << shift) - 1u));
 }
 
 // The 64-bit (tile << 32 | depth bits) view of the sorted intersections
@@ -782,9 +797,13 @@ void launch_keygen(const KeygenArgs &a, bool batch, hipStream_t s) {
 }
 
 void launch_tile_ranges(const uint32_t *keys, const uint32_t *n_dev, uint32_t n_max, const uint32_t *ntiles_dev,
-                        int ntiles, int32_t *tile_start, hipStream_t s) {
+                        int ntiles, int32_t *tile_start, hipStream_t s, int shift) {
     const uint32_t nb = (n_max + 1 + 255) / 256;
-    tile_ranges_kernel<<<nb, 256, 0, s>>>(keys, n_dev, ntiles_dev, ntiles, tile_start);
+    tile_ranges_kernel<<<nb, 256, 0, s>>>(keys, n_dev, ntiles_dev, ntiles, tile_start, shift);
+}
+
+void launch_unpack_keys(const uint32_t *packed, uint32_t n, int shift, uint32_t *tiles, int32_t *gid, hipStream_t s) {
+    if (n) unpack_keys_kernel<<<(n + 255) / 256, 256, 0, s>>>(packed, n, shift, tiles, gid);
 }
 
 void launch_keys64(const uint32_t *tiles, const int32_t *gid, const float *depths, uint32_t n, uint64_t *out,

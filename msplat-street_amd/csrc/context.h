@@ -154,8 +154,9 @@ struct msplat_hip_ctx {
     int tiles_x = 0, tiles_y = 0;
     int64_t M = 0;
     bool slot_f_clean = false;  // slot_f[0, M) is all zero: set by the key emission, cleared by a raster backward
-    uint32_t *keys_sorted = nullptr;  // tile ids of the sorted intersections
-    int32_t *gid_sorted = nullptr;
+    uint32_t *keys_sorted = nullptr;  // sorted keys: tile ids, or (tile << sort_gid_bits) | id
+    int32_t *gid_sorted = nullptr;    // sorted gaussian ids (null when packed into the keys)
+    int sort_gid_bits = 0;            // > 0: the ids are the keys' low sort_gid_bits bits
     float bg[3] = {0, 0, 0};
 
     int grow(Buf &b, size_t bytes) {

@@ -17,6 +17,7 @@ struct Tuning {
     int fwd_batch = 256;  // forward LDS batch depth: 64, 128, 256
     int bwd_batch = 128;  // backward LDS batch depth: 32, 64, 128
     int cull = 1;         // skip a tile's gaussians whose alpha >= 1/255 ellipse misses it
+    int pack_keys = 1;    // one-view tile sort on (tile | id) keys when they fit in 32 bits
 };
 
 struct ProjectArgs {
@@ -68,8 +69,9 @@ struct KeygenArgs {
     uint32_t *loff;         // offsets in list order (nvis_max entries)
     const uint32_t *m_ptr;  // device intersection count
     uint32_t m_max;         // grid bound of the emission
-    uint32_t *keys;  // global tile id
-    int32_t *vals;   // record id
+    uint32_t *keys;  // global tile id, or (tile id << gid_bits) | record id
+    int32_t *vals;   // record id (unused when gid_bits > 0)
+    uint32_t gid_bits;  // one view: > 0 packs the record id into the key's low bits
     uint32_t *big_list, *big_count;  // records with count > BIG_SLOTS
     uint8_t *slot_f;                 // null, or the slots' zero flags, cleared by the emission
     const uint32_t *overflow;        // null, or skip everything when set
@@ -82,6 +84,7 @@ struct RasterArgs {
     uint32_t width, height;
     int tiles_x, tiles_y;
     const int32_t *tile_start, *gid;
+    uint32_t gid_mask;  // record id = gid[k] & gid_mask (the sort key may hold the tile id above it)
     const float4 *splat_a, *splat_b;
     const float *splat_blue;
     float bg[3];
@@ -335,7 +338,9 @@ void launch_radix_pass(const uint32_t *kin, const int32_t *vin, uint32_t *kout, 
                        uint32_t mask = 0xFF);
 size_t radix_hist_words(uint32_t n_max);
 void launch_tile_ranges(const uint32_t *keys, const uint32_t *n_dev, uint32_t n_max, const uint32_t *ntiles_dev,
-                        int ntiles, int32_t *tile_start, hipStream_t s);
+                        int ntiles, int32_t *tile_start, hipStream_t s, int shift = 0);
+// tile ids (key >> shift) and record ids (key & (2^shift - 1)) of packed sorted keys
+void launch_unpack_keys(const uint32_t *packed, uint32_t n, int shift, uint32_t *tiles, int32_t *gid, hipStream_t s);
 void launch_keys64(const uint32_t *tiles, const int32_t *gid, const float *depths, uint32_t n, uint64_t *out,
                    hipStream_t s);
 // batch: view ids of the depth-sorted records (record / n) as sort keys

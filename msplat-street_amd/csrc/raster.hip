@@ -8,8 +8,8 @@ namespace msplat {
 namespace {
 
 __device__ inline TileData tile_data(const RasterArgs &a) {
-    return TileData{a.tile_start, a.gid,    a.splat_a, a.splat_b, a.splat_blue,     a.rect,
-                    a.offsets,    a.slot_a, a.slot_b,  a.slot_c,  a.slot_f, a.slots_written, a.cull != 0};
+    return TileData{a.tile_start, a.gid,    a.gid_mask, a.splat_a, a.splat_b, a.splat_blue,     a.rect,
+                    a.offsets,    a.slot_a, a.slot_b,   a.slot_c,  a.slot_f,  a.slots_written, a.cull != 0};
 }
 
 __device__ inline TileView tile_view(const RasterArgs &a) {
@@ -18,8 +18,8 @@ __device__ inline TileView tile_view(const RasterArgs &a) {
 }
 
 __device__ inline TileData tile_data(const BatchRasterArgs &a) {
-    return TileData{a.tile_start, a.gid,    a.splat_a, a.splat_b, a.splat_blue, a.rect,
-                    a.offsets,    a.slot_a, a.slot_b,  a.slot_c,  a.slot_f, nullptr,        a.cull != 0};
+    return TileData{a.tile_start, a.gid,    0xFFFFFFFFu, a.splat_a, a.splat_b, a.splat_blue, a.rect,
+                    a.offsets,    a.slot_a, a.slot_b,    a.slot_c,  a.slot_f,  nullptr,      a.cull != 0};
 }
 
 // View v of a batch: per-pixel buffers offset to the view's first pixel

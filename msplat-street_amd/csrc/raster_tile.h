@@ -34,6 +34,7 @@ struct TileView {
 // per-record splats and tile rectangles (in the record's view), and gradient slots.
 struct TileData {
     const int32_t *tile_start, *gid;
+    uint32_t gid_mask;  // record id = gid[k] & gid_mask
     const float4 *splat_a, *splat_b;
     const float *splat_blue;
     const uint2 *rect;
@@ -125,7 +126,7 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
         float4 A0, B0;
         float C0 = 0.f;
         if (threadIdx.x < FB && k < e) {
-            const int g = d.gid[k];
+            const int g = (int)((uint32_t)d.gid[k] & d.gid_mask);
             A0 = d.splat_a[g];
             B0 = d.splat_b[g];
             C0 = d.splat_blue[g];
@@ -276,7 +277,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             float C0 = 0.f;
             uint32_t p = 0;
             if (i < n) {
-                const int g = d.gid[start + i];
+                const int g = (int)((uint32_t)d.gid[start + i] & d.gid_mask);
                 A0 = d.splat_a[g];
                 B0 = d.splat_b[g];
                 C0 = d.splat_blue[g];

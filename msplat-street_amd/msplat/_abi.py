@@ -21,6 +21,7 @@ TUNE_BWD_PPT = 0
 TUNE_FWD_BATCH = 1
 TUNE_BWD_BATCH = 2
 TUNE_CULL = 3
+TUNE_PACK_KEYS = 4
 # stage timing: stage s is bit (1 << s) of a stage mask
 STAGE_NAMES = ("proj_sh_fwd", "prefix_sort_pack", "rast_fwd", "loss_fwd_bwd", "rast_bwd", "proj_sh_bwd_adam")
 fp = C.POINTER(C.c_float)

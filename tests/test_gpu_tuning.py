@@ -55,6 +55,43 @@ def test_variants_give_the_same_frame(ctx):
             ctx.set_tuning(k, v)
 
 
+def test_packed_sort_keys_give_the_same_frame(ctx):
+    """The tile sort on packed (tile << id bits | id) keys and on tile keys plus id values
+    give the same sorted lists, tile ranges, image and gradients, bit for bit, for a
+    scene whose ids need 13 bits (packed) and for the same frame with packing off."""
+    from msplat._abi import FRAME_KEYS64, TUNE_PACK_KEYS, DeviceArray
+    from msplat.backend import DeviceGaussians
+    from msplat.camera import look_at, make_camera
+    from msplat.scenes import random_scene
+    sc = random_scene(35, 6000, degree=3, log_scale=(-3.4, 0.5))
+    cam = make_camera(look_at((2.4, 0.9, 2.6), (0.0, 0.0, 0.0)), 131, 77, 95.0, 97.0, dtu=3)
+    dg = DeviceGaussians(ctx, sc)
+    v_img = DeviceArray.from_numpy(ctx, (np.random.default_rng(4).normal(0, 1e-3, (77, 131, 3))).astype(np.float32))
+    saved = ctx.get_tuning(TUNE_PACK_KEYS)
+    out = []
+    try:
+        ctx.set_flags(FRAME_KEYS64)
+        for pack in (1, 0):
+            ctx.set_tuning(TUNE_PACK_KEYS, pack)
+            img, fT, fi, grads = _frame(ctx, dg, cam, v_img)
+            f = ctx.frame()
+            M, T = f.num_isects, f.num_tiles
+            lists = (ctx.download(f.isect_gid, (M,), np.int32), ctx.download(f.isect_keys, (M,), np.uint64),
+                     ctx.download(f.tile_start, (T + 1,), np.int32))
+            out.append((img, fT, fi, grads, lists))
+    finally:
+        ctx.set_flags(0)
+        ctx.set_tuning(TUNE_PACK_KEYS, saved)
+    (a, b) = out
+    assert a[4][0].size > 10_000
+    for x, y in zip(a[4], b[4]):
+        np.testing.assert_array_equal(x, y)
+    for x, y in zip(a[:3], b[:3]):
+        np.testing.assert_array_equal(x, y)
+    for k in a[3]:
+        np.testing.assert_array_equal(a[3][k], b[3][k], err_msg=k)
+
+
 def test_autotuner_picks_a_valid_variant(ctx, tmp_path):
     from msplat import autotune
     from msplat.backend import DeviceGaussians, render
