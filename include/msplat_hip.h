@@ -133,8 +133,10 @@ int msplat_hip_render_backward(msplat_hip_ctx *ctx, const msplat_hip_gaussians *
  * and tools. Per gaussian: xys (N,2) as float4 records (x, y, conic.x, conic.y) and
  * (conic.z, sigmoid(opacity), r, g) plus blue (N), depths, radii, aabb (N,2), tile
  * counts, prefix offsets. Per intersection (sorted): keys (tile<<32 | depth bits,
- * filled by frame_view when MSPLAT_FRAME_KEYS64 is set, else NULL), gaussian ids.
- * tile_start (T+1): tile t owns sorted intersections [tile_start[t], tile_start[t+1]). Per pixel: image, final_T, final_idx (index into
+ * filled by frame_view when MSPLAT_FRAME_KEYS64 is set, else NULL), gaussian ids (NULL
+ * without MSPLAT_FRAME_KEYS64 when MSPLAT_TUNE_PACK_KEYS packed them into the sort keys).
+ * tile_start (T+1): tile t owns sorted intersections [tile_start[t], tile_start[t+1]).
+ * Per pixel: image, final_T, final_idx (index into
  * the sorted intersections), dL/d(image). Per gaussian reduced gradients (N,2)/(N,3)/
  * (N,3)/(N) are filled when MSPLAT_FRAME_KEEP_GRADS is set. loss_sum: the last loss's
  * double sum of the per-pixel losses (one double, device memory). slots_written: a
