@@ -310,7 +310,9 @@ __global__ __launch_bounds__(256) void rs_hist_kernel(const K *keys, const uint3
     }
 }
 
-// one block per digit: exclusive scan of that digit's counts over blocks, digit total
+// one block per digit: exclusive scan of that digit's counts over blocks, digit total;
+// each thread takes SCAN_RUN consecutive counts, so 2048 blocks take one round
+constexpr int SCAN_RUN = 8;
 template <int ITEMS>
 __global__ __launch_bounds__(256) void rs_scan_kernel(uint32_t *hist, int stride, const uint32_t *n_ptr,
                                                      uint32_t *digit_total) {
@@ -320,12 +322,23 @@ __global__ __launch_bounds__(256) void rs_scan_kernel(uint32_t *hist, int stride
     uint32_t *row = hist + (size_t)blockIdx.x * stride;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    for (int base = 0; base < nblocks; base += 256) {
-        const int i = base + threadIdx.x;
-        uint32_t v = i < nblocks ? row[i] : 0, t;
-        uint32_t ex = block_scan_u32(v, t, lds4);
-        uint32_t c = carry;
-        if (i < nblocks) row[i] = c + ex;
+    for (int base = 0; base < nblocks; base += 256 * SCAN_RUN) {
+        const int i0 = base + threadIdx.x * SCAN_RUN;
+        uint32_t v[SCAN_RUN], sum = 0, t;
+#pragma unroll
+        for (int j = 0; j < SCAN_RUN; j++) {
+            v[j] = i0 + j < nblocks ? row[i0 + j] : 0u;
+            sum += v[j];
+        }
+        const uint32_t ex = block_scan_u32(sum, t, lds4);
+        const uint32_t c = carry;
+        uint32_t run = c + ex;
+#pragma unroll
+        for (int j = 0; j < SCAN_RUN; j++)
+            if (i0 + j < nblocks) {
+                row[i0 + j] = run;
+                run += v[j];
+            }
         __syncthreads();
         if (threadIdx.x == 0) carry = c + t;
         __syncthreads();
