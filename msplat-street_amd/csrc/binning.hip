@@ -583,13 +583,28 @@ __global__ __launch_bounds__(256) void keygen_emit_kernel(KeygenArgs a) {
 __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t *keys, const uint32_t *n_ptr,
                                                          const uint32_t *ntiles_ptr, int ntiles_host,
                                                          int32_t *tile_start, int shift) {
+    // each thread takes 4 consecutive boundaries k0..k0+3 (one 16-byte load inside the list)
     const uint32_t n = *n_ptr;
     const int ntiles = ntiles_ptr ? (int)*ntiles_ptr : ntiles_host;
-    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-    if (k > n) return;
-    const int t = k < n ? (int)(keys[k] >> shift) : ntiles;
-    const int tp = k > 0 ? (int)(keys[k - 1] >> shift) : -1;
-    for (int u = tp + 1; u <= t; u++) tile_start[u] = (int32_t)k;
+    const uint32_t k0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+    if (k0 > n) return;
+    uint32_t v[4];
+    if (k0 + 4 <= n) {
+        const uint4 q = *(const uint4 *)(keys + k0);
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = k0 + j < n ? keys[k0 + j] : 0u;
+    }
+    int tp = k0 > 0 ? (int)(keys[k0 - 1] >> shift) : -1;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t k = k0 + j;
+        if (k > n) break;
+        const int t = k < n ? (int)(v[j] >> shift) : ntiles;
+        for (int u = tp + 1; u <= t; u++) tile_start[u] = (int32_t)k;
+        tp = t;
+    }
 }
 
 // View id of each depth-sorted record, the key of one more stable pass that makes the
@@ -811,7 +826,7 @@ void launch_keygen(const KeygenArgs &a, bool batch, hipStream_t s) {
 
 void launch_tile_ranges(const uint32_t *keys, const uint32_t *n_dev, uint32_t n_max, const uint32_t *ntiles_dev,
                         int ntiles, int32_t *tile_start, hipStream_t s, int shift) {
-    const uint32_t nb = (n_max + 1 + 255) / 256;
+    const uint32_t nb = (n_max + 1 + 1023) / 1024;  // 4 boundaries per thread
     tile_ranges_kernel<<<nb, 256, 0, s>>>(keys, n_dev, ntiles_dev, ntiles, tile_start, shift);
 }
 
