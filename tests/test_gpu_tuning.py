@@ -92,6 +92,34 @@ def test_packed_sort_keys_give_the_same_frame(ctx):
         np.testing.assert_array_equal(a[3][k], b[3][k], err_msg=k)
 
 
+def test_packed_sort_keys_at_street_scale(ctx):
+    """475,115 Gaussians (19 id bits) on 1264x832 (4,108 tiles, 13 bits): the packed key
+    uses all 32 bits. Packed and unpacked sorts give the same frame, bit for bit."""
+    from msplat._abi import TUNE_PACK_KEYS, DeviceArray
+    from msplat.backend import DeviceGaussians
+    from msplat.scenes import street_cameras, street_points
+    n, W, H = 475_115, 1264, 832
+    assert (n - 1) >> 18 == 1 and ((W + 15) // 16) * ((H + 15) // 16) > 4096
+    pts, rgb = street_points(0, n)
+    dg = DeviceGaussians.from_points(ctx, pts, rgb, degree=3)
+    cam = dict(street_cameras(4, W, H, 900.0, 900.0)[1], dtu=3)
+    v_img = DeviceArray.from_numpy(ctx, (np.random.default_rng(5).normal(0, 1e-6, (H, W, 3))).astype(np.float32))
+    saved = ctx.get_tuning(TUNE_PACK_KEYS)
+    out = []
+    try:
+        for pack in (1, 0):
+            ctx.set_tuning(TUNE_PACK_KEYS, pack)
+            out.append(_frame(ctx, dg, cam, v_img) + (ctx.frame().num_isects,))
+    finally:
+        ctx.set_tuning(TUNE_PACK_KEYS, saved)
+    a, b = out
+    assert a[4] == b[4] and a[4] > 1_000_000, (a[4], b[4])
+    for x, y in zip(a[:3], b[:3]):
+        np.testing.assert_array_equal(x, y)
+    for k in a[3]:
+        np.testing.assert_array_equal(a[3][k], b[3][k], err_msg=k)
+
+
 def test_autotuner_picks_a_valid_variant(ctx, tmp_path):
     from msplat import autotune
     from msplat.backend import DeviceGaussians, render
