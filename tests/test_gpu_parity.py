@@ -327,6 +327,46 @@ def test_train_steps_with_an_empty_frame_match_oracle(ctx, oracle):
         assert v < 1e-4, (k, v)
 
 
+def test_train_step_at_street_scale_matches_oracle(ctx, oracle):
+    """One train step at the headline size: 475,115 SH3 Gaussians of the street recipe
+    (GPU kNN initialisation, which the oracle's matches bit for bit), a 1264x832 view on
+    the street path, against the oracle on the same inputs (a few seconds of CPU). The
+    initial log-scales are isotropic, which makes the rotation gradient zero up to
+    rounding (noise on both sides), so they are perturbed per axis first."""
+    from msplat._abi import DeviceArray
+    from msplat.backend import DeviceGaussians, train_step
+    from msplat.scenes import street_cameras, street_points
+    n, W, H = 475_115, 1264, 832
+    pts, rgb = street_points(0, n)
+    dg = DeviceGaussians.from_points(ctx, pts, rgb, degree=3)
+    scales = dg.params["scales"].numpy()
+    scales[:n] += np.random.default_rng(37).normal(0, 0.3, (n, 3)).astype(np.float32)
+    dg.params["scales"].upload(scales)
+    st = dg.to_numpy()
+    cam = dict(street_cameras(4, W, H, 900.0, 900.0)[1], dtu=3)
+    gt_np = np.random.default_rng(36).uniform(0, 1, (H, W, 3)).astype(np.float32)
+    steps = np.array([1.6e-4, 5e-3, 1e-3, 2.5e-3, 1.25e-4, 5e-2], np.float32)
+    bc2 = np.full(6, np.float32(np.sqrt(1 - 0.999)), np.float32)
+    loss = train_step(ctx, dg, cam, BG, DeviceArray.from_numpy(ctx, gt_np), steps, bc2)
+    img = ctx.download(ctx.frame().image, (H, W, 3), np.float32)
+    M = ctx.frame().num_isects
+    ref_loss, ref_radii, ref_img = oracle.train_step(st, cam, BG, gt_np, steps, bc2)
+    assert M > 1_000_000
+    assert abs(loss - ref_loss) <= 1e-6 * abs(ref_loss), (loss, ref_loss)
+    assert np.mean(dg.radii.numpy()[:n] == ref_radii) > 0.999
+    assert _rel(img, ref_img) < 1e-4
+    got = dg.to_numpy()
+    errs = {}
+    for k in KEYS:
+        for pre in ("", "m1_", "m2_"):
+            errs[pre + k] = _rel(got[pre + k], st[pre + k].reshape(got[pre + k].shape))
+    for k in ("vis", "gn", "ms"):
+        errs[k] = _rel(got[k], st[k])
+    print(f"\nstreet-scale train step (M={M}) rel errors:", {k: f"{v:.1e}" for k, v in errs.items()})
+    for k, v in errs.items():
+        assert v < 1e-4, (k, v)
+
+
 def test_train_step_without_gaussians_matches_oracle(ctx, oracle):
     """Zero Gaussians: the frame is the background and the loss is the oracle's."""
     from msplat._abi import DeviceArray
