@@ -81,6 +81,7 @@ int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_
     ctx->begin(0);
     launch_project(a, ctx->stream);
     HIPCHECK(hipGetLastError());
+    ctx->frame_stage = msplat_hip_ctx::FRAME_PROJECTED;
     return MSPLAT_OK;
 }
 
@@ -90,6 +91,9 @@ int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_
 // the offsets are enqueued (those run on device-side counts against the gaussian
 // capacity), so the device sorts while the host reads M and sizes the key buffers.
 int do_bin(msplat_hip_ctx *ctx) {
+    if (ctx->frame_stage < msplat_hip_ctx::FRAME_PROJECTED)
+        return set_error(MSPLAT_ERR_ARG, "binning needs a projected frame on this context");
+    ctx->frame_stage = msplat_hip_ctx::FRAME_PROJECTED;  // the key buffers are rewritten below
     const int n = ctx->n;
     const uint32_t nblocks = (uint32_t)((n + 255) / 256);
     if (!ctx->h_total) HIPCHECK(hipHostMalloc((void **)&ctx->h_total, 128, hipHostMallocDefault));
@@ -212,6 +216,7 @@ int do_bin(msplat_hip_ctx *ctx) {
                        shift0);
     ctx->end();
     HIPCHECK(hipGetLastError());
+    ctx->frame_stage = msplat_hip_ctx::FRAME_BINNED;
     return MSPLAT_OK;
 }
 
@@ -257,16 +262,21 @@ RasterArgs raster_args(msplat_hip_ctx *ctx) {
 }
 
 int do_raster_forward(msplat_hip_ctx *ctx, const float bg[3]) {
+    if (ctx->frame_stage < msplat_hip_ctx::FRAME_BINNED)
+        return set_error(MSPLAT_ERR_ARG, "forward rasterization needs a binned frame on this context");
     for (int i = 0; i < 3; i++) ctx->bg[i] = bg ? bg[i] : 0.f;
     CHECKED(ensure_pixels(ctx));
     ctx->begin(2);
     launch_raster_forward(raster_args(ctx), ctx->tuning, ctx->stream);
     ctx->end();
     HIPCHECK(hipGetLastError());
+    ctx->frame_stage = msplat_hip_ctx::FRAME_RASTERED;
     return MSPLAT_OK;
 }
 
 int do_loss(msplat_hip_ctx *ctx, const float *gt, float ssim_weight, float inv_n) {
+    if (ctx->frame_stage < msplat_hip_ctx::FRAME_RASTERED)
+        return set_error(MSPLAT_ERR_ARG, "the loss needs a rendered frame on this context");
     LossArgs l;
     l.width = ctx->width;
     l.height = ctx->height;
@@ -288,6 +298,8 @@ int do_loss(msplat_hip_ctx *ctx, const float *gt, float ssim_weight, float inv_n
 }
 
 int do_raster_backward(msplat_hip_ctx *ctx) {
+    if (ctx->frame_stage < msplat_hip_ctx::FRAME_RASTERED)
+        return set_error(MSPLAT_ERR_ARG, "the backward needs a rendered frame on this context");
     const size_t Mc = ctx->M > 0 ? (size_t)ctx->M : 1;
     CHECKED(ctx->grow(ctx->slot_a, 16 * Mc));
     CHECKED(ctx->grow(ctx->slot_b, 16 * Mc));
@@ -457,6 +469,8 @@ int msplat_hip_stage_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g,
 int msplat_hip_stage_bin(msplat_hip_ctx *ctx, const msplat_hip_camera *cam) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
     CHECKED(check_camera(cam));
+    if (ctx->frame_stage < msplat_hip_ctx::FRAME_PROJECTED)
+        return set_error(MSPLAT_ERR_ARG, "binning needs a projected frame on this context");
     if (cam->tiles_x != ctx->tiles_x || cam->tiles_y != ctx->tiles_y)
         return set_error(MSPLAT_ERR_ARG, "camera differs from the projected frame");
     // rectangles, counts and per-block sums are re-derived from the frame's projection
@@ -472,6 +486,9 @@ int msplat_hip_stage_bin(msplat_hip_ctx *ctx, const msplat_hip_camera *cam) {
 int msplat_hip_stage_raster_forward(msplat_hip_ctx *ctx, const msplat_hip_camera *cam, const float background[3]) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
     CHECKED(check_camera(cam));
+    if (ctx->frame_stage >= msplat_hip_ctx::FRAME_PROJECTED &&
+        (cam->width != ctx->width || cam->height != ctx->height))
+        return set_error(MSPLAT_ERR_ARG, "camera differs from the binned frame");
     return do_raster_forward(ctx, background);
 }
 
@@ -558,6 +575,8 @@ int msplat_hip_render_backward(msplat_hip_ctx *ctx, const msplat_hip_gaussians *
     if (!ctx || !v_img || !out) return set_error(MSPLAT_ERR_ARG, "ctx, v_img or out is NULL");
     CHECKED(check_gaussians(g));
     CHECKED(check_camera(cam));
+    if (ctx->frame_stage < msplat_hip_ctx::FRAME_RASTERED)
+        return set_error(MSPLAT_ERR_ARG, "render_backward needs a render on this context first");
     if (g->num_points != ctx->n || cam->width != ctx->width || cam->height != ctx->height)
         return set_error(MSPLAT_ERR_ARG, "render_backward does not match the last render on this context");
     (void)background;
@@ -680,6 +699,7 @@ int msplat_hip_radix_sort(msplat_hip_ctx *ctx, uint64_t *keys, int32_t *vals, ui
     ctx->keys_sorted = nullptr;
     ctx->gid_sorted = nullptr;
     ctx->sort_gid_bits = 0;
+    if (ctx->frame_stage > msplat_hip_ctx::FRAME_PROJECTED) ctx->frame_stage = msplat_hip_ctx::FRAME_PROJECTED;
     return MSPLAT_OK;
 }
 

@@ -154,6 +154,10 @@ struct msplat_hip_ctx {
     int tiles_x = 0, tiles_y = 0;
     int64_t M = 0;
     bool slot_f_clean = false;  // slot_f[0, M) is all zero: set by the key emission, cleared by a raster backward
+    // How far the one-view frame in the workspace is built: the stage entry points refuse
+    // to run on a frame whose inputs are missing (they would read unset device pointers).
+    enum { FRAME_NONE = 0, FRAME_PROJECTED = 1, FRAME_BINNED = 2, FRAME_RASTERED = 3 };
+    int frame_stage = FRAME_NONE;
     uint32_t *keys_sorted = nullptr;  // sorted keys: tile ids, or (tile << sort_gid_bits) | id
     int32_t *gid_sorted = nullptr;    // sorted gaussian ids (null when packed into the keys)
     int sort_gid_bits = 0;            // > 0: the ids are the keys' low sort_gid_bits bits

@@ -159,9 +159,12 @@ def render_numpy(ctx: Context, dg: DeviceGaussians, cam: dict, bg) -> np.ndarray
 
 
 def adam_coeffs(lrs, step_count, beta1=BETA1, beta2=BETA2):
-    """(step sizes lr/bc1, bc2_sqrt) for Adam step t (model.cpp:638-651), float32."""
-    bc1 = np.float32(1) - np.float32(beta1) ** np.float32(step_count)
-    bc2 = np.float32(1) - np.float32(beta2) ** np.float32(step_count)
+    """(step sizes lr/bc1, bc2_sqrt) for Adam step t (model.cpp:638-651): the reference's
+    std::pow(float beta, int t) promotes the float beta to double, and 1 - beta^t is
+    rounded to float once; the step sizes and square root are float."""
+    t = int(step_count)
+    bc1 = np.float32(1.0 - float(np.float32(beta1)) ** t)
+    bc2 = np.float32(1.0 - float(np.float32(beta2)) ** t)
     steps = (np.asarray(lrs, np.float32) / bc1).astype(np.float32)
     return steps, np.full(6, np.sqrt(bc2), np.float32)
 

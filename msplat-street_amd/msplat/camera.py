@@ -3,14 +3,47 @@
 Follows Model::prepareCam (core/src/model.cpp:498-546) in float32: the camera-to-world
 pose is in the OpenGL convention (columns right, up, back), flipped to OpenCV (y down,
 z forward) before inversion; the projection uses znear 0.001 and zfar 1000 with the
-principal point applied later in pixel space.
+principal point applied later in pixel space. The field of view goes through the C
+library's atanf/tanf, the functions the reference's std::atan/std::tan on floats call,
+so the matrices equal the reference's bit for bit. At a downscale factor the image
+size is the pyramid level's (levelSize, image_sizing.hpp:62) and the intrinsics scale
+by the per-axis size ratio (model.cpp:502-508).
 """
 from __future__ import annotations
+
+import ctypes as C
+import ctypes.util
 
 import numpy as np
 
 TILE = 16
 F = np.float32
+
+_libm = C.CDLL(ctypes.util.find_library("m"))
+for _name in ("atanf", "tanf", "expf", "logf"):
+    getattr(_libm, _name).argtypes = [C.c_float]
+    getattr(_libm, _name).restype = C.c_float
+
+
+def atanf(x) -> np.float32:
+    return F(_libm.atanf(float(F(x))))
+
+
+def tanf(x) -> np.float32:
+    return F(_libm.tanf(float(F(x))))
+
+
+def expf(x) -> np.float32:
+    return F(_libm.expf(float(F(x))))
+
+
+def logf(x) -> np.float32:
+    return F(_libm.logf(float(F(x))))
+
+
+def level_size(full: int, factor: int) -> int:
+    """levelSize (image_sizing.hpp:62): the pyramid level's whole-pixel size."""
+    return int(full) if factor <= 1 else int(full) // int(factor)
 
 
 def look_at(eye, target, up=(0.0, 1.0, 0.0)) -> np.ndarray:
@@ -35,7 +68,8 @@ def schedule_downscale(step: int, num_downscales: int, resolution_schedule: int)
 
 
 def view_matrices(cam_to_world, W, H, fx, fy):
-    """(viewmat, projmat, cam_pos) as prepareCam builds them, all float32."""
+    """(viewmat, projmat, cam_pos) as prepareCam builds them (model.cpp:509-525), all
+    float32: every product and sum rounds to float32, left to right."""
     d = np.asarray(cam_to_world, F).reshape(16)
     R = np.zeros((3, 3), F)
     T = np.zeros(3, F)
@@ -45,10 +79,10 @@ def view_matrices(cam_to_world, W, H, fx, fy):
     Tinv = np.array([-(Rinv[i, 0] * T[0] + Rinv[i, 1] * T[1] + Rinv[i, 2] * T[2]) for i in range(3)], F)
     vm = np.array([Rinv[0, 0], Rinv[0, 1], Rinv[0, 2], Tinv[0], Rinv[1, 0], Rinv[1, 1], Rinv[1, 2], Tinv[1],
                    Rinv[2, 0], Rinv[2, 1], Rinv[2, 2], Tinv[2], 0, 0, 0, 1], F)
-    fovx = F(2.0) * F(np.arctan(F(W) / (F(2.0) * F(fx))))
-    fovy = F(2.0) * F(np.arctan(F(H) / (F(2.0) * F(fy))))
-    t_p = F(0.001) * F(np.tan(F(0.5) * fovy))
-    r_p = F(0.001) * F(np.tan(F(0.5) * fovx))
+    fovx = F(2.0) * atanf(F(W) / (F(2.0) * F(fx)))
+    fovy = F(2.0) * atanf(F(H) / (F(2.0) * F(fy)))
+    t_p = F(0.001) * tanf(F(0.5) * fovy)
+    r_p = F(0.001) * tanf(F(0.5) * fovx)
     zn, zf = F(0.001), F(1000.0)
     pm = np.array([zn / r_p, 0, 0, 0, 0, zn / t_p, 0, 0, 0, 0, (zf + zn) / (zf - zn), -zf * zn / (zf - zn),
                    0, 0, 1, 0], F)
@@ -62,10 +96,17 @@ def view_matrices(cam_to_world, W, H, fx, fy):
     return vm, pvm, T.copy()
 
 
-def make_camera(cam_to_world, W, H, fx, fy, cx=None, cy=None, dtu=0):
-    """Camera dict in the layout every backend entry point takes."""
+def make_camera(cam_to_world, W, H, fx, fy, cx=None, cy=None, dtu=0, downscale=1):
+    """Camera dict in the layout every backend entry point takes. With downscale > 1 the
+    view is the pyramid level prepareCam renders at that factor (model.cpp:500-508)."""
     cx = W / 2.0 if cx is None else cx
     cy = H / 2.0 if cy is None else cy
+    if downscale > 1:
+        w, h = level_size(W, downscale), level_size(H, downscale)
+        rx = F(w) / F(W) if W > 0 else F(1) / F(downscale)
+        ry = F(h) / F(H) if H > 0 else F(1) / F(downscale)
+        fx, cx, fy, cy = F(fx) * rx, F(cx) * rx, F(fy) * ry, F(cy) * ry
+        W, H = w, h
     vm, pm, pos = view_matrices(cam_to_world, W, H, fx, fy)
     return dict(viewmat=vm, projmat=pm, cam_pos=pos, fx=F(fx), fy=F(fy), cx=F(cx), cy=F(cy), W=int(W), H=int(H),
                 tiles_x=(int(W) + TILE - 1) // TILE, tiles_y=(int(H) + TILE - 1) // TILE, dtu=int(dtu),

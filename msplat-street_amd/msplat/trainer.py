@@ -13,10 +13,39 @@ import numpy as np
 
 from ._abi import Context, DeviceArray, check, floats, lib, make_camera_struct
 from .backend import KEYS, DeviceGaussians, adam_coeffs
+from .camera import expf, logf
 
+F = np.float32
 LR_INIT = (1.6e-4, 5e-3, 1e-3, 2.5e-3, 1.25e-4, 5e-2)  # model.cpp:100
 MEANS_LR_FINAL = 1.6e-6
 RESET_LOGIT = -1.3862943611198906  # logit(0.2)
+
+# afterTrain's decisions at one step (model.cpp:199-245)
+REFINE_DENSIFY, REFINE_RESET, REFINE_STATS, REFINE_SCREEN, REFINE_HUGE = 1, 2, 4, 8, 16
+
+
+def means_lr(step: int, max_steps: int, lr_init=LR_INIT[0], lr_final=MEANS_LR_FINAL) -> float:
+    """Model::schedulersStep (model.cpp:140-143) in float32 through the C library's
+    expf/logf: exp(log(init) (1 - t) + log(final) t), t = clamp(step / max_steps, 0, 1)."""
+    t = F(min(max(F(step) / F(max_steps), F(0)), F(1)))
+    return float(expf(logf(lr_init) * (F(1) - t) + logf(lr_final) * t))
+
+
+def refine_flags(step: int, cfg: "TrainConfig", num_cameras: int, stop_split_at: int) -> int:
+    """What Model::afterTrain does after `step` (model.cpp:199-245), as REFINE_* bits."""
+    if step % cfg.refine_every != 0 or step <= cfg.warmup_length:
+        return 0
+    reset_interval = cfg.reset_alpha_every * cfg.refine_every
+    f = REFINE_STATS
+    if step < stop_split_at and step % reset_interval > num_cameras + cfg.refine_every:
+        f |= REFINE_DENSIFY
+        if step < cfg.stop_screen_size_at:
+            f |= REFINE_SCREEN
+        if step > cfg.refine_every * cfg.reset_alpha_every:
+            f |= REFINE_HUGE
+    if step < stop_split_at and step % reset_interval == cfg.refine_every:
+        f |= REFINE_RESET
+    return f
 
 
 @dataclasses.dataclass
@@ -105,25 +134,24 @@ class Trainer:
         return loss.value if read_loss else None
 
     def schedulers_step(self, step: int):
-        t = min(max(step / self.cfg.iterations, 0.0), 1.0)
-        self.lr[0] = float(np.exp(np.log(LR_INIT[0]) * (1 - t) + np.log(MEANS_LR_FINAL) * t))
+        self.lr[0] = means_lr(step, self.cfg.iterations)
 
     def after_train(self, step: int):
         cfg = self.cfg
-        if not cfg.densify or step % cfg.refine_every != 0 or step <= cfg.warmup_length:
+        if not cfg.densify:
             return
-        reset_interval = cfg.reset_alpha_every * cfg.refine_every
-        do_densify = step < self.stop_split_at and step % reset_interval > len(self.cams) + cfg.refine_every
-        if do_densify:
+        f = refine_flags(step, cfg, len(self.cams), self.stop_split_at)
+        if not f:
+            return
+        if f & REFINE_DENSIFY:
             before = self.dg.n
             self.dg.ensure_capacity(3 * self.dg.n)
-            half_max_dim = 0.5 * float(max(self.last_hw))
-            check_screen = 1 if step < cfg.stop_screen_size_at else 0
-            check_huge = 1 if step > cfg.refine_every * cfg.reset_alpha_every else 0
+            half_max_dim = float(F(0.5) * F(max(self.last_hw)))
             new = self.dg.densify(cfg.densify_grad_thresh, cfg.densify_size_thresh, cfg.split_screen_size,
-                                  check_screen, cfg.cull_alpha_thresh, cfg.cull_scale_thresh, cfg.cull_screen_size,
-                                  check_huge, half_max_dim, (step * 2654435761) % (1 << 32))
+                                  1 if f & REFINE_SCREEN else 0, cfg.cull_alpha_thresh, cfg.cull_scale_thresh,
+                                  cfg.cull_screen_size, 1 if f & REFINE_HUGE else 0, half_max_dim,
+                                  (step * 2654435761) % (1 << 32))
             self.densify_log.append((step, before, new))
-        if step < self.stop_split_at and step % reset_interval == cfg.refine_every:
+        if f & REFINE_RESET:
             self.dg.opacity_reset(RESET_LOGIT)
         self.stats_defined = False  # statistics restart from zero (model.cpp:242-244)

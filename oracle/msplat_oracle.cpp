@@ -1191,3 +1191,83 @@ extern "C" void orc_pack_rgba8(const float *img, uint32_t n, uint8_t *out) {
 extern "C" uint32_t orc_pcg_hash(uint32_t v) { return pcg(v); }
 extern "C" float orc_rand_uniform(uint32_t seed, uint32_t ctr) { return uniform01(seed, ctr); }
 extern "C" void orc_rand_normal3(uint32_t seed, uint32_t stream, float *out) { normal3(seed, stream, out); }
+
+// ── Host-side schedule math of the reference's Model (model.cpp), restated in float ──
+
+// The integer part of Model::prepareCam (model.cpp:498-546) plus levelSize
+// (image_sizing.hpp:62): pyramid size for downscale factor sf, intrinsics scaled by the
+// per-axis ratio, fov from glibc atanf, viewmat = inverse of the OpenGL->OpenCV-flipped
+// camToWorld, projmat = P(znear 1e-3, zfar 1e3) * viewmat in float, left-to-right sums.
+extern "C" void orc_prepare_cam(const float *c2w, int W, int H, float fx, float fy, float cx, float cy, int sf,
+                                float *viewmat, float *projmat, float *cam_pos, float *intr, int *wh) {
+    const int w = sf <= 1 ? W : W / sf, h = sf <= 1 ? H : H / sf;
+    const float rx = W > 0 ? (float)w / (float)W : 1.f / (float)sf;
+    const float ry = H > 0 ? (float)h / (float)H : 1.f / (float)sf;
+    const float sfx = fx * rx, scx = cx * rx, sfy = fy * ry, scy = cy * ry;
+    const float fovX = 2.0f * std::atan(w / (2.0f * sfx));
+    const float fovY = 2.0f * std::atan(h / (2.0f * sfy));
+    float R[3][3], Ri[3][3], T[3], Ti[3];
+    for (int i = 0; i < 3; i++) {
+        R[i][0] = c2w[i * 4 + 0];
+        R[i][1] = -c2w[i * 4 + 1];
+        R[i][2] = -c2w[i * 4 + 2];
+        T[i] = c2w[i * 4 + 3];
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) Ri[i][j] = R[j][i];
+    for (int i = 0; i < 3; i++) Ti[i] = -(Ri[i][0] * T[0] + Ri[i][1] * T[1] + Ri[i][2] * T[2]);
+    const float vm[16] = {Ri[0][0], Ri[0][1], Ri[0][2], Ti[0], Ri[1][0], Ri[1][1], Ri[1][2], Ti[1],
+                          Ri[2][0], Ri[2][1], Ri[2][2], Ti[2], 0.f,      0.f,      0.f,      1.f};
+    const float zn = 0.001f, zf = 1000.0f;
+    const float t_p = zn * std::tan(0.5f * fovY), r_p = zn * std::tan(0.5f * fovX);
+    const float pm[16] = {zn / r_p, 0, 0, 0, 0, zn / t_p, 0, 0, 0, 0, (zf + zn) / (zf - zn), -zf * zn / (zf - zn),
+                          0, 0, 1, 0};
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            float acc = 0.f;
+            for (int k = 0; k < 4; k++) acc += pm[i * 4 + k] * vm[k * 4 + j];
+            projmat[i * 4 + j] = acc;
+        }
+    std::memcpy(viewmat, vm, sizeof vm);
+    for (int i = 0; i < 3; i++) cam_pos[i] = T[i];
+    intr[0] = sfx;
+    intr[1] = sfy;
+    intr[2] = scx;
+    intr[3] = scy;
+    wh[0] = w;
+    wh[1] = h;
+}
+
+// Model::schedulersStep (model.cpp:140-143): the means learning rate after `step`,
+// exp(lerp(log init, log final, clamp(step / max_steps))) in float (glibc expf/logf).
+extern "C" float orc_means_lr(int step, int max_steps, float lr_init, float lr_final) {
+    const float t = std::clamp((float)step / (float)max_steps, 0.f, 1.f);
+    return std::exp(std::log(lr_init) * (1.f - t) + std::log(lr_final) * t);
+}
+
+// Model::fullIteration's Adam scalars (model.cpp:638-651): bc = 1 - pow(beta, t) with
+// the float beta promoted to double by std::pow(float, int), rounded to float; the step
+// size lr / bc1 and sqrt(bc2) in float.
+extern "C" void orc_adam_scalars(float lr, float beta1, float beta2, int t, float *step_size, float *bc2_sqrt) {
+    const float bc1 = 1.0f - std::pow(beta1, t);
+    const float bc2 = 1.0f - std::pow(beta2, t);
+    *step_size = lr / bc1;
+    *bc2_sqrt = std::sqrt(bc2);
+}
+
+// Model::afterTrain's cadence (model.cpp:199-245): bit 0 = densify at this step, bit 1
+// = opacity reset, bit 2 = statistics restart, bit 3 = screen-size check on, bit 4 =
+// huge-cull on.
+extern "C" int orc_refine_flags(int step, int refine_every, int warmup, int reset_alpha_every, int stop_split_at,
+                                int num_cameras, int stop_screen_size_at) {
+    if (step % refine_every != 0 || step <= warmup) return 0;
+    const int reset_interval = reset_alpha_every * refine_every;
+    int f = 4;
+    if (step < stop_split_at && step % reset_interval > num_cameras + refine_every) {
+        f |= 1;
+        if (step < stop_screen_size_at) f |= 8;
+        if (step > refine_every * reset_alpha_every) f |= 16;
+    }
+    if (step < stop_split_at && step % reset_interval == refine_every) f |= 2;
+    return f;
+}

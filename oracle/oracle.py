@@ -75,6 +75,14 @@ class Oracle:
              C.c_int, C.c_int, C.c_float, C.c_uint32, C.c_uint32, f32p, f32p, f32p, C.c_float, C.c_float, vp, vp, vp,
              f32p, f32p, C.c_float, C.c_float, C.c_float, f32p, f32p, f32p, C.c_float, vp, vp])
         L.orc_f32_train_step.restype = C.c_float
+        L.orc_prepare_cam.argtypes = [f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                      f32p, f32p, f32p, f32p, i32p]
+        L.orc_means_lr.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float]
+        L.orc_means_lr.restype = C.c_float
+        L.orc_adam_scalars.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, C.POINTER(C.c_float),
+                                       C.POINTER(C.c_float)]
+        L.orc_refine_flags.argtypes = [C.c_int] * 7
+        L.orc_refine_flags.restype = C.c_int
         L.orc_densify.argtypes = ([C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_float, C.c_float,
                                    C.c_float, C.c_int, f32p, f32p, f32p, C.c_float, vp, C.c_int, C.c_uint32, vp, vp,
                                    vp, vp])
@@ -202,6 +210,28 @@ class Oracle:
                                    _c(ms, np.float32), half_max_dim, _ptrs(bufs), fr_stride, seed,
                                    split.ctypes.data, dup.ctypes.data, keep.ctypes.data, C.byref(nn))
         return out, split[:N], dup[:N], keep[:nn.value]
+
+    def prepare_cam(self, cam_to_world, W, H, fx, fy, cx, cy, sf=1):
+        """Model::prepareCam's matrices, intrinsics and size at downscale factor sf."""
+        vm, pm, pos, intr = (np.zeros(k, np.float32) for k in (16, 16, 3, 4))
+        wh = np.zeros(2, np.int32)
+        self.lib.orc_prepare_cam(_c(cam_to_world, np.float32).reshape(16), int(W), int(H), fx, fy, cx, cy, int(sf),
+                                 vm, pm, pos, intr, wh)
+        return dict(viewmat=vm, projmat=pm, cam_pos=pos, fx=intr[0], fy=intr[1], cx=intr[2], cy=intr[3],
+                    W=int(wh[0]), H=int(wh[1]))
+
+    def means_lr(self, step, max_steps, lr_init=1.6e-4, lr_final=1.6e-6):
+        return np.float32(self.lib.orc_means_lr(int(step), int(max_steps), lr_init, lr_final))
+
+    def adam_scalars(self, lr, t, beta1=0.9, beta2=0.999):
+        a, b = C.c_float(), C.c_float()
+        self.lib.orc_adam_scalars(lr, beta1, beta2, int(t), C.byref(a), C.byref(b))
+        return np.float32(a.value), np.float32(b.value)
+
+    def refine_flags(self, step, refine_every, warmup, reset_alpha_every, stop_split_at, num_cameras,
+                     stop_screen_size_at):
+        return int(self.lib.orc_refine_flags(step, refine_every, warmup, reset_alpha_every, stop_split_at,
+                                             num_cameras, stop_screen_size_at))
 
     def opacity_reset(self, o, m1, m2, reset_logit):
         self.lib.orc_opacity_reset(o, m1, m2, o.size, reset_logit)

@@ -2,7 +2,8 @@
 (core/cpu/msplat_cpu.cpp built by oracle/Makefile into oracle/_ref). Run where the
 reference tree is mounted:
 
-    make -C oracle && python tests/golden/make_golden.py
+    make -C oracle && python tests/golden/make_golden.py            # every fixture
+    make -C oracle && python tests/golden/make_golden.py NAME ...   # only these
 
 Each .npz holds the inputs and the reference's outputs for one entry point of
 core/metal/bindings.h. Inputs are seeded synthetic scenes.
@@ -35,9 +36,34 @@ def cam_fields(c, prefix):
                                                    "dtu")}
 
 
-def main():
+def opacity_reset_vectors(ref):
+    """msplat_opacity_reset (cpu:1467-1475) on logits on both sides of logit(0.2), ties
+    with it, infinities and moments of both signs."""
+    g = np.random.default_rng(31)
+    reset = np.float32(-1.3862943611198906)
+    o = np.concatenate([g.normal(0.0, 3.0, 997), [reset, np.nextafter(reset, np.float32(0)),
+                                                   np.nextafter(reset, np.float32(-10)), -np.inf, np.inf, 0.0,
+                                                   -0.0]]).astype(np.float32)
+    m1 = g.normal(0, 1, o.size).astype(np.float32)
+    m2 = np.abs(g.normal(0, 1, o.size)).astype(np.float32)
+    d = dict(o=o.copy(), m1=m1.copy(), m2=m2.copy(), reset=reset)
+    ref.opacity_reset(o, m1, m2, reset)
+    d.update(out_o=o, out_m1=m1, out_m2=m2)
+    return d
+
+
+EXTRA = {"opacity_reset": opacity_reset_vectors}
+
+
+def main(only=()):
     ref = Reference()
-    out = {}
+    if only:
+        for name in only:
+            d = EXTRA[name](ref)
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
+            print(name, os.path.getsize(os.path.join(HERE, name + ".npz")))
+        return
+    out = {name: fn(ref) for name, fn in EXTRA.items()}
     # render at SH degrees 0, 1, 3 from two cameras (non-tile-multiple sizes)
     for deg in (0, 1, 3):
         sc = random_scene(100 + deg, 400, degree=deg, log_scale=(-3.0, 0.6))
@@ -145,4 +171,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))

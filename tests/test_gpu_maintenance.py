@@ -96,6 +96,17 @@ def test_opacity_reset(ctx, oracle):
         np.testing.assert_array_equal(a.numpy(), b)
 
 
+def test_opacity_reset_matches_golden(ctx):
+    """HIP against the compiled reference's own outputs (ties with the cap, +-inf, -0)."""
+    from conftest import golden
+    from msplat._abi import DeviceArray, check, lib
+    g = golden("opacity_reset")
+    d = [DeviceArray.from_numpy(ctx, g[k]) for k in ("o", "m1", "m2")]
+    check(lib().msplat_hip_opacity_reset(ctx.ptr, d[0].ptr, d[1].ptr, d[2].ptr, g["o"].size, float(g["reset"])))
+    for a, k in zip(d, ("out_o", "out_m1", "out_m2")):
+        np.testing.assert_array_equal(a.numpy(), g[k])
+
+
 def _knn_points(kind, rng):
     if kind == "uniform":
         return rng.uniform(-1, 1, (3000, 3))

@@ -65,6 +65,34 @@ def test_densify_matches_reference(oracle, huge):
         np.testing.assert_array_equal(bufs[i][:n], d[f"out{huge}_{i}"], err_msg=f"buffer {i}")
 
 
+def test_opacity_reset_matches_reference(oracle):
+    d = golden("opacity_reset")
+    o, m1, m2 = d["o"].copy(), d["m1"].copy(), d["m2"].copy()
+    oracle.opacity_reset(o, m1, m2, d["reset"])
+    np.testing.assert_array_equal(o, d["out_o"])
+    np.testing.assert_array_equal(m1, d["out_m1"])
+    np.testing.assert_array_equal(m2, d["out_m2"])
+
+
+def test_opacity_reset_closed_form(oracle):
+    """The reset caps every opacity at 0.2 (logit -1.3862944 as a float32) and restarts
+    the opacity group's Adam moments: o' = min(o, logit 0.2), m1 = m2 = 0. Values at or
+    below the cap, and -inf, are left bit-identical; +inf and NaN-free inputs above it
+    become the cap exactly."""
+    g = np.random.default_rng(5)
+    reset = np.float32(np.log(0.2 / 0.8))
+    assert reset == np.float32(-1.3862943611198906)
+    o = np.concatenate([g.normal(0, 4, 4000), [reset, -np.inf, np.inf, 1e30, -1e30]]).astype(np.float32)
+    m1, m2 = g.normal(0, 1, o.size).astype(np.float32), g.random(o.size).astype(np.float32)
+    before = o.copy()
+    oracle.opacity_reset(o, m1, m2, reset)
+    np.testing.assert_array_equal(o, np.minimum(before, reset))
+    assert not np.any(m1) and not np.any(m2)
+    # in probability space: sigmoid(o') = min(sigmoid(o), 0.2) to float rounding
+    p = 1 / (1 + np.exp(-o.astype(np.float64)))
+    np.testing.assert_allclose(p, np.minimum(1 / (1 + np.exp(-before.astype(np.float64))), 0.2), rtol=1e-6)
+
+
 def test_knn_and_init_match_reference(oracle):
     d = golden("knn_init")
     np.testing.assert_array_equal(oracle.knn3(d["points"]), d["knn"])
