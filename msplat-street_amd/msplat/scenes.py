@@ -80,3 +80,39 @@ def street_cameras(count=64, W=1264, H=832, fx=900.0, fy=900.0, start=(-5.0, 0.0
         fwd = np.array([np.cos(yaw), -0.1, np.sin(yaw)])
         cams.append(make_camera(look_at(p, p + fwd), W, H, fx, fy, dtu=dtu))
     return cams
+
+
+def gmm_cloud(seed, n, center=(0.0, 0.0, 0.0), clusters=64, sigma=0.5, box=40.0):
+    """A Gaussian mixture point cloud: `clusters` centres uniform in a cube of side `box`
+    (m) around `center`, every point drawn from a uniformly chosen cluster with isotropic
+    deviation `sigma` (m); colours per cluster with a little per-point noise."""
+    rng = np.random.default_rng(seed)
+    cc = np.asarray(center, np.float64) + rng.uniform(-box / 2, box / 2, (clusters, 3))
+    col = rng.integers(30, 226, (clusters, 3))
+    lab = rng.integers(0, clusters, n)
+    pts = (cc[lab] + rng.normal(0, sigma, (n, 3))).astype(F)
+    rgb = np.clip(col[lab] + rng.integers(-25, 26, (n, 3)), 0, 255).astype(np.uint8)
+    return pts, rgb
+
+
+def spline_cameras(count=200, W=1920, H=1080, focal_scale=0.9, seed=0, center=(0.0, 0.0, 0.0),
+                   radius=(14.0, 22.0), height=(-3.0, 3.0), knots=8, dtu=0):
+    """Poses on a closed Catmull-Rom spline through `knots` seeded control points around
+    `center` (angles evenly spaced with jitter, distance U(radius), height U(height)),
+    sampled at `count` even parameter steps, each looking at `center` (fx = fy =
+    focal_scale W)."""
+    rng = np.random.default_rng(seed)
+    ang = 2 * np.pi * (np.arange(knots) + rng.uniform(-0.25, 0.25, knots)) / knots
+    r = rng.uniform(*radius, knots)
+    c = np.asarray(center, np.float64)
+    ctrl = c + np.stack([r * np.cos(ang), rng.uniform(*height, knots), r * np.sin(ang)], 1)
+    cams = []
+    for k in range(count):
+        u = knots * k / count
+        i, t = int(u), u - int(u)
+        p0, p1, p2, p3 = (ctrl[(i + j) % knots] for j in (-1, 0, 1, 2))
+        eye = 0.5 * (2 * p1 + (p2 - p0) * t + (2 * p0 - 5 * p1 + 4 * p2 - p3) * t * t
+                     + (3 * p1 - p0 - 3 * p2 + p3) * t ** 3)
+        f = focal_scale * W
+        cams.append(make_camera(look_at(eye, c), W, H, f, f, dtu=dtu))
+    return cams

@@ -1080,6 +1080,21 @@ extern "C" void orc_f32_project_backward_adam(
                                  rest_m1, rest_m2, dc_step, dc_bc2, rest_step, rest_bc2, b1, b2, eps);
 }
 
+// The same projection backward in double (the FP64 renderer's parameter gradients: with
+// b1 = b2 = 0 and zero step sizes the SH "Adam" leaves dc/rest unchanged and stores the
+// raw gradients in dc_m1/rest_m1).
+extern "C" void orc_f64_project_backward_adam(
+    int N, const double *means, const double *scales, double glob, const double *quats, const double *vm,
+    const double *pm, double fx, double fy, uint32_t W, uint32_t H, uint32_t degree, uint32_t dtu, const double *cam,
+    const int32_t *radii, const double *conics, const double *v_xy, const double *v_conic, const double *v_rgb,
+    const double *v_depth, double *v_mean, double *v_scale, double *v_quat, double *dc, double *rest, double *dc_m1,
+    double *dc_m2, double *rest_m1, double *rest_m2, double dc_step, double dc_bc2, double rest_step, double rest_bc2,
+    double b1, double b2, double eps) {
+    project_backward_adam<double>(N, means, scales, glob, quats, vm, pm, fx, fy, W, H, degree, dtu, cam, radii,
+                                  conics, v_xy, v_conic, v_rgb, v_depth, v_mean, v_scale, v_quat, dc, rest, dc_m1,
+                                  dc_m2, rest_m1, rest_m2, dc_step, dc_bc2, rest_step, rest_bc2, b1, b2, eps);
+}
+
 extern "C" void orc_f32_adam(float *p, const float *g, float *m1, float *m2, size_t n, float step, float b1, float b2,
                              float bc2, float eps) {
     for (size_t i = 0; i < n; i++) adam<float>(p[i], m1[i], m2[i], g[i], step, b1, b2, bc2, eps);

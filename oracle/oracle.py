@@ -68,6 +68,10 @@ class Oracle:
             [C.c_int, f32p, f32p, C.c_float, f32p, f32p, f32p, C.c_float, C.c_float, C.c_uint32, C.c_uint32,
              C.c_uint32, C.c_uint32, f32p, i32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p,
              f32p, f32p, f32p] + [C.c_float] * 7)
+        L.orc_f64_project_backward_adam.argtypes = (
+            [C.c_int, f64p, f64p, C.c_double, f64p, f64p, f64p, C.c_double, C.c_double, C.c_uint32, C.c_uint32,
+             C.c_uint32, C.c_uint32, f64p, i32p, f64p, f64p, f64p, f64p, f64p, f64p, f64p, f64p, f64p, f64p, f64p,
+             f64p, f64p, f64p] + [C.c_double] * 7)
         L.orc_f32_adam.argtypes = [f32p, f32p, f32p, f32p, C.c_size_t] + [C.c_float] * 5
         L.orc_f32_grad_stats.argtypes = [C.c_int, i32p, f32p, C.c_float, f32p, f32p, f32p]
         L.orc_f32_train_step.argtypes = (
@@ -174,6 +178,30 @@ class Oracle:
         getattr(self.lib, f"orc_{s}_reduce_slots")(
             N, bins["M"], _pad(bins["isect_gid"], Mc, np.int32), _pad(slots, Mc, dtype), out["v_xy"], out["v_conic"],
             out["v_rgb"], out["v_opac"], out["v_depth"])
+        return out
+
+    def gradients(self, scene, cam, bg, v_img, dtype=np.float32):
+        """Image, final transmittance and the parameter gradients of <image, v_img> by the
+        analytic backward (raster backward, per-Gaussian reduction, projection VJPs) in
+        float32 or float64: {img, final_T, means, scales, quats, dc, rest, opac}."""
+        s = "f32" if dtype == np.float32 else "f64"
+        p, b, r = self.render(scene, cam, bg, dtype)
+        N = scene["means"].shape[0]
+        slots = self.raster_backward(b, r, cam, bg, v_img, dtype)
+        red = self.reduce_slots(N, b, slots, dtype)
+        nb = sh_bases(scene["degree"])
+        out = dict(img=r["img"], final_T=r["final_T"], means=np.zeros((N, 3), dtype), scales=np.zeros((N, 3), dtype),
+                   quats=np.zeros((N, 4), dtype), opac=red["v_opac"].reshape(N, 1).copy())
+        dc, rest = _c(scene["dc"], dtype).copy(), _c(scene["rest"], dtype).copy()
+        g_dc, g_rest = np.zeros_like(dc), np.zeros_like(rest)
+        m2_dc, m2_rest = np.zeros_like(dc), np.zeros_like(rest)
+        getattr(self.lib, f"orc_{s}_project_backward_adam")(
+            N, _c(scene["means"], dtype), _c(scene["scales"], dtype), 1.0, _c(scene["quats"], dtype),
+            _c(cam["viewmat"], dtype), _c(cam["projmat"], dtype), cam["fx"], cam["fy"], cam["W"], cam["H"],
+            scene["degree"], cam["dtu"], _c(cam["cam_pos"], dtype), p["radii"], p["conics"], red["v_xy"],
+            red["v_conic"], red["v_rgb"], red["v_depth"], out["means"], out["scales"], out["quats"], dc, rest, g_dc,
+            m2_dc, g_rest, m2_rest, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 1e-8)
+        out["dc"], out["rest"] = g_dc, g_rest.reshape(N, max(nb - 1, 1), 3)
         return out
 
     def train_step(self, state, cam, bg, gt, steps, bc2s, ssim_weight=0.2, b1=0.9, b2=0.999, eps=1e-8):

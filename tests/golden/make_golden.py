@@ -22,6 +22,9 @@ from oracle import Reference  # noqa: E402
 from msplat.camera import look_at, make_camera  # noqa: E402
 from msplat.scenes import random_scene  # noqa: E402
 
+sys.path.insert(0, os.path.dirname(HERE))
+from workloads import C1_SAMPLE, C1_STEPS, c1_camera_order, c1_workload  # noqa: E402
+
 BG = np.array([0.613, 0.0101, 0.3984], np.float32)
 KEYS = ("means", "scales", "quats", "dc", "rest", "opac")
 
@@ -52,7 +55,49 @@ def opacity_reset_vectors(ref):
     return d
 
 
-EXTRA = {"opacity_reset": opacity_reset_vectors}
+def trajectory_c1_vectors(ref):
+    """300 Adam steps of Model::fullIteration (model.cpp:626-682) over the reference's
+    msplat_train_step (cpu:1204-1327), with Model's schedule: Adam scalars per step
+    (model.cpp:638-651), the means LR decayed after each step over a 300-step schedule
+    (model.cpp:140-143), SH degree 1 in use from step 1 (sh_degree_interval 1,
+    model.cpp:535), no densification. Keeps the per-step loss, the final parameters of a
+    seeded sample of Gaussians, per-group sums, and the final PSNR of every view."""
+    import hashlib
+
+    from oracle import Oracle
+    orc = Oracle()  # for the schedule's scalars only (pinned against model.cpp elsewhere)
+    sc, gt_scene, cs = c1_workload()
+    gts = [ref.render(gt_scene, c, BG)[0] for c in cs]
+    state = {k: sc[k].copy() for k in KEYS}
+    for k in KEYS:
+        state["m1_" + k] = np.zeros_like(sc[k])
+        state["m2_" + k] = np.zeros_like(sc[k])
+    N = sc["means"].shape[0]
+    state.update(degree=1, vis=np.zeros(N, np.float32), gn=np.zeros(N, np.float32), ms=np.zeros(N, np.float32))
+    lrs = [1.6e-4, 5e-3, 1e-3, 2.5e-3, 1.25e-4, 5e-2]
+    order = c1_camera_order(C1_STEPS)
+    losses = []
+    for t in range(1, C1_STEPS + 1):
+        sc_bc = [orc.adam_scalars(np.float32(lr), t) for lr in lrs]
+        steps = np.array([s for s, _ in sc_bc], np.float32)
+        bc2s = np.array([b for _, b in sc_bc], np.float32)
+        loss, _ = ref.train_step(state, cs[order[t - 1]], BG, gts[order[t - 1]], steps, bc2s)
+        losses.append(loss)
+        lrs[0] = float(orc.means_lr(t, C1_STEPS))
+    final = dict(state, degree=1)
+    psnr = np.array([ref.image_metrics(ref.render(final, c, BG)[0], g)[0] for c, g in zip(cs, gts)])
+    sample = np.sort(np.random.default_rng(5).choice(N, C1_SAMPLE, replace=False)).astype(np.int32)
+    d = dict(order=order, losses=np.array(losses, np.float32), psnr=psnr, sample=sample, bg=BG,
+             gt_sha256=np.frombuffer(hashlib.sha256(np.ascontiguousarray(np.stack(gts)).tobytes()).digest(),
+                                     np.uint8))
+    for k in KEYS:
+        d["final_" + k] = state[k][sample]
+        d["sum_" + k] = np.float64(state[k].astype(np.float64).sum())
+        d["sumsq_" + k] = np.float64((state[k].astype(np.float64) ** 2).sum())
+    return d
+
+
+EXTRA = {"opacity_reset": opacity_reset_vectors, "trajectory_c1": trajectory_c1_vectors}
 
 
 def main(only=()):

@@ -1,0 +1,237 @@
+"""The benchmark workloads at their real sizes on the GPU, each against the oracle on the
+same inputs (or, for the forms without a reference counterpart, against the one-view
+entry points the oracle pins):
+
+- street scale (475,115 SH3 Gaussians, 1264x832): image, transmittance and every
+  parameter gradient against the FP32 oracle (1e-4 relative) and, stated per output,
+  against the FP64 renderer;
+- large scene (3,000,000 SH3 Gaussians from a 64-cluster mixture in a 40 m box, kNN
+  scales over all points, a 1920x1080 view on the seeded spline): kNN distances, one
+  train step, a densify/prune call and an opacity reset;
+- batched views (1,000,000 SH3 Gaussians, 16 views drawn from 640x360, 1264x832,
+  1920x1080 and 2560x1440): images bit for bit equal to one-view renders, gradients
+  within 2e-5 of the per-view sums;
+- serving (four resident scenes of 0.5M, 1M, 2M and 3M SH3 Gaussians, the
+  640x360/1280x720/1920x1080/3840x2160 request mix): every request's RGBA8 image the
+  same alone and in the continuous-batching mix.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BG = np.array([0.6130, 0.0101, 0.3984], np.float32)
+KEYS = ("means", "scales", "quats", "dc", "rest", "opac")
+STEP_SIZES = np.array([1.6e-4, 5e-3, 1e-3, 2.5e-3, 1.25e-4, 5e-2], np.float32)
+BC2 = np.full(6, np.float32(np.sqrt(np.float32(1) - np.float32(0.999))), np.float32)
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _perturbed(ctx, pts, rgb, seed, capacity=None):
+    """GPU initialisation (kNN scales, random rotations, colours, logit(0.1)), then
+    per-axis log-scale noise (isotropic scales make the rotation gradient zero up to
+    rounding), SH bands N(0, 0.05) and opacity logits N(0, 1), so every gradient group
+    carries signal."""
+    from msplat.backend import DeviceGaussians
+    n = pts.shape[0]
+    dg = DeviceGaussians.from_points(ctx, pts, rgb, degree=3, capacity=capacity)
+    rng = np.random.default_rng(seed)
+    s = dg.params["scales"].numpy()
+    s[:n] += rng.normal(0, 0.3, (n, 3)).astype(np.float32)
+    dg.params["scales"].upload(s)
+    r = dg.params["rest"].numpy()
+    r[:n] = rng.normal(0, 0.05, (n, 45)).astype(np.float32)
+    dg.params["rest"].upload(r)
+    o = dg.params["opac"].numpy()
+    o[:n] = rng.normal(0, 1, (n, 1)).astype(np.float32)
+    dg.params["opac"].upload(o)
+    return dg
+
+
+def test_street_scale_errors_against_fp64(ctx, oracle):
+    """475,115 Gaussians, one 1264x832 view: HIP (float) against the FP32 oracle within
+    1e-4 relative L2 per output, and both against the FP64 renderer, stated per output.
+    The gradients are those of <image, R> for a seeded R (the analytic backward; the
+    FP64 side runs projection, sort, compositing and every VJP in double)."""
+    from msplat._abi import DeviceArray
+    from msplat.backend import render, render_backward
+    from msplat.scenes import street_cameras, street_points
+    n, W, H = 475_115, 1264, 832
+    pts, rgb = street_points(0, n)
+    dg = _perturbed(ctx, pts, rgb, 41)
+    sc = dict(dg.to_numpy(), degree=3)
+    cam = dict(street_cameras(4, W, H, 900.0, 900.0)[1], dtu=3)
+    R = np.random.default_rng(42).normal(0, 1, (H, W, 3)).astype(np.float32)
+    render(ctx, dg, cam, BG)
+    img = ctx.download(ctx.frame().image, (H, W, 3), np.float32)
+    fT = ctx.download(ctx.frame().final_T, (H, W), np.float32)
+    hip = render_backward(ctx, dg, cam, BG, DeviceArray.from_numpy(ctx, R))
+    hip.update(img=img, final_T=fT)
+    f32 = oracle.gradients(sc, cam, BG, R, dtype=np.float32)
+    f64 = oracle.gradients(sc, cam, BG, R, dtype=np.float64)
+    print("\nstreet scale, relative L2 error per output:")
+    print(f"  {'output':8s} {'HIP vs FP32 oracle':>20s} {'HIP vs FP64':>14s} {'FP32 oracle vs FP64':>21s}")
+    for k in ("img", "final_T") + KEYS:
+        a = np.asarray(hip[k]).reshape(f64[k].shape)
+        e32, e64, o64 = _rel(a, f32[k]), _rel(a, f64[k]), _rel(f32[k], f64[k])
+        print(f"  {k:8s} {e32:20.2e} {e64:14.2e} {o64:21.2e}")
+        assert e32 <= 1e-4, (k, e32)
+        # the HIP path is no less accurate than the reference's own FP32 path
+        assert e64 <= 2 * o64 + 1e-6, (k, e64, o64)
+
+
+def test_large_scene_knn_step_densify_reset(ctx, oracle):
+    """3,000,000 Gaussians: GPU kNN against the oracle (bit for bit), one train step at
+    1920x1080 against the oracle (1e-4), then densify/prune (counts and buffers bit for
+    bit, split offsets 2e-6) and an opacity reset (bit for bit), each on identical
+    inputs."""
+    from msplat._abi import DeviceArray, check, lib
+    from msplat.backend import train_step
+    from msplat.scenes import gmm_cloud, spline_cameras
+    n, W, H = 3_000_000, 1920, 1080
+    pts, rgb = gmm_cloud(0, n)
+    dpts = DeviceArray.from_numpy(ctx, pts)
+    knn = DeviceArray(ctx, (n,))
+    check(lib().msplat_hip_knn3_mean_dist(ctx.ptr, dpts.ptr, n, knn.ptr))
+    np.testing.assert_array_equal(knn.numpy(), oracle.knn3(pts))
+    del dpts, knn
+    dg = _perturbed(ctx, pts, rgb, 43, capacity=3 * n)
+    st = dg.to_numpy()
+    cam = dict(spline_cameras(200, W, H)[50], dtu=3)
+    gt = np.random.default_rng(44).uniform(0, 1, (H, W, 3)).astype(np.float32)
+    loss = train_step(ctx, dg, cam, BG, DeviceArray.from_numpy(ctx, gt), STEP_SIZES, BC2)
+    M = ctx.frame().num_isects
+    img = ctx.download(ctx.frame().image, (H, W, 3), np.float32)
+    ref_loss, ref_radii, ref_img = oracle.train_step(st, cam, BG, gt, STEP_SIZES, BC2)
+    got = dg.to_numpy()
+    print(f"\nlarge scene: {int((ref_radii > 0).sum())} visible, {M} intersections, loss {loss:.7f} "
+          f"(oracle {ref_loss:.7f})")
+    assert M > 5_000_000
+    assert abs(loss - ref_loss) <= 1e-6 * abs(ref_loss)
+    assert _rel(img, ref_img) < 1e-4
+    for k in KEYS:
+        for pre in ("", "m1_", "m2_"):
+            e = _rel(got[pre + k], st[pre + k].reshape(got[pre + k].shape))
+            assert e < 1e-4, (pre + k, e)
+    for k in ("vis", "gn", "ms"):
+        assert _rel(got[k], st[k]) < 1e-4, k
+
+    # densify/prune on the HIP state, both sides from identical inputs
+    widths = [3, 3, 4, 3, 45, 1]
+    bufs = []
+    for pre in ("", "m1_", "m2_"):
+        for k, w in zip(KEYS, widths):
+            b = np.zeros((3 * n, w), np.float32)
+            b[:n] = got[pre + k].reshape(n, w)
+            bufs.append(b)
+    # After one step the Gaussians with large gradients are almost all large splats; these
+    # thresholds (gradient 1e-5, size 0.2 m, instead of the schedule's 2e-4 and 0.01 m)
+    # give every class (split, duplicate, cull) its hundreds of members.
+    seed = (3000 * 2654435761) % 2 ** 32
+    kw = dict(grad_thr=1e-5, size_thr=0.2, screen_thr=0.05, check_screen=1, cull_alpha=0.1, cull_scale=0.5,
+              cull_screen=0.15, check_huge=1)
+    n_ref, split, dup, keep = oracle.densify(bufs, n, 3 * n, got["gn"], got["vis"], got["ms"], 0.5 * W, 45, seed,
+                                             **kw)
+    n_new = dg.densify(kw["grad_thr"], kw["size_thr"], kw["screen_thr"], kw["check_screen"], kw["cull_alpha"],
+                       kw["cull_scale"], kw["cull_screen"], kw["check_huge"], 0.5 * W, seed)
+    print(f"densify: {int(split.sum())} split, {int(dup.sum())} duplicated, {int((keep == 0).sum())} culled, "
+          f"{n} -> {n_new}")
+    assert split.sum() > 500 and dup.sum() > 100 and (keep == 0).sum() > 1000
+    assert n_new == n_ref
+    out = dg.to_numpy()
+    for i, b in enumerate(bufs):
+        pre = ("", "m1_", "m2_")[i // 6]
+        a = out[pre + KEYS[i % 6]].reshape(n_new, -1)
+        if i == 0:  # split children carry the Box-Muller offsets
+            np.testing.assert_allclose(a, b[:n_new], rtol=2e-6, atol=1e-7)
+        else:
+            np.testing.assert_array_equal(a, b[:n_new], err_msg=pre + KEYS[i % 6])
+
+    # opacity reset on the densified state
+    o, m1, m2 = (out[p + "opac"].reshape(-1).copy() for p in ("", "m1_", "m2_"))
+    dg.opacity_reset(-1.3862943611198906)
+    oracle.opacity_reset(o, m1, m2, np.float32(-1.3862943611198906))
+    after = dg.to_numpy()
+    for p, ref in zip(("", "m1_", "m2_"), (o, m1, m2)):
+        np.testing.assert_array_equal(after[p + "opac"].reshape(-1), ref)
+
+
+def test_batch_of_sixteen_mixed_views_at_one_million(ctx):
+    """16 views of 1,000,000 Gaussians in one batch, sizes drawn from the four
+    resolutions (each at least once): every image bit for bit equal to its one-view
+    render, and the batched backward within 2e-5 (of each group's largest) of the sum of
+    the 16 one-view backwards."""
+    from msplat._abi import DeviceArray
+    from msplat.backend import (BatchViews, render, render_backward, render_backward_batch, render_batch,
+                                render_batch_numpy, render_numpy)
+    from msplat.camera import make_camera
+    from msplat.scenes import street_cameras, street_points
+    n = 1_000_000
+    sizes = [(640, 360), (1264, 832), (1920, 1080), (2560, 1440)]
+    rng = np.random.default_rng(50)
+    pick = list(range(4)) + list(rng.integers(0, 4, 12))
+    path = street_cameras(16, 64, 64)
+    cams = []
+    for k, s in enumerate(pick):
+        W, H = sizes[s]
+        f = 900.0 * W / 1264.0
+        cams.append(make_camera(path[k]["cam_to_world"], W, H, f, f, dtu=3))
+    pts, rgb = street_points(51, n)
+    dg = _perturbed(ctx, pts, rgb, 52)
+    bv = BatchViews(cams, 3, [tuple(BG)])
+    imgs = render_batch_numpy(ctx, dg, bv)
+    for k, (cam, img) in enumerate(zip(cams, imgs)):
+        np.testing.assert_array_equal(img, render_numpy(ctx, dg, cam, BG), err_msg=f"view {k}")
+    v = [(rng.normal(0, 1, (c["H"], c["W"], 3)) * 1e-3).astype(np.float32) for c in cams]
+    total = {}
+    for cam, vi in zip(cams, v):
+        render(ctx, dg, cam, BG)
+        for key, x in render_backward(ctx, dg, cam, BG, DeviceArray.from_numpy(ctx, vi)).items():
+            total[key] = total.get(key, 0) + x.astype(np.float64)
+    render_batch(ctx, dg, bv)
+    got = render_backward_batch(ctx, dg, bv, DeviceArray.from_numpy(ctx, np.concatenate([x.reshape(-1, 3) for x in v])))
+    print(f"\nbatch of 16 views ({sum(c['W'] * c['H'] for c in cams) / 1e6:.1f} Mpx):")
+    for key in total:
+        err = np.abs(got[key] - total[key]).max() / max(np.abs(total[key]).max(), 1e-30)
+        print(f"  {key}: max error {err:.2e} of the largest")
+        assert err <= 2e-5, (key, err)
+
+
+def test_serving_alone_and_in_the_mix_at_real_scenes(ctx):
+    """Four resident scenes (0.5M, 1M, 2M, 3M Gaussians, 64-cluster clouds) and the
+    request mix: a burst at 2000 req/s through the continuous-batching server (two slots),
+    then every request alone; RGBA8 images bit for bit equal."""
+    from msplat.backend import DeviceGaussians
+    from msplat.scenes import gmm_cloud
+    from msplat.serving import Server, poisson_requests, serve_plain
+    centers = [(60.0 * k, 0.0, 0.0) for k in range(4)]
+    scenes = []
+    for k, (n, c) in enumerate(zip((500_000, 1_000_000, 2_000_000, 3_000_000), centers)):
+        pts, rgb = gmm_cloud(60 + k, n, c, box=8.0)
+        dg = DeviceGaussians.from_points(ctx, pts, rgb, degree=3)
+        rng = np.random.default_rng(70 + k)
+        dg.params["opac"].upload(rng.normal(1.5, 1.0, (n, 1)).astype(np.float32))
+        dg.params["rest"].upload(rng.normal(0, 0.05, (n, 45)).astype(np.float32))
+        scenes.append(dg)
+    ctx.sync()
+    reqs = poisson_requests(2000.0, 0.04, centers, seed=61)
+    assert len(reqs) >= 50
+    srv = Server(scenes, slots=2, max_views=16, background=tuple(BG), keep_images=True)
+    try:
+        done = srv.run(reqs)
+    finally:
+        srv.close()
+    assert len(done) == len(reqs) and not srv.rejected
+    assert max(r.batch_size for r in done) > 1
+    alone = serve_plain(scenes, [type(r)(r.rid, r.scene, r.cam, 0.0) for r in reqs], background=tuple(BG),
+                        keep_images=True)
+    by_id = {r.rid: r for r in alone}
+    sizes = sorted({(r.cam["W"], r.cam["H"]) for r in done})
+    print(f"\nserving: {len(done)} requests, sizes {sizes}, largest batch {max(r.batch_size for r in done)}")
+    for r in done:
+        np.testing.assert_array_equal(r.image, by_id[r.rid].image, err_msg=f"request {r.rid}")
+    assert sum(r.image[..., :3].std() > 2 for r in done) > len(done) // 2

@@ -50,3 +50,16 @@ def test_gradcheck_scene_is_smooth(oracle):
         tile = (yy.astype(int) // 16) * 2 + xx.astype(int) // 16
         assert (r["final_idx"] == 64 * tile + 63).all()  # every pixel composites all 64 of its tile
         assert p["radii"].min() > 45  # every rect covers the 2x2 tiles
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_oracle_analytic_gradients_match_fp64_finite_differences(oracle, dtype):
+    """Oracle.gradients (the analytic backward of <image, R>, in double for the FP64
+    renderer's per-output errors and in float) against central differences of the FP64
+    renderer, view-independent colours (means included)."""
+    sc, cam = gs.scene(False), gs.camera()
+    R = gs.linear_weights()
+    g = oracle.gradients(sc, cam, gs.BG, R, dtype=dtype)
+    assert g["img"].dtype == dtype
+    fds = gs.finite_differences(oracle, sc, cam, "linear")
+    assert gs.compare({k: g[k] for k in gs.KEYS}, fds, f"analytic {np.dtype(dtype).name}") >= 64 * 20

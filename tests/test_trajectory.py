@@ -1,0 +1,86 @@
+"""The 300-step trajectory of the CPU-runnable plain workload (20,000 Gaussians, SH degree
+1, 8 views at 128x128, 0.8 L1 + 0.2 (1 - SSIM), no densification) against the compiled
+reference's own run (tests/golden/trajectory_c1.npz, written by make_golden.py through
+core/cpu/msplat_cpu.cpp and Model's schedule, model.cpp:140-143, 535, 626-682).
+
+The oracle must reproduce it bit for bit; the HIP trainer within the stated tolerances
+(its per-pixel exp and reciprocal are the hardware's, so the runs drift apart by float
+rounding over 300 Adam steps)."""
+import hashlib
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from workloads import BG, C1_LRS, C1_STEPS, c1_workload
+
+KEYS = ("means", "scales", "quats", "dc", "rest", "opac")
+
+# HIP trainer against the reference's run
+LOSS_REL_TOL = 1e-4        # every step's loss
+PSNR_TOL_DB = 0.05         # final PSNR of every view
+PARAM_REL_TOL = 1e-3       # relative L2 of each group's sampled final parameters
+SUM_REL_TOL = 1e-4         # relative difference of each group's sum of squares
+
+
+def _targets(orc, gt_scene, cams, d):
+    gts = [orc.render(gt_scene, c, BG)[2]["img"] for c in cams]
+    digest = hashlib.sha256(np.ascontiguousarray(np.stack(gts)).tobytes()).digest()
+    assert digest == d["gt_sha256"].tobytes(), "the oracle's targets differ from the reference's"
+    return gts
+
+
+def test_oracle_trajectory_matches_reference(oracle):
+    d = golden("trajectory_c1")
+    sc, gt_scene, cams = c1_workload()
+    gts = _targets(oracle, gt_scene, cams, d)
+    st = {k: sc[k].copy() for k in KEYS}
+    for k in KEYS:
+        st["m1_" + k], st["m2_" + k] = np.zeros_like(sc[k]), np.zeros_like(sc[k])
+    n = sc["means"].shape[0]
+    st.update(degree=1, vis=np.zeros(n, np.float32), gn=np.zeros(n, np.float32), ms=np.zeros(n, np.float32))
+    lrs, losses = list(C1_LRS), []
+    for t in range(1, C1_STEPS + 1):
+        sb = [oracle.adam_scalars(np.float32(lr), t) for lr in lrs]
+        v = d["order"][t - 1]
+        loss, _, _ = oracle.train_step(st, cams[v], BG, gts[v], np.array([s for s, _ in sb], np.float32),
+                                       np.array([b for _, b in sb], np.float32))
+        losses.append(loss)
+        lrs[0] = float(oracle.means_lr(t, C1_STEPS))
+    np.testing.assert_array_equal(np.array(losses, np.float32), d["losses"])
+    for k in KEYS:
+        np.testing.assert_array_equal(st[k][d["sample"]], d["final_" + k])
+
+
+@pytest.mark.gpu
+def test_hip_trajectory_matches_reference(ctx, oracle):
+    from msplat._abi import DeviceArray
+    from msplat.backend import DeviceGaussians
+    from msplat.trainer import TrainConfig, Trainer
+    d = golden("trajectory_c1")
+    sc, gt_scene, cams = c1_workload()
+    gts = _targets(oracle, gt_scene, cams, d)
+    dg = DeviceGaussians(ctx, sc)
+    cfg = TrainConfig(iterations=C1_STEPS, sh_degree=1, sh_degree_interval=1, densify=False, seed=42)
+    tr = Trainer(ctx, dg, cams, [DeviceArray.from_numpy(ctx, g) for g in gts], cfg)
+    picked = []
+    pick = tr.next_camera
+    tr.next_camera = lambda: picked.append(pick()) or picked[-1]
+    losses = np.array([tr.step(read_loss=True) for _ in range(C1_STEPS)], np.float32)
+    np.testing.assert_array_equal(np.array(picked), d["order"])
+    rel = np.abs(losses - d["losses"]) / d["losses"]
+    print(f"\nc1 trajectory: loss max rel {rel.max():.3e} (step {int(rel.argmax()) + 1}), "
+          f"final {losses[-1]:.8f} vs {d['losses'][-1]:.8f}")
+    assert rel.max() <= LOSS_REL_TOL, rel.max()
+    got = dg.to_numpy()
+    psnr = np.array([oracle.image_metrics(oracle.render(dict(got, degree=1), c, BG)[2]["img"], g)[0]
+                     for c, g in zip(cams, gts)])
+    print("final PSNR per view (dB):", np.round(psnr, 4), "delta", np.round(psnr - d["psnr"], 5))
+    assert np.abs(psnr - d["psnr"]).max() <= PSNR_TOL_DB
+    for k in KEYS:
+        a = got[k].reshape(sc[k].shape)
+        e = np.linalg.norm(a[d["sample"]] - d["final_" + k]) / np.linalg.norm(d["final_" + k])
+        s = abs(float((a.astype(np.float64) ** 2).sum()) - float(d["sumsq_" + k])) / float(d["sumsq_" + k])
+        print(f"  {k}: sampled rel L2 {e:.2e}, sum of squares rel {s:.2e}")
+        assert e <= PARAM_REL_TOL, (k, e)
+        assert s <= SUM_REL_TOL, (k, s)
