@@ -237,6 +237,16 @@ int msplat_hip_opacity_reset(msplat_hip_ctx *ctx, float *opacities, float *exp_a
  * int32 payload (device) by the low key_bits bits, in place. */
 int msplat_hip_radix_sort(msplat_hip_ctx *ctx, uint64_t *keys, int32_t *vals, uint32_t n, int key_bits);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the same sort as msplat_hip_radix_sort, with identical results, that allocates
+ * nothing. Its scratch is a static device array sized for MSPLAT_NOALLOC_SORT_MAX
+ * elements (about 13.6 MB, held by every GPU the library runs on until the process
+ * ends); larger n returns MSPLAT_ERR_RANGE before any launch. All calls on one GPU
+ * share that array: calls that overlap on one GPU race. */
+#define MSPLAT_NOALLOC_SORT_MAX (1u << 20)
+int msplat_hip_radix_sort_noalloc(msplat_hip_ctx *ctx, uint64_t *keys, int32_t *vals, uint32_t n, int key_bits);
+
 /* msplat_knn3_mean_dist (bindings.h:149-151): exact mean distance to the 3 nearest
  * other points. points (n,3) and mean_dist (n) are device pointers. */
 int msplat_hip_knn3_mean_dist(msplat_hip_ctx *ctx, const float *points, uint32_t n, float *mean_dist);

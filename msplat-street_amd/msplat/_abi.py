@@ -113,6 +113,7 @@ def lib():
             "msplat_hip_stage_raster_forward": ([vp, C.POINTER(Camera), fp], C.c_int),
             "msplat_hip_stage_loss": ([vp, C.POINTER(Camera), vp, f32, f32, fp], C.c_int),
             "msplat_hip_radix_sort": ([vp, vp, vp, u32, C.c_int], C.c_int),
+            "msplat_hip_radix_sort_noalloc": ([vp, vp, vp, u32, C.c_int], C.c_int),
             "msplat_hip_densify": ([vp, i32, i32, f32, f32, f32, C.c_int, f32, f32, f32, C.c_int, vp, vp, vp, f32,
                                     C.c_void_p * 18, i32, u32, C.POINTER(i32)], C.c_int),
             "msplat_hip_opacity_reset": ([vp, vp, vp, vp, i32, f32], C.c_int),
