@@ -91,6 +91,9 @@ int msplat_hip_commit(msplat_hip_ctx *ctx);
 /* gpu_empty / gpu_zeros (bindings.h:18-19): device allocation. */
 int msplat_hip_malloc(size_t bytes, int zero, void **out);
 int msplat_hip_free(void *ptr);
+/* Free and total device memory of `device` (no reference counterpart: the serving
+ * loop's default HBM budget). */
+int msplat_hip_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
 /* Copies on the context's stream; kind 0 = host->device, 1 = device->host (syncs),
  * 2 = device->device (msplat_copy_buffer, bindings.h:143). */
 int msplat_hip_memcpy(msplat_hip_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
@@ -157,6 +160,7 @@ typedef struct {
     float *v_xy, *v_conic, *v_rgb, *v_opacity;
     double *loss_sum;
     uint64_t *slots_written;
+    int64_t num_visible; /* gaussians with a non-empty tile rectangle (radius > 0) */
 } msplat_hip_frame;
 int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *out);
 /* KEEP_GRADS: fill the reduced gradients; KEYS64: materialise isect_keys on frame_view

@@ -176,6 +176,7 @@ int do_bin(msplat_hip_ctx *ctx) {
         return set_error(MSPLAT_ERR_RANGE, "more than 2^31 tile intersections (%llu); lower the resolution",
                          (unsigned long long)M);
     ctx->M = (int64_t)M;
+    ctx->V = (int64_t)ctx->h_total[5];
     const size_t Mc = M > 0 ? M : 1;
     CHECKED(ctx->grow(ctx->keys0, 4 * Mc));
     CHECKED(ctx->grow(ctx->keys1, 4 * Mc));
@@ -431,6 +432,17 @@ int msplat_hip_malloc(size_t bytes, int zero, void **out) {
     return MSPLAT_OK;
 }
 
+int msplat_hip_device_memory(int device, size_t *free_bytes, size_t *total_bytes) {
+    if (!free_bytes || !total_bytes) return set_error(MSPLAT_ERR_ARG, "NULL output");
+    int prev = 0;
+    HIPCHECK(hipGetDevice(&prev));
+    HIPCHECK(hipSetDevice(device));
+    const hipError_t e = hipMemGetInfo(free_bytes, total_bytes);
+    HIPCHECK(hipSetDevice(prev));
+    if (e != hipSuccess) return set_error(MSPLAT_ERR_HIP, "hipMemGetInfo failed: %s", hipGetErrorString(e));
+    return MSPLAT_OK;
+}
+
 int msplat_hip_free(void *ptr) {
     if (ptr) HIPCHECK(hipFree(ptr));
     return MSPLAT_OK;
@@ -621,6 +633,7 @@ int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *f) {
     f->num_points = ctx->n;
     f->num_tiles = ctx->ntiles;
     f->num_isects = ctx->M;
+    f->num_visible = ctx->V;
     f->splat_a = (float *)ctx->splat_a.p;
     f->splat_b = (float *)ctx->splat_b.p;
     f->splat_blue = (float *)ctx->splat_blue.p;

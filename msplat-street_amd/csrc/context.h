@@ -153,6 +153,7 @@ struct msplat_hip_ctx {
     uint32_t width = 0, height = 0;
     int tiles_x = 0, tiles_y = 0;
     int64_t M = 0;
+    int64_t V = 0;  // visible gaussians of the frame (tile count > 0)
     bool slot_f_clean = false;  // slot_f[0, M) is all zero: set by the key emission, cleared by a raster backward
     // How far the one-view frame in the workspace is built: the stage entry points refuse
     // to run on a frame whose inputs are missing (they would read unset device pointers).

@@ -74,7 +74,7 @@ class Frame(C.Structure):
                 ("tile_start", C.c_void_p), ("image", C.c_void_p), ("final_T", C.c_void_p),
                 ("v_image", C.c_void_p), ("final_idx", C.c_void_p), ("v_xy", C.c_void_p),
                 ("v_conic", C.c_void_p), ("v_rgb", C.c_void_p), ("v_opacity", C.c_void_p),
-                ("loss_sum", C.c_void_p), ("slots_written", C.c_void_p)]
+                ("loss_sum", C.c_void_p), ("slots_written", C.c_void_p), ("num_visible", C.c_int64)]
 
 
 _lib = None
@@ -112,6 +112,7 @@ def lib():
             "msplat_hip_stage_bin": ([vp, C.POINTER(Camera)], C.c_int),
             "msplat_hip_stage_raster_forward": ([vp, C.POINTER(Camera), fp], C.c_int),
             "msplat_hip_stage_loss": ([vp, C.POINTER(Camera), vp, f32, f32, fp], C.c_int),
+            "msplat_hip_device_memory": ([C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
             "msplat_hip_radix_sort": ([vp, vp, vp, u32, C.c_int], C.c_int),
             "msplat_hip_radix_sort_noalloc": ([vp, vp, vp, u32, C.c_int], C.c_int),
             "msplat_hip_densify": ([vp, i32, i32, f32, f32, f32, C.c_int, f32, f32, f32, C.c_int, vp, vp, vp, f32,
@@ -263,6 +264,13 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def device_memory(device: int = 0) -> tuple:
+    """(free, total) bytes of device memory."""
+    free, total = C.c_size_t(), C.c_size_t()
+    check(lib().msplat_hip_device_memory(device, C.byref(free), C.byref(total)))
+    return free.value, total.value
 
 
 def make_camera_struct(cam: dict, degree: int, glob_scale: float = 1.0, clip: float = 0.01) -> Camera:

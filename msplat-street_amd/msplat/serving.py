@@ -63,15 +63,27 @@ class _Slot:
         self.busy = False
 
     def rgba_buffer(self, pixels: int) -> DeviceArray:
+        """The slot's RGBA8 output buffer, grown on the slot's own thread; the old one is
+        freed there explicitly (its last pack completed before the slot's sync), never
+        by the garbage collector on another thread."""
         if self.rgba is None or self.rgba.nbytes < 4 * pixels:
-            self.rgba = DeviceArray(self.ctx, (int(pixels * 1.25) + 1024, 4), np.uint8)
+            old, self.rgba = self.rgba, DeviceArray(self.ctx, (int(pixels * 1.25) + 1024, 4), np.uint8)
+            if old is not None:
+                old.free()
         return self.rgba
+
+    def close(self):
+        if self.rgba is not None:
+            self.ctx.sync()
+            self.rgba.free()
+            self.rgba = None
+        self.ctx.close()
 
 
 class Server:
     """Continuous-batching renderer over resident scenes (DeviceGaussians)."""
 
-    def __init__(self, scenes: list, slots: int = 2, max_views: int = 16, hbm_budget: float = 64e9,
+    def __init__(self, scenes: list, slots: int = 2, max_views: int = 16, hbm_budget: float | None = None,
                  background=(0.0, 0.0, 0.0), device: int = 0, keep_images: bool = False,
                  isects_per_pixel: float = 60.0, adapt: bool = True, batch_depth: int | None = None,
                  max_batch_pixels: float = 2.1e6):
@@ -84,6 +96,9 @@ class Server:
         self.bg = tuple(float(v) for v in background)
         self.keep_images = keep_images
         self.resident = sum(s.n * PARAM_BYTES for s in scenes)
+        if hbm_budget is None:  # the resident scenes plus 90% of what the device has free
+            from ._abi import device_memory
+            hbm_budget = self.resident + 0.9 * device_memory(device)[0]
         self.budget = float(hbm_budget)
         if self.resident >= self.budget:
             raise ValueError("the resident scenes alone exceed the HBM budget")
@@ -148,8 +163,8 @@ class Server:
             try:
                 self._render(slot, batch)
             except Exception as e:  # recorded; the run reports it
-                self.errors.append(f"slot {slot.index}: {e}")
                 with self.cond:
+                    self.errors.append(f"slot {slot.index}: {e}")
                     self.rejected.extend(batch)
 
     def _render(self, slot: _Slot, batch: list):
@@ -209,7 +224,7 @@ class Server:
 
     def close(self):
         for s in self.slots:
-            s.ctx.close()
+            s.close()
 
 
 def serve_plain(scenes: list, requests: list, background=(0.0, 0.0, 0.0), device: int = 0,
@@ -231,7 +246,9 @@ def serve_plain(scenes: list, requests: list, background=(0.0, 0.0, 0.0), device
             out = C.c_void_p()
             check(lib().msplat_hip_render(ctx.ptr, C.byref(g), C.byref(c), bg, C.byref(out)))
             if rgba is None or rgba.nbytes < 4 * r.pixels:
-                rgba = DeviceArray(ctx, (int(r.pixels * 1.25) + 1024, 4), np.uint8)
+                old, rgba = rgba, DeviceArray(ctx, (int(r.pixels * 1.25) + 1024, 4), np.uint8)
+                if old is not None:  # the previous request's pack completed at its sync
+                    old.free()
             check(lib().msplat_hip_pack_rgba8(ctx.ptr, out, r.pixels, C.c_void_p(rgba.ptr)))
             ctx.sync()
             r.done = time.perf_counter() - t0
@@ -240,6 +257,9 @@ def serve_plain(scenes: list, requests: list, background=(0.0, 0.0, 0.0), device
                 r.image = ctx.download(rgba.ptr, (r.pixels, 4), np.uint8).reshape(r.cam["H"], r.cam["W"], 4)
         return requests
     finally:
+        ctx.sync()
+        if rgba is not None:
+            rgba.free()
         ctx.close()
 
 

@@ -319,8 +319,10 @@ class Reference:
     """The reference's compiled CPU backend (core/cpu/msplat_cpu.cpp), built from the
     mounted reference tree into oracle/_ref by oracle/Makefile."""
 
-    def __init__(self, perf: bool = False):
-        L = self.lib = _load(os.path.join("_ref", "libmsplat_ref_perf.so" if perf else "libmsplat_ref.so"))
+    def __init__(self, perf: bool = False, isa: str = "v3"):
+        name = "libmsplat_ref.so" if not perf else ("libmsplat_ref_perf_v4.so" if isa == "v4" else
+                                                     "libmsplat_ref_perf.so")
+        L = self.lib = _load(os.path.join("_ref", name))
         L.ref_render.argtypes = ([C.c_int, f32p, f32p, f32p, f32p, f32p, C.c_int, f32p, f32p, f32p, C.c_float,
                                   C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
                                   f32p, f32p, f32p, vp])
@@ -412,6 +414,16 @@ class Reference:
         return out
 
 
-def reference_available(perf: bool = False) -> bool:
-    name = "libmsplat_ref_perf.so" if perf else "libmsplat_ref.so"
+def host_isa() -> str:
+    """'v4' when this host runs x86-64-v4 code (AVX-512 F/BW/CD/DQ/VL), else 'v3'."""
+    try:
+        flags = next((l for l in open("/proc/cpuinfo") if l.startswith("flags")), "").split()
+    except OSError:
+        return "v3"
+    return "v4" if all(f in flags for f in ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl")) else "v3"
+
+
+def reference_available(perf: bool = False, isa: str = "v3") -> bool:
+    name = "libmsplat_ref.so" if not perf else ("libmsplat_ref_perf_v4.so" if isa == "v4" else
+                                                "libmsplat_ref_perf.so")
     return os.path.exists(os.path.join(_HERE, "_ref", name))
