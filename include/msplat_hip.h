@@ -259,6 +259,55 @@ int msplat_hip_knn3_mean_dist(msplat_hip_ctx *ctx, const float *points, uint32_t
 int msplat_hip_init_gaussians(msplat_hip_ctx *ctx, const float *means, const uint8_t *rgb, uint32_t n, uint32_t seed,
                               float opacity_logit, float *scales, float *quats, float *features_dc, float *opacities);
 
+/* msplat_resize_area (bindings.h:160-163): area resampling of a (sh, sw) image into
+ * (dh, dw, 3) float RGB (device). src is RGBA8 (4 bytes per pixel, alpha ignored, scaled
+ * by 1/255) when src_is_u8, else float RGB. The ground-truth pyramid's step. */
+int msplat_hip_resize_area(msplat_hip_ctx *ctx, const void *src, int src_is_u8, int sw, int sh, float *dst, int dw,
+                           int dh);
+
+/* msplat_undistort (bindings.h:165-168): Brown-Conrady undistortion of (sh, sw, 3) float
+ * RGB (device) into the (dh, dw, 3) crop at (roi_x, roi_y) of the undistorted image.
+ * intr = fx, fy, cx, cy; dist = k1, k2, p1, p2, k3 (host). */
+int msplat_hip_undistort(msplat_hip_ctx *ctx, const float *src, int sw, int sh, const float intr[4],
+                         const float dist[5], int roi_x, int roi_y, float *dst, int dw, int dh);
+
+/* ── trained-scene files (core/src/loaders/save_gaussians.cpp, model.cpp:282-496) ──
+ * Host I/O over device parameters (each call syncs the context). rest_bases is the
+ * number of SH rest bases stored per gaussian ((degree+1)^2 - 1). keep_crs applies the
+ * inverse of the dataset's scale and translation, as GaussianParams does. */
+/* saveGaussianPly (save_gaussians.cpp:10-52): binary little-endian PLY. */
+int msplat_hip_save_ply(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, int rest_bases, const char *path,
+                        int step, float scale, const float translation[3], int keep_crs);
+/* saveGaussianSplat (save_gaussians.cpp:54-104): 32-byte records, largest first. */
+int msplat_hip_save_splat(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const char *path, float scale,
+                          const float translation[3], int keep_crs);
+/* loadGaussianPly's header (save_gaussians.cpp:106-131): vertex count, rest bases, step. */
+int msplat_hip_ply_info(const char *path, int32_t *num_points, int32_t *rest_bases, int32_t *step);
+/* loadGaussianPly (save_gaussians.cpp:106-191) into device buffers of g->num_points rows
+ * (the count and rest bases must be the file's, from msplat_hip_ply_info). */
+int msplat_hip_load_ply(msplat_hip_ctx *ctx, const char *path, const msplat_hip_gaussians *g, int rest_bases,
+                        float scale, const float translation[3], int keep_crs);
+
+/* Model::saveCheckpoint / loadCheckpoint (model.cpp:282-496): the .msplat v2 file
+ * ("MSPL", version 2): step, gaussian count, SH degree, Adam step count and learning
+ * rates, the six parameter groups and both Adam moments as shaped tensors, then the
+ * learned-sky and exposure state (written absent; skipped on load, flagged in has_sky /
+ * has_exposure). Device pointers; exp_avg[4] / exp_avg_sq[4] may be NULL at degree 0. */
+typedef struct {
+    int32_t version, step, sh_degree, adam_step_count;
+    float adam_lr[6], means_lr_init, means_lr_final;
+    msplat_hip_gaussians gaussians;
+    float *exp_avg[6], *exp_avg_sq[6];
+    int32_t has_sky, has_exposure;
+} msplat_hip_checkpoint;
+int msplat_hip_save_checkpoint(msplat_hip_ctx *ctx, const char *path, const msplat_hip_checkpoint *ckpt);
+/* The scalar header: version, step, gaussian count (gaussians.num_points), SH degree,
+ * Adam step count and learning rates; no device access. */
+int msplat_hip_checkpoint_info(const char *path, msplat_hip_checkpoint *out);
+/* Reads the tensors into ckpt's device buffers (num_points and sh_degree set to the
+ * file's) and fills the scalar fields. */
+int msplat_hip_load_checkpoint(msplat_hip_ctx *ctx, const char *path, msplat_hip_checkpoint *ckpt);
+
 /* msplat_image_metrics (bindings.h:170-172): PSNR, SSIM (clamp-to-edge), L1. Syncs. */
 int msplat_hip_image_metrics(msplat_hip_ctx *ctx, const float *rendered, const float *gt, int h, int w,
                              double out[3]);

@@ -77,6 +77,13 @@ class Frame(C.Structure):
                 ("loss_sum", C.c_void_p), ("slots_written", C.c_void_p), ("num_visible", C.c_int64)]
 
 
+class Checkpoint(C.Structure):
+    _fields_ = [("version", C.c_int32), ("step", C.c_int32), ("sh_degree", C.c_int32), ("adam_step_count", C.c_int32),
+                ("adam_lr", C.c_float * 6), ("means_lr_init", C.c_float), ("means_lr_final", C.c_float),
+                ("gaussians", Gaussians), ("exp_avg", C.c_void_p * 6), ("exp_avg_sq", C.c_void_p * 6),
+                ("has_sky", C.c_int32), ("has_exposure", C.c_int32)]
+
+
 _lib = None
 
 
@@ -113,6 +120,15 @@ def lib():
             "msplat_hip_stage_raster_forward": ([vp, C.POINTER(Camera), fp], C.c_int),
             "msplat_hip_stage_loss": ([vp, C.POINTER(Camera), vp, f32, f32, fp], C.c_int),
             "msplat_hip_device_memory": ([C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
+            "msplat_hip_resize_area": ([vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_int, C.c_int], C.c_int),
+            "msplat_hip_undistort": ([vp, vp, C.c_int, C.c_int, fp, fp, C.c_int, C.c_int, vp, C.c_int, C.c_int], C.c_int),
+            "msplat_hip_save_ply": ([vp, C.POINTER(Gaussians), C.c_int, C.c_char_p, C.c_int, f32, fp, C.c_int], C.c_int),
+            "msplat_hip_save_splat": ([vp, C.POINTER(Gaussians), C.c_char_p, f32, fp, C.c_int], C.c_int),
+            "msplat_hip_ply_info": ([C.c_char_p, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
+            "msplat_hip_load_ply": ([vp, C.c_char_p, C.POINTER(Gaussians), C.c_int, f32, fp, C.c_int], C.c_int),
+            "msplat_hip_save_checkpoint": ([vp, C.c_char_p, C.POINTER(Checkpoint)], C.c_int),
+            "msplat_hip_checkpoint_info": ([C.c_char_p, C.POINTER(Checkpoint)], C.c_int),
+            "msplat_hip_load_checkpoint": ([vp, C.c_char_p, C.POINTER(Checkpoint)], C.c_int),
             "msplat_hip_radix_sort": ([vp, vp, vp, u32, C.c_int], C.c_int),
             "msplat_hip_radix_sort_noalloc": ([vp, vp, vp, u32, C.c_int], C.c_int),
             "msplat_hip_densify": ([vp, i32, i32, f32, f32, f32, C.c_int, f32, f32, f32, C.c_int, vp, vp, vp, f32,

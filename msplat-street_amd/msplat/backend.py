@@ -35,7 +35,9 @@ class DeviceGaussians:
         for k in KEYS:
             w = self.widths[k]
             host = np.zeros((self.capacity, w), np.float32)
-            host[:self.n] = np.asarray(scene[k], np.float32).reshape(self.n, -1)[:, :w] if self.n else host[:0]
+            if self.n:  # a degree-0 scene's rest may have no columns: its placeholder stays zero
+                src = np.asarray(scene[k], np.float32).reshape(self.n, -1)[:, :w]
+                host[:self.n, :src.shape[1]] = src
             self.params[k] = DeviceArray.from_numpy(ctx, host)
             self.m1[k] = DeviceArray(ctx, (self.capacity, w))
             self.m2[k] = DeviceArray(ctx, (self.capacity, w))

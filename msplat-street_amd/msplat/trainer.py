@@ -133,6 +133,24 @@ class Trainer:
         self.after_train(step)
         return loss.value if read_loss else None
 
+    def save_checkpoint(self, path):
+        """Model::saveCheckpoint (model.cpp:311-370): the step, the Adam step count and
+        learning rates, parameters and both moments."""
+        from .scene_io import save_checkpoint
+        save_checkpoint(self.ctx, path, self.dg, self.step_count, self.adam_step_count, self.lr, LR_INIT[0],
+                        MEANS_LR_FINAL)
+
+    @classmethod
+    def from_checkpoint(cls, ctx: Context, path, cameras, targets, config: "TrainConfig") -> "Trainer":
+        """A trainer resumed from a checkpoint (Model::loadCheckpoint, model.cpp:372-496):
+        parameters and moments in fresh 4N-capacity buffers, the step, Adam step count and
+        learning rates restored; the densification statistics restart from zero."""
+        from .scene_io import load_checkpoint
+        dg, info = load_checkpoint(ctx, path)
+        tr = cls(ctx, dg, cameras, targets, config)
+        tr.step_count, tr.adam_step_count, tr.lr = info["step"], info["adam_step_count"], list(info["adam_lr"])
+        return tr
+
     def schedulers_step(self, step: int):
         self.lr[0] = means_lr(step, self.cfg.iterations)
 

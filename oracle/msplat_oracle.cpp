@@ -1203,6 +1203,69 @@ extern "C" void orc_pack_rgba8(const float *img, uint32_t n, uint8_t *out) {
     }
 }
 
+// msplat_resize_area (msplat_cpu.cpp:1609-1639): box resampling; every output pixel
+// averages the source pixels under its footprint, weighted by overlap area (rows outer,
+// columns inner). u8 sources are RGBA8 scaled by 1/255.
+extern "C" void orc_resize_area(const void *src, int src_is_u8, int sw, int sh, float *out, int dw, int dh) {
+    if (dw <= 0 || dh <= 0) return;
+    const uint8_t *u8 = src_is_u8 ? (const uint8_t *)src : nullptr;
+    const float *f = src_is_u8 ? nullptr : (const float *)src;
+    const float scx = (float)sw / (float)dw, scy = (float)sh / (float)dh;
+#pragma omp parallel for schedule(static)
+    for (int gy = 0; gy < dh; gy++) {
+        const float y0 = (float)gy * scy, y1 = (float)(gy + 1) * scy;
+        const int iy0 = (int)y0, iy1 = std::min((int)std::ceil(y1), sh);
+        for (int gx = 0; gx < dw; gx++) {
+            const float x0 = (float)gx * scx, x1 = (float)(gx + 1) * scx;
+            const int ix0 = (int)x0, ix1 = std::min((int)std::ceil(x1), sw);
+            float acc[3] = {0.f, 0.f, 0.f}, area = 0.f;
+            for (int iy = iy0; iy < iy1; iy++) {
+                const float wy = std::min((float)(iy + 1), y1) - std::max((float)iy, y0);
+                for (int ix = ix0; ix < ix1; ix++) {
+                    const float wx = std::min((float)(ix + 1), x1) - std::max((float)ix, x0);
+                    const size_t si = (size_t)iy * sw + ix;
+                    const float w = wx * wy;
+                    for (int c = 0; c < 3; c++) {
+                        const float v = u8 ? (float)u8[4 * si + c] * (1.f / 255.f) : f[3 * si + c];
+                        acc[c] = acc[c] + w * v;
+                    }
+                    area = area + w;
+                }
+            }
+            for (int c = 0; c < 3; c++) out[3 * ((size_t)gy * dw + gx) + c] = acc[c] / area;
+        }
+    }
+}
+
+// msplat_undistort (msplat_cpu.cpp:1641-1669): Brown-Conrady model from the undistorted
+// crop's pixel centres to the source, bilinear sampling with clamp-to-edge.
+extern "C" void orc_undistort(const float *in, int sw, int sh, const float *intr, const float *dist, int roi_x,
+                              int roi_y, float *out, int dw, int dh) {
+    if (dw <= 0 || dh <= 0) return;
+    const float fx = intr[0], fy = intr[1], cx = intr[2], cy = intr[3];
+    const float k1 = dist[0], k2 = dist[1], p1 = dist[2], p2 = dist[3], k3 = dist[4];
+#pragma omp parallel for schedule(static)
+    for (int gy = 0; gy < dh; gy++)
+        for (int gx = 0; gx < dw; gx++) {
+            const float x = ((float)(gx + roi_x) + 0.5f - cx) / fx, y = ((float)(gy + roi_y) + 0.5f - cy) / fy;
+            const float r2 = x * x + y * y;
+            const float radial = 1.f + k1 * r2 + k2 * r2 * r2 + k3 * r2 * r2 * r2;
+            const float xd = x * radial + 2.f * p1 * x * y + p2 * (r2 + 2.f * x * x);
+            const float yd = y * radial + p1 * (r2 + 2.f * y * y) + 2.f * p2 * x * y;
+            const float sx = xd * fx + cx - 0.5f, sy = yd * fy + cy - 0.5f;
+            const int x0 = (int)std::floor(sx), y0 = (int)std::floor(sy);
+            const float ax = sx - std::floor(sx), ay = sy - std::floor(sy);
+            const int xa = std::clamp(x0, 0, sw - 1), xb = std::clamp(x0 + 1, 0, sw - 1);
+            const int ya = std::clamp(y0, 0, sh - 1), yb = std::clamp(y0 + 1, 0, sh - 1);
+            for (int c = 0; c < 3; c++) {
+                const float v00 = in[3 * ((size_t)ya * sw + xa) + c], v10 = in[3 * ((size_t)ya * sw + xb) + c];
+                const float v01 = in[3 * ((size_t)yb * sw + xa) + c], v11 = in[3 * ((size_t)yb * sw + xb) + c];
+                const float top = v00 * (1.f - ax) + v10 * ax, bottom = v01 * (1.f - ax) + v11 * ax;
+                out[3 * ((size_t)gy * dw + gx) + c] = top * (1.f - ay) + bottom * ay;
+            }
+        }
+}
+
 extern "C" uint32_t orc_pcg_hash(uint32_t v) { return pcg(v); }
 extern "C" float orc_rand_uniform(uint32_t seed, uint32_t ctr) { return uniform01(seed, ctr); }
 extern "C" void orc_rand_normal3(uint32_t seed, uint32_t stream, float *out) { normal3(seed, stream, out); }

@@ -96,6 +96,8 @@ class Oracle:
         L.orc_init_gaussians.argtypes = [f32p, u8p, C.c_uint32, C.c_uint32, C.c_float, f32p, f32p, f32p, f32p]
         L.orc_image_metrics.argtypes = [f32p, f32p, C.c_int, C.c_int, f64p]
         L.orc_pack_rgba8.argtypes = [f32p, C.c_uint32, u8p]
+        L.orc_resize_area.argtypes = [vp, C.c_int, C.c_int, C.c_int, f32p, C.c_int, C.c_int]
+        L.orc_undistort.argtypes = [f32p, C.c_int, C.c_int, f32p, f32p, C.c_int, C.c_int, f32p, C.c_int, C.c_int]
         L.orc_pcg_hash.argtypes = [C.c_uint32]
         L.orc_pcg_hash.restype = C.c_uint32
         L.orc_rand_uniform.argtypes = [C.c_uint32, C.c_uint32]
@@ -295,6 +297,19 @@ class Oracle:
         self.lib.orc_pack_rgba8(_c(img, np.float32), n, out)
         return out
 
+    def resize_area(self, src, dw, dh):
+        """src: (sh, sw, 4) uint8 RGBA or (sh, sw, 3) float32 -> (dh, dw, 3) float32."""
+        src = np.ascontiguousarray(src)
+        out = np.zeros((dh, dw, 3), np.float32)
+        self.lib.orc_resize_area(src.ctypes.data, int(src.dtype == np.uint8), src.shape[1], src.shape[0], out, dw, dh)
+        return out
+
+    def undistort(self, src, intr, dist, roi_x, roi_y, dw, dh):
+        out = np.zeros((dh, dw, 3), np.float32)
+        self.lib.orc_undistort(_c(src, np.float32), src.shape[1], src.shape[0], _c(intr, np.float32),
+                               _c(dist, np.float32), roi_x, roi_y, out, dw, dh)
+        return out
+
     def pcg_hash(self, v):
         return self.lib.orc_pcg_hash(v)
 
@@ -338,6 +353,8 @@ class Reference:
         L.ref_init_gaussians.argtypes = [f32p, u8p, C.c_uint32, C.c_uint32, C.c_float, f32p, f32p, f32p, f32p]
         L.ref_image_metrics.argtypes = [f32p, f32p, C.c_int, C.c_int, f64p]
         L.ref_pack_rgba8.argtypes = [f32p, C.c_uint32, u8p]
+        L.ref_resize_area.argtypes = [vp, C.c_int, C.c_int, C.c_int, f32p, C.c_int, C.c_int]
+        L.ref_undistort.argtypes = [f32p, C.c_int, C.c_int, f32p, f32p, C.c_int, C.c_int, f32p, C.c_int, C.c_int]
 
     def render(self, scene, cam, bg):
         N = scene["means"].shape[0]
@@ -412,6 +429,54 @@ class Reference:
         out = np.zeros((img.shape[0], img.shape[1], 4), np.uint8)
         self.lib.ref_pack_rgba8(_c(img, np.float32), n, out)
         return out
+
+
+def _ref_resize_area(self, src, dw, dh):
+    src = np.ascontiguousarray(src)
+    out = np.zeros((dh, dw, 3), np.float32)
+    self.lib.ref_resize_area(src.ctypes.data, int(src.dtype == np.uint8), src.shape[1], src.shape[0], out, dw, dh)
+    return out
+
+
+def _ref_undistort(self, src, intr, dist, roi_x, roi_y, dw, dh):
+    out = np.zeros((dh, dw, 3), np.float32)
+    self.lib.ref_undistort(_c(src, np.float32), src.shape[1], src.shape[0], _c(intr, np.float32),
+                          _c(dist, np.float32), roi_x, roi_y, out, dw, dh)
+    return out
+
+
+def _ref_save(self, scene, path, step=0, scale=1.0, translation=(0.0, 0.0, 0.0), keep_crs=False, splat=False):
+    """saveGaussianPly (or saveGaussianSplat) of the reference on a host scene."""
+    n = scene["means"].shape[0]
+    rb = sh_bases(scene["degree"]) - 1
+    arrs = [_c(scene[k], np.float32) for k in ("means", "scales", "quats", "dc")]
+    arrs.append(_c(np.asarray(scene["rest"], np.float32).reshape(n, rb, 3), np.float32))
+    arrs.append(_c(scene["opac"], np.float32))
+    self.lib.ref_save_ply.argtypes = [C.c_int, vp, C.c_int, C.c_char_p, C.c_int, C.c_float, f32p, C.c_int, C.c_int]
+    if self.lib.ref_save_ply(n, _ptrs(arrs), rb, path.encode(), step, scale, np.asarray(translation, np.float32),
+                             int(keep_crs), int(splat)):
+        raise RuntimeError("reference save failed")
+
+
+def _ref_load_ply(self, path, n, rest_bases, scale=1.0, translation=(0.0, 0.0, 0.0), keep_crs=False):
+    out = dict(means=np.zeros((n, 3), np.float32), scales=np.zeros((n, 3), np.float32),
+               quats=np.zeros((n, 4), np.float32), dc=np.zeros((n, 3), np.float32),
+               rest=np.zeros((n, rest_bases, 3), np.float32), opac=np.zeros((n, 1), np.float32))
+    step = C.c_int(0)
+    self.lib.ref_load_ply.argtypes = [C.c_char_p, C.c_float, f32p, C.c_int, C.c_int, C.c_int, vp, vp]
+    rc = self.lib.ref_load_ply(path.encode(), scale, np.asarray(translation, np.float32), int(keep_crs), n, rest_bases,
+                               _ptrs([out[k] for k in ("means", "scales", "quats", "dc", "rest", "opac")]),
+                               C.byref(step))
+    if rc:
+        raise RuntimeError("reference load failed")
+    out["step"] = step.value
+    return out
+
+
+Reference.save = _ref_save
+Reference.load_ply = _ref_load_ply
+Reference.resize_area = _ref_resize_area
+Reference.undistort = _ref_undistort
 
 
 def host_isa() -> str:

@@ -4,6 +4,7 @@
 // into oracle/_ref/ where /root/reference exists, used to write the golden vectors and
 // as the CPU baseline; never part of the product.
 #include "bindings.h"
+#include "loaders.hpp"
 
 #include <cstdio>
 #include <cstring>
@@ -183,4 +184,56 @@ extern "C" void ref_image_metrics(const float *a, const float *b, int h, int w, 
 extern "C" void ref_pack_rgba8(const float *img, uint32_t n, uint8_t *out) {
     MTensor t = upload(img, {(int64_t)n, 3}, DType::Float32);
     msplat_pack_rgba8(t, n, out);
+}
+
+extern "C" void ref_resize_area(const void *src, int src_is_u8, int sw, int sh, float *out, int dw, int dh) {
+    MTensor t = src_is_u8 ? upload(src, {sh, sw, 4}, DType::UInt8) : upload(src, {sh, sw, 3}, DType::Float32);
+    MTensor d = upload(nullptr, {dh, dw, 3}, DType::Float32);
+    msplat_resize_area(t, src_is_u8 != 0, sw, sh, d, dw, dh);
+    msplat_gpu_sync();
+    download(out, d);
+}
+
+extern "C" void ref_undistort(const float *src, int sw, int sh, const float *intr, const float *dist, int roi_x,
+                              int roi_y, float *out, int dw, int dh) {
+    MTensor t = upload(src, {sh, sw, 3}, DType::Float32);
+    MTensor d = upload(nullptr, {dh, dw, 3}, DType::Float32);
+    msplat_undistort(t, sw, sh, intr, dist, roi_x, roi_y, d, dw, dh);
+    msplat_gpu_sync();
+    download(out, d);
+}
+
+// saveGaussianPly / saveGaussianSplat / loadGaussianPly (core/src/loaders/save_gaussians.cpp)
+// on host arrays: params[6] = means, scales, quats, dc, rest (n, rest_bases, 3), opacity.
+extern "C" int ref_save_ply(int n, float **params, int rest_bases, const char *path, int step, float scale,
+                            const float *translation, int keep_crs, int splat) {
+    try {
+        MTensor m = upload(params[0], {n, 3}, DType::Float32), s = upload(params[1], {n, 3}, DType::Float32),
+                q = upload(params[2], {n, 4}, DType::Float32), d = upload(params[3], {n, 3}, DType::Float32),
+                r = upload(params[4], {n, rest_bases, 3}, DType::Float32), o = upload(params[5], {n, 1}, DType::Float32);
+        GaussianParams p{m, s, q, d, r, o, scale, {translation[0], translation[1], translation[2]}, keep_crs != 0};
+        if (splat) saveGaussianSplat(path, p);
+        else saveGaussianPly(path, p, step);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+extern "C" int ref_load_ply(const char *path, float scale, const float *translation, int keep_crs, int n,
+                            int rest_bases, float **params, int *step) {
+    try {
+        LoadedGaussians g = loadGaussianPly(path, scale, translation, keep_crs != 0);
+        if (g.means.size(0) != n || g.featuresRest.size(1) != rest_bases) return -2;
+        download(params[0], g.means);
+        download(params[1], g.scales);
+        download(params[2], g.quats);
+        download(params[3], g.featuresDc);
+        download(params[4], g.featuresRest);
+        download(params[5], g.opacities);
+        *step = g.step;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
 }

@@ -23,7 +23,8 @@ from msplat.camera import look_at, make_camera  # noqa: E402
 from msplat.scenes import random_scene  # noqa: E402
 
 sys.path.insert(0, os.path.dirname(HERE))
-from workloads import C1_SAMPLE, C1_STEPS, c1_camera_order, c1_workload  # noqa: E402
+from workloads import (C1_SAMPLE, C1_STEPS, RESIZE_CASES, UNDISTORT_CASES, c1_camera_order,  # noqa: E402
+                       c1_workload)
 
 BG = np.array([0.613, 0.0101, 0.3984], np.float32)
 KEYS = ("means", "scales", "quats", "dc", "rest", "opac")
@@ -97,7 +98,59 @@ def trajectory_c1_vectors(ref):
     return d
 
 
-EXTRA = {"opacity_reset": opacity_reset_vectors, "trajectory_c1": trajectory_c1_vectors}
+def image_pipeline_vectors(ref):
+    """msplat_resize_area (cpu:1609-1639) on RGBA8 and float sources, down, up and at
+    equal size; msplat_undistort (cpu:1641-1669) with barrel and pincushion distortion,
+    crops inside and outside the source, and no distortion."""
+    g = np.random.default_rng(41)
+    d = {}
+    for i, (kind, sw, sh, dw, dh) in enumerate(RESIZE_CASES):
+        src = (g.integers(0, 256, (sh, sw, 4), dtype=np.uint8) if kind == "u8"
+               else g.random((sh, sw, 3), dtype=np.float32))
+        d[f"resize_src{i}"] = src
+        d[f"resize_out{i}"] = ref.resize_area(src, dw, dh)
+    src = g.random((60, 80, 3), dtype=np.float32)
+    d["undistort_src"] = src
+    for i, (intr, dist, rx, ry, dw, dh) in enumerate(UNDISTORT_CASES):
+        d[f"undistort_out{i}"] = ref.undistort(src, np.array(intr, np.float32), np.array(dist, np.float32), rx, ry,
+                                               dw, dh)
+    return d
+
+
+def scene_file_vectors(ref):
+    """saveGaussianPly, saveGaussianSplat and loadGaussianPly (save_gaussians.cpp:10-191)
+    of the reference on a 120-Gaussian SH degree 3 scene with non-unit quaternions and
+    two equal splat sort weights, plain and with the dataset transform (scale 2.5,
+    translation (10, -3, 1.5)), and a degree 0 scene: the written files' bytes, and what
+    the reader makes of the transformed PLY."""
+    import tempfile
+    sc = random_scene(51, 120, degree=3, log_scale=(-3.0, 0.6))
+    sc["quats"] = (sc["quats"] * np.random.default_rng(52).uniform(0.5, 1.5, (120, 1))).astype(np.float32)
+    sc["scales"][7], sc["opac"][7] = sc["scales"][3], sc["opac"][3]  # a tie in the .splat order
+    sc0 = random_scene(53, 40, degree=0, log_scale=(-3.0, 0.6))
+    sc0["rest"] = np.zeros((40, 0, 3), np.float32)
+    d = {k: sc[k] for k in KEYS}
+    d.update({"deg0_" + k: sc0[k] for k in KEYS})
+    tr = (10.0, -3.0, 1.5)
+    with tempfile.TemporaryDirectory() as tmp:
+        def written(name, scene, **kw):
+            path = os.path.join(tmp, name)
+            ref.save(scene, path, **kw)
+            d[name.replace(".", "_")] = np.frombuffer(open(path, "rb").read(), np.uint8)
+            return path
+        written("plain.ply", sc, step=7000)
+        crs = written("crs.ply", sc, step=30000, scale=2.5, translation=tr, keep_crs=True)
+        written("plain.splat", sc, splat=True)
+        written("crs.splat", sc, scale=2.5, translation=tr, keep_crs=True, splat=True)
+        written("deg0.ply", sc0, step=5)
+        back = ref.load_ply(crs, 120, 15, scale=2.5, translation=tr, keep_crs=True)
+    d.update({"loaded_" + k: back[k] for k in KEYS})
+    d["loaded_step"] = np.int32(back["step"])
+    return d
+
+
+EXTRA = {"opacity_reset": opacity_reset_vectors, "trajectory_c1": trajectory_c1_vectors,
+         "image_pipeline": image_pipeline_vectors, "scene_files": scene_file_vectors}
 
 
 def main(only=()):

@@ -271,3 +271,34 @@ def test_render_double_agrees_with_float(oracle):
     _, _, f64 = oracle.render(sc, cam, bg, dtype=np.float64)
     err = np.abs(f32["img"] - f64["img"])
     assert np.quantile(err, 0.999) < 1e-5
+
+
+def test_image_pipeline_matches_reference(oracle):
+    """msplat_resize_area and msplat_undistort: the oracle against the compiled
+    reference's outputs, bit for bit."""
+    from workloads import RESIZE_CASES, UNDISTORT_CASES
+    d = golden("image_pipeline")
+    for i, (_, sw, sh, dw, dh) in enumerate(RESIZE_CASES):
+        np.testing.assert_array_equal(oracle.resize_area(d[f"resize_src{i}"], dw, dh), d[f"resize_out{i}"])
+    for i, (intr, dist, rx, ry, dw, dh) in enumerate(UNDISTORT_CASES):
+        got = oracle.undistort(d["undistort_src"], np.array(intr, np.float32), np.array(dist, np.float32), rx, ry,
+                               dw, dh)
+        np.testing.assert_array_equal(got, d[f"undistort_out{i}"])
+
+
+def test_image_pipeline_closed_forms(oracle):
+    """Resizing by whole factors is the block mean; equal sizes convert (u8 / 255); a
+    constant image stays constant at any size; undistortion without distortion at roi 0
+    reproduces the image (bilinear samples at the source pixel centres)."""
+    g = np.random.default_rng(3)
+    img = g.random((24, 36, 3), dtype=np.float32)
+    block = img.reshape(8, 3, 12, 3, 3).astype(np.float64).mean(axis=(1, 3))
+    np.testing.assert_allclose(oracle.resize_area(img, 12, 8), block, rtol=2e-6, atol=2e-7)
+    u8 = g.integers(0, 256, (9, 7, 4), dtype=np.uint8)
+    np.testing.assert_array_equal(oracle.resize_area(u8, 7, 9), u8[..., :3].astype(np.float32) * np.float32(1 / 255))
+    const = np.full((13, 17, 3), 0.375, np.float32)
+    for dw, dh in ((5, 4), (17, 13), (40, 29)):
+        np.testing.assert_allclose(oracle.resize_area(const, dw, dh), 0.375, rtol=1e-6)
+    intr = np.array([50.0, 50.0, 18.0, 12.0], np.float32)
+    out = oracle.undistort(img, intr, np.zeros(5, np.float32), 0, 0, 36, 24)
+    np.testing.assert_allclose(out, img, rtol=0, atol=1e-6)

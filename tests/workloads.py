@@ -27,3 +27,11 @@ def c1_camera_order(steps, count=8, seed=42):
     while len(order) < steps:
         order.extend(int(i) for i in rng.permutation(count))
     return np.array(order[:steps], np.int32)
+
+
+# image-pipeline fixtures: (source kind, sw, sh, dw, dh); (intrinsics, distortion, roi x, roi y, dw, dh)
+RESIZE_CASES = [("u8", 97, 61, 40, 25), ("u8", 40, 30, 40, 30), ("u8", 31, 17, 62, 40), ("f32", 64, 48, 21, 13),
+                ("f32", 33, 20, 33, 20), ("f32", 1, 1, 3, 2), ("f32", 96, 64, 48, 32)]
+UNDISTORT_CASES = [((70.0, 72.0, 40.3, 29.7), (0.1, -0.05, 0.001, -0.002, 0.01), 3, 2, 70, 50),
+                   ((70.0, 72.0, 40.3, 29.7), (-0.3, 0.12, -0.004, 0.003, -0.02), -4, -3, 90, 70),
+                   ((60.0, 60.0, 40.0, 30.0), (0.0, 0.0, 0.0, 0.0, 0.0), 0, 0, 80, 60)]
