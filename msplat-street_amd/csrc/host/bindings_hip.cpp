@@ -4,15 +4,24 @@
 // an MI355X unchanged.
 //
 // Memory: the reference's MTensor on non-Apple builds is a host block that callers read
-// and write through data_ptr() (core/include/metal_tensor.hpp:86-114). This adapter keeps
-// a device mirror per host block (keyed by its base address) and moves exactly the bytes
-// each call reads and writes: inputs go up before the launch, written tensors come back
-// after a sync. That keeps the reference's host-visible contract; the device-resident
-// path (parameters and moments never leaving HBM) is the C ABI used directly, as
-// msplat-street_amd/msplat/trainer.py does.
+// and write through data_ptr() (core/include/metal_tensor.hpp:86-114), and results are
+// visible there after msplat_gpu_sync (bindings.h:25). This adapter keeps a device mirror
+// per host block (keyed by its base address):
+//  - a tensor a call reads is uploaded before the launch, unless the device holds a
+//    newer copy (a backend call wrote it and no sync has come since): then the device
+//    copy is used as it is;
+//  - a tensor a call writes is marked device-newer and stays on the device;
+//  - msplat_gpu_sync downloads every device-newer tensor, after which host and device
+//    agree and the host is authoritative again.
+// So a training loop that syncs only when it reads (as Model and python/bindings.cpp
+// do) keeps parameters, moments and statistics in HBM between syncs; per step only the
+// read-only inputs (target image, matrices) cross PCIe. msplat_copy_buffer copies on the
+// device when the source is device-newer. gpu_empty / gpu_zeros drop the mirror of a
+// reused block address, and mirrors unused for many calls are freed once they exceed a
+// byte budget.
 //
 // Out of scope here (they throw): the prior-guided extras (PriorStep with anything
-// enabled), the GT image pipeline (resize/undistort), aux depth outputs and GPU capture.
+// enabled), aux depth outputs and GPU capture.
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -29,14 +38,19 @@ namespace {
 struct Mirror {
     void *dev = nullptr;
     size_t bytes = 0;
+    bool dev_newer = false;  // written on the device since the last sync
+    uint64_t last_use = 0;
 };
 
 struct Backend {
     msplat_hip_ctx *ctx = nullptr;
     std::unordered_map<const void *, Mirror> mirrors;
+    size_t mirror_bytes = 0;
+    uint64_t clock = 0;
     MTensor out_img, radii;
     bool timing = false;
     std::mutex mu;
+    static constexpr size_t kBudget = size_t(16) << 30;  // idle mirrors are freed above this
 
     msplat_hip_ctx *get() {
         if (!ctx) check(msplat_hip_ctx_create(0, nullptr, &ctx), "context");
@@ -45,29 +59,76 @@ struct Backend {
     static void check(int rc, const char *what) {
         if (rc != MSPLAT_OK) throw std::runtime_error(std::string("msplat HIP (") + what + "): " + msplat_hip_last_error());
     }
-    void *dev(const MTensor &t, size_t bytes) {
+    // The block's mirror with at least `bytes`; a grown mirror keeps its device-newer bytes.
+    Mirror &mirror(const MTensor &t, size_t bytes) {
         const void *key = t.data_ptr();
         Mirror &m = mirrors[key];
+        m.last_use = ++clock;
         if (m.bytes < bytes) {
-            if (m.dev) check(msplat_hip_free(m.dev), "free");
-            check(msplat_hip_malloc(bytes, 1, &m.dev), "malloc");
+            void *d = nullptr;
+            check(msplat_hip_malloc(bytes, 1, &d), "malloc");
+            if (m.dev) {
+                if (m.dev_newer) check(msplat_hip_memcpy(get(), d, m.dev, m.bytes, 2), "grow");
+                check(msplat_hip_sync(get()), "sync");
+                check(msplat_hip_free(m.dev), "free");
+                mirror_bytes -= m.bytes;
+            }
+            if (m.dev_newer && bytes > m.bytes)  // the part the device never held comes from the host
+                check(msplat_hip_memcpy(get(), (char *)d + m.bytes, (const char *)key + m.bytes, bytes - m.bytes, 0),
+                      "upload");
+            m.dev = d;
             m.bytes = bytes;
+            mirror_bytes += bytes;
         }
-        return m.dev;
+        return m;
     }
+    void *dev(const MTensor &t, size_t bytes) { return mirror(t, bytes).dev; }
+    // An input: its device copy, uploaded unless the device copy is newer.
     float *up(const MTensor &t) {
-        void *d = dev(t, t.nbytes());
-        check(msplat_hip_memcpy(get(), d, t.data_ptr(), t.nbytes(), 0), "upload");
-        return (float *)d;
+        Mirror &m = mirror(t, t.nbytes());
+        if (!m.dev_newer && t.nbytes()) check(msplat_hip_memcpy(get(), m.dev, t.data_ptr(), t.nbytes(), 0), "upload");
+        return (float *)m.dev;
     }
-    void down(MTensor &t) {
-        void *d = dev(t, t.nbytes());
-        check(msplat_hip_memcpy(get(), t.data_ptr(), d, t.nbytes(), 1), "download");
+    // Written on the device by the call just enqueued: host reads wait for msplat_gpu_sync.
+    void wrote(const MTensor &t) {
+        Mirror &m = mirror(t, t.nbytes());
+        m.dev_newer = true;
+    }
+    void sync() {
+        check(msplat_hip_sync(get()), "sync");
+        for (auto &kv : mirrors)
+            if (kv.second.dev_newer) {
+                check(msplat_hip_memcpy(get(), (void *)kv.first, kv.second.dev, kv.second.bytes, 1), "download");
+                kv.second.dev_newer = false;
+            }
+    }
+    // A freshly allocated block at `key` has no mirror yet (a freed block's address reused).
+    void forget(const void *key) {
+        auto it = mirrors.find(key);
+        if (it == mirrors.end()) return;
+        if (it->second.dev) {
+            check(msplat_hip_sync(get()), "sync");
+            check(msplat_hip_free(it->second.dev), "free");
+        }
+        mirror_bytes -= it->second.bytes;
+        mirrors.erase(it);
+    }
+    // Frees the least recently used clean mirrors while their total exceeds the budget.
+    void trim() {
+        while (mirror_bytes > kBudget) {
+            auto victim = mirrors.end();
+            for (auto it = mirrors.begin(); it != mirrors.end(); ++it)
+                if (!it->second.dev_newer && (victim == mirrors.end() || it->second.last_use < victim->second.last_use))
+                    victim = it;
+            if (victim == mirrors.end()) return;
+            forget(victim->first);
+        }
     }
     void release() {
         for (auto &kv : mirrors)
             if (kv.second.dev) msplat_hip_free(kv.second.dev);
         mirrors.clear();
+        mirror_bytes = 0;
         if (ctx) msplat_hip_ctx_destroy(ctx);
         ctx = nullptr;
         out_img = MTensor();
@@ -121,11 +182,20 @@ void cleanup_msplat_metal() {
 
 void *msplat_device() { return g.get(); }
 
-MTensor gpu_zeros(std::vector<int64_t> shape, DType dtype) { return MTensor(MTensor::HostDevice{}, std::move(shape), dtype); }
-MTensor gpu_empty(std::vector<int64_t> shape, DType dtype) { return MTensor(MTensor::HostDevice{}, std::move(shape), dtype); }
+MTensor gpu_zeros(std::vector<int64_t> shape, DType dtype) {
+    MTensor t(MTensor::HostDevice{}, std::move(shape), dtype);
+    std::lock_guard<std::mutex> lk(g.mu);
+    g.forget(t.data_ptr());
+    return t;
+}
+MTensor gpu_empty(std::vector<int64_t> shape, DType dtype) { return gpu_zeros(std::move(shape), dtype); }
 
 void msplat_commit() { Backend::check(msplat_hip_commit(g.get()), "commit"); }
-void msplat_gpu_sync() { Backend::check(msplat_hip_sync(g.get()), "sync"); }
+void msplat_gpu_sync() {
+    std::lock_guard<std::mutex> lk(g.mu);
+    g.sync();
+    g.trim();
+}
 
 bool msplat_begin_gpu_capture(const char *) {
     std::fprintf(stderr, "msplat: GPU capture is a Metal feature; use rocprofv3 on the HIP backend\n");
@@ -176,10 +246,13 @@ MTensor msplat_render(int num_points, MTensor &means3d, MTensor &scales, float g
                                  glob_scale, degree, degrees_to_use, cam_pos);
     float *img = nullptr;
     Backend::check(msplat_hip_render(g.get(), &gs, &c, background.data<float>(), &img), "render");
-    if (!g.out_img.defined() || g.out_img.size(0) != (int64_t)img_height || g.out_img.size(1) != (int64_t)img_width)
-        g.out_img = gpu_zeros({(int64_t)img_height, (int64_t)img_width, 3}, DType::Float32);
-    Backend::check(msplat_hip_memcpy(g.get(), g.out_img.data_ptr(), img, g.out_img.nbytes(), 1), "image");
-    return g.out_img;  // aliases the backend's cache until the next call (bindings.h:93-96)
+    if (!g.out_img.defined() || g.out_img.size(0) != (int64_t)img_height || g.out_img.size(1) != (int64_t)img_width) {
+        g.out_img = MTensor(MTensor::HostDevice{}, {(int64_t)img_height, (int64_t)img_width, 3}, DType::Float32);
+        g.forget(g.out_img.data_ptr());
+    }
+    Backend::check(msplat_hip_memcpy(g.get(), g.dev(g.out_img, g.out_img.nbytes()), img, g.out_img.nbytes(), 2), "image");
+    g.wrote(g.out_img);
+    return g.out_img;  // aliases the backend's cache until the next call; read after msplat_gpu_sync
 }
 
 void msplat_render_aux_outputs(MTensor &, MTensor &) {
@@ -218,7 +291,10 @@ std::tuple<MTensor, float> msplat_train_step(
     a.beta2 = adam_beta2;
     a.eps = adam_eps;
     msplat_hip_grad_stats st{g.up(vis_counts), g.up(xys_grad_norm), g.up(max_2d_size)};
-    if (!g.radii.defined() || g.radii.numel() != num_points) g.radii = gpu_zeros({std::max(num_points, 1)}, DType::Int32);
+    if (!g.radii.defined() || g.radii.numel() != std::max(num_points, 1)) {
+        g.radii = MTensor(MTensor::HostDevice{}, {std::max(num_points, 1)}, DType::Int32);
+        g.forget(g.radii.data_ptr());
+    }
     int32_t *d_radii = (int32_t *)g.dev(g.radii, g.radii.nbytes());
     float loss = 0.f;
     Backend::check(msplat_hip_train_step(g.get(), &gs, &c, background.data<float>(), g.up(gt), ssim_weight,
@@ -227,10 +303,11 @@ std::tuple<MTensor, float> msplat_train_step(
     MTensor *written[] = {&means3d, &scales, &quats, &features_dc, &features_rest, &opacities,
                           &vis_counts, &xys_grad_norm, &max_2d_size, &g.radii};
     for (MTensor *t : written)
-        if (t->numel()) g.down(*t);
+        if (t->numel()) g.wrote(*t);
+    g.trim();
     for (int i = 0; i < 6; i++) {
-        g.down(adam_exp_avg[i]);
-        g.down(adam_exp_avg_sq[i]);
+        g.wrote(adam_exp_avg[i]);
+        g.wrote(adam_exp_avg_sq[i]);
     }
     return std::make_tuple(g.radii, loss);
 }
@@ -256,7 +333,7 @@ int msplat_densify(int N, int buf_capacity, float grad_thresh, float size_thresh
                                       g.up(xys_grad_norm), g.up(vis_counts), g.up(max_2d_size), half_max_dim, d,
                                       fr_stride, seed, &out),
                    "densify");
-    for (int i = 0; i < 18; i++) g.down(*b[i]);
+    for (int i = 0; i < 18; i++) g.wrote(*b[i]);
     return out;
 }
 
@@ -265,13 +342,25 @@ void msplat_opacity_reset(MTensor &opacities, MTensor &exp_avg, MTensor &exp_avg
     std::lock_guard<std::mutex> lk(g.mu);
     float *o = g.up(opacities), *m1 = g.up(exp_avg), *m2 = g.up(exp_avg_sq);
     Backend::check(msplat_hip_opacity_reset(g.get(), o, m1, m2, num_points, reset_logit), "opacity_reset");
-    g.down(opacities);
-    g.down(exp_avg);
-    g.down(exp_avg_sq);
+    g.wrote(opacities);
+    g.wrote(exp_avg);
+    g.wrote(exp_avg_sq);
 }
 
 void msplat_copy_buffer(MTensor &dst, const MTensor &src, size_t bytes) {
-    if (bytes) std::memcpy(dst.data_ptr(), src.data_ptr(), bytes);  // host blocks: a plain copy
+    if (!bytes) return;
+    std::lock_guard<std::mutex> lk(g.mu);
+    auto it = g.mirrors.find(src.data_ptr());
+    if (it == g.mirrors.end() || !it->second.dev_newer) {  // the host holds the source: a plain copy
+        if (g.mirrors.count(dst.data_ptr()) && g.mirrors[dst.data_ptr()].dev_newer) g.sync();
+        std::memcpy(dst.data_ptr(), src.data_ptr(), bytes);
+        return;
+    }
+    const void *s = g.mirror(src, bytes).dev;  // covers `bytes` (a view's mirror may hold fewer)
+    float *d = g.up(dst);  // the destination's other bytes, then the source's prefix on the device
+    s = g.mirrors.at(src.data_ptr()).dev;  // looked up again: up() may have rehashed the map
+    Backend::check(msplat_hip_memcpy(g.get(), d, s, bytes, 2), "copy_buffer");
+    g.wrote(dst);
 }
 
 void msplat_radix_sort(MTensor &keys, MTensor &vals, uint32_t n, int key_bits) {
@@ -279,8 +368,8 @@ void msplat_radix_sort(MTensor &keys, MTensor &vals, uint32_t n, int key_bits) {
     uint64_t *k = (uint64_t *)g.up(keys);
     int32_t *v = (int32_t *)g.up(vals);
     Backend::check(msplat_hip_radix_sort(g.get(), k, v, n, key_bits), "radix_sort");
-    g.down(keys);
-    g.down(vals);
+    g.wrote(keys);
+    g.wrote(vals);
 }
 
 void msplat_knn3_mean_dist(MTensor &points, uint32_t n, MTensor &mean_dist) {
@@ -288,7 +377,7 @@ void msplat_knn3_mean_dist(MTensor &points, uint32_t n, MTensor &mean_dist) {
     float *p = g.up(points);
     float *d = (float *)g.dev(mean_dist, mean_dist.nbytes());
     Backend::check(msplat_hip_knn3_mean_dist(g.get(), p, n, d), "knn3");
-    g.down(mean_dist);
+    g.wrote(mean_dist);
 }
 
 void msplat_init_gaussians(MTensor &means, MTensor &rgb, uint32_t n, uint32_t seed, float opacity_logit,
@@ -299,18 +388,31 @@ void msplat_init_gaussians(MTensor &means, MTensor &rgb, uint32_t n, uint32_t se
     float *s = (float *)g.dev(scales, scales.nbytes()), *q = (float *)g.dev(quats, quats.nbytes());
     float *dc = (float *)g.dev(features_dc, features_dc.nbytes()), *o = (float *)g.dev(opacities, opacities.nbytes());
     Backend::check(msplat_hip_init_gaussians(g.get(), m, c, n, seed, opacity_logit, s, q, dc, o), "init");
-    g.down(scales);
-    g.down(quats);
-    g.down(features_dc);
-    g.down(opacities);
+    g.wrote(scales);
+    g.wrote(quats);
+    g.wrote(features_dc);
+    g.wrote(opacities);
 }
 
-void msplat_resize_area(const MTensor &, bool, int, int, MTensor &, int, int) {
-    throw std::invalid_argument("msplat HIP backend: the GT image pipeline (resize) is not part of this backend");
+void msplat_resize_area(const MTensor &src, bool src_is_u8, int sw, int sh, MTensor &dst, int dw, int dh) {
+    if (dw <= 0 || dh <= 0) return;
+    std::lock_guard<std::mutex> lk(g.mu);
+    const void *s = g.up(src);
+    Backend::check(msplat_hip_resize_area(g.get(), s, src_is_u8 ? 1 : 0, sw, sh, (float *)g.dev(dst, dst.nbytes()), dw,
+                                          dh),
+                   "resize_area");
+    g.wrote(dst);
 }
 
-void msplat_undistort(const MTensor &, int, int, const float[4], const float[5], int, int, MTensor &, int, int) {
-    throw std::invalid_argument("msplat HIP backend: the GT image pipeline (undistort) is not part of this backend");
+void msplat_undistort(const MTensor &src, int sw, int sh, const float intr[4], const float dist[5], int roi_x,
+                      int roi_y, MTensor &dst, int dw, int dh) {
+    if (dw <= 0 || dh <= 0) return;
+    std::lock_guard<std::mutex> lk(g.mu);
+    const float *s = g.up(src);
+    Backend::check(msplat_hip_undistort(g.get(), s, sw, sh, intr, dist, roi_x, roi_y,
+                                        (float *)g.dev(dst, dst.nbytes()), dw, dh),
+                   "undistort");
+    g.wrote(dst);
 }
 
 void msplat_image_metrics(const MTensor &rendered, const MTensor &gt, int h, int w, double out[3]) {
