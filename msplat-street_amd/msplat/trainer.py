@@ -56,6 +56,8 @@ class TrainConfig:
     sh_degree: int = 3
     sh_degree_interval: int = 1000
     ssim_weight: float = 0.2
+    num_downscales: int = 0         # the reference's default is 2 (bindings.cpp:494); 0 trains at full size
+    resolution_schedule: int = 3000
     refine_every: int = 100
     warmup_length: int = 500
     reset_alpha_every: int = 30
@@ -86,6 +88,8 @@ class Trainer:
         self.stop_split_at = config.iterations // 2
         self.densify_log = []
         self._struct_cache = {}
+        self._views = {}
+        self._pyramids = None
         self.stats_defined = False
 
     def next_camera(self) -> int:
@@ -96,13 +100,43 @@ class Trainer:
         self.order_pos += 1
         return int(i)
 
-    def _camera(self, i: int, dtu: int):
-        key = (i, dtu)
+    def _camera(self, i: int, dtu: int, sf: int = 1):
+        """The view of camera i at downscale factor sf (Model::prepareCam, model.cpp:498-546:
+        the pyramid level's size and intrinsics scaled by the per-axis size ratio)."""
+        key = (i, dtu, sf)
         c = self._struct_cache.get(key)
         if c is None:
-            cam = dict(self.cams[i], dtu=dtu)
-            c = self._struct_cache[key] = make_camera_struct(cam, self.dg.degree)
+            cam = self._view(i, sf)
+            c = self._struct_cache[key] = make_camera_struct(dict(cam, dtu=dtu), self.dg.degree)
         return c
+
+    def _view(self, i: int, sf: int) -> dict:
+        cam = self.cams[i]
+        if sf <= 1:
+            return cam
+        v = self._views.get((i, sf))
+        if v is None:
+            from .camera import make_camera
+            v = self._views[(i, sf)] = make_camera(cam["cam_to_world"], cam["W"], cam["H"], cam["fx"], cam["fy"],
+                                                   cam["cx"], cam["cy"], downscale=sf)
+        return v
+
+    def _target(self, i: int, sf: int):
+        """Camera i's target at downscale sf: the pyramid level (Camera::getGPUImage)."""
+        if sf <= 1:
+            return self.targets[i]
+        if self._pyramids is None:
+            self._pyramids = {}
+        pyr = self._pyramids.get(i)
+        if pyr is None:
+            from .images import Pyramid
+            pyr = self._pyramids[i] = Pyramid(self.ctx, self.targets[i])
+        return pyr(sf)
+
+    def downscale_factor(self, step: int) -> int:
+        """Model::getDownscaleFactor (model.cpp:195-197, image_sizing.cpp:129-133)."""
+        from .camera import schedule_downscale
+        return schedule_downscale(step, self.cfg.num_downscales, self.cfg.resolution_schedule)
 
     def step(self, read_loss: bool = False):
         """One iteration: train step, LR schedule, densification. Returns the loss when
@@ -111,9 +145,11 @@ class Trainer:
         self.step_count += 1
         step = self.step_count
         i = self.next_camera()
-        cam = self.cams[i]
+        sf = self.downscale_factor(step)
+        cam = self._view(i, sf)
         dtu = min(step // max(self.cfg.sh_degree_interval, 1), self.cfg.sh_degree, self.dg.degree)
-        c = self._camera(i, dtu)
+        c = self._camera(i, dtu, sf)
+        target = self._target(i, sf)
         self.last_hw = (cam["H"], cam["W"])
         self.adam_step_count += 1
         steps, bc2s = adam_coeffs(self.lr, self.adam_step_count)
@@ -126,7 +162,7 @@ class Trainer:
             self.stats_defined = True
         loss = C.c_float(0.0)
         check(lib().msplat_hip_train_step(self.ctx.ptr, C.byref(g), C.byref(c), self.bg,
-                                          C.c_void_p(self.targets[i].ptr), self.cfg.ssim_weight, inv_n, C.byref(a),
+                                          C.c_void_p(target.ptr), self.cfg.ssim_weight, inv_n, C.byref(a),
                                           C.byref(s), inv_max, None,
                                           C.byref(loss) if read_loss else None))
         self.schedulers_step(step)

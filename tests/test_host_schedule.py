@@ -185,3 +185,38 @@ def test_refine_flags_equal_oracle(oracle):
             assert refine_flags(s, cfg, cams, stop) == oracle.refine_flags(
                 s, cfg.refine_every, cfg.warmup_length, cfg.reset_alpha_every, stop, cams,
                 cfg.stop_screen_size_at), s
+
+
+def test_core_surface_names_and_defaults():
+    """msplat._core keeps the reference module's names and TrainingConfig defaults
+    (python/bindings.cpp:494-527); a bad background is rejected as there."""
+    import msplat
+    from msplat import _core
+    cfg = _core.TrainingConfig()
+    want = dict(iterations=30000, sh_degree=3, sh_degree_interval=1000, ssim_weight=0.2, num_downscales=2,
+                resolution_schedule=3000, refine_every=100, warmup_length=500, reset_alpha_every=30,
+                densify_grad_thresh=0.0002, densify_size_thresh=0.01, stop_screen_size_at=4000, split_screen_size=0.05,
+                keep_crs=False, downscale_factor=1.0, output="splat.ply", save_every=-1, depth_weight=0.0,
+                depth_weight_final=-1.0, depth_huber_delta=0.05, depth_min_alpha=0.25, sky_alpha_weight=0.0,
+                fill_weight=0.0, use_masks=False, learn_sky=False, sky_width=512, sky_height=128, sky_lr=0.01,
+                exposure_compensation=False, exposure_lr=5e-3, exposure_reg=1e-2, max_scale_ratio=0.0)
+    for k, v in want.items():
+        assert getattr(cfg, k) == v, k
+    assert cfg.bg_color == [0.6130, 0.0101, 0.3984] and not cfg.priors_enabled()
+    with pytest.raises(ValueError):
+        _core.TrainingConfig(bg_color=[1, 0])
+    for name in ("TrainingConfig", "TrainingStats", "Dataset", "GaussianTrainer", "sync", "cleanup",
+                 "_gpu_radix_sort", "_gpu_knn3_mean_dist", "_gpu_resize_area", "_gpu_undistort", "_gpu_image_metrics",
+                 "_gpu_pack_rgba8"):
+        assert callable(getattr(_core, name)), name
+    assert msplat.TrainingConfig is _core.TrainingConfig
+
+
+def test_undistort_roi_properties():
+    """undistortROI (image_sizing.cpp:33-69): no distortion keeps the image less its last
+    row and column (border pixel centres map to themselves); barrel distortion has
+    nothing to crop beyond that; pincushion crops on every side."""
+    from msplat._core import undistort_roi
+    assert undistort_roi(120, 90, 100, 102, 60, 45, 0, 0, 0, 0, 0) == (0, 0, 119, 89)
+    x, y, w, h = undistort_roi(120, 90, 100, 102, 61, 44, 0.25, 0.05, 0.001, -0.002, 0.01)
+    assert x > 0 and y > 0 and x + w < 120 and y + h < 90
