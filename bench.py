@@ -545,7 +545,7 @@ def cpu_model() -> str:
     return platform.processor() or platform.machine()
 
 
-def cpu_baseline_sample(dg, cams, targets, step_index, seconds=20.0):
+def cpu_baseline_sample(dg, cams, targets, step_index, seconds=20.0, plain=None):
     """Times the reference's CPU train step (core/cpu/msplat_cpu.cpp compiled into
     oracle/_ref, optimised build for the widest x86-64 level the host runs) — or the
     oracle when the reference was not built — on this host's cores (MSPLAT_THREADS = the
@@ -582,11 +582,99 @@ def cpu_baseline_sample(dg, cams, targets, step_index, seconds=20.0):
             break
     per = float(np.median(times))
     build = f"-O3 -march=x86-64-{isa}" if kind == "reference" else "-O2 (oracle)"
-    return {"value": round(1.0 / per, 4), "unit": "iter/s", "cores": threads, "kind": kind,
+    plain_cpu = cpu_plain_run(plain, impl, threads) if plain else None
+    return {"value": round(1.0 / per, 4), "unit": "iter/s", "cores": threads, "kind": kind, "plain": plain_cpu,
             "host_cpus": os.cpu_count(), "cpu_model": cpu_model(), "build": build, "sh_degree_in_use": dtu,
             "sample": f"{len(times)} train step(s) of the timed {n:,}-Gaussian scene on view 0 at {W_IMG}x{H_IMG} "
                       f"(SH degree in use {dtu}, as in the timed steps), median {per:.3f} s/step, "
                       f"MSPLAT_THREADS={threads}, {cpu_model()}"}
+
+
+PLAIN_STEPS = 300
+
+
+def plain_workload(ctx):
+    """The CPU-runnable plain workload: 20,000 Gaussians (means U[-1,1]^3, log-scales
+    N(-4.5, 0.5), uniform rotations, logits N(0,1), SH degree 1), 8 pinhole views at
+    128x128 (fx = fy = 120) on a radius-4 ring, targets rendered from a second seeded
+    20,000-Gaussian set; 300 Adam steps of 0.8 L1 + 0.2 D-SSIM, no densification."""
+    from msplat._abi import DeviceArray
+    from msplat.backend import DeviceGaussians, render_numpy
+    from msplat.scenes import random_scene, ring_cameras
+    sc = random_scene(1000, 20000, degree=1)
+    cams = ring_cameras(8, 4.0, 128, 128, 120.0, 120.0, dtu=1)
+    gt = DeviceGaussians(ctx, random_scene(1001, 20000, degree=1))
+    targets = [DeviceArray.from_numpy(ctx, render_numpy(ctx, gt, c, BG)) for c in cams]
+    return sc, cams, targets
+
+
+def bench_plain(repeats=3):
+    """The plain workload's whole 300-step run on the GPU (best of `repeats` after one
+    warm-up run), with the per-tile rasterizers and with depth chunks chosen per frame
+    (64 tiles per view: every frame is chunked)."""
+    from msplat._abi import TUNE_CHUNKS, Context
+    from msplat.backend import DeviceGaussians
+    from msplat.trainer import TrainConfig, Trainer
+    ctx = Context(0)
+    out = {"workload": "20,000 Gaussians SH1, 8 views at 128x128, 300 Adam steps, no densification"}
+    try:
+        sc, cams, targets = plain_workload(ctx)
+        for name, chunks in (("per_tile", 1), ("depth_chunked", 0)):
+            ctx.set_tuning(TUNE_CHUNKS, chunks)
+            best, loss = None, None
+            for r in range(repeats + 1):
+                dg = DeviceGaussians(ctx, sc)
+                cfg = TrainConfig(iterations=PLAIN_STEPS, sh_degree=1, sh_degree_interval=1, densify=False, seed=42)
+                tr = Trainer(ctx, dg, cams, targets, cfg)
+                ctx.sync()
+                t0 = time.perf_counter()
+                for _ in range(PLAIN_STEPS - 1):
+                    tr.step()
+                loss = tr.step(read_loss=True)
+                el = time.perf_counter() - t0
+                if r:
+                    best = el if best is None else min(best, el)
+            out[name] = {"iter_per_s": round(PLAIN_STEPS / best, 1), "ms_per_step": round(1e3 * best / PLAIN_STEPS, 4),
+                         "run_s": round(best, 4), "final_loss": round(float(loss), 6),
+                         "depth_chunks": ctx.frame().depth_chunks}
+        out["targets"] = [t.numpy() for t in targets]
+    finally:
+        ctx.close()
+    return out
+
+
+def cpu_plain_run(plain, impl, threads):
+    """The plain workload's whole 300-step run through the reference's CPU train step
+    (the same schedule: Adam bias corrections per step, means LR decayed over the run),
+    on the targets the GPU leg rendered."""
+    from msplat.backend import adam_coeffs
+    from msplat.scenes import random_scene, ring_cameras
+    from msplat.trainer import LR_INIT, means_lr
+    sc = random_scene(1000, 20000, degree=1)
+    cams = ring_cameras(8, 4.0, 128, 128, 120.0, 120.0, dtu=1)
+    keys = ("means", "scales", "quats", "dc", "rest", "opac")
+    st = {k: np.ascontiguousarray(sc[k], np.float32).copy() for k in keys}
+    for k in keys:
+        st["m1_" + k] = np.zeros_like(st[k])
+        st["m2_" + k] = np.zeros_like(st[k])
+    n = st["means"].shape[0]
+    st.update(degree=1, vis=np.zeros(n, np.float32), gn=np.zeros(n, np.float32), ms=np.zeros(n, np.float32))
+    rng = np.random.default_rng(42)
+    order = []
+    while len(order) < PLAIN_STEPS:
+        order.extend(int(i) for i in rng.permutation(len(cams)))
+    lrs, times, loss = list(LR_INIT), [], None
+    for t in range(1, PLAIN_STEPS + 1):
+        steps, bc2 = adam_coeffs(lrs, t)
+        i = order[t - 1]
+        t0 = time.perf_counter()
+        loss = impl.train_step(st, cams[i], np.array(BG, np.float32), plain["targets"][i], steps, bc2)[0]
+        times.append(time.perf_counter() - t0)
+        lrs[0] = means_lr(t, PLAIN_STEPS)
+    total = float(np.sum(times))
+    return {"value": round(PLAIN_STEPS / total, 2), "unit": "iter/s", "cores": threads, "run_s": round(total, 3),
+            "median_step_ms": round(1e3 * float(np.median(times)), 3), "final_loss": round(float(loss), 6),
+            "sample": f"the whole {PLAIN_STEPS}-step run"}
 
 
 def time_sh3_steps(ctx, dg, cams, targets, steps=100, warmup=10):
@@ -628,6 +716,7 @@ def main():
     ap.add_argument("--stage-steps", type=int, default=200,
                     help="steps after the timed ones that time every stage for the breakdown")
     ap.add_argument("--no-sh3", action="store_true", help="skip timing steps with SH degree 3 in use")
+    ap.add_argument("--no-plain", action="store_true", help="skip the 300-step plain workload (128x128 views)")
     ap.add_argument("--summarize-profile", nargs="+", metavar="DIR",
                     help="summarise rocprofv3 output directories of this benchmark (kernel trace with the "
                          "'timed_steps' range, --pmc runs) as JSON and exit; --warmup/--steps give the window")
@@ -786,10 +875,18 @@ def main():
     if not args.no_serving:
         log("serving: 4 resident scenes, Poisson arrivals")
         serving = bench_serving(duration=args.serving_seconds)
+    plain = None
+    if not args.no_plain:
+        log("plain workload: 20k gaussians, 8 views at 128x128, 300 steps")
+        plain = bench_plain()
+        log(f"plain: {plain['per_tile']['iter_per_s']} iter/s per tile, "
+            f"{plain['depth_chunked']['iter_per_s']} iter/s depth-chunked")
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU baseline")
-        cpu = cpu_baseline_sample(dg, cams, targets, first_timed_step, args.cpu_seconds)
+        cpu = cpu_baseline_sample(dg, cams, targets, first_timed_step, args.cpu_seconds, plain)
+    if plain:
+        del plain["targets"]
     out = {
         "metric": METRIC, "value": round(iters, 3), "unit": "iter/s", "n_gpus": 1, "steps": K,
         "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / K, 4), "higher_is_better": True,
@@ -807,6 +904,7 @@ def main():
         "roofline": roofline,
         "batched": batched,
         "serving": serving,
+        "plain": plain,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -161,6 +161,7 @@ typedef struct {
     double *loss_sum;
     uint64_t *slots_written;
     int64_t num_visible; /* gaussians with a non-empty tile rectangle (radius > 0) */
+    int32_t depth_chunks; /* depth chunks per tile of the last one-view forward (1: none) */
 } msplat_hip_frame;
 int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *out);
 /* KEEP_GRADS: fill the reduced gradients; KEYS64: materialise isect_keys on frame_view
@@ -183,6 +184,15 @@ int msplat_hip_set_flags(msplat_hip_ctx *ctx, int flags);
  *                          the one-view tile sort moves (tile << id bits | id) keys alone
  *                          instead of tile keys plus id values (same order, half the
  *                          bytes); 0: always keys plus values.
+ *   MSPLAT_TUNE_CHUNKS     depth chunks per tile of the one-view rasterizers (the depth-
+ *                          chunked form of msplat_metal.mm:782-907 for views with few
+ *                          tiles): 0 (default) picks per frame (fewer than 400 tiles:
+ *                          chunks of at least 16 intersections on average, up to about
+ *                          16384 workgroups, 2 to 64 chunks; else none), 1 never chunks,
+ *                          2..64 fixes the count. Chunked frames equal the one-workgroup-
+ *                          per-tile result up to float rounding of the transmittance
+ *                          products and colour sums, not bit for bit. The batched
+ *                          rasterizers do not chunk.
  * The binning tile stays 16x16: it decides which gaussians reach a pixel. */
 enum {
     MSPLAT_TUNE_BWD_PPT = 0,
@@ -190,7 +200,8 @@ enum {
     MSPLAT_TUNE_BWD_BATCH = 2,
     MSPLAT_TUNE_CULL = 3,
     MSPLAT_TUNE_PACK_KEYS = 4,
-    MSPLAT_NUM_TUNABLES = 5
+    MSPLAT_TUNE_CHUNKS = 5,
+    MSPLAT_NUM_TUNABLES = 6
 };
 int msplat_hip_set_tuning(msplat_hip_ctx *ctx, int key, int value);
 int msplat_hip_get_tuning(msplat_hip_ctx *ctx, int key, int *value);

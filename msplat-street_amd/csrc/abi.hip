@@ -259,6 +259,10 @@ RasterArgs raster_args(msplat_hip_ctx *ctx) {
     r.slot_c = (float *)ctx->slot_c.p;
     r.slot_f = (uint8_t *)ctx->slot_f.p;
     r.slots_written = nullptr;
+    r.chunks = ctx->frame_chunks;
+    r.chunk_prod = (float *)ctx->chunk_prod.p;
+    r.chunk_state = (float4 *)ctx->chunk_state.p;
+    r.chunk_last = (int32_t *)ctx->chunk_last.p;
     return r;
 }
 
@@ -267,6 +271,14 @@ int do_raster_forward(msplat_hip_ctx *ctx, const float bg[3]) {
         return set_error(MSPLAT_ERR_ARG, "forward rasterization needs a binned frame on this context");
     for (int i = 0; i < 3; i++) ctx->bg[i] = bg ? bg[i] : 0.f;
     CHECKED(ensure_pixels(ctx));
+    const int tc = ctx->tuning.chunks;
+    ctx->frame_chunks = ctx->M <= 0 ? 1 : tc == 0 ? auto_chunks(ctx->ntiles, ctx->M) : tc;
+    if (ctx->frame_chunks > 1) {
+        const size_t planes = (size_t)ctx->frame_chunks * ctx->width * ctx->height;
+        CHECKED(ctx->grow(ctx->chunk_prod, 4 * planes));
+        CHECKED(ctx->grow(ctx->chunk_state, 16 * planes));
+        CHECKED(ctx->grow(ctx->chunk_last, 4 * planes));
+    }
     ctx->begin(2);
     launch_raster_forward(raster_args(ctx), ctx->tuning, ctx->stream);
     ctx->end();
@@ -634,6 +646,7 @@ int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *f) {
     f->num_tiles = ctx->ntiles;
     f->num_isects = ctx->M;
     f->num_visible = ctx->V;
+    f->depth_chunks = ctx->frame_chunks;
     f->splat_a = (float *)ctx->splat_a.p;
     f->splat_b = (float *)ctx->splat_b.p;
     f->splat_blue = (float *)ctx->splat_blue.p;
@@ -873,6 +886,10 @@ int msplat_hip_set_tuning(msplat_hip_ctx *ctx, int key, int value) {
             return set_error(MSPLAT_ERR_ARG, "backward pixels per thread must be 1, 2 or 4");
         ctx->tuning.bwd_ppt = value;
         return MSPLAT_OK;
+    case MSPLAT_TUNE_CHUNKS:
+        if (value < 0 || value > 64) return set_error(MSPLAT_ERR_ARG, "depth chunks must be 0 (per frame) or 1..64");
+        ctx->tuning.chunks = value;
+        return MSPLAT_OK;
     case MSPLAT_TUNE_FWD_BATCH:
         if (value != 64 && value != 128 && value != 256)
             return set_error(MSPLAT_ERR_ARG, "forward LDS batch depth must be 64, 128 or 256");
@@ -904,6 +921,7 @@ int msplat_hip_get_tuning(msplat_hip_ctx *ctx, int key, int *value) {
     case MSPLAT_TUNE_BWD_BATCH: *value = ctx->tuning.bwd_batch; return MSPLAT_OK;
     case MSPLAT_TUNE_CULL: *value = ctx->tuning.cull; return MSPLAT_OK;
     case MSPLAT_TUNE_PACK_KEYS: *value = ctx->tuning.pack_keys; return MSPLAT_OK;
+    case MSPLAT_TUNE_CHUNKS: *value = ctx->tuning.chunks; return MSPLAT_OK;
     default: return set_error(MSPLAT_ERR_ARG, "unknown tunable %d", key);
     }
 }

@@ -83,6 +83,8 @@ struct msplat_hip_ctx {
     // per tile / pixel
     Buf tile_start, image, final_T, final_idx, v_image, row_loss, loss_scratch;
     Buf grad_scratch;  // stands in for NULL gradient outputs of render_backward
+    Buf chunk_prod, chunk_state, chunk_last;  // depth-chunked rasterization planes
+    int frame_chunks = 1;                     // depth chunks per tile of the frame's forward
     Buf aux;           // densify flags/scans/scratch, kNN and metrics workspaces
     int32_t *vis_sorted = nullptr;  // visible gaussians in depth order (the frame's list)
     msplat::BatchState *batch = nullptr;  // batched-view frame and graph (created on first use)
@@ -190,7 +192,8 @@ struct msplat_hip_ctx {
         Buf *all[] = {&splat_a, &splat_b, &splat_blue, &depths, &aabb, &rect, &radii, &count, &offsets, &block_sums,
                       &v_xy, &v_conic, &v_rgb, &v_opac, &sh_scratch, &big, &keys0, &keys1, &vals0, &vals1, &slot_a, &slot_b, &slot_c, &slot_f,
                       &hist, &digit_total, &keys64, &block_vis, &dk0, &dk1, &dv0, &dv1, &ord_sums, &scan_part, &loff, &tile_start, &image, &final_T, &final_idx, &v_image,
-                      &row_loss, &loss_scratch, &scalars, &grad_scratch, &aux};
+                      &row_loss, &loss_scratch, &scalars, &grad_scratch, &aux, &chunk_prod, &chunk_state,
+                      &chunk_last};
         for (Buf *b : all) release(*b);
         if (batch) msplat::destroy_batch_state(batch);
         if (h_total) (void)hipHostFree(h_total);

@@ -13,6 +13,7 @@ constexpr int RS_ITEMS = 4096;  // radix-sort elements per block (256 threads x 
 
 // Kernel variants of the rasterizer, picked per context (msplat_hip_set_tuning)
 struct Tuning {
+    int chunks = 0;       // depth chunks per tile: 0 = chosen per frame, 1 = off, 2..64 = fixed
     int bwd_ppt = 4;      // backward pixels per thread: 1, 2, 4
     int fwd_batch = 256;  // forward LDS batch depth: 64, 128, 256
     int bwd_batch = 128;  // backward LDS batch depth: 32, 64, 128
@@ -101,6 +102,12 @@ struct RasterArgs {
                       // all 0 when the raster backward starts
     unsigned long long *slots_written;  // null, or a running count of written slots
     int cull;  // set by the launchers from Tuning::cull
+    // depth chunks per tile (1: one workgroup per tile) and the chunked form's per-chunk
+    // planes of chunks * width * height entries
+    int chunks;
+    float *chunk_prod;
+    float4 *chunk_state;
+    int32_t *chunk_last;
 };
 
 // Loss blocks: 8 rows x 118 columns; LOSS_PLANES floats of scratch per pixel (15
@@ -349,6 +356,7 @@ void launch_view_keys(const int32_t *recs, const uint32_t *n_dev, uint32_t n_max
 // batch: ranges over (local tile, view) pairs, index tile * num_views + view
 void launch_tile_ranges_views(const uint32_t *keys, const int32_t *recs, const uint32_t *n_dev, uint32_t n_max, int n,
                               const BatchTable *tab, int32_t *range_start, hipStream_t s);
+int auto_chunks(int nt, int64_t m);
 void launch_raster_forward(const RasterArgs &a, const Tuning &t, hipStream_t s);
 void launch_raster_backward(const RasterArgs &a, const Tuning &t, hipStream_t s);
 void launch_loss(const LossArgs &a, hipStream_t s);

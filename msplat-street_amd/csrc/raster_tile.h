@@ -189,6 +189,144 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
     }
 }
 
+// ── Depth-chunked compositing for views with few tiles ─────────────────────────
+// A view with few tiles runs few workgroups, each over a long list. The chunked form
+// splits every tile's list into K consecutive parts of equal length (chunk c of a list
+// [s, e) is [s + c len / K, s + (c + 1) len / K)) and runs one workgroup per (tile,
+// chunk), in four launches:
+//  1. each chunk's per-pixel product of (1 - alpha) over its valid records, stopped once
+//     it reaches T_MIN (every later chunk then starts at or below T_MIN);
+//  2. per pixel, the exclusive prefix products in chunk order: the transmittance at each
+//     chunk's start;
+//  3. each chunk composites front to back from that transmittance with the reference's
+//     absolute termination test T (1 - alpha) <= T_MIN; a pixel whose starting
+//     transmittance is already at or below T_MIN has terminated in an earlier chunk and
+//     composites nothing; a chunk records its colour sum, its end transmittance (negated
+//     when the pixel terminated in it) and its last index;
+//  4. a merge per pixel: the chunks' colours summed in order up to the terminating
+//     chunk, the final transmittance and last contributor.
+// The result is the sequential composite of rasterize_forward (cpu:574-626) up to float
+// rounding of the transmittance products and colour sums. The backward first computes,
+// per pixel and chunk, S at the chunk's end (the background term plus the colours of the
+// later chunks up to the terminating one), then runs each chunk back to front from its
+// end transmittance and that S.
+struct ChunkView {
+    int k, K;           // this chunk, chunks per tile
+    size_t npix;        // pixels of the view (the stride between chunk planes)
+    float *prod;        // [K][npix] the chunk's product of (1 - alpha); after the prefix pass the
+                        // transmittance at the chunk's start; before a backward, S at its end
+    float4 *state;      // [K][npix] (colour r, g, b, end transmittance; negated: terminated here)
+    int32_t *last;      // [K][npix] last index the chunk composited (or k - 1 at termination), -1: none
+};
+
+__device__ inline void chunk_range(int s, int e, int k, int K, int &cs, int &ce) {
+    const long len = e - s;
+    cs = s + (int)(len * k / K);
+    ce = s + (int)(len * (k + 1) / K);
+}
+
+// Stages records [base, base + FB) of [s, e) that can reach the tile into LDS, in list
+// order, padded with zero-opacity records to a multiple of 8; returns the padded count.
+template <int FB>
+__device__ inline int stage_forward_batch(const TileData &d, int base, int e, float x0, float y0, float4 *la,
+                                          float4 *lb, float *lc, int *lk, int *wcount) {
+    const int k = base + threadIdx.x;
+    bool keep = false;
+    float4 A0, B0;
+    float C0 = 0.f;
+    if (threadIdx.x < FB && k < e) {
+        const int g = (int)((uint32_t)d.gid[k] & d.gid_mask);
+        A0 = d.splat_a[g];
+        B0 = d.splat_b[g];
+        C0 = d.splat_blue[g];
+        keep = !(d.cull && splat_misses_tile(A0, B0, x0, y0));
+    }
+    int n;
+    const int pos = kept_position<4>(keep, wcount, n);
+    if (keep) {
+        la[pos] = A0;
+        lb[pos] = make_float4(B0.x, B0.y, fmaxf(B0.z + 0.5f, 0.f), fmaxf(B0.w + 0.5f, 0.f));
+        lc[pos] = fmaxf(C0 + 0.5f, 0.f);
+        lk[pos] = k;
+    }
+    const int n8 = (n + 7) & ~7;
+    if ((int)threadIdx.x >= n && (int)threadIdx.x < n8) {
+        la[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+        lb[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+        lc[threadIdx.x] = 0.f;
+        lk[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    return n8;
+}
+
+// Launch 1 (PRODUCT) or 2 of the chunked forward for chunk ch.k of global tile gt.
+template <int FB, bool PRODUCT>
+__device__ inline void forward_chunk(const TileView &v, const TileData &d, const ChunkView &ch, int gt, int lt) {
+#pragma clang fp contract(fast)
+    static_assert(FB <= 256, "one staged record per thread");
+    __shared__ float4 la[FB], lb[FB];
+    __shared__ float lc[FB];
+    __shared__ int lk[FB];
+    __shared__ int wcount[4];
+    const int tx = lt % v.tiles_x, ty = lt / v.tiles_x;
+    const float x0 = (float)(tx * TILE), y0 = (float)(ty * TILE);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int px = tx * TILE + (w & 1) * 8 + (lane & 7);
+    const int py = ty * TILE + (w >> 1) * 8 + (lane >> 3);
+    const bool inside = px < (int)v.width && py < (int)v.height;
+    const size_t pid = inside ? (size_t)py * v.width + px : 0;
+    int s, e;
+    chunk_range(d.tile_start[gt], d.tile_start[gt + 1], ch.k, ch.K, s, e);
+    const float fx = (float)px, fy = (float)py;
+    float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f;
+    if (!PRODUCT && inside) T = ch.prod[(size_t)ch.k * ch.npix + pid];  // at the chunk's start
+    int last = -1;
+    bool done = !inside || (!PRODUCT && !(T > T_MIN)), term = false;
+    for (int base = s; base < e; base += FB) {
+        if (__syncthreads_count(done) == 256) break;
+        const int n8 = stage_forward_batch<FB>(d, base, e, x0, y0, la, lb, lc, lk, wcount);
+        if (__ballot(!done) == 0) continue;
+        for (int j0 = 0; j0 < n8; j0 += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int j = j0 + u;
+                const float4 A = la[j], B = lb[j];
+                const float dx = A.x - fx, dy = A.y - fy;
+                const float sig = splat_sigma((A.z * dx) * dx, A.w * dx, B.x, dy);
+                const float alpha = fminf(ALPHA_MAX, B.y * __expf(-sig));
+                const bool valid = !done && sig >= 0.f && sig < SIGMA_MAX && alpha >= ALPHA_MIN;
+                const float nT = T * (1.f - alpha);
+                if (PRODUCT) {  // once T <= T_MIN every later chunk starts terminated: stop there
+                    T = valid ? nT : T;
+                    done = done || !(T > T_MIN);
+                } else {
+                    const bool t = valid && nT <= T_MIN;
+                    const bool accept = valid && !t;
+                    const float vis = accept ? alpha * T : 0.f;
+                    cr += B.z * vis;
+                    cg += B.w * vis;
+                    cb += lc[j] * vis;
+                    T = accept ? nT : T;
+                    const int kj = lk[j];
+                    last = t ? kj - 1 : (accept ? kj : last);
+                    term = term || t;
+                    done = done || t;
+                }
+            }
+            if (__ballot(!done) == 0) break;
+        }
+    }
+    if (!inside) return;
+    const size_t o = (size_t)ch.k * ch.npix + pid;
+    if (PRODUCT) {
+        ch.prod[o] = T;
+    } else {
+        ch.state[o] = make_float4(cr, cg, cb, term ? -T : T);
+        ch.last[o] = last;
+    }
+}
+
 // rasterize_backward (core/cpu/msplat_cpu.cpp:959-1031), slot values per intersection:
 // a = (v_x, v_y, v_conic.x, v_conic.y), b = (v_conic.z, v_r, v_g, v_b), c = v_logit.
 //
@@ -214,8 +352,12 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
 // or to a nonzero value returns it unchanged.
 constexpr int NQ = 9;  // sum vs, vs dx, vs dy, vs dx^2, vs dx dy, vs dy^2, fac vr, fac vg, fac vb
 
-template <int PPT, int BATCH>
-__device__ inline void backward_tile(const TileView &v, const TileData &d, int gt, int lt) {
+// With CHUNK the workgroup runs chunk ch.k of the tile's list (a depth-chunked forward
+// made the frame): each pixel starts from the chunk's end transmittance and from S over
+// the final transmittance's background plus the later chunks' colours.
+template <int PPT, int BATCH, bool CHUNK = false>
+__device__ inline void backward_tile(const TileView &v, const TileData &d, int gt, int lt,
+                                     const ChunkView *ch = nullptr) {
 #pragma clang fp contract(fast)
     constexpr int NT = 256 / PPT, NW = NT / 64;
     __shared__ float4 la[BATCH], lb[BATCH];
@@ -229,7 +371,8 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
     __shared__ int wcount[2][NW];
     const int tx = lt % v.tiles_x, ty = lt / v.tiles_x;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int s = d.tile_start[gt], e = d.tile_start[gt + 1];
+    int s = d.tile_start[gt], e = d.tile_start[gt + 1];
+    if (CHUNK) chunk_range(s, e, ch->k, ch->K, s, e);
     if (s == e) return;
     const SumLanes sl(lane);
     const int px = tx * TILE + (lane & 15);
@@ -251,6 +394,11 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             vg[k] = v.v_image[3 * pid + 1];
             vb[k] = v.v_image[3 * pid + 2];
             S[k] = T[k] * (v.bg[0] * vr[k] + v.bg[1] * vg[k] + v.bg[2] * vb[k]);
+            if (CHUNK) {  // the chunk's end transmittance and S over everything behind it
+                const size_t o = (size_t)ch->k * ch->npix + pid;
+                T[k] = fabsf(ch->state[o].w);
+                S[k] = ch->prod[o];
+            }
         }
         m = max(m, last[k]);
     }

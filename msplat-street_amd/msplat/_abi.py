@@ -22,6 +22,7 @@ TUNE_FWD_BATCH = 1
 TUNE_BWD_BATCH = 2
 TUNE_CULL = 3
 TUNE_PACK_KEYS = 4
+TUNE_CHUNKS = 5
 # stage timing: stage s is bit (1 << s) of a stage mask
 STAGE_NAMES = ("proj_sh_fwd", "prefix_sort_pack", "rast_fwd", "loss_fwd_bwd", "rast_bwd", "proj_sh_bwd_adam")
 fp = C.POINTER(C.c_float)
@@ -74,7 +75,8 @@ class Frame(C.Structure):
                 ("tile_start", C.c_void_p), ("image", C.c_void_p), ("final_T", C.c_void_p),
                 ("v_image", C.c_void_p), ("final_idx", C.c_void_p), ("v_xy", C.c_void_p),
                 ("v_conic", C.c_void_p), ("v_rgb", C.c_void_p), ("v_opacity", C.c_void_p),
-                ("loss_sum", C.c_void_p), ("slots_written", C.c_void_p), ("num_visible", C.c_int64)]
+                ("loss_sum", C.c_void_p), ("slots_written", C.c_void_p), ("num_visible", C.c_int64),
+                ("depth_chunks", C.c_int32)]
 
 
 class Checkpoint(C.Structure):

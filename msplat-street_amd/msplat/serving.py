@@ -29,7 +29,7 @@ import time
 
 import numpy as np
 
-from ._abi import Context, DeviceArray, check, floats, lib, make_camera_struct
+from ._abi import TUNE_CHUNKS, Context, DeviceArray, check, floats, lib, make_camera_struct
 from .backend import BatchViews, DeviceGaussians, render_batch
 
 REC_BYTES = 116    # per record (view x gaussian) of a batched frame
@@ -59,6 +59,9 @@ class _Slot:
     def __init__(self, index: int, device: int):
         self.index = index
         self.ctx = Context(device)
+        # requests render per tile, alone or batched (the batched rasterizer has no depth
+        # chunks): a request's image does not depend on the batch it landed in
+        self.ctx.set_tuning(TUNE_CHUNKS, 1)
         self.rgba = None
         self.busy = False
 
@@ -228,10 +231,12 @@ class Server:
 
 
 def serve_plain(scenes: list, requests: list, background=(0.0, 0.0, 0.0), device: int = 0,
-                keep_images: bool = False) -> list:
+                keep_images: bool = False, depth_chunks: int = 0) -> list:
     """The plain form: one msplat_hip_render (+ RGBA8 pack) per request, first come first
-    served, on one context."""
+    served, on one context; depth_chunks as MSPLAT_TUNE_CHUNKS (0: chosen per frame, 1:
+    per tile, as the server renders)."""
     ctx = Context(device)
+    ctx.set_tuning(TUNE_CHUNKS, depth_chunks)
     bg = floats(background)
     rgba = None
     t0 = time.perf_counter()

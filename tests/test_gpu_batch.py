@@ -12,6 +12,17 @@ BG = (0.2, 0.35, 0.5)
 SIZES = [(64, 48), (97, 71), (128, 128), (50, 70), (33, 17), (160, 90)]  # (W, H), not all tile multiples
 
 
+@pytest.fixture(autouse=True)
+def _per_tile_one_view_frames(ctx):
+    """The batched rasterizer runs one workgroup per (view, tile); the one-view frames it
+    is compared with bit for bit run the same per-tile form (no depth chunks, which give
+    the same frame up to float rounding: tests/test_gpu_chunked.py)."""
+    from msplat._abi import TUNE_CHUNKS
+    ctx.set_tuning(TUNE_CHUNKS, 1)
+    yield
+    ctx.set_tuning(TUNE_CHUNKS, 0)
+
+
 def _setup(seed=3, n=4000, deg=3, sizes=SIZES, dtu=3):
     from msplat.camera import look_at, make_camera
     from msplat.scenes import random_scene

@@ -36,11 +36,18 @@ def test_results_are_the_same_alone_and_in_the_mix(ctx):
     assert len(done) == len(reqs) and not srv.rejected
     assert max(r.batch_size for r in done) > 1  # requests were batched
     alone = serve_plain(scenes, [type(r)(r.rid, r.scene, r.cam, 0.0) for r in reqs], background=BG,
-                        keep_images=True)
+                        keep_images=True, depth_chunks=1)
     by_id = {r.rid: r for r in alone}
     for r in done:
         np.testing.assert_array_equal(r.image, by_id[r.rid].image, err_msg=f"request {r.rid}")
     assert any(r.image[..., :3].std() > 2 for r in done)  # scenes are visible
+    # the plain form's default depth-chunked frames: the same up to float rounding, at
+    # most one RGBA8 step on a few pixels
+    chunked = serve_plain(scenes, [type(r)(r.rid, r.scene, r.cam, 0.0) for r in reqs], background=BG,
+                          keep_images=True)
+    diffs = [np.abs(r.image.astype(int) - by_id[r.rid].image.astype(int)) for r in chunked]
+    assert max(int(d.max()) for d in diffs) <= 1
+    assert np.mean([np.mean(d > 0) for d in diffs]) < 1e-3
 
 
 def test_admission_keeps_batches_within_the_budget(ctx):

@@ -228,7 +228,7 @@ def test_serving_alone_and_in_the_mix_at_real_scenes(ctx):
     assert len(done) == len(reqs) and not srv.rejected
     assert max(r.batch_size for r in done) > 1
     alone = serve_plain(scenes, [type(r)(r.rid, r.scene, r.cam, 0.0) for r in reqs], background=tuple(BG),
-                        keep_images=True)
+                        keep_images=True, depth_chunks=1)
     by_id = {r.rid: r for r in alone}
     sizes = sorted({(r.cam["W"], r.cam["H"]) for r in done})
     print(f"\nserving: {len(done)} requests, sizes {sizes}, largest batch {max(r.batch_size for r in done)}")

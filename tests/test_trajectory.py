@@ -19,8 +19,15 @@ KEYS = ("means", "scales", "quats", "dc", "rest", "opac")
 # HIP trainer against the reference's run
 LOSS_REL_TOL = 1e-4        # every step's loss
 PSNR_TOL_DB = 0.05         # final PSNR of every view
-PARAM_REL_TOL = 1e-3       # relative L2 of each group's sampled final parameters
+PARAM_REL_TOL = 1e-3       # relative L2 of each group's sampled final parameters (per-tile frames)
 SUM_REL_TOL = 1e-4         # relative difference of each group's sum of squares
+# The 128x128 views have 64 tiles, so by default every frame is depth-chunked
+# (MSPLAT_TUNE_CHUNKS): transmittance products and colour sums are formed per chunk and
+# then combined, a different float rounding from the sequential composite. 300 Adam steps
+# amplify it most in the parameters whose gradients sit near zero (opacity: 1.3e-3
+# relative L2 after 300 steps against 3.3e-4 per tile); every step's loss and the final
+# PSNR keep the bars above.
+PARAM_REL_TOL_CHUNKED = 3e-3
 
 
 def _targets(orc, gt_scene, cams, d):
@@ -53,8 +60,9 @@ def test_oracle_trajectory_matches_reference(oracle):
 
 
 @pytest.mark.gpu
-def test_hip_trajectory_matches_reference(ctx, oracle):
-    from msplat._abi import DeviceArray
+@pytest.mark.parametrize("chunks", [1, 0], ids=["per_tile", "depth_chunked"])
+def test_hip_trajectory_matches_reference(ctx, oracle, chunks):
+    from msplat._abi import DeviceArray, TUNE_CHUNKS
     from msplat.backend import DeviceGaussians
     from msplat.trainer import TrainConfig, Trainer
     d = golden("trajectory_c1")
@@ -66,10 +74,16 @@ def test_hip_trajectory_matches_reference(ctx, oracle):
     picked = []
     pick = tr.next_camera
     tr.next_camera = lambda: picked.append(pick()) or picked[-1]
-    losses = np.array([tr.step(read_loss=True) for _ in range(C1_STEPS)], np.float32)
+    ctx.set_tuning(TUNE_CHUNKS, chunks)
+    try:
+        losses = np.array([tr.step(read_loss=True) for _ in range(C1_STEPS)], np.float32)
+        assert (ctx.frame().depth_chunks > 1) == (chunks == 0)
+    finally:
+        ctx.set_tuning(TUNE_CHUNKS, 0)
     np.testing.assert_array_equal(np.array(picked), d["order"])
     rel = np.abs(losses - d["losses"]) / d["losses"]
-    print(f"\nc1 trajectory: loss max rel {rel.max():.3e} (step {int(rel.argmax()) + 1}), "
+    print(f"\nc1 trajectory ({'per tile' if chunks else 'depth-chunked'}): loss max rel {rel.max():.3e} "
+          f"(step {int(rel.argmax()) + 1}), "
           f"final {losses[-1]:.8f} vs {d['losses'][-1]:.8f}")
     assert rel.max() <= LOSS_REL_TOL, rel.max()
     got = dg.to_numpy()
@@ -82,5 +96,5 @@ def test_hip_trajectory_matches_reference(ctx, oracle):
         e = np.linalg.norm(a[d["sample"]] - d["final_" + k]) / np.linalg.norm(d["final_" + k])
         s = abs(float((a.astype(np.float64) ** 2).sum()) - float(d["sumsq_" + k])) / float(d["sumsq_" + k])
         print(f"  {k}: sampled rel L2 {e:.2e}, sum of squares rel {s:.2e}")
-        assert e <= PARAM_REL_TOL, (k, e)
+        assert e <= (PARAM_REL_TOL if chunks else PARAM_REL_TOL_CHUNKED), (k, e)
         assert s <= SUM_REL_TOL, (k, s)
