@@ -525,23 +525,27 @@ __global__ __launch_bounds__(256) void keygen_emit_kernel(KeygenArgs a) {
     const uint32_t o0 = blockIdx.x * EMIT_CHUNK;
     if (o0 >= M) return;
     const uint32_t o1 = min(o0 + (uint32_t)EMIT_CHUNK, M);
-    if (threadIdx.x == 0) {
-        // records first..last cover [o0, o1): the last i with loff[i] <= o0, o1 - 1
+    {
+        // records first..last cover [o0, o1): the last i with loff[i] <= o0, o1 - 1 (loff
+        // rises from loff[0] = 0). A 256-ary search: each round every thread tests one
+        // probe of each range, and the count of probes at or below the target narrows
+        // the range [lo, hi) 256-fold (3 rounds for 3M records, every thread in step).
         const uint32_t nvis = *a.nvis_ptr;
-        uint32_t lo = 0, hi = nvis - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (a.loff[mid] <= o0) lo = mid;
-            else hi = mid - 1;
+        uint32_t lo1 = 0, hi1 = nvis, lo2 = 0, hi2 = nvis;
+        while (hi1 - lo1 > 1 || hi2 - lo2 > 1) {
+            const uint32_t st1 = (hi1 - lo1 + 255) / 256, st2 = (hi2 - lo2 + 255) / 256;
+            const uint32_t i1 = lo1 + threadIdx.x * st1, i2 = lo2 + threadIdx.x * st2;
+            const int c1 = __syncthreads_count(i1 < hi1 && a.loff[i1] <= o0);
+            const int c2 = __syncthreads_count(i2 < hi2 && a.loff[i2] <= o1 - 1);
+            lo1 += (uint32_t)(c1 - 1) * st1;
+            hi1 = min(lo1 + st1, hi1);
+            lo2 += (uint32_t)(c2 - 1) * st2;
+            hi2 = min(lo2 + st2, hi2);
         }
-        uint32_t lo2 = lo, hi2 = nvis - 1;
-        while (lo2 < hi2) {
-            const uint32_t mid = (lo2 + hi2 + 1) >> 1;
-            if (a.loff[mid] <= o1 - 1) lo2 = mid;
-            else hi2 = mid - 1;
+        if (threadIdx.x == 0) {
+            s_first = lo1;
+            s_n = lo2 - lo1 + 1;  // <= o1 - o0: every record after the first starts inside the slice
         }
-        s_first = lo;
-        s_n = lo2 - lo + 1;  // <= o1 - o0: every record after the first starts inside the slice
     }
     __syncthreads();
     const uint32_t first = s_first, nrec = s_n;

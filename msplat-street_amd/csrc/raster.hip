@@ -104,6 +104,12 @@ __global__ __launch_bounds__(256, 8) void raster_forward_chunk_kernel(RasterArgs
     forward_chunk<FB, PRODUCT>(tile_view(a), tile_data(a), ch, t, t);
 }
 
+// The per-pixel chunk passes below walk a pixel's K chunk planes in groups of CG: a
+// group's loads are issued together, then used in chunk order, so a pixel waits for
+// memory K / CG times instead of K times (with few tiles there are few pixels to hide
+// the latency behind).
+constexpr int CG = 8;
+
 // The merge of the chunked forward, one thread per pixel: colours summed in chunk order
 // up to the chunk the pixel terminated in, the final transmittance and last index, the
 // composite over the background.
@@ -114,14 +120,26 @@ __global__ __launch_bounds__(256) void raster_chunk_merge_kernel(RasterArgs a) {
     const int tile = (int)(y / TILE) * a.tiles_x + (int)(x / TILE);
     float cr = 0.f, cg = 0.f, cb = 0.f, T = 1.f;
     int last = a.tile_start[tile] - 1;
-    for (int k = 0; k < a.chunks; k++) {
-        const float4 c = a.chunk_state[(size_t)k * np + pid];
-        cr += c.x;
-        cg += c.y;
-        cb += c.z;
-        T = fabsf(c.w);
-        last = max(last, a.chunk_last[(size_t)k * np + pid]);
-        if (c.w < 0.f) break;  // terminated in this chunk
+    bool done = false;
+    for (int k0 = 0; k0 < a.chunks && !done; k0 += CG) {
+        float4 c[CG];
+        int l[CG];
+#pragma unroll
+        for (int u = 0; u < CG; u++) {
+            const size_t o = (size_t)min(k0 + u, a.chunks - 1) * np + pid;
+            c[u] = a.chunk_state[o];
+            l[u] = a.chunk_last[o];
+        }
+#pragma unroll
+        for (int u = 0; u < CG; u++) {
+            if (done || k0 + u >= a.chunks) break;
+            cr += c[u].x;
+            cg += c[u].y;
+            cb += c[u].z;
+            T = fabsf(c[u].w);
+            last = max(last, l[u]);
+            done = c[u].w < 0.f;  // terminated in this chunk
+        }
     }
     a.final_T[pid] = T;
     a.final_idx[pid] = last;
@@ -137,11 +155,16 @@ __global__ __launch_bounds__(256) void raster_chunk_prefix_kernel(RasterArgs a) 
     if (x >= a.width || y >= a.height) return;
     const size_t pid = (size_t)y * a.width + x, np = (size_t)a.width * a.height;
     float T = 1.f;
-    for (int k = 0; k < a.chunks; k++) {
-        const size_t o = (size_t)k * np + pid;
-        const float p = a.chunk_prod[o];
-        a.chunk_prod[o] = T;
-        T *= p;
+    for (int k0 = 0; k0 < a.chunks; k0 += CG) {
+        float p[CG];
+#pragma unroll
+        for (int u = 0; u < CG; u++) p[u] = a.chunk_prod[(size_t)min(k0 + u, a.chunks - 1) * np + pid];
+#pragma unroll
+        for (int u = 0; u < CG; u++) {
+            if (k0 + u >= a.chunks) break;
+            a.chunk_prod[(size_t)(k0 + u) * np + pid] = T;
+            T *= p[u];
+        }
     }
 }
 
@@ -157,12 +180,17 @@ __global__ __launch_bounds__(256) void raster_chunk_suffix_kernel(RasterArgs a) 
     const float vr = a.v_image[3 * pid], vg = a.v_image[3 * pid + 1], vb = a.v_image[3 * pid + 2];
     const float base = a.final_T[pid] * (a.bg[0] * vr + a.bg[1] * vg + a.bg[2] * vb);
     float S = base;
-    for (int k = a.chunks - 1; k >= 0; k--) {
-        const size_t o = (size_t)k * np + pid;
-        const float4 c = a.chunk_state[o];
-        if (c.w < 0.f) S = base;
-        a.chunk_prod[o] = S;
-        S += c.x * vr + c.y * vg + c.z * vb;
+    for (int k0 = a.chunks - 1; k0 >= 0; k0 -= CG) {
+        float4 c[CG];
+#pragma unroll
+        for (int u = 0; u < CG; u++) c[u] = a.chunk_state[(size_t)max(k0 - u, 0) * np + pid];
+#pragma unroll
+        for (int u = 0; u < CG; u++) {
+            if (k0 - u < 0) break;
+            if (c[u].w < 0.f) S = base;
+            a.chunk_prod[(size_t)(k0 - u) * np + pid] = S;
+            S += c[u].x * vr + c[u].y * vg + c[u].z * vb;
+        }
     }
 }
 
