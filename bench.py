@@ -83,6 +83,39 @@ def slots_written(ctx) -> int:
     return int(ctx.download(ctx.frame().slots_written, (1,), np.uint64)[0])
 
 
+def frame_counts(ctx) -> tuple:
+    """The context's running counts of one-view intersections and visible gaussians."""
+    f = ctx.frame()
+    return (int(ctx.download(f.isects_running, (1,), np.uint64)[0]),
+            int(ctx.download(f.visible_running, (1,), np.uint64)[0]))
+
+
+def time_step_graph(ctx, tr, windows=4, steps=50):
+    """Alternating windows of launch-by-launch steps and replays of the captured step
+    (msplat_hip_set_step_graph) on the same schedule; each mode's first step (which sizes
+    the capacity) is outside its window. Returns both rates."""
+    rates = {False: [0.0, 0], True: [0.0, 0]}
+    c0 = ctx.step_graph_stats()
+    for w in range(2 * windows):
+        graph = w % 2 == 1
+        ctx.set_step_graph(graph)
+        tr.step()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            tr.step()
+        ctx.sync()
+        rates[graph][0] += time.perf_counter() - t0
+        rates[graph][1] += steps
+    ctx.set_step_graph(False)
+    c1 = ctx.step_graph_stats()
+    return {"graph_iter_per_s": round(rates[True][1] / rates[True][0], 2),
+            "eager_iter_per_s": round(rates[False][1] / rates[False][0], 2),
+            "windows": f"{windows} x {steps} steps per mode, alternating, steps {tr.step_count - 2 * windows * (steps + 1) + 1}"
+                       f"-{tr.step_count}",
+            "captures": c1[0] - c0[0], "replays": c1[1] - c0[1], "reruns": c1[2] - c0[2]}
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -610,8 +643,9 @@ def plain_workload(ctx):
 
 def bench_plain(repeats=3):
     """The plain workload's whole 300-step run on the GPU (best of `repeats` after one
-    warm-up run), with the per-tile rasterizers and with depth chunks chosen per frame
-    (64 tiles per view: every frame is chunked)."""
+    warm-up run), with the per-tile rasterizers, with depth chunks chosen per frame (64
+    tiles per view: every frame is chunked), and with those steps replayed from a captured
+    graph."""
     from msplat._abi import TUNE_CHUNKS, Context
     from msplat.backend import DeviceGaussians
     from msplat.trainer import TrainConfig, Trainer
@@ -619,12 +653,14 @@ def bench_plain(repeats=3):
     out = {"workload": "20,000 Gaussians SH1, 8 views at 128x128, 300 Adam steps, no densification"}
     try:
         sc, cams, targets = plain_workload(ctx)
-        for name, chunks in (("per_tile", 1), ("depth_chunked", 0)):
+        for name, chunks, graph in (("per_tile", 1, False), ("depth_chunked", 0, False),
+                                    ("depth_chunked_step_graph", 0, True)):
             ctx.set_tuning(TUNE_CHUNKS, chunks)
             best, loss = None, None
             for r in range(repeats + 1):
                 dg = DeviceGaussians(ctx, sc)
-                cfg = TrainConfig(iterations=PLAIN_STEPS, sh_degree=1, sh_degree_interval=1, densify=False, seed=42)
+                cfg = TrainConfig(iterations=PLAIN_STEPS, sh_degree=1, sh_degree_interval=1, densify=False, seed=42,
+                                  step_graph=graph)
                 tr = Trainer(ctx, dg, cams, targets, cfg)
                 ctx.sync()
                 t0 = time.perf_counter()
@@ -717,6 +753,8 @@ def main():
                     help="steps after the timed ones that time every stage for the breakdown")
     ap.add_argument("--no-sh3", action="store_true", help="skip timing steps with SH degree 3 in use")
     ap.add_argument("--no-plain", action="store_true", help="skip the 300-step plain workload (128x128 views)")
+    ap.add_argument("--no-graph-compare", action="store_true",
+                    help="skip timing replays of the captured step against launch-by-launch steps")
     ap.add_argument("--summarize-profile", nargs="+", metavar="DIR",
                     help="summarise rocprofv3 output directories of this benchmark (kernel trace with the "
                          "'timed_steps' range, --pmc runs) as JSON and exit; --warmup/--steps give the window")
@@ -770,22 +808,22 @@ def main():
     ctx.enable_stage_timing(True)
     ctx.set_stage_mask((DOMINANT,))
     ctx.drain_stage_times()
-    m_total, n_total, v_total = 0, 0, 0
+    n_total = 0
     ctx.sync()
     written0 = slots_written(ctx)
+    counts0 = frame_counts(ctx)
     rtx = _Roctx()
     rtx.push("timed_steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         tr.step()
-        f = ctx.frame()
-        m_total += f.num_isects
-        v_total += f.num_visible
         n_total += dg.n
     ctx.sync()
     elapsed = time.perf_counter() - t0
     rtx.pop()
     written = slots_written(ctx) - written0
+    counts1 = frame_counts(ctx)
+    m_total, v_total = counts1[0] - counts0[0], counts1[1] - counts0[1]
     timed = ctx.drain_stage_times()
     dtu_timed = min((first_timed_step + args.steps - 1) // 1000, 3)
     # stage breakdown over a separate window of steps that times every stage
@@ -862,6 +900,10 @@ def main():
     heaviest = max(stages, key=lambda k: stages[k][0]) if stage_rows else DOMINANT
     if heaviest != DOMINANT:
         log(f"note: the stage breakdown's heaviest stage is {heaviest}, not {DOMINANT}")
+    graph = None
+    if not args.no_graph_compare:
+        graph = time_step_graph(ctx, tr)
+        log(f"captured step: {graph['graph_iter_per_s']} iter/s replayed, {graph['eager_iter_per_s']} launch by launch")
     sh3 = None
     if not args.no_sh3:
         sh3 = round(time_sh3_steps(ctx, dg, cams, targets), 2)
@@ -901,6 +943,7 @@ def main():
                    "tuning": {k: tuning[k] for k in ("bwd_ppt", "fwd_batch", "bwd_batch")} if tuning else None},
         "mpix_per_s": round(P * iters / 1e6, 1),
         "sh3_iter_per_s": sh3,
+        "step_graph": graph,
         "roofline": roofline,
         "batched": batched,
         "serving": serving,

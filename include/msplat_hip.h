@@ -162,6 +162,11 @@ typedef struct {
     uint64_t *slots_written;
     int64_t num_visible; /* gaussians with a non-empty tile rectangle (radius > 0) */
     int32_t depth_chunks; /* depth chunks per tile of the last one-view forward (1: none) */
+    /* device pointers to running counts (uint64) of the one-view frames' intersections and
+     * visible gaussians since the context's first frame (a replay run again after an
+     * intersection overflow counts twice); read before and after a window of steps */
+    uint64_t *isects_running;
+    uint64_t *visible_running;
 } msplat_hip_frame;
 int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *out);
 /* KEEP_GRADS: fill the reduced gradients; KEYS64: materialise isect_keys on frame_view
@@ -375,6 +380,26 @@ int msplat_hip_render_backward_batch(msplat_hip_ctx *ctx, const msplat_hip_gauss
 int msplat_hip_set_batch_graph(msplat_hip_ctx *ctx, int enable);
 /* Graph statistics: captures so far and replays so far. */
 int msplat_hip_batch_graph_stats(msplat_hip_ctx *ctx, int64_t *captures, int64_t *replays);
+
+/* Captured one-view train step (no counterpart in the reference, whose Metal backend
+ * encodes a step into one command buffer, msplat_metal.mm:1698-1735). With enable = 1,
+ * msplat_hip_train_step runs as a replay of a hipGraph of the whole step: no host wait
+ * inside a step, one launch per step. A step whose shape differs from the current
+ * graph's (gaussian count or buffers, view size, SH degree, background, loss weights,
+ * Adam constants, output pointers, tuning, flags) runs launch by launch and sets the
+ * intersection capacity (1.25 x its count + 4096; with initial_capacity > 0, that value
+ * or the capacity an overflow has raised it to); the next step of that shape captures. Camera poses, the SH bands in use, Adam step sizes and
+ * bias corrections and targets change between replays through device memory. A replay
+ * over the capacity changes nothing; the library finds it at the next call, waits, runs
+ * the replays that were not applied again launch by launch in order, raises the
+ * capacity and recaptures, so the steps are those of the launch-by-launch path. Every
+ * other call on the context (msplat_hip_sync included) first settles the replays in
+ * flight; read parameters only after one of them. With MSPLAT_TUNE_CHUNKS = 0 a graph's
+ * depth-chunk count is fixed at capture from the last launch-by-launch step's count.
+ * Stage timing records nothing for replays. Needs a context with its own stream. */
+int msplat_hip_set_step_graph(msplat_hip_ctx *ctx, int enable, uint64_t initial_capacity);
+/* Captures, replays and replays run again after an overflow, so far. */
+int msplat_hip_step_graph_stats(msplat_hip_ctx *ctx, int64_t *captures, int64_t *replays, int64_t *reruns);
 /* Sizes of the last batched frame on this context: tile intersections and visible
  * records (views x gaussians that touch a tile). */
 int msplat_hip_batch_frame_info(msplat_hip_ctx *ctx, uint64_t *num_isects, uint64_t *num_visible);

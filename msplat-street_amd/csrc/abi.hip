@@ -33,7 +33,8 @@ int msplat::set_error(int code, const char *fmt, ...) {
 namespace {
 
 // projection (per-block intersection and visible counts)
-int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam) {
+int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam,
+               const StepParams *sp = nullptr) {
     const int n = g->num_points;
     const size_t N = (size_t)(n > 0 ? n : 1);
     const int nblocks = (n + 255) / 256;
@@ -68,6 +69,7 @@ int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_
     a.rest = g->features_rest;
     a.opac = g->opacities;
     a.cam = *cam;
+    a.sp = sp;
     a.splat_a = (float4 *)ctx->splat_a.p;
     a.splat_b = (float4 *)ctx->splat_b.p;
     a.splat_blue = (float *)ctx->splat_blue.p;
@@ -90,7 +92,10 @@ int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_
 // and the tile ranges. The host waits for the counts only after the depth sort and
 // the offsets are enqueued (those run on device-side counts against the gaussian
 // capacity), so the device sorts while the host reads M and sizes the key buffers.
-int do_bin(msplat_hip_ctx *ctx) {
+// With an intersection capacity `cap` (a captured step) nothing is read back: every
+// launch after the counts is sized for the capacity and reads the device-side count,
+// and a frame over capacity sets the overflow word and leaves an empty tile list.
+int do_bin(msplat_hip_ctx *ctx, uint32_t cap = 0) {
     if (ctx->frame_stage < msplat_hip_ctx::FRAME_PROJECTED)
         return set_error(MSPLAT_ERR_ARG, "binning needs a projected frame on this context");
     ctx->frame_stage = msplat_hip_ctx::FRAME_PROJECTED;  // the key buffers are rewritten below
@@ -113,13 +118,19 @@ int do_bin(msplat_hip_ctx *ctx) {
     uint32_t *part = (uint32_t *)ctx->scan_part.p;
     const size_t pw = scan_partial_words(std::max(nblocks, vblocks));
     hipStream_t s = ctx->stream;
-    launch_scan_u32_pair((uint32_t *)ctx->block_sums.p,
-                         ScanOut{ctx->d_total64(), ctx->d_total32(), 0xFFFFFFFFu, nullptr, ctx->d_big_count()},
-                         (uint32_t *)ctx->block_vis.p, ScanOut{ctx->d_vis64(), ctx->d_vis32(), 0xFFFFFFFFu, nullptr, nullptr},
+    const ScanOut isects = cap ? ScanOut{ctx->d_total64(), ctx->d_total32(), cap, ctx->d_overflow(), ctx->d_big_count(),
+                                         ctx->d_applied(), ctx->d_isects_running()}
+                               : ScanOut{ctx->d_total64(), ctx->d_total32(), 0xFFFFFFFFu, nullptr, ctx->d_big_count(),
+                                         nullptr, ctx->d_isects_running()};
+    launch_scan_u32_pair((uint32_t *)ctx->block_sums.p, isects, (uint32_t *)ctx->block_vis.p,
+                         ScanOut{ctx->d_vis64(), ctx->d_vis32(), 0xFFFFFFFFu, nullptr, nullptr, nullptr,
+                                 ctx->d_visible_running()},
                          nblocks, part, part + pw, s);
     ctx->end();
-    HIPCHECK(hipMemcpyAsync(ctx->h_total, ctx->scalars.p, 64, hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipEventRecord(ctx->counts_ready, s));
+    if (!cap) {
+        HIPCHECK(hipMemcpyAsync(ctx->h_total, ctx->scalars.p, 64, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipEventRecord(ctx->counts_ready, s));
+    }
     ctx->begin(1);
     // visible gaussians in id order, stably sorted by depth bits
     CompactArgs c;
@@ -166,17 +177,22 @@ int do_bin(msplat_hip_ctx *ctx) {
     k.big_list = ctx->big_list();
     k.big_count = ctx->d_big_count();
     k.tiles_x = ctx->tiles_x;
+    k.overflow = cap ? ctx->d_overflow() : nullptr;
     launch_keygen_offsets(k, s);
     HIPCHECK(hipGetLastError());
-    // the counts: size the key buffers, then keys (tile ids), stable tile sort, ranges
-    HIPCHECK(hipEventSynchronize(ctx->counts_ready));
-    ctx->collect();
-    const uint64_t M = ctx->h_total[0];
-    if (M > 0x7FFFFFFFull)
-        return set_error(MSPLAT_ERR_RANGE, "more than 2^31 tile intersections (%llu); lower the resolution",
-                         (unsigned long long)M);
-    ctx->M = (int64_t)M;
-    ctx->V = (int64_t)ctx->h_total[5];
+    // the counts (or the capacity): size the key buffers, then keys (tile ids), stable
+    // tile sort, ranges
+    uint64_t M = cap;
+    if (!cap) {
+        HIPCHECK(hipEventSynchronize(ctx->counts_ready));
+        ctx->collect();
+        M = ctx->h_total[0];
+        if (M > 0x7FFFFFFFull)
+            return set_error(MSPLAT_ERR_RANGE, "more than 2^31 tile intersections (%llu); lower the resolution",
+                             (unsigned long long)M);
+        ctx->M = (int64_t)M;
+        ctx->V = (int64_t)ctx->h_total[5];
+    }
     const size_t Mc = M > 0 ? M : 1;
     CHECKED(ctx->grow(ctx->keys0, 4 * Mc));
     CHECKED(ctx->grow(ctx->keys1, 4 * Mc));
@@ -287,7 +303,7 @@ int do_raster_forward(msplat_hip_ctx *ctx, const float bg[3]) {
     return MSPLAT_OK;
 }
 
-int do_loss(msplat_hip_ctx *ctx, const float *gt, float ssim_weight, float inv_n) {
+int do_loss(msplat_hip_ctx *ctx, const float *gt, float ssim_weight, float inv_n, const StepParams *sp = nullptr) {
     if (ctx->frame_stage < msplat_hip_ctx::FRAME_RASTERED)
         return set_error(MSPLAT_ERR_ARG, "the loss needs a rendered frame on this context");
     LossArgs l;
@@ -295,6 +311,7 @@ int do_loss(msplat_hip_ctx *ctx, const float *gt, float ssim_weight, float inv_n
     l.height = ctx->height;
     l.img = (const float *)ctx->image.p;
     l.gt = gt;
+    l.sp = sp;
     l.ssim_weight = ssim_weight;
     l.inv_n = inv_n;
     l.v_img = (float *)ctx->v_image.p;
@@ -313,7 +330,7 @@ int do_loss(msplat_hip_ctx *ctx, const float *gt, float ssim_weight, float inv_n
 int do_raster_backward(msplat_hip_ctx *ctx) {
     if (ctx->frame_stage < msplat_hip_ctx::FRAME_RASTERED)
         return set_error(MSPLAT_ERR_ARG, "the backward needs a rendered frame on this context");
-    const size_t Mc = ctx->M > 0 ? (size_t)ctx->M : 1;
+    const size_t Mc = ctx->capturing ? (size_t)ctx->step_cap : ctx->M > 0 ? (size_t)ctx->M : 1;
     CHECKED(ctx->grow(ctx->slot_a, 16 * Mc));
     CHECKED(ctx->grow(ctx->slot_b, 16 * Mc));
     CHECKED(ctx->grow(ctx->slot_c, 4 * Mc));
@@ -383,7 +400,267 @@ int backward_args(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const mspl
     return MSPLAT_OK;
 }
 
+// The step's projection, SH and Adam backward (from the raster backward's slots) and the
+// radii copy; in a capture the camera and Adam scalars come from the device StepParams
+int eager_step_tail(msplat_hip_ctx *ctx, const StepRecord &r) {
+    BackwardArgs b;
+    CHECKED(backward_args(ctx, &r.g, &r.cam, b));
+    for (int i = 0; i < 6; i++) {
+        b.m1[i] = r.adam.exp_avg[i];
+        b.m2[i] = r.adam.exp_avg_sq[i];
+        b.step[i] = r.adam.step_size[i];
+        b.bc2[i] = r.adam.bc2_sqrt[i];
+    }
+    b.beta1 = r.adam.beta1;
+    b.beta2 = r.adam.beta2;
+    b.eps = r.adam.eps;
+    b.vis = r.stats.vis_counts;
+    b.gn = r.stats.xys_grad_norm;
+    b.ms = r.stats.max_2d_size;
+    b.inv_max_dim = r.inv_max;
+    b.sp = ctx->capturing ? (const StepParams *)ctx->step_params.p : nullptr;
+    b.overflow = ctx->capturing ? ctx->d_overflow() : nullptr;
+    ctx->begin(5);
+    launch_backward(b, BWD_ADAM, ctx->stream);
+    ctx->end();
+    HIPCHECK(hipGetLastError());
+    if (r.radii_out && r.g.num_points > 0)
+        HIPCHECK(hipMemcpyAsync(r.radii_out, ctx->radii.p, 4 * (size_t)r.g.num_points, hipMemcpyDeviceToDevice,
+                                ctx->stream));
+    return MSPLAT_OK;
+}
+
+// One train step, launch by launch (the counts are read back to size the key buffers)
+int eager_step(msplat_hip_ctx *ctx, const StepRecord &r, float *loss_out) {
+    CHECKED(do_project(ctx, &r.g, &r.cam));
+    CHECKED(do_bin(ctx));
+    CHECKED(do_raster_forward(ctx, r.bg));
+    CHECKED(do_loss(ctx, r.gt, r.ssim_weight, r.inv_n));
+    CHECKED(do_raster_backward(ctx));
+    CHECKED(eager_step_tail(ctx, r));
+    if (ctx->step_graph_on) ctx->step_m_hint = ctx->M;
+    if (loss_out) {
+        HIPCHECK(hipMemcpyAsync(loss_out, ctx->d_loss(), 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
+    return MSPLAT_OK;
+}
+
+// ── the captured one-view train step ──────────────────────────────────────────
+// A graph holds one step for one shape: gaussian count and buffers, view size, SH
+// degree, background, loss weights, Adam constants, tuning. The per-step values
+// (camera pose, SH bands in use, Adam step sizes and bias corrections, target) travel in
+// a device StepParams block uploaded before each replay. The intersection buffers have a
+// fixed capacity: a replay over capacity sets a sticky overflow word, sees an empty tile
+// list and changes no parameter, moment or statistic, and every later replay does the
+// same until the host clears the word. The host keeps the inputs of the replays it has
+// not yet seen complete (STEP_RING of them): when it finds the word set, it waits for the
+// stream, runs the replays that were not applied again, launch by launch and in order,
+// raises the capacity and recaptures. The result is the same sequence of steps as the
+// launch-by-launch path, with no host wait inside a step.
+
+std::vector<uint64_t> shape_key(msplat_hip_ctx *ctx, const StepRecord &r) {
+    auto bits = [](float f) {
+        uint32_t u;
+        memcpy(&u, &f, 4);
+        return (uint64_t)u;
+    };
+    std::vector<uint64_t> k = {(uint64_t)r.g.num_points, r.cam.width, r.cam.height, r.cam.degree,
+                               bits(r.bg[0]), bits(r.bg[1]), bits(r.bg[2]), bits(r.ssim_weight), bits(r.inv_n),
+                               bits(r.inv_max), bits(r.adam.beta1), bits(r.adam.beta2), bits(r.adam.eps),
+                               (uint64_t)ctx->flags, (uint64_t)ctx->tuning.chunks, (uint64_t)ctx->tuning.bwd_ppt,
+                               (uint64_t)ctx->tuning.fwd_batch, (uint64_t)ctx->tuning.bwd_batch,
+                               (uint64_t)ctx->tuning.cull, (uint64_t)ctx->tuning.pack_keys,
+                               (uint64_t)(uintptr_t)r.radii_out};
+    const void *ptrs[] = {r.g.means, r.g.scales, r.g.quats, r.g.features_dc, r.g.features_rest, r.g.opacities,
+                          r.stats.vis_counts, r.stats.xys_grad_norm, r.stats.max_2d_size};
+    for (const void *p : ptrs) k.push_back((uint64_t)(uintptr_t)p);
+    for (int i = 0; i < 6; i++) {
+        k.push_back((uint64_t)(uintptr_t)r.adam.exp_avg[i]);
+        k.push_back((uint64_t)(uintptr_t)r.adam.exp_avg_sq[i]);
+    }
+    return k;
+}
+
+void drop_step_graph(msplat_hip_ctx *ctx) {
+    if (ctx->step_exec) (void)hipGraphExecDestroy(ctx->step_exec);
+    if (ctx->step_graph) (void)hipGraphDestroy(ctx->step_graph);
+    ctx->step_exec = nullptr;
+    ctx->step_graph = nullptr;
+}
+
+// Depth chunks of a frame with m intersections on the context's current view
+int frame_chunks_for(msplat_hip_ctx *ctx, int64_t m) {
+    const int tc = ctx->tuning.chunks;
+    return m <= 0 ? 1 : tc == 0 ? auto_chunks(ctx->ntiles, m) : tc;
+}
+
+int capture_step(msplat_hip_ctx *ctx, const StepRecord &r) {
+    drop_step_graph(ctx);
+    // the capacity's buffers; every other one was sized by this shape's last eager step
+    const size_t Mc = (size_t)ctx->step_cap, N = (size_t)(r.g.num_points > 0 ? r.g.num_points : 1);
+    CHECKED(ctx->grow(ctx->keys0, 4 * Mc));
+    CHECKED(ctx->grow(ctx->keys1, 4 * Mc));
+    CHECKED(ctx->grow(ctx->vals0, 4 * Mc));
+    CHECKED(ctx->grow(ctx->vals1, 4 * Mc));
+    CHECKED(ctx->grow(ctx->hist, 4 * radix_hist_words((uint32_t)std::max(Mc, N))));
+    CHECKED(ctx->grow(ctx->slot_f, (Mc + 3) & ~(size_t)3));
+    CHECKED(ctx->grow(ctx->slot_a, 16 * Mc));
+    CHECKED(ctx->grow(ctx->slot_b, 16 * Mc));
+    CHECKED(ctx->grow(ctx->slot_c, 4 * Mc));
+    ctx->ntiles = r.cam.tiles_x * r.cam.tiles_y;
+    const int K = frame_chunks_for(ctx, ctx->step_m_hint);
+    if (K > 1) {
+        const size_t planes = (size_t)K * r.cam.width * r.cam.height;
+        CHECKED(ctx->grow(ctx->chunk_prod, 4 * planes));
+        CHECKED(ctx->grow(ctx->chunk_state, 16 * planes));
+        CHECKED(ctx->grow(ctx->chunk_last, 4 * planes));
+    }
+    CHECKED(ctx->grow(ctx->step_params, sizeof(StepParams)));
+    if (!ctx->h_params)
+        HIPCHECK(hipHostMalloc((void **)&ctx->h_params, sizeof(StepParams) * STEP_RING, hipHostMallocDefault));
+    if (!ctx->h_flag) {
+        HIPCHECK(hipHostMalloc((void **)&ctx->h_flag, 64, hipHostMallocDefault));
+        *ctx->h_flag = 0;
+    }
+    for (hipEvent_t &e : ctx->ring_ev)
+        if (!e) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ctx->M = ctx->step_m_hint;  // the chunk choice inside the capture
+    HIPCHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    const StepParams *sp = (const StepParams *)ctx->step_params.p;
+    int rc = do_project(ctx, &r.g, &r.cam, sp);
+    if (rc == MSPLAT_OK) rc = do_bin(ctx, (uint32_t)ctx->step_cap);
+    if (rc == MSPLAT_OK) rc = do_raster_forward(ctx, r.bg);
+    if (rc == MSPLAT_OK) rc = do_loss(ctx, r.gt, r.ssim_weight, r.inv_n, sp);
+    if (rc == MSPLAT_OK) rc = do_raster_backward(ctx);
+    if (rc == MSPLAT_OK) rc = eager_step_tail(ctx, r);
+    const hipError_t launch_err = hipGetLastError();
+    if (rc == MSPLAT_OK && launch_err == hipSuccess)
+        (void)hipMemcpyAsync(ctx->h_flag, ctx->d_overflow(), 4, hipMemcpyDeviceToHost, ctx->stream);
+    ctx->capturing = false;
+    hipGraph_t graph = nullptr;
+    const hipError_t end_err = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != MSPLAT_OK || launch_err != hipSuccess || end_err != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        ctx->frame_stage = msplat_hip_ctx::FRAME_NONE;
+        if (rc != MSPLAT_OK) return rc;
+        return set_error(MSPLAT_ERR_HIP, "train-step capture failed: %s",
+                         hipGetErrorString(launch_err != hipSuccess ? launch_err : end_err));
+    }
+    ctx->step_graph = graph;
+    HIPCHECK(hipGraphInstantiate(&ctx->step_exec, graph, nullptr, nullptr, 0));
+    ctx->step_gen = ctx->generation;
+    ctx->step_captures++;
+    ctx->step_frame = StepFrame{ctx->n,           ctx->width,      ctx->height,        ctx->tiles_x,
+                                ctx->tiles_y,     ctx->ntiles,     ctx->keys_sorted,   ctx->gid_sorted,
+                                ctx->sort_gid_bits, ctx->vis_sorted, ctx->frame_chunks, {ctx->bg[0], ctx->bg[1], ctx->bg[2]}};
+    return MSPLAT_OK;
+}
+
+// The frame state a replay leaves: the captured step's
+void restore_step_frame(msplat_hip_ctx *ctx) {
+    const StepFrame &f = ctx->step_frame;
+    ctx->n = f.n;
+    ctx->width = f.width;
+    ctx->height = f.height;
+    ctx->tiles_x = f.tiles_x;
+    ctx->tiles_y = f.tiles_y;
+    ctx->ntiles = f.ntiles;
+    ctx->keys_sorted = f.keys_sorted;
+    ctx->gid_sorted = f.gid_sorted;
+    ctx->sort_gid_bits = f.sort_gid_bits;
+    ctx->vis_sorted = f.vis_sorted;
+    ctx->frame_chunks = f.chunks;
+    for (int i = 0; i < 3; i++) ctx->bg[i] = f.bg[i];
+    ctx->frame_stage = msplat_hip_ctx::FRAME_RASTERED;
+    ctx->slot_f_clean = false;
+}
+
+// Waits for the replays in flight; runs again, launch by launch, those an overflow left
+// unapplied, raises the capacity for them and drops the graph. Leaves the frame state of
+// the last step.
+int settle_steps(msplat_hip_ctx *ctx) {
+    if (ctx->pending == 0) return MSPLAT_OK;
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    const int pending = ctx->pending;
+    ctx->pending = 0;
+    HIPCHECK(hipMemcpy(ctx->h_total, ctx->scalars.p, 128, hipMemcpyDeviceToHost));
+    const uint32_t *w = (const uint32_t *)ctx->h_total;
+    const uint32_t over = w[16], applied = w[17];
+    ctx->M = (int64_t)ctx->h_total[0];
+    ctx->V = (int64_t)ctx->h_total[5];
+    if (!over) return MSPLAT_OK;
+    const uint32_t unapplied = (uint32_t)ctx->issued - applied;
+    zero_u32<<<1, 64, 0, ctx->stream>>>(ctx->d_overflow(), 2);
+    HIPCHECK(hipGetLastError());
+    *ctx->h_flag = 0;
+    ctx->issued = 0;
+    if (unapplied == 0 || unapplied > (uint32_t)pending)
+        return set_error(MSPLAT_ERR_HIP, "internal: %u unapplied replays of %d in flight", unapplied, pending);
+    drop_step_graph(ctx);
+    int64_t mmax = 0;
+    for (uint32_t i = unapplied; i > 0; i--) {
+        const StepRecord &r = ctx->ring[(ctx->next_slot - i) % STEP_RING];
+        CHECKED(eager_step(ctx, r, nullptr));
+        mmax = std::max(mmax, ctx->M);
+    }
+    ctx->step_reruns += unapplied;
+    ctx->step_cap = std::max<uint64_t>(ctx->step_cap, (uint64_t)mmax + (uint64_t)mmax / 4 + 4096);
+    return MSPLAT_OK;
+}
+
+int graph_step(msplat_hip_ctx *ctx, const StepRecord &r, float *loss_out) {
+    if (!ctx->stream) return set_error(MSPLAT_ERR_ARG, "graph capture needs a non-null stream");
+    if (ctx->pending && *ctx->h_flag) CHECKED(settle_steps(ctx));  // a replay went over capacity
+    const std::vector<uint64_t> shape = shape_key(ctx, r);
+    if (!ctx->step_cap || shape != ctx->step_key) {
+        // a new shape: this step runs launch by launch, sizing the buffers and the capacity
+        CHECKED(settle_steps(ctx));
+        drop_step_graph(ctx);
+        CHECKED(eager_step(ctx, r, loss_out));
+        const uint64_t m = (uint64_t)std::max<int64_t>(ctx->M, 0);
+        ctx->step_cap = ctx->step_cap_request ? std::max<uint64_t>(ctx->step_cap_request, ctx->step_cap)
+                                              : m + m / 4 + 4096;
+        ctx->step_key = shape;
+        return MSPLAT_OK;
+    }
+    const uint64_t slot = ctx->next_slot % STEP_RING;
+    if (ctx->pending == STEP_RING) {  // the slot's replay: wait for it and check it applied
+        HIPCHECK(hipEventSynchronize(ctx->ring_ev[slot]));
+        if (*ctx->h_flag) CHECKED(settle_steps(ctx));  // it (or one after it) was over capacity
+        else ctx->pending--;
+    }
+    if (!ctx->step_exec || ctx->step_gen != ctx->generation) {
+        CHECKED(settle_steps(ctx));
+        CHECKED(capture_step(ctx, r));
+    }
+    StepParams &p = ctx->h_params[slot];
+    p.cam = r.cam;
+    for (int i = 0; i < 6; i++) {
+        p.step[i] = r.adam.step_size[i];
+        p.bc2[i] = r.adam.bc2_sqrt[i];
+    }
+    p.gt = r.gt;
+    ctx->ring[slot] = r;
+    HIPCHECK(hipMemcpyAsync(ctx->step_params.p, &p, sizeof(StepParams), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipGraphLaunch(ctx->step_exec, ctx->stream));
+    HIPCHECK(hipEventRecord(ctx->ring_ev[slot], ctx->stream));
+    ctx->next_slot++;
+    ctx->pending++;
+    ctx->issued++;
+    ctx->step_replays++;
+    restore_step_frame(ctx);
+    if (loss_out) {
+        CHECKED(settle_steps(ctx));  // a re-run (if any) ends with this step, whose loss is then the last
+        HIPCHECK(hipMemcpy(loss_out, ctx->d_loss(), 4, hipMemcpyDeviceToHost));
+    }
+    return MSPLAT_OK;
+}
+
 }  // namespace
+
+int msplat::settle_replays(msplat_hip_ctx *ctx) { return settle_steps(ctx); }
 
 // ═══════════════════════════════ public API ════════════════════════════════
 
@@ -411,12 +688,14 @@ int msplat_hip_ctx_create(int device, void *stream, msplat_hip_ctx **out) {
 }
 
 int msplat_hip_ctx_destroy(msplat_hip_ctx *ctx) {
+    if (ctx) (void)settle_steps(ctx);
     delete ctx;
     return MSPLAT_OK;
 }
 
 int msplat_hip_ctx_set_stream(msplat_hip_ctx *ctx, void *stream) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     if (ctx->own_stream) HIPCHECK(hipStreamDestroy(ctx->stream));
     ctx->own_stream = false;
@@ -428,6 +707,7 @@ void *msplat_hip_ctx_stream(msplat_hip_ctx *ctx) { return ctx ? (void *)ctx->str
 
 int msplat_hip_sync(msplat_hip_ctx *ctx) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     return MSPLAT_OK;
 }
@@ -462,6 +742,7 @@ int msplat_hip_free(void *ptr) {
 
 int msplat_hip_memcpy(msplat_hip_ctx *ctx, void *dst, const void *src, size_t bytes, int kind) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
     if (bytes == 0) return MSPLAT_OK;
     if (kind == 0) {
         HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -479,12 +760,14 @@ int msplat_hip_memcpy(msplat_hip_ctx *ctx, void *dst, const void *src, size_t by
 
 int msplat_hip_memset(msplat_hip_ctx *ctx, void *dst, int value, size_t bytes) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
     if (bytes) HIPCHECK(hipMemsetAsync(dst, value, bytes, ctx->stream));
     return MSPLAT_OK;
 }
 
 int msplat_hip_stage_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
     CHECKED(check_gaussians(g));
     CHECKED(check_camera(cam));
     return do_project(ctx, g, cam);
@@ -492,6 +775,7 @@ int msplat_hip_stage_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g,
 
 int msplat_hip_stage_bin(msplat_hip_ctx *ctx, const msplat_hip_camera *cam) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
     CHECKED(check_camera(cam));
     if (ctx->frame_stage < msplat_hip_ctx::FRAME_PROJECTED)
         return set_error(MSPLAT_ERR_ARG, "binning needs a projected frame on this context");
@@ -509,6 +793,7 @@ int msplat_hip_stage_bin(msplat_hip_ctx *ctx, const msplat_hip_camera *cam) {
 
 int msplat_hip_stage_raster_forward(msplat_hip_ctx *ctx, const msplat_hip_camera *cam, const float background[3]) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
     CHECKED(check_camera(cam));
     if (ctx->frame_stage >= msplat_hip_ctx::FRAME_PROJECTED &&
         (cam->width != ctx->width || cam->height != ctx->height))
@@ -519,6 +804,7 @@ int msplat_hip_stage_raster_forward(msplat_hip_ctx *ctx, const msplat_hip_camera
 int msplat_hip_stage_loss(msplat_hip_ctx *ctx, const msplat_hip_camera *cam, const float *gt, float ssim_weight,
                           float loss_inv_n, float *loss_out) {
     if (!ctx || !gt) return set_error(MSPLAT_ERR_ARG, "ctx or gt is NULL");
+    CHECKED(settle_steps(ctx));
     CHECKED(check_camera(cam));
     CHECKED(do_loss(ctx, gt, ssim_weight, loss_inv_n));
     if (loss_out) {
@@ -531,6 +817,7 @@ int msplat_hip_stage_loss(msplat_hip_ctx *ctx, const msplat_hip_camera *cam, con
 int msplat_hip_render(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam,
                       const float background[3], float **out_img) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
     CHECKED(check_gaussians(g));
     CHECKED(check_camera(cam));
     CHECKED(do_project(ctx, g, cam));
@@ -544,7 +831,8 @@ int msplat_hip_train_step(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, co
                           const float background[3], const float *gt, float ssim_weight, float loss_inv_n,
                           const msplat_hip_adam *adam, const msplat_hip_grad_stats *stats, float inv_max_dim,
                           int32_t *radii_out, float *loss_out) {
-    if (!ctx || !gt || !adam || !stats) return set_error(MSPLAT_ERR_ARG, "ctx, gt, adam or stats is NULL");
+    if (!ctx || !gt || !adam || !stats || !background)
+        return set_error(MSPLAT_ERR_ARG, "ctx, background, gt, adam or stats is NULL");
     CHECKED(check_gaussians(g));
     CHECKED(check_camera(cam));
     for (int i = 0; i < 6; i++)
@@ -552,42 +840,46 @@ int msplat_hip_train_step(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, co
             return set_error(MSPLAT_ERR_ARG, "Adam moment %d is NULL", i);
     if (g->num_points > 0 && cam->degree > 0 && !g->features_rest)
         return set_error(MSPLAT_ERR_ARG, "features_rest is NULL");
-    CHECKED(do_project(ctx, g, cam));
-    CHECKED(do_bin(ctx));
-    CHECKED(do_raster_forward(ctx, background));
-    CHECKED(do_loss(ctx, gt, ssim_weight, loss_inv_n));
-    CHECKED(do_raster_backward(ctx));
-    BackwardArgs b;
-    CHECKED(backward_args(ctx, g, cam, b));
-    for (int i = 0; i < 6; i++) {
-        b.m1[i] = adam->exp_avg[i];
-        b.m2[i] = adam->exp_avg_sq[i];
-        b.step[i] = adam->step_size[i];
-        b.bc2[i] = adam->bc2_sqrt[i];
-    }
-    b.beta1 = adam->beta1;
-    b.beta2 = adam->beta2;
-    b.eps = adam->eps;
-    b.vis = stats->vis_counts;
-    b.gn = stats->xys_grad_norm;
-    b.ms = stats->max_2d_size;
-    b.inv_max_dim = inv_max_dim;
-    ctx->begin(5);
-    launch_backward(b, BWD_ADAM, ctx->stream);
-    ctx->end();
-    HIPCHECK(hipGetLastError());
-    if (radii_out && g->num_points > 0)
-        HIPCHECK(hipMemcpyAsync(radii_out, ctx->radii.p, 4 * (size_t)g->num_points, hipMemcpyDeviceToDevice,
-                                ctx->stream));
-    if (loss_out) {
-        HIPCHECK(hipMemcpyAsync(loss_out, ctx->d_loss(), 4, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHECK(hipStreamSynchronize(ctx->stream));
-    }
+    StepRecord r;
+    r.g = *g;
+    r.cam = *cam;
+    for (int i = 0; i < 3; i++) r.bg[i] = background[i];
+    r.gt = gt;
+    r.ssim_weight = ssim_weight;
+    r.inv_n = loss_inv_n;
+    r.inv_max = inv_max_dim;
+    r.adam = *adam;
+    r.stats = *stats;
+    r.radii_out = radii_out;
+    if (ctx->step_graph_on) return graph_step(ctx, r, loss_out);
+    CHECKED(settle_steps(ctx));
+    return eager_step(ctx, r, loss_out);
+}
+
+int msplat_hip_set_step_graph(msplat_hip_ctx *ctx, int enable, uint64_t initial_capacity) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    if (enable && !ctx->stream) return set_error(MSPLAT_ERR_ARG, "graph capture needs a non-null stream");
+    if (initial_capacity > 0x7FFFFFFFull) return set_error(MSPLAT_ERR_RANGE, "capacity above 2^31 intersections");
+    CHECKED(settle_steps(ctx));
+    drop_step_graph(ctx);
+    ctx->step_graph_on = enable != 0;
+    ctx->step_cap_request = (uint32_t)initial_capacity;
+    ctx->step_cap = 0;  // the next step runs launch by launch and sizes the capacity
+    ctx->step_key.clear();
+    return MSPLAT_OK;
+}
+
+int msplat_hip_step_graph_stats(msplat_hip_ctx *ctx, int64_t *captures, int64_t *replays, int64_t *reruns) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    if (captures) *captures = ctx->step_captures;
+    if (replays) *replays = ctx->step_replays;
+    if (reruns) *reruns = ctx->step_reruns;
     return MSPLAT_OK;
 }
 
 int msplat_hip_last_loss(msplat_hip_ctx *ctx, float *loss) {
     if (!ctx || !loss) return set_error(MSPLAT_ERR_ARG, "ctx or loss is NULL");
+    CHECKED(settle_steps(ctx));
     if (!ctx->scalars.p) return set_error(MSPLAT_ERR_ARG, "no train step has run on this context");
     HIPCHECK(hipMemcpyAsync(loss, ctx->d_loss(), 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
@@ -597,6 +889,7 @@ int msplat_hip_last_loss(msplat_hip_ctx *ctx, float *loss) {
 int msplat_hip_render_backward(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam,
                                const float background[3], const float *v_img, const msplat_hip_grads *out) {
     if (!ctx || !v_img || !out) return set_error(MSPLAT_ERR_ARG, "ctx, v_img or out is NULL");
+    CHECKED(settle_steps(ctx));
     CHECKED(check_gaussians(g));
     CHECKED(check_camera(cam));
     if (ctx->frame_stage < msplat_hip_ctx::FRAME_RASTERED)
@@ -641,6 +934,7 @@ int msplat_hip_render_backward(msplat_hip_ctx *ctx, const msplat_hip_gaussians *
 
 int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *f) {
     if (!ctx || !f) return set_error(MSPLAT_ERR_ARG, "ctx or out is NULL");
+    CHECKED(settle_steps(ctx));
     memset(f, 0, sizeof *f);
     f->num_points = ctx->n;
     f->num_tiles = ctx->ntiles;
@@ -680,6 +974,8 @@ int msplat_hip_frame_view(msplat_hip_ctx *ctx, msplat_hip_frame *f) {
     f->v_image = (float *)ctx->v_image.p;
     f->loss_sum = ctx->scalars.p ? ctx->d_loss_sum() : nullptr;
     f->slots_written = ctx->scalars.p ? (uint64_t *)ctx->d_slots_written() : nullptr;
+    f->isects_running = ctx->scalars.p ? ctx->d_isects_running() : nullptr;
+    f->visible_running = ctx->scalars.p ? ctx->d_visible_running() : nullptr;
     f->v_xy = (float *)ctx->v_xy.p;
     f->v_conic = (float *)ctx->v_conic.p;
     f->v_rgb = (float *)ctx->v_rgb.p;
@@ -695,6 +991,7 @@ int msplat_hip_set_flags(msplat_hip_ctx *ctx, int flags) {
 
 int msplat_hip_radix_sort(msplat_hip_ctx *ctx, uint64_t *keys, int32_t *vals, uint32_t n, int key_bits) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
     if (key_bits < 0 || key_bits > 64) return set_error(MSPLAT_ERR_ARG, "key_bits must be in [0, 64]");
     if (n == 0 || key_bits == 0) return MSPLAT_OK;
     if (!keys || !vals) return set_error(MSPLAT_ERR_ARG, "keys or vals is NULL");
@@ -735,6 +1032,7 @@ int msplat_hip_densify(msplat_hip_ctx *ctx, int32_t n, int32_t capacity, float g
                        const float *max_2d_size, float half_max_dim, float *const bufs[18], int32_t fr_stride,
                        uint32_t seed, int32_t *new_count) {
     if (!ctx || !bufs || !new_count) return set_error(MSPLAT_ERR_ARG, "ctx, bufs or new_count is NULL");
+    CHECKED(settle_steps(ctx));
     if (n < 0 || fr_stride < 1) return set_error(MSPLAT_ERR_ARG, "bad gaussian count or rest stride");
     if ((int64_t)3 * n > capacity) return set_error(MSPLAT_ERR_CAPACITY, "msplat_densify: 3*N exceeds buf_capacity");
     for (int i = 0; i < 18; i++)
@@ -815,6 +1113,7 @@ int msplat_hip_densify(msplat_hip_ctx *ctx, int32_t n, int32_t capacity, float g
 int msplat_hip_opacity_reset(msplat_hip_ctx *ctx, float *opacities, float *exp_avg, float *exp_avg_sq, int32_t n,
                              float reset_logit) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
     if (n > 0 && (!opacities || !exp_avg || !exp_avg_sq)) return set_error(MSPLAT_ERR_ARG, "NULL buffer");
     launch_opacity_reset(opacities, exp_avg, exp_avg_sq, n, reset_logit, ctx->stream);
     HIPCHECK(hipGetLastError());
@@ -823,6 +1122,7 @@ int msplat_hip_opacity_reset(msplat_hip_ctx *ctx, float *opacities, float *exp_a
 
 int msplat_hip_knn3_mean_dist(msplat_hip_ctx *ctx, const float *points, uint32_t n, float *mean_dist) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
     if (n == 0) return MSPLAT_OK;
     if (!points || !mean_dist) return set_error(MSPLAT_ERR_ARG, "points or mean_dist is NULL");
     CHECKED(ctx->grow(ctx->aux, knn_scratch_bytes(n) + 1024));
@@ -837,6 +1137,7 @@ int msplat_hip_init_gaussians(msplat_hip_ctx *ctx, const float *means, const uin
                               float opacity_logit, float *scales, float *quats, float *features_dc,
                               float *opacities) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
     if (n == 0) return MSPLAT_OK;
     if (!means || !rgb || !scales || !quats || !features_dc || !opacities)
         return set_error(MSPLAT_ERR_ARG, "NULL buffer");
@@ -855,6 +1156,7 @@ int msplat_hip_init_gaussians(msplat_hip_ctx *ctx, const float *means, const uin
 int msplat_hip_image_metrics(msplat_hip_ctx *ctx, const float *rendered, const float *gt, int h, int w,
                              double out[3]) {
     if (!ctx || !out) return set_error(MSPLAT_ERR_ARG, "ctx or out is NULL");
+    CHECKED(settle_steps(ctx));
     out[0] = out[1] = out[2] = 0.0;
     if (h <= 0 || w <= 0) return MSPLAT_OK;
     if (!rendered || !gt) return set_error(MSPLAT_ERR_ARG, "NULL image");
@@ -871,6 +1173,7 @@ int msplat_hip_image_metrics(msplat_hip_ctx *ctx, const float *rendered, const f
 
 int msplat_hip_pack_rgba8(msplat_hip_ctx *ctx, const float *img, uint32_t n, uint8_t *out) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
     if (n == 0) return MSPLAT_OK;
     if (!img || !out) return set_error(MSPLAT_ERR_ARG, "img or out is NULL");
     launch_pack_rgba8(img, n, out, ctx->stream);

@@ -91,6 +91,7 @@ template <int MODE>
 __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g >= a.n) return;
+    if (MODE == BWD_ADAM && a.overflow && *a.overflow) return;  // over capacity: nothing changes
     // the dense Adam groups' parameters and moments, loaded before the slot sums so that
     // their loads are in flight during the visible path's dependent loads
     float dp[11], d1[11], d2[11];
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
         for (int i = 0; i < 3; i++) { a.v_conic[3 * g + i] = vcn[i]; a.v_rgb[3 * g + i] = vrgb[i]; }
         a.v_opac[g] = vop;
     }
-    const msplat_hip_camera &cam = a.cam;
+    const msplat_hip_camera &cam = a.sp ? a.sp->cam : a.cam;
     float vm[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
     float dirv[3] = {0.f, 0.f, 0.f};
     if (vis) {
@@ -178,21 +179,22 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
     }
 
     const float b1 = a.beta1, b2 = a.beta2, eps = a.eps;
+    const float *stp = a.sp ? a.sp->step : a.step, *bc2 = a.sp ? a.sp->bc2 : a.bc2;
     if (vis) {  // SH DC: visible gaussians only
         for (int c = 0; c < 3; c++) {
             const size_t i = 3 * (size_t)g + c;
-            adam_step(a.dc[i], a.m1[3][i], a.m2[3][i], SH_C0 * vrgb[c], a.step[3], b1, b2, a.bc2[3], eps);
+            adam_step(a.dc[i], a.m1[3][i], a.m2[3][i], SH_C0 * vrgb[c], stp[3], b1, b2, bc2[3], eps);
         }
     }
     // dense groups: means, scales, quats, opacity (from the values loaded at the start)
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const size_t k = 3 * (size_t)g + i;
-        adam_step(dp[i], d1[i], d2[i], vm[i], a.step[0], b1, b2, a.bc2[0], eps);
+        adam_step(dp[i], d1[i], d2[i], vm[i], stp[0], b1, b2, bc2[0], eps);
         a.means_rw[k] = dp[i];
         a.m1[0][k] = d1[i];
         a.m2[0][k] = d2[i];
-        adam_step(dp[3 + i], d1[3 + i], d2[3 + i], vs[i], a.step[1], b1, b2, a.bc2[1], eps);
+        adam_step(dp[3 + i], d1[3 + i], d2[3 + i], vs[i], stp[1], b1, b2, bc2[1], eps);
         a.scales[k] = dp[3 + i];
         a.m1[1][k] = d1[3 + i];
         a.m2[1][k] = d2[3 + i];
@@ -200,12 +202,12 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const size_t k = 4 * (size_t)g + i;
-        adam_step(dp[6 + i], d1[6 + i], d2[6 + i], vq[i], a.step[2], b1, b2, a.bc2[2], eps);
+        adam_step(dp[6 + i], d1[6 + i], d2[6 + i], vq[i], stp[2], b1, b2, bc2[2], eps);
         a.quats[k] = dp[6 + i];
         a.m1[2][k] = d1[6 + i];
         a.m2[2][k] = d2[6 + i];
     }
-    adam_step(dp[10], d1[10], d2[10], vop, a.step[5], b1, b2, a.bc2[5], eps);
+    adam_step(dp[10], d1[10], d2[10], vop, stp[5], b1, b2, bc2[5], eps);
     a.opac[g] = dp[10];
     a.m1[5][g] = d1[10];
     a.m2[5][g] = d2[10];
@@ -305,8 +307,11 @@ __global__ __launch_bounds__(256) void sh_rest_kernel(BackwardArgs a) {
 // gaussians outside the view, whose coefficients the fused SH Adam leaves alone, cost
 // no threads. A gaussian's coefficients are contiguous, so its threads stay coalesced.
 __global__ __launch_bounds__(256) void sh_rest_adam_kernel(BackwardArgs a) {
-    const uint32_t stride = (sh_bases(a.cam.degree) - 1) * 3;
-    const int dtu = (int)min(a.cam.degrees_to_use, 3u);
+    if (a.overflow && *a.overflow) return;  // over capacity: nothing changes
+    const msplat_hip_camera &cam = a.sp ? a.sp->cam : a.cam;
+    const float step = a.sp ? a.sp->step[4] : a.step[4], bc2 = a.sp ? a.sp->bc2[4] : a.bc2[4];
+    const uint32_t stride = (sh_bases(cam.degree) - 1) * 3;
+    const int dtu = (int)min(cam.degrees_to_use, 3u);
     const uint32_t nbasis = dtu >= 3 ? 15 : dtu == 2 ? 8 : dtu == 1 ? 3 : 0;
     const size_t total = (size_t)*a.vis_count * stride;
     for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
@@ -320,7 +325,7 @@ __global__ __launch_bounds__(256) void sh_rest_adam_kernel(BackwardArgs a) {
         const float4 V = a.sh_vrgb[g];
         const float vc = c == 0 ? V.x : c == 1 ? V.y : V.z;
         const size_t k = (size_t)g * stride + j;
-        adam_step(a.rest[k], a.m1[4][k], a.m2[4][k], basis[b] * vc, a.step[4], a.beta1, a.beta2, a.bc2[4], a.eps);
+        adam_step(a.rest[k], a.m1[4][k], a.m2[4][k], basis[b] * vc, step, a.beta1, a.beta2, bc2, a.eps);
     }
 }
 

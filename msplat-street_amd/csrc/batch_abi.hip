@@ -589,6 +589,7 @@ extern "C" {
 int msplat_hip_render_batch(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_views *views,
                             float **out_images, uint64_t *pixel_offsets) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_replays(ctx));
     CHECKED(check_views(g, views, false));
     BatchState *b;
     CHECKED(state(ctx, b));
@@ -614,6 +615,7 @@ int msplat_hip_train_step_batch(msplat_hip_ctx *ctx, const msplat_hip_gaussians 
                                 float ssim_weight, const msplat_hip_adam *adam, const msplat_hip_grad_stats *stats,
                                 float *losses) {
     if (!ctx || !adam || !stats) return set_error(MSPLAT_ERR_ARG, "ctx, adam or stats is NULL");
+    CHECKED(settle_replays(ctx));
     CHECKED(check_views(g, views, true));
     BatchState *b;
     CHECKED(state(ctx, b));
@@ -638,6 +640,7 @@ int msplat_hip_train_step_batch(msplat_hip_ctx *ctx, const msplat_hip_gaussians 
 int msplat_hip_render_backward_batch(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_views *views,
                                      const float *v_images, const msplat_hip_grads *out) {
     if (!ctx || !v_images || !out) return set_error(MSPLAT_ERR_ARG, "ctx, v_images or out is NULL");
+    CHECKED(settle_replays(ctx));
     CHECKED(check_views(g, views, false));
     BatchState *b = ctx->batch;
     if (!b || !b->rendered || b->frame_n != g->num_points || b->views != views->num_views)
@@ -679,6 +682,7 @@ int msplat_hip_batch_graph_stats(msplat_hip_ctx *ctx, int64_t *captures, int64_t
 
 int msplat_hip_batch_frame_info(msplat_hip_ctx *ctx, uint64_t *num_isects, uint64_t *num_visible) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_replays(ctx));
     BatchState *b = ctx->batch;
     if (num_isects) *num_isects = b ? b->h_sc->total : 0;
     if (num_visible) *num_visible = b ? b->h_sc->nvis64 : 0;

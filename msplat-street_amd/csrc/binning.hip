@@ -50,6 +50,7 @@ __device__ inline uint32_t block_scan_u32(uint32_t v, uint32_t &total, uint32_t 
 // Also writes the workgroup's intersection count and visible-record count.
 __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
     __shared__ uint32_t lds4[4];
+    const msplat_hip_camera &cam = a.sp ? a.sp->cam : a.cam;
     const int g = blockIdx.x * 256 + threadIdx.x;
     uint32_t count = 0;
     if (g < a.n) {
@@ -57,25 +58,25 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
         a.count[g] = 0;
         const float p[3] = {a.means[3 * g], a.means[3 * g + 1], a.means[3 * g + 2]};
         float pv[3];
-        cam_space(a.cam.viewmat, p, pv);
-        if (pv[2] > a.cam.clip_thresh) {
+        cam_space(cam.viewmat, p, pv);
+        if (pv[2] > cam.clip_thresh) {
             const float s[3] = {exp_cr(a.scales[3 * g]), exp_cr(a.scales[3 * g + 1]), exp_cr(a.scales[3 * g + 2])};
             const float q[4] = {a.quats[4 * g], a.quats[4 * g + 1], a.quats[4 * g + 2], a.quats[4 * g + 3]};
             float cov[6], c2[3], conic[3], radius;
-            covariance3d(s, a.cam.glob_scale, q, cov);
-            const float tfx = 0.5f * (float)a.cam.width / a.cam.fx, tfy = 0.5f * (float)a.cam.height / a.cam.fy;
-            ewa_cov2d(cov, a.cam.viewmat, a.cam.fx, a.cam.fy, tfx, tfy, pv, c2);
+            covariance3d(s, cam.glob_scale, q, cov);
+            const float tfx = 0.5f * (float)cam.width / cam.fx, tfy = 0.5f * (float)cam.height / cam.fy;
+            ewa_cov2d(cov, cam.viewmat, cam.fx, cam.fy, tfx, tfy, pv, c2);
             if (conic_radius(c2, conic, radius)) {
                 float px, py;
-                to_pixel(a.cam.projmat, p, a.cam.width, a.cam.height, a.cam.cx, a.cam.cy, px, py);
+                to_pixel(cam.projmat, p, cam.width, cam.height, cam.cx, cam.cy, px, py);
                 const float ax = ceilf(3.f * sqrtf(c2[0])), ay = ceilf(3.f * sqrtf(c2[2]));
                 int x0, y0, x1, y1;
-                tile_rect(px, py, ax, ay, a.cam.tiles_x, a.cam.tiles_y, x0, y0, x1, y1);
+                tile_rect(px, py, ax, ay, cam.tiles_x, cam.tiles_y, x0, y0, x1, y1);
                 if ((x1 - x0) * (y1 - y0) > 0) {
                     float d[3], col[3];
-                    view_direction(p, a.cam.cam_pos, d);
-                    const uint32_t nb = sh_bases(a.cam.degree);
-                    sh_color(a.cam.degrees_to_use, d, a.dc + 3 * (size_t)g, a.rest + (size_t)(nb - 1) * 3 * g, col);
+                    view_direction(p, cam.cam_pos, d);
+                    const uint32_t nb = sh_bases(cam.degree);
+                    sh_color(cam.degrees_to_use, d, a.dc + 3 * (size_t)g, a.rest + (size_t)(nb - 1) * 3 * g, col);
                     const float opac = 1.f / (1.f + exp_cr(-a.opac[g]));
                     a.splat_a[g] = make_float4(px, py, conic[0], conic[1]);
                     a.splat_b[g] = make_float4(conic[2], opac, col[0], col[1]);
@@ -170,7 +171,8 @@ __global__ __launch_bounds__(256) void scan_partial_kernel(const uint32_t *a, ui
 }
 
 // Scans the chunk totals; the grand total goes to total/total32. Over `cap`, total32 is
-// 0 and *overflow is set (when overflow is given).
+// 0 and *overflow is set (when overflow is given; with `applied`, a set overflow also
+// counts as over capacity).
 __global__ __launch_bounds__(256) void scan_top_kernel(uint32_t *partial, int nchunks, ScanOut o) {
     __shared__ uint32_t lds4[4];
     __shared__ uint64_t carry;
@@ -188,11 +190,13 @@ __global__ __launch_bounds__(256) void scan_top_kernel(uint32_t *partial, int nc
     }
     if (threadIdx.x == 0) {
         const uint64_t tot = carry;
-        const bool over = tot > (uint64_t)o.cap;
+        const bool over = tot > (uint64_t)o.cap || (o.applied && *o.overflow);
         if (o.total) *o.total = tot;
         if (o.total32) *o.total32 = over ? (o.overflow ? 0u : 0xFFFFFFFFu) : (uint32_t)tot;
         if (o.overflow) *o.overflow = over ? 1u : 0u;
         if (o.zero_also) *o.zero_also = 0u;
+        if (o.applied && !over) *o.applied += 1u;
+        if (o.running) *o.running += tot;
     }
 }
 
@@ -226,11 +230,13 @@ __device__ inline void scan_small(uint32_t *a, uint32_t L, const ScanOut &o) {
     }
     if (threadIdx.x == 0) {
         const uint64_t tot = carry;
-        const bool over = tot > (uint64_t)o.cap;
+        const bool over = tot > (uint64_t)o.cap || (o.applied && *o.overflow);
         if (o.total) *o.total = tot;
         if (o.total32) *o.total32 = over ? (o.overflow ? 0u : 0xFFFFFFFFu) : (uint32_t)tot;
         if (o.overflow) *o.overflow = over ? 1u : 0u;
         if (o.zero_also) *o.zero_also = 0u;
+        if (o.applied && !over) *o.applied += 1u;
+        if (o.running) *o.running += tot;
     }
 }
 

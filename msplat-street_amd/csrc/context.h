@@ -67,6 +67,38 @@ struct Buf {
     size_t cap = 0;
 };
 
+namespace msplat {
+// Replays of the captured one-view train step that may be in flight at once; the inputs
+// of each are kept until it is known to have been applied (an over-capacity replay is
+// run again from them).
+constexpr int STEP_RING = 4;
+
+// The frame state a captured step leaves behind (restored after each replay)
+struct StepFrame {
+    int n;
+    uint32_t width, height;
+    int tiles_x, tiles_y, ntiles;
+    uint32_t *keys_sorted;
+    int32_t *gid_sorted;
+    int sort_gid_bits;
+    int32_t *vis_sorted;
+    int chunks;
+    float bg[3];
+};
+
+// The inputs of one train step (msplat_hip_train_step's arguments)
+struct StepRecord {
+    msplat_hip_gaussians g;
+    msplat_hip_camera cam;
+    float bg[3];
+    const float *gt;
+    float ssim_weight, inv_n, inv_max;
+    msplat_hip_adam adam;
+    msplat_hip_grad_stats stats;
+    int32_t *radii_out;
+};
+}  // namespace msplat
+
 struct msplat_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -88,6 +120,28 @@ struct msplat_hip_ctx {
     Buf aux;           // densify flags/scans/scratch, kNN and metrics workspaces
     int32_t *vis_sorted = nullptr;  // visible gaussians in depth order (the frame's list)
     msplat::BatchState *batch = nullptr;  // batched-view frame and graph (created on first use)
+    // captured one-view train step (msplat_hip_set_step_graph)
+    bool step_graph_on = false;
+    bool capturing = false;         // a capture is being recorded: no allocation, no timing
+    uint32_t step_cap_request = 0;  // the caller's intersection capacity (0: from the counts)
+    uint64_t step_cap = 0;          // the capacity of the current graph (0: none yet)
+    int64_t step_m_hint = 0;        // intersections of the last eager step (the chunk choice)
+    hipGraph_t step_graph = nullptr;
+    hipGraphExec_t step_exec = nullptr;
+    std::vector<uint64_t> step_key;
+    Buf step_params;                       // device StepParams the replays read
+    msplat::StepParams *h_params = nullptr;  // pinned, STEP_RING slots
+    uint32_t *h_flag = nullptr;            // pinned: the overflow word each replay copies back
+    hipEvent_t ring_ev[msplat::STEP_RING] = {};
+    msplat::StepRecord ring[msplat::STEP_RING];
+    uint64_t issued = 0;   // replays since the device's applied count was cleared
+    int pending = 0;       // replays not yet settled, the newest `pending` ring entries
+    uint64_t next_slot = 0;
+    int64_t step_captures = 0, step_replays = 0, step_reruns = 0;
+    uint64_t generation = 0;  // bumped whenever a workspace buffer moves
+    uint64_t step_gen = 0;    // the generation the current graph was captured at
+    msplat::StepFrame step_frame = {};
+
     // stage timing
     bool timing = false;
     uint32_t stage_mask = ~0u;  // stages timed while timing is on (bit s = stage s)
@@ -113,13 +167,13 @@ struct msplat_hip_ctx {
         return e;
     }
     void begin(int stage) {
-        if (!timing || !((stage_mask >> stage) & 1u)) return;
+        if (!timing || capturing || !((stage_mask >> stage) & 1u)) return;
         open_stage = stage;
         open_event = take_event();
         if (open_event) (void)hipEventRecord(open_event, stream);
     }
     void end() {
-        if (!timing || open_stage < 0 || !open_event) return;
+        if (!timing || capturing || open_stage < 0 || !open_event) return;
         hipEvent_t e = take_event();
         if (!e) return;
         (void)hipEventRecord(e, stream);
@@ -146,7 +200,9 @@ struct msplat_hip_ctx {
     }
     // device scalars: [0] u64 isect total, [8] u32 total, [16] double loss sum, [24] float loss,
     // [32] u32 big count, [40] u64 / [48] u32 visible gaussians, [56] u64 running count of
-    // written gradient slots (zeroed at allocation)
+    // written gradient slots, [64] u32 overflow of a captured step, [68] u32 replays applied
+    // since it was cleared, [72] u64 / [80] u64 running counts of the one-view frames'
+    // intersections and visible gaussians (zeroed at allocation)
     Buf scalars;
     uint64_t *h_total = nullptr;  // pinned
     hipEvent_t counts_ready = nullptr;  // h_total holds this frame's counts
@@ -169,6 +225,8 @@ struct msplat_hip_ctx {
     int grow(Buf &b, size_t bytes) {
         if (bytes == 0) bytes = 16;
         if (b.cap >= bytes) return MSPLAT_OK;
+        if (capturing) return msplat::set_error(MSPLAT_ERR_ARG, "internal: a buffer grows during graph capture");
+        generation++;
         if (b.p) {
             HIPCHECK(hipStreamSynchronize(stream));
             HIPCHECK(hipFree(b.p));
@@ -196,6 +254,13 @@ struct msplat_hip_ctx {
                       &chunk_last};
         for (Buf *b : all) release(*b);
         if (batch) msplat::destroy_batch_state(batch);
+        if (step_exec) (void)hipGraphExecDestroy(step_exec);
+        if (step_graph) (void)hipGraphDestroy(step_graph);
+        release(step_params);
+        if (h_params) (void)hipHostFree(h_params);
+        if (h_flag) (void)hipHostFree(h_flag);
+        for (hipEvent_t e : ring_ev)
+            if (e) (void)hipEventDestroy(e);
         if (h_total) (void)hipHostFree(h_total);
         if (counts_ready) (void)hipEventDestroy(counts_ready);
         collect();
@@ -213,5 +278,15 @@ struct msplat_hip_ctx {
     uint32_t *d_vis32() { return (uint32_t *)((char *)scalars.p + 48); }
     float *d_loss() { return (float *)((char *)scalars.p + 24); }
     unsigned long long *d_slots_written() { return (unsigned long long *)((char *)scalars.p + 56); }
+    uint32_t *d_overflow() { return (uint32_t *)((char *)scalars.p + 64); }
+    uint32_t *d_applied() { return (uint32_t *)((char *)scalars.p + 68); }
+    uint64_t *d_isects_running() { return (uint64_t *)((char *)scalars.p + 72); }
+    uint64_t *d_visible_running() { return (uint64_t *)((char *)scalars.p + 80); }
 };
 
+namespace msplat {
+// Settles the captured train-step replays in flight on the context (waits for them and
+// runs again those an intersection overflow left unapplied). Every entry point that reads
+// or writes a context's frame, or parameters a step may update, calls it first.
+int settle_replays(msplat_hip_ctx *ctx);
+}  // namespace msplat

@@ -96,6 +96,7 @@ extern "C" {
 int msplat_hip_resize_area(msplat_hip_ctx *ctx, const void *src, int src_is_u8, int sw, int sh, float *dst, int dw,
                            int dh) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_replays(ctx));
     if (dw <= 0 || dh <= 0) return MSPLAT_OK;  // as the reference: nothing to write
     if (!src || !dst) return set_error(MSPLAT_ERR_ARG, "src or dst is NULL");
     if (sw <= 0 || sh <= 0) return set_error(MSPLAT_ERR_ARG, "source size must be positive");
@@ -111,6 +112,7 @@ int msplat_hip_resize_area(msplat_hip_ctx *ctx, const void *src, int src_is_u8, 
 int msplat_hip_undistort(msplat_hip_ctx *ctx, const float *src, int sw, int sh, const float intr[4],
                          const float dist[5], int roi_x, int roi_y, float *dst, int dw, int dh) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_replays(ctx));
     if (dw <= 0 || dh <= 0) return MSPLAT_OK;
     if (!src || !dst || !intr || !dist) return set_error(MSPLAT_ERR_ARG, "NULL argument");
     if (sw <= 0 || sh <= 0) return set_error(MSPLAT_ERR_ARG, "source size must be positive");

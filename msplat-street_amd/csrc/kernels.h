@@ -21,10 +21,20 @@ struct Tuning {
     int pack_keys = 1;    // one-view tile sort on (tile | id) keys when they fit in 32 bits
 };
 
+// Per-step values of a captured one-view train step, in device memory: uploaded before
+// each replay and read by the kernels in place of their by-value copies, so that one
+// graph serves every view of the same size.
+struct StepParams {
+    msplat_hip_camera cam;
+    float step[6], bc2[6];  // Adam step sizes and bias corrections
+    const float *gt;        // the target image
+};
+
 struct ProjectArgs {
     int n;
     const float *means, *scales, *quats, *dc, *rest, *opac;
     msplat_hip_camera cam;
+    const StepParams *sp = nullptr;  // non-null: the camera is sp->cam
     float4 *splat_a, *splat_b;
     float *splat_blue, *depths;
     float2 *aabb;
@@ -45,6 +55,9 @@ struct ScanOut {
     uint32_t cap;
     uint32_t *overflow;
     uint32_t *zero_also;  // a counter cleared by the same launch (or null)
+    uint32_t *applied;    // null, or: an overflow already set stays set (this scan is over
+                          // capacity too), and a scan within capacity adds one here
+    uint64_t *running;    // null, or the total is added here (a running count over frames)
 };
 
 // Visible records (count > 0) in id order with their depth bits as sort keys
@@ -120,6 +133,7 @@ inline uint32_t loss_blocks(uint32_t W, uint32_t H) {
 struct LossArgs {
     uint32_t width, height;
     const float *img, *gt;
+    const StepParams *sp = nullptr;  // non-null: the target is sp->gt
     float ssim_weight, inv_n;
     float *v_img;
     float *scratch;     // LOSS_PLANES planes of W x H floats
@@ -135,6 +149,8 @@ struct BackwardArgs {
     const float *means;
     float *scales, *quats, *dc, *rest, *opac, *means_rw;  // updated in place in Adam mode
     msplat_hip_camera cam;
+    const StepParams *sp;      // non-null: camera, step sizes and bias corrections from here
+    const uint32_t *overflow;  // non-null and set: the frame was over capacity, Adam does nothing
     const int32_t *radii;
     const uint32_t *count, *offsets;
     const float4 *splat_a, *splat_b;

@@ -36,6 +36,9 @@ __device__ constexpr float GW[11] = {0.0010283801f, 0.0075987581f, 0.0360007721f
                                      0.2660117249f, 0.2130055377f, 0.1093606895f, 0.0360007721f, 0.0075987581f,
                                      0.0010283801f};
 
+// The target image: the per-step value of a captured step when there is one
+__device__ inline const float *target(const LossArgs &a) { return a.sp ? a.sp->gt : a.gt; }
+
 // ssim_h_fwd for row y, columns [X0, X0 + 118): the row's pixels from X0 - 5 to
 // X0 + 122 are staged in LDS (zero outside the image), then thread t adds the 11 taps of
 // column X0 + t per channel. Plane 5c + q holds moment q (mu_x, mu_y, E[x^2], E[y^2],
@@ -50,7 +53,7 @@ __device__ inline void ssim_h_row(const LossArgs &a, int X0, int y) {
         float gv = 0.f, rv = 0.f;
         if (x >= 0 && x < W) {
             const size_t idx = ((size_t)y * W + x) * 3 + i % 3;
-            gv = a.gt[idx];
+            gv = target(a)[idx];
             rv = a.img[idx];
         }
         sg[i] = gv;
@@ -193,7 +196,7 @@ __device__ inline void ssim_vb_block(const LossArgs &a, int X0, int Y0, int c) {
         const int y = Y0 + r;
         if (y >= H) break;
         const size_t i = ((size_t)y * W + x) * 3 + c;
-        const float rv = a.img[i], gv = a.gt[i];
+        const float rv = a.img[i], gv = target(a)[i];
         const float vs = cf[r][0] + rv * cf[r][1] + gv * cf[r][2];
         const float vl = (gv > rv) ? -1.0f : ((gv < rv) ? 1.0f : 0.0f);
         a.v_img[i] = a.inv_n * (-w * vs + (1.0f - w) * vl);
@@ -231,7 +234,7 @@ __device__ inline float pixel_loss(const LossArgs &a, size_t P, size_t p) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         s += a.scratch[(size_t)(PL_SSIM + c) * P + p];
-        l += fabsf(a.gt[3 * p + c] - a.img[3 * p + c]);
+        l += fabsf(target(a)[3 * p + c] - a.img[3 * p + c]);
     }
     const float w = a.ssim_weight;
     return w * (1.0f - s / 3.0f) + (1.0f - w) * l / 3.0f;

@@ -76,7 +76,7 @@ class Frame(C.Structure):
                 ("v_image", C.c_void_p), ("final_idx", C.c_void_p), ("v_xy", C.c_void_p),
                 ("v_conic", C.c_void_p), ("v_rgb", C.c_void_p), ("v_opacity", C.c_void_p),
                 ("loss_sum", C.c_void_p), ("slots_written", C.c_void_p), ("num_visible", C.c_int64),
-                ("depth_chunks", C.c_int32)]
+                ("depth_chunks", C.c_int32), ("isects_running", C.c_void_p), ("visible_running", C.c_void_p)]
 
 
 class Checkpoint(C.Structure):
@@ -155,6 +155,9 @@ def lib():
                                                   C.POINTER(Grads)], C.c_int),
             "msplat_hip_set_batch_graph": ([vp, C.c_int], C.c_int),
             "msplat_hip_batch_graph_stats": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+            "msplat_hip_set_step_graph": ([vp, C.c_int, C.c_uint64], C.c_int),
+            "msplat_hip_step_graph_stats": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+                                            C.c_int),
             "msplat_hip_batch_frame_info": ([vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
         }
         for name, (args, res) in sig.items():
@@ -224,6 +227,16 @@ class Context:
 
     def set_batch_graph(self, on: bool = True):
         check(lib().msplat_hip_set_batch_graph(self.ptr, 1 if on else 0))
+
+    def set_step_graph(self, on: bool = True, initial_capacity: int = 0):
+        """Replay msplat_hip_train_step from a captured hipGraph (msplat_hip_set_step_graph)."""
+        check(lib().msplat_hip_set_step_graph(self.ptr, 1 if on else 0, int(initial_capacity)))
+
+    def step_graph_stats(self) -> tuple:
+        """(captures, replays, replays run again after an intersection overflow)"""
+        c, r, x = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().msplat_hip_step_graph_stats(self.ptr, C.byref(c), C.byref(r), C.byref(x)))
+        return c.value, r.value, x.value
 
     def batch_graph_stats(self) -> tuple:
         c, r = C.c_int64(), C.c_int64()

@@ -71,6 +71,7 @@ class TrainConfig:
     bg_color: tuple = (0.6130, 0.0101, 0.3984)
     densify: bool = True
     seed: int = 42
+    step_graph: bool = False        # replay each step from a captured hipGraph (msplat_hip_set_step_graph)
 
 
 class Trainer:
@@ -91,6 +92,7 @@ class Trainer:
         self._views = {}
         self._pyramids = None
         self.stats_defined = False
+        ctx.set_step_graph(config.step_graph)
 
     def next_camera(self) -> int:
         if self.order_pos >= len(self.order):
