@@ -374,9 +374,20 @@ template <> struct Thr<float> {
 };
 
 // ── 3. forward compositing (rasterize_forward, msplat_cpu.cpp:574-626) ────────
+// With `aux` (the prior-guided forward, rasterize_forward_aux): the expected-depth sum
+// sum_i z_i vis_i (z = the gaussian's view depth, depths[isect_gid[k]]) and a per-pixel
+// background (bg_of: the learned sky's colour, else the constant) written to bg_img.
+template <typename T> struct AuxFwd {
+    const T *depths = nullptr;
+    const int32_t *gid = nullptr;
+    T *out_depth = nullptr, *bg_img = nullptr;
+    const T *sky_bg = nullptr;  // per-pixel background (W x H x 3), or null: the constant
+};
+
 template <typename T>
 void raster_forward(uint32_t W, uint32_t H, int tx, int ty, const int32_t *ts, const int32_t *te, const T *pk_xyo,
-                    const T *pk_conic, const T *pk_rgb, const T *bg, T *out, T *final_T, int32_t *final_idx) {
+                    const T *pk_conic, const T *pk_rgb, const T *bg, T *out, T *final_T, int32_t *final_idx,
+                    const AuxFwd<T> *aux = nullptr) {
     const int nt = tx * ty;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int t = 0; t < nt; t++) {
@@ -384,7 +395,7 @@ void raster_forward(uint32_t W, uint32_t H, int tx, int ty, const int32_t *ts, c
         for (int i = by * TILE; i < std::min((int)H, (by + 1) * TILE); i++)
             for (int j = bx * TILE; j < std::min((int)W, (bx + 1) * TILE); j++) {
                 const T px = (T)j, py = (T)i;
-                T Tr = 1, r = 0, gg = 0, b = 0;
+                T Tr = 1, r = 0, gg = 0, b = 0, depth = 0;
                 int last = ts[t] - 1;
                 for (int k = ts[t]; k < te[t]; k++) {
                     const T *xyo = pk_xyo + 3 * (size_t)k;
@@ -402,15 +413,21 @@ void raster_forward(uint32_t W, uint32_t H, int tx, int ty, const int32_t *ts, c
                     r = std::max(c[0] + K<T>::c(0.5), T(0)) * vis + r;
                     gg = std::max(c[1] + K<T>::c(0.5), T(0)) * vis + gg;
                     b = std::max(c[2] + K<T>::c(0.5), T(0)) * vis + b;
+                    if (aux) depth = depth + aux->depths[aux->gid[k]] * vis;
                     Tr = nT;
                     last = k;
                 }
                 const size_t pid = (size_t)i * W + j;
                 final_T[pid] = Tr;
                 final_idx[pid] = last;
-                out[3 * pid] = std::min(std::max(bg[0] * Tr + r, T(0)), T(1));
-                out[3 * pid + 1] = std::min(std::max(bg[1] * Tr + gg, T(0)), T(1));
-                out[3 * pid + 2] = std::min(std::max(bg[2] * Tr + b, T(0)), T(1));
+                const T *bgp = aux && aux->sky_bg ? aux->sky_bg + 3 * pid : bg;
+                if (aux) {
+                    aux->out_depth[pid] = depth;
+                    for (int c = 0; c < 3; c++) aux->bg_img[3 * pid + c] = bgp[c];
+                }
+                out[3 * pid] = std::min(std::max(bgp[0] * Tr + r, T(0)), T(1));
+                out[3 * pid + 1] = std::min(std::max(bgp[1] * Tr + gg, T(0)), T(1));
+                out[3 * pid + 2] = std::min(std::max(bgp[2] * Tr + b, T(0)), T(1));
             }
     }
 }
@@ -524,10 +541,20 @@ double loss_fwd_bwd(uint32_t W, uint32_t H, const float *ren, const float *gt, f
 
 // ── 5. backward compositing (rasterize_backward, msplat_cpu.cpp:959-1031) ─────
 // Per-intersection gradient slots, 10 floats: v_xy(2) v_conic(3) v_rgb(3) v_logit v_depth.
+// With `aux` (rasterize_backward with the aux inputs, msplat_cpu.cpp:963-1025): the per
+// pixel background of the forward, and the prior losses' gradients with respect to the
+// expected-depth sum D and alpha A (v_aux: 2 per pixel), which add to v_alpha and give
+// each intersection a depth gradient (slot 9).
+template <typename T> struct AuxBwd {
+    const T *depths = nullptr;
+    const int32_t *gid = nullptr;
+    const T *bg_img = nullptr, *v_aux = nullptr;
+};
+
 template <typename T>
 void raster_backward(uint32_t W, uint32_t H, int tx, int ty, const int32_t *ts, const int32_t *te, const T *pk_xyo,
                      const T *pk_conic, const T *pk_rgb, const T *bg, const T *final_T, const int32_t *final_idx,
-                     const T *v_out, T *slots) {
+                     const T *v_out, T *slots, const AuxBwd<T> *aux = nullptr) {
     const int nt = tx * ty;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int t = 0; t < nt; t++) {
@@ -543,8 +570,11 @@ void raster_backward(uint32_t W, uint32_t H, int tx, int ty, const int32_t *ts, 
                 const T Tf = final_T[pid];
                 T Tr = Tf;
                 v3<T> buf{0, 0, 0};
+                T buf_d = 0, buf_a = 0;
                 const v3<T> vo{v_out[3 * pid], v_out[3 * pid + 1], v_out[3 * pid + 2]};
-                const v3<T> tbg{Tf * bg[0], Tf * bg[1], Tf * bg[2]};
+                const T *bgp = aux ? aux->bg_img + 3 * pid : bg;
+                const T v_d = aux ? aux->v_aux[2 * pid] : T(0), v_ax = aux ? aux->v_aux[2 * pid + 1] : T(0);
+                const v3<T> tbg{Tf * bgp[0], Tf * bgp[1], Tf * bgp[2]};
                 for (int k = std::min(final_idx[pid], s1 - 1); k >= s0; k--) {
                     const T *xyo = pk_xyo + 3 * (size_t)k;
                     const T *cn = pk_conic + 3 * (size_t)k;
@@ -563,11 +593,21 @@ void raster_backward(uint32_t W, uint32_t H, int tx, int ty, const int32_t *ts, 
                     v3<T> nb{-buf.x, -buf.y, -buf.z};
                     v3<T> e = madd(rgb, Tr, madd(nb, ra, mul(tbg, -ra)));
                     T v_a = dot(e, vo);
+                    T z = 0;
+                    if (aux) {
+                        z = aux->depths[aux->gid[k]];
+                        v_a += (z * Tr - buf_d * ra) * v_d + (Tr - buf_a * ra) * v_ax;
+                    }
                     buf = madd(rgb, fac, buf);
                     T *g = sl + (size_t)(k - s0) * 10;
                     if (raw[0] + h >= T(0)) g[5] += fac * vo.x;
                     if (raw[1] + h >= T(0)) g[6] += fac * vo.y;
                     if (raw[2] + h >= T(0)) g[7] += fac * vo.z;
+                    if (aux) {
+                        buf_d += z * fac;
+                        buf_a += fac;
+                        g[9] += fac * v_d;
+                    }
                     if (a < Thr<T>::a_max()) {
                         const T vs = -a * v_a;
                         g[2] += h * vs * dx * dx;
@@ -1105,15 +1145,288 @@ extern "C" void orc_f32_grad_stats(int N, const int32_t *radii, const float *v_x
     grad_stats<float>(N, radii, v_xy, inv_max_dim, vis, gn, ms);
 }
 
-// One training step (msplat_train_step, msplat_cpu.cpp:1204-1327) without the
-// prior-guided extras. params/m1/m2 hold the six Adam groups (means, scales, quats, dc,
-// rest, opacity). Returns the mean per-pixel loss; radii_out receives the radii.
-extern "C" float orc_f32_train_step(int N, float glob, const float *vm, const float *pm, float fx, float fy, float cx,
-                                    float cy, uint32_t H, uint32_t W, int tx, int ty, float clip, uint32_t degree,
-                                    uint32_t dtu, const float *cam, const float *bg, const float *gt,
-                                    float ssim_weight, float loss_inv_n, float **params, float **m1, float **m2,
-                                    const float *steps, const float *bc2s, float b1, float b2, float eps, float *vis,
-                                    float *gn, float *ms, float inv_max_dim, int32_t *radii_out, float *img_out) {
+// ── 9. prior-guided extras (msplat_cpu.cpp:327-402, 787-950, 1219-1309) ─────────
+// PriorStep (core/metal/bindings.h:43-77) with plain host pointers; the HIP library's
+// msplat_hip_prior has the same layout.
+struct Prior {
+    int32_t aux, sky;
+    float *sky_tex, *sky_grad, *sky_m1, *sky_m2;
+    int32_t sky_w, sky_h;
+    float sky_frame[9];  // up, e1 (azimuth 0), e2 (azimuth 90 deg)
+    float sky_step, sky_bc2;
+    const float *prior_depth;
+    const uint8_t *prior_aux;  // (prior_h, prior_w, 4): confidence, sky, keep, -
+    int32_t prior_w, prior_h, has_depth, has_sky_mask;
+    float depth_weight, sky_weight, fill_weight, huber_delta, min_alpha;
+    int32_t mask_photometric;
+    float *loss_terms;  // [4]: depth, sky, fill, -
+    int32_t exposure, cam_index, num_cams;
+    float *expo_params, *expo_m1, *expo_m2, *expo_grad;  // (num_cams, 12), (num_cams, 12) x 2, [12]
+    float expo_step, expo_bc2, expo_reg;
+    float log_max_scale_ratio;
+};
+
+// make_sky_params / pixel_world_dir / sky_uv / sky_taps / sky_sample (msplat_cpu.cpp:331-402)
+struct Sky {
+    float up[3] = {0, 0, 0}, e1[3] = {0, 0, 0}, e2[3] = {0, 0, 0};
+    float fx = 1, fy = 1, cx = 0, cy = 0;
+    uint32_t w = 1, h = 1;
+    bool on = false;
+};
+
+inline Sky sky_of(const Prior *p, float fx, float fy, float cx, float cy) {
+    Sky k;
+    k.fx = fx; k.fy = fy; k.cx = cx; k.cy = cy;
+    if (p && p->sky && p->sky_tex) {
+        for (int i = 0; i < 3; i++) {
+            k.up[i] = p->sky_frame[i];
+            k.e1[i] = p->sky_frame[3 + i];
+            k.e2[i] = p->sky_frame[6 + i];
+        }
+        k.w = (uint32_t)p->sky_w;
+        k.h = (uint32_t)p->sky_h;
+        k.on = true;
+    }
+    return k;
+}
+
+// world direction of the ray through pixel (px, py); vm is world-to-camera
+inline v3<float> pixel_dir(float px, float py, const float *vm, const Sky &k) {
+    const float dx = (px + 0.5f - k.cx) / k.fx, dy = (py + 0.5f - k.cy) / k.fy;
+    return {vm[0] * dx + vm[4] * dy + vm[8], vm[1] * dx + vm[5] * dy + vm[9], vm[2] * dx + vm[6] * dy + vm[10]};
+}
+
+// equirect texel coordinates of direction d: azimuth around `up` from e1 towards e2
+inline void sky_uv(v3<float> d, const Sky &k, float &u, float &v) {
+    const float pi = 3.14159265358979323846f;
+    d = mul(d, 1.f / std::sqrt(dot(d, d)));
+    const v3<float> up{k.up[0], k.up[1], k.up[2]}, e1{k.e1[0], k.e1[1], k.e1[2]}, e2{k.e2[0], k.e2[1], k.e2[2]};
+    const float el = std::asin(std::min(std::max(dot(d, up), -1.f), 1.f));
+    const float az = std::atan2(dot(d, e2), dot(d, e1));
+    u = (az * (0.5f / pi) + 0.5f) * (float)k.w - 0.5f;
+    v = (0.5f - el / pi) * (float)k.h - 0.5f;
+}
+
+// bilinear taps: u wraps around, v clamps to the texture
+inline void sky_taps(float u, float v, uint32_t w, uint32_t h, uint32_t idx[4], float wt[4]) {
+    const float u0f = std::floor(u), fu = u - u0f;
+    int u0 = (int)u0f % (int)w;
+    if (u0 < 0) u0 += (int)w;
+    const int u1 = (u0 + 1) % (int)w;
+    const float vc = std::min(std::max(v, 0.f), (float)h - 1.f);
+    const int v0 = (int)vc, v1 = std::min(v0 + 1, (int)h - 1);
+    const float fv = vc - (float)v0;
+    idx[0] = (uint32_t)(v0 * (int)w + u0); wt[0] = (1.f - fu) * (1.f - fv);
+    idx[1] = (uint32_t)(v0 * (int)w + u1); wt[1] = fu * (1.f - fv);
+    idx[2] = (uint32_t)(v1 * (int)w + u0); wt[2] = (1.f - fu) * fv;
+    idx[3] = (uint32_t)(v1 * (int)w + u1); wt[3] = fu * fv;
+}
+
+// the sky colour behind every pixel (sky_sample of the pixel's ray), W x H x 3
+void sky_image(uint32_t W, uint32_t H, const float *vm, const Sky &k, const float *tex, float *out) {
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < (int)H; y++)
+        for (uint32_t x = 0; x < W; x++) {
+            float u, v;
+            sky_uv(pixel_dir((float)x, (float)y, vm, k), k, u, v);
+            uint32_t idx[4];
+            float wt[4];
+            sky_taps(u, v, k.w, k.h, idx, wt);
+            float c[3] = {0, 0, 0};
+            for (int t = 0; t < 4; t++)
+                for (int ch = 0; ch < 3; ch++) c[ch] = tex[3 * (size_t)idx[t] + ch] * wt[t] + c[ch];
+            const size_t pid = (size_t)y * W + x;
+            for (int ch = 0; ch < 3; ch++) out[3 * pid + ch] = c[ch];
+        }
+}
+
+// sample_prior_depth (msplat_cpu.cpp:790-806): bilinear, but nearest across a depth edge
+// (or next to an invalid sample), so that no surface is invented between two
+inline float prior_depth_at(const float *d, float x, float y, uint32_t w, uint32_t h) {
+    x = std::min(std::max(x, 0.f), (float)w - 1.f);
+    y = std::min(std::max(y, 0.f), (float)h - 1.f);
+    const uint32_t x0 = (uint32_t)x, y0 = (uint32_t)y;
+    const uint32_t x1 = std::min(x0 + 1, w - 1), y1 = std::min(y0 + 1, h - 1);
+    const float fx = x - (float)x0, fy = y - (float)y0;
+    const float d00 = d[y0 * w + x0], d01 = d[y0 * w + x1], d10 = d[y1 * w + x0], d11 = d[y1 * w + x1];
+    const float lo = std::min(std::min(d00, d01), std::min(d10, d11));
+    const float hi = std::max(std::max(d00, d01), std::max(d10, d11));
+    if (lo <= 0.f || hi > 1.05f * lo) return d[((fy < 0.5f) ? y0 : y1) * w + ((fx < 0.5f) ? x0 : x1)];
+    const float top = d00 + (d01 - d00) * fx, bottom = d10 + (d11 - d10) * fx;
+    return top + (bottom - top) * fy;
+}
+
+// prior_loss (msplat_cpu.cpp:809-884): per pixel dL/dD and dL/dA into v_aux, the
+// photometric gradient zeroed where the prior says not to keep the pixel, and the depth,
+// sky and fill terms (double per row, rows in order) added to loss_terms
+void prior_loss(uint32_t W, uint32_t H, const Prior &p, const float *final_T, const float *out_depth, float *v_photo,
+                float *v_aux) {
+    const bool has_prior = p.prior_depth && p.prior_aux && p.prior_w > 0 && p.prior_h > 0;
+    const uint32_t pw = has_prior ? (uint32_t)p.prior_w : 1u, ph = has_prior ? (uint32_t)p.prior_h : 1u;
+    static const uint8_t kDefaultAux[4] = {0, 0, 255, 0};  // keep, not sky, no confidence
+    static const float kDefaultDepth[1] = {0.f};
+    const uint8_t *pa = has_prior ? p.prior_aux : kDefaultAux;
+    const float *pd = has_prior ? p.prior_depth : kDefaultDepth;
+    const bool has_depth = has_prior && p.has_depth, has_sky = has_prior && p.has_sky_mask;
+    const float huber = std::max(p.huber_delta, 1e-6f);
+    const float inv_npix = 1.0f / (float)(W * H);
+    std::vector<double> terms((size_t)H * 3, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int gy = 0; gy < (int)H; gy++) {
+        double ld = 0, ls = 0, lf = 0;
+        for (uint32_t gx = 0; gx < W; gx++) {
+            const size_t pix = (size_t)gy * W + gx;
+            const float sx = ((float)gx + 0.5f) * (float)pw / (float)W;
+            const float sy = ((float)gy + 0.5f) * (float)ph / (float)H;
+            const uint32_t q = std::min((uint32_t)sy, ph - 1) * pw + std::min((uint32_t)sx, pw - 1);
+            const float conf = (float)pa[4 * q] * (1.f / 255.f);
+            const bool is_sky = has_sky && pa[4 * q + 1] >= 128, keep = pa[4 * q + 2] >= 128;
+            float v_d = 0.f, v_a = 0.f;
+            if (!keep) {
+                if (p.mask_photometric) v_photo[3 * pix] = v_photo[3 * pix + 1] = v_photo[3 * pix + 2] = 0.f;
+            } else {
+                const float A = 1.f - final_T[pix];
+                if (is_sky) {
+                    ls += p.sky_weight * A;
+                    v_a = p.sky_weight * inv_npix;
+                } else {
+                    if (has_sky && p.fill_weight > 0.f) {
+                        lf += p.fill_weight * (1.f - A);
+                        v_a -= p.fill_weight * inv_npix;
+                    }
+                    const float D = out_depth[pix];
+                    if (has_depth && p.depth_weight > 0.f && conf > 0.f && A > p.min_alpha && D > 0.f) {
+                        const float dp = prior_depth_at(pd, sx - 0.5f, sy - 0.5f, pw, ph);
+                        if (dp > 0.f) {
+                            const float r = std::log(D / A) - std::log(dp), ar = std::fabs(r);
+                            float dr;
+                            if (ar < huber) {
+                                ld += p.depth_weight * conf * 0.5f * r * r / huber;
+                                dr = r / huber;
+                            } else {
+                                ld += p.depth_weight * conf * (ar - 0.5f * huber);
+                                dr = r > 0.f ? 1.f : (r < 0.f ? -1.f : 0.f);
+                            }
+                            const float sc = p.depth_weight * conf * dr * inv_npix;
+                            v_d = sc / D;
+                            v_a -= sc / A;
+                        }
+                    }
+                }
+            }
+            v_aux[2 * pix] = v_d;
+            v_aux[2 * pix + 1] = v_a;
+        }
+        terms[3 * (size_t)gy] = ld;
+        terms[3 * (size_t)gy + 1] = ls;
+        terms[3 * (size_t)gy + 2] = lf;
+    }
+    if (p.loss_terms) {
+        double t[3] = {0, 0, 0};
+        for (uint32_t y = 0; y < H; y++)
+            for (int c = 0; c < 3; c++) t[c] += terms[3 * (size_t)y + c];
+        for (int c = 0; c < 3; c++) p.loss_terms[c] += (float)(t[c] * inv_npix);
+    }
+}
+
+// exposure_apply (msplat_cpu.cpp:887-896): rows 0-2 of P are the 3x3 colour matrix, 9-11 the offset
+void exposure_apply(const float *img, const float *P, size_t n, float *out) {
+    for (size_t i = 0; i < n; i++) {
+        const float c0 = img[3 * i], c1 = img[3 * i + 1], c2 = img[3 * i + 2];
+        out[3 * i] = P[0] * c0 + P[1] * c1 + P[2] * c2 + P[9];
+        out[3 * i + 1] = P[3] * c0 + P[4] * c1 + P[5] * c2 + P[10];
+        out[3 * i + 2] = P[6] * c0 + P[7] * c1 + P[8] * c2 + P[11];
+    }
+}
+
+// exposure_backward_adam (msplat_cpu.cpp:899-924): the render's gradient P^T v_adj, the
+// 12 parameter gradients (double sums over the pixels in order) plus the caller's extra
+// gradient and the pull towards the identity, one Adam step on camera cam_index's row
+void exposure_backward_adam(const float *img, const Prior &p, size_t n, const float *v_adj, float *v_img, float b1,
+                            float b2, float eps) {
+    const uint32_t cam = (uint32_t)std::max(0, p.cam_index);
+    float *params = p.expo_params;
+    const float *P = params + 12 * cam;
+    double g[12] = {0};
+    for (size_t i = 0; i < n; i++) {
+        const float c0 = img[3 * i], c1 = img[3 * i + 1], c2 = img[3 * i + 2];
+        const float vx = v_adj[3 * i], vy = v_adj[3 * i + 1], vz = v_adj[3 * i + 2];
+        v_img[3 * i] = P[0] * vx + P[3] * vy + P[6] * vz;
+        v_img[3 * i + 1] = P[1] * vx + P[4] * vy + P[7] * vz;
+        v_img[3 * i + 2] = P[2] * vx + P[5] * vy + P[8] * vz;
+        g[0] += vx * c0; g[1] += vx * c1; g[2] += vx * c2;
+        g[3] += vy * c0; g[4] += vy * c1; g[5] += vy * c2;
+        g[6] += vz * c0; g[7] += vz * c1; g[8] += vz * c2;
+        g[9] += vx; g[10] += vy; g[11] += vz;
+    }
+    for (int k = 0; k < 12; k++) {
+        const uint32_t i = 12 * cam + k;
+        const float ident = (k == 0 || k == 4 || k == 8) ? 1.f : 0.f;
+        const float gk = (float)g[k] + (p.expo_grad ? p.expo_grad[k] : 0.f) + p.expo_reg * (params[i] - ident);
+        if (p.expo_grad) p.expo_grad[k] = 0.f;
+        adam<float>(params[i], p.expo_m1[i], p.expo_m2[i], gk, p.expo_step, b1, b2, p.expo_bc2, eps);
+    }
+}
+
+// sky_backward_adam (msplat_cpu.cpp:927-950): each pixel with T >= T_MIN adds T v_img to
+// its ray's four taps (pixels in row order), then one Adam step on every texel
+void sky_backward_adam(uint32_t W, uint32_t H, const Prior &p, const Sky &k, const float *vm, const float *final_T,
+                       const float *v_img, float b1, float b2, float eps) {
+    float *grad = p.sky_grad;
+    for (uint32_t y = 0; y < H; y++)
+        for (uint32_t x = 0; x < W; x++) {
+            const size_t pix = (size_t)y * W + x;
+            const float T = final_T[pix];
+            if (T < 1e-4f) continue;
+            const float g[3] = {T * v_img[3 * pix], T * v_img[3 * pix + 1], T * v_img[3 * pix + 2]};
+            float u, v;
+            sky_uv(pixel_dir((float)x, (float)y, vm, k), k, u, v);
+            uint32_t idx[4];
+            float wt[4];
+            sky_taps(u, v, k.w, k.h, idx, wt);
+            for (int t = 0; t < 4; t++) {
+                if (wt[t] == 0.f) continue;
+                float *d = grad + 3 * (size_t)idx[t];
+                d[0] += wt[t] * g[0];
+                d[1] += wt[t] * g[1];
+                d[2] += wt[t] * g[2];
+            }
+        }
+    const size_t n = (size_t)p.sky_w * p.sky_h * 3;
+    for (size_t i = 0; i < n; i++) adam<float>(p.sky_tex[i], p.sky_m1[i], p.sky_m2[i], grad[i], p.sky_step, b1, b2, p.sky_bc2, eps);
+}
+
+// scale_ratio_cap (msplat_cpu.cpp:1295-1309): a gaussian's largest log-scale is lowered to
+// its middle one plus log_max (the largest of equal values: the first)
+void scale_cap(int N, float *s, float log_max) {
+    for (int i = 0; i < N; i++) {
+        const float a = s[3 * i], b = s[3 * i + 1], c = s[3 * i + 2];
+        const float mx = std::max(a, std::max(b, c)), mn = std::min(a, std::min(b, c));
+        const float cap = (a + b + c - mx - mn) + log_max;
+        if (mx <= cap) continue;
+        if (a == mx) s[3 * i] = cap;
+        else if (b == mx) s[3 * i + 1] = cap;
+        else s[3 * i + 2] = cap;
+    }
+}
+
+// One training step (msplat_train_step, msplat_cpu.cpp:1204-1327), with the
+// prior-guided extras when prior is given (aux forward with the learned sky, exposure on
+// the photometric loss, prior losses, sky and exposure Adam, aux backward, needle cap).
+// params/m1/m2 hold the six Adam groups (means, scales, quats, dc, rest, opacity).
+// Returns the mean per-pixel photometric loss; radii_out receives the radii, img_out the
+// render, depth_out / final_T_out (aux) the expected-depth sum and final transmittance.
+static float train_step_f32(int N, float glob, const float *vm, const float *pm, float fx, float fy, float cx,
+                            float cy, uint32_t H, uint32_t W, int tx, int ty, float clip, uint32_t degree,
+                            uint32_t dtu, const float *cam, const float *bg, const float *gt, float ssim_weight,
+                            float loss_inv_n, float **params, float **m1, float **m2, const float *steps,
+                            const float *bc2s, float b1, float b2, float eps, float *vis, float *gn, float *ms,
+                            float inv_max_dim, int32_t *radii_out, float *img_out, const Prior *prior,
+                            float *depth_out, float *final_T_out) {
+    const bool aux = prior && prior->aux;
+    const bool sky = aux && prior->sky && prior->sky_tex && prior->sky_grad;
+    const bool exposure = prior && prior->exposure && prior->expo_params;
+    const bool cap = prior && prior->log_max_scale_ratio > 0.f;
     std::vector<float> xys(2 * (size_t)N), depths(N), conics(3 * (size_t)N), colors(3 * (size_t)N), aabb(2 * (size_t)N);
     std::vector<int32_t> radii(N);
     project<float>(N, params[0], params[1], glob, params[2], vm, pm, fx, fy, cx, cy, W, H, tx, ty, clip, degree, dtu,
@@ -1129,12 +1442,55 @@ extern "C" float orc_f32_train_step(int N, float glob, const float *vm, const fl
     const size_t P = (size_t)W * H;
     std::vector<float> img(3 * P), fT(P), vimg(3 * P);
     std::vector<int32_t> fidx(P);
+    std::vector<float> depth, bg_img, sky_bg, v_aux;
+    AuxFwd<float> af;
+    const Sky sky_fwd = sky_of(aux ? prior : nullptr, fx, fy, cx, cy);
+    if (aux) {
+        depth.assign(P, 0.f);
+        bg_img.assign(3 * P, 0.f);
+        af.depths = depths.data();
+        af.gid = gid.data();
+        af.out_depth = depth.data();
+        af.bg_img = bg_img.data();
+        if (sky_fwd.on) {
+            sky_bg.resize(3 * P);
+            sky_image(W, H, vm, sky_fwd, prior->sky_tex, sky_bg.data());
+            af.sky_bg = sky_bg.data();
+        }
+    }
     raster_forward<float>(W, H, tx, ty, ts.data(), te.data(), pxyo.data(), pcn.data(), prgb.data(), bg, img.data(),
-                          fT.data(), fidx.data());
-    double lsum = loss_fwd_bwd(W, H, img.data(), gt, ssim_weight, loss_inv_n, vimg.data());
+                          fT.data(), fidx.data(), aux ? &af : nullptr);
+    if (aux && prior->loss_terms) std::fill(prior->loss_terms, prior->loss_terms + 4, 0.f);
+    if (sky) std::fill(prior->sky_grad, prior->sky_grad + (size_t)prior->sky_w * prior->sky_h * 3, 0.f);
+    // photometric loss, on the exposure-adjusted render with exposure
+    const float *loss_in = img.data();
+    float *loss_grad = vimg.data();
+    std::vector<float> adj, v_adj;
+    if (exposure) {
+        adj.resize(3 * P);
+        v_adj.resize(3 * P);
+        exposure_apply(img.data(), prior->expo_params + 12 * std::max(0, prior->cam_index), P, adj.data());
+        loss_in = adj.data();
+        loss_grad = v_adj.data();
+    }
+    double lsum = loss_fwd_bwd(W, H, loss_in, gt, ssim_weight, loss_inv_n, loss_grad);
+    if (aux) {
+        v_aux.resize(2 * P);
+        prior_loss(W, H, *prior, fT.data(), depth.data(), loss_grad, v_aux.data());
+    }
+    if (exposure) exposure_backward_adam(img.data(), *prior, P, v_adj.data(), vimg.data(), b1, b2, eps);
+    if (sky)
+        sky_backward_adam(W, H, *prior, sky_of(prior, fx, fy, cx, cy), vm, fT.data(), vimg.data(), b1, b2, eps);
     std::vector<float> slots(10 * Mc);
+    AuxBwd<float> ab;
+    if (aux) {
+        ab.depths = depths.data();
+        ab.gid = gid.data();
+        ab.bg_img = bg_img.data();
+        ab.v_aux = v_aux.data();
+    }
     raster_backward<float>(W, H, tx, ty, ts.data(), te.data(), pxyo.data(), pcn.data(), prgb.data(), bg, fT.data(),
-                           fidx.data(), vimg.data(), slots.data());
+                           fidx.data(), vimg.data(), slots.data(), aux ? &ab : nullptr);
     std::vector<float> vxy(2 * (size_t)N), vcn(3 * (size_t)N), vrgb(3 * (size_t)N), vop(N), vdp(N);
     reduce_slots<float>(N, M, gid.data(), slots.data(), vxy.data(), vcn.data(), vrgb.data(), vop.data(), vdp.data());
     std::vector<float> vmean(3 * (size_t)N), vsc(3 * (size_t)N), vq(4 * (size_t)N);
@@ -1147,10 +1503,78 @@ extern "C" float orc_f32_train_step(int N, float glob, const float *vm, const fl
     for (int g : {0, 1, 2, 5})
         for (size_t i = 0; i < (size_t)N * width[g]; i++)
             adam<float>(params[g][i], m1[g][i], m2[g][i], grads[g][i], steps[g], b1, b2, bc2s[g], eps);
+    if (cap) scale_cap(N, params[1], prior->log_max_scale_ratio);
     grad_stats<float>(N, radii.data(), vxy.data(), inv_max_dim, vis, gn, ms);
     if (radii_out) std::memcpy(radii_out, radii.data(), sizeof(int32_t) * N);
     if (img_out) std::memcpy(img_out, img.data(), sizeof(float) * 3 * P);
+    if (depth_out && aux) std::memcpy(depth_out, depth.data(), sizeof(float) * P);
+    if (final_T_out) std::memcpy(final_T_out, fT.data(), sizeof(float) * P);
     return (float)(lsum / (double)P);
+}
+
+extern "C" float orc_f32_train_step(int N, float glob, const float *vm, const float *pm, float fx, float fy, float cx,
+                                    float cy, uint32_t H, uint32_t W, int tx, int ty, float clip, uint32_t degree,
+                                    uint32_t dtu, const float *cam, const float *bg, const float *gt,
+                                    float ssim_weight, float loss_inv_n, float **params, float **m1, float **m2,
+                                    const float *steps, const float *bc2s, float b1, float b2, float eps, float *vis,
+                                    float *gn, float *ms, float inv_max_dim, int32_t *radii_out, float *img_out) {
+    return train_step_f32(N, glob, vm, pm, fx, fy, cx, cy, H, W, tx, ty, clip, degree, dtu, cam, bg, gt, ssim_weight,
+                          loss_inv_n, params, m1, m2, steps, bc2s, b1, b2, eps, vis, gn, ms, inv_max_dim, radii_out,
+                          img_out, nullptr, nullptr, nullptr);
+}
+
+// msplat_train_step with a PriorStep (msplat_cpu.cpp:1204-1327)
+extern "C" float orc_f32_train_step_prior(int N, float glob, const float *vm, const float *pm, float fx, float fy,
+                                          float cx, float cy, uint32_t H, uint32_t W, int tx, int ty, float clip,
+                                          uint32_t degree, uint32_t dtu, const float *cam, const float *bg,
+                                          const float *gt, float ssim_weight, float loss_inv_n, float **params,
+                                          float **m1, float **m2, const float *steps, const float *bc2s, float b1,
+                                          float b2, float eps, float *vis, float *gn, float *ms, float inv_max_dim,
+                                          int32_t *radii_out, float *img_out, const Prior *prior, float *depth_out,
+                                          float *final_T_out) {
+    return train_step_f32(N, glob, vm, pm, fx, fy, cx, cy, H, W, tx, ty, clip, degree, dtu, cam, bg, gt, ssim_weight,
+                          loss_inv_n, params, m1, m2, steps, bc2s, b1, b2, eps, vis, gn, ms, inv_max_dim, radii_out,
+                          img_out, prior, depth_out, final_T_out);
+}
+
+// msplat_render with a PriorStep (msplat_cpu.cpp:1183-1197, 631-657): the aux forward
+// with the learned sky; depth_out / final_T_out as msplat_render_aux_outputs
+extern "C" void orc_f32_render_prior(int N, float glob, const float *vm, const float *pm, float fx, float fy, float cx,
+                                     float cy, uint32_t H, uint32_t W, int tx, int ty, float clip, uint32_t degree,
+                                     uint32_t dtu, const float *cam, float **params, const float *bg,
+                                     const Prior *prior, float *img_out, float *depth_out, float *final_T_out,
+                                     int32_t *radii_out) {
+    const bool aux = prior && prior->aux;
+    std::vector<float> xys(2 * (size_t)N), depths(N), conics(3 * (size_t)N), colors(3 * (size_t)N), aabb(2 * (size_t)N);
+    std::vector<int32_t> radii(N);
+    project<float>(N, params[0], params[1], glob, params[2], vm, pm, fx, fy, cx, cy, W, H, tx, ty, clip, degree, dtu,
+                   cam, params[3], params[4], xys.data(), depths.data(), conics.data(), colors.data(), aabb.data(),
+                   radii.data());
+    const int nt = tx * ty;
+    int64_t M = count_isects<float>(N, xys.data(), aabb.data(), radii.data(), tx, ty, nullptr);
+    const size_t Mc = (size_t)std::max<int64_t>(M, 1);
+    std::vector<int32_t> ts(nt), te(nt), gid(Mc), fidx((size_t)W * H);
+    std::vector<float> pxyo(3 * Mc), pcn(3 * Mc), prgb(3 * Mc), fT((size_t)W * H);
+    bin_sort<float>(N, xys.data(), depths.data(), aabb.data(), radii.data(), params[5], conics.data(), colors.data(),
+                    tx, ty, ts.data(), te.data(), gid.data(), pxyo.data(), pcn.data(), prgb.data(), nullptr);
+    const size_t P = (size_t)W * H;
+    std::vector<float> depth(P, 0.f), bg_img(3 * P), sky_bg;
+    AuxFwd<float> af;
+    af.depths = depths.data();
+    af.gid = gid.data();
+    af.out_depth = depth.data();
+    af.bg_img = bg_img.data();
+    const Sky k = sky_of(aux ? prior : nullptr, fx, fy, cx, cy);
+    if (k.on) {
+        sky_bg.resize(3 * P);
+        sky_image(W, H, vm, k, prior->sky_tex, sky_bg.data());
+        af.sky_bg = sky_bg.data();
+    }
+    raster_forward<float>(W, H, tx, ty, ts.data(), te.data(), pxyo.data(), pcn.data(), prgb.data(), bg, img_out,
+                          fT.data(), fidx.data(), aux ? &af : nullptr);
+    if (depth_out && aux) std::memcpy(depth_out, depth.data(), sizeof(float) * P);
+    if (final_T_out) std::memcpy(final_T_out, fT.data(), sizeof(float) * P);
+    if (radii_out) std::memcpy(radii_out, radii.data(), sizeof(int32_t) * N);
 }
 
 extern "C" int orc_densify(int N, int cap, float grad_thr, float size_thr, float screen_thr, int check_screen,

@@ -41,6 +41,59 @@ def sh_bases(degree: int) -> int:
     return {0: 1, 1: 4, 2: 9, 3: 16}.get(degree, 25)
 
 
+class PriorStruct(C.Structure):
+    """PriorStep (core/metal/bindings.h:43-77) as plain pointers: the oracle's Prior and
+    ref_driver.cpp's RefPrior."""
+    _fields_ = [("aux", C.c_int32), ("sky", C.c_int32), ("sky_tex", vp), ("sky_grad", vp), ("sky_m1", vp),
+                ("sky_m2", vp), ("sky_w", C.c_int32), ("sky_h", C.c_int32), ("sky_frame", C.c_float * 9),
+                ("sky_step", C.c_float), ("sky_bc2", C.c_float), ("prior_depth", vp), ("prior_aux", vp),
+                ("prior_w", C.c_int32), ("prior_h", C.c_int32), ("has_depth", C.c_int32), ("has_sky_mask", C.c_int32),
+                ("depth_weight", C.c_float), ("sky_weight", C.c_float), ("fill_weight", C.c_float),
+                ("huber_delta", C.c_float), ("min_alpha", C.c_float), ("mask_photometric", C.c_int32),
+                ("loss_terms", vp), ("exposure", C.c_int32), ("cam_index", C.c_int32), ("num_cams", C.c_int32),
+                ("expo_params", vp), ("expo_m1", vp), ("expo_m2", vp), ("expo_grad", vp), ("expo_step", C.c_float),
+                ("expo_bc2", C.c_float), ("expo_reg", C.c_float), ("log_max_scale_ratio", C.c_float)]
+
+
+PRIOR_ARRAYS = {"sky_tex": np.float32, "sky_grad": np.float32, "sky_m1": np.float32, "sky_m2": np.float32,
+                "prior_depth": np.float32, "prior_aux": np.uint8, "loss_terms": np.float32, "expo_params": np.float32,
+                "expo_m1": np.float32, "expo_m2": np.float32, "expo_grad": np.float32}
+PRIOR_DEFAULTS = dict(aux=False, sky=False, sky_frame=(0.0,) * 9, sky_step=0.0, sky_bc2=1.0, has_depth=False,
+                      has_sky_mask=False, depth_weight=0.0, sky_weight=0.0, fill_weight=0.0, huber_delta=0.05,
+                      min_alpha=0.25, mask_photometric=True, exposure=False, cam_index=0, expo_step=0.0, expo_bc2=1.0,
+                      expo_reg=0.0, log_max_scale_ratio=0.0)
+
+
+PRIOR_INTS = ("aux", "sky", "has_depth", "has_sky_mask", "mask_photometric", "exposure", "cam_index")
+
+
+def prior_struct(prior: dict) -> PriorStruct:
+    """The C record of a prior given as a dict of PriorStep's fields (numpy arrays for the
+    tensors, which must be C-contiguous arrays of the listed dtypes: the C side updates
+    them in place)."""
+    st = PriorStruct()
+    for k, v in PRIOR_DEFAULTS.items():
+        v = prior.get(k, v)
+        if k == "sky_frame":
+            st.sky_frame[:] = [float(x) for x in v]
+        else:
+            setattr(st, k, int(v) if k in PRIOR_INTS else float(v))
+    for k, dt in PRIOR_ARRAYS.items():
+        a = prior.get(k)
+        if a is None:
+            continue
+        if a.dtype != dt or not a.flags["C_CONTIGUOUS"]:
+            raise ValueError(f"prior[{k!r}] must be a C-contiguous {np.dtype(dt).name} array")
+        setattr(st, k, a.ctypes.data)
+    if prior.get("sky_tex") is not None:
+        st.sky_h, st.sky_w = prior["sky_tex"].shape[:2]
+    if prior.get("prior_depth") is not None:
+        st.prior_h, st.prior_w = prior["prior_depth"].shape[:2]
+    if prior.get("expo_params") is not None:
+        st.num_cams = prior["expo_params"].shape[0]
+    return st
+
+
 class Oracle:
     """The CPU restatement. Methods take/return numpy arrays in the reference layouts."""
 
@@ -79,6 +132,12 @@ class Oracle:
              C.c_int, C.c_int, C.c_float, C.c_uint32, C.c_uint32, f32p, f32p, f32p, C.c_float, C.c_float, vp, vp, vp,
              f32p, f32p, C.c_float, C.c_float, C.c_float, f32p, f32p, f32p, C.c_float, vp, vp])
         L.orc_f32_train_step.restype = C.c_float
+        L.orc_f32_train_step_prior.argtypes = L.orc_f32_train_step.argtypes + [C.POINTER(PriorStruct), vp, vp]
+        L.orc_f32_train_step_prior.restype = C.c_float
+        L.orc_f32_render_prior.argtypes = (
+            [C.c_int, C.c_float, f32p, f32p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint32,
+             C.c_int, C.c_int, C.c_float, C.c_uint32, C.c_uint32, f32p, vp, f32p, C.POINTER(PriorStruct), f32p, vp, vp,
+             vp])
         L.orc_prepare_cam.argtypes = [f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
                                       f32p, f32p, f32p, f32p, i32p]
         L.orc_means_lr.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float]
@@ -227,6 +286,49 @@ class Oracle:
             np.float32(1.0) / np.float32(max(H, W)), radii.ctypes.data, img.ctypes.data)
         return loss, radii, img
 
+    def train_step_prior(self, state, cam, bg, gt, steps, bc2s, prior: dict, ssim_weight=0.2, b1=0.9, b2=0.999,
+                         eps=1e-8):
+        """msplat_train_step with a PriorStep (given as a dict, see prior_struct), in place
+        on `state` and on the prior's arrays. Returns (loss, radii, image, depth, final_T)."""
+        N = state["means"].shape[0]
+        names = ("means", "scales", "quats", "dc", "rest", "opac")
+        for n in names:
+            for pre in ("", "m1_", "m2_"):
+                state[pre + n] = _c(state[pre + n], np.float32)
+        H, W = cam["H"], cam["W"]
+        radii = np.zeros(N, np.int32)
+        img = np.zeros((H, W, 3), np.float32)
+        depth = np.zeros((H, W), np.float32)
+        fT = np.zeros((H, W), np.float32)
+        st = prior_struct(prior)
+        loss = self.lib.orc_f32_train_step_prior(
+            N, 1.0, _c(cam["viewmat"], np.float32), _c(cam["projmat"], np.float32), cam["fx"], cam["fy"], cam["cx"],
+            cam["cy"], H, W, cam["tiles_x"], cam["tiles_y"], 0.01, state["degree"], cam["dtu"],
+            _c(cam["cam_pos"], np.float32), _c(bg, np.float32), _c(gt, np.float32), ssim_weight,
+            np.float32(1.0) / np.float32(H * W * 3), _ptrs([state[n] for n in names]),
+            _ptrs([state["m1_" + n] for n in names]), _ptrs([state["m2_" + n] for n in names]),
+            _c(steps, np.float32), _c(bc2s, np.float32), b1, b2, eps, state["vis"], state["gn"], state["ms"],
+            np.float32(1.0) / np.float32(max(H, W)), radii.ctypes.data, img.ctypes.data, C.byref(st),
+            depth.ctypes.data, fT.ctypes.data)
+        return loss, radii, img, depth, fT
+
+    def render_prior(self, scene, cam, bg, prior: dict):
+        """msplat_render with a PriorStep: (image, expected-depth sum, final transmittance, radii)."""
+        N = scene["means"].shape[0]
+        H, W = cam["H"], cam["W"]
+        img = np.zeros((H, W, 3), np.float32)
+        depth = np.zeros((H, W), np.float32)
+        fT = np.zeros((H, W), np.float32)
+        radii = np.zeros(N, np.int32)
+        params = [_c(scene[k], np.float32) for k in ("means", "scales", "quats", "dc", "rest", "opac")]
+        st = prior_struct(prior)
+        self.lib.orc_f32_render_prior(
+            N, 1.0, _c(cam["viewmat"], np.float32), _c(cam["projmat"], np.float32), cam["fx"], cam["fy"], cam["cx"],
+            cam["cy"], H, W, cam["tiles_x"], cam["tiles_y"], 0.01, scene["degree"], cam["dtu"],
+            _c(cam["cam_pos"], np.float32), _ptrs(params), _c(bg, np.float32), C.byref(st), img, depth.ctypes.data,
+            fT.ctypes.data, radii.ctypes.data)
+        return img, depth, fT, radii
+
     def densify(self, bufs, N, cap, gn, vc, ms, half_max_dim, fr_stride, seed, grad_thr=2e-4, size_thr=0.01,
                 screen_thr=0.05, check_screen=1, cull_alpha=0.1, cull_scale=0.5, cull_screen=0.15, check_huge=0):
         """bufs: list of 18 float32 arrays with `cap` rows, modified in place. Returns
@@ -347,6 +449,10 @@ class Reference:
                                       vp])
         L.ref_densify.argtypes = ([C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_float, C.c_float,
                                    C.c_float, C.c_int, f32p, f32p, f32p, C.c_float, vp, C.c_int, C.c_uint32])
+        L.ref_train_step_prior.argtypes = L.ref_train_step.argtypes + [C.POINTER(PriorStruct), vp, vp]
+        L.ref_render_prior.argtypes = ([C.c_int, f32p, f32p, f32p, f32p, f32p, C.c_int, f32p, f32p, f32p, C.c_float,
+                                        C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_float, C.c_int,
+                                        C.c_int, f32p, f32p, C.POINTER(PriorStruct), f32p, vp, vp])
         L.ref_opacity_reset.argtypes = [f32p, f32p, f32p, C.c_int, C.c_float]
         L.ref_radix_sort.argtypes = [u64p, i32p, C.c_uint32, C.c_int]
         L.ref_knn3.argtypes = [f32p, C.c_uint32, f32p]
@@ -389,6 +495,49 @@ class Reference:
         if rc:
             raise RuntimeError("reference train_step failed")
         return loss.value, radii
+
+    def train_step_prior(self, state, cam, bg, gt, steps, bc2s, prior: dict, ssim_weight=0.2, b1=0.9, b2=0.999,
+                         eps=1e-8):
+        N = state["means"].shape[0]
+        names = ("means", "scales", "quats", "dc", "rest", "opac")
+        for n in names:
+            for pre in ("", "m1_", "m2_"):
+                state[pre + n] = _c(state[pre + n], np.float32)
+        H, W = cam["H"], cam["W"]
+        radii = np.zeros(N, np.int32)
+        depth = np.zeros((H, W), np.float32)
+        fT = np.zeros((H, W), np.float32)
+        loss = C.c_float(0)
+        st = prior_struct(prior)
+        rc = self.lib.ref_train_step_prior(
+            N, _ptrs([state[n] for n in names]), sh_bases(state["degree"]) - 1,
+            _ptrs([state["m1_" + n] for n in names]), _ptrs([state["m2_" + n] for n in names]),
+            _c(cam["viewmat"], np.float32), _c(cam["projmat"], np.float32), cam["fx"], cam["fy"], cam["cx"],
+            cam["cy"], H, W, 0.01, state["degree"], cam["dtu"], _c(cam["cam_pos"], np.float32),
+            _c(bg, np.float32), _c(gt, np.float32), ssim_weight, _c(steps, np.float32), _c(bc2s, np.float32), b1, b2,
+            eps, state["vis"], state["gn"], state["ms"], radii, C.byref(loss), C.byref(st), depth.ctypes.data,
+            fT.ctypes.data)
+        if rc:
+            raise RuntimeError("reference train_step with a prior failed")
+        return loss.value, radii, depth, fT
+
+    def render_prior(self, scene, cam, bg, prior: dict):
+        N = scene["means"].shape[0]
+        H, W = cam["H"], cam["W"]
+        img = np.zeros((H, W, 3), np.float32)
+        depth = np.zeros((H, W), np.float32)
+        fT = np.zeros((H, W), np.float32)
+        rb = sh_bases(scene["degree"]) - 1
+        st = prior_struct(prior)
+        rc = self.lib.ref_render_prior(
+            N, _c(scene["means"], np.float32), _c(scene["scales"], np.float32), _c(scene["quats"], np.float32),
+            _c(scene["dc"], np.float32), _c(scene["rest"], np.float32), rb, _c(scene["opac"], np.float32),
+            _c(cam["viewmat"], np.float32), _c(cam["projmat"], np.float32), cam["fx"], cam["fy"], cam["cx"],
+            cam["cy"], H, W, 0.01, scene["degree"], cam["dtu"], _c(cam["cam_pos"], np.float32),
+            _c(bg, np.float32), C.byref(st), img, depth.ctypes.data, fT.ctypes.data)
+        if rc:
+            raise RuntimeError("reference render with a prior failed")
+        return img, depth, fT
 
     def densify(self, bufs, N, cap, gn, vc, ms, half_max_dim, fr_stride, seed, grad_thr=2e-4, size_thr=0.01,
                 screen_thr=0.05, check_screen=1, cull_alpha=0.1, cull_scale=0.5, cull_screen=0.15, check_huge=0):

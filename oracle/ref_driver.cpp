@@ -115,6 +115,161 @@ extern "C" int ref_train_step(int N, float **params, int rest_bases, float **m1,
     }
 }
 
+// PriorStep as plain host arrays (the layout of the oracle's Prior). The tensors the
+// step updates (sky texture, gradient and moments, exposure rows and moments, exposure
+// gradient, loss terms) are copied back.
+struct RefPrior {
+    int32_t aux, sky;
+    float *sky_tex, *sky_grad, *sky_m1, *sky_m2;
+    int32_t sky_w, sky_h;
+    float sky_frame[9];
+    float sky_step, sky_bc2;
+    const float *prior_depth;
+    const uint8_t *prior_aux;
+    int32_t prior_w, prior_h, has_depth, has_sky_mask;
+    float depth_weight, sky_weight, fill_weight, huber_delta, min_alpha;
+    int32_t mask_photometric;
+    float *loss_terms;
+    int32_t exposure, cam_index, num_cams;
+    float *expo_params, *expo_m1, *expo_m2, *expo_grad;
+    float expo_step, expo_bc2, expo_reg;
+    float log_max_scale_ratio;
+};
+
+namespace {
+struct PriorTensors {
+    PriorStep p;
+    MTensor sky_tex, sky_grad, sky_m1, sky_m2, depth, aux, terms, expo, expo_m1, expo_m2, expo_grad;
+    explicit PriorTensors(const RefPrior &r) {
+        p.aux = r.aux != 0;
+        p.sky = r.sky != 0;
+        const std::vector<int64_t> tex = {r.sky_h, r.sky_w, 3};
+        if (r.sky_tex) { sky_tex = upload(r.sky_tex, tex, DType::Float32); p.sky_tex = &sky_tex; }
+        if (r.sky_grad) { sky_grad = upload(r.sky_grad, tex, DType::Float32); p.sky_grad = &sky_grad; }
+        if (r.sky_m1) { sky_m1 = upload(r.sky_m1, tex, DType::Float32); p.sky_exp_avg = &sky_m1; }
+        if (r.sky_m2) { sky_m2 = upload(r.sky_m2, tex, DType::Float32); p.sky_exp_avg_sq = &sky_m2; }
+        p.sky_w = r.sky_w;
+        p.sky_h = r.sky_h;
+        std::memcpy(p.sky_frame, r.sky_frame, sizeof p.sky_frame);
+        p.sky_step_size = r.sky_step;
+        p.sky_bc2_sqrt = r.sky_bc2;
+        if (r.prior_depth) { depth = upload(r.prior_depth, {r.prior_h, r.prior_w}, DType::Float32); p.prior_depth = &depth; }
+        if (r.prior_aux) { aux = upload(r.prior_aux, {r.prior_h, r.prior_w, 4}, DType::UInt8); p.prior_aux = &aux; }
+        p.prior_w = r.prior_w;
+        p.prior_h = r.prior_h;
+        p.has_depth = r.has_depth != 0;
+        p.has_sky_mask = r.has_sky_mask != 0;
+        p.depth_weight = r.depth_weight;
+        p.sky_weight = r.sky_weight;
+        p.fill_weight = r.fill_weight;
+        p.huber_delta = r.huber_delta;
+        p.min_alpha = r.min_alpha;
+        p.mask_photometric = r.mask_photometric != 0;
+        if (r.loss_terms) { terms = upload(r.loss_terms, {4}, DType::Float32); p.loss_terms = &terms; }
+        p.exposure = r.exposure != 0;
+        p.cam_index = r.cam_index;
+        const std::vector<int64_t> ex = {r.num_cams, 12};
+        if (r.expo_params) { expo = upload(r.expo_params, ex, DType::Float32); p.expo_params = &expo; }
+        if (r.expo_m1) { expo_m1 = upload(r.expo_m1, ex, DType::Float32); p.expo_exp_avg = &expo_m1; }
+        if (r.expo_m2) { expo_m2 = upload(r.expo_m2, ex, DType::Float32); p.expo_exp_avg_sq = &expo_m2; }
+        if (r.expo_grad) { expo_grad = upload(r.expo_grad, {12}, DType::Float32); p.expo_grad = &expo_grad; }
+        p.expo_step_size = r.expo_step;
+        p.expo_bc2_sqrt = r.expo_bc2;
+        p.expo_reg = r.expo_reg;
+        p.log_max_scale_ratio = r.log_max_scale_ratio;
+    }
+    void back(const RefPrior &r) const {
+        if (r.sky_tex) download(r.sky_tex, sky_tex);
+        if (r.sky_grad) download(r.sky_grad, sky_grad);
+        if (r.sky_m1) download(r.sky_m1, sky_m1);
+        if (r.sky_m2) download(r.sky_m2, sky_m2);
+        if (r.loss_terms) download(r.loss_terms, terms);
+        if (r.expo_params) download(r.expo_params, expo);
+        if (r.expo_m1) download(r.expo_m1, expo_m1);
+        if (r.expo_m2) download(r.expo_m2, expo_m2);
+        if (r.expo_grad) download(r.expo_grad, expo_grad);
+    }
+};
+}  // namespace
+
+// msplat_train_step with a PriorStep; arguments as ref_train_step, plus the prior and
+// the aux outputs (expected-depth sum, final transmittance)
+extern "C" int ref_train_step_prior(int N, float **params, int rest_bases, float **m1, float **m2,
+                                    const float *viewmat, const float *projmat, float fx, float fy, float cx, float cy,
+                                    int H, int W, float clip, int degree, int dtu, const float *cam_pos,
+                                    const float *bg, const float *gt, float ssim_weight, const float *steps,
+                                    const float *bc2s, float b1, float b2, float eps, float *vis, float *gn, float *ms,
+                                    int32_t *radii_out, float *loss_out, const RefPrior *prior, float *depth_out,
+                                    float *final_T_out) {
+    try {
+        Scene s(N, params[0], params[1], params[2], params[3], params[4], rest_bases, params[5], viewmat, projmat, bg);
+        MTensor P[6] = {s.means, s.scales, s.quats, s.dc, s.rest, s.opac};
+        MTensor A[6], B[6];
+        for (int g = 0; g < 6; g++) {
+            A[g] = upload(m1[g], P[g].shape(), DType::Float32);
+            B[g] = upload(m2[g], P[g].shape(), DType::Float32);
+        }
+        MTensor gtt = upload(gt, {H, W, 3}, DType::Float32);
+        MTensor win = gpu_zeros({11, 11}, DType::Float32);
+        MTensor tv = upload(vis, {N}, DType::Float32), tg = upload(gn, {N}, DType::Float32),
+                tm = upload(ms, {N}, DType::Float32);
+        float ss[6], bb[6], cam[3] = {cam_pos[0], cam_pos[1], cam_pos[2]};
+        for (int g = 0; g < 6; g++) { ss[g] = steps[g]; bb[g] = bc2s[g]; }
+        PriorTensors pt(*prior);
+        auto [radii, loss] = msplat_train_step(
+            N, s.means, s.scales, 1.0f, s.quats, s.vm, s.pm, fx, fy, cx, cy, (unsigned)H, (unsigned)W, tiles(W, H), clip,
+            (unsigned)degree, (unsigned)dtu, cam, s.dc, s.rest, s.opac, s.bg, gtt, win, ssim_weight,
+            1.0f / (float)(H * W * 3), rest_bases, 6, P, A, B, ss, bb, b1, b2, eps, tv, tg, tm,
+            1.0f / (float)(H > W ? H : W), &pt.p);
+        msplat_gpu_sync();
+        for (int g = 0; g < 6; g++) {
+            download(params[g], P[g]);
+            download(m1[g], A[g]);
+            download(m2[g], B[g]);
+        }
+        download(vis, tv);
+        download(gn, tg);
+        download(ms, tm);
+        download(radii_out, radii);
+        pt.back(*prior);
+        if (depth_out || final_T_out) {
+            MTensor depth, fT;
+            msplat_render_aux_outputs(depth, fT);
+            if (depth_out && prior->aux) download(depth_out, depth);
+            if (final_T_out) download(final_T_out, fT);
+        }
+        if (loss_out) *loss_out = loss;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+// msplat_render with a PriorStep, and msplat_render_aux_outputs
+extern "C" int ref_render_prior(int N, const float *means, const float *scales, const float *quats, const float *dc,
+                                const float *rest, int rest_bases, const float *opac, const float *viewmat,
+                                const float *projmat, float fx, float fy, float cx, float cy, int H, int W, float clip,
+                                int degree, int dtu, const float *cam_pos, const float *bg, const RefPrior *prior,
+                                float *out_img, float *depth_out, float *final_T_out) {
+    try {
+        Scene s(N, means, scales, quats, dc, rest, rest_bases, opac, viewmat, projmat, bg);
+        float cam[3] = {cam_pos[0], cam_pos[1], cam_pos[2]};
+        PriorTensors pt(*prior);
+        MTensor img = msplat_render(N, s.means, s.scales, 1.0f, s.quats, s.vm, s.pm, fx, fy, cx, cy, (unsigned)H,
+                                    (unsigned)W, tiles(W, H), clip, (unsigned)degree, (unsigned)dtu, cam, s.dc, s.rest,
+                                    s.opac, s.bg, &pt.p);
+        msplat_gpu_sync();
+        download(out_img, img);
+        MTensor depth, fT;
+        msplat_render_aux_outputs(depth, fT);
+        if (depth_out && prior->aux) download(depth_out, depth);
+        if (final_T_out) download(final_T_out, fT);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
 // msplat_densify over capacity-sized buffers: bufs[0..5] params, bufs[6..11] first
 // moments, bufs[12..17] second moments, each with `cap` rows. Returns the new count.
 extern "C" int ref_densify(int N, int cap, float grad_thr, float size_thr, float screen_thr, int check_screen,

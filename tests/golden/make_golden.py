@@ -23,8 +23,8 @@ from msplat.camera import look_at, make_camera  # noqa: E402
 from msplat.scenes import random_scene  # noqa: E402
 
 sys.path.insert(0, os.path.dirname(HERE))
-from workloads import (C1_SAMPLE, C1_STEPS, RESIZE_CASES, UNDISTORT_CASES, c1_camera_order,  # noqa: E402
-                       c1_workload)
+from workloads import (C1_SAMPLE, C1_STEPS, PRIOR_VARIANTS, RESIZE_CASES, UNDISTORT_CASES,  # noqa: E402
+                       c1_camera_order, c1_workload, prior_workload)
 
 BG = np.array([0.613, 0.0101, 0.3984], np.float32)
 KEYS = ("means", "scales", "quats", "dc", "rest", "opac")
@@ -149,8 +149,30 @@ def scene_file_vectors(ref):
     return d
 
 
+def prior_step_vectors(ref):
+    """msplat_train_step with a PriorStep (cpu:1204-1327): the prior-guided case of
+    tests/workloads.py with every feature (aux rasterizer, learned sky, depth prior, sky
+    mask, fill, exposure, needle cap, unkept pixels), one step from zero moments. The
+    inputs are regenerated from their seed; the outputs are stored."""
+    sc, cam, gt, prior = prior_workload(seed=3, features=PRIOR_VARIANTS["all"])
+    st = {k: np.ascontiguousarray(sc[k], np.float32).copy() for k in KEYS}
+    for k in KEYS:
+        st["m1_" + k] = np.zeros_like(st[k])
+        st["m2_" + k] = np.zeros_like(st[k])
+    n = st["means"].shape[0]
+    st.update(degree=sc["degree"], vis=np.zeros(n, np.float32), gn=np.zeros(n, np.float32), ms=np.zeros(n, np.float32))
+    steps = np.array([1.6e-4, 5e-3, 1e-3, 2.5e-3, 1.25e-4, 5e-2], np.float32)
+    bc2 = np.full(6, np.float32(np.sqrt(np.float32(1 - 0.999))), np.float32)
+    loss, radii, depth, fT = ref.train_step_prior(st, cam, BG, gt, steps, bc2, prior)
+    d = {"loss": np.float32(loss), "depth": depth, "final_T": fT, "radii": radii}
+    d.update({"out_" + k: st[k] for k in KEYS})
+    d.update({"out_" + k: prior[k] for k in ("sky_tex", "expo_params", "loss_terms")})
+    return d
+
+
 EXTRA = {"opacity_reset": opacity_reset_vectors, "trajectory_c1": trajectory_c1_vectors,
-         "image_pipeline": image_pipeline_vectors, "scene_files": scene_file_vectors}
+         "image_pipeline": image_pipeline_vectors, "scene_files": scene_file_vectors,
+         "prior_step": prior_step_vectors}
 
 
 def main(only=()):
