@@ -120,6 +120,54 @@ int msplat_hip_train_step(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, co
 /* The loss of the last train step (host; syncs). */
 int msplat_hip_last_loss(msplat_hip_ctx *ctx, float *loss);
 
+/* ── prior-guided training (PriorStep, bindings.h:41-77) ─────────────────────
+ * The fields of PriorStep with device pointers for its tensors. aux rasterizes the
+ * expected-depth sum and alpha over a per-pixel background (the learned sky, else the
+ * constant) and is required by the sky and by every prior loss; sky: equirect texture
+ * (sky_h, sky_w, 3) seen by view direction in the frame sky_frame (up, azimuth 0, azimuth
+ * 90 deg), with its gradient and Adam moments (train only); prior_depth (prior_h,
+ * prior_w) float (0 = invalid) and prior_aux (prior_h, prior_w, 4) uint8 (confidence,
+ * sky, keep, -) at the prior's own resolution; loss_terms [4] float: depth, sky, fill, 0
+ * (rewritten every step); exposure: per-camera affine colour rows expo_params (num_cams,
+ * 12) with their moments and an extra gradient expo_grad [12] (cleared after use);
+ * log_max_scale_ratio > 0 caps each gaussian's largest scale at exp(ratio) x its middle
+ * one after the Adam step. A prior-guided frame is rasterized one workgroup per tile. */
+typedef struct {
+    int32_t aux, sky;
+    float *sky_tex, *sky_grad, *sky_exp_avg, *sky_exp_avg_sq;
+    int32_t sky_w, sky_h;
+    float sky_frame[9];
+    float sky_step_size, sky_bc2_sqrt;
+    const float *prior_depth;
+    const uint8_t *prior_aux;
+    int32_t prior_w, prior_h, has_depth, has_sky_mask;
+    float depth_weight, sky_weight, fill_weight, huber_delta, min_alpha;
+    int32_t mask_photometric;
+    float *loss_terms;
+    int32_t exposure, cam_index, num_cams;
+    float *expo_params, *expo_exp_avg, *expo_exp_avg_sq, *expo_grad;
+    float expo_step_size, expo_bc2_sqrt, expo_reg;
+    float log_max_scale_ratio;
+} msplat_hip_prior;
+
+/* msplat_train_step with a PriorStep (bindings.h:100-118, cpu:1204-1327): the aux
+ * forward, the photometric loss on the exposure-adjusted render, the prior losses (which
+ * mask the photometric gradient where the prior does not keep a pixel), the exposure and
+ * sky backward with their Adam steps, the aux raster backward (depth gradients move the
+ * means along the view axis), Adam on the six groups, the needle cap, the statistics.
+ * Other arguments as msplat_hip_train_step; loss_out is the photometric loss. */
+int msplat_hip_train_step_prior(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam,
+                                const float background[3], const float *gt, float ssim_weight, float loss_inv_n,
+                                const msplat_hip_adam *adam, const msplat_hip_grad_stats *stats, float inv_max_dim,
+                                int32_t *radii_out, float *loss_out, const msplat_hip_prior *prior);
+/* msplat_render with a PriorStep (bindings.h:81-91): with aux, the learned sky behind
+ * the gaussians and the aux outputs. */
+int msplat_hip_render_prior(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam,
+                            const float background[3], const msplat_hip_prior *prior, float **out_img);
+/* msplat_render_aux_outputs (bindings.h:93-96): after an aux frame, the expected-depth sum
+ * sum_i w_i z_i and the final transmittance (device (H, W), the context's caches). */
+int msplat_hip_render_aux_outputs(msplat_hip_ctx *ctx, float **depth, float **final_T);
+
 /* ── differentiable rendering (new: torch.autograd.Function backend) ─────────
  * render_backward takes dL/d(image) (device (H, W, 3)) of the last msplat_hip_render
  * on this context and writes the parameter gradients (device, reference layouts). The

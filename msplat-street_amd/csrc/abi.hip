@@ -35,6 +35,7 @@ namespace {
 // projection (per-block intersection and visible counts)
 int do_project(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam,
                const StepParams *sp = nullptr) {
+    ctx->frame_aux = false;  // a prior-guided entry point sets it after the projection
     const int n = g->num_points;
     const size_t N = (size_t)(n > 0 ? n : 1);
     const int nblocks = (n + 255) / 256;
@@ -279,6 +280,12 @@ RasterArgs raster_args(msplat_hip_ctx *ctx) {
     r.chunk_prod = (float *)ctx->chunk_prod.p;
     r.chunk_state = (float4 *)ctx->chunk_state.p;
     r.chunk_last = (int32_t *)ctx->chunk_last.p;
+    r.aux = ctx->frame_aux ? 1 : 0;
+    r.depths = ctx->frame_aux ? (const float *)ctx->depths.p : nullptr;
+    r.bg_img = ctx->frame_aux ? (const float *)ctx->bg_img.p : nullptr;
+    r.out_depth = ctx->frame_aux ? (float *)ctx->out_depth.p : nullptr;
+    r.v_aux = ctx->frame_aux ? (const float *)ctx->v_aux.p : nullptr;
+    r.slot_d = ctx->frame_aux ? (float *)ctx->slot_d.p : nullptr;
     return r;
 }
 
@@ -288,7 +295,7 @@ int do_raster_forward(msplat_hip_ctx *ctx, const float bg[3]) {
     for (int i = 0; i < 3; i++) ctx->bg[i] = bg ? bg[i] : 0.f;
     CHECKED(ensure_pixels(ctx));
     const int tc = ctx->tuning.chunks;
-    ctx->frame_chunks = ctx->M <= 0 ? 1 : tc == 0 ? auto_chunks(ctx->ntiles, ctx->M) : tc;
+    ctx->frame_chunks = ctx->M <= 0 || ctx->frame_aux ? 1 : tc == 0 ? auto_chunks(ctx->ntiles, ctx->M) : tc;
     if (ctx->frame_chunks > 1) {
         const size_t planes = (size_t)ctx->frame_chunks * ctx->width * ctx->height;
         CHECKED(ctx->grow(ctx->chunk_prod, 4 * planes));
@@ -303,18 +310,20 @@ int do_raster_forward(msplat_hip_ctx *ctx, const float bg[3]) {
     return MSPLAT_OK;
 }
 
-int do_loss(msplat_hip_ctx *ctx, const float *gt, float ssim_weight, float inv_n, const StepParams *sp = nullptr) {
+// The photometric loss of the frame's image (or of `img`, with its gradient into `v_img`)
+int do_loss(msplat_hip_ctx *ctx, const float *gt, float ssim_weight, float inv_n, const StepParams *sp = nullptr,
+            const float *img = nullptr, float *v_img = nullptr) {
     if (ctx->frame_stage < msplat_hip_ctx::FRAME_RASTERED)
         return set_error(MSPLAT_ERR_ARG, "the loss needs a rendered frame on this context");
     LossArgs l;
     l.width = ctx->width;
     l.height = ctx->height;
-    l.img = (const float *)ctx->image.p;
+    l.img = img ? img : (const float *)ctx->image.p;
     l.gt = gt;
     l.sp = sp;
     l.ssim_weight = ssim_weight;
     l.inv_n = inv_n;
-    l.v_img = (float *)ctx->v_image.p;
+    l.v_img = v_img ? v_img : (float *)ctx->v_image.p;
     CHECKED(ensure_loss_scratch(ctx));
     l.scratch = (float *)ctx->loss_scratch.p;
     l.row_loss = (double *)ctx->row_loss.p;
@@ -335,6 +344,7 @@ int do_raster_backward(msplat_hip_ctx *ctx) {
     CHECKED(ctx->grow(ctx->slot_b, 16 * Mc));
     CHECKED(ctx->grow(ctx->slot_c, 4 * Mc));
     CHECKED(ctx->grow(ctx->slot_f, (Mc + 3) & ~(size_t)3));
+    if (ctx->frame_aux) CHECKED(ctx->grow(ctx->slot_d, 4 * Mc));
     // the zero flags of the frame's slots: cleared by its key emission, or here when an
     // earlier backward of the same frame has set some
     if (!ctx->slot_f_clean) {
@@ -385,6 +395,11 @@ int backward_args(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const mspl
         b.big_a = ba;
         b.big_b = ba + N;
         b.big_c = (float *)(ba + 2 * N);
+        if (ctx->frame_aux) {
+            CHECKED(ctx->grow(ctx->big_d, 4 * N));
+            b.slot_d = (const float *)ctx->slot_d.p;
+            b.big_d = (float *)ctx->big_d.p;
+        }
     }
     if (ctx->flags & MSPLAT_FRAME_KEEP_GRADS) {
         const size_t N = (size_t)(b.n > 0 ? b.n : 1);
@@ -575,6 +590,7 @@ void restore_step_frame(msplat_hip_ctx *ctx) {
     for (int i = 0; i < 3; i++) ctx->bg[i] = f.bg[i];
     ctx->frame_stage = msplat_hip_ctx::FRAME_RASTERED;
     ctx->slot_f_clean = false;
+    ctx->frame_aux = false;
 }
 
 // Waits for the replays in flight; runs again, launch by launch, those an overflow left
@@ -654,6 +670,133 @@ int graph_step(msplat_hip_ctx *ctx, const StepRecord &r, float *loss_out) {
     if (loss_out) {
         CHECKED(settle_steps(ctx));  // a re-run (if any) ends with this step, whose loss is then the last
         HIPCHECK(hipMemcpy(loss_out, ctx->d_loss(), 4, hipMemcpyDeviceToHost));
+    }
+    return MSPLAT_OK;
+}
+
+// ── prior-guided frames (PriorStep) ───────────────────────────────────────────
+int check_prior(const msplat_hip_prior *p, bool train) {
+    if (!p) return set_error(MSPLAT_ERR_ARG, "prior is NULL");
+    if (p->aux && p->sky && p->sky_tex && (p->sky_w <= 0 || p->sky_h <= 0))
+        return set_error(MSPLAT_ERR_ARG, "sky texture size must be positive");
+    if (train && p->aux && p->sky && p->sky_tex && p->sky_grad && (!p->sky_exp_avg || !p->sky_exp_avg_sq))
+        return set_error(MSPLAT_ERR_ARG, "sky Adam moments are NULL");
+    if (p->aux && (p->prior_depth || p->prior_aux) &&
+        (!p->prior_depth || !p->prior_aux || p->prior_w <= 0 || p->prior_h <= 0))
+        return set_error(MSPLAT_ERR_ARG, "a prior needs both its depth and aux images and a positive size");
+    if (train && p->exposure && p->expo_params) {
+        if (!p->expo_exp_avg || !p->expo_exp_avg_sq) return set_error(MSPLAT_ERR_ARG, "exposure moments are NULL");
+        if (p->num_cams <= 0 || std::max(0, p->cam_index) >= p->num_cams)
+            return set_error(MSPLAT_ERR_ARG, "exposure cam_index %d outside [0, %d)", p->cam_index, p->num_cams);
+    }
+    return MSPLAT_OK;
+}
+
+SkyArgs sky_args(const msplat_hip_prior *p, const msplat_hip_camera *cam, bool on) {
+    SkyArgs k;
+    memset(&k, 0, sizeof k);
+    for (int i = 0; i < 3; i++) {
+        k.up[i] = p->sky_frame[i];
+        k.e1[i] = p->sky_frame[3 + i];
+        k.e2[i] = p->sky_frame[6 + i];
+    }
+    k.fx = cam->fx;
+    k.fy = cam->fy;
+    k.cx = cam->cx;
+    k.cy = cam->cy;
+    for (int i = 0; i < 12; i++) k.vm[i] = cam->viewmat[i];
+    k.tex = on ? p->sky_tex : nullptr;
+    k.w = on ? (uint32_t)p->sky_w : 1u;
+    k.h = on ? (uint32_t)p->sky_h : 1u;
+    return k;
+}
+
+// An aux frame after its projection: the per-pixel background (the learned sky where
+// it is on) and the aux buffers
+int prepare_aux(msplat_hip_ctx *ctx, const msplat_hip_prior *p, const msplat_hip_camera *cam, const float bg[3]) {
+    ctx->frame_aux = true;
+    const size_t P = (size_t)cam->width * cam->height;
+    CHECKED(ctx->grow(ctx->bg_img, 12 * P));
+    CHECKED(ctx->grow(ctx->out_depth, 4 * P));
+    CHECKED(ctx->grow(ctx->v_aux, 8 * P));
+    launch_sky_background(sky_args(p, cam, p->sky && p->sky_tex), cam->width, cam->height, bg, (float *)ctx->bg_img.p,
+                          ctx->stream);
+    HIPCHECK(hipGetLastError());
+    return MSPLAT_OK;
+}
+
+// msplat_train_step with a PriorStep (cpu:1219-1327), launch by launch
+int prior_step(msplat_hip_ctx *ctx, const StepRecord &r, const msplat_hip_prior *p, float *loss_out) {
+    const bool aux = p->aux != 0;
+    const bool sky = aux && p->sky && p->sky_tex && p->sky_grad;
+    const bool exposure = p->exposure && p->expo_params;
+    const msplat_hip_camera *cam = &r.cam;
+    const uint32_t W = cam->width, H = cam->height;
+    const size_t P = (size_t)W * H;
+    CHECKED(do_project(ctx, &r.g, cam));
+    if (aux) CHECKED(prepare_aux(ctx, p, cam, r.bg));
+    CHECKED(do_bin(ctx));
+    CHECKED(do_raster_forward(ctx, r.bg));
+    hipStream_t s = ctx->stream;
+    // the photometric loss, on the exposure-adjusted render with exposure
+    float *loss_grad = (float *)ctx->v_image.p;
+    if (exposure) {
+        CHECKED(ctx->grow(ctx->adj, 12 * P));
+        CHECKED(ctx->grow(ctx->v_adj, 12 * P));
+        launch_exposure_apply((const float *)ctx->image.p, p->expo_params + 12 * (size_t)std::max(0, p->cam_index), P,
+                              (float *)ctx->adj.p, s);
+        loss_grad = (float *)ctx->v_adj.p;
+        CHECKED(do_loss(ctx, r.gt, r.ssim_weight, r.inv_n, nullptr, (const float *)ctx->adj.p, loss_grad));
+    } else {
+        CHECKED(do_loss(ctx, r.gt, r.ssim_weight, r.inv_n));
+    }
+    if (aux) {  // prior losses: dL/dD, dL/dA; the photometric gradient masked
+        CHECKED(ctx->grow(ctx->prior_rows, 24 * (size_t)H));
+        PriorLossArgs a;
+        memset(&a, 0, sizeof a);
+        a.width = W;
+        a.height = H;
+        a.final_T = (const float *)ctx->final_T.p;
+        a.out_depth = (const float *)ctx->out_depth.p;
+        a.v_photo = loss_grad;
+        a.v_aux = (float *)ctx->v_aux.p;
+        a.rows = (double *)ctx->prior_rows.p;
+        const bool has_prior = p->prior_depth && p->prior_aux && p->prior_w > 0 && p->prior_h > 0;
+        a.prior_depth = has_prior ? p->prior_depth : nullptr;
+        a.prior_aux = has_prior ? p->prior_aux : nullptr;
+        a.pw = has_prior ? (uint32_t)p->prior_w : 1u;
+        a.ph = has_prior ? (uint32_t)p->prior_h : 1u;
+        a.has_depth = has_prior && p->has_depth;
+        a.has_sky = has_prior && p->has_sky_mask;
+        a.depth_weight = p->depth_weight;
+        a.sky_weight = p->sky_weight;
+        a.fill_weight = p->fill_weight;
+        a.huber = std::max(p->huber_delta, 1e-6f);
+        a.min_alpha = p->min_alpha;
+        a.mask_photometric = p->mask_photometric;
+        launch_prior_loss(a, p->loss_terms, s);
+    }
+    if (exposure) {  // v_image = M^T v_adj, and the exposure row's Adam step
+        CHECKED(ctx->grow(ctx->expo_part, 96 * std::max<size_t>(exposure_partials(P), 1)));
+        launch_exposure_backward_adam((const float *)ctx->image.p, (const float *)ctx->v_adj.p, P,
+                                      (float *)ctx->v_image.p, (double *)ctx->expo_part.p, p->expo_params,
+                                      p->expo_exp_avg, p->expo_exp_avg_sq, p->expo_grad, std::max(0, p->cam_index),
+                                      p->expo_step_size, p->expo_bc2_sqrt, p->expo_reg, r.adam.beta1, r.adam.beta2,
+                                      r.adam.eps, s);
+    }
+    if (sky)
+        launch_sky_backward_adam(sky_args(p, cam, true), W, H, (const float *)ctx->final_T.p,
+                                 (const float *)ctx->v_image.p, p->sky_grad, p->sky_tex, p->sky_exp_avg,
+                                 p->sky_exp_avg_sq, p->sky_step_size, p->sky_bc2_sqrt, r.adam.beta1, r.adam.beta2,
+                                 r.adam.eps, s);
+    HIPCHECK(hipGetLastError());
+    CHECKED(do_raster_backward(ctx));
+    CHECKED(eager_step_tail(ctx, r));
+    if (p->log_max_scale_ratio > 0.f) launch_scale_cap(r.g.scales, r.g.num_points, p->log_max_scale_ratio, s);
+    HIPCHECK(hipGetLastError());
+    if (loss_out) {
+        HIPCHECK(hipMemcpyAsync(loss_out, ctx->d_loss(), 4, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
     }
     return MSPLAT_OK;
 }
@@ -856,6 +999,60 @@ int msplat_hip_train_step(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, co
     return eager_step(ctx, r, loss_out);
 }
 
+int msplat_hip_train_step_prior(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam,
+                                const float background[3], const float *gt, float ssim_weight, float loss_inv_n,
+                                const msplat_hip_adam *adam, const msplat_hip_grad_stats *stats, float inv_max_dim,
+                                int32_t *radii_out, float *loss_out, const msplat_hip_prior *prior) {
+    if (!ctx || !gt || !adam || !stats || !background)
+        return set_error(MSPLAT_ERR_ARG, "ctx, background, gt, adam or stats is NULL");
+    CHECKED(settle_steps(ctx));
+    CHECKED(check_gaussians(g));
+    CHECKED(check_camera(cam));
+    CHECKED(check_prior(prior, true));
+    for (int i = 0; i < 6; i++)
+        if (g->num_points > 0 && (!adam->exp_avg[i] || !adam->exp_avg_sq[i]))
+            return set_error(MSPLAT_ERR_ARG, "Adam moment %d is NULL", i);
+    if (g->num_points > 0 && cam->degree > 0 && !g->features_rest)
+        return set_error(MSPLAT_ERR_ARG, "features_rest is NULL");
+    StepRecord r;
+    r.g = *g;
+    r.cam = *cam;
+    for (int i = 0; i < 3; i++) r.bg[i] = background[i];
+    r.gt = gt;
+    r.ssim_weight = ssim_weight;
+    r.inv_n = loss_inv_n;
+    r.inv_max = inv_max_dim;
+    r.adam = *adam;
+    r.stats = *stats;
+    r.radii_out = radii_out;
+    return prior_step(ctx, r, prior, loss_out);
+}
+
+int msplat_hip_render_prior(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, const msplat_hip_camera *cam,
+                            const float background[3], const msplat_hip_prior *prior, float **out_img) {
+    if (!ctx || !background) return set_error(MSPLAT_ERR_ARG, "ctx or background is NULL");
+    CHECKED(settle_steps(ctx));
+    CHECKED(check_gaussians(g));
+    CHECKED(check_camera(cam));
+    CHECKED(check_prior(prior, false));
+    CHECKED(do_project(ctx, g, cam));
+    if (prior->aux) CHECKED(prepare_aux(ctx, prior, cam, background));
+    CHECKED(do_bin(ctx));
+    CHECKED(do_raster_forward(ctx, background));
+    if (out_img) *out_img = (float *)ctx->image.p;
+    return MSPLAT_OK;
+}
+
+int msplat_hip_render_aux_outputs(msplat_hip_ctx *ctx, float **depth, float **final_T) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
+    if (ctx->frame_stage < msplat_hip_ctx::FRAME_RASTERED || !ctx->frame_aux)
+        return set_error(MSPLAT_ERR_ARG, "no prior-guided (aux) frame on this context");
+    if (depth) *depth = (float *)ctx->out_depth.p;
+    if (final_T) *final_T = (float *)ctx->final_T.p;
+    return MSPLAT_OK;
+}
+
 int msplat_hip_set_step_graph(msplat_hip_ctx *ctx, int enable, uint64_t initial_capacity) {
     if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
     if (enable && !ctx->stream) return set_error(MSPLAT_ERR_ARG, "graph capture needs a non-null stream");
@@ -899,6 +1096,9 @@ int msplat_hip_render_backward(msplat_hip_ctx *ctx, const msplat_hip_gaussians *
     (void)background;
     const size_t P = (size_t)ctx->width * ctx->height;
     HIPCHECK(hipMemcpyAsync(ctx->v_image.p, v_img, 12 * P, hipMemcpyDeviceToDevice, ctx->stream));
+    if (ctx->frame_aux)  // a prior-guided render: no prior-loss gradients, only the image's
+        zero_u32<<<(unsigned)std::min<size_t>((2 * P + 255) / 256, 4096), 256, 0, ctx->stream>>>(
+            (uint32_t *)ctx->v_aux.p, 2 * P);
     CHECKED(do_raster_backward(ctx));
     BackwardArgs b;
     CHECKED(backward_args(ctx, g, cam, b));

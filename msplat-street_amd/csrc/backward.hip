@@ -87,6 +87,16 @@ __device__ inline void sum_slots(const float4 *sa, const float4 *sb, const float
     }
 }
 
+// Sum of the written depth slots among o + (set bits of m), in increasing order
+__device__ inline float sum_slot_d(const float *sd, uint32_t o, uint64_t m) {
+    float v = 0.f;
+    while (m) {
+        v += sd[o + (uint32_t)__builtin_ctzll(m)];
+        m &= m - 1ull;
+    }
+    return v;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
     const int g = blockIdx.x * 256 + threadIdx.x;
@@ -116,19 +126,22 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
         d2[10] = a.m2[5][g];
     }
     const bool vis = a.radii[g] > 0;
-    float vxy[2] = {0.f, 0.f}, vcn[3] = {0.f, 0.f, 0.f}, vrgb[3] = {0.f, 0.f, 0.f}, vop = 0.f;
+    float vxy[2] = {0.f, 0.f}, vcn[3] = {0.f, 0.f, 0.f}, vrgb[3] = {0.f, 0.f, 0.f}, vop = 0.f, vdep = 0.f;
     if (vis && a.count[g] > BIG_SLOTS) {  // reduced by big_reduce_kernel
         const float4 A = a.big_a[g], B = a.big_b[g];
         vxy[0] = A.x; vxy[1] = A.y; vcn[0] = A.z; vcn[1] = A.w;
         vcn[2] = B.x; vrgb[0] = B.y; vrgb[1] = B.z; vrgb[2] = B.w;
         vop = a.big_c[g];
+        if (a.slot_d) vdep = a.big_d[g];
     } else if (vis) {  // the written slots, in tile order
         const uint32_t o = a.offsets[g];
         float v[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        sum_slots(a.slot_a, a.slot_b, a.slot_c, o, slot_mask(a.slot_f, o, a.count[g]), v);
+        const uint64_t m = slot_mask(a.slot_f, o, a.count[g]);
+        sum_slots(a.slot_a, a.slot_b, a.slot_c, o, m, v);
         vxy[0] = v[0]; vxy[1] = v[1]; vcn[0] = v[2]; vcn[1] = v[3];
         vcn[2] = v[4]; vrgb[0] = v[5]; vrgb[1] = v[6]; vrgb[2] = v[7];
         vop = v[8];
+        if (a.slot_d) vdep = sum_slot_d(a.slot_d, o, m);
     }
     if (a.v_xy) {
         a.v_xy[2 * g] = vxy[0]; a.v_xy[2 * g + 1] = vxy[1];
@@ -142,9 +155,9 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
         const float p[3] = {a.means[3 * g], a.means[3 * g + 1], a.means[3 * g + 2]};
         pixel_vjp(cam.projmat, p, cam.width, cam.height, vxy[0], vxy[1], vm);
         // + viewmat row 2 * v_depth, with v_depth = 0 outside the prior-guided path
-        vm[0] = vm[0] + cam.viewmat[8] * 0.f;
-        vm[1] = vm[1] + cam.viewmat[9] * 0.f;
-        vm[2] = vm[2] + cam.viewmat[10] * 0.f;
+        vm[0] = vm[0] + cam.viewmat[8] * vdep;
+        vm[1] = vm[1] + cam.viewmat[9] * vdep;
+        vm[2] = vm[2] + cam.viewmat[10] * vdep;
         const float4 SA = a.splat_a[g], SB = a.splat_b[g];
         const float cn[3] = {SA.z, SA.w, SB.x};
         float vc2[3];
@@ -232,11 +245,14 @@ __global__ __launch_bounds__(256) void big_reduce_kernel(BackwardArgs a) {
     for (uint32_t i = wave; i < nbig; i += nwaves) {
         const uint32_t o = a.offsets[g], end = o + a.count[g];
         const uint32_t gn = i + nwaves < nbig ? a.big_list[i + nwaves] : 0u;
-        float v[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        float v[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, vd = 0.f;
         for (uint32_t p0 = o + lane; p0 < end; p0 += 256) {
             bool f[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) f[u] = p0 + 64 * u < end && a.slot_f[p0 + 64 * u] != 0;
+            if (a.slot_d)
+#pragma unroll
+                for (int u = 0; u < 4; u++) vd += f[u] ? a.slot_d[p0 + 64 * u] : 0.f;
             float4 A[4], B[4];
             float Cs[4];
 #pragma unroll
@@ -259,6 +275,10 @@ __global__ __launch_bounds__(256) void big_reduce_kernel(BackwardArgs a) {
             }
         }
         const float t = wave_sums9(sl, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]);
+        if (a.slot_d) {
+            for (int dd = 32; dd >= 1; dd >>= 1) vd += __shfl_xor(vd, dd);
+            if (lane == 0) a.big_d[g] = vd;
+        }
         if (lane < 9) {
             float *dst = sl.q < 4 ? &a.big_a[g].x + sl.q : sl.q < 8 ? &a.big_b[g].x + (sl.q - 4) : &a.big_c[g];
             *dst = t;

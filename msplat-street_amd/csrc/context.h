@@ -116,6 +116,11 @@ struct msplat_hip_ctx {
     Buf tile_start, image, final_T, final_idx, v_image, row_loss, loss_scratch;
     Buf grad_scratch;  // stands in for NULL gradient outputs of render_backward
     Buf chunk_prod, chunk_state, chunk_last;  // depth-chunked rasterization planes
+    // prior-guided frames: per pixel background, expected-depth sum, dL/dD and dL/dA, the
+    // exposure-adjusted render and its gradient; per row prior terms; exposure partials;
+    // per intersection depth slots and per gaussian big-record sums of them
+    Buf bg_img, out_depth, v_aux, adj, v_adj, prior_rows, expo_part, slot_d, big_d;
+    bool frame_aux = false;  // the frame in the workspace is a prior-guided (aux) one
     int frame_chunks = 1;                     // depth chunks per tile of the frame's forward
     Buf aux;           // densify flags/scans/scratch, kNN and metrics workspaces
     int32_t *vis_sorted = nullptr;  // visible gaussians in depth order (the frame's list)
@@ -251,7 +256,8 @@ struct msplat_hip_ctx {
                       &v_xy, &v_conic, &v_rgb, &v_opac, &sh_scratch, &big, &keys0, &keys1, &vals0, &vals1, &slot_a, &slot_b, &slot_c, &slot_f,
                       &hist, &digit_total, &keys64, &block_vis, &dk0, &dk1, &dv0, &dv1, &ord_sums, &scan_part, &loff, &tile_start, &image, &final_T, &final_idx, &v_image,
                       &row_loss, &loss_scratch, &scalars, &grad_scratch, &aux, &chunk_prod, &chunk_state,
-                      &chunk_last};
+                      &chunk_last, &bg_img, &out_depth, &v_aux, &adj, &v_adj, &prior_rows, &expo_part, &slot_d,
+                      &big_d};
         for (Buf *b : all) release(*b);
         if (batch) msplat::destroy_batch_state(batch);
         if (step_exec) (void)hipGraphExecDestroy(step_exec);

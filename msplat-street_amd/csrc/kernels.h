@@ -121,6 +121,14 @@ struct RasterArgs {
     float *chunk_prod;
     float4 *chunk_state;
     int32_t *chunk_last;
+    // prior-guided (aux) frames (0: none): per-gaussian view depth, per-pixel background,
+    // expected-depth sum out, prior-loss gradients (dL/dD, dL/dA) in, depth slots out
+    int aux;
+    const float *depths;
+    const float *bg_img;
+    float *out_depth;
+    const float *v_aux;
+    float *slot_d;
 };
 
 // Loss blocks: 8 rows x 118 columns; LOSS_PLANES floats of scratch per pixel (15
@@ -151,6 +159,10 @@ struct BackwardArgs {
     msplat_hip_camera cam;
     const StepParams *sp;      // non-null: camera, step sizes and bias corrections from here
     const uint32_t *overflow;  // non-null and set: the frame was over capacity, Adam does nothing
+    // prior-guided frames: the intersections' depth gradient slots (null: none) and the
+    // big records' sums of them; their sum v_depth moves the mean along the view axis
+    const float *slot_d;
+    float *big_d;
     const int32_t *radii;
     const uint32_t *count, *offsets;
     const float4 *splat_a, *splat_b;
@@ -212,6 +224,46 @@ void knn_scratch_carve(void *base, uint32_t n, KnnScratch &w);
 int knn3_launch(const float *pts, uint32_t n, float *mean_dist, KnnScratch &w, hipStream_t s);
 void launch_init_gaussians(const float *md, const uint8_t *rgb, uint32_t n, uint32_t seed, float logit, float *scales,
                            float *quats, float *dc, float *opac, hipStream_t s);
+
+// ── prior-guided extras (PriorStep, core/metal/bindings.h:43-77) ────────────────
+// The learned sky's equirect frame and texture and the camera it is seen from
+struct SkyArgs {
+    float up[3], e1[3], e2[3];
+    float fx, fy, cx, cy;
+    float vm[12];  // world-to-camera rows 0-2
+    const float *tex;  // (h, w, 3), or null: the constant background
+    uint32_t w, h;
+};
+
+// The prior losses of one frame (prior_loss, cpu:809-884)
+struct PriorLossArgs {
+    uint32_t width, height;
+    const float *final_T, *out_depth;
+    float *v_photo;  // the photometric gradient, zeroed where the prior does not keep the pixel
+    float *v_aux;    // out: dL/dD, dL/dA per pixel
+    double *rows;    // out: per row the depth, sky and fill terms
+    const float *prior_depth;  // (ph, pw), or null: no prior image (every pixel kept, no sky)
+    const uint8_t *prior_aux;  // (ph, pw, 4): confidence, sky, keep, -
+    uint32_t pw, ph;
+    int has_depth, has_sky;
+    float depth_weight, sky_weight, fill_weight, huber, min_alpha;
+    int mask_photometric;
+};
+
+void launch_sky_background(const SkyArgs &k, uint32_t W, uint32_t H, const float bg[3], float *bg_img, hipStream_t s);
+void launch_exposure_apply(const float *img, const float *P, size_t n, float *out, hipStream_t s);
+void launch_prior_loss(const PriorLossArgs &a, float *loss_terms, hipStream_t s);
+// v_img = P^T v_adj per pixel, the 12 gradient sums (double partials in `part`, one row of
+// 12 per block), then one Adam step on the camera's row (`row` = params + 12 cam)
+void launch_exposure_backward_adam(const float *img, const float *v_adj, size_t n, float *v_img, double *part,
+                                   float *params, float *m1, float *m2, float *grad, int cam, float step, float bc2,
+                                   float reg, float b1, float b2, float eps, hipStream_t s);
+size_t exposure_partials(size_t n);
+// the sky texture's gradient (zeroed first) from T v_img at each pixel's taps, then Adam
+void launch_sky_backward_adam(const SkyArgs &k, uint32_t W, uint32_t H, const float *final_T, const float *v_img,
+                              float *grad, float *tex, float *m1, float *m2, float step, float bc2, float b1, float b2,
+                              float eps, hipStream_t s);
+void launch_scale_cap(float *scales, int n, float log_max, hipStream_t s);
 
 // ── batched views ───────────────────────────────────────────────────────────
 // Several views of one scene go through one projection, one depth sort of the visible

@@ -9,12 +9,13 @@ namespace {
 
 __device__ inline TileData tile_data(const RasterArgs &a) {
     return TileData{a.tile_start, a.gid,    a.gid_mask, a.splat_a, a.splat_b, a.splat_blue,     a.rect,
-                    a.offsets,    a.slot_a, a.slot_b,   a.slot_c,  a.slot_f,  a.slots_written, a.cull != 0};
+                    a.offsets,    a.slot_a, a.slot_b,   a.slot_c,  a.slot_f,  a.slots_written, a.cull != 0,
+                    a.depths,     a.slot_d};
 }
 
 __device__ inline TileView tile_view(const RasterArgs &a) {
-    return TileView{a.width, a.height, a.tiles_x, {a.bg[0], a.bg[1], a.bg[2]}, a.image, a.final_T, a.final_idx,
-                    a.v_image};
+    return TileView{a.width,   a.height, a.tiles_x, {a.bg[0], a.bg[1], a.bg[2]}, a.image, a.final_T, a.final_idx,
+                    a.v_image, a.bg_img, a.out_depth, a.v_aux};
 }
 
 __device__ inline TileData tile_data(const BatchRasterArgs &a) {
@@ -78,16 +79,16 @@ __device__ inline uint32_t next_item(ItemQueue &q, uint32_t *slot) {
 
 // Forward kernels are capped at 64 VGPRs (8 waves per SIMD): the unrolled compositing
 // loop would otherwise hold more registers than the extra waves are worth.
-template <int FB>
+template <int FB, bool AUX = false>
 __global__ __launch_bounds__(256, 8) void raster_forward_kernel(RasterArgs a) {
     const int t = xcd_item(blockIdx.x, gridDim.x);
-    forward_tile<FB>(tile_view(a), tile_data(a), t, t);
+    forward_tile<FB, AUX>(tile_view(a), tile_data(a), t, t);
 }
 
-template <int PPT, int BB>
+template <int PPT, int BB, bool AUX = false>
 __global__ __launch_bounds__(256 / PPT) void raster_backward_kernel(RasterArgs a) {
     const int t = xcd_item(blockIdx.x, gridDim.x);
-    backward_tile<PPT, BB>(tile_view(a), tile_data(a), t, t);
+    backward_tile<PPT, BB, false, AUX>(tile_view(a), tile_data(a), t, t);
 }
 
 __device__ inline ChunkView chunk_view(const RasterArgs &a, int k) {
@@ -262,6 +263,10 @@ void launch_raster_forward(const RasterArgs &args, const Tuning &t, hipStream_t 
     a.cull = t.cull;
     const int nt = a.tiles_x * a.tiles_y;
     if (nt <= 0) return;
+    if (a.aux) {  // one variant: the prior-guided path is not the tuned one
+        raster_forward_kernel<256, true><<<nt, 256, 0, s>>>(a);
+        return;
+    }
     if (a.chunks > 1) {
         if (t.fwd_batch == 64) forward_chunked<64>(a, nt, s);
         else if (t.fwd_batch == 128) forward_chunked<128>(a, nt, s);
@@ -275,6 +280,10 @@ void launch_raster_forward(const RasterArgs &args, const Tuning &t, hipStream_t 
 
 template <int PPT>
 static void backward_ppt(const RasterArgs &a, int nt, int bb, hipStream_t s) {
+    if (a.aux) {
+        raster_backward_kernel<PPT, 64, true><<<nt, 256 / PPT, 0, s>>>(a);
+        return;
+    }
     if (a.chunks > 1) {
         const int g = nt * a.chunks;
         raster_chunk_suffix_kernel<<<dim3((a.width + 15) / 16, (a.height + 15) / 16), 256, 0, s>>>(a);

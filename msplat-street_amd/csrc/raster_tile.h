@@ -28,6 +28,11 @@ struct TileView {
     float *image, *final_T;
     int32_t *final_idx;
     const float *v_image;
+    // prior-guided (aux) frames: the per-pixel background, the expected-depth sum, and
+    // the prior losses' gradients dL/dD, dL/dA (2 per pixel)
+    const float *bg_img = nullptr;
+    float *out_depth = nullptr;
+    const float *v_aux = nullptr;
 };
 
 // Buffers shared by every view: ranges over global tile ids, the sorted record ids,
@@ -44,6 +49,9 @@ struct TileData {
     uint8_t *slot_f;  // 1: the slot's nine values were written; 0 (set before the launch): all zero
     unsigned long long *slots_written;  // null, or a running count of written slots
     bool cull;  // skip gaussians that splat_misses_tile proves invisible in the tile
+    // aux frames: per record view depth, and the intersections' depth gradient slots
+    const float *depths = nullptr;
+    float *slot_d = nullptr;
 };
 
 // True when the splat composites no pixel of the tile whose top-left pixel is (x0, y0):
@@ -99,13 +107,16 @@ __device__ inline int kept_position(bool keep, int *wcount, int &total) {
 // nd_rasterize_forward (core/cpu/msplat_cpu.cpp:574-626) for global tile `gt`, which is
 // tile `lt` of view `v`. Every thread of the 256-thread workgroup calls it; FB gaussian
 // records are staged in LDS per batch (FB <= 256: one record per thread).
-template <int FB>
+// With AUX (rasterize_forward_aux, cpu:574-626 with f.aux): also the expected-depth sum
+// sum_i z_i alpha_i T_i into out_depth, over the per-pixel background bg_img.
+template <int FB, bool AUX = false>
 __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt, int lt) {
 #pragma clang fp contract(fast)
     static_assert(FB <= 256, "one staged record per thread");
     __shared__ float4 la[FB], lb[FB];
     __shared__ float lc[FB];
     __shared__ int lk[FB];  // the staged record's position in the tile's list
+    __shared__ float lz[AUX ? FB : 1];  // the record's view depth
     __shared__ int wcount[4];
     const int tx = lt % v.tiles_x, ty = lt / v.tiles_x;
     const float x0 = (float)(tx * TILE), y0 = (float)(ty * TILE);
@@ -115,7 +126,7 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
     const bool inside = px < (int)v.width && py < (int)v.height;
     const int s = d.tile_start[gt], e = d.tile_start[gt + 1];
     const float fx = (float)px, fy = (float)py;
-    float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f;
+    float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, depth = 0.f;
     int last = s - 1;
     bool done = !inside;
     for (int base = s; base < e; base += FB) {
@@ -124,12 +135,13 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
         const int k = base + threadIdx.x;
         bool keep = false;
         float4 A0, B0;
-        float C0 = 0.f;
+        float C0 = 0.f, Z0 = 0.f;
         if (threadIdx.x < FB && k < e) {
             const int g = (int)((uint32_t)d.gid[k] & d.gid_mask);
             A0 = d.splat_a[g];
             B0 = d.splat_b[g];
             C0 = d.splat_blue[g];
+            if (AUX) Z0 = d.depths[g];
             keep = !(d.cull && splat_misses_tile(A0, B0, x0, y0));
         }
         int n;
@@ -139,6 +151,7 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
             lb[pos] = make_float4(B0.x, B0.y, fmaxf(B0.z + 0.5f, 0.f), fmaxf(B0.w + 0.5f, 0.f));
             lc[pos] = fmaxf(C0 + 0.5f, 0.f);
             lk[pos] = k;
+            if (AUX) lz[pos] = Z0;
         }
         // pad to a multiple of 8 with records of zero opacity (alpha 0 < ALPHA_MIN: they
         // composite nothing), so that the loop below runs whole groups of 8
@@ -148,6 +161,7 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
             lb[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
             lc[threadIdx.x] = 0.f;
             lk[threadIdx.x] = 0;
+            if (AUX) lz[threadIdx.x] = 0.f;
         }
         __syncthreads();
         if (__ballot(!done) != 0) {
@@ -170,6 +184,7 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
                     cr += B.z * vis;
                     cg += B.w * vis;
                     cb += lc[j] * vis;
+                    if (AUX) depth += lz[j] * vis;
                     T = accept ? nT : T;
                     const int kj = lk[j];
                     last = term ? kj - 1 : (accept ? kj : last);
@@ -183,9 +198,16 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
         const size_t pid = (size_t)py * v.width + px;
         v.final_T[pid] = T;
         v.final_idx[pid] = last;
-        v.image[3 * pid] = fminf(fmaxf(v.bg[0] * T + cr, 0.f), 1.f);
-        v.image[3 * pid + 1] = fminf(fmaxf(v.bg[1] * T + cg, 0.f), 1.f);
-        v.image[3 * pid + 2] = fminf(fmaxf(v.bg[2] * T + cb, 0.f), 1.f);
+        float bg0 = v.bg[0], bg1 = v.bg[1], bg2 = v.bg[2];
+        if (AUX) {
+            bg0 = v.bg_img[3 * pid];
+            bg1 = v.bg_img[3 * pid + 1];
+            bg2 = v.bg_img[3 * pid + 2];
+            v.out_depth[pid] = depth;
+        }
+        v.image[3 * pid] = fminf(fmaxf(bg0 * T + cr, 0.f), 1.f);
+        v.image[3 * pid + 1] = fminf(fmaxf(bg1 * T + cg, 0.f), 1.f);
+        v.image[3 * pid + 2] = fminf(fmaxf(bg2 * T + cb, 0.f), 1.f);
     }
 }
 
@@ -355,13 +377,20 @@ constexpr int NQ = 9;  // sum vs, vs dx, vs dy, vs dx^2, vs dx dy, vs dy^2, fac 
 // With CHUNK the workgroup runs chunk ch.k of the tile's list (a depth-chunked forward
 // made the frame): each pixel starts from the chunk's end transmittance and from S over
 // the final transmittance's background plus the later chunks' colours.
-template <int PPT, int BATCH, bool CHUNK = false>
+// With AUX (rasterize_backward with f.aux, cpu:963-1025): each pixel also carries the
+// prior losses' gradients v_d = dL/dD and v_a = dL/dA and the sums of z fac and fac
+// over the records behind, which add (z T - Sd / (1 - alpha)) v_d + (T - Sa / (1 - alpha)) v_a
+// to v_alpha; the per-pixel background is bg_img, and each record's sum of fac v_d goes
+// to its depth slot.
+template <int PPT, int BATCH, bool CHUNK = false, bool AUX = false>
 __device__ inline void backward_tile(const TileView &v, const TileData &d, int gt, int lt,
                                      const ChunkView *ch = nullptr) {
 #pragma clang fp contract(fast)
     constexpr int NT = 256 / PPT, NW = NT / 64;
     __shared__ float4 la[BATCH], lb[BATCH];
     __shared__ float lc[BATCH];
+    __shared__ float lz[AUX ? BATCH : 1];             // the record's view depth
+    __shared__ float partd[AUX ? NW : 1][AUX ? BATCH : 1];  // per-wave sums of fac v_d
     __shared__ uint32_t lslot[BATCH];
     __shared__ int lk[BATCH];       // the staged record's position in the tile's list
     __shared__ uint8_t lmask[BATCH];  // bit c: raw colour c + 0.5 >= 0 (its gradient passes)
@@ -378,6 +407,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
     const int px = tx * TILE + (lane & 15);
     const float x0 = (float)(tx * TILE), y0 = (float)(ty * TILE);
     float fy[PPT], T[PPT], S[PPT], vr[PPT], vg[PPT], vb[PPT];
+    float vd[AUX ? PPT : 1], va[AUX ? PPT : 1], Sd[AUX ? PPT : 1], Sa[AUX ? PPT : 1];
     int last[PPT];
     int m = s - 1;
 #pragma unroll
@@ -385,6 +415,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
         const int py = ty * TILE + 4 * PPT * w + (lane >> 4) + 4 * k;
         fy[k] = (float)py;
         T[k] = 0.f; S[k] = 0.f; vr[k] = 0.f; vg[k] = 0.f; vb[k] = 0.f;
+        if (AUX) { vd[k] = 0.f; va[k] = 0.f; Sd[k] = 0.f; Sa[k] = 0.f; }
         last[k] = s - 1;
         if (px < (int)v.width && py < (int)v.height) {
             const size_t pid = (size_t)py * v.width + px;
@@ -393,7 +424,13 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             vr[k] = v.v_image[3 * pid];
             vg[k] = v.v_image[3 * pid + 1];
             vb[k] = v.v_image[3 * pid + 2];
-            S[k] = T[k] * (v.bg[0] * vr[k] + v.bg[1] * vg[k] + v.bg[2] * vb[k]);
+            if (AUX) {
+                vd[k] = v.v_aux[2 * pid];
+                va[k] = v.v_aux[2 * pid + 1];
+                S[k] = T[k] * (v.bg_img[3 * pid] * vr[k] + v.bg_img[3 * pid + 1] * vg[k] + v.bg_img[3 * pid + 2] * vb[k]);
+            } else {
+                S[k] = T[k] * (v.bg[0] * vr[k] + v.bg[1] * vg[k] + v.bg[2] * vb[k]);
+            }
             if (CHUNK) {  // the chunk's end transmittance and S over everything behind it
                 const size_t o = (size_t)ch->k * ch->npix + pid;
                 T[k] = fabsf(ch->state[o].w);
@@ -424,11 +461,13 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             float4 A0, B0;
             float C0 = 0.f;
             uint32_t p = 0;
+            float Z0 = 0.f;
             if (i < n) {
                 const int g = (int)((uint32_t)d.gid[start + i] & d.gid_mask);
                 A0 = d.splat_a[g];
                 B0 = d.splat_b[g];
                 C0 = d.splat_blue[g];
+                if (AUX) Z0 = d.depths[g];
                 keep = !(d.cull && splat_misses_tile(A0, B0, x0, y0));
                 if (keep) {
                     const uint2 r = d.rect[g];
@@ -447,10 +486,13 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
                 lslot[pos] = p;
                 lk[pos] = start + i;
                 ltouch[pos] = 0;
+                if (AUX) lz[pos] = Z0;
             }
             kept += total;
         }
         for (int i = threadIdx.x; i < NW * NQ * BATCH; i += NT) (&part[0][0][0])[i] = 0.f;
+        if (AUX)
+            for (int i = threadIdx.x; i < NW * BATCH; i += NT) (&partd[0][0])[i] = 0.f;
         __syncthreads();
         for (int j = kept - 1; j >= 0; j--) {
             const int kk = lk[j];
@@ -460,7 +502,8 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             // the dx-weighted sums follow from q0 and q2 after the rows
             const float dx = A.x - fx;
             const float adx2 = (A.z * dx) * dx, bdx = A.w * dx;
-            float q0 = 0.f, q2 = 0.f, q5 = 0.f, q6 = 0.f, q7 = 0.f, q8 = 0.f;
+            float q0 = 0.f, q2 = 0.f, q5 = 0.f, q6 = 0.f, q7 = 0.f, q8 = 0.f, q9 = 0.f;
+            const float z = AUX ? lz[j] : 0.f;
             bool contrib = false;
 #pragma unroll
             for (int k = 0; k < PPT; k++) {
@@ -474,7 +517,13 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
                 const float Tn = T[k] * ra;
                 const float fac = valid ? alpha * Tn : 0.f;
                 const float cv = cr * vr[k] + cg * vg[k] + cbl * vb[k];
-                const float v_alpha = Tn * cv - S[k] * ra;
+                float v_alpha = Tn * cv - S[k] * ra;
+                if (AUX) {
+                    v_alpha += (z * Tn - Sd[k] * ra) * vd[k] + (Tn - Sa[k] * ra) * va[k];
+                    Sd[k] += z * fac;
+                    Sa[k] += fac;
+                    q9 += fac * vd[k];
+                }
                 T[k] = valid ? Tn : T[k];
                 S[k] += fac * cv;
                 q6 += fac * vr[k];
@@ -491,6 +540,10 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
                 const float t = wave_sums9(sl, q0, q1, q2, q3, q4, q5, q6, q7, q8);
                 if (lane < NQ) part[w][j][sl.q] = t;
                 if (lane == 0) ltouch[j] = 1;
+                if (AUX) {
+                    for (int dd = 32; dd >= 1; dd >>= 1) q9 += __shfl_xor(q9, dd);
+                    if (lane == 0) partd[w][j] = q9;
+                }
             }
         }
         __syncthreads();
@@ -514,6 +567,12 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             d.slot_b[p] = make_float4(0.5f * q[5], mr ? q[6] : 0.f, mg ? q[7] : 0.f, mb ? q[8] : 0.f);
             d.slot_c[p] = -(1.f - B.y) * q[0];
             d.slot_f[p] = 1;
+            if (AUX) {
+                float acc = partd[0][j];
+#pragma unroll
+                for (int r = 1; r < NW; r++) acc += partd[r][j];
+                d.slot_d[p] = acc;
+            }
         }
     }
     if (d.slots_written) {  // one atomic per wave

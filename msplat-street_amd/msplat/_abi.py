@@ -50,6 +50,22 @@ class Adam(C.Structure):
                 ("bc2_sqrt", C.c_float * 6), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float)]
 
 
+class Prior(C.Structure):
+    """msplat_hip_prior: PriorStep (core/metal/bindings.h:43-77) with device pointers."""
+    _fields_ = [("aux", C.c_int32), ("sky", C.c_int32), ("sky_tex", C.c_void_p), ("sky_grad", C.c_void_p),
+                ("sky_exp_avg", C.c_void_p), ("sky_exp_avg_sq", C.c_void_p), ("sky_w", C.c_int32),
+                ("sky_h", C.c_int32), ("sky_frame", C.c_float * 9), ("sky_step_size", C.c_float),
+                ("sky_bc2_sqrt", C.c_float), ("prior_depth", C.c_void_p), ("prior_aux", C.c_void_p),
+                ("prior_w", C.c_int32), ("prior_h", C.c_int32), ("has_depth", C.c_int32),
+                ("has_sky_mask", C.c_int32), ("depth_weight", C.c_float), ("sky_weight", C.c_float),
+                ("fill_weight", C.c_float), ("huber_delta", C.c_float), ("min_alpha", C.c_float),
+                ("mask_photometric", C.c_int32), ("loss_terms", C.c_void_p), ("exposure", C.c_int32),
+                ("cam_index", C.c_int32), ("num_cams", C.c_int32), ("expo_params", C.c_void_p),
+                ("expo_exp_avg", C.c_void_p), ("expo_exp_avg_sq", C.c_void_p), ("expo_grad", C.c_void_p),
+                ("expo_step_size", C.c_float), ("expo_bc2_sqrt", C.c_float), ("expo_reg", C.c_float),
+                ("log_max_scale_ratio", C.c_float)]
+
+
 class GradStats(C.Structure):
     _fields_ = [("vis_counts", C.c_void_p), ("xys_grad_norm", C.c_void_p), ("max_2d_size", C.c_void_p)]
 
@@ -113,6 +129,12 @@ def lib():
             "msplat_hip_train_step": ([vp, C.POINTER(Gaussians), C.POINTER(Camera), fp, vp, f32, f32,
                                        C.POINTER(Adam), C.POINTER(GradStats), f32, vp, fp], C.c_int),
             "msplat_hip_last_loss": ([vp, fp], C.c_int),
+            "msplat_hip_train_step_prior": ([vp, C.POINTER(Gaussians), C.POINTER(Camera), fp, vp, f32, f32,
+                                             C.POINTER(Adam), C.POINTER(GradStats), f32, vp, fp, C.POINTER(Prior)],
+                                            C.c_int),
+            "msplat_hip_render_prior": ([vp, C.POINTER(Gaussians), C.POINTER(Camera), fp, C.POINTER(Prior),
+                                         C.POINTER(vp)], C.c_int),
+            "msplat_hip_render_aux_outputs": ([vp, C.POINTER(vp), C.POINTER(vp)], C.c_int),
             "msplat_hip_render_backward": ([vp, C.POINTER(Gaussians), C.POINTER(Camera), fp, vp,
                                             C.POINTER(Grads)], C.c_int),
             "msplat_hip_frame_view": ([vp, C.POINTER(Frame)], C.c_int),
