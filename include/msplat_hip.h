@@ -167,6 +167,10 @@ int msplat_hip_render_prior(msplat_hip_ctx *ctx, const msplat_hip_gaussians *g, 
 /* msplat_render_aux_outputs (bindings.h:93-96): after an aux frame, the expected-depth sum
  * sum_i w_i z_i and the final transmittance (device (H, W), the context's caches). */
 int msplat_hip_render_aux_outputs(msplat_hip_ctx *ctx, float **depth, float **final_T);
+/* msplat_finalize_depth (bindings.h:177-180): alpha = 1 - final_T and the expected depth
+ * depth_num / alpha * inv_scale (0 where alpha <= 1e-4), n pixels, device arrays. */
+int msplat_hip_finalize_depth(msplat_hip_ctx *ctx, const float *depth_num, const float *final_T, uint32_t n,
+                              float inv_scale, float *depth_out, float *alpha_out);
 
 /* ── differentiable rendering (new: torch.autograd.Function backend) ─────────
  * render_backward takes dL/d(image) (device (H, W, 3)) of the last msplat_hip_render

@@ -784,11 +784,13 @@ int prior_step(msplat_hip_ctx *ctx, const StepRecord &r, const msplat_hip_prior 
                                       p->expo_step_size, p->expo_bc2_sqrt, p->expo_reg, r.adam.beta1, r.adam.beta2,
                                       r.adam.eps, s);
     }
-    if (sky)
+    if (sky) {
+        CHECKED(ctx->grow(ctx->sky_acc, 24 * (size_t)p->sky_w * p->sky_h));
         launch_sky_backward_adam(sky_args(p, cam, true), W, H, (const float *)ctx->final_T.p,
-                                 (const float *)ctx->v_image.p, p->sky_grad, p->sky_tex, p->sky_exp_avg,
-                                 p->sky_exp_avg_sq, p->sky_step_size, p->sky_bc2_sqrt, r.adam.beta1, r.adam.beta2,
-                                 r.adam.eps, s);
+                                 (const float *)ctx->v_image.p, (unsigned long long *)ctx->sky_acc.p, p->sky_grad,
+                                 p->sky_tex, p->sky_exp_avg, p->sky_exp_avg_sq, p->sky_step_size, p->sky_bc2_sqrt,
+                                 r.adam.beta1, r.adam.beta2, r.adam.eps, s);
+    }
     HIPCHECK(hipGetLastError());
     CHECKED(do_raster_backward(ctx));
     CHECKED(eager_step_tail(ctx, r));
@@ -1050,6 +1052,16 @@ int msplat_hip_render_aux_outputs(msplat_hip_ctx *ctx, float **depth, float **fi
         return set_error(MSPLAT_ERR_ARG, "no prior-guided (aux) frame on this context");
     if (depth) *depth = (float *)ctx->out_depth.p;
     if (final_T) *final_T = (float *)ctx->final_T.p;
+    return MSPLAT_OK;
+}
+
+int msplat_hip_finalize_depth(msplat_hip_ctx *ctx, const float *depth_num, const float *final_T, uint32_t n,
+                              float inv_scale, float *depth_out, float *alpha_out) {
+    if (!ctx) return set_error(MSPLAT_ERR_ARG, "ctx is NULL");
+    CHECKED(settle_steps(ctx));
+    if (n && (!depth_num || !final_T || !depth_out || !alpha_out)) return set_error(MSPLAT_ERR_ARG, "NULL array");
+    launch_finalize_depth(depth_num, final_T, n, inv_scale, depth_out, alpha_out, ctx->stream);
+    HIPCHECK(hipGetLastError());
     return MSPLAT_OK;
 }
 

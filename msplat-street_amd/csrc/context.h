@@ -119,7 +119,7 @@ struct msplat_hip_ctx {
     // prior-guided frames: per pixel background, expected-depth sum, dL/dD and dL/dA, the
     // exposure-adjusted render and its gradient; per row prior terms; exposure partials;
     // per intersection depth slots and per gaussian big-record sums of them
-    Buf bg_img, out_depth, v_aux, adj, v_adj, prior_rows, expo_part, slot_d, big_d;
+    Buf bg_img, out_depth, v_aux, adj, v_adj, prior_rows, expo_part, slot_d, big_d, sky_acc;
     bool frame_aux = false;  // the frame in the workspace is a prior-guided (aux) one
     int frame_chunks = 1;                     // depth chunks per tile of the frame's forward
     Buf aux;           // densify flags/scans/scratch, kNN and metrics workspaces
@@ -257,7 +257,7 @@ struct msplat_hip_ctx {
                       &hist, &digit_total, &keys64, &block_vis, &dk0, &dk1, &dv0, &dv1, &ord_sums, &scan_part, &loff, &tile_start, &image, &final_T, &final_idx, &v_image,
                       &row_loss, &loss_scratch, &scalars, &grad_scratch, &aux, &chunk_prod, &chunk_state,
                       &chunk_last, &bg_img, &out_depth, &v_aux, &adj, &v_adj, &prior_rows, &expo_part, &slot_d,
-                      &big_d};
+                      &big_d, &sky_acc};
         for (Buf *b : all) release(*b);
         if (batch) msplat::destroy_batch_state(batch);
         if (step_exec) (void)hipGraphExecDestroy(step_exec);

@@ -20,8 +20,11 @@
 // reused block address, and mirrors unused for many calls are freed once they exceed a
 // byte budget.
 //
-// Out of scope here (they throw): the prior-guided extras (PriorStep with anything
-// enabled), aux depth outputs and GPU capture.
+// PriorStep's tensors follow the same rule: inputs are uploaded unless device-newer, and
+// the ones a train step updates (sky texture, gradient and moments, exposure rows,
+// moments and gradient, loss terms) become device-newer. The aux outputs of the last
+// prior-guided frame are kept in two backend-owned tensors. GPU capture (a Metal tool)
+// returns false.
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -47,7 +50,8 @@ struct Backend {
     std::unordered_map<const void *, Mirror> mirrors;
     size_t mirror_bytes = 0;
     uint64_t clock = 0;
-    MTensor out_img, radii;
+    MTensor out_img, radii, aux_depth, aux_T;
+    bool aux_frame = false;
     bool timing = false;
     std::mutex mu;
     static constexpr size_t kBudget = size_t(16) << 30;  // idle mirrors are freed above this
@@ -133,14 +137,85 @@ struct Backend {
         ctx = nullptr;
         out_img = MTensor();
         radii = MTensor();
+        aux_depth = MTensor();
+        aux_T = MTensor();
+        aux_frame = false;
     }
 };
 
 Backend g;
 
-void no_priors(const PriorStep *p) {
-    if (p && (p->aux || p->sky || p->exposure || p->log_max_scale_ratio > 0.f))
-        throw std::invalid_argument("msplat HIP backend: the prior-guided extras are not supported");
+// A PriorStep on device mirrors; `written` lists the tensors a train step updates
+struct DevicePrior {
+    msplat_hip_prior p;
+    std::vector<MTensor *> written;
+};
+
+DevicePrior device_prior(const PriorStep *ps, bool train) {
+    DevicePrior d;
+    std::memset(&d.p, 0, sizeof d.p);
+    msplat_hip_prior &p = d.p;
+    auto in = [](MTensor *t) -> float * { return t && t->numel() ? g.up(*t) : nullptr; };
+    p.aux = ps->aux;
+    p.sky = ps->sky;
+    p.sky_tex = in(ps->sky_tex);
+    p.sky_w = ps->sky_w;
+    p.sky_h = ps->sky_h;
+    std::memcpy(p.sky_frame, ps->sky_frame, sizeof p.sky_frame);
+    if (train) {
+        p.sky_grad = in(ps->sky_grad);
+        p.sky_exp_avg = in(ps->sky_exp_avg);
+        p.sky_exp_avg_sq = in(ps->sky_exp_avg_sq);
+        p.sky_step_size = ps->sky_step_size;
+        p.sky_bc2_sqrt = ps->sky_bc2_sqrt;
+        p.prior_depth = in(ps->prior_depth);
+        p.prior_aux = (const uint8_t *)in(ps->prior_aux);
+        p.prior_w = ps->prior_w;
+        p.prior_h = ps->prior_h;
+        p.has_depth = ps->has_depth;
+        p.has_sky_mask = ps->has_sky_mask;
+        p.depth_weight = ps->depth_weight;
+        p.sky_weight = ps->sky_weight;
+        p.fill_weight = ps->fill_weight;
+        p.huber_delta = ps->huber_delta;
+        p.min_alpha = ps->min_alpha;
+        p.mask_photometric = ps->mask_photometric;
+        p.loss_terms = in(ps->loss_terms);
+        p.exposure = ps->exposure;
+        p.cam_index = ps->cam_index;
+        if (ps->expo_params) p.num_cams = (int32_t)ps->expo_params->size(0);
+        p.expo_params = in(ps->expo_params);
+        p.expo_exp_avg = in(ps->expo_exp_avg);
+        p.expo_exp_avg_sq = in(ps->expo_exp_avg_sq);
+        p.expo_grad = in(ps->expo_grad);
+        p.expo_step_size = ps->expo_step_size;
+        p.expo_bc2_sqrt = ps->expo_bc2_sqrt;
+        p.expo_reg = ps->expo_reg;
+        p.log_max_scale_ratio = ps->log_max_scale_ratio;
+        MTensor *upd[] = {ps->sky_tex, ps->sky_grad, ps->sky_exp_avg, ps->sky_exp_avg_sq, ps->loss_terms,
+                          ps->expo_params, ps->expo_exp_avg, ps->expo_exp_avg_sq, ps->expo_grad};
+        for (MTensor *t : upd)
+            if (t && t->numel()) d.written.push_back(t);
+    }
+    return d;
+}
+
+// The aux outputs of the frame just rendered, copied into the backend's tensors
+void keep_aux_outputs(unsigned H, unsigned W) {
+    float *depth = nullptr, *fT = nullptr;
+    Backend::check(msplat_hip_render_aux_outputs(g.get(), &depth, &fT), "aux outputs");
+    MTensor *outs[2] = {&g.aux_depth, &g.aux_T};
+    float *src[2] = {depth, fT};
+    for (int i = 0; i < 2; i++) {
+        MTensor &t = *outs[i];
+        if (!t.defined() || t.size(0) != (int64_t)H || t.size(1) != (int64_t)W) {
+            t = MTensor(MTensor::HostDevice{}, {(int64_t)H, (int64_t)W}, DType::Float32);
+            g.forget(t.data_ptr());
+        }
+        Backend::check(msplat_hip_memcpy(g.get(), g.dev(t, t.nbytes()), src[i], t.nbytes(), 2), "aux copy");
+        g.wrote(t);
+    }
+    g.aux_frame = true;
 }
 
 msplat_hip_camera camera(MTensor &viewmat, MTensor &projmat, float fx, float fy, float cx, float cy, unsigned H,
@@ -240,12 +315,18 @@ MTensor msplat_render(int num_points, MTensor &means3d, MTensor &scales, float g
                       MTensor &features_dc, MTensor &features_rest, MTensor &opacities, MTensor &background,
                       const PriorStep *prior) {
     std::lock_guard<std::mutex> lk(g.mu);
-    no_priors(prior);
     msplat_hip_gaussians gs = gaussians(num_points, means3d, scales, quats, features_dc, features_rest, opacities);
     msplat_hip_camera c = camera(viewmat, projmat, fx, fy, cx, cy, img_height, img_width, tile_bounds, clip_thresh,
                                  glob_scale, degree, degrees_to_use, cam_pos);
     float *img = nullptr;
-    Backend::check(msplat_hip_render(g.get(), &gs, &c, background.data<float>(), &img), "render");
+    if (prior) {
+        DevicePrior dp = device_prior(prior, false);
+        Backend::check(msplat_hip_render_prior(g.get(), &gs, &c, background.data<float>(), &dp.p, &img), "render");
+    } else {
+        Backend::check(msplat_hip_render(g.get(), &gs, &c, background.data<float>(), &img), "render");
+    }
+    g.aux_frame = false;
+    if (prior && prior->aux) keep_aux_outputs(img_height, img_width);
     if (!g.out_img.defined() || g.out_img.size(0) != (int64_t)img_height || g.out_img.size(1) != (int64_t)img_width) {
         g.out_img = MTensor(MTensor::HostDevice{}, {(int64_t)img_height, (int64_t)img_width, 3}, DType::Float32);
         g.forget(g.out_img.data_ptr());
@@ -255,8 +336,11 @@ MTensor msplat_render(int num_points, MTensor &means3d, MTensor &scales, float g
     return g.out_img;  // aliases the backend's cache until the next call; read after msplat_gpu_sync
 }
 
-void msplat_render_aux_outputs(MTensor &, MTensor &) {
-    throw std::invalid_argument("msplat HIP backend: aux depth outputs belong to the prior-guided path");
+void msplat_render_aux_outputs(MTensor &depth, MTensor &final_T) {
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (!g.aux_frame) throw std::invalid_argument("msplat HIP backend: no prior-guided (aux) frame rendered");
+    depth = g.aux_depth;  // alias the backend's tensors until the next frame; read after msplat_gpu_sync
+    final_T = g.aux_T;
 }
 
 std::tuple<MTensor, float> msplat_train_step(
@@ -271,7 +355,6 @@ std::tuple<MTensor, float> msplat_train_step(
     (void)window2d;
     (void)features_rest_bases;
     std::lock_guard<std::mutex> lk(g.mu);
-    no_priors(prior);
     if (num_adam_groups < 6) throw std::invalid_argument("msplat HIP backend: six Adam groups expected");
     MTensor *params[6] = {&means3d, &scales, &quats, &features_dc, &features_rest, &opacities};
     for (int i = 0; i < 6; i++)
@@ -297,9 +380,19 @@ std::tuple<MTensor, float> msplat_train_step(
     }
     int32_t *d_radii = (int32_t *)g.dev(g.radii, g.radii.nbytes());
     float loss = 0.f;
-    Backend::check(msplat_hip_train_step(g.get(), &gs, &c, background.data<float>(), g.up(gt), ssim_weight,
-                                         loss_inv_n, &a, &st, inv_max_dim, d_radii, &loss),
-                   "train_step");
+    g.aux_frame = false;
+    if (prior) {
+        DevicePrior dp = device_prior(prior, true);
+        Backend::check(msplat_hip_train_step_prior(g.get(), &gs, &c, background.data<float>(), g.up(gt), ssim_weight,
+                                                   loss_inv_n, &a, &st, inv_max_dim, d_radii, &loss, &dp.p),
+                       "train_step");
+        for (MTensor *t : dp.written) g.wrote(*t);
+        if (prior->aux) keep_aux_outputs(img_height, img_width);
+    } else {
+        Backend::check(msplat_hip_train_step(g.get(), &gs, &c, background.data<float>(), g.up(gt), ssim_weight,
+                                             loss_inv_n, &a, &st, inv_max_dim, d_radii, &loss),
+                       "train_step");
+    }
     MTensor *written[] = {&means3d, &scales, &quats, &features_dc, &features_rest, &opacities,
                           &vis_counts, &xys_grad_norm, &max_2d_size, &g.radii};
     for (MTensor *t : written)
@@ -431,6 +524,12 @@ void msplat_pack_rgba8(const MTensor &img, uint32_t n, uint8_t *out) {
     Backend::check(rc, "pack_rgba8");
 }
 
-void msplat_finalize_depth(const MTensor &, const MTensor &, uint32_t, float, MTensor &, MTensor &) {
-    throw std::invalid_argument("msplat HIP backend: expected depth belongs to the prior-guided path");
+void msplat_finalize_depth(const MTensor &depth_num, const MTensor &final_T, uint32_t n, float inv_scale,
+                           MTensor &depth_out, MTensor &alpha_out) {
+    std::lock_guard<std::mutex> lk(g.mu);
+    float *d = g.up(depth_num), *t = g.up(final_T);
+    float *od = (float *)g.dev(depth_out, depth_out.nbytes()), *oa = (float *)g.dev(alpha_out, alpha_out.nbytes());
+    Backend::check(msplat_hip_finalize_depth(g.get(), d, t, n, inv_scale, od, oa), "finalize_depth");
+    g.wrote(depth_out);
+    g.wrote(alpha_out);
 }

@@ -259,11 +259,14 @@ void launch_exposure_backward_adam(const float *img, const float *v_adj, size_t 
                                    float *params, float *m1, float *m2, float *grad, int cam, float step, float bc2,
                                    float reg, float b1, float b2, float eps, hipStream_t s);
 size_t exposure_partials(size_t n);
-// the sky texture's gradient (zeroed first) from T v_img at each pixel's taps, then Adam
+// the sky texture's gradient from T v_img at each pixel's taps (fixed-point sums in acc,
+// w x h x 3 words, order-independent), written to grad, then Adam
 void launch_sky_backward_adam(const SkyArgs &k, uint32_t W, uint32_t H, const float *final_T, const float *v_img,
-                              float *grad, float *tex, float *m1, float *m2, float step, float bc2, float b1, float b2,
-                              float eps, hipStream_t s);
+                              unsigned long long *acc, float *grad, float *tex, float *m1, float *m2, float step,
+                              float bc2, float b1, float b2, float eps, hipStream_t s);
 void launch_scale_cap(float *scales, int n, float log_max, hipStream_t s);
+void launch_finalize_depth(const float *depth_num, const float *final_T, uint32_t n, float inv_scale,
+                           float *depth_out, float *alpha_out, hipStream_t s);
 
 // ── batched views ───────────────────────────────────────────────────────────
 // Several views of one scene go through one projection, one depth sort of the visible

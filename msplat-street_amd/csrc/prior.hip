@@ -3,8 +3,9 @@
 // fill prior losses with their gradients, the Adam steps of the sky texture and of the
 // exposure rows, and the needle cap (core/cpu/msplat_cpu.cpp:327-402, 787-950,
 // 1219-1309). One thread per pixel, texel or gaussian. Where the reference sums in
-// double, so does this file, in fixed order; the sky gradient's taps are added with
-// float atomics (many pixels hit one texel), so its rounding order varies.
+// double, so does this file, in fixed order. The sky gradient gathers many pixels into
+// each texel: the taps are added as 64-bit fixed-point integers (2^-48 units), whose
+// atomic sums do not depend on their order, so every run gives the same texture.
 #include "common.h"
 #include "kernels.h"
 
@@ -233,9 +234,11 @@ __global__ void exposure_adam_kernel(const double *part, int nblocks, float *par
     m2[i] = v;
 }
 
+constexpr double kFix = 281474976710656.0;  // 2^48: fixed-point units of the sky gradient sums
+
 // sky_backward (cpu:927-943): T v_img added to the four taps of each pixel's ray
 __global__ __launch_bounds__(256) void sky_grad_kernel(SkyArgs k, uint32_t W, uint32_t H, const float *final_T,
-                                                       const float *v_img, float *grad) {
+                                                       const float *v_img, unsigned long long *acc) {
     const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= W || y >= H) return;
     const size_t p = (size_t)y * W + x;
@@ -247,15 +250,21 @@ __global__ __launch_bounds__(256) void sky_grad_kernel(SkyArgs k, uint32_t W, ui
     sky_taps(k, (float)x, (float)y, idx, wt);
     for (int t = 0; t < 4; t++) {
         if (wt[t] == 0.f) continue;
-        for (int c = 0; c < 3; c++) atomicAdd(grad + 3 * (size_t)idx[t] + c, wt[t] * g[c]);
+        for (int c = 0; c < 3; c++) {
+            const long long q = __double2ll_rn((double)(wt[t] * g[c]) * kFix);
+            atomicAdd(acc + 3 * (size_t)idx[t] + c, (unsigned long long)q);
+        }
     }
 }
 
-__global__ __launch_bounds__(256) void sky_adam_kernel(float *tex, float *m1, float *m2, const float *grad, size_t n,
-                                                       float step, float bc2, float b1, float b2, float eps) {
+// The texture's gradient (written out, as the reference leaves it in sky_grad) and its Adam step
+__global__ __launch_bounds__(256) void sky_adam_kernel(float *tex, float *m1, float *m2, float *grad,
+                                                       const unsigned long long *acc, size_t n, float step, float bc2,
+                                                       float b1, float b2, float eps) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const float g = grad[i];
+    const float g = (float)((double)(long long)acc[i] * (1.0 / kFix));
+    grad[i] = g;
     const float m = b1 * m1[i] + (1.0f - b1) * g;
     const float v = b2 * m2[i] + (1.0f - b2) * g * g;
     tex[i] -= step * m / (sqrtf(v) / bc2 + eps);
@@ -263,9 +272,9 @@ __global__ __launch_bounds__(256) void sky_adam_kernel(float *tex, float *m1, fl
     m2[i] = v;
 }
 
-__global__ __launch_bounds__(256) void zero_f32_kernel(float *p, size_t n) {
+__global__ __launch_bounds__(256) void zero_u64_kernel(unsigned long long *p, size_t n) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = 0.f;
+    if (i < n) p[i] = 0ull;
 }
 
 // scale_ratio_cap (cpu:1295-1309): the largest log-scale is lowered to the middle one plus
@@ -282,7 +291,23 @@ __global__ __launch_bounds__(256) void scale_cap_kernel(float *s, int n, float l
     else s[3 * i + 2] = cap;
 }
 
+// msplat_finalize_depth (cpu:1736-1745): alpha = 1 - T and the expected depth
+// depth_num / alpha * inv_scale (0 where alpha <= 1e-4)
+__global__ __launch_bounds__(256) void finalize_depth_kernel(const float *d, const float *t, uint32_t n,
+                                                             float inv_scale, float *od, float *oa) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float a = 1.f - t[i];
+    oa[i] = a;
+    od[i] = a > 1e-4f ? d[i] / a * inv_scale : 0.f;
+}
+
 }  // namespace
+
+void launch_finalize_depth(const float *depth_num, const float *final_T, uint32_t n, float inv_scale,
+                           float *depth_out, float *alpha_out, hipStream_t s) {
+    if (n) finalize_depth_kernel<<<(n + 255) / 256, 256, 0, s>>>(depth_num, final_T, n, inv_scale, depth_out, alpha_out);
+}
 
 void launch_sky_background(const SkyArgs &k, uint32_t W, uint32_t H, const float bg[3], float *bg_img, hipStream_t s) {
     if (W && H)
@@ -310,12 +335,12 @@ void launch_exposure_backward_adam(const float *img, const float *v_adj, size_t 
 }
 
 void launch_sky_backward_adam(const SkyArgs &k, uint32_t W, uint32_t H, const float *final_T, const float *v_img,
-                              float *grad, float *tex, float *m1, float *m2, float step, float bc2, float b1, float b2,
-                              float eps, hipStream_t s) {
+                              unsigned long long *acc, float *grad, float *tex, float *m1, float *m2, float step,
+                              float bc2, float b1, float b2, float eps, hipStream_t s) {
     const size_t n = (size_t)k.w * k.h * 3;
-    zero_f32_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(grad, n);
-    if (W && H) sky_grad_kernel<<<dim3((W + 15) / 16, (H + 15) / 16), 256, 0, s>>>(k, W, H, final_T, v_img, grad);
-    sky_adam_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(tex, m1, m2, grad, n, step, bc2, b1, b2, eps);
+    zero_u64_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(acc, n);
+    if (W && H) sky_grad_kernel<<<dim3((W + 15) / 16, (H + 15) / 16), 256, 0, s>>>(k, W, H, final_T, v_img, acc);
+    sky_adam_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(tex, m1, m2, grad, acc, n, step, bc2, b1, b2, eps);
 }
 
 void launch_scale_cap(float *scales, int n, float log_max, hipStream_t s) {

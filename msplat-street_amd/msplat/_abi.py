@@ -135,6 +135,7 @@ def lib():
             "msplat_hip_render_prior": ([vp, C.POINTER(Gaussians), C.POINTER(Camera), fp, C.POINTER(Prior),
                                          C.POINTER(vp)], C.c_int),
             "msplat_hip_render_aux_outputs": ([vp, C.POINTER(vp), C.POINTER(vp)], C.c_int),
+            "msplat_hip_finalize_depth": ([vp, vp, vp, C.c_uint32, f32, vp, vp], C.c_int),
             "msplat_hip_render_backward": ([vp, C.POINTER(Gaussians), C.POINTER(Camera), fp, vp,
                                             C.POINTER(Grads)], C.c_int),
             "msplat_hip_frame_view": ([vp, C.POINTER(Frame)], C.c_int),

@@ -101,3 +101,13 @@ def aux_outputs(ctx: Context, H: int, W: int) -> tuple:
     d, t = C.c_void_p(), C.c_void_p()
     check(lib().msplat_hip_render_aux_outputs(ctx.ptr, C.byref(d), C.byref(t)))
     return ctx.download(d.value, (H, W), np.float32), ctx.download(t.value, (H, W), np.float32)
+
+
+def finalize_depth(ctx: Context, depth_num, final_T, inv_scale: float = 1.0) -> tuple:
+    """msplat_hip_finalize_depth on host arrays: (expected depth, alpha)."""
+    d = DeviceArray.from_numpy(ctx, np.ascontiguousarray(depth_num, np.float32))
+    t = DeviceArray.from_numpy(ctx, np.ascontiguousarray(final_T, np.float32))
+    od, oa = DeviceArray(ctx, d.shape), DeviceArray(ctx, d.shape)
+    check(lib().msplat_hip_finalize_depth(ctx.ptr, C.c_void_p(d.ptr), C.c_void_p(t.ptr), int(np.size(depth_num)),
+                                          float(inv_scale), C.c_void_p(od.ptr), C.c_void_p(oa.ptr)))
+    return od.numpy(), oa.numpy()

@@ -1577,6 +1577,15 @@ extern "C" void orc_f32_render_prior(int N, float glob, const float *vm, const f
     if (radii_out) std::memcpy(radii_out, radii.data(), sizeof(int32_t) * N);
 }
 
+// msplat_finalize_depth (msplat_cpu.cpp:1736-1745)
+extern "C" void orc_finalize_depth(const float *d, const float *t, uint32_t n, float inv_scale, float *od, float *oa) {
+    for (uint32_t i = 0; i < n; i++) {
+        const float a = 1.f - t[i];
+        oa[i] = a;
+        od[i] = a > 1e-4f ? d[i] / a * inv_scale : 0.f;
+    }
+}
+
 extern "C" int orc_densify(int N, int cap, float grad_thr, float size_thr, float screen_thr, int check_screen,
                            float cull_alpha, float cull_scale, float cull_screen, int check_huge, const float *gn,
                            const float *vc, const float *ms, float half_max_dim, float **bufs, int fr_stride,

@@ -157,6 +157,7 @@ class Oracle:
         L.orc_pack_rgba8.argtypes = [f32p, C.c_uint32, u8p]
         L.orc_resize_area.argtypes = [vp, C.c_int, C.c_int, C.c_int, f32p, C.c_int, C.c_int]
         L.orc_undistort.argtypes = [f32p, C.c_int, C.c_int, f32p, f32p, C.c_int, C.c_int, f32p, C.c_int, C.c_int]
+        L.orc_finalize_depth.argtypes = [f32p, f32p, C.c_uint32, C.c_float, f32p, f32p]
         L.orc_pcg_hash.argtypes = [C.c_uint32]
         L.orc_pcg_hash.restype = C.c_uint32
         L.orc_rand_uniform.argtypes = [C.c_uint32, C.c_uint32]
@@ -412,6 +413,12 @@ class Oracle:
                                _c(dist, np.float32), roi_x, roi_y, out, dw, dh)
         return out
 
+    def finalize_depth(self, depth_num, final_T, inv_scale=1.0):
+        d, t = _c(depth_num, np.float32).ravel(), _c(final_T, np.float32).ravel()
+        od, oa = np.zeros_like(d), np.zeros_like(d)
+        self.lib.orc_finalize_depth(d, t, d.size, inv_scale, od, oa)
+        return od.reshape(np.shape(depth_num)), oa.reshape(np.shape(depth_num))
+
     def pcg_hash(self, v):
         return self.lib.orc_pcg_hash(v)
 
@@ -450,6 +457,7 @@ class Reference:
         L.ref_densify.argtypes = ([C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_float, C.c_float,
                                    C.c_float, C.c_int, f32p, f32p, f32p, C.c_float, vp, C.c_int, C.c_uint32])
         L.ref_train_step_prior.argtypes = L.ref_train_step.argtypes + [C.POINTER(PriorStruct), vp, vp]
+        L.ref_finalize_depth.argtypes = [f32p, f32p, C.c_uint32, C.c_float, f32p, f32p]
         L.ref_render_prior.argtypes = ([C.c_int, f32p, f32p, f32p, f32p, f32p, C.c_int, f32p, f32p, f32p, C.c_float,
                                         C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_float, C.c_int,
                                         C.c_int, f32p, f32p, C.POINTER(PriorStruct), f32p, vp, vp])
@@ -520,6 +528,13 @@ class Reference:
         if rc:
             raise RuntimeError("reference train_step with a prior failed")
         return loss.value, radii, depth, fT
+
+    def finalize_depth(self, depth_num, final_T, inv_scale=1.0):
+        d, t = _c(depth_num, np.float32).ravel(), _c(final_T, np.float32).ravel()
+        od, oa = np.zeros_like(d), np.zeros_like(d)
+        if self.lib.ref_finalize_depth(d, t, d.size, inv_scale, od, oa):
+            raise RuntimeError("reference finalize_depth failed")
+        return od.reshape(np.shape(depth_num)), oa.reshape(np.shape(depth_num))
 
     def render_prior(self, scene, cam, bg, prior: dict):
         N = scene["means"].shape[0]

@@ -270,6 +270,19 @@ extern "C" int ref_render_prior(int N, const float *means, const float *scales, 
     }
 }
 
+extern "C" int ref_finalize_depth(const float *d, const float *t, uint32_t n, float inv_scale, float *od, float *oa) {
+    try {
+        MTensor dn = upload(d, {(int64_t)n}, DType::Float32), ft = upload(t, {(int64_t)n}, DType::Float32);
+        MTensor o1 = gpu_zeros({(int64_t)n}, DType::Float32), o2 = gpu_zeros({(int64_t)n}, DType::Float32);
+        msplat_finalize_depth(dn, ft, n, inv_scale, o1, o2);
+        download(od, o1);
+        download(oa, o2);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
 // msplat_densify over capacity-sized buffers: bufs[0..5] params, bufs[6..11] first
 // moments, bufs[12..17] second moments, each with `cap` rows. Returns the new count.
 extern "C" int ref_densify(int N, int cap, float grad_thr, float size_thr, float screen_thr, int check_screen,

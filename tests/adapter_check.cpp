@@ -39,7 +39,7 @@ struct Dev {
     size_t bytes = 0;
     Dev(const void *src, size_t b, msplat_hip_ctx *ctx) : bytes(b) {
         rc(msplat_hip_malloc(b ? b : 4, 1, &p), "malloc");
-        if (b) rc(msplat_hip_memcpy(ctx, p, src, b, 0), "upload");
+        if (b && src) rc(msplat_hip_memcpy(ctx, p, src, b, 0), "upload");
     }
     ~Dev() { msplat_hip_free(p); }
     float *f() const { return (float *)p; }
@@ -276,6 +276,128 @@ int run() {
     expect(same(lvl.data<float>(), dlvl.get(ctx, 25 * 33 * 3).data(), 25 * 33 * 3) &&
                same(und.data<float>(), dund.get(ctx, 20 * 30 * 3).data(), 20 * 30 * 3),
            "resize_area + undistort");
+
+    // 6. prior-guided steps (aux rasterizer, learned sky, depth and sky priors, exposure,
+    // needle cap) without a sync in between, a prior render, its aux outputs, finalize_depth
+    {
+        for (int k = 0; k < 6; k++) {  // views of the densified count
+            view[k] = bufs[k].view(nn);
+            e1[k] = m1[k].view(nn);
+            e2[k] = m2[k].view(nn);
+        }
+        visv = vis.view(nn);
+        gnv = gn.view(nn);
+        msv = ms.view(nn);
+        const int pw = W / 2 + 3, ph = H / 2 - 1, sw = 32, sh = 16;
+        MTensor pdepth = gpu_zeros({ph, pw}, DType::Float32), paux = gpu_zeros({ph, pw, 4}, DType::UInt8);
+        for (int y = 0; y < ph; y++)
+            for (int x = 0; x < pw; x++) {
+                float dv = 3.2f * (1.f + 0.25f * std::sin(0.3f * x) * std::cos(0.2f * y));
+                if ((x * 7 + y * 3) % 23 == 0) dv = 0.f;
+                pdepth.data<float>()[y * pw + x] = dv;
+                uint8_t *a = paux.data<uint8_t>() + 4 * (y * pw + x);
+                a[0] = (uint8_t)(60 + (x * 13 + y * 7) % 190);
+                a[1] = y < ph / 4 ? 255 : 0;
+                a[2] = (x + 2 * y) % 11 == 0 ? 0 : 255;
+            }
+        MTensor sky = gpu_zeros({sh, sw, 3}, DType::Float32), skyg = gpu_zeros({sh, sw, 3}, DType::Float32),
+                skym1 = gpu_zeros({sh, sw, 3}, DType::Float32), skym2 = gpu_zeros({sh, sw, 3}, DType::Float32);
+        for (int i = 0; i < sw * sh * 3; i++) sky.data<float>()[i] = 0.5f + 0.3f * U(rng);
+        MTensor expo = gpu_zeros({2, 12}, DType::Float32), em1 = gpu_zeros({2, 12}, DType::Float32),
+                em2 = gpu_zeros({2, 12}, DType::Float32), egrad = gpu_zeros({12}, DType::Float32),
+                terms = gpu_zeros({4}, DType::Float32);
+        for (int k = 0; k < 24; k++) expo.data<float>()[k] = ((k % 12) % 4 == 0 && k % 12 < 9 ? 1.f : 0.f) + 0.05f * U(rng);
+        PriorStep pr;
+        pr.aux = true;
+        pr.sky = true;
+        pr.sky_tex = &sky; pr.sky_grad = &skyg; pr.sky_exp_avg = &skym1; pr.sky_exp_avg_sq = &skym2;
+        pr.sky_w = sw; pr.sky_h = sh;
+        const float frame[9] = {0, -1, 0, 0, 0, 1, 1, 0, 0};
+        std::memcpy(pr.sky_frame, frame, sizeof frame);
+        pr.sky_step_size = 0.01f; pr.sky_bc2_sqrt = 0.0316f;
+        pr.prior_depth = &pdepth; pr.prior_aux = &paux; pr.prior_w = pw; pr.prior_h = ph;
+        pr.has_depth = pr.has_sky_mask = true;
+        pr.depth_weight = 0.5f; pr.sky_weight = 0.3f; pr.fill_weight = 0.1f;
+        pr.loss_terms = &terms;
+        pr.exposure = true; pr.cam_index = 1;
+        pr.expo_params = &expo; pr.expo_exp_avg = &em1; pr.expo_exp_avg_sq = &em2; pr.expo_grad = &egrad;
+        pr.expo_step_size = 0.002f; pr.expo_bc2_sqrt = 0.0316f; pr.expo_reg = 0.01f;
+        pr.log_max_scale_ratio = std::log(3.f);
+        // the same prior on the C ABI
+        Dev dpd(pdepth.data_ptr(), pdepth.nbytes(), ctx), dpa(paux.data_ptr(), paux.nbytes(), ctx);
+        Dev dsky(sky.data_ptr(), sky.nbytes(), ctx), dskyg(skyg.data_ptr(), skyg.nbytes(), ctx),
+            dskym1(skym1.data_ptr(), skym1.nbytes(), ctx), dskym2(skym2.data_ptr(), skym2.nbytes(), ctx);
+        Dev dexpo(expo.data_ptr(), expo.nbytes(), ctx), dem1(em1.data_ptr(), em1.nbytes(), ctx),
+            dem2(em2.data_ptr(), em2.nbytes(), ctx), degrad(egrad.data_ptr(), egrad.nbytes(), ctx),
+            dterms(terms.data_ptr(), terms.nbytes(), ctx);
+        msplat_hip_prior hp{};
+        hp.aux = 1; hp.sky = 1;
+        hp.sky_tex = dsky.f(); hp.sky_grad = dskyg.f(); hp.sky_exp_avg = dskym1.f(); hp.sky_exp_avg_sq = dskym2.f();
+        hp.sky_w = sw; hp.sky_h = sh;
+        std::memcpy(hp.sky_frame, frame, sizeof frame);
+        hp.sky_step_size = 0.01f; hp.sky_bc2_sqrt = 0.0316f;
+        hp.prior_depth = dpd.f(); hp.prior_aux = (const uint8_t *)dpa.p; hp.prior_w = pw; hp.prior_h = ph;
+        hp.has_depth = hp.has_sky_mask = 1;
+        hp.depth_weight = 0.5f; hp.sky_weight = 0.3f; hp.fill_weight = 0.1f;
+        hp.huber_delta = pr.huber_delta; hp.min_alpha = pr.min_alpha; hp.mask_photometric = 1;
+        hp.loss_terms = dterms.f();
+        hp.exposure = 1; hp.cam_index = 1; hp.num_cams = 2;
+        hp.expo_params = dexpo.f(); hp.expo_exp_avg = dem1.f(); hp.expo_exp_avg_sq = dem2.f(); hp.expo_grad = degrad.f();
+        hp.expo_step_size = 0.002f; hp.expo_bc2_sqrt = 0.0316f; hp.expo_reg = 0.01f;
+        hp.log_max_scale_ratio = std::log(3.f);
+        float pl[2], rpl[2];
+        for (int t = 0; t < 2; t++) {
+            auto res = msplat_train_step(nn, view[0], view[1], 1.f, view[2], viewmat, projmat, fx, fy, cx, cy, H, W,
+                                         tiles, 0.01f, 3, 3, cam_pos, view[3], view[4], view[5], bg, gt, win, 0.2f,
+                                         1.f / (float)(H * W * 3), RB, 6, view, e1, e2, ss, bc2, 0.9f, 0.999f, 1e-8f,
+                                         visv, gnv, msv, 1.f / (float)W, &pr);
+            pl[t] = std::get<1>(res);
+            msplat_hip_adam a{};
+            for (int k = 0; k < 6; k++) {
+                a.exp_avg[k] = d1[k]->f();
+                a.exp_avg_sq[k] = d2[k]->f();
+                a.step_size[k] = ss[k];
+                a.bc2_sqrt[k] = bc2[k];
+            }
+            a.beta1 = 0.9f; a.beta2 = 0.999f; a.eps = 1e-8f;
+            msplat_hip_grad_stats st{dvis.f(), dgn.f(), dms.f()};
+            msplat_hip_gaussians gg = gauss(nn);
+            rc(msplat_hip_train_step_prior(ctx, &gg, &c, bgv, dgt.f(), 0.2f, 1.f / (float)(H * W * 3), &a, &st,
+                                           1.f / (float)W, nullptr, &rpl[t], &hp),
+               "train_step_prior");
+        }
+        MTensor pimg = msplat_render(nn, view[0], view[1], 1.f, view[2], viewmat, projmat, fx, fy, cx, cy, H, W, tiles,
+                                     0.01f, 3, 3, cam_pos, view[3], view[4], view[5], bg, &pr);
+        MTensor dnum, fT;
+        msplat_render_aux_outputs(dnum, fT);
+        MTensor depth_out = gpu_zeros({H, W}, DType::Float32), alpha_out = gpu_zeros({H, W}, DType::Float32);
+        msplat_finalize_depth(dnum, fT, (uint32_t)(H * W), 0.5f, depth_out, alpha_out);
+        msplat_gpu_sync();
+        float *rimg = nullptr, *rdep = nullptr, *rT = nullptr;
+        msplat_hip_gaussians g1 = gauss(nn);
+        rc(msplat_hip_render_prior(ctx, &g1, &c, bgv, &hp, &rimg), "render_prior");
+        rc(msplat_hip_render_aux_outputs(ctx, &rdep, &rT), "aux outputs");
+        Dev fd(nullptr, 4 * (size_t)H * W, ctx), fa(nullptr, 4 * (size_t)H * W, ctx);
+        rc(msplat_hip_finalize_depth(ctx, rdep, rT, (uint32_t)(H * W), 0.5f, fd.f(), fa.f()), "finalize");
+        std::vector<float> ri(H * W * 3), rfd(H * W), rfa(H * W);
+        rc(msplat_hip_memcpy(ctx, ri.data(), rimg, 4 * ri.size(), 1), "image");
+        bool pok = std::memcmp(pl, rpl, sizeof pl) == 0;
+        for (int k = 0; k < 6; k++) {
+            pok = pok && same(bufs[k].data<float>(), dp[k]->get(ctx, (size_t)nn * widths[k]).data(),
+                              (size_t)nn * widths[k]);
+            pok = pok && same(m2[k].data<float>(), d2[k]->get(ctx, (size_t)nn * widths[k]).data(),
+                              (size_t)nn * widths[k]);
+        }
+        const bool sky_ok = same(sky.data<float>(), dsky.get(ctx, sw * sh * 3).data(), sw * sh * 3) &&
+                            same(expo.data<float>(), dexpo.get(ctx, 24).data(), 24) &&
+                            same(terms.data<float>(), dterms.get(ctx, 4).data(), 4) &&
+                            same(egrad.data<float>(), degrad.get(ctx, 12).data(), 12);
+        expect(pok && sky_ok, "prior train_step x2: parameters, moments, sky, exposure, terms");
+        expect(same(pimg.data<float>(), ri.data(), ri.size()) &&
+                   same(depth_out.data<float>(), fd.get(ctx, H * W).data(), H * W) &&
+                   same(alpha_out.data<float>(), fa.get(ctx, H * W).data(), H * W),
+               "prior render, aux outputs, finalize_depth");
+    }
 
     for (int k = 0; k < 6; k++) {
         delete dp[k];

@@ -112,3 +112,14 @@ def test_aux_outputs_need_an_aux_frame(ctx):
     render(ctx, DeviceGaussians(ctx, sc), cam, BG)
     with pytest.raises(MsplatError, match="aux"):
         aux_outputs(ctx, cam["H"], cam["W"])
+
+
+def test_finalize_depth_matches_oracle(ctx, oracle):
+    from msplat.prior import finalize_depth
+    rng = np.random.default_rng(4)
+    t = np.concatenate([[1.0, 1.0 - 1e-5, 0.5, 0.0], rng.uniform(0, 1, 4996)]).astype(np.float32)
+    d = (rng.uniform(0.5, 40, 5000) * (1 - t)).astype(np.float32)
+    gd, ga = finalize_depth(ctx, d, t, 0.25)
+    od, oa = oracle.finalize_depth(d, t, 0.25)
+    np.testing.assert_array_equal(ga, oa)
+    np.testing.assert_array_equal(gd, od)

@@ -122,3 +122,21 @@ def test_prior_closed_forms(oracle):
     assert keep.all()
     np.testing.assert_allclose(pr2["loss_terms"][1], 0.3 * float(np.mean(1.0 - fT2.astype(np.float64))), rtol=1e-5)
     assert pr2["loss_terms"][0] == 0 and pr2["loss_terms"][2] == 0
+
+
+def test_finalize_depth_equals_reference_and_closed_form(oracle):
+    """msplat_finalize_depth (cpu:1736-1745): alpha = 1 - T; depth = num / alpha * s, 0 at
+    alpha <= 1e-4 (T = 1, T just below 1, T = 0.5 and T = 0 included)."""
+    rng = np.random.default_rng(4)
+    t = np.concatenate([[1.0, 1.0 - 1e-5, 1.0 - 2e-4, 0.5, 0.0], rng.uniform(0, 1, 995)]).astype(np.float32)
+    d = (rng.uniform(0.5, 40, 1000) * (1 - t)).astype(np.float32)
+    od, oa = oracle.finalize_depth(d, t, 0.25)
+    np.testing.assert_array_equal(oa, (np.float32(1) - t))
+    assert od[0] == 0 and od[1] == 0 and od[2] != 0
+    np.testing.assert_allclose(od[3:], (d[3:].astype(np.float64) / oa[3:] * 0.25), rtol=1e-6)
+    from oracle import reference_available
+    if reference_available():
+        from oracle import Reference
+        rd, ra = Reference().finalize_depth(d, t, 0.25)
+        np.testing.assert_array_equal(od, rd)
+        np.testing.assert_array_equal(oa, ra)
