@@ -18,7 +18,7 @@ import time
 
 import numpy as np
 
-from ._abi import Context, DeviceArray, check, lib
+from ._abi import Context, DeviceArray, Prior, check, lib, make_camera_struct
 
 _ctx: Context | None = None
 
@@ -120,11 +120,14 @@ class Dataset:
 
     @classmethod
     def from_arrays(cls, cam_to_world, intrinsics, images, points, colors, eval_mode: bool = False,
-                    test_every: int = 8) -> "Dataset":
+                    test_every: int = 8, priors=None) -> "Dataset":
         """cam_to_world: (n, 4, 4) OpenGL poses; intrinsics: (n, 6) rows (W, H, fx, fy, cx, cy);
         images: n decoded images, (H, W, 4) uint8 RGBA or (H, W, 3) float32, resampled to
         (W, H) on the GPU; points (m, 3) float32 and colors (m, 3) uint8: the initial cloud.
-        With eval_mode every test_every-th camera is held out (as the loaders split)."""
+        With eval_mode every test_every-th camera is held out (as the loaders split).
+        priors: per camera None or the geometric priors the reference loads from prior_dir
+        (Camera::loadPriors): a dict with depth (ph, pw) float32 (0 = invalid) or None, aux
+        (ph, pw, 4) uint8 (confidence, sky, keep, -), has_depth, has_sky, has_mask."""
         from .camera import make_camera
         from .images import load_target, resize_area
         ds = cls.__new__(cls)
@@ -144,6 +147,8 @@ class Dataset:
         train = [i for i in range(len(cams)) if i not in test]
         ds._train = [(cams[i], tgts[i]) for i in train]
         ds._test = [(cams[i], tgts[i]) for i in test]
+        pri = list(priors) if priors is not None else [None] * len(cams)
+        ds._train_priors = [pri[i] for i in train]
         ds.points = np.ascontiguousarray(points, np.float32)
         ds.colors = np.ascontiguousarray(colors, np.uint8)
         return ds
@@ -165,6 +170,20 @@ class Dataset:
         return (self._test if use_test else self._train)[index][1].numpy()
 
 
+def _write_png(path: str, rgb: np.ndarray):
+    """An 8-bit RGB PNG (no filtering, zlib-deflated scanlines)."""
+    import struct
+    import zlib
+    h, w = rgb.shape[:2]
+    raw = b"".join(b"\x00" + rgb[y].tobytes() for y in range(h))
+
+    def chunk(tag, data):
+        return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+    with open(path, "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0)) +
+                chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b""))
+
+
 # ── GaussianTrainer (bindings.cpp:598-653) ───────────────────────────────────
 
 class GaussianTrainer:
@@ -172,9 +191,7 @@ class GaussianTrainer:
 
     def __init__(self, dataset: Dataset, config: TrainingConfig):
         from .backend import DeviceGaussians
-        from .trainer import TrainConfig, Trainer
-        if config.priors_enabled():
-            raise NotImplementedError("the prior-guided options are not part of this backend")
+        from .trainer import PriorOptions, TrainConfig, Trainer
         self.dataset, self.config = dataset, config
         self.ctx = _context()
         dg = DeviceGaussians.from_points(self.ctx, dataset.points, dataset.colors, degree=config.sh_degree,
@@ -187,7 +204,17 @@ class GaussianTrainer:
                          densify_size_thresh=config.densify_size_thresh,
                          stop_screen_size_at=config.stop_screen_size_at, split_screen_size=config.split_screen_size,
                          bg_color=tuple(config.bg_color))
-        self._tr = Trainer(self.ctx, dg, [c for c, _ in dataset._train], [t for _, t in dataset._train], tc)
+        po = None
+        if config.priors_enabled():  # Model::configurePriors (model.cpp:814-864)
+            po = PriorOptions(depth_weight=config.depth_weight, depth_weight_final=config.depth_weight_final,
+                              depth_huber_delta=config.depth_huber_delta, depth_min_alpha=config.depth_min_alpha,
+                              sky_alpha_weight=config.sky_alpha_weight, fill_weight=config.fill_weight,
+                              use_masks=config.use_masks, learn_sky=config.learn_sky, sky_width=config.sky_width,
+                              sky_height=config.sky_height, sky_lr=config.sky_lr,
+                              exposure=config.exposure_compensation, exposure_lr=config.exposure_lr,
+                              exposure_reg=config.exposure_reg, max_scale_ratio=config.max_scale_ratio)
+        self._tr = Trainer(self.ctx, dg, [c for c, _ in dataset._train], [t for _, t in dataset._train], tc,
+                           priors=po, camera_priors=getattr(dataset, "_train_priors", None))
 
     # training
     @property
@@ -227,9 +254,19 @@ class GaussianTrainer:
 
     # rendering
     def _render(self, cam: dict) -> int:
+        """Model::render (model.cpp:563-572): with a learned sky, the sky behind the gaussians."""
+        import ctypes as C
         from .backend import render
-        dtu = min(self._tr.step_count // max(self.config.sh_degree_interval, 1), self.config.sh_degree)
-        return render(self.ctx, self._tr.dg, dict(cam, dtu=dtu), tuple(self.config.bg_color))
+        tr = self._tr
+        dtu = min(tr.step_count // max(self.config.sh_degree_interval, 1), self.config.sh_degree)
+        if tr.priors is not None and tr.priors.learn_sky:
+            p = tr._render_prior(False)
+            g, c = tr.dg.struct(), make_camera_struct(dict(cam, dtu=dtu), tr.dg.degree)
+            out = C.c_void_p()
+            check(lib().msplat_hip_render_prior(self.ctx.ptr, C.byref(g), C.byref(c), tr.bg, C.byref(p),
+                                                C.byref(out)))
+            return out.value
+        return render(self.ctx, tr.dg, dict(cam, dtu=dtu), tuple(self.config.bg_color))
 
     def _cam(self, cam_idx: int, use_test: bool) -> dict:
         return (self.dataset._test if use_test else self.dataset._train)[cam_idx][0]
@@ -304,13 +341,34 @@ class GaussianTrainer:
 
     # the prior-guided path
     def render_depth(self, cam_idx: int, use_test: bool = False):
-        raise NotImplementedError("expected depth belongs to the prior-guided path")
+        """Expected depth (dataset units, 0 where empty) and accumulated alpha: two numpy
+        (H, W) float32 arrays (Model::renderDepth, model.cpp:866-881)."""
+        import ctypes as C
+        from .prior import aux_outputs, finalize_depth
+        cam = self._cam(cam_idx, use_test)
+        tr = self._tr
+        dtu = min(tr.step_count // max(self.config.sh_degree_interval, 1), self.config.sh_degree)
+        p = tr._render_prior(True) if tr.priors is not None else Prior()
+        p.aux = 1
+        g, c = tr.dg.struct(), make_camera_struct(dict(cam, dtu=dtu), tr.dg.degree)
+        out = C.c_void_p()
+        check(lib().msplat_hip_render_prior(self.ctx.ptr, C.byref(g), C.byref(c), tr.bg, C.byref(p), C.byref(out)))
+        d, t = aux_outputs(self.ctx, cam["H"], cam["W"])
+        return finalize_depth(self.ctx, d, t, 1.0)
 
-    def prior_losses(self):
-        raise NotImplementedError("prior losses belong to the prior-guided path")
+    def prior_losses(self) -> dict:
+        """Mean prior losses of the last step: dict with depth, sky, fill. Syncs the GPU."""
+        return self._tr.prior_losses()
 
     def export_sky(self, path: str):
-        raise NotImplementedError("the learned sky belongs to the prior-guided path")
+        """Save the learned sky as an equirect PNG (requires learn_sky=True): 8-bit RGB,
+        round(clamp(v * 255, 0, 255)) as the reference's imwriteRGB."""
+        tr = self._tr
+        if tr.priors is None or not tr.priors.learn_sky:
+            raise RuntimeError("No learned sky (enable learn_sky)")
+        tex = tr.prior_state.numpy("sky_tex")
+        rgb = (np.clip(tex * np.float32(255.0), 0, 255) + np.float32(0.5)).astype(np.uint8)
+        _write_png(path, rgb)
 
 
 # ── GPU primitive hooks (bindings.cpp:655-738) ───────────────────────────────

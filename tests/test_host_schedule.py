@@ -220,3 +220,37 @@ def test_undistort_roi_properties():
     assert undistort_roi(120, 90, 100, 102, 60, 45, 0, 0, 0, 0, 0) == (0, 0, 119, 89)
     x, y, w, h = undistort_roi(120, 90, 100, 102, 61, 44, 0.25, 0.05, 0.001, -0.002, 0.01)
     assert x > 0 and y > 0 and x + w < 120 and y + h < 90
+
+
+def test_prior_schedules_and_sky_frame():
+    """Model::depthWeightAt (model.cpp:693-701): constant without a final weight, linear
+    to zero with final 0, log-linear between two weights; computeSkyFrame (704-737): an
+    orthonormal frame whose zenith is the mean camera up and whose azimuth 0 is the mean
+    heading; initSkyTexture (739-812) on one camera looking along azimuth 0."""
+    from msplat.camera import look_at, make_camera
+    from msplat.trainer import PriorOptions, depth_weight_at, init_sky_texture, sky_frame
+    o = PriorOptions(depth_weight=0.2)
+    assert depth_weight_at(o, 0, 100) == np.float32(0.2) and depth_weight_at(o, 90, 100) == np.float32(0.2)
+    o = PriorOptions(depth_weight=0.2, depth_weight_final=0.0)
+    assert abs(depth_weight_at(o, 25, 100) - 0.15) < 1e-7 and depth_weight_at(o, 200, 100) == 0.0
+    o = PriorOptions(depth_weight=0.2, depth_weight_final=0.05)
+    assert abs(depth_weight_at(o, 50, 100) - 0.1) < 1e-6 and abs(depth_weight_at(o, 100, 100) - 0.05) < 1e-7
+    assert depth_weight_at(PriorOptions(), 10, 100) == 0.0
+    cams = [make_camera(look_at((0.0, 0.0, 0.0), (1.0, 0.02 * k, 0.0)), 64, 48, 50.0, 50.0, 32.0, 24.0)
+            for k in range(-2, 3)]
+    fr = sky_frame(cams).reshape(3, 3).astype(np.float64)
+    np.testing.assert_allclose(fr @ fr.T, np.eye(3), atol=1e-6)
+    np.testing.assert_allclose(fr[0], [0, 1, 0], atol=1e-3)   # zenith: the cameras' up
+    np.testing.assert_allclose(fr[1], [1, 0, 0], atol=1e-3)   # azimuth 0: their heading
+    cam = cams[2]
+    img = np.zeros((48, 64, 3), np.float32)
+    img[:24] = (0.2, 0.5, 0.9)                                 # sky above the horizon
+    img[24:] = (0.3, 0.3, 0.3)
+    aux = np.zeros((24, 32, 4), np.uint8)
+    aux[:12, :, 1] = 255
+    tex = init_sky_texture([cam], [img], [dict(aux=aux, has_sky=True)], sky_frame([cam]), 32, 16)
+    assert tex.shape == (16, 32, 3)
+    np.testing.assert_allclose(tex[:8].reshape(-1, 3), np.broadcast_to((0.2, 0.5, 0.9), (8 * 32, 3)), atol=1e-6)  # seen or row-filled
+    np.testing.assert_allclose(tex[8:].reshape(-1, 3), np.broadcast_to((0.2, 0.5, 0.9), (8 * 32, 3)), atol=1e-6)
+    blank = init_sky_texture([cam], [img], [None], sky_frame([cam]), 8, 4)
+    np.testing.assert_allclose(blank, np.broadcast_to((0.6, 0.7, 0.85), (4, 8, 3)), atol=1e-7)
