@@ -432,7 +432,8 @@ def copy_calibration(mib=256, times=5):
 
 
 def kernel_key(name: str) -> str:
-    """'void msplat::raster_backward_kernel<2, 64>(msplat::RasterArgs)' -> 'raster_backward_kernel<2, 64>'."""
+    """'void msplat::raster_backward_kernel<2, 64, false>(msplat::RasterArgs)' ->
+    'raster_backward_kernel<2, 64, false>'."""
     n = name.strip().strip('"')
     if n.startswith("void "):
         n = n[5:]
@@ -449,6 +450,12 @@ def kernel_key(name: str) -> str:
     return n[:cut]
 
 
+def bwd_kernel_name(ppt: int, bb: int) -> str:
+    """The one-view raster backward's demangled name (template: pixels per thread, LDS
+    batch depth, prior-guided aux sums off)."""
+    return "raster_backward_kernel<%d, %d, false>" % (ppt, bb)
+
+
 def profiled_counters(variant: dict):
     """Per-launch counters of the raster backward variant `variant` on the timed steps,
     from the committed counter summary (rocprofv3 --pmc runs of the same workload and
@@ -456,7 +463,7 @@ def profiled_counters(variant: dict):
     if not os.path.exists(COUNTERS):
         return None
     prof = json.load(open(COUNTERS))
-    name = "raster_backward_kernel<%d, %d>" % (variant.get("bwd_ppt", 4), variant.get("bwd_batch", 128))
+    name = bwd_kernel_name(variant.get("bwd_ppt", 4), variant.get("bwd_batch", 128))
     entry = prof.get("kernels", {}).get(name)
     return dict(entry, kernel=name) if entry else None
 
@@ -858,7 +865,7 @@ def main():
     valu = prof.get("SQ_INSTS_VALU") if prof else None
     achieved = valu * 64 * 2 / (avg_ms * 1e-3) / 1e12 if valu else None
     roofline = {
-        "bound": "valu", "kernel": "raster_backward_kernel<%d, %d>" % (cur["bwd_ppt"], cur["bwd_batch"]),
+        "bound": "valu", "kernel": bwd_kernel_name(cur["bwd_ppt"], cur["bwd_batch"]),
         "achieved": round(achieved, 3) if achieved else None, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": round(achieved / FP32_PEAK_TFLOPS, 4) if achieved else None,
         "traffic": prof.get("hbm_bytes") if prof else None,

@@ -97,7 +97,10 @@ __device__ inline float sum_slot_d(const float *sd, uint32_t o, uint64_t m) {
     return v;
 }
 
-template <int MODE>
+// SP: the camera and Adam step sizes come from the device StepParams block (a captured
+// step) rather than the kernel arguments; a compile-time choice, so that the kernel
+// arguments stay scalar loads in the launch-by-launch step
+template <int MODE, bool SP>
 __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g >= a.n) return;
@@ -148,7 +151,7 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
         for (int i = 0; i < 3; i++) { a.v_conic[3 * g + i] = vcn[i]; a.v_rgb[3 * g + i] = vrgb[i]; }
         a.v_opac[g] = vop;
     }
-    const msplat_hip_camera &cam = a.sp ? a.sp->cam : a.cam;
+    const msplat_hip_camera &cam = SP ? a.sp->cam : a.cam;
     float vm[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
     float dirv[3] = {0.f, 0.f, 0.f};
     if (vis) {
@@ -192,7 +195,7 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
     }
 
     const float b1 = a.beta1, b2 = a.beta2, eps = a.eps;
-    const float *stp = a.sp ? a.sp->step : a.step, *bc2 = a.sp ? a.sp->bc2 : a.bc2;
+    const float *stp = SP ? a.sp->step : a.step, *bc2 = SP ? a.sp->bc2 : a.bc2;
     if (vis) {  // SH DC: visible gaussians only
         for (int c = 0; c < 3; c++) {
             const size_t i = 3 * (size_t)g + c;
@@ -326,10 +329,11 @@ __global__ __launch_bounds__(256) void sh_rest_kernel(BackwardArgs a) {
 // the frame's visible list (grid-stride over its device-side count), so the many
 // gaussians outside the view, whose coefficients the fused SH Adam leaves alone, cost
 // no threads. A gaussian's coefficients are contiguous, so its threads stay coalesced.
+template <bool SP>
 __global__ __launch_bounds__(256) void sh_rest_adam_kernel(BackwardArgs a) {
     if (a.overflow && *a.overflow) return;  // over capacity: nothing changes
-    const msplat_hip_camera &cam = a.sp ? a.sp->cam : a.cam;
-    const float step = a.sp ? a.sp->step[4] : a.step[4], bc2 = a.sp ? a.sp->bc2[4] : a.bc2[4];
+    const msplat_hip_camera &cam = SP ? a.sp->cam : a.cam;
+    const float step = SP ? a.sp->step[4] : a.step[4], bc2 = SP ? a.sp->bc2[4] : a.bc2[4];
     const uint32_t stride = (sh_bases(cam.degree) - 1) * 3;
     const int dtu = (int)min(cam.degrees_to_use, 3u);
     const uint32_t nbasis = dtu >= 3 ? 15 : dtu == 2 ? 8 : dtu == 1 ? 3 : 0;
@@ -504,10 +508,15 @@ void launch_backward(const BackwardArgs &a, BackwardMode mode, hipStream_t s) {
     const int nbe = (int)((ne + 255) / 256);
     big_reduce_kernel<<<BIG_REDUCE_GRID, 256, 0, s>>>(a);
     if (mode == BWD_ADAM) {
-        backward_kernel<BWD_ADAM><<<nb, 256, 0, s>>>(a);
-        if (nbe) sh_rest_adam_kernel<<<std::min(nbe, SH_ADAM_GRID), 256, 0, s>>>(a);
+        if (a.sp) {
+            backward_kernel<BWD_ADAM, true><<<nb, 256, 0, s>>>(a);
+            if (nbe) sh_rest_adam_kernel<true><<<std::min(nbe, SH_ADAM_GRID), 256, 0, s>>>(a);
+        } else {
+            backward_kernel<BWD_ADAM, false><<<nb, 256, 0, s>>>(a);
+            if (nbe) sh_rest_adam_kernel<false><<<std::min(nbe, SH_ADAM_GRID), 256, 0, s>>>(a);
+        }
     } else {
-        backward_kernel<BWD_GRADS><<<nb, 256, 0, s>>>(a);
+        backward_kernel<BWD_GRADS, false><<<nb, 256, 0, s>>>(a);
         if (nbe) sh_rest_kernel<BWD_GRADS><<<nbe, 256, 0, s>>>(a);
     }
 }

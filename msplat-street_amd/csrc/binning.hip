@@ -48,9 +48,12 @@ __device__ inline uint32_t block_scan_u32(uint32_t v, uint32_t &total, uint32_t 
 
 // ── projection + SH colour (project_and_sh, msplat_cpu.cpp:452-500) ────────────
 // Also writes the workgroup's intersection count and visible-record count.
+// SP: the camera comes from the device StepParams block (a captured step), chosen at
+// compile time so that the launch-by-launch step reads it from the kernel arguments
+template <bool SP>
 __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
     __shared__ uint32_t lds4[4];
-    const msplat_hip_camera &cam = a.sp ? a.sp->cam : a.cam;
+    const msplat_hip_camera &cam = SP ? a.sp->cam : a.cam;
     const int g = blockIdx.x * 256 + threadIdx.x;
     uint32_t count = 0;
     if (g < a.n) {
@@ -749,7 +752,10 @@ void launch_scan_u32_pair(uint32_t *a0, const ScanOut &o0, uint32_t *a1, const S
 
 void launch_project(const ProjectArgs &a, hipStream_t s) {
     const int nb = (a.n + 255) / 256;
-    if (nb > 0) project_kernel<<<nb, 256, 0, s>>>(a);
+    if (nb > 0) {
+        if (a.sp) project_kernel<true><<<nb, 256, 0, s>>>(a);
+        else project_kernel<false><<<nb, 256, 0, s>>>(a);
+    }
 }
 
 void launch_rect_count(int n, const float4 *splat_a, const float2 *aabb, const int32_t *radii, int tx, int ty,
