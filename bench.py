@@ -318,12 +318,12 @@ SERVE_SCENES = (500_000, 1_000_000, 2_000_000, 3_000_000)
 SERVE_RATES = (200, 500, 1000, 2000)
 
 
-def bench_serving(rates=SERVE_RATES, duration=1.0, slots=2, max_views=16, seed=20):
+def bench_serving(rates=SERVE_RATES, duration=1.0, slots=6, max_views=16, seed=20):
     """Serving: four resident SH3 scenes (0.5M, 1M, 2M, 3M Gaussians, 64-cluster clouds in
     an 8 m box, kNN scales, opacity logits N(1.5, 1), higher SH bands N(0, 0.05)).
     Poisson arrivals at each rate for `duration` s; pose on a sphere of radius 3-10 m
     around the scene's centre, resolution 640x360 40% / 1280x720 30% / 1920x1080 20% /
-    3840x2160 10%. The continuous-batching server (2 slots, up to 16 views per batch, an
+    3840x2160 10%. The continuous-batching server (6 slots, up to 16 views per batch, an
     HBM budget of the resident scenes plus 90% of the device's free memory) against the
     plain form (one msplat_hip_render per request, first come first served), each
     followed by the RGBA8 pack. A last run at the top rate gives the server a budget that

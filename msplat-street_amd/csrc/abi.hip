@@ -155,7 +155,7 @@ int do_bin(msplat_hip_ctx *ctx, uint32_t cap = 0) {
     ctx->vis_sorted = dvi;
     // intersection offsets in depth order
     launch_order_sums(dvi, ctx->d_vis32(), (uint32_t)Nc, (const uint32_t *)ctx->count.p, (uint32_t *)ctx->ord_sums.p,
-                      s);
+                      (uint32_t *)ctx->loff.p, s);
     KeygenArgs k;
     memset(&k, 0, sizeof k);
     k.list = dvi;

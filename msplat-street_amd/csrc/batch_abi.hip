@@ -279,6 +279,13 @@ BatchProjectArgs project_args(BatchState *b, const msplat_hip_gaussians *g) {
     a.count = (uint32_t *)b->count.p;
     a.block_sums = (uint32_t *)b->block_sums.p;
     a.block_vis = (uint32_t *)b->block_vis.p;
+    uint32_t dtu = 0, deg = 0;
+    for (int v = 0; v < b->views; v++) {
+        dtu = std::max(dtu, b->h_tab->views[v].cam.degrees_to_use);
+        deg = std::max(deg, b->h_tab->views[v].cam.degree);
+    }
+    a.sh_u = deg > 3 ? 0 : 3 * (int)sh_bases(dtu);  // degree 4: read in place
+    a.sh_stride = 3 * (size_t)(sh_bases(deg) - 1);
     return a;
 }
 
@@ -353,7 +360,8 @@ void enqueue_sort_and_forward(msplat_hip_ctx *ctx, BatchState *b, const msplat_h
     launch_radix_pass(dki, dvi, dko, dvo, &sc->nvis, R, 0, (uint32_t *)b->hist.p, (uint32_t *)b->digit_total.p, s);
     std::swap(dki, dko);
     std::swap(dvi, dvo);
-    launch_order_sums(dvi, &sc->nvis, R, (const uint32_t *)b->count.p, (uint32_t *)b->ord_sums.p, s);
+    launch_order_sums(dvi, &sc->nvis, R, (const uint32_t *)b->count.p, (uint32_t *)b->ord_sums.p,
+                      (uint32_t *)b->loff.p, s);
     launch_scan_u32((uint32_t *)b->ord_sums.p, (R + 255) / 256, scan_parts(b, 2),
                     ScanOut{nullptr, &sc->total32_b, b->cap, &sc->overflow_b, nullptr}, s);
     KeygenArgs k;
@@ -535,6 +543,9 @@ std::vector<uint64_t> graph_key(msplat_hip_ctx *ctx, BatchState *b, const msplat
                                (uint64_t)ctx->tuning.fwd_batch, (uint64_t)ctx->tuning.bwd_batch,
                                (uint64_t)ctx->tuning.cull,
                                b->h_tab->views[0].cam.degree};
+    uint32_t dtu = 0;  // the SH floats the projection stages (project_args)
+    for (int v = 0; v < b->views; v++) dtu = std::max(dtu, b->h_tab->views[v].cam.degrees_to_use);
+    k.push_back(dtu);
     const void *ptrs[] = {g->means, g->scales, g->quats, g->features_dc, g->features_rest, g->opacities,
                           st->vis_counts, st->xys_grad_norm, st->max_2d_size};
     for (const void *p : ptrs) k.push_back((uint64_t)(uintptr_t)p);

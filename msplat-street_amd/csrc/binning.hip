@@ -49,50 +49,65 @@ __device__ inline uint32_t block_scan_u32(uint32_t v, uint32_t &total, uint32_t 
 // ── projection + SH colour (project_and_sh, msplat_cpu.cpp:452-500) ────────────
 // Also writes the workgroup's intersection count and visible-record count.
 // SP: the camera comes from the device StepParams block (a captured step), chosen at
-// compile time so that the launch-by-launch step reads it from the kernel arguments
+// compile time so that the launch-by-launch step reads it from the kernel arguments.
+// The SH coefficients of a wave's visible gaussians are staged in LDS by stage_sh
+// (dynamic LDS: project_sh_lds bytes for the degree in use).
 template <bool SP>
 __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
+    extern __shared__ float shl[];
     __shared__ uint32_t lds4[4];
+    __shared__ uint8_t widx[4][64];
     const msplat_hip_camera &cam = SP ? a.sp->cam : a.cam;
     const int g = blockIdx.x * 256 + threadIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t count = 0;
+    bool need = false;
+    float p[3] = {0.f, 0.f, 0.f}, pv[3] = {0.f, 0.f, 0.f}, conic[3] = {0.f, 0.f, 0.f}, radius = 0.f;
+    float px = 0.f, py = 0.f, ax = 0.f, ay = 0.f;
+    int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
     if (g < a.n) {
         a.radii[g] = 0;
         a.count[g] = 0;
-        const float p[3] = {a.means[3 * g], a.means[3 * g + 1], a.means[3 * g + 2]};
-        float pv[3];
+        p[0] = a.means[3 * g]; p[1] = a.means[3 * g + 1]; p[2] = a.means[3 * g + 2];
         cam_space(cam.viewmat, p, pv);
         if (pv[2] > cam.clip_thresh) {
             const float s[3] = {exp_cr(a.scales[3 * g]), exp_cr(a.scales[3 * g + 1]), exp_cr(a.scales[3 * g + 2])};
             const float q[4] = {a.quats[4 * g], a.quats[4 * g + 1], a.quats[4 * g + 2], a.quats[4 * g + 3]};
-            float cov[6], c2[3], conic[3], radius;
+            float cov[6], c2[3];
             covariance3d(s, cam.glob_scale, q, cov);
             const float tfx = 0.5f * (float)cam.width / cam.fx, tfy = 0.5f * (float)cam.height / cam.fy;
             ewa_cov2d(cov, cam.viewmat, cam.fx, cam.fy, tfx, tfy, pv, c2);
             if (conic_radius(c2, conic, radius)) {
-                float px, py;
                 to_pixel(cam.projmat, p, cam.width, cam.height, cam.cx, cam.cy, px, py);
-                const float ax = ceilf(3.f * sqrtf(c2[0])), ay = ceilf(3.f * sqrtf(c2[2]));
-                int x0, y0, x1, y1;
+                ax = ceilf(3.f * sqrtf(c2[0]));
+                ay = ceilf(3.f * sqrtf(c2[2]));
                 tile_rect(px, py, ax, ay, cam.tiles_x, cam.tiles_y, x0, y0, x1, y1);
-                if ((x1 - x0) * (y1 - y0) > 0) {
-                    float d[3], col[3];
-                    view_direction(p, cam.cam_pos, d);
-                    const uint32_t nb = sh_bases(cam.degree);
-                    sh_color(cam.degrees_to_use, d, a.dc + 3 * (size_t)g, a.rest + (size_t)(nb - 1) * 3 * g, col);
-                    const float opac = 1.f / (1.f + exp_cr(-a.opac[g]));
-                    a.splat_a[g] = make_float4(px, py, conic[0], conic[1]);
-                    a.splat_b[g] = make_float4(conic[2], opac, col[0], col[1]);
-                    a.splat_blue[g] = col[2];
-                    a.depths[g] = pv[2];
-                    a.aabb[g] = make_float2(ax, ay);
-                    a.rect[g] = make_uint2((uint32_t)x0 | ((uint32_t)x1 << 16), (uint32_t)y0 | ((uint32_t)y1 << 16));
-                    a.radii[g] = (int)radius;
-                    count = (uint32_t)((x1 - x0) * (y1 - y0));
-                    a.count[g] = count;
-                }
+                need = (x1 - x0) * (y1 - y0) > 0;
             }
         }
+    }
+    // staged up to SH degree 3 (project_sh_lds); degree 4 reads its coefficients in place
+    const size_t stride = 3 * (size_t)(sh_bases(cam.degree) - 1);
+    const bool staged = cam.degree <= 3;
+    const int u = 3 * (int)sh_bases(cam.degrees_to_use), us = u | 1;
+    float *wl = shl + (size_t)w * 64 * us;
+    if (staged) stage_sh(need, g - lane, a.dc, a.rest, stride, u, us, wl, widx[w]);
+    if (need) {
+        float d[3], col[3];
+        view_direction(p, cam.cam_pos, d);
+        const float *mine = wl + lane * us;
+        if (staged) sh_color(cam.degrees_to_use, d, mine, mine + 3, col);
+        else sh_color(cam.degrees_to_use, d, a.dc + 3 * (size_t)g, a.rest + stride * g, col);
+        const float opac = 1.f / (1.f + exp_cr(-a.opac[g]));
+        a.splat_a[g] = make_float4(px, py, conic[0], conic[1]);
+        a.splat_b[g] = make_float4(conic[2], opac, col[0], col[1]);
+        a.splat_blue[g] = col[2];
+        a.depths[g] = pv[2];
+        a.aabb[g] = make_float2(ax, ay);
+        a.rect[g] = make_uint2((uint32_t)x0 | ((uint32_t)x1 << 16), (uint32_t)y0 | ((uint32_t)y1 << 16));
+        a.radii[g] = (int)radius;
+        count = (uint32_t)((x1 - x0) * (y1 - y0));
+        a.count[g] = count;
     }
     uint32_t total, vis;
     block_scan_u32(count, total, lds4);
@@ -355,13 +370,6 @@ __global__ __launch_bounds__(256) void rs_scan_kernel(uint32_t *hist, int stride
     if (threadIdx.x == 0) digit_total[blockIdx.x] = carry;
 }
 
-// Orders LDS accesses of the lanes of one wave (no instruction is emitted).
-__device__ inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Wave w ranks its own contiguous quarter of the workgroup's elements, 64 at a time,
 // with private per-digit counters (no workgroup barrier inside the loop); the waves'
 // counts are then combined in wave order, which keeps the sort stable.
@@ -463,9 +471,11 @@ __global__ __launch_bounds__(256) void rs_scatter_kernel(const K *kin, const int
 // ── intersections in depth order ──────────────────────────────────────────────
 
 // Per 256-entry block of the depth-ordered list: the sum of its records' counts
-// (0 for blocks past the list's device-side length).
+// (0 for blocks past the list's device-side length). Each record's count also goes to
+// lcount[i] in list order, so the offsets kernel reads it without a second gather.
 __global__ __launch_bounds__(256) void order_sums_kernel(const int32_t *list, const uint32_t *nvis_ptr,
-                                                        const uint32_t *count, uint32_t *sums, uint32_t nchunks_max) {
+                                                        const uint32_t *count, uint32_t *sums, uint32_t nchunks_max,
+                                                        uint32_t *lcount) {
     __shared__ uint32_t lds4[4];
     const uint32_t nvis = *nvis_ptr;
     for (uint32_t chunk = blockIdx.x; chunk < nchunks_max; chunk += gridDim.x) {
@@ -475,6 +485,7 @@ __global__ __launch_bounds__(256) void order_sums_kernel(const int32_t *list, co
         }
         const uint32_t i = chunk * 256 + threadIdx.x;
         const uint32_t c = i < nvis ? count[list[i]] : 0u;
+        if (i < nvis) lcount[i] = c;
         uint32_t t;
         block_scan_u32(c, t, lds4);
         if (threadIdx.x == 0) sums[chunk] = t;
@@ -484,6 +495,7 @@ __global__ __launch_bounds__(256) void order_sums_kernel(const int32_t *list, co
 // Record list[i] writes its tiles in row-major order at offsets[r] = the exclusive
 // prefix of the counts in list order (also kept in list order, loff[i], for the
 // emission), and joins the big list when it has more than BIG_SLOTS intersections.
+// loff[i] holds the record's count on entry (order_sums_kernel).
 __global__ __launch_bounds__(256) void keygen_offsets_kernel(KeygenArgs a) {
     __shared__ uint32_t lds4[4];
     __shared__ uint32_t s_bigbase;
@@ -492,7 +504,7 @@ __global__ __launch_bounds__(256) void keygen_offsets_kernel(KeygenArgs a) {
     for (uint32_t chunk = blockIdx.x; chunk * 256 < nvis; chunk += gridDim.x) {
         const uint32_t i = chunk * 256 + threadIdx.x;
         const int32_t r = i < nvis ? a.list[i] : -1;
-        const uint32_t c = r >= 0 ? a.count[r] : 0u;
+        const uint32_t c = r >= 0 ? a.loff[i] : 0u;
         uint32_t total;
         uint32_t base;
         if (a.chunk_sums) {  // the sums of the chunks before this one, added here
@@ -517,10 +529,15 @@ __global__ __launch_bounds__(256) void keygen_offsets_kernel(KeygenArgs a) {
 }
 
 // Key emission balanced over the output: workgroup b writes intersections
-// [b * EMIT_CHUNK, (b + 1) * EMIT_CHUNK). It finds the records covering its slice by
-// binary search over loff, stages their offsets and ids in LDS, and each output finds
-// its record there; consecutive threads write consecutive keys whatever the records'
-// footprints (depth order puts the large near splats next to each other).
+// [b * EMIT_CHUNK, (b + 1) * EMIT_CHUNK). It finds the records covering its slice by a
+// block search over loff and stages their offsets, ids and tile rectangles in LDS, marking
+// where each record after the first starts. Thread t then writes the slice's outputs
+// 8t..8t+7: the record of output p is the number of marks at or before p (a block scan of
+// the threads' mark counts), and its tile follows from the previous output's by one step
+// along the rectangle's row, so one division per thread places its first output. The eight
+// keys (and ids) go out as two 16-byte stores; consecutive threads write consecutive
+// keys whatever the records' footprints (depth order puts the large near splats next to
+// each other).
 constexpr int EMIT_ITEMS = 8;
 constexpr int EMIT_CHUNK = 256 * EMIT_ITEMS;
 
@@ -528,7 +545,10 @@ template <bool BATCH>
 __global__ __launch_bounds__(256) void keygen_emit_kernel(KeygenArgs a) {
     __shared__ uint32_t s_off[EMIT_CHUNK];
     __shared__ int32_t s_rec[EMIT_CHUNK];
+    __shared__ uint2 s_rect[EMIT_CHUNK];
+    __shared__ __align__(8) uint8_t s_mark[EMIT_CHUNK];
     __shared__ uint32_t s_first, s_n;
+    __shared__ uint32_t lds4[4];
     if (a.overflow && *a.overflow) return;
     const uint32_t M = *a.m_ptr;
     const uint32_t o0 = blockIdx.x * EMIT_CHUNK;
@@ -555,40 +575,81 @@ __global__ __launch_bounds__(256) void keygen_emit_kernel(KeygenArgs a) {
             s_first = lo1;
             s_n = lo2 - lo1 + 1;  // <= o1 - o0: every record after the first starts inside the slice
         }
+        ((uint32_t *)s_mark)[threadIdx.x] = 0u;
+        ((uint32_t *)s_mark)[threadIdx.x + 256] = 0u;
     }
     __syncthreads();
     const uint32_t first = s_first, nrec = s_n;
     for (uint32_t j = threadIdx.x; j < nrec; j += 256) {
-        s_off[j] = a.loff[first + j];
-        s_rec[j] = a.list[first + j];
+        const uint32_t off = a.loff[first + j];
+        const int32_t rec = a.list[first + j];
+        s_off[j] = off;
+        s_rec[j] = rec;
+        s_rect[j] = a.rect[rec];
+        if (j > 0) s_mark[off - o0] = 1;  // in (0, o1 - o0): records after the first start inside
     }
     __syncthreads();
+    const uint32_t p0 = EMIT_ITEMS * threadIdx.x;
+    const uint2 mk = *(const uint2 *)(s_mark + p0);
+    uint32_t m[EMIT_ITEMS];
 #pragma unroll
-    for (int k = 0; k < EMIT_ITEMS; k++) {
-        const uint32_t o = o0 + k * 256 + threadIdx.x;
-        if (o >= o1) break;
-        uint32_t lo = 0, hi = nrec - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= o) lo = mid;
-            else hi = mid - 1;
+    for (int i = 0; i < 4; i++) {
+        m[i] = (mk.x >> (8 * i)) & 0xFFu;
+        m[4 + i] = (mk.y >> (8 * i)) & 0xFFu;
+    }
+    uint32_t total;
+    uint32_t j = block_scan_u32(m[0] + m[1] + m[2] + m[3] + m[4] + m[5] + m[6] + m[7], total, lds4);
+    if (o0 + p0 >= o1) return;
+    uint32_t kv[EMIT_ITEMS], vv[EMIT_ITEMS];
+    uint32_t q = 0, rem = 0, w = 1, x0 = 0, y0 = 0, tiles_x = (uint32_t)a.tiles_x;
+    int32_t r = 0;
+#pragma unroll
+    for (int i = 0; i < EMIT_ITEMS; i++) {
+        const bool start = m[i] != 0u;  // record j + 1 starts at this output
+        j += m[i];
+        if (i == 0 || start) {
+            r = s_rec[j];
+            const uint2 rr = s_rect[j];
+            x0 = rr.x & 0xFFFF;
+            w = (rr.x >> 16) - x0;
+            y0 = rr.y & 0xFFFF;
+            const uint32_t local = o0 + p0 + (uint32_t)i - s_off[j];
+            q = local / w;
+            rem = local - q * w;
+            // local tile id (a batch keeps its views apart by the emission order)
+            if (BATCH) tiles_x = (uint32_t)a.tab->views[r / a.n].cam.tiles_x;
+        } else if (++rem == w) {
+            rem = 0;
+            q++;
         }
-        const int32_t r = s_rec[lo];
-        const uint32_t local = o - s_off[lo];
-        const uint2 rr = a.rect[r];
-        const int x0 = rr.x & 0xFFFF, x1 = rr.x >> 16, y0 = rr.y & 0xFFFF;
-        const uint32_t w = (uint32_t)(x1 - x0);
-        const int ty = y0 + (int)(local / w), tx = x0 + (int)(local % w);
-        // local tile id (a batch keeps its views apart by the emission order)
-        const uint32_t tiles_x = BATCH ? (uint32_t)a.tab->views[r / a.n].cam.tiles_x : (uint32_t)a.tiles_x;
-        const uint32_t tile = (uint32_t)ty * tiles_x + (uint32_t)tx;
+        const uint32_t tile = (y0 + q) * tiles_x + (x0 + rem);
         if (!BATCH && a.gid_bits) {
-            a.keys[o] = (tile << a.gid_bits) | (uint32_t)r;
+            kv[i] = (tile << a.gid_bits) | (uint32_t)r;
         } else {
-            a.keys[o] = tile;
-            a.vals[o] = r;
+            kv[i] = tile;
+            vv[i] = (uint32_t)r;
         }
-        if (a.slot_f) a.slot_f[o] = 0;  // output o is the record's slot o (emission order)
+    }
+    const bool vals = BATCH || !a.gid_bits;
+    const uint32_t ob = o0 + p0;
+    if (ob + EMIT_ITEMS <= o1) {  // 32-byte aligned: o0 is a multiple of EMIT_CHUNK
+        uint4 *kd = (uint4 *)(a.keys + ob);
+        kd[0] = make_uint4(kv[0], kv[1], kv[2], kv[3]);
+        kd[1] = make_uint4(kv[4], kv[5], kv[6], kv[7]);
+        if (vals) {
+            uint4 *vd = (uint4 *)(a.vals + ob);
+            vd[0] = make_uint4(vv[0], vv[1], vv[2], vv[3]);
+            vd[1] = make_uint4(vv[4], vv[5], vv[6], vv[7]);
+        }
+        if (a.slot_f) *(uint2 *)(a.slot_f + ob) = make_uint2(0u, 0u);  // output o is the record's slot o
+    } else {
+#pragma unroll
+        for (int i = 0; i < EMIT_ITEMS; i++) {
+            if (ob + i >= o1) break;
+            a.keys[ob + i] = kv[i];
+            if (vals) a.vals[ob + i] = (int32_t)vv[i];
+            if (a.slot_f) a.slot_f[ob + i] = 0;
+        }
     }
 }
 
@@ -665,9 +726,14 @@ __global__ __launch_bounds__(256) void keys64_kernel(const uint32_t *tiles, cons
 // and record v * n + g is written per view, with the same arithmetic as project_kernel
 // (the records are bit-identical to one-view projections). Per view, the workgroup's
 // intersection and visible-record counts go to block_sums / block_vis[v * nbx + b].
+// The SH coefficients of the bands in use by any view (a.sh_u floats per gaussian) are
+// staged in LDS once for all views (dynamic LDS: project_sh_lds bytes).
 __global__ __launch_bounds__(256) void project_batch_kernel(BatchProjectArgs a) {
+    extern __shared__ float shl[];
     __shared__ uint32_t lds4[4];
+    __shared__ uint8_t widx[4][64];
     const int g = blockIdx.x * 256 + threadIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const bool live = g < a.n;
     float p[3] = {0.f, 0.f, 0.f}, s[3] = {0.f, 0.f, 0.f}, q[4] = {1.f, 0.f, 0.f, 0.f}, logit = 0.f;
     if (live) {
@@ -677,6 +743,11 @@ __global__ __launch_bounds__(256) void project_batch_kernel(BatchProjectArgs a) 
         logit = a.opac[g];
     }
     const float opac = 1.f / (1.f + exp_cr(-logit));
+    // staged up to SH degree 3 (sh_u > 0); degree 4 reads its coefficients in place
+    const int us = a.sh_u | 1;
+    float *wl = shl + (size_t)w * 64 * us;
+    if (a.sh_u) stage_sh(live, g - lane, a.dc, a.rest, a.sh_stride, a.sh_u, us, wl, widx[w]);
+    const float *mine = wl + lane * us;
     for (int v = 0; v < a.num_views; v++) {
         const msplat_hip_camera &cam = a.tab->views[v].cam;
         const size_t r = (size_t)v * a.n + g;
@@ -700,8 +771,8 @@ __global__ __launch_bounds__(256) void project_batch_kernel(BatchProjectArgs a) 
                     if ((x1 - x0) * (y1 - y0) > 0) {
                         float d[3], col[3];
                         view_direction(p, cam.cam_pos, d);
-                        const uint32_t nb = sh_bases(cam.degree);
-                        sh_color(cam.degrees_to_use, d, a.dc + 3 * (size_t)g, a.rest + (size_t)(nb - 1) * 3 * g, col);
+                        if (a.sh_u) sh_color(cam.degrees_to_use, d, mine, mine + 3, col);
+                        else sh_color(cam.degrees_to_use, d, a.dc + 3 * (size_t)g, a.rest + a.sh_stride * g, col);
                         a.splat_a[r] = make_float4(px, py, conic[0], conic[1]);
                         a.splat_b[r] = make_float4(conic[2], opac, col[0], col[1]);
                         a.splat_blue[r] = col[2];
@@ -750,11 +821,18 @@ void launch_scan_u32_pair(uint32_t *a0, const ScanOut &o0, uint32_t *a1, const S
     launch_scan_u32(a1, L, partial1, o1, s);
 }
 
+// LDS of project_kernel's SH staging (four waves of 64 lanes, an odd count of floats per
+// lane) for the bands up to `degree`; none when the stored degree is 4 (read in place)
+size_t project_sh_lds(uint32_t degree, uint32_t stored) {
+    return stored > 3 ? 0 : 4 * 64 * (size_t)((3 * sh_bases(degree)) | 1) * sizeof(float);
+}
+
 void launch_project(const ProjectArgs &a, hipStream_t s) {
     const int nb = (a.n + 255) / 256;
     if (nb > 0) {
-        if (a.sp) project_kernel<true><<<nb, 256, 0, s>>>(a);
-        else project_kernel<false><<<nb, 256, 0, s>>>(a);
+        // a captured step takes the degree in use from device memory: room for every band stored
+        if (a.sp) project_kernel<true><<<nb, 256, project_sh_lds(a.cam.degree, a.cam.degree), s>>>(a);
+        else project_kernel<false><<<nb, 256, project_sh_lds(a.cam.degrees_to_use, a.cam.degree), s>>>(a);
     }
 }
 
@@ -818,9 +896,9 @@ void launch_radix_pass(const uint32_t *kin, const int32_t *vin, uint32_t *kout, 
 }
 
 void launch_order_sums(const int32_t *list, const uint32_t *nvis_dev, uint32_t nvis_max, const uint32_t *count,
-                       uint32_t *sums, hipStream_t s) {
+                       uint32_t *sums, uint32_t *lcount, hipStream_t s) {
     const uint32_t nb = (nvis_max + 255) / 256;
-    if (nb > 0) order_sums_kernel<<<std::min(nb, LIST_GRID), 256, 0, s>>>(list, nvis_dev, count, sums, nb);
+    if (nb > 0) order_sums_kernel<<<std::min(nb, LIST_GRID), 256, 0, s>>>(list, nvis_dev, count, sums, nb, lcount);
 }
 
 void launch_keygen_offsets(const KeygenArgs &a, hipStream_t s) {
@@ -866,7 +944,7 @@ void launch_tile_ranges_views(const uint32_t *keys, const int32_t *recs, const u
 }
 
 void launch_batch_project(const BatchProjectArgs &a, hipStream_t s) {
-    if (a.nbx > 0) project_batch_kernel<<<a.nbx, 256, 0, s>>>(a);
+    if (a.nbx > 0) project_batch_kernel<<<a.nbx, 256, 4 * 64 * (size_t)(a.sh_u | 1) * sizeof(float), s>>>(a);
 }
 
 }  // namespace msplat

@@ -144,6 +144,51 @@ __device__ inline float wave_sums9(const SumLanes &L, float q0, float q1, float 
 
 __device__ inline uint64_t ballot(bool p) { return __ballot(p); }
 
+// Orders LDS accesses of the lanes of one wave (no instruction is emitted).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The SH coefficients (3 DC floats, then the first u - 3 rest floats) of the lanes of a
+// wave with `need`, loaded together: consecutive lanes read consecutive floats of one
+// gaussian's coefficients, so a load instruction touches a few cache lines instead of one
+// per lane. Lane l's coefficients land at wl[l * us, l * us + u); us is odd, so each lane
+// reading its own coefficients hits a different LDS bank. gw is the wave's first
+// gaussian, stride the rest floats per gaussian, widx 64 bytes of LDS. Every lane of
+// the wave calls it.
+__device__ inline void stage_sh(bool need, int gw, const float *dc, const float *rest, size_t stride, int u, int us,
+                                float *wl, uint8_t *widx) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t m = __ballot(need);
+    if (need) widx[__popcll(m & ((1ull << lane) - 1ull))] = (uint8_t)lane;
+    wave_sync();
+    const int total = __popcll(m) * u;
+    constexpr int B = 16;  // loads in flight per lane before their LDS writes
+    for (int t0 = 0; t0 < total; t0 += 64 * B) {
+        float v[B];
+        int at[B];
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            const int t = t0 + 64 * k + lane;
+            at[k] = -1;
+            v[k] = 0.f;
+            if (t < total) {
+                const int j = t / u, c = t - j * u;
+                const int l = widx[j];
+                const size_t gg = (size_t)(gw + l);
+                v[k] = c < 3 ? dc[3 * gg + c] : rest[stride * gg + (size_t)(c - 3)];
+                at[k] = l * us + c;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < B; k++)
+            if (at[k] >= 0) wl[at[k]] = v[k];
+    }
+    wave_sync();
+}
+
 // ── counter-based RNG: PCG hash (Jarzynski & Olano 2020), 24-bit uniforms in (0, 1),
 //    Box-Muller normals (the reference's pcg_hash / rand_uniform / rand_normal3) ──
 __host__ __device__ inline uint32_t pcg_hash(uint32_t v) {

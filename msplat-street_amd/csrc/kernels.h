@@ -324,6 +324,8 @@ struct BatchProjectArgs {
     uint2 *rect;
     int32_t *radii;
     uint32_t *count, *block_sums, *block_vis;  // [v * nbx + b]
+    int sh_u;          // SH floats staged per gaussian: 3 sh_bases(largest degree in use of the views); 0 at degree 4
+    size_t sh_stride;  // rest floats per gaussian
 };
 
 struct BatchLossArgs {
@@ -402,8 +404,9 @@ void launch_scan_u32(uint32_t *a, uint32_t L, uint32_t *partial, const ScanOut &
 void launch_scan_u32_pair(uint32_t *a0, const ScanOut &o0, uint32_t *a1, const ScanOut &o1, uint32_t L,
                           uint32_t *partial0, uint32_t *partial1, hipStream_t s);
 void launch_compact(const CompactArgs &a, hipStream_t s);
+// also writes each record's count in list order to lcount (the offsets kernel's loff)
 void launch_order_sums(const int32_t *list, const uint32_t *nvis_dev, uint32_t nvis_max, const uint32_t *count,
-                       uint32_t *sums, hipStream_t s);
+                       uint32_t *sums, uint32_t *lcount, hipStream_t s);
 void launch_keygen_offsets(const KeygenArgs &a, hipStream_t s);  // offsets, loff, big list
 void launch_keygen_emit(const KeygenArgs &a, bool batch, hipStream_t s);  // the keys
 void launch_keygen(const KeygenArgs &a, bool batch, hipStream_t s);
