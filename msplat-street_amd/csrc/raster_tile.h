@@ -57,9 +57,10 @@ struct TileData {
 // True when the splat composites no pixel of the tile whose top-left pixel is (x0, y0):
 // the tile lies outside the bounding box of the ellipse {0.5 d^T Q d <= L'}, with Q the
 // conic and L' above the largest sigma that passes both sigma < SIGMA_MAX and
-// opacity exp(-sigma) >= ALPHA_MIN. The pixel loops compute sigma to a relative error
-// below 16 u kappa (u = 2^-24, kappa the conic's condition number) and alpha to a few
-// ulp, so the test is made only for kappa < 1024 and L' is the exact level raised by
+// opacity exp(-sigma) >= ALPHA_MIN. The pixel loops compute -log2(e) sigma from the
+// scaled conic (one more rounding per coefficient) to a relative error below 24 u kappa
+// (u = 2^-24, kappa the conic's condition number) and alpha to a few ulp, so the test is
+// made only for kappa < 1024 (24 u kappa < 1.5e-3) and L' is the exact level raised by
 // 2^-8 relative plus 2^-8 absolute; the box is widened by 0.1% plus 0.01 px for the
 // rounding of its own arithmetic. Any pair it rejects would fail the validity test in
 // every pixel, and an invalid pair changes nothing (it adds exact zeros), so skipping
@@ -76,12 +77,24 @@ __device__ inline bool splat_misses_tile(const float4 &A, const float4 &B, float
     return A.x + hx < x0 || A.x - hx > x1 || A.y + hy < y0 || A.y - hy > y1;
 }
 
-// sigma = 1/2 (a dx^2 + c dy^2) + b dx dy in one fixed evaluation order, shared by the
-// forward and backward loops so that both make the same validity decisions. With
-// adx2 = (a dx) dx and bdx = b dx, which the backward hoists out of its pixel rows.
-__device__ inline float splat_sigma(float adx2, float bdx, float c, float dy) {
-    return __builtin_fmaf(bdx, dy, 0.5f * __builtin_fmaf(c * dy, dy, adx2));
+// The pixel loops work with p = -log2(e) sigma, the exponent of alpha's power of two
+// (alpha = opacity 2^p, one v_exp_f32), from a conic scaled when it is staged in LDS:
+// a' = -log2(e)/2 a, b' = -log2(e) b, c' = -log2(e)/2 c, so that
+// p = (a' dx) dx + (c' dy) dy + (b' dx) dy; sigma >= 0 is p <= 0 and sigma < SIGMA_MAX
+// is p > P_MIN. One fixed evaluation order is shared by the forward and backward loops,
+// so that both make the same validity decisions. adx2 = (a' dx) dx and bdx = b' dx are
+// per record, which the backward hoists out of its pixel rows.
+constexpr float POW_HALF = -0.72134752044448170f;  // -log2(e) / 2
+constexpr float POW_ONE = -1.44269504088896341f;   // -log2(e)
+__device__ constexpr float P_MIN = SIGMA_MAX * POW_ONE;
+__device__ inline float4 staged_a(const float4 &A) { return make_float4(A.x, A.y, A.z * POW_HALF, A.w * POW_ONE); }
+__device__ inline float splat_power(float adx2, float bdx, float c, float dy) {
+    return __builtin_fmaf(bdx, dy, __builtin_fmaf(c * dy, dy, adx2));
 }
+__device__ inline float splat_alpha(float opacity, float p) {
+    return fminf(ALPHA_MAX, opacity * __builtin_amdgcn_exp2f(p));
+}
+__device__ inline bool splat_valid(float p, float alpha) { return p <= 0.f && p > P_MIN && alpha >= ALPHA_MIN; }
 
 // Position of the calling thread's item among the workgroup's kept items (in thread
 // order) and, in `total`, their number. Every thread of the NW-wave workgroup calls it;
@@ -147,8 +160,8 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
         int n;
         const int pos = kept_position<4>(keep, wcount, n);
         if (keep) {  // colours stored clamped: max(raw + 0.5, 0), as composited
-            la[pos] = A0;
-            lb[pos] = make_float4(B0.x, B0.y, fmaxf(B0.z + 0.5f, 0.f), fmaxf(B0.w + 0.5f, 0.f));
+            la[pos] = staged_a(A0);
+            lb[pos] = make_float4(B0.x * POW_HALF, B0.y, fmaxf(B0.z + 0.5f, 0.f), fmaxf(B0.w + 0.5f, 0.f));
             lc[pos] = fmaxf(C0 + 0.5f, 0.f);
             lk[pos] = k;
             if (AUX) lz[pos] = Z0;
@@ -174,9 +187,9 @@ __device__ inline void forward_tile(const TileView &v, const TileData &d, int gt
                     const int j = j0 + u;
                     const float4 A = la[j], B = lb[j];
                     const float dx = A.x - fx, dy = A.y - fy;
-                    const float sig = splat_sigma((A.z * dx) * dx, A.w * dx, B.x, dy);
-                    const float alpha = fminf(ALPHA_MAX, B.y * __expf(-sig));
-                    const bool valid = !done && sig >= 0.f && sig < SIGMA_MAX && alpha >= ALPHA_MIN;
+                    const float pw = splat_power((A.z * dx) * dx, A.w * dx, B.x, dy);
+                    const float alpha = splat_alpha(B.y, pw);
+                    const bool valid = !done && splat_valid(pw, alpha);
                     const float nT = T * (1.f - alpha);
                     const bool term = valid && nT <= T_MIN;
                     const bool accept = valid && !term;
@@ -266,8 +279,8 @@ __device__ inline int stage_forward_batch(const TileData &d, int base, int e, fl
     int n;
     const int pos = kept_position<4>(keep, wcount, n);
     if (keep) {
-        la[pos] = A0;
-        lb[pos] = make_float4(B0.x, B0.y, fmaxf(B0.z + 0.5f, 0.f), fmaxf(B0.w + 0.5f, 0.f));
+        la[pos] = staged_a(A0);
+        lb[pos] = make_float4(B0.x * POW_HALF, B0.y, fmaxf(B0.z + 0.5f, 0.f), fmaxf(B0.w + 0.5f, 0.f));
         lc[pos] = fmaxf(C0 + 0.5f, 0.f);
         lk[pos] = k;
     }
@@ -315,9 +328,9 @@ __device__ inline void forward_chunk(const TileView &v, const TileData &d, const
                 const int j = j0 + u;
                 const float4 A = la[j], B = lb[j];
                 const float dx = A.x - fx, dy = A.y - fy;
-                const float sig = splat_sigma((A.z * dx) * dx, A.w * dx, B.x, dy);
-                const float alpha = fminf(ALPHA_MAX, B.y * __expf(-sig));
-                const bool valid = !done && sig >= 0.f && sig < SIGMA_MAX && alpha >= ALPHA_MIN;
+                const float pw = splat_power((A.z * dx) * dx, A.w * dx, B.x, dy);
+                const float alpha = splat_alpha(B.y, pw);
+                const bool valid = !done && splat_valid(pw, alpha);
                 const float nT = T * (1.f - alpha);
                 if (PRODUCT) {  // once T <= T_MIN every later chunk starts terminated: stop there
                     T = valid ? nT : T;
@@ -388,6 +401,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
 #pragma clang fp contract(fast)
     constexpr int NT = 256 / PPT, NW = NT / 64;
     __shared__ float4 la[BATCH], lb[BATCH];
+    __shared__ float4 lconic[BATCH];  // the record's conic as projected (la and lb hold it scaled)
     __shared__ float lc[BATCH];
     __shared__ float lz[AUX ? BATCH : 1];             // the record's view depth
     __shared__ float partd[AUX ? NW : 1][AUX ? BATCH : 1];  // per-wave sums of fac v_d
@@ -478,8 +492,9 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             int total;
             const int pos = kept + kept_position<NW>(keep, wcount[(c / NT) & 1], total);
             if (keep) {  // colours stored clamped, max(raw + 0.5, 0), as composited
-                la[pos] = A0;
-                lb[pos] = make_float4(B0.x, B0.y, fmaxf(B0.z + 0.5f, 0.f), fmaxf(B0.w + 0.5f, 0.f));
+                la[pos] = staged_a(A0);
+                lb[pos] = make_float4(B0.x * POW_HALF, B0.y, fmaxf(B0.z + 0.5f, 0.f), fmaxf(B0.w + 0.5f, 0.f));
+                lconic[pos] = make_float4(A0.z, A0.w, B0.x, 0.f);
                 lc[pos] = fmaxf(C0 + 0.5f, 0.f);
                 lmask[pos] = (uint8_t)((B0.z + 0.5f >= 0.f ? 1 : 0) | (B0.w + 0.5f >= 0.f ? 2 : 0) |
                                        (C0 + 0.5f >= 0.f ? 4 : 0));
@@ -508,9 +523,9 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
 #pragma unroll
             for (int k = 0; k < PPT; k++) {
                 const float dy = A.y - fy[k];
-                const float sig = splat_sigma(adx2, bdx, B.x, dy);
-                const float alpha = fminf(ALPHA_MAX, B.y * __expf(-sig));
-                const bool valid = kk <= last[k] && sig >= 0.f && sig < SIGMA_MAX && alpha >= ALPHA_MIN;
+                const float pw = splat_power(adx2, bdx, B.x, dy);
+                const float alpha = splat_alpha(B.y, pw);
+                const bool valid = kk <= last[k] && splat_valid(pw, alpha);
                 contrib = contrib || valid;
                 // predicated: an invalid pair leaves T and S unchanged and adds zeros
                 const float ra = __builtin_amdgcn_rcpf(1.f - alpha);
@@ -559,13 +574,13 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
                 for (int r = 1; r < NW; r++) acc += part[r][j][i];
                 q[i] = acc;
             }
-            const float4 A = la[j], B = lb[j];
-            const float ox = A.z * q[1] + A.w * q[2], oy = A.w * q[1] + B.x * q[2];
+            const float4 Q = lconic[j];  // the unscaled conic (a, b, c)
+            const float ox = Q.x * q[1] + Q.y * q[2], oy = Q.y * q[1] + Q.z * q[2];
             const uint32_t cm = lmask[j];
             const bool mr = cm & 1u, mg = cm & 2u, mb = cm & 4u;
             d.slot_a[p] = make_float4(ox, oy, 0.5f * q[3], 0.5f * q[4]);
             d.slot_b[p] = make_float4(0.5f * q[5], mr ? q[6] : 0.f, mg ? q[7] : 0.f, mb ? q[8] : 0.f);
-            d.slot_c[p] = -(1.f - B.y) * q[0];
+            d.slot_c[p] = -(1.f - lb[j].y) * q[0];
             d.slot_f[p] = 1;
             if (AUX) {
                 float acc = partd[0][j];

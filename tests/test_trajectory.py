@@ -19,15 +19,17 @@ KEYS = ("means", "scales", "quats", "dc", "rest", "opac")
 # HIP trainer against the reference's run
 LOSS_REL_TOL = 1e-4        # every step's loss
 PSNR_TOL_DB = 0.05         # final PSNR of every view
-PARAM_REL_TOL = 1e-3       # relative L2 of each group's sampled final parameters (per-tile frames)
 SUM_REL_TOL = 1e-4         # relative difference of each group's sum of squares
-# The 128x128 views have 64 tiles, so by default every frame is depth-chunked
-# (MSPLAT_TUNE_CHUNKS): transmittance products and colour sums are formed per chunk and
-# then combined, a different float rounding from the sequential composite. 300 Adam steps
-# amplify it most in the parameters whose gradients sit near zero (opacity: 1.3e-3
-# relative L2 after 300 steps against 3.3e-4 per tile); every step's loss and the final
-# PSNR keep the bars above.
-PARAM_REL_TOL_CHUNKED = 3e-3
+# Relative L2 of each group's sampled final parameters. 300 Adam steps amplify float
+# rounding most in the parameters whose gradients sit near zero (Adam moves them by about
+# the step size whatever the gradient's size): the opacities drift by 3.3e-4 to 1.4e-3
+# relative L2 from the reference's run depending only on how the per-pixel exponent is
+# rounded (sigma times -log2(e), or the conic scaled by -log2(e) before the pixel loop),
+# and by 1.3e-3 with depth chunks (128x128 views have 64 tiles, so by default every frame
+# is chunked: transmittance products and colour sums formed per chunk, then combined).
+# One step's gradients agree with the CPU's to 1e-6 either way; every step's loss and the
+# final PSNR keep the bars above.
+PARAM_REL_TOL = 3e-3
 
 
 def _targets(orc, gt_scene, cams, d):
@@ -96,5 +98,5 @@ def test_hip_trajectory_matches_reference(ctx, oracle, chunks):
         e = np.linalg.norm(a[d["sample"]] - d["final_" + k]) / np.linalg.norm(d["final_" + k])
         s = abs(float((a.astype(np.float64) ** 2).sum()) - float(d["sumsq_" + k])) / float(d["sumsq_" + k])
         print(f"  {k}: sampled rel L2 {e:.2e}, sum of squares rel {s:.2e}")
-        assert e <= (PARAM_REL_TOL if chunks else PARAM_REL_TOL_CHUNKED), (k, e)
+        assert e <= PARAM_REL_TOL, (k, e)
         assert s <= SUM_REL_TOL, (k, s)
