@@ -456,13 +456,17 @@ def bwd_kernel_name(ppt: int, bb: int) -> str:
     return "raster_backward_kernel<%d, %d, false>" % (ppt, bb)
 
 
-def profiled_counters(variant: dict):
+def profiled_counters(variant: dict, warmup: int, steps: int):
     """Per-launch counters of the raster backward variant `variant` on the timed steps,
     from the committed counter summary (rocprofv3 --pmc runs of the same workload and
-    window, HBM counters scaled by a copy of known size); None when absent."""
+    window, HBM counters scaled by a copy of known size); None when absent or when the
+    summary's step window is not this run's (a launch's work changes over the schedule:
+    the splats shrink as training goes on)."""
     if not os.path.exists(COUNTERS):
         return None
     prof = json.load(open(COUNTERS))
+    if prof.get("window") != {"warmup": warmup, "steps": steps}:
+        return None
     name = bwd_kernel_name(variant.get("bwd_ppt", 4), variant.get("bwd_batch", 128))
     entry = prof.get("kernels", {}).get(name)
     return dict(entry, kernel=name) if entry else None
@@ -861,7 +865,7 @@ def main():
     M, V, N = m_total / K, v_total / K, n_total / K
     hbm_8d = alg_bytes_8d(DOMINANT, M, V, N, P, T, 1)
     hbm_build = alg_bytes_build(DOMINANT, M, V, N, P, T, 1, written / max(dom_calls, 1))
-    prof = profiled_counters(cur)
+    prof = profiled_counters(cur, args.warmup, args.steps)
     valu = prof.get("SQ_INSTS_VALU") if prof else None
     achieved = valu * 64 * 2 / (avg_ms * 1e-3) / 1e12 if valu else None
     roofline = {
@@ -878,6 +882,8 @@ def main():
         "lds_bank_conflict_pct": prof.get("LDSBankConflict_pct") if prof else None,
         "lds_insts_per_launch": prof.get("SQ_INSTS_LDS") if prof else None,
         "counter_source": os.path.relpath(COUNTERS, ROOT) if prof else None,
+        "counter_note": None if prof else "counters are profiled on --warmup 5 --steps 20 (steps 6-25); "
+                                          "other windows report no VALU figures",
         "avg_launch_ms": round(avg_ms, 4),
         "intersections_per_launch": int(M), "visible_per_launch": int(V), "pixels": P,
         "slots_written_per_launch": int(written / max(dom_calls, 1)),
