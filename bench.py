@@ -803,7 +803,7 @@ def main():
     tuning = None
     if not args.no_autotune and not pinned:
         tuning = tune_train_step(ctx, cams, targets)
-        log("autotuned: " + str({k: tuning[k] for k in ("bwd_ppt", "fwd_batch", "bwd_batch", "ms")}))
+        log("autotuned: " + str({k: tuning[k] for k in ("bwd_ppt", "fwd_batch", "bwd_batch", "depth_chunks", "ms")}))
     cfg = TrainConfig(iterations=args.iterations, sh_degree=3, warmup_length=499, cull_alpha_thresh=0.005)
     tr = Trainer(ctx, dg, cams, targets, cfg)
     n_start = dg.n
@@ -953,7 +953,8 @@ def main():
                                                                                       first_timed_step + K - 1],
                    "sh_degree_in_use": dtu_timed, "mean_intersections_per_step": int(M),
                    "densify_events": tr.densify_log, "final_loss": loss, "setup_s": round(setup_s, 1),
-                   "tuning": {k: tuning[k] for k in ("bwd_ppt", "fwd_batch", "bwd_batch")} if tuning else None},
+                   "tuning": {k: tuning[k] for k in ("bwd_ppt", "fwd_batch", "bwd_batch", "depth_chunks")}
+                   if tuning else None},
         "mpix_per_s": round(P * iters / 1e6, 1),
         "sh3_iter_per_s": sh3,
         "step_graph": graph,

@@ -1,10 +1,13 @@
 """Autotuner for the rasterizer's kernel variants (msplat_hip_set_tuning).
 
-The tunables are the backward rasterizer's pixels per thread (the work tile of one lane)
-and the forward and backward LDS batch depths (gaussian records staged per batch).
-Every variant computes the same result (images and gradients bit for bit for the batch
-depths; the pixels-per-thread variants within float rounding of the per-wave sums), so
-the choice only moves time. The tuner times the caller's own step (a train step, a
+The tunables are the backward rasterizer's pixels per thread (the work tile of one lane),
+the forward and backward LDS batch depths (gaussian records staged per batch) and the
+depth chunks per tile (the work unit of one workgroup along a tile's list: 0 = chosen per
+frame, 1 = the whole list, 2 or 4 chunks). The binning tile stays 16x16: it decides which
+gaussians reach a pixel. Every variant computes the same result (images and gradients
+bit for bit for the batch depths; the pixels-per-thread variants within float rounding
+of the per-wave sums, the chunk counts within float rounding of the transmittance
+products), so the choice only moves time. The tuner times the caller's own step (a train step, a
 render, a batched step) on the context's stream: coordinate descent over the tunables,
 each tried at every value with the others at their best so far, a few timed repeats per
 trial after a warm-up, the fastest kept. A result can be cached in a JSON file keyed by
@@ -16,12 +19,13 @@ import json
 import os
 import time
 
-from ._abi import TUNE_BWD_BATCH, TUNE_BWD_PPT, TUNE_FWD_BATCH, Context
+from ._abi import TUNE_BWD_BATCH, TUNE_BWD_PPT, TUNE_CHUNKS, TUNE_FWD_BATCH, Context
 
 SPACE = {
     "bwd_ppt": (TUNE_BWD_PPT, (4, 2, 1)),
     "fwd_batch": (TUNE_FWD_BATCH, (256, 128, 64)),
     "bwd_batch": (TUNE_BWD_BATCH, (128, 64, 32)),
+    "depth_chunks": (TUNE_CHUNKS, (0, 1, 2, 4)),
 }
 
 

@@ -132,7 +132,7 @@ def test_autotuner_picks_a_valid_variant(ctx, tmp_path):
     try:
         cache = str(tmp_path / "tune.json")
         res = autotune.autotune(ctx, lambda: render(ctx, dg, cam, BG), warmup=1, repeats=3, cache=cache, key="k")
-        assert not res["cached"] and len(res["trials"]) == 7
+        assert not res["cached"] and len(res["trials"]) == 10
         for name, (_, values) in autotune.SPACE.items():
             assert res[name] in values and autotune.current(ctx)[name] == res[name]
         again = autotune.autotune(ctx, lambda: render(ctx, dg, cam, BG), cache=cache, key="k")
