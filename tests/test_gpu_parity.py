@@ -79,13 +79,31 @@ def test_binning_bit_exact_on_oracle_inputs(ctx, oracle, ties):
     """Keys, the stable sorts and the tile ranges, fed the oracle's projection. With ties,
     a third of the gaussians repeat another's mean (equal depth bits): the order of equal
     depths is the gaussian id order."""
-    from msplat._abi import FRAME_KEYS64, check, lib, make_camera_struct
-    from msplat.backend import DeviceGaussians
     sc, cam = _scene_cam(seed=5, n=4000, deg=1, W=130, H=90)
     if ties:
         rng = np.random.default_rng(55)
         dst = rng.choice(4000, 1300, replace=False)
         sc["means"][dst] = sc["means"][rng.integers(0, 4000, 1300)]
+    _check_binning(ctx, oracle, sc, cam)
+
+
+def test_binning_bit_exact_with_huge_splats(ctx, oracle):
+    """Key emission across its 2048-key workgroup slices: at 1920x1080 (8160 tiles) a few
+    splats cover most of the image, so their keys span several slices and slices start
+    inside them, among thousands of small splats whose records start anywhere in a
+    thread's eight outputs."""
+    from msplat.camera import look_at, make_camera
+    from msplat.scenes import random_scene
+    sc = random_scene(9, 4000, degree=1, log_scale=(-3.5, 0.5))
+    sc["scales"][:6] = np.float32(0.2)
+    cam = make_camera(look_at((2.5, 0.7, 2.9), (0.1, -0.2, 0)), 1920, 1080, 1500.0, 1500.0, dtu=1)
+    M, counts = _check_binning(ctx, oracle, sc, cam)
+    assert counts.max() > 3 * 2048 and (counts > 2048).sum() >= 3, counts.max()
+
+
+def _check_binning(ctx, oracle, sc, cam):
+    from msplat._abi import FRAME_KEYS64, check, lib, make_camera_struct
+    from msplat.backend import DeviceGaussians
     dg = DeviceGaussians(ctx, sc)
     g, c = dg.struct(), make_camera_struct(cam, 1)
     check(lib().msplat_hip_stage_project(ctx.ptr, C.byref(g), C.byref(c)))
@@ -115,6 +133,7 @@ def test_binning_bit_exact_on_oracle_inputs(ctx, oracle, ties):
     np.testing.assert_array_equal(ts[:-1], bins["tile_start"])
     np.testing.assert_array_equal(ts[1:], bins["tile_end"])
     np.testing.assert_array_equal((keys & np.uint64(0xFFFFFFFF)), bins["keys"] >> np.uint64(32))
+    return M, np.bincount(bins["isect_gid"], minlength=n)  # intersections per gaussian
 
 
 @pytest.mark.parametrize("deg,W,H", [(0, 97, 71), (3, 97, 71), (3, 256, 160)])
