@@ -284,7 +284,7 @@ BatchProjectArgs project_args(BatchState *b, const msplat_hip_gaussians *g) {
         dtu = std::max(dtu, b->h_tab->views[v].cam.degrees_to_use);
         deg = std::max(deg, b->h_tab->views[v].cam.degree);
     }
-    a.sh_u = deg > 3 ? 0 : 3 * (int)sh_bases(dtu);  // degree 4: read in place
+    a.sh_u = deg > 3 || dtu == 0 ? 0 : 3 * (int)sh_bases(dtu);  // degree 4 or the DC alone: read in place
     a.sh_stride = 3 * (size_t)(sh_bases(deg) - 1);
     return a;
 }

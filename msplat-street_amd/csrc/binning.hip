@@ -86,9 +86,10 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
             }
         }
     }
-    // staged up to SH degree 3 (project_sh_lds); degree 4 reads its coefficients in place
+    // staged when higher bands are in use, up to SH degree 3 (project_sh_lds); the DC
+    // alone (three floats a lane) and degree 4 are read in place
     const size_t stride = 3 * (size_t)(sh_bases(cam.degree) - 1);
-    const bool staged = cam.degree <= 3;
+    const bool staged = cam.degree <= 3 && cam.degrees_to_use >= 1;
     const int u = 3 * (int)sh_bases(cam.degrees_to_use), us = u | 1;
     float *wl = shl + (size_t)w * 64 * us;
     if (staged) stage_sh(need, g - lane, a.dc, a.rest, stride, u, us, wl, widx[w]);
@@ -822,9 +823,10 @@ void launch_scan_u32_pair(uint32_t *a0, const ScanOut &o0, uint32_t *a1, const S
 }
 
 // LDS of project_kernel's SH staging (four waves of 64 lanes, an odd count of floats per
-// lane) for the bands up to `degree`; none when the stored degree is 4 (read in place)
+// lane) for the bands up to `degree`; none when the stored degree is 4 or only the DC is
+// in use (read in place)
 size_t project_sh_lds(uint32_t degree, uint32_t stored) {
-    return stored > 3 ? 0 : 4 * 64 * (size_t)((3 * sh_bases(degree)) | 1) * sizeof(float);
+    return stored > 3 || degree == 0 ? 0 : 4 * 64 * (size_t)((3 * sh_bases(degree)) | 1) * sizeof(float);
 }
 
 void launch_project(const ProjectArgs &a, hipStream_t s) {

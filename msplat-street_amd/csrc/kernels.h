@@ -324,7 +324,8 @@ struct BatchProjectArgs {
     uint2 *rect;
     int32_t *radii;
     uint32_t *count, *block_sums, *block_vis;  // [v * nbx + b]
-    int sh_u;          // SH floats staged per gaussian: 3 sh_bases(largest degree in use of the views); 0 at degree 4
+    int sh_u;          // SH floats staged per gaussian: 3 sh_bases(largest degree in use of the views);
+                       // 0 (read in place) at degree 4 or with only the DC in use
     size_t sh_stride;  // rest floats per gaussian
 };
 
