@@ -238,16 +238,27 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
 // (grid-stride over the device-side list; every wave exits when the list is done).
 // Lane l adds the written slots among l, l + 64, ... in order, in rounds of four slots
 // whose flag and value loads are in flight together; the nine lane sums are added over
-// the wave by wave_sums9. The next record's id is loaded while this one is summed.
+// the wave by wave_sums9. The next record's id and slot range are loaded while this one
+// is summed.
 __global__ __launch_bounds__(256) void big_reduce_kernel(BackwardArgs a) {
     const uint32_t nbig = *a.big_count;
     const int lane = threadIdx.x & 63;
     const SumLanes sl(lane);
     const uint32_t wave = (blockIdx.x * 256 + threadIdx.x) >> 6, nwaves = (gridDim.x * 256) >> 6;
     uint32_t g = wave < nbig ? a.big_list[wave] : 0u;
+    uint32_t o = 0, end = 0;
+    if (wave < nbig) {
+        o = a.offsets[g];
+        end = o + a.count[g];
+    }
     for (uint32_t i = wave; i < nbig; i += nwaves) {
-        const uint32_t o = a.offsets[g], end = o + a.count[g];
+        // the next record's id and slot range, loaded while this one is summed
         const uint32_t gn = i + nwaves < nbig ? a.big_list[i + nwaves] : 0u;
+        uint32_t on = 0, cn = 0;
+        if (i + nwaves < nbig) {
+            on = a.offsets[gn];
+            cn = a.count[gn];
+        }
         float v[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, vd = 0.f;
         for (uint32_t p0 = o + lane; p0 < end; p0 += 256) {
             bool f[4];
@@ -287,6 +298,8 @@ __global__ __launch_bounds__(256) void big_reduce_kernel(BackwardArgs a) {
             *dst = t;
         }
         g = gn;
+        o = on;
+        end = on + cn;
     }
 }
 
