@@ -397,11 +397,14 @@ def tune_train_step(ctx, cams, targets, seed=5):
     """Picks the rasterizer variants (msplat.autotune) on the headline train step, run on
     a scratch copy of the scene (the same street recipe) so the trainer's schedule is not
     advanced; the choice stays set on the context."""
+    import ctypes as C
     from msplat import autotune
-    from msplat.backend import DeviceGaussians, adam_coeffs, train_step
+    from msplat._abi import check, lib
+    from msplat.backend import KEYS, DeviceGaussians, adam_coeffs, train_step
     from msplat.scenes import street_points
     pts, rgb = street_points(seed, N_GAUSSIANS)
     scratch = DeviceGaussians.from_points(ctx, pts, rgb, degree=3)
+    start = {k: ctx.download(scratch.params[k].ptr, (scratch.params[k].nbytes,), np.uint8) for k in KEYS}
     lrs = [1.6e-4, 5e-3, 1e-3, 2.5e-3, 1.25e-4, 5e-2]
     k = [0]
 
@@ -411,7 +414,18 @@ def tune_train_step(ctx, cams, targets, seed=5):
         i = k[0] % len(cams)
         train_step(ctx, scratch, dict(cams[i], dtu=3), BG, targets[i], s_, b_, read_loss=False)
 
-    res = autotune.autotune(ctx, step, warmup=3, repeats=12)
+    def reset():  # every trial trains the same first steps of the same scene
+        for key in KEYS:
+            ctx.upload(scratch.params[key].ptr, start[key])
+            for m in (scratch.m1[key], scratch.m2[key]):
+                check(lib().msplat_hip_memset(ctx.ptr, C.c_void_p(m.ptr), 0, m.nbytes))
+        scratch.reset_stats()
+        k[0] = 0
+
+    # two rounds: the best pixels per lane depends on the batch depth picked after it
+    res = autotune.autotune(ctx, step, warmup=3, repeats=24, rounds=2, reset=reset)
+    log("tuning trials (ms per step): " + "; ".join(
+        "%s %.4f" % ("/".join(str(c[n]) for n in autotune.SPACE), ms) for c, ms in res["trials"]))
     del scratch
     return res
 
@@ -467,7 +481,7 @@ def profiled_counters(variant: dict, warmup: int, steps: int):
     prof = json.load(open(COUNTERS))
     if prof.get("window") != {"warmup": warmup, "steps": steps}:
         return None
-    name = bwd_kernel_name(variant.get("bwd_ppt", 4), variant.get("bwd_batch", 128))
+    name = bwd_kernel_name(variant.get("bwd_ppt", 4), variant.get("bwd_batch", 64))
     entry = prof.get("kernels", {}).get(name)
     return dict(entry, kernel=name) if entry else None
 

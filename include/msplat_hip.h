@@ -232,7 +232,7 @@ int msplat_hip_set_flags(msplat_hip_ctx *ctx, int flags);
  *   MSPLAT_TUNE_BWD_PPT    backward rasterizer pixels per thread: 1, 2, 4 (default 4)
  *   MSPLAT_TUNE_FWD_BATCH  forward LDS batch depth (gaussian records staged per batch):
  *                          64, 128, 256 (default 256)
- *   MSPLAT_TUNE_BWD_BATCH  backward LDS batch depth: 32, 64, 128 (default 128)
+ *   MSPLAT_TUNE_BWD_BATCH  backward LDS batch depth: 32, 64, 128 (default 64)
  *   MSPLAT_TUNE_CULL       1 (default): the rasterizers skip a tile's gaussians whose
  *                          alpha >= 1/255 ellipse (widened for float rounding) misses
  *                          the tile; 0: every listed gaussian is visited. Skipped pairs

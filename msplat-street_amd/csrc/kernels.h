@@ -16,7 +16,7 @@ struct Tuning {
     int chunks = 0;       // depth chunks per tile: 0 = chosen per frame, 1 = off, 2..64 = fixed
     int bwd_ppt = 4;      // backward pixels per thread: 1, 2, 4
     int fwd_batch = 256;  // forward LDS batch depth: 64, 128, 256
-    int bwd_batch = 128;  // backward LDS batch depth: 32, 64, 128
+    int bwd_batch = 64;   // backward LDS batch depth: 32, 64, 128
     int cull = 1;         // skip a tile's gaussians whose alpha >= 1/255 ellipse misses it
     int pack_keys = 1;    // one-view tile sort on (tile | id) keys when they fit in 32 bits
 };

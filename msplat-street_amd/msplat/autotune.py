@@ -24,7 +24,7 @@ from ._abi import TUNE_BWD_BATCH, TUNE_BWD_PPT, TUNE_CHUNKS, TUNE_FWD_BATCH, Con
 SPACE = {
     "bwd_ppt": (TUNE_BWD_PPT, (4, 2, 1)),
     "fwd_batch": (TUNE_FWD_BATCH, (256, 128, 64)),
-    "bwd_batch": (TUNE_BWD_BATCH, (128, 64, 32)),
+    "bwd_batch": (TUNE_BWD_BATCH, (64, 128, 32)),
     "depth_chunks": (TUNE_CHUNKS, (0, 1, 2, 4)),
 }
 
@@ -39,7 +39,9 @@ def apply(ctx: Context, choice: dict):
             ctx.set_tuning(SPACE[name][0], int(value))
 
 
-def _time(ctx: Context, step, warmup: int, repeats: int) -> float:
+def _time(ctx: Context, step, warmup: int, repeats: int, reset=None) -> float:
+    if reset is not None:
+        reset()
     for _ in range(warmup):
         step()
     ctx.sync()
@@ -51,11 +53,15 @@ def _time(ctx: Context, step, warmup: int, repeats: int) -> float:
 
 
 def autotune(ctx: Context, step, warmup: int = 2, repeats: int = 8, space: dict | None = None,
-             cache: str | None = None, key: str | None = None, rounds: int = 1, min_gain: float = 0.02) -> dict:
+             cache: str | None = None, key: str | None = None, rounds: int = 1, min_gain: float = 0.02,
+             reset=None) -> dict:
     """Picks the fastest variant for `step()` (a callable that enqueues one unit of the
     workload on ctx) and leaves it set on ctx. A variant replaces the best so far only
     when it is faster by more than `min_gain` (a fraction), so timing noise does not flip
-    the choice. Returns {tunable: value, ..., "ms": best ms per step, "trials":
+    the choice. `reset()`, when given, runs before every trial: a step that changes its
+    own input (a train step moves the parameters, and a scene early in training gets
+    cheaper step by step) must start each trial from the same state, or the variants
+    tried last look fastest. Returns {tunable: value, ..., "ms": best ms per step, "trials":
     [(choice, ms), ...], "cached": bool}."""
     space = space or SPACE
     if cache and key:
@@ -68,7 +74,10 @@ def autotune(ctx: Context, step, warmup: int = 2, repeats: int = 8, space: dict 
             apply(ctx, hit)
             return dict(hit, cached=True)
     best = current(ctx)
-    best_ms = _time(ctx, step, warmup, repeats)
+    # one discarded trial first: the first steps on a context also grow its buffers and
+    # load the kernels, and timing them would make every later variant look faster
+    _time(ctx, step, warmup, repeats, reset)
+    best_ms = _time(ctx, step, warmup, repeats, reset)
     trials = [(dict(best), best_ms)]
     for _ in range(rounds):
         for name, (tkey, values) in space.items():
@@ -77,7 +86,7 @@ def autotune(ctx: Context, step, warmup: int = 2, repeats: int = 8, space: dict 
                     continue
                 trial = dict(best, **{name: v})
                 apply(ctx, trial)
-                ms = _time(ctx, step, warmup, repeats)
+                ms = _time(ctx, step, warmup, repeats, reset)
                 trials.append((trial, ms))
                 if ms < best_ms * (1.0 - min_gain):
                     best, best_ms = trial, ms
