@@ -318,7 +318,51 @@ SERVE_SCENES = (500_000, 1_000_000, 2_000_000, 3_000_000)
 SERVE_RATES = (200, 500, 1000, 2000)
 
 
-def bench_serving(rates=SERVE_RATES, duration=1.0, slots=6, max_views=16, seed=20):
+def serve_runtime_scene(srv, ctx, path, centers, seed, trained_steps=None, rate=1000.0, duration=0.6,
+                        evict_frac=0.6):
+    """A trained scene loaded from its PLY file is admitted into the running server beside
+    the resident scenes (Server.add_scene), served in the mix of a Poisson burst, and
+    evicted part-way through it (Server.remove_scene, from another thread): its later
+    requests are refused and its memory returns to the slots."""
+    import threading
+    from msplat.scene_io import load_ply, ply_info
+    from msplat.serving import latency_stats, poisson_requests
+    t0 = time.perf_counter()
+    info = ply_info(path)
+    scene, _ = load_ply(ctx, path, capacity=info["num_points"])
+    ctx.sync()
+    load_s = time.perf_counter() - t0
+    center = tuple(float(v) for v in scene.params["means"].numpy()[:scene.n].mean(axis=0))
+    before, nbytes = srv.resident, scene.nbytes
+    idx = srv.add_scene(scene)
+    out = {"file": "PLY written from the benchmark's trained street scene", "gaussians": scene.n,
+           "trained_steps": trained_steps, "bytes": nbytes, "load_s": round(load_s, 2), "admitted": idx is not None}
+    if idx is None:
+        scene.free()
+        return out
+    reqs = poisson_requests(rate, duration, list(centers) + [center], seed=seed)
+    for r in reqs:
+        if r.scene == len(centers):
+            r.scene = idx
+    timer = threading.Timer(evict_frac * duration, srv.remove_scene, args=(idx,))
+    timer.start()
+    done = srv.run(reqs)
+    timer.join()
+    mine = [r for r in done if r.scene == idx]
+    refused = [r for r in srv.rejected if r.scene == idx]
+    stats = latency_stats(mine)
+    out.update({"index": idx, "resident_gb": [round(before / 1e9, 2), round((before + nbytes) / 1e9, 2),
+                                              round(srv.resident / 1e9, 2)],
+                "rate": rate, "duration_s": duration, "evicted_at_s": round(evict_frac * duration, 2),
+                "requests": sum(r.scene == idx for r in reqs), "served": len(mine),
+                "refused_after_eviction": len(refused), "p50_ms": stats.get("p50_ms"), "p99_ms": stats.get("p99_ms"),
+                "others": {k: latency_stats([r for r in done if r.scene != idx]).get(k)
+                           for k in ("completed", "p50_ms", "p99_ms")}})
+    return out
+
+
+def bench_serving(rates=SERVE_RATES, duration=1.0, slots=6, max_views=16, seed=20, trained_ply=None,
+                  trained_steps=None):
     """Serving: four resident SH3 scenes (0.5M, 1M, 2M, 3M Gaussians, 64-cluster clouds in
     an 8 m box, kNN scales, opacity logits N(1.5, 1), higher SH bands N(0, 0.05)).
     Poisson arrivals at each rate for `duration` s; pose on a sphere of radius 3-10 m
@@ -369,6 +413,9 @@ def bench_serving(rates=SERVE_RATES, duration=1.0, slots=6, max_views=16, seed=2
         }
         log(f"serving {rate} req/s: batched {b['views_per_s']} views/s p50 {b['p50_ms']} p99 {b['p99_ms']} ms; "
             f"plain {plain['views_per_s']} views/s p50 {plain['p50_ms']} p99 {plain['p99_ms']} ms")
+    if trained_ply:
+        out["runtime_scene"] = serve_runtime_scene(srv, ctx, trained_ply, centers, seed + 7, trained_steps)
+        log("runtime scene: " + json.dumps(out["runtime_scene"]))
     srv.close()
     # admission under a tight budget at the top rate
     probe = Server(scenes, slots=1, background=BG)
@@ -942,8 +989,17 @@ def main():
         log(f"batched: {batched['train_views_per_s']} train views/s, {batched['render_views_per_s']} render views/s")
     serving = None
     if not args.no_serving:
-        log("serving: 4 resident scenes, Poisson arrivals")
-        serving = bench_serving(duration=args.serving_seconds)
+        import shutil
+        import tempfile
+        from msplat.scene_io import save_ply
+        tmpdir = tempfile.mkdtemp(prefix="msplat_bench_")
+        try:
+            ply = os.path.join(tmpdir, "street_trained.ply")
+            save_ply(ctx, dg, ply, step=tr.step_count)
+            log("serving: 4 resident scenes, Poisson arrivals; then the trained street scene from its PLY file")
+            serving = bench_serving(duration=args.serving_seconds, trained_ply=ply, trained_steps=tr.step_count)
+        finally:
+            shutil.rmtree(tmpdir, ignore_errors=True)
     plain = None
     if not args.no_plain:
         log("plain workload: 20k gaussians, 8 views at 128x128, 300 steps")

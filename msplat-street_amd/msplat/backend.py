@@ -47,6 +47,21 @@ class DeviceGaussians:
         self.radii = DeviceArray(ctx, (self.capacity,), np.int32)
         self.step_count = 0
 
+    def arrays(self) -> list:
+        """Every device array the scene owns (parameters, moments, statistics, radii)."""
+        return ([self.params[k] for k in KEYS] + [self.m1[k] for k in KEYS] + [self.m2[k] for k in KEYS]
+                + [self.vis, self.gn, self.ms, self.radii])
+
+    @property
+    def nbytes(self) -> int:
+        """Device memory the scene holds."""
+        return sum(a.nbytes for a in self.arrays() if a.ptr)
+
+    def free(self):
+        """Frees the scene's device memory now (after the work that reads it has completed)."""
+        for a in self.arrays():
+            a.free()
+
     def struct(self) -> Gaussians:
         g = Gaussians()
         g.num_points = self.n

@@ -15,10 +15,13 @@ request of a batch completes when the batch does) and form only when requests qu
 pack (msplat_pack_rgba8, core/cpu/msplat_cpu.cpp:1723-1734). The plain form renders one
 request per msplat_hip_render, first come first served, on one context.
 
-Admission: the resident scenes' parameters plus every slot's working set stay within the
-HBM budget; a batch's working set is estimated from its record count (views x
-gaussians), pixels and intersections (pixels x the scene's measured intersections per
-pixel).
+Admission: the resident scenes plus every slot's working set stay within the HBM budget;
+a batch's working set is estimated from its record count (views x gaussians), pixels and
+intersections (pixels x the scene's measured intersections per pixel). Scenes are
+admitted and evicted while the server runs (add_scene, remove_scene): a scene is admitted
+when, with it resident, every slot's share still holds the smallest view of every
+resident scene, and an evicted scene's pending requests are refused and its memory freed
+once no slot renders it.
 """
 from __future__ import annotations
 
@@ -36,6 +39,7 @@ REC_BYTES = 116    # per record (view x gaussian) of a batched frame
 PIX_BYTES = 40     # per pixel: image, final T and index, gradient image, loss, RGBA8
 ISECT_BYTES = 52   # per intersection: keys, ids (double-buffered), gradient slots
 PARAM_BYTES = 236  # per resident SH3 gaussian (59 floats)
+MIN_VIEW = (640, 360)  # the smallest view every slot must keep room for
 
 
 @dataclasses.dataclass
@@ -98,16 +102,21 @@ class Server:
         self.max_batch_pixels = float(max_batch_pixels)
         self.bg = tuple(float(v) for v in background)
         self.keep_images = keep_images
-        self.resident = sum(s.n * PARAM_BYTES for s in scenes)
+        self.scenes = list(scenes)  # None where a scene was evicted (indices stay)
+        self.resident = sum(s.nbytes for s in self.scenes)
         if hbm_budget is None:  # the resident scenes plus 90% of what the device has free
             from ._abi import device_memory
             hbm_budget = self.resident + 0.9 * device_memory(device)[0]
         self.budget = float(hbm_budget)
         if self.resident >= self.budget:
             raise ValueError("the resident scenes alone exceed the HBM budget")
+        self.nslots = slots
         self.slots = [_Slot(i, device) for i in range(slots)]
         self.share = (self.budget - self.resident) / slots
-        self.ipp = [float(isects_per_pixel)] * len(scenes)  # intersections per pixel
+        self.ipp0 = float(isects_per_pixel)
+        self.ipp = [self.ipp0] * len(self.scenes)  # intersections per pixel
+        self.inflight = [0] * len(self.scenes)  # batches of each scene being rendered
+        self.evicting: set = set()
         self.adapt = adapt  # refine ipp from each rendered frame
         self.pending: list[Request] = []
         self.finished: list[Request] = []
@@ -120,22 +129,69 @@ class Server:
 
     # ── admission ────────────────────────────────────────────────────────────
     def working_set(self, reqs) -> float:
-        s = self.scenes[reqs[0].scene]
-        px = sum(r.pixels for r in reqs)
-        return len(reqs) * s.n * REC_BYTES + px * PIX_BYTES + px * self.ipp[reqs[0].scene] * ISECT_BYTES
+        return self._working_set(reqs[0].scene, len(reqs), sum(r.pixels for r in reqs))
+
+    def _working_set(self, scene: int, views: int, px: int) -> float:
+        return views * self.scenes[scene].n * REC_BYTES + px * PIX_BYTES + px * self.ipp[scene] * ISECT_BYTES
+
+    def _live(self, scene: int) -> bool:
+        return 0 <= scene < len(self.scenes) and self.scenes[scene] is not None and scene not in self.evicting
+
+    def add_scene(self, scene) -> int | None:
+        """Admits a scene (DeviceGaussians on this device) while the server runs. It is
+        refused (None) when, with it resident, a slot's share of the budget would not hold
+        the smallest view (MIN_VIEW) of every resident scene; else its index is returned
+        and requests for it are served from then on."""
+        need = scene.nbytes
+        px = MIN_VIEW[0] * MIN_VIEW[1]
+        with self.cond:
+            share = (self.budget - self.resident - need) / self.nslots
+            self.scenes.append(scene)
+            self.ipp.append(self.ipp0)
+            self.inflight.append(0)
+            idx = len(self.scenes) - 1
+            fits = all(share >= self._working_set(i, 1, px) for i in range(len(self.scenes)) if self._live(i))
+            if not fits:
+                self.scenes[idx] = None
+                return None
+            self.resident += need
+            self.share = share
+            return idx
+
+    def remove_scene(self, index: int, free: bool = True):
+        """Evicts a scene: its pending requests are refused now, and so are later ones; the
+        call returns once no slot renders it, having freed its device memory (free=True)
+        and returned it to the slots' shares."""
+        with self.cond:
+            if not self._live(index):
+                raise ValueError(f"scene {index} is not resident")
+            self.evicting.add(index)
+            keep = []
+            for r in self.pending:
+                (self.rejected if r.scene == index else keep).append(r)
+            self.pending = keep
+            while self.inflight[index] > 0:
+                self.cond.wait()
+            scene, self.scenes[index] = self.scenes[index], None
+            self.evicting.discard(index)
+            self.resident -= scene.nbytes
+            self.share = (self.budget - self.resident) / self.nslots
+        if free:
+            scene.free()
 
     def _take_batch(self) -> list:
         """The oldest pending request; when more than batch_depth are pending, also the
         later ones of its scene, in arrival order, up to max_views, max_batch_pixels and
         the slot's share of the budget."""
         head = self.pending[0]
-        if self.working_set([head]) > self.share:
+        if not self._live(head.scene) or self.working_set([head]) > self.share:
             self.pending.pop(0)
             self.rejected.append(head)
             return []
         batch = [head]
         if len(self.pending) <= self.batch_depth:
             self.pending.pop(0)
+            self.inflight[head.scene] += 1
             return batch
         pixels = head.pixels
         for r in self.pending[1:]:
@@ -147,6 +203,7 @@ class Server:
                 pixels += r.pixels
         taken = set(id(r) for r in batch)
         self.pending = [r for r in self.pending if id(r) not in taken]
+        self.inflight[head.scene] += 1
         return batch
 
     # ── workers ──────────────────────────────────────────────────────────────
@@ -169,6 +226,10 @@ class Server:
                 with self.cond:
                     self.errors.append(f"slot {slot.index}: {e}")
                     self.rejected.extend(batch)
+            finally:
+                with self.cond:  # an eviction waits for the scene's batches in flight
+                    self.inflight[batch[0].scene] -= 1
+                    self.cond.notify_all()
 
     def _render(self, slot: _Slot, batch: list):
         scene = self.scenes[batch[0].scene]

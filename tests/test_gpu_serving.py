@@ -51,10 +51,10 @@ def test_results_are_the_same_alone_and_in_the_mix(ctx):
 
 
 def test_admission_keeps_batches_within_the_budget(ctx):
-    from msplat.serving import PARAM_BYTES, Request, Server, poisson_requests
+    from msplat.serving import Request, Server, poisson_requests
     scenes, centers = _scenes(ctx)
     reqs = poisson_requests(4000.0, 0.006, centers, seed=5, radius=(3.0, 6.0), resolutions=[((96, 96), 1.0)])
-    resident = sum(s.n * PARAM_BYTES for s in scenes)
+    resident = sum(s.nbytes for s in scenes)
     probe = Server(scenes, slots=1, background=BG)
     one = max(probe.working_set([r]) for r in reqs)
     probe.close()
@@ -69,3 +69,63 @@ def test_admission_keeps_batches_within_the_budget(ctx):
         srv.close()
     assert [r.rid for r in srv.rejected] == [big.rid]
     assert len(done) == len(reqs) and all(r.batch_size == 1 for r in done)
+
+
+def test_scenes_admitted_and_evicted_while_serving(ctx, tmp_path):
+    """A scene written to a PLY file and loaded back is admitted into a running server
+    (add_scene), served beside the resident one with the images it gives alone, evicted
+    from another thread part-way through a burst (remove_scene: its later requests are
+    refused, the other scene's served), and a scene too large for the budget is refused."""
+    import threading
+    import time
+    from msplat.scene_io import load_ply, save_ply
+    from msplat.serving import MIN_VIEW, Server, poisson_requests, serve_plain
+    scenes, centers = _scenes(ctx)
+    path = tmp_path / "scene1.ply"
+    save_ply(ctx, scenes[1], path)
+    loaded, _ = load_ply(ctx, path, capacity=scenes[1].n)
+    srv = Server(scenes[:1], slots=2, max_views=8, background=BG, keep_images=True)
+    try:
+        idx = srv.add_scene(loaded)
+        assert idx == 1 and srv.resident == scenes[0].nbytes + loaded.nbytes
+        reqs = poisson_requests(2000.0, 0.3, centers, seed=8, radius=(3.0, 6.0), resolutions=SMALL)
+        evicted = []
+
+        def evict():
+            srv.remove_scene(idx)
+            evicted.append(srv._now())
+
+        timer = threading.Timer(0.15, evict)
+        timer.start()
+        done = srv.run(reqs)
+        timer.join()
+        t_evict = evicted[0]
+        assert srv.scenes[idx] is None and srv.resident == scenes[0].nbytes
+        mine = [r for r in done if r.scene == idx]
+        refused = [r for r in srv.rejected if r.scene == idx]
+        assert mine and refused and not [r for r in srv.rejected if r.scene == 0]
+        assert len(mine) + len(refused) == sum(r.scene == idx for r in reqs)
+        assert all(r.arrival >= t_evict - 0.05 for r in refused)  # refused only around and after the eviction
+        alone = serve_plain([scenes[0], scenes[1]], [type(r)(r.rid, r.scene, r.cam, 0.0) for r in done],
+                            background=BG, keep_images=True, depth_chunks=1)
+        by_id = {r.rid: r for r in alone}
+        for r in done:
+            np.testing.assert_array_equal(r.image, by_id[r.rid].image, err_msg=f"request {r.rid}")
+        with pytest.raises(ValueError):
+            srv.remove_scene(idx)
+    finally:
+        srv.close()
+    # a budget one byte short of the resident scenes plus each slot's smallest view
+    small = Server(scenes[:1], slots=1, background=BG)
+    need = small._working_set(0, 1, MIN_VIEW[0] * MIN_VIEW[1])
+    small.close()
+    tight = Server(scenes[:1], slots=2, background=BG, hbm_budget=scenes[0].nbytes + 2 * need + scenes[1].nbytes - 1)
+    try:
+        assert tight.add_scene(scenes[1]) is None and tight.resident == scenes[0].nbytes
+    finally:
+        tight.close()
+    roomy = Server(scenes[:1], slots=2, background=BG, hbm_budget=scenes[0].nbytes + 4 * need + scenes[1].nbytes)
+    try:
+        assert roomy.add_scene(scenes[1]) is not None
+    finally:
+        roomy.close()
