@@ -7,8 +7,8 @@
 // Every function names the reference function it restates (file:line). The float
 // instantiation keeps the reference's floating-point operation order (left-to-right
 // sums, the same products), so that compiled with -ffp-contract=off it reproduces the
-// reference bit for bit; the double instantiation is the FP64 renderer used to judge
-// the FP32 outputs. The whole file is built with -ffp-contract=off.
+// reference bit for bit; the double instantiation is the FP64 renderer against which
+// the FP32 outputs' errors are measured. The whole file is built with -ffp-contract=off.
 //
 // Pinning (what checks each function without trusting this file):
 //   project/bin/raster/loss/backward/adam/stats/densify/knn/init/metrics/pack: the
