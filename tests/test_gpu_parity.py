@@ -45,11 +45,15 @@ def _frame_arrays(ctx, n):
                 radii=ctx.download(f.radii, (n,), np.int32), f=f)
 
 
-@pytest.mark.parametrize("deg", [0, 1, 3])
-def test_projection_bit_exact(ctx, oracle, deg):
+@pytest.mark.parametrize("deg,dtu", [(0, 0), (1, 1), (2, 2), (3, 3), (3, 1), (4, 4), (4, 2)])
+def test_projection_bit_exact(ctx, oracle, deg, dtu):
+    """Projection and SH colour against the oracle, for every stored degree and fewer bands
+    in use than stored (the colour reads the bands in use of each gaussian's stored ones:
+    staged in LDS up to degree 3, read in place at degree 4)."""
     from msplat._abi import check, lib, make_camera_struct
     from msplat.backend import DeviceGaussians
     sc, cam = _scene_cam(seed=10 + deg, deg=deg)
+    cam["dtu"] = dtu
     dg = DeviceGaussians(ctx, sc)
     g, c = dg.struct(), make_camera_struct(cam, deg)
     check(lib().msplat_hip_stage_project(ctx.ptr, C.byref(g), C.byref(c)))
