@@ -802,6 +802,184 @@ def time_sh3_steps(ctx, dg, cams, targets, steps=100, warmup=10):
     return steps / (time.perf_counter() - t0)
 
 
+STREET_PRIORS = dict(depth_weight=0.2, depth_weight_final=0.05, sky_alpha_weight=0.05, fill_weight=0.01,
+                     use_masks=True, learn_sky=True, exposure=True, max_scale_ratio=10.0)
+
+
+def street_camera_priors(cams, seed=30):
+    """Per view, a depth prior and an aux image (confidence, sky mask, keep mask) at half
+    the view's resolution, shaped like a forward-looking street capture: sky above the
+    horizon row the camera's pitch puts it on (no depth there), a ground plane 1.6 m
+    below the camera below it (depth = 1.6 f / rows below the horizon, at most 80 m), 2%
+    of the depth samples invalid, and the bottom 6% of rows (the vehicle's hood) not kept."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in cams:
+        pw, ph = c["W"] // 2, c["H"] // 2
+        fy, horizon = 0.5 * c["fy"], 0.5 * (c["cy"] - 0.1 * c["fy"])
+        yy = np.mgrid[0:ph, 0:pw][0].astype(np.float32)
+        below = yy - np.float32(horizon)
+        depth = np.where(below > 0.5, np.minimum(1.6 * fy / np.maximum(below, 0.5), 80.0), 0.0).astype(np.float32)
+        depth[rng.random((ph, pw)) < 0.02] = 0.0
+        aux = np.zeros((ph, pw, 4), np.uint8)
+        aux[..., 0] = rng.integers(120, 256, (ph, pw))
+        aux[..., 1] = np.where(below < -4.0, 255, 0)
+        aux[..., 2] = np.where(yy < 0.94 * ph, 255, 0)
+        out.append(dict(depth=depth, aux=aux, has_depth=True, has_sky=True, has_mask=True))
+    return out
+
+
+def time_prior_steps(ctx, dg, cams, targets, steps=100, warmup=10):
+    """Prior-guided train steps with the street preset (python/msplat/__init__.py:61-72:
+    depth prior 0.2 -> 0.05, sky alpha 0.05, fill 0.01, keep masks, learned 512x128 sky,
+    per-view exposure, scale cap 10; SH degree 1) on the current scene and views, with
+    street_camera_priors; and, beside them, plain steps at SH degree 1 on the same scene
+    for the prior path's cost. Returns the rates and what the CPU leg needs."""
+    from msplat.trainer import PriorOptions, TrainConfig, Trainer
+    cps = street_camera_priors(cams)
+    rates = {}
+    for name, priors in (("plain_sh1", None), ("prior", PriorOptions(**STREET_PRIORS))):
+        cfg = TrainConfig(iterations=steps + warmup, sh_degree=1, sh_degree_interval=1, densify=False)
+        tr = Trainer(ctx, dg, cams, targets, cfg, priors=priors, camera_priors=cps if priors else None)
+        for _ in range(warmup):
+            tr.step()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            tr.step()
+        ctx.sync()
+        rates[name] = steps / (time.perf_counter() - t0)
+    sky = tr.prior_state.numpy("sky_tex")
+    return {"iter_per_s": round(rates["prior"], 2), "plain_sh1_iter_per_s": round(rates["plain_sh1"], 2),
+            "prior_cost_pct": round(100.0 * (rates["plain_sh1"] / rates["prior"] - 1.0), 1),
+            "workload": f"street preset on the timed scene ({dg.n:,} Gaussians at the end), one {W_IMG}x{H_IMG} "
+                        f"view per step, priors at {W_IMG // 2}x{H_IMG // 2} (synthetic ground-plane depth, sky "
+                        f"mask above the horizon, hood not kept), {steps} steps after {warmup}, no densification",
+            "_cpu": dict(sky=sky, sky_frame=tr.sky_frame, prior=cps[0], n_cams=len(cams))}
+
+
+def cpu_prior_sample(dg, cams, targets, leg, seconds=10.0):
+    """The reference's CPU prior-guided train step (ref_train_step_prior, the compiled
+    core/cpu/msplat_cpu.cpp) on the same scene, view 0 and its priors, the sky texture the
+    GPU leg ended with and the preset's weights at the first step; bounded to `seconds`
+    or 5 steps. None when the reference was not built."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Reference, host_isa, reference_available
+    isa = host_isa()
+    if not reference_available(perf=True, isa=isa):
+        return None
+    threads = len(os.sched_getaffinity(0))
+    os.environ["MSPLAT_THREADS"] = str(threads)
+    impl = Reference(perf=True, isa=isa)
+    full = dg.to_numpy()
+    keys = ("means", "scales", "quats", "dc", "rest", "opac")
+    st = {k: full[k].copy() for k in keys}
+    for k in keys:
+        st["m1_" + k] = np.zeros_like(full[k])
+        st["m2_" + k] = np.zeros_like(full[k])
+    n = full["means"].shape[0]
+    st.update(degree=dg.degree, vis=np.zeros(n, np.float32), gn=np.zeros(n, np.float32), ms=np.zeros(n, np.float32))
+    c, o = leg["_cpu"], STREET_PRIORS
+    sky = np.ascontiguousarray(c["sky"], np.float32)
+    ident = np.zeros((c["n_cams"], 12), np.float32)
+    ident[:, [0, 4, 8]] = 1.0
+    bc2 = float(np.sqrt(np.float32(1 - 0.999)))
+    prior = dict(aux=True, sky=True, sky_tex=sky, sky_grad=np.zeros_like(sky), sky_m1=np.zeros_like(sky),
+                 sky_m2=np.zeros_like(sky), sky_frame=tuple(float(v) for v in c["sky_frame"]),
+                 sky_step=0.01 / 0.1, sky_bc2=bc2, prior_depth=c["prior"]["depth"], prior_aux=c["prior"]["aux"],
+                 has_depth=True, has_sky_mask=True, depth_weight=o["depth_weight"], sky_weight=o["sky_alpha_weight"],
+                 fill_weight=o["fill_weight"], huber_delta=0.05, min_alpha=0.25, mask_photometric=True, exposure=True,
+                 cam_index=0, expo_params=ident, expo_m1=np.zeros_like(ident), expo_m2=np.zeros_like(ident),
+                 expo_grad=np.zeros(12, np.float32), expo_step=5e-3 / 0.1, expo_bc2=bc2, expo_reg=1e-2,
+                 log_max_scale_ratio=float(np.log(np.float32(o["max_scale_ratio"]))), loss_terms=np.zeros(4, np.float32))
+    lrs = np.array([1.6e-4, 5e-3, 1e-3, 2.5e-3, 1.25e-4, 5e-2], np.float32)
+    steps, bc2s = lrs / np.float32(0.1), np.full(6, np.sqrt(np.float32(0.001)), np.float32)
+    times, t_start = [], time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        impl.train_step_prior(st, dict(cams[0], dtu=1), np.array(BG, np.float32), targets[0].numpy(), steps, bc2s,
+                              prior)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start > seconds or len(times) >= 5:
+            break
+    per = float(np.median(times))
+    return {"value": round(1.0 / per, 4), "unit": "iter/s", "cores": threads, "kind": "reference",
+            "build": f"-O3 -march=x86-64-{isa}",
+            "sample": f"{len(times)} prior-guided train step(s) on view 0 (SH degree 1 in use), "
+                      f"median {per:.3f} s/step, MSPLAT_THREADS={threads}"}
+
+
+def bench_images(repeats=200, cpu_seconds=5.0):
+    """The training images' pipeline for one 1264x832 view: area resampling of a decoded
+    RGBA8 camera image (2528x1664, and a 4032x3024 one at a non-integer ratio) to the
+    working size, one pyramid level (1264x832 -> 632x416 float), and Brown-Conrady
+    undistortion into the full frame (k1 -0.08, k2 0.02, p1 1e-3, p2 -5e-4). Per call:
+    `repeats` back-to-back launches from the host, wall clock over them (so launch costs
+    are included), bytes = source read once + output written once. Beside them the
+    oracle's single-thread C restatement (the reference's msplat_resize_area /
+    msplat_undistort, core/cpu/msplat_cpu.cpp:1609-1669) on the same inputs."""
+    import ctypes as C
+    from msplat._abi import Context, DeviceArray, check, floats, lib
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    ctx = Context(0)
+    rng = np.random.default_rng(40)
+    intr, dist = (FOCAL, FOCAL, W_IMG / 2.0, H_IMG / 2.0), (-0.08, 0.02, 1e-3, -5e-4, 0.0)
+    out = {"view": f"{W_IMG}x{H_IMG}", "ops": {}}
+    orc = Oracle()
+    try:
+        cases = []
+        for sw, sh in ((2 * W_IMG, 2 * H_IMG), (4032, 3024)):
+            img = rng.integers(0, 256, (sh, sw, 4), dtype=np.uint8)
+            cases.append((f"resize_u8_{sw}x{sh}", img, W_IMG, H_IMG))
+        lvl = rng.uniform(0, 1, (H_IMG, W_IMG, 3)).astype(np.float32)
+        cases.append((f"resize_f32_{W_IMG}x{H_IMG}_to_half", lvl, W_IMG // 2, H_IMG // 2))
+        cases.append((f"undistort_f32_{W_IMG}x{H_IMG}", lvl, W_IMG, H_IMG))
+        for name, host, dw, dh in cases:
+            src = DeviceArray.from_numpy(ctx, host, host.dtype)
+            dst = DeviceArray(ctx, (dh, dw, 3), np.float32, zero=False)
+            sh, sw = host.shape[:2]
+            if name.startswith("undistort"):
+                fi, fd = floats(intr), floats(dist)
+
+                def call():
+                    check(lib().msplat_hip_undistort(ctx.ptr, C.c_void_p(src.ptr), sw, sh, fi, fd, 0, 0,
+                                                     C.c_void_p(dst.ptr), dw, dh))
+            else:
+                u8 = int(host.dtype == np.uint8)
+
+                def call():
+                    check(lib().msplat_hip_resize_area(ctx.ptr, C.c_void_p(src.ptr), u8, sw, sh, C.c_void_p(dst.ptr),
+                                                       dw, dh))
+            for _ in range(10):
+                call()
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(repeats):
+                call()
+            ctx.sync()
+            per = (time.perf_counter() - t0) / repeats
+            got = dst.numpy().reshape(dh, dw, 3)
+            times, t_start = [], time.perf_counter()
+            while True:
+                t1 = time.perf_counter()
+                ref = (orc.undistort(host, intr, dist, 0, 0, dw, dh) if name.startswith("undistort")
+                       else orc.resize_area(host, dw, dh))
+                times.append(time.perf_counter() - t1)
+                if time.perf_counter() - t_start > cpu_seconds / len(cases) or len(times) >= 20:
+                    break
+            nbytes = host.nbytes + dw * dh * 12
+            out["ops"][name] = {"us_per_call": round(1e6 * per, 2), "bytes": int(nbytes),
+                                "gbs": round(nbytes / per / 1e9, 1), "frac": round(nbytes / per / 1e9 / HBM_PEAK_GBS, 4),
+                                "bit_exact_vs_oracle": bool(np.array_equal(got, ref)),
+                                "cpu_port_us": round(1e6 * float(np.median(times)), 1)}
+            src.free()
+            dst.free()
+    finally:
+        ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -824,6 +1002,8 @@ def main():
     ap.add_argument("--stage-steps", type=int, default=200,
                     help="steps after the timed ones that time every stage for the breakdown")
     ap.add_argument("--no-sh3", action="store_true", help="skip timing steps with SH degree 3 in use")
+    ap.add_argument("--no-prior", action="store_true", help="skip timing prior-guided steps (street preset)")
+    ap.add_argument("--no-images", action="store_true", help="skip timing the training images' pipeline")
     ap.add_argument("--no-plain", action="store_true", help="skip the 300-step plain workload (128x128 views)")
     ap.add_argument("--no-graph-compare", action="store_true",
                     help="skip timing replays of the captured step against launch-by-launch steps")
@@ -1000,6 +1180,15 @@ def main():
             serving = bench_serving(duration=args.serving_seconds, trained_ply=ply, trained_steps=tr.step_count)
         finally:
             shutil.rmtree(tmpdir, ignore_errors=True)
+    prior = None
+    if not args.no_prior:
+        log("prior-guided steps: street preset, SH degree 1")
+        prior = time_prior_steps(ctx, dg, cams, targets)
+        log(f"prior-guided: {prior['iter_per_s']} iter/s, plain SH1 {prior['plain_sh1_iter_per_s']} iter/s")
+    images = None
+    if not args.no_images:
+        images = bench_images()
+        log("images: " + json.dumps(images["ops"]))
     plain = None
     if not args.no_plain:
         log("plain workload: 20k gaussians, 8 views at 128x128, 300 steps")
@@ -1010,6 +1199,10 @@ def main():
     if not args.no_cpu_baseline:
         log("timing the CPU baseline")
         cpu = cpu_baseline_sample(dg, cams, targets, first_timed_step, args.cpu_seconds, plain)
+        if prior:
+            prior["cpu_reference"] = cpu_prior_sample(dg, cams, targets, prior)
+    if prior:
+        del prior["_cpu"]
     if plain:
         del plain["targets"]
     out = {
@@ -1027,6 +1220,8 @@ def main():
                    if tuning else None},
         "mpix_per_s": round(P * iters / 1e6, 1),
         "sh3_iter_per_s": sh3,
+        "prior_step": prior,
+        "images": images,
         "step_graph": graph,
         "roofline": roofline,
         "batched": batched,

@@ -170,6 +170,22 @@ __global__ __launch_bounds__(256) void prior_loss_kernel(PriorLossArgs a) {
     if (threadIdx.x < 3) a.rows[3 * (size_t)gy + threadIdx.x] = acc;
 }
 
+// x[0] + x[stride] + ... + x[(n - 1) stride] in double, added in that order; the loads go
+// out 16 at a time (one thread walking the sum would wait out a memory latency per term)
+__device__ inline double sum_in_order(const double *x, size_t stride, uint32_t n) {
+    double t = 0.0;
+    uint32_t i = 0;
+    for (; i + 16 <= n; i += 16) {
+        double v[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) v[j] = x[stride * (i + j)];
+#pragma unroll
+        for (int j = 0; j < 16; j++) t += v[j];
+    }
+    for (; i < n; i++) t += x[stride * i];
+    return t;
+}
+
 // the terms' sums over the rows, in order, scaled by 1 / (W H) (the slot after them is 0)
 __global__ void prior_terms_kernel(const double *rows, uint32_t H, float inv_npix, float *lt) {
     if (threadIdx.x >= 4) return;
@@ -177,9 +193,7 @@ __global__ void prior_terms_kernel(const double *rows, uint32_t H, float inv_npi
         lt[3] = 0.f;
         return;
     }
-    double t = 0.0;
-    for (uint32_t y = 0; y < H; y++) t += rows[3 * (size_t)y + threadIdx.x];
-    lt[threadIdx.x] = (float)(t * (double)inv_npix);
+    lt[threadIdx.x] = (float)(sum_in_order(rows + threadIdx.x, 3, H) * (double)inv_npix);
 }
 
 constexpr int EX_ITEMS = 4096;  // pixels per exposure block
@@ -221,8 +235,7 @@ __global__ void exposure_adam_kernel(const double *part, int nblocks, float *par
                                      float eps) {
     const int k = threadIdx.x;
     if (k >= 12) return;
-    double g = 0.0;
-    for (int b = 0; b < nblocks; b++) g += part[12 * (size_t)b + k];
+    const double g = sum_in_order(part + k, 12, (uint32_t)nblocks);
     const int i = 12 * cam + k;
     const float ident = (k == 0 || k == 4 || k == 8) ? 1.f : 0.f;
     const float gk = (float)g + (grad ? grad[k] : 0.f) + reg * (params[i] - ident);
@@ -236,23 +249,45 @@ __global__ void exposure_adam_kernel(const double *part, int nblocks, float *par
 
 constexpr double kFix = 281474976710656.0;  // 2^48: fixed-point units of the sky gradient sums
 
-// sky_backward (cpu:927-943): T v_img added to the four taps of each pixel's ray
+// sky_backward (cpu:927-943): T v_img added to the four taps of each pixel's ray. A view
+// covers a few thousand texels, so a hundred or more pixels land on each: the wave first
+// sums the fixed-point terms of its lanes that share a texel (one pass per distinct texel
+// of a tap, led by the lowest pending lane) and adds each sum with one atomic. Integer sums
+// do not depend on their order, so the texture's gradient is the one per-term atomics give.
 __global__ __launch_bounds__(256) void sky_grad_kernel(SkyArgs k, uint32_t W, uint32_t H, const float *final_T,
                                                        const float *v_img, unsigned long long *acc) {
     const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= W || y >= H) return;
-    const size_t p = (size_t)y * W + x;
-    const float T = final_T[p];
-    if (T < 1e-4f) return;
-    const float g[3] = {T * v_img[3 * p], T * v_img[3 * p + 1], T * v_img[3 * p + 2]};
-    uint32_t idx[4];
-    float wt[4];
-    sky_taps(k, (float)x, (float)y, idx, wt);
+    const int lane = (int)(threadIdx.x & 63);
+    bool live = x < W && y < H;
+    uint32_t idx[4] = {0, 0, 0, 0};
+    float wt[4] = {0.f, 0.f, 0.f, 0.f}, g[3] = {0.f, 0.f, 0.f};
+    if (live) {
+        const size_t p = (size_t)y * W + x;
+        const float T = final_T[p];
+        live = !(T < 1e-4f);
+        if (live) {
+            g[0] = T * v_img[3 * p];
+            g[1] = T * v_img[3 * p + 1];
+            g[2] = T * v_img[3 * p + 2];
+            sky_taps(k, (float)x, (float)y, idx, wt);
+        }
+    }
     for (int t = 0; t < 4; t++) {
-        if (wt[t] == 0.f) continue;
-        for (int c = 0; c < 3; c++) {
-            const long long q = __double2ll_rn((double)(wt[t] * g[c]) * kFix);
-            atomicAdd(acc + 3 * (size_t)idx[t] + c, (unsigned long long)q);
+        bool pend = live && wt[t] != 0.f;
+        unsigned long long q[3] = {0ull, 0ull, 0ull};
+        if (pend)
+            for (int c = 0; c < 3; c++) q[c] = (unsigned long long)__double2ll_rn((double)(wt[t] * g[c]) * kFix);
+        for (uint64_t m = __ballot(pend); m; m = __ballot(pend)) {
+            const int leader = __builtin_ctzll(m);
+            const uint32_t key = (uint32_t)__builtin_amdgcn_readlane((int)idx[t], leader);
+            const bool mine = pend && idx[t] == key;
+            unsigned long long s[3];
+            for (int c = 0; c < 3; c++) s[c] = mine ? q[c] : 0ull;
+            for (int o = 32; o >= 1; o >>= 1)
+                for (int c = 0; c < 3; c++) s[c] += __shfl_xor(s[c], o);
+            if (lane == leader)
+                for (int c = 0; c < 3; c++) atomicAdd(acc + 3 * (size_t)key + c, s[c]);
+            pend = pend && !mine;
         }
     }
 }
