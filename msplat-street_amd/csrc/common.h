@@ -142,6 +142,23 @@ __device__ inline float wave_sums9(const SumLanes &L, float q0, float q1, float 
     return sum_rows(L.q == 8 ? e : u);  // lane position r holds the same value in every row
 }
 
+// wave_sums9 with a tenth value paired with q8: lanes r >= 8 return the total of q8 where
+// s1 is clear (r = 8 among them) and of q9 where it is set (r = 9 among them). q0..q8
+// are added exactly as in wave_sums9.
+__device__ inline float wave_sums10(const SumLanes &L, float q0, float q1, float q2, float q3, float q4, float q5,
+                                    float q6, float q7, float q8, float q9) {
+    const float p0 = pair_step<0xb1>(q0, q1, L.s1), p1 = pair_step<0xb1>(q2, q3, L.s1);
+    const float p2 = pair_step<0xb1>(q4, q5, L.s1), p3 = pair_step<0xb1>(q6, q7, L.s1);
+    float e = pair_step<0xb1>(q8, q9, L.s1);  // the partners r^2, r^7 and r^15 share s1
+    const float r0 = pair_step<0x4e>(p0, p1, L.s2), r1 = pair_step<0x4e>(p2, p3, L.s2);
+    e += dpp_f<0x4e>(e);
+    float u = pair_step<0x141>(r0, r1, L.s3);
+    e += dpp_f<0x141>(e);
+    u += dpp_f<0x140>(u);
+    e += dpp_f<0x140>(e);
+    return sum_rows(L.q == 8 ? e : u);
+}
+
 __device__ inline uint64_t ballot(bool p) { return __ballot(p); }
 
 // Orders LDS accesses of the lanes of one wave (no instruction is emitted).

@@ -391,10 +391,12 @@ constexpr int NQ = 9;  // sum vs, vs dx, vs dy, vs dx^2, vs dx dy, vs dy^2, fac 
 // made the frame): each pixel starts from the chunk's end transmittance and from S over
 // the final transmittance's background plus the later chunks' colours.
 // With AUX (rasterize_backward with f.aux, cpu:963-1025): each pixel also carries the
-// prior losses' gradients v_d = dL/dD and v_a = dL/dA and the sums of z fac and fac
-// over the records behind, which add (z T - Sd / (1 - alpha)) v_d + (T - Sa / (1 - alpha)) v_a
-// to v_alpha; the per-pixel background is bg_img, and each record's sum of fac v_d goes
-// to its depth slot.
+// prior losses' gradients v_d = dL/dD and v_a = dL/dA, which add
+// (z T - Sd / (1 - alpha)) v_d + (T - Sa / (1 - alpha)) v_a to v_alpha, Sd and Sa being
+// the sums of z fac and fac over the records behind. These terms have the colour term's
+// form T c - S / (1 - alpha) with c = z v_d + v_a, so one running sum S of fac (c_rgb +
+// z v_d + v_a) carries all three; the per-pixel background is bg_img, and each record's
+// sum of fac v_d goes to its depth slot.
 template <int PPT, int BATCH, bool CHUNK = false, bool AUX = false>
 __device__ inline void backward_tile(const TileView &v, const TileData &d, int gt, int lt,
                                      const ChunkView *ch = nullptr) {
@@ -421,7 +423,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
     const int px = tx * TILE + (lane & 15);
     const float x0 = (float)(tx * TILE), y0 = (float)(ty * TILE);
     float fy[PPT], T[PPT], S[PPT], vr[PPT], vg[PPT], vb[PPT];
-    float vd[AUX ? PPT : 1], va[AUX ? PPT : 1], Sd[AUX ? PPT : 1], Sa[AUX ? PPT : 1];
+    float vd[AUX ? PPT : 1], va[AUX ? PPT : 1];
     int last[PPT];
     int m = s - 1;
 #pragma unroll
@@ -429,7 +431,7 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
         const int py = ty * TILE + 4 * PPT * w + (lane >> 4) + 4 * k;
         fy[k] = (float)py;
         T[k] = 0.f; S[k] = 0.f; vr[k] = 0.f; vg[k] = 0.f; vb[k] = 0.f;
-        if (AUX) { vd[k] = 0.f; va[k] = 0.f; Sd[k] = 0.f; Sa[k] = 0.f; }
+        if (AUX) { vd[k] = 0.f; va[k] = 0.f; }
         last[k] = s - 1;
         if (px < (int)v.width && py < (int)v.height) {
             const size_t pid = (size_t)py * v.width + px;
@@ -531,14 +533,12 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
                 const float ra = __builtin_amdgcn_rcpf(1.f - alpha);
                 const float Tn = T[k] * ra;
                 const float fac = valid ? alpha * Tn : 0.f;
-                const float cv = cr * vr[k] + cg * vg[k] + cbl * vb[k];
-                float v_alpha = Tn * cv - S[k] * ra;
+                float cv = cr * vr[k] + cg * vg[k] + cbl * vb[k];
                 if (AUX) {
-                    v_alpha += (z * Tn - Sd[k] * ra) * vd[k] + (Tn - Sa[k] * ra) * va[k];
-                    Sd[k] += z * fac;
-                    Sa[k] += fac;
+                    cv += z * vd[k] + va[k];
                     q9 += fac * vd[k];
                 }
+                const float v_alpha = Tn * cv - S[k] * ra;
                 T[k] = valid ? Tn : T[k];
                 S[k] += fac * cv;
                 q6 += fac * vr[k];
@@ -552,13 +552,11 @@ __device__ inline void backward_tile(const TileView &v, const TileData &d, int g
             }
             const float q1 = q0 * dx, q3 = q1 * dx, q4 = q2 * dx;
             if (__ballot(contrib) != 0) {
-                const float t = wave_sums9(sl, q0, q1, q2, q3, q4, q5, q6, q7, q8);
+                const float t = AUX ? wave_sums10(sl, q0, q1, q2, q3, q4, q5, q6, q7, q8, q9)
+                                    : wave_sums9(sl, q0, q1, q2, q3, q4, q5, q6, q7, q8);
                 if (lane < NQ) part[w][j][sl.q] = t;
                 if (lane == 0) ltouch[j] = 1;
-                if (AUX) {
-                    for (int dd = 32; dd >= 1; dd >>= 1) q9 += __shfl_xor(q9, dd);
-                    if (lane == 0) partd[w][j] = q9;
-                }
+                if (AUX && lane == 9) partd[w][j] = t;  // lane 9: the total of q9
             }
         }
         __syncthreads();
