@@ -171,16 +171,17 @@ __global__ __launch_bounds__(256) void prior_loss_kernel(PriorLossArgs a) {
 }
 
 // x[0] + x[stride] + ... + x[(n - 1) stride] in double, added in that order; the loads go
-// out 16 at a time (one thread walking the sum would wait out a memory latency per term)
+// out 64 at a time (one thread walking the sum would wait out a memory latency per term)
 __device__ inline double sum_in_order(const double *x, size_t stride, uint32_t n) {
+    constexpr uint32_t B = 64;
     double t = 0.0;
     uint32_t i = 0;
-    for (; i + 16 <= n; i += 16) {
-        double v[16];
+    for (; i + B <= n; i += B) {
+        double v[B];
 #pragma unroll
-        for (int j = 0; j < 16; j++) v[j] = x[stride * (i + j)];
+        for (uint32_t j = 0; j < B; j++) v[j] = x[stride * (i + j)];
 #pragma unroll
-        for (int j = 0; j < 16; j++) t += v[j];
+        for (uint32_t j = 0; j < B; j++) t += v[j];
     }
     for (; i < n; i++) t += x[stride * i];
     return t;
