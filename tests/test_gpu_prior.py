@@ -35,11 +35,11 @@ def _state(sc):
     return st
 
 
-def _compare(ctx, oracle, variant, seed=3, n=600, W=64, H=48, steps=2):
+def _compare(ctx, oracle, variant, seed=3, n=600, W=64, H=48, steps=2, sky=(16, 32)):
     from msplat._abi import DeviceArray
     from msplat.backend import DeviceGaussians
     from msplat.prior import DevicePrior, aux_outputs, train_step_prior
-    sc, cam, gt, prior = prior_workload(seed=seed, n=n, W=W, H=H, features=PRIOR_VARIANTS[variant])
+    sc, cam, gt, prior = prior_workload(seed=seed, n=n, W=W, H=H, features=PRIOR_VARIANTS[variant], sky=sky)
     dg = DeviceGaussians(ctx, sc)
     dp = DevicePrior(ctx, prior)
     gt_d = DeviceArray.from_numpy(ctx, gt)
@@ -83,6 +83,17 @@ def test_prior_steps_at_a_larger_view_match_oracle(ctx, oracle):
     errs = _compare(ctx, oracle, "all", seed=11, n=20000, W=320, H=240)
     worst = max(errs, key=errs.get)
     print(f"\n320x240: worst {worst} {errs[worst]:.2e}")
+    for k, v in errs.items():
+        assert v < 1e-4, (k, v)
+
+
+def test_prior_steps_with_a_full_sky_texture_match_oracle(ctx, oracle):
+    """Every feature on 20,000 gaussians in a 640x416 view with the default 512x128 sky: a
+    texel spans about 8 pixels across, so each wave of the sky gradient sums many pixels
+    per texel before its atomics (the texture's gradient, moments and values are checked)."""
+    errs = _compare(ctx, oracle, "all", seed=12, n=20000, W=640, H=416, sky=(128, 512))
+    worst = max(errs, key=errs.get)
+    print(f"\n640x416, 512x128 sky: worst {worst} {errs[worst]:.2e}")
     for k, v in errs.items():
         assert v < 1e-4, (k, v)
 

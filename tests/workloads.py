@@ -38,11 +38,11 @@ UNDISTORT_CASES = [((70.0, 72.0, 40.3, 29.7), (0.1, -0.05, 0.001, -0.002, 0.01),
 
 
 def prior_workload(seed=0, n=600, W=64, H=48, features=("aux", "sky", "depth", "sky_mask", "fill", "exposure", "cap",
-                                                         "unkept")):
+                                                         "unkept"), sky=(16, 32)):
     """A prior-guided training case after the reference's gradcheck priors run: a few
     hundred SH1 gaussians in front of a W x H camera (so that sky shows between them), a
-    target, and a PriorStep dict with the named features: the aux rasterizer, a 32x16
-    learned sky, a depth prior at its own resolution (W/2 + 3 by H/2 - 1, with invalid
+    target, and a PriorStep dict with the named features: the aux rasterizer, a learned
+    sky (`sky` = (rows, columns), 32x16 by default), a depth prior at its own resolution (W/2 + 3 by H/2 - 1, with invalid
     samples), a sky mask over its top quarter, the fill term, exposure rows for 2 cameras,
     the needle cap, and pixels the prior does not keep. Every array is freshly made."""
     from msplat.scenes import random_scene, ring_cameras
@@ -64,9 +64,10 @@ def prior_workload(seed=0, n=600, W=64, H=48, features=("aux", "sky", "depth", "
     prior = dict(aux="aux" in f, loss_terms=np.zeros(4, np.float32), huber_delta=0.05, min_alpha=0.25,
                  mask_photometric=True)
     if "sky" in f:
-        prior.update(sky=True, sky_tex=rng.uniform(0.2, 0.8, (16, 32, 3)).astype(np.float32),
-                     sky_grad=np.zeros((16, 32, 3), np.float32), sky_m1=np.zeros((16, 32, 3), np.float32),
-                     sky_m2=np.zeros((16, 32, 3), np.float32), sky_frame=(0, -1, 0, 0, 0, 1, 1, 0, 0),
+        sh_, sw_ = sky
+        prior.update(sky=True, sky_tex=rng.uniform(0.2, 0.8, (sh_, sw_, 3)).astype(np.float32),
+                     sky_grad=np.zeros((sh_, sw_, 3), np.float32), sky_m1=np.zeros((sh_, sw_, 3), np.float32),
+                     sky_m2=np.zeros((sh_, sw_, 3), np.float32), sky_frame=(0, -1, 0, 0, 0, 1, 1, 0, 0),
                      sky_step=0.01, sky_bc2=float(np.sqrt(np.float32(1 - 0.999))))
     if f & {"depth", "sky_mask", "fill"}:
         prior.update(prior_depth=depth, prior_aux=aux, has_depth="depth" in f, has_sky_mask="sky_mask" in f,
