@@ -980,6 +980,59 @@ def bench_images(repeats=200, cpu_seconds=5.0):
     return out
 
 
+def bench_scene_files(ctx, dg, step, tmpdir, repeats=3):
+    """The trained scene's files (save_gaussians.cpp:10-191, model.cpp:282-496): PLY and
+    .splat exports, the PLY import and the .msplat checkpoint, each the best of `repeats`
+    (host I/O into `tmpdir` after one device-to-host staging copy, so the page cache and
+    the host's write speed bound them); beside them the reference's own writers and reader
+    (the compiled core/src/loaders/save_gaussians.cpp) on the same parameters, whose files
+    are compared byte for byte. The reference is skipped when it was not built."""
+    from msplat.scene_io import load_checkpoint, load_ply, save_checkpoint, save_ply, save_splat
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Reference, reference_available
+
+    def best(fn):
+        times = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+        return min(times)
+
+    p = {k: os.path.join(tmpdir, "scene." + k) for k in ("ply", "splat", "msplat", "ref.ply", "ref.splat")}
+    lrs = [1.6e-4, 5e-3, 1e-3, 2.5e-3, 1.25e-4, 5e-2]
+    loaded = []
+
+    def load(fn):
+        loaded.append(fn())
+        loaded.pop()[0].free()
+
+    gpu = {"save_ply": best(lambda: save_ply(ctx, dg, p["ply"], step=step)),
+           "save_splat": best(lambda: save_splat(ctx, dg, p["splat"])),
+           "save_checkpoint": best(lambda: save_checkpoint(ctx, p["msplat"], dg, step, step, lrs)),
+           "load_ply": best(lambda: load(lambda: load_ply(ctx, p["ply"], capacity=dg.n))),
+           "load_checkpoint": best(lambda: load(lambda: load_checkpoint(ctx, p["msplat"], capacity=dg.n)))}
+    size = {"save_ply": p["ply"], "save_splat": p["splat"], "save_checkpoint": p["msplat"], "load_ply": p["ply"],
+            "load_checkpoint": p["msplat"]}
+    out = {"gaussians": dg.n, "sh_degree": dg.degree,
+           "ops": {k: {"s": round(t, 4), "mb": round(os.path.getsize(size[k]) / 1e6, 1),
+                       "mb_per_s": round(os.path.getsize(size[k]) / t / 1e6, 1)} for k, t in gpu.items()}}
+    if reference_available():
+        ref = Reference()
+        host = dg.to_numpy()
+        scene = {k: host[k] for k in ("means", "scales", "quats", "dc", "rest", "opac")}
+        scene["degree"] = dg.degree
+        rb = dg.rest_bases
+        out["reference"] = {
+            "save_ply_s": round(best(lambda: ref.save(scene, p["ref.ply"], step=step)), 4),
+            "save_splat_s": round(best(lambda: ref.save(scene, p["ref.splat"], splat=True)), 4),
+            "load_ply_s": round(best(lambda: ref.load_ply(p["ref.ply"], dg.n, rb)), 4),
+            "ply_bytes_equal": open(p["ply"], "rb").read() == open(p["ref.ply"], "rb").read(),
+            "splat_bytes_equal": open(p["splat"], "rb").read() == open(p["ref.splat"], "rb").read(),
+            "build": "oracle/_ref/libmsplat_ref.so (the reference's -O2 build), one thread"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -1004,6 +1057,7 @@ def main():
     ap.add_argument("--no-sh3", action="store_true", help="skip timing steps with SH degree 3 in use")
     ap.add_argument("--no-prior", action="store_true", help="skip timing prior-guided steps (street preset)")
     ap.add_argument("--no-images", action="store_true", help="skip timing the training images' pipeline")
+    ap.add_argument("--no-scene-files", action="store_true", help="skip timing the trained scene's file exports")
     ap.add_argument("--no-plain", action="store_true", help="skip the 300-step plain workload (128x128 views)")
     ap.add_argument("--no-graph-compare", action="store_true",
                     help="skip timing replays of the captured step against launch-by-launch steps")
@@ -1180,6 +1234,16 @@ def main():
             serving = bench_serving(duration=args.serving_seconds, trained_ply=ply, trained_steps=tr.step_count)
         finally:
             shutil.rmtree(tmpdir, ignore_errors=True)
+    scene_files = None
+    if not args.no_scene_files:
+        import shutil
+        import tempfile
+        tmpdir = tempfile.mkdtemp(prefix="msplat_files_")
+        try:
+            scene_files = bench_scene_files(ctx, dg, tr.step_count, tmpdir)
+            log("scene files: " + json.dumps(scene_files))
+        finally:
+            shutil.rmtree(tmpdir, ignore_errors=True)
     prior = None
     if not args.no_prior:
         log("prior-guided steps: street preset, SH degree 1")
@@ -1222,6 +1286,7 @@ def main():
         "sh3_iter_per_s": sh3,
         "prior_step": prior,
         "images": images,
+        "scene_files": scene_files,
         "step_graph": graph,
         "roofline": roofline,
         "batched": batched,
