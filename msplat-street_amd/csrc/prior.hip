@@ -250,11 +250,32 @@ __global__ void exposure_adam_kernel(const double *part, int nblocks, float *par
 
 constexpr double kFix = 281474976710656.0;  // 2^48: fixed-point units of the sky gradient sums
 
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline unsigned long long dpp_u64(unsigned long long v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, ROW_MASK, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, ROW_MASK, 0xf, false);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// The wave's total of a 64-bit integer, in lane 63 (the DPP steps of a float wave sum:
+// partners r^1, r^2, r^7, r^15 within each row of 16, then the row broadcasts); every lane
+// of the wave calls it
+__device__ inline unsigned long long wave_total_u64(unsigned long long v) {
+    v += dpp_u64<0xb1>(v);        // quad_perm [1,0,3,2]
+    v += dpp_u64<0x4e>(v);        // quad_perm [2,3,0,1]
+    v += dpp_u64<0x141>(v);       // row_half_mirror
+    v += dpp_u64<0x140>(v);       // row_mirror
+    v += dpp_u64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_u64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 // sky_backward (cpu:927-943): T v_img added to the four taps of each pixel's ray. A view
 // covers a few thousand texels, so a hundred or more pixels land on each: the wave first
 // sums the fixed-point terms of its lanes that share a texel (one pass per distinct texel
-// of a tap, led by the lowest pending lane) and adds each sum with one atomic. Integer sums
-// do not depend on their order, so the texture's gradient is the one per-term atomics give.
+// of a tap, led by the lowest pending lane; a DPP reduction leaves the sum in lane 63),
+// which adds it with one atomic. Integer sums do not depend on their order, so the
+// texture's gradient is the one per-term atomics give.
 __global__ __launch_bounds__(256) void sky_grad_kernel(SkyArgs k, uint32_t W, uint32_t H, const float *final_T,
                                                        const float *v_img, unsigned long long *acc) {
     const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
@@ -283,10 +304,8 @@ __global__ __launch_bounds__(256) void sky_grad_kernel(SkyArgs k, uint32_t W, ui
             const uint32_t key = (uint32_t)__builtin_amdgcn_readlane((int)idx[t], leader);
             const bool mine = pend && idx[t] == key;
             unsigned long long s[3];
-            for (int c = 0; c < 3; c++) s[c] = mine ? q[c] : 0ull;
-            for (int o = 32; o >= 1; o >>= 1)
-                for (int c = 0; c < 3; c++) s[c] += __shfl_xor(s[c], o);
-            if (lane == leader)
+            for (int c = 0; c < 3; c++) s[c] = wave_total_u64(mine ? q[c] : 0ull);
+            if (lane == 63)
                 for (int c = 0; c < 3; c++) atomicAdd(acc + 3 * (size_t)key + c, s[c]);
             pend = pend && !mine;
         }
