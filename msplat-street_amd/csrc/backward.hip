@@ -237,8 +237,8 @@ __global__ __launch_bounds__(256) void backward_kernel(BackwardArgs a) {
 // Slot sums of the gaussians with more than BIG_SLOTS intersections, one wave each
 // (grid-stride over the device-side list; every wave exits when the list is done).
 // Lane l adds the written slots among l, l + 64, ... in order, in rounds of four slots
-// whose flag and value loads are in flight together; the nine lane sums are added over
-// the wave by wave_sums9. The next record's id and slot range are loaded while this one
+// whose flag and value loads are in flight together; the nine lane sums (ten with depth
+// slots) are added over the wave by wave_sums9 (wave_sums10). The next record's id and slot range are loaded while this one
 // is summed.
 __global__ __launch_bounds__(256) void big_reduce_kernel(BackwardArgs a) {
     const uint32_t nbig = *a.big_count;
@@ -288,11 +288,10 @@ __global__ __launch_bounds__(256) void big_reduce_kernel(BackwardArgs a) {
                 v[8] += Cs[u];
             }
         }
-        const float t = wave_sums9(sl, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]);
-        if (a.slot_d) {
-            for (int dd = 32; dd >= 1; dd >>= 1) vd += __shfl_xor(vd, dd);
-            if (lane == 0) a.big_d[g] = vd;
-        }
+        // with depth slots the tenth sum rides with the ninth: lane 9 ends with its total
+        const float t = a.slot_d ? wave_sums10(sl, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], vd)
+                                 : wave_sums9(sl, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]);
+        if (a.slot_d && lane == 9) a.big_d[g] = t;
         if (lane < 9) {
             float *dst = sl.q < 4 ? &a.big_a[g].x + sl.q : sl.q < 8 ? &a.big_b[g].x + (sl.q - 4) : &a.big_c[g];
             *dst = t;
