@@ -197,17 +197,39 @@ __global__ void prior_terms_kernel(const double *rows, uint32_t H, float inv_npi
     lt[threadIdx.x] = (float)(sum_in_order(rows + threadIdx.x, 3, H) * (double)inv_npix);
 }
 
+// A double moved across lanes by DPP (both halves with the same control)
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline double dpp_d(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROW_MASK, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The wave's total of a double in lane 63, in a fixed order (the steps of a float wave sum)
+__device__ inline double wave_total_d(double v) {
+    v += dpp_d<0xb1>(v);
+    v += dpp_d<0x4e>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    v += dpp_d<0x142, 0xa>(v);
+    v += dpp_d<0x143, 0xc>(v);
+    return v;
+}
+
 constexpr int EX_ITEMS = 4096;  // pixels per exposure block
+constexpr int EX_THREADS = 1024;
 
 // exposure_backward (cpu:899-913): v_img = M^T v_adj per pixel, and the block's sums of
-// the 12 gradient terms (double, pixels in order per thread, threads in a fixed tree)
-__global__ __launch_bounds__(256) void exposure_backward_kernel(const float *img, const float *v_adj, size_t n,
-                                                               const float *P, float *v_img, double *part) {
-    __shared__ double red[12][256];
+// the 12 gradient terms in double: each thread's pixels in order, then each wave's lanes
+// by a fixed DPP reduction, then the 16 waves in order
+__global__ __launch_bounds__(EX_THREADS) void exposure_backward_kernel(const float *img, const float *v_adj, size_t n,
+                                                                      const float *P, float *v_img, double *part) {
+    __shared__ double red[EX_THREADS / 64][12];
     double g[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const size_t base = (size_t)blockIdx.x * EX_ITEMS;
-    for (int u = 0; u < EX_ITEMS / 256; u++) {
-        const size_t i = base + (size_t)u * 256 + threadIdx.x;
+    for (int u = 0; u < EX_ITEMS / EX_THREADS; u++) {
+        const size_t i = base + (size_t)u * EX_THREADS + threadIdx.x;
         if (i >= n) break;
         const float c0 = img[3 * i], c1 = img[3 * i + 1], c2 = img[3 * i + 2];
         const float vx = v_adj[3 * i], vy = v_adj[3 * i + 1], vz = v_adj[3 * i + 2];
@@ -219,14 +241,17 @@ __global__ __launch_bounds__(256) void exposure_backward_kernel(const float *img
         g[6] += vz * c0; g[7] += vz * c1; g[8] += vz * c2;
         g[9] += vx; g[10] += vy; g[11] += vz;
     }
-    for (int k = 0; k < 12; k++) red[k][threadIdx.x] = g[k];
-    __syncthreads();
-    for (int w = 128; w >= 1; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            for (int k = 0; k < 12; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
-        __syncthreads();
+    const int w = (int)(threadIdx.x >> 6);
+    for (int k = 0; k < 12; k++) {
+        const double t = wave_total_d(g[k]);
+        if ((threadIdx.x & 63) == 63) red[w][k] = t;
     }
-    if (threadIdx.x < 12) part[12 * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        double t = 0.0;
+        for (int i = 0; i < EX_THREADS / 64; i++) t += red[i][threadIdx.x];
+        part[12 * (size_t)blockIdx.x + threadIdx.x] = t;
+    }
 }
 
 // exposure_adam (cpu:914-923): the sums of the block partials in order, plus the caller's
@@ -385,7 +410,7 @@ void launch_exposure_backward_adam(const float *img, const float *v_adj, size_t 
                                    float *params, float *m1, float *m2, float *grad, int cam, float step, float bc2,
                                    float reg, float b1, float b2, float eps, hipStream_t s) {
     const size_t nb = exposure_partials(n);
-    if (nb) exposure_backward_kernel<<<(unsigned)nb, 256, 0, s>>>(img, v_adj, n, params + 12 * (size_t)cam, v_img, part);
+    if (nb) exposure_backward_kernel<<<(unsigned)nb, EX_THREADS, 0, s>>>(img, v_adj, n, params + 12 * (size_t)cam, v_img, part);
     exposure_adam_kernel<<<1, 64, 0, s>>>(part, (int)nb, params, m1, m2, grad, cam, step, bc2, reg, b1, b2, eps);
 }
 
