@@ -92,17 +92,39 @@ __device__ inline float prior_depth_at(const float *d, float x, float y, uint32_
     return top + (bottom - top) * fy;
 }
 
+// A double moved across lanes by DPP (both halves with the same control)
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline double dpp_d(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROW_MASK, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The wave's total of a double in lane 63, in a fixed order (the steps of a float wave sum)
+__device__ inline double wave_total_d(double v) {
+    v += dpp_d<0xb1>(v);
+    v += dpp_d<0x4e>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    v += dpp_d<0x142, 0xa>(v);
+    v += dpp_d<0x143, 0xc>(v);
+    return v;
+}
+
 constexpr int PL_CHUNK = 1024;  // pixels of a row per round (4 per thread)
 
-// prior_loss (cpu:809-884), one workgroup per row. Each pixel's three float terms go to
-// LDS; threads 0-2 add them in pixel order in double, as the reference's row loop.
+// prior_loss (cpu:809-884), one workgroup per row. The row's three sums of the pixels'
+// float terms are taken in double in a fixed order: each thread's pixels in order, each
+// wave's lanes by a DPP reduction, the four waves in order (the reference adds the row
+// in pixel order; the two orders round differently by about 1e-16 relative).
 __global__ __launch_bounds__(256) void prior_loss_kernel(PriorLossArgs a) {
-    __shared__ float terms[3][PL_CHUNK];
+    __shared__ double wsum[4][3];
+    double tsum[3] = {0.0, 0.0, 0.0};
     const uint32_t gy = blockIdx.x, W = a.width, H = a.height;
     const bool has_prior = a.prior_depth != nullptr;
     const uint32_t pw = has_prior ? a.pw : 1u, ph = has_prior ? a.ph : 1u;
     const float inv_npix = 1.0f / (float)(W * H);
-    double acc = 0.0;
     for (uint32_t x0 = 0; x0 < W; x0 += PL_CHUNK) {
         for (int u = 0; u < PL_CHUNK / 256; u++) {
             const uint32_t j = u * 256 + threadIdx.x, gx = x0 + j;
@@ -156,65 +178,38 @@ __global__ __launch_bounds__(256) void prior_loss_kernel(PriorLossArgs a) {
                 a.v_aux[2 * pix] = v_d;
                 a.v_aux[2 * pix + 1] = v_a;
             }
-            terms[0][j] = ld;
-            terms[1][j] = ls;
-            terms[2][j] = lf;
+            tsum[0] += (double)ld;
+            tsum[1] += (double)ls;
+            tsum[2] += (double)lf;
         }
-        __syncthreads();
-        if (threadIdx.x < 3) {
-            const uint32_t n = min((uint32_t)PL_CHUNK, W - x0);
-            for (uint32_t j = 0; j < n; j++) acc += (double)terms[threadIdx.x][j];
-        }
-        __syncthreads();
     }
-    if (threadIdx.x < 3) a.rows[3 * (size_t)gy + threadIdx.x] = acc;
-}
-
-// x[0] + x[stride] + ... + x[(n - 1) stride] in double, added in that order; the loads go
-// out 64 at a time (one thread walking the sum would wait out a memory latency per term)
-__device__ inline double sum_in_order(const double *x, size_t stride, uint32_t n) {
-    constexpr uint32_t B = 64;
-    double t = 0.0;
-    uint32_t i = 0;
-    for (; i + B <= n; i += B) {
-        double v[B];
-#pragma unroll
-        for (uint32_t j = 0; j < B; j++) v[j] = x[stride * (i + j)];
-#pragma unroll
-        for (uint32_t j = 0; j < B; j++) t += v[j];
+    for (int k = 0; k < 3; k++) {
+        const double t = wave_total_d(tsum[k]);
+        if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6][k] = t;
     }
-    for (; i < n; i++) t += x[stride * i];
-    return t;
+    __syncthreads();
+    if (threadIdx.x < 3)
+        a.rows[3 * (size_t)gy + threadIdx.x] =
+            ((wsum[0][threadIdx.x] + wsum[1][threadIdx.x]) + wsum[2][threadIdx.x]) + wsum[3][threadIdx.x];
 }
 
-// the terms' sums over the rows, in order, scaled by 1 / (W H) (the slot after them is 0)
-__global__ void prior_terms_kernel(const double *rows, uint32_t H, float inv_npix, float *lt) {
-    if (threadIdx.x >= 4) return;
-    if (threadIdx.x == 3) {
-        lt[3] = 0.f;
-        return;
+// the terms' sums over the rows, scaled by 1 / (W H) (the slot after them is 0), in a
+// fixed order: each thread's rows y = t, t + 256, ... in order, each wave by a DPP
+// reduction, the four waves in order
+__global__ __launch_bounds__(256) void prior_terms_kernel(const double *rows, uint32_t H, float inv_npix, float *lt) {
+    __shared__ double wsum[4][3];
+    double t[3] = {0.0, 0.0, 0.0};
+    for (uint32_t y = threadIdx.x; y < H; y += 256)
+        for (int k = 0; k < 3; k++) t[k] += rows[3 * (size_t)y + k];
+    for (int k = 0; k < 3; k++) {
+        const double w = wave_total_d(t[k]);
+        if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6][k] = w;
     }
-    lt[threadIdx.x] = (float)(sum_in_order(rows + threadIdx.x, 3, H) * (double)inv_npix);
-}
-
-// A double moved across lanes by DPP (both halves with the same control)
-template <int CTRL, int ROW_MASK = 0xf>
-__device__ inline double dpp_d(double v) {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROW_MASK, 0xf, false);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROW_MASK, 0xf, false);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// The wave's total of a double in lane 63, in a fixed order (the steps of a float wave sum)
-__device__ inline double wave_total_d(double v) {
-    v += dpp_d<0xb1>(v);
-    v += dpp_d<0x4e>(v);
-    v += dpp_d<0x141>(v);
-    v += dpp_d<0x140>(v);
-    v += dpp_d<0x142, 0xa>(v);
-    v += dpp_d<0x143, 0xc>(v);
-    return v;
+    __syncthreads();
+    if (threadIdx.x < 3)
+        lt[threadIdx.x] = (float)((((wsum[0][threadIdx.x] + wsum[1][threadIdx.x]) + wsum[2][threadIdx.x]) +
+                                   wsum[3][threadIdx.x]) * (double)inv_npix);
+    if (threadIdx.x == 3) lt[3] = 0.f;
 }
 
 constexpr int EX_ITEMS = 4096;  // pixels per exposure block
@@ -252,6 +247,23 @@ __global__ __launch_bounds__(EX_THREADS) void exposure_backward_kernel(const flo
         for (int i = 0; i < EX_THREADS / 64; i++) t += red[i][threadIdx.x];
         part[12 * (size_t)blockIdx.x + threadIdx.x] = t;
     }
+}
+
+// x[0] + x[stride] + ... + x[(n - 1) stride] in double, added in that order; the loads go
+// out 64 at a time (one thread walking the sum would wait out a memory latency per term)
+__device__ inline double sum_in_order(const double *x, size_t stride, uint32_t n) {
+    constexpr uint32_t B = 64;
+    double t = 0.0;
+    uint32_t i = 0;
+    for (; i + B <= n; i += B) {
+        double v[B];
+#pragma unroll
+        for (uint32_t j = 0; j < B; j++) v[j] = x[stride * (i + j)];
+#pragma unroll
+        for (uint32_t j = 0; j < B; j++) t += v[j];
+    }
+    for (; i < n; i++) t += x[stride * i];
+    return t;
 }
 
 // exposure_adam (cpu:914-923): the sums of the block partials in order, plus the caller's
@@ -401,7 +413,7 @@ void launch_exposure_apply(const float *img, const float *P, size_t n, float *ou
 void launch_prior_loss(const PriorLossArgs &a, float *loss_terms, hipStream_t s) {
     if (!a.width || !a.height) return;
     prior_loss_kernel<<<a.height, 256, 0, s>>>(a);
-    if (loss_terms) prior_terms_kernel<<<1, 64, 0, s>>>(a.rows, a.height, 1.0f / (float)(a.width * a.height), loss_terms);
+    if (loss_terms) prior_terms_kernel<<<1, 256, 0, s>>>(a.rows, a.height, 1.0f / (float)(a.width * a.height), loss_terms);
 }
 
 size_t exposure_partials(size_t n) { return (n + EX_ITEMS - 1) / EX_ITEMS; }
