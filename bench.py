@@ -361,6 +361,9 @@ def serve_runtime_scene(srv, ctx, path, centers, seed, trained_steps=None, rate=
     return out
 
 
+SHED_WAIT_MS = 8.0  # queueing bound of the load-shedding run
+
+
 def bench_serving(rates=SERVE_RATES, duration=1.0, slots=6, max_views=16, seed=20, trained_ply=None,
                   trained_steps=None):
     """Serving: four resident SH3 scenes (0.5M, 1M, 2M, 3M Gaussians, 64-cluster clouds in
@@ -372,7 +375,8 @@ def bench_serving(rates=SERVE_RATES, duration=1.0, slots=6, max_views=16, seed=2
     plain form (one msplat_hip_render per request, first come first served), each
     followed by the RGBA8 pack. A last run at the top rate gives the server a budget that
     fits two 1920x1080 working sets per slot but no 3840x2160 one, to show admission
-    refusing what cannot fit."""
+    refusing what cannot fit; another sheds the requests still queued SHED_WAIT_MS after
+    their arrival, to show the latency of what is served under overload."""
     from msplat._abi import Context
     from msplat.backend import DeviceGaussians
     from msplat.scenes import gmm_cloud
@@ -435,6 +439,20 @@ def bench_serving(rates=SERVE_RATES, duration=1.0, slots=6, max_views=16, seed=2
                         "completed": b.get("completed", 0), "rejected_by_resolution": rej,
                         "p99_ms": b.get("p99_ms"), "views_per_s": b.get("views_per_s")}
     log(f"admission at a {tight / 1e9:.2f} GB budget: rejected {rej}")
+    srv.close()
+    # load shedding at the top rate: requests queued longer than max_wait_ms are dropped
+    srv = Server(scenes, slots=slots, max_views=max_views, background=BG, max_wait_ms=SHED_WAIT_MS)
+    reqs = poisson_requests(float(rates[-1]), duration, centers, seed=seed + rates[-1])
+    b = latency_stats(srv.run(reqs))
+    shed = {}
+    for r in srv.shed:
+        k = f"{r.cam['W']}x{r.cam['H']}"
+        shed[k] = shed.get(k, 0) + 1
+    out["shedding"] = {"rate": rates[-1], "max_wait_ms": SHED_WAIT_MS, "requests": len(reqs),
+                       "completed": b.get("completed", 0), "shed": len(srv.shed), "shed_by_resolution": shed,
+                       "views_per_s": b.get("views_per_s"), "p50_ms": b.get("p50_ms"), "p99_ms": b.get("p99_ms"),
+                       "p99_1080p_ms": b.get("by_resolution", {}).get("1920x1080", {}).get("p99_ms")}
+    log("shedding: " + json.dumps(out["shedding"]))
     srv.close()
     ctx.close()
     return out

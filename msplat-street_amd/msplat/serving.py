@@ -93,13 +93,17 @@ class Server:
     def __init__(self, scenes: list, slots: int = 2, max_views: int = 16, hbm_budget: float | None = None,
                  background=(0.0, 0.0, 0.0), device: int = 0, keep_images: bool = False,
                  isects_per_pixel: float = 60.0, adapt: bool = True, batch_depth: int | None = None,
-                 max_batch_pixels: float = 2.1e6):
+                 max_batch_pixels: float = 2.1e6, max_wait_ms: float | None = None):
         self.scenes = scenes
         self.max_views = int(max_views)
         # pending requests beyond which a free slot batches (default: 2 per slot), and the
         # pixels a batch may hold in all (default: about one 1920x1080 view)
         self.batch_depth = 2 * slots if batch_depth is None else int(batch_depth)
         self.max_batch_pixels = float(max_batch_pixels)
+        # load shedding: a request still queued this long after its arrival is dropped
+        # (to self.shed) instead of rendered, so that under overload the served requests
+        # keep their latency bound; None serves everything
+        self.max_wait = None if max_wait_ms is None else float(max_wait_ms) / 1e3
         self.bg = tuple(float(v) for v in background)
         self.keep_images = keep_images
         self.scenes = list(scenes)  # None where a scene was evicted (indices stay)
@@ -121,6 +125,7 @@ class Server:
         self.pending: list[Request] = []
         self.finished: list[Request] = []
         self.rejected: list[Request] = []
+        self.shed: list[Request] = []
         self.cond = threading.Condition()
         self.stop = False
         self.t0 = 0.0
@@ -182,7 +187,18 @@ class Server:
     def _take_batch(self) -> list:
         """The oldest pending request; when more than batch_depth are pending, also the
         later ones of its scene, in arrival order, up to max_views, max_batch_pixels and
-        the slot's share of the budget."""
+        the slot's share of the budget. With max_wait, the requests at the front that have
+        waited longer are shed first (later arrivals have waited less)."""
+        if self.max_wait is not None:
+            now = self._now()
+            k = 0
+            while k < len(self.pending) and now - self.pending[k].arrival > self.max_wait:
+                k += 1
+            if k:
+                self.shed.extend(self.pending[:k])
+                del self.pending[:k]
+                if not self.pending:
+                    return []
         head = self.pending[0]
         if not self._live(head.scene) or self.working_set([head]) > self.share:
             self.pending.pop(0)
@@ -265,7 +281,7 @@ class Server:
         """Feeds the requests at their arrival times (seconds from now) and returns them
         completed (or in self.rejected)."""
         threads = [threading.Thread(target=self._worker, args=(s,), daemon=True) for s in self.slots]
-        self.finished, self.rejected, self.errors, self.batches = [], [], [], 0
+        self.finished, self.rejected, self.shed, self.errors, self.batches = [], [], [], [], 0
         self.stop = False
         self.t0 = time.perf_counter()
         for th in threads:

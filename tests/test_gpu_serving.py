@@ -129,3 +129,29 @@ def test_scenes_admitted_and_evicted_while_serving(ctx, tmp_path):
         assert roomy.add_scene(scenes[1]) is not None
     finally:
         roomy.close()
+
+
+def test_requests_queued_past_the_deadline_are_shed(ctx):
+    """Load shedding: two hundred requests that all arrive at once, with a 2 ms queueing
+    bound. The two slots cannot start them all within 2 ms, so the rest are shed; every
+    served request started rendering within the bound (plus the host's scheduling slack),
+    and every request is either served or shed. Without the bound all are served."""
+    from msplat.serving import Request, Server, poisson_requests
+    scenes, centers = _scenes(ctx)
+    base = poisson_requests(4000.0, 0.05, centers, seed=5, radius=(3.0, 6.0), resolutions=SMALL)
+    reqs = [Request(r.rid, r.scene, r.cam, 0.0) for r in base]
+    assert len(reqs) >= 150
+    srv = Server(scenes, slots=2, max_views=8, background=BG, max_wait_ms=2.0)
+    try:
+        done = srv.run(reqs)
+    finally:
+        srv.close()
+    assert len(done) + len(srv.shed) == len(reqs) and not srv.rejected
+    assert done and srv.shed
+    assert max(r.launched - r.arrival for r in done) <= 0.002 + 0.05
+    everything = Server(scenes, slots=2, max_views=8, background=BG)
+    try:
+        assert len(everything.run([Request(r.rid, r.scene, r.cam, 0.0) for r in base])) == len(reqs)
+        assert not everything.shed
+    finally:
+        everything.close()
