@@ -361,7 +361,7 @@ def serve_runtime_scene(srv, ctx, path, centers, seed, trained_steps=None, rate=
     return out
 
 
-SHED_WAIT_MS = 8.0  # queueing bound of the load-shedding run
+SHED_WAIT_MS = 4.0  # queueing bound of the load-shedding run
 
 
 def bench_serving(rates=SERVE_RATES, duration=1.0, slots=6, max_views=16, seed=20, trained_ply=None,
