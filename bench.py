@@ -927,7 +927,7 @@ def cpu_prior_sample(dg, cams, targets, leg, seconds=10.0):
                       f"median {per:.3f} s/step, MSPLAT_THREADS={threads}"}
 
 
-def bench_images(repeats=200, cpu_seconds=5.0):
+def bench_images(repeats=200, cpu_seconds=5.0, cpu=True):
     """The training images' pipeline for one 1264x832 view: area resampling of a decoded
     RGBA8 camera image (2528x1664, and a 4032x3024 one at a non-integer ratio) to the
     working size, one pyramid level (1264x832 -> 632x416 float), and Brown-Conrady
@@ -935,16 +935,19 @@ def bench_images(repeats=200, cpu_seconds=5.0):
     `repeats` back-to-back launches from the host, wall clock over them (so launch costs
     are included), bytes = source read once + output written once. Beside them the
     oracle's single-thread C restatement (the reference's msplat_resize_area /
-    msplat_undistort, core/cpu/msplat_cpu.cpp:1609-1669) on the same inputs."""
+    msplat_undistort, core/cpu/msplat_cpu.cpp:1609-1669) on the same inputs, as the
+    checker and the CPU baseline (skipped when `cpu` is False)."""
     import ctypes as C
     from msplat._abi import Context, DeviceArray, check, floats, lib
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import Oracle
+    orc = None
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle
+        orc = Oracle()
     ctx = Context(0)
     rng = np.random.default_rng(40)
     intr, dist = (FOCAL, FOCAL, W_IMG / 2.0, H_IMG / 2.0), (-0.08, 0.02, 1e-3, -5e-4, 0.0)
     out = {"view": f"{W_IMG}x{H_IMG}", "ops": {}}
-    orc = Oracle()
     try:
         cases = []
         for sw, sh in ((2 * W_IMG, 2 * H_IMG), (4032, 3024)):
@@ -977,20 +980,21 @@ def bench_images(repeats=200, cpu_seconds=5.0):
                 call()
             ctx.sync()
             per = (time.perf_counter() - t0) / repeats
-            got = dst.numpy().reshape(dh, dw, 3)
-            times, t_start = [], time.perf_counter()
-            while True:
-                t1 = time.perf_counter()
-                ref = (orc.undistort(host, intr, dist, 0, 0, dw, dh) if name.startswith("undistort")
-                       else orc.resize_area(host, dw, dh))
-                times.append(time.perf_counter() - t1)
-                if time.perf_counter() - t_start > cpu_seconds / len(cases) or len(times) >= 20:
-                    break
             nbytes = host.nbytes + dw * dh * 12
             out["ops"][name] = {"us_per_call": round(1e6 * per, 2), "bytes": int(nbytes),
-                                "gbs": round(nbytes / per / 1e9, 1), "frac": round(nbytes / per / 1e9 / HBM_PEAK_GBS, 4),
-                                "bit_exact_vs_oracle": bool(np.array_equal(got, ref)),
-                                "cpu_port_us": round(1e6 * float(np.median(times)), 1)}
+                                "gbs": round(nbytes / per / 1e9, 1), "frac": round(nbytes / per / 1e9 / HBM_PEAK_GBS, 4)}
+            if orc is not None:
+                got = dst.numpy().reshape(dh, dw, 3)
+                times, t_start = [], time.perf_counter()
+                while True:
+                    t1 = time.perf_counter()
+                    ref = (orc.undistort(host, intr, dist, 0, 0, dw, dh) if name.startswith("undistort")
+                           else orc.resize_area(host, dw, dh))
+                    times.append(time.perf_counter() - t1)
+                    if time.perf_counter() - t_start > cpu_seconds / len(cases) or len(times) >= 20:
+                        break
+                out["ops"][name].update(bit_exact_vs_oracle=bool(np.array_equal(got, ref)),
+                                        cpu_port_us=round(1e6 * float(np.median(times)), 1))
             src.free()
             dst.free()
     finally:
@@ -998,16 +1002,15 @@ def bench_images(repeats=200, cpu_seconds=5.0):
     return out
 
 
-def bench_scene_files(ctx, dg, step, tmpdir, repeats=3):
+def bench_scene_files(ctx, dg, step, tmpdir, repeats=3, cpu=True):
     """The trained scene's files (save_gaussians.cpp:10-191, model.cpp:282-496): PLY and
     .splat exports, the PLY import and the .msplat checkpoint, each the best of `repeats`
     (host I/O into `tmpdir` after one device-to-host staging copy, so the page cache and
     the host's write speed bound them); beside them the reference's own writers and reader
     (the compiled core/src/loaders/save_gaussians.cpp) on the same parameters, whose files
-    are compared byte for byte. The reference is skipped when it was not built."""
+    are compared byte for byte. The reference is skipped when it was not built or `cpu` is
+    False."""
     from msplat.scene_io import load_checkpoint, load_ply, save_checkpoint, save_ply, save_splat
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import Reference, reference_available
 
     def best(fn):
         times = []
@@ -1035,7 +1038,10 @@ def bench_scene_files(ctx, dg, step, tmpdir, repeats=3):
     out = {"gaussians": dg.n, "sh_degree": dg.degree,
            "ops": {k: {"s": round(t, 4), "mb": round(os.path.getsize(size[k]) / 1e6, 1),
                        "mb_per_s": round(os.path.getsize(size[k]) / t / 1e6, 1)} for k, t in gpu.items()}}
-    if reference_available():
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Reference, reference_available
+    if cpu and reference_available():
         ref = Reference()
         host = dg.to_numpy()
         scene = {k: host[k] for k in ("means", "scales", "quats", "dc", "rest", "opac")}
@@ -1258,7 +1264,7 @@ def main():
         import tempfile
         tmpdir = tempfile.mkdtemp(prefix="msplat_files_")
         try:
-            scene_files = bench_scene_files(ctx, dg, tr.step_count, tmpdir)
+            scene_files = bench_scene_files(ctx, dg, tr.step_count, tmpdir, cpu=not args.no_cpu_baseline)
             log("scene files: " + json.dumps(scene_files))
         finally:
             shutil.rmtree(tmpdir, ignore_errors=True)
@@ -1269,7 +1275,7 @@ def main():
         log(f"prior-guided: {prior['iter_per_s']} iter/s, plain SH1 {prior['plain_sh1_iter_per_s']} iter/s")
     images = None
     if not args.no_images:
-        images = bench_images()
+        images = bench_images(cpu=not args.no_cpu_baseline)
         log("images: " + json.dumps(images["ops"]))
     plain = None
     if not args.no_plain:
