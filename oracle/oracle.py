@@ -1,6 +1,7 @@
 """ctypes wrappers over the oracle (liboracle.so) and the compiled reference
 (_ref/libmsplat_ref*.so). TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check
-and bench.py's cpu_baseline leg, never by the product.
+and the CPU-baseline parts of bench.py (skipped by --no-cpu-baseline), never by the
+product.
 
 The oracle restates core/cpu/msplat_cpu.cpp (see msplat_oracle.cpp for the per-function
 citations); `Reference` calls the reference's own CPU backend through ref_driver.cpp.
